@@ -1,0 +1,234 @@
+"""Synthetic meshes and control files for the Lagrangian DG solver.
+
+Builds the structured quadrilateral meshes of the benchmark problems (straight
+Q1 quads for the Sod tube, jittered curved Q2 quads for the Sedov blast) in the
+node/element/boundary layout of the reference's mesh reader
+(Preprocess.f90:156-221 and :622-670), and writes the reference's two input
+files, ``dgflo2d.ctrl`` (Preprocess.f90:28-55) and ``dgflo2d.domn``.
+
+Conventions (GetFE.f90:91-99, Preprocess.f90:564-578): a Q2 quad lists its
+corners counter-clockwise, then the mid-edge nodes (5 on edge 1-2, 6 on 2-3,
+7 on 3-4, 8 on 4-1), then the centre node 9.  Connectivity is 1-based as in
+the reference.  Boundary faces carry ``(n1, n2, code, subcode, mid)``; code 22
+is a slip wall, subcode 221 keeps v=0 (wall normal to y), 222 keeps u=0
+(Limiterlag.f90:607-651).
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+import numpy as np
+
+
+@dataclasses.dataclass
+class Mesh:
+    coord: np.ndarray      # (npoin, 2) float64
+    ipqua: np.ndarray      # (nquad, nvqua) int32, 1-based
+    bface: np.ndarray      # (nbfac, 5) int32, 1-based nodes
+    ncurv: int             # 0 straight Q1, 1 quadratic Q2
+    nx: int
+    ny: int
+
+    @property
+    def npoin(self) -> int:
+        return self.coord.shape[0]
+
+    @property
+    def nquad(self) -> int:
+        return self.ipqua.shape[0]
+
+    @property
+    def nbfac(self) -> int:
+        return self.bface.shape[0]
+
+
+def _boundary_faces(nid, nx, ny, q2):
+    """Boundary faces in the order bottom, right, top, left.
+
+    ``nid(I, J)`` maps lattice indices to 1-based node ids; on a Q2 lattice a
+    cell edge spans two lattice steps and its middle lattice point is the mid
+    node."""
+    s = 2 if q2 else 1
+    faces = []
+    for i in range(nx):          # bottom y = ymin: v = 0
+        a, b = nid(s * i, 0), nid(s * i + s, 0)
+        faces.append((a, b, 22, 221, nid(s * i + 1, 0) if q2 else 0))
+    for j in range(ny):          # right x = xmax: u = 0
+        a, b = nid(s * nx, s * j), nid(s * nx, s * j + s)
+        faces.append((a, b, 22, 222, nid(s * nx, s * j + 1) if q2 else 0))
+    for i in range(nx - 1, -1, -1):   # top y = ymax: v = 0
+        a, b = nid(s * i + s, s * ny), nid(s * i, s * ny)
+        faces.append((a, b, 22, 221, nid(s * i + 1, s * ny) if q2 else 0))
+    for j in range(ny - 1, -1, -1):   # left x = xmin: u = 0
+        a, b = nid(0, s * j + s), nid(0, s * j)
+        faces.append((a, b, 22, 222, nid(0, s * j + 1) if q2 else 0))
+    return np.asarray(faces, dtype=np.int32)
+
+
+def quad_q1(nx, ny, x0, x1, y0, y1) -> Mesh:
+    """Uniform straight quads (ncurv=0), nodes numbered row by row."""
+    xs = np.linspace(x0, x1, nx + 1)
+    ys = np.linspace(y0, y1, ny + 1)
+    X, Y = np.meshgrid(xs, ys, indexing="xy")
+    coord = np.stack([X.ravel(), Y.ravel()], axis=1)
+
+    def nid(I, J):
+        return I + (nx + 1) * J + 1
+
+    ipq = np.empty((nx * ny, 4), dtype=np.int32)
+    for j in range(ny):
+        for i in range(nx):
+            ipq[i + nx * j] = (nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1))
+    return Mesh(coord, ipq, _boundary_faces(nid, nx, ny, False), 0, nx, ny)
+
+
+def quad_q2(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1, bulge=0.05) -> Mesh:
+    """Q2 quads (ncurv=1) on a (2nx+1) x (2ny+1) node lattice.
+
+    With ``seed`` set, interior vertices move by U(-jitter*h, jitter*h) in
+    each coordinate and interior edge mid-nodes move along the edge normal by
+    U(-bulge*h, bulge*h), which curves the faces; boundary nodes stay on the
+    walls.  Mid-nodes start at the chord midpoint and every centre node is
+    placed by the centre-node formula the time step applies after every stage
+    (Getrhslag.f90:1962-1971), so the first step does not move it."""
+    Nx, Ny = 2 * nx + 1, 2 * ny + 1
+    hx = (x1 - x0) / nx
+    hy = (y1 - y0) / ny
+    h = min(hx, hy)
+    P = np.zeros((Ny, Nx, 2))
+    P[..., 0] = x0 + 0.5 * hx * np.arange(Nx)[None, :]
+    P[..., 1] = y0 + 0.5 * hy * np.arange(Ny)[:, None]
+    rng = np.random.default_rng(seed) if seed is not None else None
+    if rng is not None and nx > 1 and ny > 1:
+        d = rng.uniform(-jitter * h, jitter * h, size=(ny - 1, nx - 1, 2))
+        P[2:-2:2, 2:-2:2, :] += d
+    # mid-nodes of horizontal edges (even J, odd I) and vertical edges (odd J, even I)
+    for J in range(0, Ny, 2):
+        for I in range(1, Nx, 2):
+            a, b = P[J, I - 1], P[J, I + 1]
+            P[J, I] = 0.5 * (a + b)
+    for J in range(1, Ny, 2):
+        for I in range(0, Nx, 2):
+            a, b = P[J - 1, I], P[J + 1, I]
+            P[J, I] = 0.5 * (a + b)
+    if rng is not None:
+        for J in range(0, Ny, 2):
+            for I in range(1, Nx, 2):
+                if J in (0, Ny - 1):
+                    continue
+                t = P[J, I + 1] - P[J, I - 1]
+                n = np.array([-t[1], t[0]]) / np.hypot(t[0], t[1])
+                P[J, I] += rng.uniform(-bulge * h, bulge * h) * n
+        for J in range(1, Ny, 2):
+            for I in range(0, Nx, 2):
+                if I in (0, Nx - 1):
+                    continue
+                t = P[J + 1, I] - P[J - 1, I]
+                n = np.array([-t[1], t[0]]) / np.hypot(t[0], t[1])
+                P[J, I] += rng.uniform(-bulge * h, bulge * h) * n
+    for j in range(ny):
+        for i in range(nx):
+            I, J = 2 * i, 2 * j
+            c = P[J, I] + P[J, I + 2] + P[J + 2, I + 2] + P[J + 2, I]
+            m = P[J, I + 1] + P[J + 1, I + 2] + P[J + 2, I + 1] + P[J + 1, I]
+            P[J + 1, I + 1] = -0.25 * c + 0.5 * m
+    coord = P.reshape(-1, 2).copy()
+
+    def nid(I, J):
+        return I + Nx * J + 1
+
+    ipq = np.empty((nx * ny, 9), dtype=np.int32)
+    for j in range(ny):
+        for i in range(nx):
+            I, J = 2 * i, 2 * j
+            ipq[i + nx * j] = (nid(I, J), nid(I + 2, J), nid(I + 2, J + 2), nid(I, J + 2),
+                               nid(I + 1, J), nid(I + 2, J + 1), nid(I + 1, J + 2), nid(I, J + 1),
+                               nid(I + 1, J + 1))
+    return Mesh(coord, ipq, _boundary_faces(nid, nx, ny, True), 1, nx, ny)
+
+
+# --------------------------------------------------------------------------
+# problem definitions (the control-file switches of each benchmark problem)
+
+@dataclasses.dataclass
+class Control:
+    """Switches of ``dgflo2d.ctrl`` (Preprocess.f90:28-55) used by the
+    Lagrangian path.  Defaults are those of the planar Sod tube."""
+    cfl: float = 0.3
+    dtfix: float = 0.0
+    gamlg: float = 1.4
+    cdrho: float = 0.0
+    tend: float = 0.2
+    cimpd: float = 1.0
+    ntime: int = 100000
+    ndegr: int = 3
+    npoly: int = 1
+    nlimi: int = 6
+    ncurv: int = 0
+    ndens: int = 1
+    ndt: int = 1
+    nstag: int = 2
+    ncase: int = 6
+    nriem: int = 1
+    nfint: int = 1
+    nrz: int = 0
+
+
+def sod_problem(nx=100, ny=4):
+    mesh = quad_q1(nx, ny, 0.0, 1.0, 0.0, 0.04 * ny / 4.0)
+    return mesh, Control()
+
+
+def sedov_problem(nx=120, ny=None, seed=2024, length=1.2):
+    ny = nx if ny is None else ny
+    mesh = quad_q2(nx, ny, 0.0, length, 0.0, length * ny / nx, seed=seed)
+    ctl = Control(cfl=0.25, tend=1.0, ndegr=6, npoly=2, ncurv=1, nstag=3, ncase=7,
+                  nfint=4, cdrho=0.0)
+    return mesh, ctl
+
+
+def write_ctrl(path, c: Control):
+    lines = [
+        "amach cfl nflux breta dtfix", f"0.0 {c.cfl!r} 1 0.0 {c.dtfix!r}",
+        "nmatel", "1",
+        "gamlg cdrho tend cimpd", f"{c.gamlg!r} {c.cdrho!r} {c.tend!r} {c.cimpd!r}",
+        "ntime ndegr mdegr npoly nreco", f"{c.ntime} {c.ndegr} {c.ndegr} {c.npoly} 0",
+        "nlimi ngaus ngausf ngausd nint", f"{c.nlimi} 2 2 7 1",
+        "ngausf_geo ngausd_geo", "2 7",
+        "ncurv", f"{c.ncurv}",
+        "nresi", "1000000",
+        "nmeth nq ndens", f"2 4 {c.ndens}",
+        "ndt nstag", f"{c.ndt} {c.nstag}",
+        "ncase nriem nfint", f"{c.ncase} {c.nriem} {c.nfint}",
+        "nrz", f"{c.nrz}",
+        "nout ndump", "0 0",
+    ]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_domn(path, m: Mesh):
+    nv = m.ipqua.shape[1]
+    with open(path, "w") as f:
+        f.write("1\nLagrangian DG mesh\n")
+        f.write(" ndimn ntype\n 2 1\n")
+        f.write(" ntria npoin nbfac tinpt nquad\n")
+        f.write(f" 0 {m.npoin} {m.nbfac} 0.0 {m.nquad}\n")
+        f.write(" element connectivity\n")
+        for e in range(m.nquad):
+            f.write(f"{e + 1} " + " ".join(str(int(v)) for v in m.ipqua[e]) + "\n")
+        f.write(" coordinates\n")
+        for p in range(m.npoin):
+            f.write(f"{p + 1} {float(m.coord[p, 0])!r} {float(m.coord[p, 1])!r}\n")
+        f.write(" initial values (not read)\n")
+        for p in range(m.npoin):
+            f.write(f"{p + 1} 0.0 0.0 0.0 0.0\n")
+        f.write(" boundary faces\n")
+        for b in range(m.nbfac):
+            f.write(f"{b + 1} " + " ".join(str(int(v)) for v in m.bface[b]) + "\n")
+
+
+def write_case(directory, mesh: Mesh, ctl: Control):
+    os.makedirs(directory, exist_ok=True)
+    write_ctrl(os.path.join(directory, "dgflo2d.ctrl"), ctl)
+    write_domn(os.path.join(directory, "dgflo2d.domn"), mesh)

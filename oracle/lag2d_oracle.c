@@ -1,0 +1,885 @@
+/* CPU restatement of the reference's Lagrangian DG step (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity tests, the smoke check and the bench's
+ * CPU-baseline leg compare the HIP path against this code; the product never
+ * links or calls it.  Built with -ffp-contract=off so every product and sum is
+ * rounded where the reference rounds it; expressions keep the reference's
+ * left-to-right evaluation order.  Each function names the reference
+ * subroutine (file:line) whose arithmetic it restates.
+ *
+ * Paths covered: planar Lagrangian DG with nriem=1, nlimi=6, ndens=1 and
+ *   - ncurv=0, nfint=1, P1 (straight quads, Barth + symmetry-preserving
+ *     velocity limiter), Getrhslagmc.f90:6810-7405.
+ */
+#include "lag2d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define U(c, a, id, iq, ie) (a)[(id) + (c)->ndegr * ((iq) + (c)->nq * (ie))]
+#define GE(c, k, ie) (c)->geoel[(k) + 24 * (ie)]
+#define X(a, d, ip) (a)[(d) + 2 * (ip)]
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+/* Fortran default-real literal 1.e-6 widened to REAL(8) */
+static const double EPS_SP6 = (double)1.e-6f;
+
+/* ------------------------------------------------------------------------ */
+/* Quadrature: ruqope (dgflo2d.f90:2646-2783), 2-D tensor rules, ngaus = n*n */
+static void ruqope2(int n, double *pos /* [2*n*n] */, double *w /* [n*n] */)
+{
+    double pg[6], wg[6];
+    if (n == 5) {
+        pg[2] = 0.0;
+        pg[3] = 0.5384693101056830910363144;
+        pg[1] = -0.5384693101056830910363144;
+        pg[4] = 0.9061798459386639927976269;
+        pg[0] = -0.9061798459386639927976269;
+        wg[2] = 0.5688888888888888888888889;
+        wg[3] = 0.4786286704993664680412915;
+        wg[1] = 0.4786286704993664680412915;
+        wg[4] = 0.2369268850561890875142640;
+        wg[0] = 0.2369268850561890875142640;
+    } else if (n == 6) {
+        pg[0] = 0.2386191860831969086305017;
+        pg[1] = -0.2386191860831969086305017;
+        pg[2] = 0.6612093864662645136613996;
+        pg[3] = -0.6612093864662645136613996;
+        pg[4] = -0.9324695142031520278123016;
+        pg[5] = 0.9324695142031520278123016;
+        wg[0] = 0.4679139345726910;
+        wg[1] = 0.4679139345726910;
+        wg[2] = 0.3607615730481386;
+        wg[3] = 0.3607615730481386;
+        wg[4] = 0.1713244923791704;
+        wg[5] = 0.1713244923791704;
+    } else { /* n == 2 */
+        pg[0] = -0.5773502691896257645091488;
+        pg[1] = 0.5773502691896257645091488;
+        wg[0] = 1.0;
+        wg[1] = 1.0;
+    }
+    int g = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            w[g] = wg[i] * wg[j];
+            pos[2 * g] = pg[i];
+            pos[2 * g + 1] = pg[j];
+            ++g;
+        }
+}
+
+/* Q2 shape functions and reference derivatives (Limiterlag.f90:2053-2091) */
+static void shape_q2(double r, double s, double *sh, double *dr_, double *ds_)
+{
+    const double c10 = 1.0;
+    if (sh) {
+        sh[0] = 0.25 * (c10 - r) * r * s * (c10 - s);
+        sh[1] = -0.25 * (c10 + r) * (c10 - s) * r * s;
+        sh[2] = 0.25 * (c10 + r) * (c10 + s) * r * s;
+        sh[3] = -0.25 * (c10 - r) * (c10 + s) * r * s;
+        sh[4] = -0.5 * (c10 - r * r) * (c10 - s) * s;
+        sh[5] = 0.5 * (c10 + r) * (c10 - s * s) * r;
+        sh[6] = 0.5 * (c10 - r * r) * (c10 + s) * s;
+        sh[7] = -0.5 * (c10 - r) * (c10 - s * s) * r;
+        sh[8] = (c10 - r * r) * (c10 - s * s);
+    }
+    dr_[0] = 0.25 * (2.0 * r - c10) * s * (s - 1.0);
+    dr_[1] = 0.25 * (2.0 * r + c10) * s * (s - 1.0);
+    dr_[2] = 0.25 * (2.0 * r + c10) * s * (s + 1.0);
+    dr_[3] = 0.25 * (2.0 * r - c10) * s * (s + 1.0);
+    dr_[4] = -r * s * (s - c10);
+    dr_[5] = 0.5 * (2.0 * r + c10) * (c10 - s * s);
+    dr_[6] = -r * s * (s + c10);
+    dr_[7] = 0.5 * (2.0 * r - c10) * (1.0 - s * s);
+    dr_[8] = -2.0 * r * (1.0 - s * s);
+    ds_[0] = 0.25 * (2.0 * s - c10) * r * (r - 1.0);
+    ds_[1] = 0.25 * (2.0 * s - c10) * r * (r + 1.0);
+    ds_[2] = 0.25 * (2.0 * s + c10) * r * (r + 1.0);
+    ds_[3] = 0.25 * (2.0 * s + c10) * r * (r - 1.0);
+    ds_[4] = 0.5 * (1.0 - r * r) * (2.0 * s - c10);
+    ds_[5] = -r * s * (r + c10);
+    ds_[6] = 0.5 * (1.0 - r * r) * (2.0 * s + c10);
+    ds_[7] = -r * s * (r - c10);
+    ds_[8] = -2.0 * s * (1.0 - r * r);
+}
+
+/* The 9 geometry points of a quad: Q2 nodes, or for straight quads the
+ * corners plus chord midpoints (Limiterlag.f90:2008-2021). */
+static void quad_points9(const orc_case *c, const double *x, int ie, double xp[2][9])
+{
+    const int *ip = c->ipqua + c->nvqua * ie;
+    if (c->ncurv == 0) {
+        for (int k = 0; k < 4; ++k) {
+            xp[0][k] = X(x, 0, ip[k]);
+            xp[1][k] = X(x, 1, ip[k]);
+        }
+        for (int d = 0; d < 2; ++d) {
+            xp[d][4] = 0.5 * (xp[d][0] + xp[d][1]);
+            xp[d][5] = 0.5 * (xp[d][1] + xp[d][2]);
+            xp[d][6] = 0.5 * (xp[d][2] + xp[d][3]);
+            xp[d][7] = 0.5 * (xp[d][3] + xp[d][0]);
+            xp[d][8] = 0.5 * (xp[d][4] + xp[d][6]);
+        }
+    } else {
+        for (int k = 0; k < 9; ++k) {
+            xp[0][k] = X(x, 0, ip[k]);
+            xp[1][k] = X(x, 1, ip[k]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* getcellvol_lag, quad loop (Limiterlag.f90:1999-2162): current volume
+ * geoel(3) and volume centre geoel(5:6) by the 6x6 Gauss rule. */
+void orc_cellvol(orc_case *c, const double *x)
+{
+    double pos[72], w[36];
+    ruqope2(6, pos, w);
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        double xp[2][9];
+        quad_points9(c, x, ie, xp);
+        double rcv = 0.0, scv = 0.0, volel = 0.0, areel = 0.0, rcvrz = 0.0, scvrz = 0.0;
+        for (int g = 0; g < 36; ++g) {
+            double r = pos[2 * g], s = pos[2 * g + 1], wi = w[g];
+            double sh[9], dr_[9], ds_[9];
+            shape_q2(r, s, sh, dr_, ds_);
+            double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+            for (int k = 0; k < 9; ++k) {
+                dxdr = dxdr + dr_[k] * xp[0][k];
+                dxds = dxds + ds_[k] * xp[0][k];
+                dydr = dydr + dr_[k] * xp[1][k];
+                dyds = dyds + ds_[k] * xp[1][k];
+            }
+            double yg = 0.0;
+            for (int k = 0; k < 9; ++k) yg = yg + sh[k] * xp[1][k];
+            double rcoef = 1.0 - c->alfrz + c->alfrz * yg;
+            double djaco = wi * (dxdr * dyds - dydr * dxds);
+            volel = volel + djaco * rcoef;
+            rcv = rcv + r * djaco * rcoef;
+            scv = scv + s * djaco * rcoef;
+            if (c->nrz == 2) {
+                areel = areel + djaco;
+                rcvrz = rcvrz + r * djaco;
+                scvrz = scvrz + s * djaco;
+            }
+        }
+        GE(c, 2, ie) = volel;
+        GE(c, 4, ie) = rcv / volel;
+        GE(c, 5, ie) = scv / volel;
+        if (c->nrz == 2) {
+            GE(c, 4, ie) = rcvrz / areel;
+            GE(c, 5, ie) = scvrz / areel;
+            GE(c, 10, ie) = volel / areel;
+            GE(c, 11, ie) = areel;
+            GE(c, 14, ie) = rcvrz / areel;
+            GE(c, 15, ie) = scvrz / areel;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* getfnds_lag_mc_hybrid, quad part (Getrhslagmc.f90:2212-2260): unit normal
+ * and length of the four straight edges. gq[(k + 3*edge) + 12*ie]. */
+static void fnds_q1(const orc_case *c, const double *x, double *gq)
+{
+    static const int ea[4] = {0, 1, 2, 3}, eb[4] = {1, 2, 3, 0};
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        for (int e = 0; e < 4; ++e) {
+            double anx = X(x, 1, ip[eb[e]]) - X(x, 1, ip[ea[e]]);
+            double any = -(X(x, 0, ip[eb[e]]) - X(x, 0, ip[ea[e]]));
+            double *g = gq + 3 * e + 12 * ie;
+            g[0] = anx / sqrt(anx * anx + any * any);
+            g[1] = any / sqrt(anx * anx + any * any);
+            g[2] = sqrt(anx * anx + any * any);
+        }
+    }
+}
+
+/* barthlimit_lag_vtxunknew, quad loop (Getrhslagmctria.f90:2522-2636): per
+ * node max/min over surrounding cells of (tau, u, v, p, E, div u).
+ * unmax[q + 6*ip]. The divergence row uses dr, ds that the reference only
+ * sets in its triangle loop; it feeds nothing on these paths and is left
+ * unset here. */
+static void vtx_minmax(const orc_case *c, double *unmax, double *unmin)
+{
+    for (int i = 0; i < 6 * c->npoin; ++i) {
+        unmax[i] = -1.e10;
+        unmin[i] = 1.e10;
+    }
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        double rhom = U(c, c->unold, 0, 0, ie);
+        double uctr = U(c, c->unold, 0, 1, ie);
+        double vctr = U(c, c->unold, 0, 2, ie);
+        double ectr = U(c, c->unold, 0, 3, ie);
+        double rhoct = 1.0 / rhom;
+        double pctr = dmax(EPS_SP6, (c->gamlg - 1.0) * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        double unctr[5] = {U(c, c->unold, 0, 0, ie), uctr, vctr, pctr, ectr};
+        for (int iv = 0; iv < 4; ++iv) {
+            int p = ip[iv];
+            for (int q = 0; q < 5; ++q) {
+                if (unctr[q] > unmax[q + 6 * p]) unmax[q + 6 * p] = unctr[q];
+                if (unctr[q] < unmin[q + 6 * p]) unmin[q + 6 * p] = unctr[q];
+            }
+        }
+    }
+}
+
+/* barthfct (Getrhslagmc.f90:9171-9219) */
+static double barthfct(double umax, double umin, double uctr, double dunk)
+{
+    const double coef = 0.7;
+    double fiy;
+    if (dunk > 1.e-8) {
+        fiy = coef * (umax - uctr) / dunk;
+        return dmin((fiy * fiy + 2.0 * fiy) / (fiy * fiy + fiy + 2.0), 1.0);
+    } else if (dunk < -1.e-8) {
+        fiy = coef * (umin - uctr) / dunk;
+        return dmin((fiy * fiy + 2.0 * fiy) / (fiy * fiy + fiy + 2.0), 1.0);
+    }
+    return 1.0;
+}
+
+static const double XVQ[4] = {-1.0, 1.0, 1.0, -1.0};
+static const double YVQ[4] = {-1.0, -1.0, 1.0, 1.0};
+
+/* barthlimit_sympre_quadlb (Limiterlag.f90:8127-8692): velocity limiter in
+ * the principal-strain frame; writes afvec. Parts 5 and "further technique"
+ * only write geoel(10), which nothing on this path reads; they are omitted. */
+static void sympre_quad(orc_case *c, const int *indbd, double *afvec)
+{
+    const int nq = c->nq;
+    double *mapmt = (double *)calloc((size_t)4 * c->nquad, sizeof(double));
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        double xp[2][4];
+        for (int k = 0; k < 4; ++k) {
+            xp[0][k] = X(c->cooro, 0, ip[k]);
+            xp[1][k] = X(c->cooro, 1, ip[k]);
+        }
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double dudr = U(c, c->unold, 1, 1, ie) / 1.0;
+        double duds = U(c, c->unold, 2, 1, ie) / 1.0;
+        double dvdr = U(c, c->unold, 1, 2, ie) / 1.0;
+        double dvds = U(c, c->unold, 2, 2, ie) / 1.0;
+        double r = rc, s = sc;
+        double rp = 1.0 + r, rm = 1.0 - r, sp = 1.0 + s, sm = 1.0 - s;
+        double dsr[4] = {-0.25 * sm, 0.25 * sm, 0.25 * sp, -0.25 * sp};
+        double dss[4] = {-0.25 * rm, -0.25 * rp, 0.25 * rp, 0.25 * rm};
+        double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            dxdr = dxdr + dsr[k] * xp[0][k];
+            dxds = dxds + dss[k] * xp[0][k];
+            dydr = dydr + dsr[k] * xp[1][k];
+            dyds = dyds + dss[k] * xp[1][k];
+        }
+        double jacom = dxdr * dyds - dydr * dxds;
+        double drdx = dyds / jacom, drdy = -dxds / jacom;
+        double dsdx = -dydr / jacom, dsdy = dxdr / jacom;
+        double dudx = dudr * drdx + duds * dsdx;
+        double dudy = dudr * drdy + duds * dsdy;
+        double dvdx = dvdr * drdx + dvds * dsdx;
+        double dvdy = dvdr * drdy + dvds * dsdy;
+        double matra = dudx, matrb = 0.5 * (dudy + dvdx), matrc = matrb, matrd = dvdy;
+        double mapt = matra + matrd;
+        double mapd = matra * matrd - matrb * matrc;
+        double lamda1 = 0.5 * mapt + sqrt(0.25 * (mapt * mapt) - mapd);
+        double lamda2 = 0.5 * mapt - sqrt(0.25 * (mapt * mapt) - mapd);
+        double *m = mapmt + 4 * ie; /* m[0]=(1,1) m[1]=(1,2) m[2]=(2,1) m[3]=(2,2) */
+        if (fabs(matrc) > 1.e-8) {
+            double l1 = sqrt((lamda1 - matrd) * (lamda1 - matrd) + matrc * matrc);
+            double l2 = sqrt((lamda2 - matrd) * (lamda2 - matrd) + matrc * matrc);
+            if (matrc > 0.0) {
+                m[0] = (lamda1 - matrd) / l1; m[1] = matrc / l1;
+                m[2] = (lamda2 - matrd) / l2; m[3] = matrc / l2;
+            } else {
+                m[2] = (lamda1 - matrd) / l1; m[3] = matrc / l1;
+                m[0] = (lamda2 - matrd) / l2; m[1] = matrc / l2;
+            }
+        } else {
+            m[0] = m[1] = m[2] = m[3] = 0.0;
+        }
+    }
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        const double *m = mapmt + 4 * ie;
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double alfa[2][4];
+        double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie);
+        double uc2 = uctr * m[0] + vctr * m[1];
+        double uc3 = uctr * m[2] + vctr * m[3];
+        for (int iv = 0; iv < 4; ++iv) {
+            double bq[3] = {1.0, (XVQ[iv] - rc) / 1.0, (YVQ[iv] - sc) / 1.0};
+            double uv = 0.0, vv = 0.0;
+            for (int id = 0; id < c->ndegr && id < 3; ++id) {
+                uv = uv + U(c, c->unold, id, 1, ie) * bq[id];
+                vv = vv + U(c, c->unold, id, 2, ie) * bq[id];
+            }
+            double un2 = uv * m[0] + vv * m[1];
+            double un3 = uv * m[2] + vv * m[3];
+            double mx2 = uc2, mx3 = uc3, mn2 = uc2, mn3 = uc3;
+            int p = ip[iv];
+            for (int k = c->esuv2[p]; k < c->esuv2[p + 1]; ++k) {
+                int je = c->esuv1[k];
+                double ucj = U(c, c->unold, 0, 1, je), vcj = U(c, c->unold, 0, 2, je);
+#define ACC(UU, VV)                                          \
+    do {                                                     \
+        double ul = (UU) * m[0] + (VV) * m[1];               \
+        double vl = (UU) * m[2] + (VV) * m[3];               \
+        mx2 = dmax(mx2, ul); mx3 = dmax(mx3, vl);            \
+        mn2 = dmin(mn2, ul); mn3 = dmin(mn3, vl);            \
+    } while (0)
+                ACC(ucj, vcj);
+                if (c->ncase != 1) {
+                    if (indbd[p] == 3) {
+                        ACC(ucj, -vcj);
+                    } else if (indbd[p] == 2) {
+                        ACC(-ucj, vcj);
+                    } else if (indbd[p] == 10) {
+                        ACC(ucj, -vcj);
+                        ACC(-ucj, vcj);
+                        ACC(-ucj, -vcj);
+                    }
+                }
+#undef ACC
+            }
+            alfa[0][iv] = barthfct(mx2, mn2, uc2, un2 - uc2);
+            alfa[1][iv] = barthfct(mx3, mn3, uc3, un3 - uc3);
+        }
+        double fixi = 1.e300, fiet = 1.e300;
+        for (int iv = 0; iv < 4; ++iv) {
+            fixi = dmin(fixi, alfa[0][iv]);
+            fiet = dmin(fiet, alfa[1][iv]);
+        }
+        (void)nq;
+        double xicx = m[0], xicy = m[1], etcx = m[2], etcy = m[3];
+        double fi11 = xicx * xicx * fixi + etcx * etcx * fiet;
+        double fi12 = xicx * xicy * fixi + etcx * etcy * fiet;
+        double fi22 = xicy * xicy * fixi + etcy * etcy * fiet;
+        double *a = afvec + 4 * ie; /* a[0]=(1,1) a[1]=(2,1) a[2]=(1,2) a[3]=(2,2) */
+        a[0] = fi11;
+        a[2] = fi12;
+        a[1] = fi12;
+        a[3] = fi22;
+    }
+    free(mapmt);
+}
+
+/* barthlimit_lagsym_quad (Limiterlag.f90:4-404) for ndens=1, P1.
+ * aflim[q + 5*ie] rows tau,u,v,p,E; afvec column-major 2x2 per cell. */
+static void barth_quad(orc_case *c, const double *unmax, const double *unmin,
+                       double *aflim, double *afvec)
+{
+    const double g1 = c->gamlg - 1.0;
+    int *indbd = (int *)calloc((size_t)c->npoin, sizeof(int));
+    /* node colouring of barthlimit_sympre_quadlb (Limiterlag.f90:8212-8250) */
+    if (c->ncase != 1) {
+        for (int f = 0; f < c->nbfac; ++f) {
+            const int *b = c->bface + 5 * f;
+            if (b[2] == 22 && b[3] == 221) {
+                indbd[b[0]] = 3; indbd[b[1]] = 3;
+                if (c->ncurv == 1) indbd[b[4]] = 3;
+            }
+        }
+        for (int f = 0; f < c->nbfac; ++f) {
+            const int *b = c->bface + 5 * f;
+            if (b[2] == 22 && b[3] == 222) {
+                int nn = c->ncurv == 1 ? 3 : 2;
+                for (int k = 0; k < nn; ++k) {
+                    int p = (k < 2) ? b[k] : b[4];
+                    indbd[p] = (indbd[p] == 3) ? 10 : 2;
+                }
+            }
+        }
+    }
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double unk[4][4];
+        for (int iv = 0; iv < 4; ++iv) {
+            double bq[3] = {1.0, (XVQ[iv] - rc) / 1.0, (YVQ[iv] - sc) / 1.0};
+            for (int q = 0; q < 4; ++q) unk[q][iv] = 0.0;
+            for (int id = 0; id < 3; ++id)
+                for (int q = 0; q < 4; ++q) unk[q][iv] = unk[q][iv] + U(c, c->unold, id, q, ie) * bq[id];
+            double rhov = 1.0 / unk[0][iv];
+            double uv = unk[1][iv], vv = unk[2][iv], ev = unk[3][iv];
+            double pv = dmax(EPS_SP6, g1 * rhov * (ev - 0.5 * (uv * uv + vv * vv)));
+            unk[0][iv] = 1.0 / rhov;
+            unk[3][iv] = pv;
+        }
+        double rhom = U(c, c->unold, 0, 0, ie);
+        double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie);
+        double ectr = U(c, c->unold, 0, 3, ie);
+        double rhoct = 1.0 / rhom;
+        double pctr = dmax(EPS_SP6, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        double unctr[4] = {1.0 / rhoct, uctr, vctr, pctr};
+        for (int q = 0; q < 4; ++q) {
+            double a = 1.e300;
+            for (int iv = 0; iv < 4; ++iv) {
+                int p = ip[iv];
+                double al = barthfct(unmax[q + 6 * p], unmin[q + 6 * p], unctr[q], unk[q][iv] - unctr[q]);
+                a = dmin(a, al);
+            }
+            aflim[q + 5 * ie] = a;
+        }
+    }
+    sympre_quad(c, indbd, afvec);
+    /* Part 3: total-energy coefficient (Limiterlag.f90:253-381) */
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double rhom = U(c, c->unold, 0, 0, ie);
+        double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie);
+        double ectr = U(c, c->unold, 0, 3, ie);
+        double rhoct = 1.0 / rhom;
+        double pctr = dmax(EPS_SP6, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        double a5 = 1.e300;
+        for (int iv = 0; iv < 4; ++iv) {
+            double bq[3] = {1.0, (XVQ[iv] - rc) / 1.0, (YVQ[iv] - sc) / 1.0};
+            double u[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int id = 0; id < 3; ++id)
+                for (int q = 0; q < 4; ++q) u[q] = u[q] + U(c, c->unold, id, q, ie) * bq[id];
+            double rhov = 1.0 / u[0];
+            double pv = dmax(EPS_SP6, g1 * rhov * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2])));
+            double t = rhom + aflim[0 + 5 * ie] * (1.0 / rhov - rhom);
+            double pl = pctr + aflim[3 + 5 * ie] * (pv - pctr);
+            double e5 = pl / g1 * t + 0.5 * (u[1] * u[1] + u[2] * u[2]);
+            int p = ip[iv];
+            double al = barthfct(unmax[4 + 6 * p], unmin[4 + 6 * p], ectr, e5 - ectr);
+            a5 = dmin(a5, al);
+        }
+        aflim[4 + 5 * ie] = a5;
+    }
+    free(indbd);
+}
+
+/* getriem_quad_matrixsym (Limiterlag.f90:708-1033) + getriecoef_matrixnew
+ * (:1037-1076) and the nodal solve of getndvelo_lag_mc_matrixsym
+ * (:408-704), straight P1 quads. fstarq[(d + 2*(f + 2*iv)) + 16*ie]. */
+static void ndvelo_q1(orc_case *c, const double *gq, const double *aflim, const double *afvec,
+                      double *fstarq)
+{
+    const int np = c->npoin;
+    const double g1 = c->gamlg - 1.0;
+    const double eps = 1.0e-6;
+    double *mn = (double *)calloc((size_t)4 * np, sizeof(double)); /* munacn(2,2,np) col-major */
+    double *mu = (double *)calloc((size_t)2 * np, sizeof(double));
+    double *sn = (double *)calloc((size_t)2 * np, sizeof(double));
+    double *mcl = (double *)malloc(sizeof(double) * 32 * c->nquad); /* munaclq(2,2,2,4) */
+    double *mul = (double *)malloc(sizeof(double) * 16 * c->nquad);
+    double *snl = (double *)malloc(sizeof(double) * 16 * c->nquad);
+    /* corner faces: (edge before, edge after) of each vertex */
+    static const int fe[4][2] = {{3, 0}, {0, 1}, {1, 2}, {2, 3}};
+    static const int nb[4][2] = {{3, 1}, {0, 2}, {1, 3}, {2, 0}};
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double rco[4];
+        for (int k = 0; k < 4; ++k) rco[k] = 1.0 - c->alfrz + c->alfrz * X(c->cooro, 1, ip[k]);
+        double vn[4][2][3];
+        for (int iv = 0; iv < 4; ++iv)
+            for (int f = 0; f < 2; ++f) {
+                const double *g = gq + 3 * fe[iv][f] + 12 * ie;
+                vn[iv][f][0] = g[0];
+                vn[iv][f][1] = g[1];
+                vn[iv][f][2] = (2.0 * rco[iv] + rco[nb[iv][f]]) / 3.0 * 0.5 * g[2];
+            }
+        double rhomc = U(c, c->unold, 0, 0, ie);
+        double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie);
+        double ectr = U(c, c->unold, 0, 3, ie);
+        double rhoct = 1.0 / rhomc;
+        double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        const double *af = afvec + 4 * ie;
+        double dudr = af[0] * U(c, c->unold, 1, 1, ie) + af[2] * U(c, c->unold, 1, 2, ie);
+        double duds = af[0] * U(c, c->unold, 2, 1, ie) + af[2] * U(c, c->unold, 2, 2, ie);
+        double dvdr = af[1] * U(c, c->unold, 1, 1, ie) + af[3] * U(c, c->unold, 1, 2, ie);
+        double dvds = af[1] * U(c, c->unold, 2, 1, ie) + af[3] * U(c, c->unold, 2, 2, ie);
+        double uvx[4], vvx[4], pvx[4], a3[4];
+        for (int iv = 0; iv < 4; ++iv) {
+            double bq[3] = {1.0, (XVQ[iv] - rc) / 1.0, (YVQ[iv] - sc) / 1.0};
+            double u[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int id = 0; id < c->ndegr; ++id)
+                for (int q = 0; q < 4; ++q) u[q] = u[q] + U(c, c->unold, id, q, ie) * bq[id];
+            double rhovt = 1.0 / u[0];
+            double pv = dmax(eps, g1 * rhovt * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2])));
+            uvx[iv] = U(c, c->unold, 0, 1, ie) + dudr * bq[1] + duds * bq[2];
+            vvx[iv] = U(c, c->unold, 0, 2, ie) + dvdr * bq[1] + dvds * bq[2];
+            pvx[iv] = pctr + aflim[3 + 5 * ie] * (pv - pctr);
+            double ax = X(c->ustar, 0, ip[iv]) - uvx[iv];
+            double ay = X(c->ustar, 1, ip[iv]) - vvx[iv];
+            a3[iv] = sqrt(ax * ax + ay * ay);
+        }
+        double sdctr = sqrt(dmax(eps, c->gamlg * pctr / rhoct));
+        for (int iv = 0; iv < 4; ++iv) {
+            a3[iv] = a3[iv] / sdctr;
+            int p = ip[iv];
+            double dux = X(c->ustar, 0, p) - uvx[iv];
+            double duy = X(c->ustar, 1, p) - vvx[iv];
+            for (int f = 0; f < 2; ++f) {
+                double deltu = fabs(dux * vn[iv][f][0] + duy * vn[iv][f][1]);
+                double mur = rhoct * sdctr + c->cimpd * rhoct * c->slpdu * deltu;
+                double area = vn[iv][f][2];
+                /* getriecoef_matrixnew: aujmp(3) < 1e12 gives vnau = 1 */
+                double m11, m22;
+                if (a3[iv] < 1.e12) {
+                    m11 = mur * area * fabs(1.0);
+                    m22 = mur * area * fabs(1.0);
+                } else {
+                    double ax = dux, ay = duy, nrm = sqrt(ax * ax + ay * ay);
+                    double vnau = vn[iv][f][0] * (-ax / nrm) + vn[iv][f][1] * (-ay / nrm);
+                    m11 = mur * area * fabs(vnau);
+                    m22 = m11;
+                }
+                double m21 = 0.0, m12 = 0.0;
+                double mu1 = m11 * uvx[iv] + m21 * vvx[iv];
+                double mu2 = m21 * uvx[iv] + m22 * vvx[iv];
+                double sxx = -pvx[iv], sxy = 0.0, syx = 0.0, syy = -pvx[iv];
+                double sn1 = sxx * area * vn[iv][f][0] + sxy * area * vn[iv][f][1];
+                double sn2 = syx * area * vn[iv][f][0] + syy * area * vn[iv][f][1];
+                mn[0 + 4 * p] = mn[0 + 4 * p] + m11;
+                mn[1 + 4 * p] = mn[1 + 4 * p] + m21;
+                mn[2 + 4 * p] = mn[2 + 4 * p] + m12;
+                mn[3 + 4 * p] = mn[3 + 4 * p] + m22;
+                mu[0 + 2 * p] = mu[0 + 2 * p] + mu1;
+                mu[1 + 2 * p] = mu[1 + 2 * p] + mu2;
+                sn[0 + 2 * p] = sn[0 + 2 * p] + sn1;
+                sn[1 + 2 * p] = sn[1 + 2 * p] + sn2;
+                int o = f + 2 * iv;
+                mcl[0 + 4 * o + 32 * ie] = m11;
+                mcl[1 + 4 * o + 32 * ie] = m21;
+                mcl[2 + 4 * o + 32 * ie] = m12;
+                mcl[3 + 4 * o + 32 * ie] = m22;
+                mul[0 + 2 * o + 16 * ie] = mu1;
+                mul[1 + 2 * o + 16 * ie] = mu2;
+                snl[0 + 2 * o + 16 * ie] = sn1;
+                snl[1 + 2 * o + 16 * ie] = sn2;
+            }
+        }
+    }
+    /* fixed walls (code 25) keep their nodal velocity (Limiterlag.f90:476-493) */
+    int *indnd = (int *)calloc((size_t)np, sizeof(int));
+    for (int f = 0; f < c->nbfac; ++f) {
+        const int *b = c->bface + 5 * f;
+        if (b[2] == 25) {
+            indnd[b[0]] = 1; indnd[b[1]] = 1;
+            if (c->ncurv == 1) indnd[b[4]] = 1;
+        }
+    }
+    for (int p = 0; p < np; ++p) {
+        if (indnd[p]) continue;
+        double a11 = mn[0 + 4 * p], a21 = mn[1 + 4 * p], a12 = mn[2 + 4 * p], a22 = mn[3 + 4 * p];
+        double det = a11 * a22 - a21 * a12;
+        double i11 = a22 / det, i12 = -a12 / det, i21 = -a21 / det, i22 = a11 / det;
+        double r1 = mu[0 + 2 * p] - sn[0 + 2 * p];
+        double r2 = mu[1 + 2 * p] - sn[1 + 2 * p];
+        X(c->ustar, 0, p) = i11 * r1 + i12 * r2;
+        X(c->ustar, 1, p) = i21 * r1 + i22 * r2;
+    }
+    /* wall projection (Limiterlag.f90:607-651) */
+    for (int f = 0; f < c->nbfac; ++f) {
+        const int *b = c->bface + 5 * f;
+        if (b[2] == 22) {
+            if (b[3] == 221) { X(c->ustar, 1, b[0]) = 0.0; X(c->ustar, 1, b[1]) = 0.0; }
+            else if (b[3] == 222) { X(c->ustar, 0, b[0]) = 0.0; X(c->ustar, 0, b[1]) = 0.0; }
+        } else if (b[2] == 25) {
+            X(c->ustar, 1, b[0]) = 0.0; X(c->ustar, 1, b[1]) = 0.0;
+            X(c->ustar, 0, b[0]) = 0.0; X(c->ustar, 0, b[1]) = 0.0;
+        }
+    }
+    /* corner forces (Limiterlag.f90:680-698) */
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        for (int iv = 0; iv < 4; ++iv)
+            for (int f = 0; f < 2; ++f) {
+                int o = f + 2 * iv;
+                const double *m = mcl + 4 * o + 32 * ie;
+                double us1 = X(c->ustar, 0, ip[iv]), us2 = X(c->ustar, 1, ip[iv]);
+                fstarq[0 + 2 * o + 16 * ie] = snl[0 + 2 * o + 16 * ie] + m[0] * us1 + m[1] * us2 - mul[0 + 2 * o + 16 * ie];
+                fstarq[1 + 2 * o + 16 * ie] = snl[1 + 2 * o + 16 * ie] + m[2] * us1 + m[3] * us2 - mul[1 + 2 * o + 16 * ie];
+            }
+    }
+    free(indnd);
+    free(mn); free(mu); free(sn); free(mcl); free(mul); free(snl);
+}
+
+/* rhsifacedg_lag_mc_hybridquad2 (Getrhslagmc.f90:14468-14618): lumped P1
+ * face integral per corner; overwrites rhsel. */
+static void iface_q1(orc_case *c, const double *gq, const double *fstarq)
+{
+    const double c13 = 0.3333333333333333, c16 = 0.1666666666666666;
+    static const int fe[4][2] = {{3, 0}, {0, 1}, {1, 2}, {2, 3}};
+    const int nd = c->ndegr;
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double bq[4][3];
+        for (int iv = 0; iv < 4; ++iv) {
+            bq[iv][0] = 1.0;
+            bq[iv][1] = (XVQ[iv] - rc) / 1.0;
+            bq[iv][2] = (YVQ[iv] - sc) / 1.0;
+        }
+        double ul[3] = {0, 0, 0}, pl1[3] = {0, 0, 0}, pl2[3] = {0, 0, 0}, el[3] = {0, 0, 0};
+        for (int iv = 0; iv < 4; ++iv) {
+            double us1 = X(c->ustar, 0, ip[iv]), us2 = X(c->ustar, 1, ip[iv]);
+            const double *g0 = gq + 3 * fe[iv][0] + 12 * ie;
+            const double *g1 = gq + 3 * fe[iv][1] + 12 * ie;
+            const double *f = fstarq + 16 * ie + 4 * iv; /* f[d + 2*face] */
+            for (int id = 0; id < nd; ++id) {
+                double w = 2.0 * bq[iv][id] + bq[iv][id];
+                double l11 = c16 * w * g0[0] * g0[2], l21 = c16 * w * g0[1] * g0[2];
+                double l12 = c16 * w * g1[0] * g1[2], l22 = c16 * w * g1[1] * g1[2];
+                ul[id] = ul[id] + us1 * l11 + us2 * l21 + us1 * l12 + us2 * l22;
+                pl1[id] = pl1[id] + f[0] * c13 * w + f[2] * c13 * w;
+                pl2[id] = pl2[id] + f[1] * c13 * w + f[3] * c13 * w;
+                el[id] = el[id] + us1 * f[0] * c13 * w + us2 * f[1] * c13 * w +
+                         us1 * f[2] * c13 * w + us2 * f[3] * c13 * w;
+            }
+        }
+        for (int id = 0; id < nd; ++id) {
+            U(c, c->rhsel, id, 0, ie) = ul[id];
+            U(c, c->rhsel, id, 1, ie) = pl1[id];
+            U(c, c->rhsel, id, 2, ie) = pl2[id];
+            U(c, c->rhsel, id, 3, ie) = el[id];
+        }
+    }
+}
+
+/* rhsdomndg_lag_mc_quadsym (Limiterlag.f90:1108-1490): -int grad(b).F on the
+ * 5x5 Gauss rule with the cofactor form (djaco = w); P1 terms only.  The
+ * artificial stress of getAS_llnl is identically zero (Limiterlag.f90:15701). */
+static void domn_q1(orc_case *c, const double *aflim, const double *afvec)
+{
+    double pos[50], w[25];
+    ruqope2(5, pos, w);
+    const double g1 = c->gamlg - 1.0;
+    const double eps = 1.0e-6;
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        double xp[2][4];
+        for (int k = 0; k < 4; ++k) {
+            xp[0][k] = X(c->cooro, 0, ip[k]);
+            xp[1][k] = X(c->cooro, 1, ip[k]);
+        }
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double rhomc = U(c, c->unold, 0, 0, ie);
+        double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie);
+        double ectr = U(c, c->unold, 0, 3, ie);
+        double rhoct = 1.0 / rhomc;
+        double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        const double *af = afvec + 4 * ie;
+        double dudr = af[0] * U(c, c->unold, 1, 1, ie) + af[2] * U(c, c->unold, 1, 2, ie);
+        double duds = af[0] * U(c, c->unold, 2, 1, ie) + af[2] * U(c, c->unold, 2, 2, ie);
+        double dvdr = af[1] * U(c, c->unold, 1, 1, ie) + af[3] * U(c, c->unold, 1, 2, ie);
+        double dvds = af[1] * U(c, c->unold, 2, 1, ie) + af[3] * U(c, c->unold, 2, 2, ie);
+        for (int g = 0; g < 25; ++g) {
+            double r = pos[2 * g], s = pos[2 * g + 1], wi = w[g];
+            double rp = 1.0 + r, rm = 1.0 - r, sp = 1.0 + s, sm = 1.0 - s;
+            double sh[4] = {0.25 * rm * sm, 0.25 * rp * sm, 0.25 * rp * sp, 0.25 * rm * sp};
+            double dsr[4] = {-0.25 * sm, 0.25 * sm, 0.25 * sp, -0.25 * sp};
+            double dss[4] = {-0.25 * rm, -0.25 * rp, 0.25 * rp, 0.25 * rm};
+            double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+            for (int k = 0; k < 4; ++k) {
+                dxdr = dxdr + dsr[k] * xp[0][k];
+                dxds = dxds + dss[k] * xp[0][k];
+                dydr = dydr + dsr[k] * xp[1][k];
+                dyds = dyds + dss[k] * xp[1][k];
+            }
+            double yg = 0.0;
+            for (int k = 0; k < 4; ++k) yg = yg + sh[k] * xp[1][k];
+            double rcoef = 1.0 - c->alfrz + c->alfrz * yg;
+            double djaco = wi;
+            /* cofactor G . grad_ref(b) for b2 = r - rc, b3 = s - sc */
+            double gd[2][3];
+            double dbdr[3] = {0.0, 1.0 / 1.0, 0.0}, dbds[3] = {0.0, 0.0, 1.0 / 1.0};
+            for (int id = 0; id < 3; ++id) {
+                gd[0][id] = dyds * dbdr[id] + (-dydr) * dbds[id];
+                gd[1][id] = (-dxds) * dbdr[id] + dxdr * dbds[id];
+            }
+            double b[3] = {1.0, (r - rc) / 1.0, (s - sc) / 1.0};
+            double un[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int id = 0; id < c->ndegr; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + U(c, c->unold, id, q, ie) * b[id];
+            double rhoma = un[0];
+            double rhoad = 1.0 / rhoma;
+            double pres = g1 * rhoad * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]));
+            double uadv = U(c, c->unold, 0, 1, ie) + dudr * b[1] + duds * b[2];
+            double vadv = U(c, c->unold, 0, 2, ie) + dvdr * b[1] + dvds * b[2];
+            pres = pctr + aflim[3 + 5 * ie] * (pres - pctr);
+            double s11 = -pres + 0.0, s12 = 0.0 + 0.0, s21 = 0.0 + 0.0, s22 = -pres + 0.0;
+            double fl[3][4];
+            for (int id = 0; id < 3; ++id) {
+                fl[id][0] = gd[0][id] * uadv + gd[1][id] * vadv;
+                fl[id][1] = gd[0][id] * s11 + gd[1][id] * s12;
+                fl[id][2] = gd[0][id] * s21 + gd[1][id] * s22;
+                fl[id][3] = gd[0][id] * (s11 * uadv + s12 * vadv) + gd[1][id] * (s21 * uadv + s22 * vadv);
+            }
+            for (int id = 0; id < 3; ++id)
+                for (int q = 0; q < 4; ++q)
+                    U(c, c->rhsel, id, q, ie) = U(c, c->rhsel, id, q, ie) - fl[id][q] * djaco * rcoef;
+        }
+    }
+}
+
+/* getrhsdg_lag_mcnew, ncurv=0 / nriem=1 / nfint=1 / nlimi=6 / nrz=0 branch
+ * (Getrhslagmc.f90:6810-6939 and the derivative update :7283-7296). */
+static int rhs_q1(orc_case *c)
+{
+    if (c->npoly != 1 || c->nrz != 0 || c->ndens != 1 || c->nlimi != 6) return 2;
+    double *gq = (double *)malloc(sizeof(double) * 12 * c->nquad);
+    double *unmax = (double *)malloc(sizeof(double) * 6 * c->npoin);
+    double *unmin = (double *)malloc(sizeof(double) * 6 * c->npoin);
+    double *fstarq = (double *)malloc(sizeof(double) * 16 * c->nquad);
+    fnds_q1(c, c->cooro, gq);
+    vtx_minmax(c, unmax, unmin);
+    barth_quad(c, unmax, unmin, c->aflim, c->afvec);
+    ndvelo_q1(c, gq, c->aflim, c->afvec, fstarq);
+    iface_q1(c, gq, fstarq);
+    domn_q1(c, c->aflim, c->afvec);
+    for (int ie = 0; ie < c->ncell; ++ie) {
+        const double *al = c->aflim + 5 * ie, *af = c->afvec + 4 * ie;
+        U(c, c->unold, 1, 0, ie) = U(c, c->unold, 1, 0, ie) * al[0];
+        U(c, c->unold, 2, 0, ie) = U(c, c->unold, 2, 0, ie) * al[0];
+        double dudr = af[0] * U(c, c->unold, 1, 1, ie) + af[2] * U(c, c->unold, 1, 2, ie);
+        double duds = af[0] * U(c, c->unold, 2, 1, ie) + af[2] * U(c, c->unold, 2, 2, ie);
+        double dvdr = af[1] * U(c, c->unold, 1, 1, ie) + af[3] * U(c, c->unold, 1, 2, ie);
+        double dvds = af[1] * U(c, c->unold, 2, 1, ie) + af[3] * U(c, c->unold, 2, 2, ie);
+        U(c, c->unold, 1, 1, ie) = dudr;
+        U(c, c->unold, 2, 1, ie) = duds;
+        U(c, c->unold, 1, 2, ie) = dvdr;
+        U(c, c->unold, 2, 2, ie) = dvds;
+        U(c, c->unold, 1, 3, ie) = U(c, c->unold, 1, 3, ie) * al[4];
+        U(c, c->unold, 2, 3, ie) = U(c, c->unold, 2, 3, ie) * al[4];
+    }
+    free(gq); free(unmax); free(unmin); free(fstarq);
+    return 0;
+}
+
+int orc_rhs(orc_case *c)
+{
+    if (c->ncurv == 0 && c->nfint == 1) return rhs_q1(c);
+    return 2;
+}
+
+/* getdelt_lag (Getrhslag.f90:20600-20713): dt on stage 1 from the CFL
+ * condition over quads (edge length / sound speed) and the volume change. */
+static void delt(orc_case *c, const double *dvdt)
+{
+    const double eps = 1.0e-6, g1 = c->gamlg - 1.0;
+    double dtcfl = 1e10, dtvol = 1e10;
+    for (int ie = 0; ie < c->nquad; ++ie) {
+        const int *ip = c->ipqua + c->nvqua * ie;
+        double xp[2][9];
+        int nv = c->ncurv == 0 ? 4 : 9;
+        for (int k = 0; k < nv; ++k) {
+            xp[0][k] = X(c->coord, 0, ip[k]);
+            xp[1][k] = X(c->coord, 1, ip[k]);
+        }
+        double dl[4];
+#define LEN(a, b) sqrt((xp[0][a] - xp[0][b]) * (xp[0][a] - xp[0][b]) + (xp[1][a] - xp[1][b]) * (xp[1][a] - xp[1][b]))
+        if (c->ncurv == 0) {
+            dl[0] = LEN(0, 1); dl[1] = LEN(1, 2); dl[2] = LEN(2, 3); dl[3] = LEN(3, 0);
+        } else {
+            dl[0] = LEN(0, 4) + LEN(4, 1);
+            dl[1] = LEN(1, 5) + LEN(5, 2);
+            dl[2] = LEN(2, 6) + LEN(6, 3);
+            dl[3] = LEN(3, 7) + LEN(7, 0);
+        }
+#undef LEN
+        double rhomc = U(c, c->unold, 0, 0, ie);
+        double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie);
+        double ectr = U(c, c->unold, 0, 3, ie);
+        double rhoct = 1.0 / rhomc;
+        double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        double sdctr = sqrt(dmax(eps, c->gamlg * pctr / rhoct));
+        double dlen = 1.e300;
+        for (int k = 0; k < 4; ++k) dlen = dmin(dlen, dl[k]);
+        dtcfl = dmin(dtcfl, dlen / sdctr);
+    }
+    for (int ie = 0; ie < c->nquad; ++ie) dtvol = dmin(dtvol, GE(c, 2, ie) / fabs(dvdt[ie]));
+    if (c->itime == 1)
+        c->dtfix = dmin(c->cfl * dtcfl, 0.1 * dtvol);
+    else
+        c->dtfix = dmin(dmin(c->cfl * dtcfl, 0.1 * dtvol), 1.01 * c->dtfix);
+}
+
+/* getsolvdg_lag (Getrhslag.f90:1553-2035), ndens=1, planar, and the driver's
+ * copy-back of stage state into U^n and x^n (dgflo2d.f90:408-422). */
+int orc_step(orc_case *c)
+{
+    static const double rklag[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
+    const int nd = c->ndegr, nq = c->nq;
+    orc_cellvol(c, c->coord);
+    double *dvdt = (double *)malloc(sizeof(double) * c->ncell);
+    double m[6][6];
+    for (int istag = 0; istag < c->nstag; ++istag) {
+        double alfa = rklag[c->nstag - 2][istag];
+        int rc = orc_rhs(c);
+        if (rc) { free(dvdt); return rc; }
+        if (c->ndt == 1) {
+            if (istag == 0) {
+                for (int ie = 0; ie < c->ncell; ++ie) dvdt[ie] = U(c, c->rhsel, 0, 0, ie);
+                delt(c, dvdt);
+                if (c->tacru + c->dtfix > c->tend) c->dtfix = c->tend - c->tacru;
+                c->tacru = c->tacru + c->dtfix;
+            }
+        } else if (istag == 0) {
+            c->tacru = c->tacru + c->dtfix;
+        }
+        for (int i = 0; i < nd * nq * c->ncell; ++i) c->rhsel[i] = c->dtfix * c->rhsel[i];
+        for (int ie = 0; ie < c->ncell; ++ie) {
+            const double *a = c->amatr + c->nmatr * ie;
+            if (c->npoly == 1) {
+                m[0][0] = a[3]; m[0][1] = 0.0; m[0][2] = 0.0;
+                m[1][0] = m[0][1]; m[1][1] = a[0]; m[1][2] = a[1];
+                m[2][0] = m[0][2]; m[2][1] = a[1]; m[2][2] = a[2];
+            } else {
+                for (int i = 0; i < 6; ++i)
+                    for (int j = 0; j < 6; ++j) m[i][j] = 0.0;
+                m[0][0] = a[15];
+                int k = 0;
+                for (int i = 1; i < nd; ++i)
+                    for (int j = i; j < nd; ++j) {
+                        m[i][j] = a[k];
+                        m[j][i] = a[k];
+                        ++k;
+                    }
+            }
+            for (int id = 0; id < nd; ++id) {
+                double un[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int k = 0; k < nd; ++k)
+                    for (int q = 0; q < nq; ++q) un[q] = un[q] + m[id][k] * U(c, c->rhsel, k, q, ie);
+                for (int q = 0; q < nq; ++q)
+                    U(c, c->unold, id, q, ie) = alfa * U(c, c->unkno, id, q, ie) +
+                                                (1.0 - alfa) * (U(c, c->unold, id, q, ie) + un[q]);
+            }
+        }
+        for (int p = 0; p < c->npoin; ++p)
+            for (int d = 0; d < 2; ++d)
+                X(c->cooro, d, p) = alfa * X(c->coord, d, p) +
+                                    (1.0 - alfa) * (X(c->cooro, d, p) + c->dtfix * X(c->ustar, d, p));
+        if (c->ncurv == 1) {
+            for (int ie = 0; ie < c->nquad; ++ie) {
+                const int *ip = c->ipqua + c->nvqua * ie;
+                for (int d = 0; d < 2; ++d) {
+                    double x1 = X(c->cooro, d, ip[0]), x2 = X(c->cooro, d, ip[1]);
+                    double x3 = X(c->cooro, d, ip[2]), x4 = X(c->cooro, d, ip[3]);
+                    double x5 = X(c->cooro, d, ip[4]), x6 = X(c->cooro, d, ip[5]);
+                    double x7 = X(c->cooro, d, ip[6]), x8 = X(c->cooro, d, ip[7]);
+                    X(c->cooro, d, ip[8]) = -0.25 * (x1 + x2 + x3 + x4) + 0.5 * (x5 + x6 + x7 + x8);
+                }
+            }
+        }
+        orc_cellvol(c, c->cooro);
+    }
+    /* driver copy-back */
+    memcpy(c->unkno, c->unold, sizeof(double) * nd * nq * c->ncell);
+    memcpy(c->coord, c->cooro, sizeof(double) * 2 * c->npoin);
+    free(dvdt);
+    return 0;
+}
+
+double orc_flops(void) { return 0.0; }

@@ -1,0 +1,61 @@
+/* CPU restatement of the reference's Lagrangian DG step (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker the parity tests, the
+ * smoke check and the bench's CPU-baseline leg compare the HIP path against;
+ * the product never links or calls it.
+ *
+ * Layouts follow the reference's Fortran arrays (column-major), with 0-based
+ * indices:
+ *   unkno[id + ndegr*(iq + nq*ie)]      unkno(ndegr,nq,nsize)  dgflo2d.f90:121
+ *   geoel[k + 24*ie]                    geoel(24,nsize)        dgflo2d.f90:119
+ *   amatr[k + nmatr*ie]                 amatr(nmatr,ncell)     dgflo2d.f90:238-245
+ *   coord[d + 2*ip]                     coord(2,npoin)         dgflo2d.f90:118
+ *   ipqua[iv + nvqua*ie]   0-based nodes
+ *   bface[k + 5*ifa]       nodes (k=0,1,4) 0-based; k=2 BC code, k=3 subcode
+ *   esuv2[npoin+1] offsets, esuv1[] 0-based element ids   Preprocess.f90:720-766
+ */
+#ifndef LAG2D_ORACLE_H
+#define LAG2D_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_case {
+    /* sizes */
+    int ncell, nquad, npoin, nbfac;
+    int ndegr, nq, npoly, ncurv, nvqua, nmatr;
+    /* switches (dgflo2d.ctrl, Preprocess.f90:28-55) */
+    int nstag, ndens, nlimi, ncase, nrz, nfint, ndt;
+    double gamlg, cfl, cimpd, cdrho, tend;
+    double slpdu, alfrz;            /* dgflo2d.f90:147-156 */
+    /* mutable driver scalars (module constant) */
+    double dtfix, tacru;
+    int itime;
+    /* mesh */
+    const int *ipqua, *bface, *esuv1, *esuv2;
+    /* state: coord = x^n, cooro = stage x, coold = x^0 */
+    double *coord, *cooro, *coold;
+    double *geoel, *amatr;
+    double *unkno, *unold;          /* U^n and stage state, nsize cells */
+    double *ustar;                  /* nodal velocity (2, npoin) */
+    /* per-stage outputs kept for inspection */
+    double *rhsel;                  /* (ndegr, nq, ncell) */
+    double *aflim, *afvec;          /* (nq+1, ncell), (2,2,ncell) */
+} orc_case;
+
+/* getcellvol_lag (Limiterlag.f90:1869-2164), quads */
+void orc_cellvol(orc_case *c, const double *x);
+/* getrhsdg_lag_mcnew (Getrhslagmc.f90:6693-8731), one stage right-hand side
+ * on the stage coordinates; limits c->unold in place like the reference. */
+int orc_rhs(orc_case *c);
+/* getsolvdg_lag (Getrhslag.f90:1553-2035), one full SSP-RK step followed by
+ * the driver's copy-back (dgflo2d.f90:408-422). */
+int orc_step(orc_case *c);
+/* operation counter (FP64 flops) of the last orc_step, when built with
+ * ORC_COUNT_FLOPS */
+double orc_flops(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
