@@ -65,11 +65,18 @@ def _boundary_faces(nid, nx, ny, q2):
     return np.asarray(faces, dtype=np.int32)
 
 
-def quad_q1(nx, ny, x0, x1, y0, y1) -> Mesh:
-    """Uniform straight quads (ncurv=0), nodes numbered row by row."""
+def quad_q1(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1) -> Mesh:
+    """Straight quads (ncurv=0), nodes numbered row by row; with ``seed`` set
+    the interior vertices move by U(-jitter*h, jitter*h) per coordinate."""
     xs = np.linspace(x0, x1, nx + 1)
     ys = np.linspace(y0, y1, ny + 1)
     X, Y = np.meshgrid(xs, ys, indexing="xy")
+    if seed is not None and nx > 1 and ny > 1:
+        h = min((x1 - x0) / nx, (y1 - y0) / ny)
+        rng = np.random.default_rng(seed)
+        d = rng.uniform(-jitter * h, jitter * h, size=(ny - 1, nx - 1, 2))
+        X[1:-1, 1:-1] += d[..., 0]
+        Y[1:-1, 1:-1] += d[..., 1]
     coord = np.stack([X.ravel(), Y.ravel()], axis=1)
 
     def nid(I, J):
@@ -147,6 +154,33 @@ def quad_q2(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1, bulge=0.05) -> Mesh:
     return Mesh(coord, ipq, _boundary_faces(nid, nx, ny, True), 1, nx, ny)
 
 
+SEDOV_ENERGY = 0.244816   # quadrant blast energy (Getrhslagmc.f90:5623-5639)
+
+
+def sedov_inject(unkno, geoel):
+    """Move the reference initialisation's blast energy to the corner cell.
+
+    The reference deposits it in four hard-coded element ids
+    (Getrhslagmc.f90:5634-5639); the benchmark places the quadrant blast in
+    cell 1, the cell at the origin: its mass-averaged specific total energy
+    becomes E0/m (rho = 1, u = 0), every other cell keeps the ambient state.
+    ``unkno`` has shape (ncell, 4, ndegr); returns a modified copy."""
+    u = unkno.copy()
+    ncell = u.shape[0]
+    amb = None
+    for e in range(ncell):
+        if e not in (1769, 1770, 1829, 1830):
+            amb = u[e, 3, 0]
+            break
+    for e in (1769, 1770, 1829, 1830):
+        if e < ncell:
+            u[e, 3, :] = 0.0
+            u[e, 3, 0] = amb
+    u[0, 3, :] = 0.0
+    u[0, 3, 0] = SEDOV_ENERGY / geoel[0, 3]
+    return u
+
+
 # --------------------------------------------------------------------------
 # problem definitions (the control-file switches of each benchmark problem)
 
@@ -177,6 +211,15 @@ class Control:
 def sod_problem(nx=100, ny=4):
     mesh = quad_q1(nx, ny, 0.0, 1.0, 0.0, 0.04 * ny / 4.0)
     return mesh, Control()
+
+
+def sedov_q1_problem(nx=16, ny=None, seed=7, length=1.2):
+    """Planar Sedov blast on a jittered straight mesh, P1 / Barth limiter
+    (the straight-quad path with genuinely two-dimensional flow)."""
+    ny = nx if ny is None else ny
+    mesh = quad_q1(nx, ny, 0.0, length, 0.0, length * ny / nx, seed=seed)
+    ctl = Control(cfl=0.25, tend=1.0, ncase=7)
+    return mesh, ctl
 
 
 def sedov_problem(nx=120, ny=None, seed=2024, length=1.2):

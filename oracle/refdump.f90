@@ -143,7 +143,7 @@ do itime = 1, nsteps
     enddo
   enddo
   coord = cooro
-  if(mod(itime,every).eq.0 .or. itime.eq.nsteps)then
+  if(mod(itime,every).eq.0 .or. itime.eq.nsteps .or. abs(tacru-tend)/tend.le.1e-11)then
     write(tag,'(a,i0)') 's', itime
     call dumpr(tag,'unkno',unkno,size(unkno))
     call dumpr(tag,'coord',coord,size(coord))

@@ -1,0 +1,115 @@
+"""Write the golden fixtures from the reference solver (run in the build
+container only; the reference never travels to the GPU box).
+
+Each fixture is the reference's own initial state for a small problem, the
+result of one right-hand side on it, and its state after a number of steps,
+produced by oracle/_ref/lag2d_refdump (oracle/refdump.f90 driving the
+reference's subroutines, built by ``make -f oracle/ref.mk``).  The .npz files
+hold data only: inputs and expected outputs.
+
+    python tests/golden/gen_fixtures.py [name ...]
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from lag2d_amd import meshgen as mg  # noqa: E402
+
+REFDUMP = os.path.join(ROOT, "oracle", "_ref", "lag2d_refdump")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def _sod():
+    return mg.sod_problem(100, 4), None
+
+
+def _sedov_q1():
+    return mg.sedov_q1_problem(16), "sedov"
+
+
+def _sedov_q2():
+    return mg.sedov_problem(12, seed=11), "sedov"
+
+
+SPECS = {
+    # name: (builder, steps, dump-every)
+    "sod_q1_100x4": (_sod, 400, 25),
+    "sedov_q1_16x16": (_sedov_q1, 60, 20),
+    "sedov_q2_12x12": (_sedov_q2, 20, 5),
+}
+
+
+def _run(work, nsteps, every, inject):
+    with open(os.path.join(work, "refdump.in"), "w") as f:
+        f.write(f"{nsteps} {every} {1 if inject else 0}\n")
+    r = subprocess.run([REFDUMP], cwd=work, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference run failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+
+
+def _rd(work, name, dtype=np.float64):
+    return np.fromfile(os.path.join(work, name), dtype=dtype)
+
+
+def make(name):
+    build, nsteps, every = SPECS[name]
+    (mesh, ctl), ic = build()
+    work = tempfile.mkdtemp(prefix=f"lag2d_{name}_")
+    try:
+        mg.write_case(work, mesh, ctl)
+        nd = ctl.ndegr
+        _run(work, 0, 1, False)
+        ncell, npoin, nbfac, nafac = (int(v) for v in _rd(work, "init_sizes.bin"))
+        if ic == "sedov":
+            un = _rd(work, "init_unkno.bin").reshape(-1, 4, nd)[:ncell]
+            ge = _rd(work, "init_geoel.bin").reshape(-1, 24)[:ncell]
+            inj = mg.sedov_inject(un, ge)
+            inj.astype(np.float64).tofile(os.path.join(work, "inject.bin"))
+        _run(work, nsteps, every, ic is not None)
+        nvqua = 4 if ctl.ncurv == 0 else 9
+        nmatr = 4 if ctl.npoly == 1 else 16
+        d = {
+            "ctl": np.array(json.dumps(dataclasses.asdict(ctl))),
+            "ipqua": _rd(work, "init_ipqua.bin", np.int32).reshape(ncell, nvqua),
+            "bface": _rd(work, "init_bface.bin", np.int32).reshape(nbfac, 5),
+            "esuv1": _rd(work, "init_esuv1.bin", np.int32),
+            "esuv2": _rd(work, "init_esuv2.bin", np.int32),
+            "coord": _rd(work, "init_coord.bin").reshape(npoin, 2),
+            "geoel": _rd(work, "init_geoel.bin").reshape(-1, 24)[:ncell],
+            "amatr": _rd(work, "init_amatr.bin").reshape(ncell, nmatr),
+            "unkno": _rd(work, "init_unkno.bin").reshape(-1, 4, nd)[:ncell],
+            "rhs_rhsel": _rd(work, "rhs_rhsel.bin").reshape(ncell, 4, nd),
+            "rhs_ustar": _rd(work, "rhs_ustar.bin").reshape(npoin, 2),
+            "rhs_unkno": _rd(work, "rhs_unkno.bin").reshape(-1, 4, nd)[:ncell],
+            "rhs_geoel": _rd(work, "rhs_geoel.bin").reshape(-1, 24)[:ncell],
+        }
+        steps = sorted(int(f[1:].split("_")[0]) for f in os.listdir(work)
+                       if f.startswith("s") and f.endswith("_time.bin"))
+        d["steps"] = np.array(steps, dtype=np.int32)
+        for k in steps:
+            d[f"s{k}_unkno"] = _rd(work, f"s{k}_unkno.bin").reshape(-1, 4, nd)[:ncell]
+            d[f"s{k}_coord"] = _rd(work, f"s{k}_coord.bin").reshape(npoin, 2)
+            d[f"s{k}_ustar"] = _rd(work, f"s{k}_ustar.bin").reshape(npoin, 2)
+            d[f"s{k}_time"] = _rd(work, f"s{k}_time.bin")[:2]
+            d[f"s{k}_geoel"] = _rd(work, f"s{k}_geoel.bin").reshape(-1, 24)[:ncell, [2, 4, 5]]
+        path = os.path.join(OUT, f"{name}.npz")
+        np.savez_compressed(path, **d)
+        print(f"{name}: ncell={ncell} npoin={npoin} steps={steps[-1] if steps else 0} -> {path}")
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or list(SPECS)
+    for n in names:
+        make(n)
