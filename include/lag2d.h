@@ -1,0 +1,119 @@
+/* lag2d: MI355X (gfx950) implementation of the Lagrangian DG right-hand side
+ * and SSP-RK step of liuspace/Lag2D (srccurv-lag), behind a plain C ABI.
+ *
+ * The reference has no library boundary of its own: its time loop calls
+ *   getsolvdg_lag(...)        once per step   (dgflo2d.f90:372-373)
+ * which calls
+ *   getrhsdg_lag_mcnew(...)   once per stage  (Getrhslag.f90:1747-1750).
+ * The entry points below replace those two subroutines.  A Fortran caller
+ * binds them with ISO_C_BINDING (lag2d_amd/fortran/lag2d_shim.f90); arrays are
+ * the reference's own column-major arrays, connectivity 1-based, sizes and
+ * switches those of module `constant` (dgflo2d.f90:4-57).
+ *
+ * Every entry returns 0 on success or one of the LAG2D_E* codes; the message
+ * of the last failure on the calling thread is lag2d_last_error().
+ */
+#ifndef LAG2D_H
+#define LAG2D_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LAG2D_OK = 0,
+    LAG2D_EINVAL = 1,        /* bad argument or size */
+    LAG2D_EUNSUPPORTED = 2,  /* switch combination not implemented */
+    LAG2D_EDEVICE = 3,       /* HIP runtime failure */
+    LAG2D_ENONFINITE = 4,    /* non-finite state detected by lag2d_check */
+    LAG2D_EVOLUME = 5        /* non-positive cell volume detected by lag2d_check */
+};
+
+/* Switches and driver scalars of module `constant` (dgflo2d.f90:4-57) that the
+ * Lagrangian step reads.  dtfix and tacru are in/out like the module globals
+ * getsolvdg_lag mutates (Getrhslag.f90:1663-1785, :20705-20709). */
+typedef struct lag2d_config {
+    int ndegr;    /* 3 (P1) or 6 (P2) */
+    int npoly;    /* 1 or 2 */
+    int ncurv;    /* 0 straight quads (4 nodes), 1 quadratic quads (9 nodes) */
+    int nstag;    /* 2: SSP-RK2, 3: SSP-RK3 */
+    int ndens;    /* 1: specific volume evolved as a DG variable */
+    int nlimi;    /* 6 */
+    int ncase;    /* test-problem id (only boundary-colouring rules read it) */
+    int nrz;      /* 0 planar */
+    int nfint;    /* 1 (ncurv=0) or 4 (ncurv=1 sub-cell scheme) */
+    int nriem;    /* 1 */
+    int ndt;      /* 1: CFL-controlled dt, 0: fixed dtfix */
+    double gamlg, cfl, cimpd, cdrho, tend;
+    double dtfix, tacru;
+} lag2d_config;
+
+typedef struct lag2d_ctx lag2d_ctx;
+
+/* Create a context and upload the mesh (Preprocess.f90 outputs):
+ *   ipqua(nvqua, nquad)  quad nodes, 1-based, nvqua = 4 or 9
+ *   bface(5, nbfac)      n1, n2, BC code, BC sub-code, mid node (1-based)
+ *   esuv1(esuv2(npoin+1)), esuv2(npoin+1)  elements around points (CSR)
+ * Quads only (ntria = 0). */
+int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
+                 const int *ipqua, const int *bface, const int *esuv1,
+                 const int *esuv2, lag2d_ctx **ctx);
+
+/* Upload the solution state (reference layouts, nsize >= ncell):
+ *   coord(2,npoin) x^n, coold(2,npoin) x^0, geoel(24,ncell), amatr(nmatr,ncell),
+ *   unkno(ndegr,4,ncell), ustar(2,npoin).  Recomputes geoel(3,5,6) from coord
+ *   as getcellvol_lag does (Limiterlag.f90:1869). */
+int lag2d_upload(lag2d_ctx *ctx, const double *coord, const double *coold,
+                 const double *geoel, const double *amatr, const double *unkno,
+                 const double *ustar);
+
+/* One SSP-RK time step: getsolvdg_lag (Getrhslag.f90:1553-2035) followed by
+ * the driver's copy of the stage state into unkno/coord
+ * (dgflo2d.f90:408-422).  itime is the driver's 1-based step counter.
+ * Asynchronous on the context's stream. */
+int lag2d_step(lag2d_ctx *ctx, int itime);
+
+/* nsteps steps (itime0, itime0+1, ...), stopping after the step that reaches
+ * tend, as the driver does (dgflo2d.f90:458-463).  Returns the number taken
+ * in *taken (synchronises). */
+int lag2d_run(lag2d_ctx *ctx, int itime0, int nsteps, int *taken);
+
+/* One right-hand side: getrhsdg_lag_mcnew (Getrhslagmc.f90:6693-8731) on the
+ * current state x^n / unkno, with ustar as the previous nodal velocity.
+ * Writes rhsel(ndegr,4,ncell) and updates the resident ustar and the limited
+ * DOFs (the reference limits its unkno argument in place,
+ * Getrhslagmc.f90:7283-7296).  Synchronises. */
+int lag2d_rhs(lag2d_ctx *ctx, double *rhsel);
+
+/* Download the state (same layouts as lag2d_upload); NULL pointers skip.
+ * geoel receives rows 3, 5, 6 (current volume and volume centre) updated and
+ * the uploaded values elsewhere. */
+int lag2d_download(lag2d_ctx *ctx, double *coord, double *geoel, double *unkno,
+                   double *ustar);
+
+/* Driver scalars after the last step (module constant dtfix, tacru). */
+int lag2d_get_time(lag2d_ctx *ctx, double *dtfix, double *tacru);
+
+/* Scan the resident state for non-finite DOFs or non-positive volumes:
+ * returns LAG2D_ENONFINITE / LAG2D_EVOLUME (the reference has no such check
+ * and would propagate them silently). */
+int lag2d_check(lag2d_ctx *ctx);
+
+/* Wait for all work queued on the context's stream. */
+int lag2d_sync(lag2d_ctx *ctx);
+
+/* Kernel timing: average device time (ms) of the dominant kernel over the
+ * last lag2d_run, measured with HIP events on the context's stream, and its
+ * name.  Returns LAG2D_EINVAL if no timed run happened. */
+int lag2d_kernel_time(lag2d_ctx *ctx, int enable, double *avg_ms, const char **name);
+
+/* Per-kernel totals of the last timed run: kernel index k = 0, 1, ...;
+ * returns LAG2D_EINVAL past the last kernel. */
+int lag2d_kernel_stats(lag2d_ctx *ctx, int k, double *total_ms, int *calls, const char **name);
+
+void lag2d_destroy(lag2d_ctx *ctx);
+const char *lag2d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

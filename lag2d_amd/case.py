@@ -77,6 +77,22 @@ def build_esuv(ipqua0: np.ndarray, npoin: int):
     return esuv1, esuv2
 
 
+def face_neighbours(ipqua0: np.ndarray, esuv1: np.ndarray, esuv2: np.ndarray) -> np.ndarray:
+    """Quad face neighbours [ncell, 4]: face f joins corners f and f+1 (CCW);
+    the value is the neighbour's 0-based id, or ncell + f for a boundary face
+    (the reference keeps a ghost-cell id there, dgflo2d.f90:1220-1330)."""
+    nc = ipqua0.shape[0]
+    out = np.empty((nc, 4), dtype=np.int32)
+    for ie in range(nc):
+        for f in range(4):
+            a, b = ipqua0[ie, f], ipqua0[ie, (f + 1) % 4]
+            ea = set(esuv1[esuv2[a]:esuv2[a + 1]].tolist())
+            eb = set(esuv1[esuv2[b]:esuv2[b + 1]].tolist())
+            nb = sorted((ea & eb) - {ie})
+            out[ie, f] = nb[0] if nb else nc + f
+    return out
+
+
 def mesh_arrays(mesh: Mesh):
     ipq = (mesh.ipqua - 1).astype(np.int32)
     bf = mesh.bface.astype(np.int32).copy()

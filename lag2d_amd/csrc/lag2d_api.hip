@@ -1,0 +1,503 @@
+// Host side of the C ABI (include/lag2d.h): context, uploads in the
+// reference's array layouts, and the per-stage launch sequence of the step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lag2d.h"
+#include "lag2d_dev.h"
+#include "lag2d_kernels.h"
+#include "lag2d_host.h"
+
+using namespace lag2d;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return fail(LAG2D_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *lag2d_last_error(void) { return g_err.c_str(); }
+
+template <class T>
+static int dalloc(T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(LAG2D_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+static inline int nblocks(int n) { return (n + LAG2D_BLOCK - 1) / LAG2D_BLOCK; }
+
+lag2d_ctx::~lag2d_ctx()
+{
+    for (void *p : allocs) hipFree(p);
+    for (auto &t : timers) {
+        for (auto &ev : t.ev) {
+            hipEventDestroy(ev.first);
+            hipEventDestroy(ev.second);
+        }
+    }
+    if (stream) hipStreamDestroy(stream);
+}
+
+template <class T>
+int lag2d_ctx::alloc(T **p, size_t n)
+{
+    int rc = dalloc(p, n);
+    if (!rc) allocs.push_back((void *)*p);
+    return rc;
+}
+
+extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac, const int *ipqua,
+                            const int *bface, const int *esuv1, const int *esuv2, lag2d_ctx **out)
+{
+    if (!cfg || !out || !ipqua || !bface || !esuv1 || !esuv2) return fail(LAG2D_EINVAL, "null argument");
+    if (nquad <= 0 || npoin <= 0 || nbfac < 0) return fail(LAG2D_EINVAL, "bad sizes");
+    const lag2d_config &c = *cfg;
+    const bool q1 = c.ncurv == 0 && c.nfint == 1 && c.npoly == 1 && c.ndegr == 3;
+    const bool q2 = c.ncurv == 1 && c.nfint == 4 && c.npoly == 2 && c.ndegr == 6;
+    if (!(q1 || q2))
+        return fail(LAG2D_EUNSUPPORTED, "supported: ncurv=0/nfint=1/P1 or ncurv=1/nfint=4/P2");
+    if (c.nrz != 0 || c.ndens != 1 || c.nlimi != 6 || c.nriem != 1)
+        return fail(LAG2D_EUNSUPPORTED, "supported: nrz=0, ndens=1, nlimi=6, nriem=1");
+    if (c.nstag != 2 && c.nstag != 3) return fail(LAG2D_EUNSUPPORTED, "nstag must be 2 or 3");
+    if (q2) return fail(LAG2D_EUNSUPPORTED, "curved P2 path not built in this library");
+
+    lag2d_ctx *x = new lag2d_ctx();
+    x->cfg = c;
+    const int nv = c.ncurv == 0 ? 4 : 9;
+    x->nc = nquad;
+    x->np = npoin;
+    x->nb = nbfac;
+    x->nd = c.ndegr;
+    x->nvq = nv;
+    x->nmatr = c.npoly == 1 ? 4 : 16;
+    int rc = 0;
+    auto bad = [&](int r) {
+        delete x;
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(fail(LAG2D_EDEVICE, "hipStreamCreate failed"));
+
+    // ---- host-side mesh preprocessing (0-based, SoA)
+    std::vector<int> ipq((size_t)nv * nquad);
+    for (int ie = 0; ie < nquad; ++ie)
+        for (int k = 0; k < nv; ++k) {
+            int p = ipqua[(size_t)k + (size_t)nv * ie] - 1;
+            if (p < 0 || p >= npoin) return bad(fail(LAG2D_EINVAL, "ipqua node out of range"));
+            ipq[(size_t)k * nquad + ie] = p;
+        }
+    const int ne = esuv2[npoin];
+    if (ne != nv * nquad) return bad(fail(LAG2D_EINVAL, "esuv2(npoin+1) != nvqua*nquad"));
+    std::vector<int> e1(ne), e2(esuv2, esuv2 + npoin + 1);
+    for (int k = 0; k < ne; ++k) {
+        e1[k] = esuv1[k] - 1;
+        if (e1[k] < 0 || e1[k] >= nquad) return bad(fail(LAG2D_EINVAL, "esuv1 element out of range"));
+    }
+    for (int p = 0; p < npoin; ++p)
+        if (e2[p + 1] < e2[p]) return bad(fail(LAG2D_EINVAL, "esuv2 not monotone"));
+    // node -> corner CSR in the reference's accumulation order
+    std::vector<int> n2c, n2c_off;
+    if ((rc = build_node_corners(*x, ipq, e1, e2, n2c, n2c_off))) return bad(rc);
+    // boundary flags and the symmetry colouring of the velocity limiter
+    std::vector<uint8_t> nflag(npoin, 0), ncol(npoin, 0);
+    for (int f = 0; f < nbfac; ++f) {
+        const int *b = bface + 5 * (size_t)f;
+        int nodes[3] = {b[0] - 1, b[1] - 1, nv == 9 ? b[4] - 1 : -1};
+        for (int k = 0; k < 3; ++k)
+            if (nodes[k] >= npoin) return bad(fail(LAG2D_EINVAL, "bface node out of range"));
+        if (nodes[0] < 0 || nodes[1] < 0 || (nv == 9 && nodes[2] < 0))
+            return bad(fail(LAG2D_EINVAL, "bface node out of range"));
+        // wall projection acts on the face's two end nodes (Limiterlag.f90:608-619);
+        // fixed walls exclude all face nodes from the solve (:476-481)
+        if (b[2] == 22) {
+            for (int k = 0; k < 2; ++k) nflag[nodes[k]] |= (b[3] == 221) ? 1 : (b[3] == 222 ? 2 : 0);
+        } else if (b[2] == 25) {
+            for (int k = 0; k < 2; ++k) nflag[nodes[k]] |= 3;
+            for (int k = 0; k < (nv == 9 ? 3 : 2); ++k) nflag[nodes[k]] |= 4;
+        }
+    }
+    if (c.ncase != 1) {  // Limiterlag.f90:8212-8238
+        const int nf = nv == 9 ? 3 : 2;
+        for (int f = 0; f < nbfac; ++f) {
+            const int *b = bface + 5 * (size_t)f;
+            int nodes[3] = {b[0] - 1, b[1] - 1, nv == 9 ? b[4] - 1 : 0};
+            if (b[2] == 22 && b[3] == 221)
+                for (int k = 0; k < nf; ++k) ncol[nodes[k]] = 3;
+        }
+        for (int f = 0; f < nbfac; ++f) {
+            const int *b = bface + 5 * (size_t)f;
+            int nodes[3] = {b[0] - 1, b[1] - 1, nv == 9 ? b[4] - 1 : 0};
+            if (b[2] == 22 && b[3] == 222)
+                for (int k = 0; k < nf; ++k) ncol[nodes[k]] = (ncol[nodes[k]] == 3) ? 10 : 2;
+        }
+    }
+
+    Dev &d = x->d;
+    memset(&d, 0, sizeof(d));
+    d.nc = nquad; d.np = npoin; d.nb = nbfac; d.nd = c.ndegr; d.nvq = nv; d.nmatr = x->nmatr;
+    d.nstag = c.nstag; d.ncase = c.ncase; d.ncurv = c.ncurv; d.npoly = c.npoly; d.ndt = c.ndt;
+    d.gamlg = c.gamlg; d.cfl = c.cfl; d.cimpd = c.cimpd; d.cdrho = c.cdrho; d.tend = c.tend;
+    d.slpdu = (c.gamlg + 1.0) / 2.0;
+    int *t_ipq, *t_e1, *t_e2, *t_n2c, *t_n2co;
+    uint8_t *t_fl, *t_col;
+    const size_t nU = (size_t)4 * c.ndegr * nquad;
+    x->nblk_dt = std::min(nblocks(nquad), 1024);
+    if ((rc = x->alloc(&t_ipq, ipq.size())) || (rc = x->alloc(&t_e1, e1.size())) ||
+        (rc = x->alloc(&t_e2, e2.size())) || (rc = x->alloc(&t_n2c, n2c.size())) ||
+        (rc = x->alloc(&t_n2co, n2c_off.size())) || (rc = x->alloc(&t_fl, (size_t)npoin)) ||
+        (rc = x->alloc(&t_col, (size_t)npoin)) || (rc = x->alloc(&d.xn, (size_t)npoin)) ||
+        (rc = x->alloc(&d.yn, (size_t)npoin)) || (rc = x->alloc(&d.xs, (size_t)npoin)) ||
+        (rc = x->alloc(&d.ys, (size_t)npoin)) || (rc = x->alloc(&x->x0, (size_t)2 * npoin)) ||
+        (rc = x->alloc(&d.Un, nU)) || (rc = x->alloc(&d.Us, nU)) || (rc = x->alloc(&d.R, nU)) ||
+        (rc = x->alloc(&x->A, (size_t)x->nmatr * nquad)) || (rc = x->alloc(&d.G, (size_t)24 * nquad)) ||
+        (rc = x->alloc(&d.usx, (size_t)npoin)) || (rc = x->alloc(&d.usy, (size_t)npoin)) ||
+        (rc = x->alloc(&d.cavg, (size_t)5 * nquad)) || (rc = x->alloc(&d.frame, (size_t)4 * nquad)) ||
+        (rc = x->alloc(&d.nmm, (size_t)10 * npoin)) || (rc = x->alloc(&d.lim, (size_t)9 * nquad)) ||
+        (rc = x->alloc(&d.rec, (size_t)x->nrec * nquad)) || (rc = x->alloc(&d.red, (size_t)2 * x->nblk_dt)) ||
+        (rc = x->alloc(&d.scal, 4)))
+        return bad(rc);
+    d.ipq = t_ipq; d.esuv1 = t_e1; d.esuv2 = t_e2; d.n2c = t_n2c; d.n2c_off = t_n2co;
+    d.nflag = t_fl; d.ncol = t_col; d.A = x->A;
+    d.x0 = x->x0; d.y0 = x->x0 + npoin;
+    hipError_t e = hipSuccess;
+    e = hipMemcpy(t_ipq, ipq.data(), ipq.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t_e1, e1.data(), e1.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t_e2, e2.data(), e2.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t_n2c, n2c.data(), n2c.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t_n2co, n2c_off.data(), n2c_off.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t_fl, nflag.data(), npoin, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t_col, ncol.data(), npoin, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bad(fail(LAG2D_EDEVICE, std::string("mesh upload: ") + hipGetErrorString(e)));
+    double sc[4] = {c.dtfix, c.tacru, 0.0, 0.0};
+    if (hipMemcpy(d.scal, sc, sizeof(sc), hipMemcpyHostToDevice) != hipSuccess)
+        return bad(fail(LAG2D_EDEVICE, "scalar upload"));
+    x->ipq_host = std::move(ipq);
+    *out = x;
+    return 0;
+}
+
+extern "C" int lag2d_upload(lag2d_ctx *x, const double *coord, const double *coold, const double *geoel,
+                            const double *amatr, const double *unkno, const double *ustar)
+{
+    if (!x || !coord || !coold || !geoel || !amatr || !unkno || !ustar) return fail(LAG2D_EINVAL, "null argument");
+    const int nc = x->nc, np = x->np, nd = x->nd, nm = x->nmatr;
+    Dev &d = x->d;
+    std::vector<double> h((size_t)std::max({2 * np, 4 * nd * nc, 24 * nc, nm * nc}));
+    auto up = [&](double *dst, size_t n) { return hipMemcpy(dst, h.data(), n * sizeof(double), hipMemcpyHostToDevice); };
+    for (int p = 0; p < np; ++p) h[p] = coord[2 * (size_t)p];
+    HIPCHK(up(d.xn, np));
+    for (int p = 0; p < np; ++p) h[p] = coord[2 * (size_t)p + 1];
+    HIPCHK(up(d.yn, np));
+    for (int p = 0; p < np; ++p) {
+        h[p] = coold[2 * (size_t)p];
+        h[np + p] = coold[2 * (size_t)p + 1];
+    }
+    HIPCHK(up(x->x0, 2 * (size_t)np));
+    for (int p = 0; p < np; ++p) h[p] = ustar[2 * (size_t)p];
+    HIPCHK(up(d.usx, np));
+    for (int p = 0; p < np; ++p) h[p] = ustar[2 * (size_t)p + 1];
+    HIPCHK(up(d.usy, np));
+    for (int ie = 0; ie < nc; ++ie)
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < nd; ++id)
+                h[((size_t)q * nd + id) * nc + ie] = unkno[id + (size_t)nd * (q + 4 * (size_t)ie)];
+    HIPCHK(up(d.Un, (size_t)4 * nd * nc));
+    for (int ie = 0; ie < nc; ++ie)
+        for (int k = 0; k < 24; ++k) h[(size_t)k * nc + ie] = geoel[k + 24 * (size_t)ie];
+    HIPCHK(up(d.G, (size_t)24 * nc));
+    x->geo_host.assign(geoel, geoel + (size_t)24 * nc);
+    for (int ie = 0; ie < nc; ++ie)
+        for (int k = 0; k < nm; ++k) h[(size_t)k * nc + ie] = amatr[k + (size_t)nm * ie];
+    HIPCHK(up(x->A, (size_t)nm * nc));
+    hipLaunchKernelGGL(k_cellvol, dim3(nblocks(nc)), dim3(LAG2D_BLOCK), 0, x->stream, d, d.xn, d.yn);
+    HIPCHK(hipGetLastError());
+    int rc = upload_extra(*x);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+// ---- launch helpers with optional per-kernel event timing
+void lag2d_ctx::tic(int k)
+{
+    if (!timing) return;
+    auto &t = timers[k];
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, stream);
+    t.ev.push_back({a, b});
+}
+
+void lag2d_ctx::toc(int k)
+{
+    if (!timing) return;
+    hipEventRecord(timers[k].ev.back().second, stream);
+}
+
+#define LAUNCH(kid, kern, grid, ...)                                                   \
+    do {                                                                               \
+        x->tic(kid);                                                                   \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(LAG2D_BLOCK), 0, x->stream, __VA_ARGS__); \
+        x->toc(kid);                                                                   \
+    } while (0)
+
+const char *const lag2d_kernel_names[KID_COUNT] = {
+    "k_cellavg_q1", "k_nodeminmax", "k_limit_corner_q1", "k_node_q1", "k_cellrhs_q1", "k_dt_partial",
+    "k_dt_final",   "k_update",     "k_nodemove",        "k_center_q2", "k_cellvol",
+    "k_q2_prep",    "k_q2_weno",    "k_q2_corner",       "k_node_q2", "k_cellrhs_q2"};
+
+static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
+
+// RHS of one stage on state Uin and stage coordinates (ds.xs/ds.ys); leaves
+// the limited DOFs in Us.  apply_update fuses the M^-1 solve and RK blend.
+static int launch_rhs(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int apply_update)
+{
+    const int nc = x->nc, np = x->np;
+    if (x->cfg.ncurv == 0) {
+        LAUNCH(KID_CELLAVG_Q1, k_cellavg_q1, nblocks(nc), ds, Uin);
+        LAUNCH(KID_NODEMINMAX, k_nodeminmax, nblocks(np), ds);
+        LAUNCH(KID_LIMIT_Q1, k_limit_corner_q1, nblocks(nc), ds, Uin);
+        LAUNCH(KID_NODE_Q1, k_node_q1, nblocks(np), ds);
+        LAUNCH(KID_CELLRHS_Q1, k_cellrhs_q1, nblocks(nc), ds, Uin, alfa, apply_update);
+    } else {
+        int rc = launch_rhs_q2(x, ds, Uin, alfa, apply_update);
+        if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int step_impl(lag2d_ctx *x, int itime)
+{
+    Dev &d = x->d;
+    const int nc = x->nc, np = x->np;
+    const int ns = x->cfg.nstag;
+    for (int s = 0; s < ns; ++s) {
+        const double alfa = RKLAG[ns - 2][s];
+        Dev ds = d;
+        if (s == 0) {
+            ds.xs = d.xn;
+            ds.ys = d.yn;
+        }
+        const double *Uin = (s == 0) ? d.Un : d.Us;
+        int rc = launch_rhs(x, ds, Uin, alfa, s > 0);
+        if (rc) return rc;
+        if (s == 0) {
+            LAUNCH(KID_DT_PARTIAL, k_dt_partial, x->nblk_dt, ds, Uin, x->nblk_dt);
+            LAUNCH(KID_DT_FINAL, k_dt_final, 1, d, itime, x->nblk_dt);
+            LAUNCH(KID_UPDATE, k_update, nblocks(nc), d, alfa);
+        }
+        LAUNCH(KID_NODEMOVE, k_nodemove, nblocks(np), d, alfa, s == 0 ? d.xn : d.xs, s == 0 ? d.yn : d.ys);
+        if (x->cfg.ncurv == 1) LAUNCH(KID_CENTER_Q2, k_center_q2, nblocks(nc), d);
+        if (x->cfg.ncurv == 1 && s + 1 < ns) LAUNCH(KID_CELLVOL, k_cellvol, nblocks(nc), d, d.xs, d.ys);
+    }
+    LAUNCH(KID_CELLVOL, k_cellvol, nblocks(nc), d, d.xs, d.ys);
+    HIPCHK(hipGetLastError());
+    std::swap(d.Un, d.Us);
+    std::swap(d.xn, d.xs);
+    std::swap(d.yn, d.ys);
+    return 0;
+}
+
+extern "C" int lag2d_step(lag2d_ctx *x, int itime)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    if (itime < 1) return fail(LAG2D_EINVAL, "itime is 1-based");
+    return step_impl(x, itime);
+}
+
+extern "C" int lag2d_run(lag2d_ctx *x, int itime0, int nsteps, int *taken)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    int n = 0;
+    // With a CFL-controlled dt the end-time test reads tacru back after every
+    // step (the driver stops at the step that reaches tend); lag2d_step queues
+    // steps without any host synchronisation.
+    double sc[2];
+    HIPCHK(hipMemcpyAsync(sc, x->d.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    bool watch = x->cfg.ndt == 1;
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = step_impl(x, itime0 + k);
+        if (rc) return rc;
+        ++n;
+        if (watch) {
+            HIPCHK(hipMemcpyAsync(sc, x->d.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+            HIPCHK(hipStreamSynchronize(x->stream));
+            if (std::fabs(sc[1] - x->cfg.tend) / x->cfg.tend <= 1e-11) break;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (taken) *taken = n;
+    return 0;
+}
+
+extern "C" int lag2d_rhs(lag2d_ctx *x, double *rhsel)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    Dev &d = x->d;
+    Dev ds = d;
+    ds.xs = d.xn;
+    ds.ys = d.yn;
+    int rc = launch_rhs(x, ds, d.Un, 0.0, 0);
+    if (rc) return rc;
+    const int nc = x->nc, nd = x->nd;
+    const size_t nU = (size_t)4 * nd * nc;
+    HIPCHK(hipMemcpyAsync(d.Un, d.Us, nU * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (rhsel) {
+        std::vector<double> h(nU);
+        HIPCHK(hipMemcpy(h.data(), d.R, nU * sizeof(double), hipMemcpyDeviceToHost));
+        for (int ie = 0; ie < nc; ++ie)
+            for (int q = 0; q < 4; ++q)
+                for (int id = 0; id < nd; ++id)
+                    rhsel[id + (size_t)nd * (q + 4 * (size_t)ie)] = h[((size_t)q * nd + id) * nc + ie];
+    }
+    return 0;
+}
+
+extern "C" int lag2d_download(lag2d_ctx *x, double *coord, double *geoel, double *unkno, double *ustar)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    const int nc = x->nc, np = x->np, nd = x->nd;
+    Dev &d = x->d;
+    std::vector<double> h((size_t)std::max({2 * np, 4 * nd * nc, 24 * nc}));
+    if (coord) {
+        HIPCHK(hipMemcpy(h.data(), d.xn, np * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h.data() + np, d.yn, np * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; ++p) {
+            coord[2 * (size_t)p] = h[p];
+            coord[2 * (size_t)p + 1] = h[np + p];
+        }
+    }
+    if (ustar) {
+        HIPCHK(hipMemcpy(h.data(), d.usx, np * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h.data() + np, d.usy, np * sizeof(double), hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; ++p) {
+            ustar[2 * (size_t)p] = h[p];
+            ustar[2 * (size_t)p + 1] = h[np + p];
+        }
+    }
+    if (unkno) {
+        HIPCHK(hipMemcpy(h.data(), d.Un, (size_t)4 * nd * nc * sizeof(double), hipMemcpyDeviceToHost));
+        for (int ie = 0; ie < nc; ++ie)
+            for (int q = 0; q < 4; ++q)
+                for (int id = 0; id < nd; ++id)
+                    unkno[id + (size_t)nd * (q + 4 * (size_t)ie)] = h[((size_t)q * nd + id) * nc + ie];
+    }
+    if (geoel) {
+        HIPCHK(hipMemcpy(h.data(), d.G, (size_t)24 * nc * sizeof(double), hipMemcpyDeviceToHost));
+        for (int ie = 0; ie < nc; ++ie)
+            for (int k = 0; k < 24; ++k) {
+                bool live = (k == 2 || k == 4 || k == 5);
+                geoel[k + 24 * (size_t)ie] = live ? h[(size_t)k * nc + ie] : x->geo_host[k + 24 * (size_t)ie];
+            }
+    }
+    return 0;
+}
+
+extern "C" int lag2d_get_time(lag2d_ctx *x, double *dtfix, double *tacru)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    double sc[2];
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(sc, x->d.scal, sizeof(sc), hipMemcpyDeviceToHost));
+    if (dtfix) *dtfix = sc[0];
+    if (tacru) *tacru = sc[1];
+    return 0;
+}
+
+extern "C" int lag2d_check(lag2d_ctx *x)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    const int nc = x->nc, nd = x->nd;
+    std::vector<double> u((size_t)4 * nd * nc), g((size_t)3 * nc);
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(u.data(), x->d.Un, u.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(g.data(), x->d.G + 2 * (size_t)nc, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost));
+    for (double v : u)
+        if (!std::isfinite(v)) return fail(LAG2D_ENONFINITE, "non-finite DOF in the state");
+    for (int ie = 0; ie < nc; ++ie)
+        if (!(g[ie] > 0.0)) return fail(LAG2D_EVOLUME, "non-positive cell volume at cell " + std::to_string(ie + 1));
+    return 0;
+}
+
+extern "C" int lag2d_sync(lag2d_ctx *x)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+extern "C" int lag2d_kernel_time(lag2d_ctx *x, int enable, double *avg_ms, const char **name)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    if (enable >= 0) {
+        HIPCHK(hipStreamSynchronize(x->stream));
+        for (auto &t : x->timers) {
+            for (auto &ev : t.ev) {
+                hipEventDestroy(ev.first);
+                hipEventDestroy(ev.second);
+            }
+            t.ev.clear();
+        }
+        x->timing = enable != 0;
+        return 0;
+    }
+    // enable < 0: report
+    HIPCHK(hipStreamSynchronize(x->stream));
+    int best = -1;
+    double best_tot = 0.0, best_avg = 0.0;
+    for (int k = 0; k < KID_COUNT; ++k) {
+        auto &t = x->timers[k];
+        if (t.ev.empty()) continue;
+        double tot = 0.0;
+        for (auto &ev : t.ev) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev.first, ev.second);
+            tot += ms;
+        }
+        x->kernel_total_ms[k] = tot;
+        x->kernel_calls[k] = (int)t.ev.size();
+        if (tot > best_tot) {
+            best_tot = tot;
+            best_avg = tot / t.ev.size();
+            best = k;
+        }
+    }
+    if (best < 0) return fail(LAG2D_EINVAL, "no timed launches");
+    if (avg_ms) *avg_ms = best_avg;
+    if (name) *name = lag2d_kernel_names[best];
+    return 0;
+}
+
+extern "C" int lag2d_kernel_stats(lag2d_ctx *x, int k, double *total_ms, int *calls, const char **name)
+{
+    if (!x || k < 0 || k >= KID_COUNT) return fail(LAG2D_EINVAL, "bad argument");
+    if (total_ms) *total_ms = x->kernel_total_ms[k];
+    if (calls) *calls = x->kernel_calls[k];
+    if (name) *name = lag2d_kernel_names[k];
+    return 0;
+}
+
+extern "C" void lag2d_destroy(lag2d_ctx *x) { delete x; }

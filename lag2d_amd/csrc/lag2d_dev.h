@@ -1,0 +1,76 @@
+// Device-side layout and helpers shared by the lag2d kernels (gfx950, FP64).
+//
+// HBM layout (structure of arrays, one entry per cell or node so that a
+// wavefront of 64 consecutive cells/nodes reads 64 consecutive doubles):
+//   U[(q*nd + id)*nc + ie]   DOF id of variable q (tau, u, v, E) of cell ie
+//   A[k*nc + ie]             packed inverse mass matrix (amatr row k)
+//   G[k*nc + ie]             geometry rows of geoel (row k, 0-based)
+//   X[ip], Y[ip]             node coordinates
+//   IPQ[iv*nc + ie]          quad nodes (0-based)
+// Arithmetic is FP64 throughout and is built without FMA contraction so the
+// results round exactly where the reference's do.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LAG2D_BLOCK 256
+
+namespace lag2d {
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+// Fortran default-real literal 1.e-6 widened to REAL(8), as the limiter
+// routines write it (Limiterlag.f90:53, :8239; Getrhslagmctria.f90:2408).
+static constexpr double EPS_SP6 = (double)1.e-6f;
+
+__device__ __forceinline__ double barthfct(double umax, double umin, double uctr, double dunk)
+{
+    // Getrhslagmc.f90:9171-9219 (coef 0.7 on both branches of its time test)
+    const double coef = 0.7;
+    if (dunk > 1.e-8) {
+        double fiy = coef * (umax - uctr) / dunk;
+        return dmin((fiy * fiy + 2.0 * fiy) / (fiy * fiy + fiy + 2.0), 1.0);
+    } else if (dunk < -1.e-8) {
+        double fiy = coef * (umin - uctr) / dunk;
+        return dmin((fiy * fiy + 2.0 * fiy) / (fiy * fiy + fiy + 2.0), 1.0);
+    }
+    return 1.0;
+}
+
+// Reference-quad vertex coordinates (corner order 1..4, CCW)
+__constant__ static const double XVQ[4] = {-1.0, 1.0, 1.0, -1.0};
+__constant__ static const double YVQ[4] = {-1.0, -1.0, 1.0, 1.0};
+
+struct Dev {
+    int nc, np, nb, nd, nvq, nmatr;
+    int nstag, ncase, ncurv, npoly, ndt;
+    double gamlg, cfl, cimpd, cdrho, tend, slpdu;
+    // mesh
+    const int *ipq;          // [nvq][nc]
+    const int *esuv1, *esuv2;
+    const int *n2c;          // node -> corner CSR entries (ie*NCORN + corner), offsets n2c_off
+    const int *n2c_off;
+    const uint8_t *nflag;    // bit0: v=0 wall, bit1: u=0 wall, bit2: fixed wall (no solve)
+    const uint8_t *ncol;     // symmetry colouring for the velocity limiter (0,2,3,10)
+    // state
+    double *xn, *yn;         // x^n
+    double *xs, *ys;         // stage coordinates
+    const double *x0, *y0;   // initial coordinates
+    double *Un, *Us;         // U^n and stage state
+    const double *A;         // inverse mass matrix
+    double *G;               // geoel rows [24][nc]
+    double *usx, *usy;       // nodal velocity
+    double *R;               // right-hand side [(q*nd+id)*nc + ie]
+    double *cavg;            // [5][nc]  tau, u, v, p, E cell averages
+    double *frame;           // [4][nc]  principal-strain frame (row-major 2x2)
+    double *nmm;             // [10][np] node max (0..4) and min (5..9)
+    double *lim;             // [9][nc]  aflim(1..5), afvec (col-major 2x2)
+    double *rec;             // corner-face records
+    double *red;             // reduction scratch
+    double *scal;            // device scalars: 0 dtfix, 1 tacru
+    // sub-cell (Q2) data
+    double *msub;            // [4][nc] initial sub-cell masses
+};
+
+}  // namespace lag2d

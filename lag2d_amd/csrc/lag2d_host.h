@@ -1,0 +1,53 @@
+// Host-side context of the lag2d library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+#include <vector>
+
+#include "../../include/lag2d.h"
+#include "lag2d_dev.h"
+
+enum {
+    KID_CELLAVG_Q1 = 0, KID_NODEMINMAX, KID_LIMIT_Q1, KID_NODE_Q1, KID_CELLRHS_Q1, KID_DT_PARTIAL,
+    KID_DT_FINAL, KID_UPDATE, KID_NODEMOVE, KID_CENTER_Q2, KID_CELLVOL,
+    KID_Q2_PREP, KID_Q2_WENO, KID_Q2_CORNER, KID_NODE_Q2, KID_CELLRHS_Q2, KID_COUNT
+};
+extern const char *const lag2d_kernel_names[KID_COUNT];
+
+struct lag2d_timer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+struct lag2d_ctx {
+    lag2d_config cfg{};
+    int nc = 0, np = 0, nb = 0, nd = 0, nvq = 0, nmatr = 0;
+    int nrec = 40;          // corner-face record doubles per cell
+    int nblk_dt = 1;
+    hipStream_t stream = nullptr;
+    lag2d::Dev d{};
+    double *A = nullptr;    // inverse mass matrix (SoA)
+    double *x0 = nullptr;   // initial coordinates [2][np]
+    std::vector<void *> allocs;
+    std::vector<int> ipq_host;
+    std::vector<double> geo_host;
+    bool timing = false;
+    lag2d_timer timers[KID_COUNT];
+    double kernel_total_ms[KID_COUNT] = {};
+    int kernel_calls[KID_COUNT] = {};
+
+    ~lag2d_ctx();
+    template <class T>
+    int alloc(T **p, size_t n);
+    void tic(int k);
+    void toc(int k);
+};
+
+// Node -> corner CSR: for every node the (cell, corner) pairs in the order
+// the reference accumulates their contributions.
+int build_node_corners(lag2d_ctx &x, const std::vector<int> &ipq, const std::vector<int> &esuv1,
+                       const std::vector<int> &esuv2, std::vector<int> &n2c, std::vector<int> &n2c_off);
+// Extra per-path data computed once the state is uploaded (e.g. the
+// time-invariant initial sub-cell masses of the curved path).
+int upload_extra(lag2d_ctx &x);
+int launch_rhs_q2(lag2d_ctx *x, lag2d::Dev &ds, const double *Uin, double alfa, int apply_update);

@@ -1,0 +1,22 @@
+// Kernel declarations shared between the kernel sources and the host driver.
+#pragma once
+#include "lag2d_dev.h"
+
+namespace lag2d {
+
+// straight-quad P1 path
+__global__ void k_cellavg_q1(Dev d, const double *__restrict__ Uin);
+__global__ void k_nodeminmax(Dev d);
+__global__ void k_limit_corner_q1(Dev d, const double *__restrict__ Uin);
+__global__ void k_node_q1(Dev d);
+__global__ void k_cellrhs_q1(Dev d, const double *__restrict__ Uin, double alfa, int apply_update);
+
+// common to both paths
+__global__ void k_dt_partial(Dev d, const double *__restrict__ Uin, int nblk);
+__global__ void k_dt_final(Dev d, int itime, int nblk);
+__global__ void k_update(Dev d, double alfa);
+__global__ void k_nodemove(Dev d, double alfa, const double *__restrict__ xsrc, const double *__restrict__ ysrc);
+__global__ void k_center_q2(Dev d);
+__global__ void k_cellvol(Dev d, const double *__restrict__ x, const double *__restrict__ y);
+
+}  // namespace lag2d

@@ -1,0 +1,34 @@
+// Mesh-derived tables built on the host at context creation.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "lag2d_host.h"
+
+// Straight quads: each node appears at exactly one corner of each of its
+// cells, and the reference scatters corner contributions cell by cell in
+// increasing cell id (Limiterlag.f90:1011-1016), which is the order of the
+// elements-around-points lists.  Entry = cell*4 + local vertex.
+int build_node_corners(lag2d_ctx &x, const std::vector<int> &ipq, const std::vector<int> &esuv1,
+                       const std::vector<int> &esuv2, std::vector<int> &n2c, std::vector<int> &n2c_off)
+{
+    const int nc = x.nc, np = x.np, nv = x.nvq;
+    n2c_off.assign(np + 1, 0);
+    n2c.clear();
+    for (int p = 0; p < np; ++p) {
+        int prev = -1;
+        for (int k = esuv2[p]; k < esuv2[p + 1]; ++k) {
+            int ie = esuv1[k];
+            if (ie < prev) return LAG2D_EINVAL;  // element lists must be ascending
+            prev = ie;
+            int iv = -1;
+            for (int j = 0; j < (nv == 4 ? 4 : 9); ++j)
+                if (ipq[(size_t)j * nc + ie] == p) iv = j;
+            if (iv < 0) return LAG2D_EINVAL;
+            if (nv == 4) n2c.push_back(ie * 4 + iv);
+            else n2c.push_back(ie * 16 + iv);
+        }
+        n2c_off[p + 1] = (int)n2c.size();
+    }
+    return 0;
+}

@@ -1,0 +1,473 @@
+// Straight-quad P1 path (ncurv=0, nfint=1, nlimi=6): Barth limiter with the
+// symmetry-preserving velocity limiter, nodal Riemann solver, lumped face
+// integral and 5x5 Gauss domain integral.  One thread per cell or per node.
+//
+// Reference routines restated here, in order of a stage
+// (Getrhslagmc.f90:6810-6939, :7283-7296):
+//   getfnds_lag_mc_hybrid        Getrhslagmc.f90:2117-2262
+//   barthlimit_lag_vtxunknew     Getrhslagmctria.f90:2373-2640
+//   barthlimit_lagsym_quad       Limiterlag.f90:4-404
+//   barthlimit_sympre_quadlb     Limiterlag.f90:8127-8692
+//   getndvelo_lag_mc_matrixsym   Limiterlag.f90:408-704
+//   getriem_quad_matrixsym       Limiterlag.f90:708-1033
+//   getriecoef_matrixnew         Limiterlag.f90:1037-1076
+//   rhsifacedg_lag_mc_hybridquad2 Getrhslagmc.f90:14468-14618
+//   rhsdomndg_lag_mc_quadsym     Limiterlag.f90:1108-1490
+#include "lag2d_dev.h"
+#include "lag2d_kernels.h"
+
+namespace lag2d {
+
+__device__ __forceinline__ double xvq(int iv) { return (iv == 1 || iv == 2) ? 1.0 : -1.0; }
+__device__ __forceinline__ double yvq(int iv) { return (iv >= 2) ? 1.0 : -1.0; }
+
+#define UI(q, id) Uin[((q) * nd + (id)) * nc + ie]
+
+// Cell averages (tau, u, v, p, E) and the principal-strain frame of
+// barthlimit_sympre_quadlb Part 1 (Limiterlag.f90:8262-8387).
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_cellavg_q1(Dev d, const double *__restrict__ Uin)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc, nd = d.nd;
+    if (ie >= nc) return;
+    const double g1 = d.gamlg - 1.0;
+    double tau = UI(0, 0), u = UI(1, 0), v = UI(2, 0), e = UI(3, 0);
+    double rhoct = 1.0 / tau;
+    double pctr = dmax(EPS_SP6, g1 * rhoct * (e - 0.5 * (u * u + v * v)));
+    d.cavg[0 * nc + ie] = tau;
+    d.cavg[1 * nc + ie] = u;
+    d.cavg[2 * nc + ie] = v;
+    d.cavg[3 * nc + ie] = pctr;
+    d.cavg[4 * nc + ie] = e;
+
+    double xp[4], yp[4];
+    for (int k = 0; k < 4; ++k) {
+        int p = d.ipq[k * nc + ie];
+        xp[k] = d.xs[p];
+        yp[k] = d.ys[p];
+    }
+    double r = d.G[0 * nc + ie], s = d.G[1 * nc + ie];
+    double dudr = UI(1, 1) / 1.0, duds = UI(1, 2) / 1.0;
+    double dvdr = UI(2, 1) / 1.0, dvds = UI(2, 2) / 1.0;
+    double rp = 1.0 + r, rm = 1.0 - r, sp = 1.0 + s, sm = 1.0 - s;
+    double dsr[4] = {-0.25 * sm, 0.25 * sm, 0.25 * sp, -0.25 * sp};
+    double dss[4] = {-0.25 * rm, -0.25 * rp, 0.25 * rp, 0.25 * rm};
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        dxdr = dxdr + dsr[k] * xp[k];
+        dxds = dxds + dss[k] * xp[k];
+        dydr = dydr + dsr[k] * yp[k];
+        dyds = dyds + dss[k] * yp[k];
+    }
+    double jacom = dxdr * dyds - dydr * dxds;
+    double drdx = dyds / jacom, drdy = -dxds / jacom;
+    double dsdx = -dydr / jacom, dsdy = dxdr / jacom;
+    double dudx = dudr * drdx + duds * dsdx;
+    double dudy = dudr * drdy + duds * dsdy;
+    double dvdx = dvdr * drdx + dvds * dsdx;
+    double dvdy = dvdr * drdy + dvds * dsdy;
+    double ma = dudx, mb = 0.5 * (dudy + dvdx), mc = mb, md = dvdy;
+    double mapt = ma + md, mapd = ma * md - mb * mc;
+    double l1 = 0.5 * mapt + sqrt(0.25 * (mapt * mapt) - mapd);
+    double l2 = 0.5 * mapt - sqrt(0.25 * (mapt * mapt) - mapd);
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    if (fabs(mc) > 1.e-8) {
+        double n1 = sqrt((l1 - md) * (l1 - md) + mc * mc);
+        double n2 = sqrt((l2 - md) * (l2 - md) + mc * mc);
+        if (mc > 0.0) {
+            m0 = (l1 - md) / n1; m1 = mc / n1; m2 = (l2 - md) / n2; m3 = mc / n2;
+        } else {
+            m2 = (l1 - md) / n1; m3 = mc / n1; m0 = (l2 - md) / n2; m1 = mc / n2;
+        }
+    }
+    d.frame[0 * nc + ie] = m0;
+    d.frame[1 * nc + ie] = m1;
+    d.frame[2 * nc + ie] = m2;
+    d.frame[3 * nc + ie] = m3;
+}
+
+// Node max/min of the cell averages over the surrounding cells
+// (barthlimit_lag_vtxunknew, Getrhslagmctria.f90:2522-2636), as a CSR gather.
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_nodeminmax(Dev d)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.np) return;
+    const int nc = d.nc;
+    double mx[5], mn[5];
+    for (int q = 0; q < 5; ++q) { mx[q] = -1.e10; mn[q] = 1.e10; }
+    for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
+        int je = d.esuv1[k];
+        for (int q = 0; q < 5; ++q) {
+            double v = d.cavg[q * nc + je];
+            if (v > mx[q]) mx[q] = v;
+            if (v < mn[q]) mn[q] = v;
+        }
+    }
+    for (int q = 0; q < 5; ++q) {
+        d.nmm[q * d.np + p] = mx[q];
+        d.nmm[(5 + q) * d.np + p] = mn[q];
+    }
+}
+
+// Limiter coefficients (aflim, afvec) and the corner-face Riemann records of
+// one quad.  Records per corner face cf = f + 2*iv (face f=0 is the edge
+// entering vertex iv, f=1 the edge leaving it): m = mu*A (the isotropic
+// corner matrix), m*u_c (2), A*sigma*n (2).
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_limit_corner_q1(Dev d, const double *__restrict__ Uin)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc, nd = d.nd, np = d.np;
+    if (ie >= nc) return;
+    const double g1 = d.gamlg - 1.0;
+    int ip[4];
+    double xp[4], yp[4];
+    for (int k = 0; k < 4; ++k) {
+        ip[k] = d.ipq[k * nc + ie];
+        xp[k] = d.xs[ip[k]];
+        yp[k] = d.ys[ip[k]];
+    }
+    const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
+    double bq[4][3];
+    for (int iv = 0; iv < 4; ++iv) {
+        bq[iv][0] = 1.0;
+        bq[iv][1] = (xvq(iv) - rc) / 1.0;
+        bq[iv][2] = (yvq(iv) - sc) / 1.0;
+    }
+    double un[4][4];  // [iv][q] vertex values of the unlimited P1 polynomial
+    for (int iv = 0; iv < 4; ++iv) {
+        for (int q = 0; q < 4; ++q) un[iv][q] = 0.0;
+        for (int id = 0; id < 3; ++id)
+            for (int q = 0; q < 4; ++q) un[iv][q] = un[iv][q] + UI(q, id) * bq[iv][id];
+    }
+    const double tau = UI(0, 0), uctr = UI(1, 0), vctr = UI(2, 0), ectr = UI(3, 0);
+    const double rhoct = 1.0 / tau;
+    const double pctr_sp = dmax(EPS_SP6, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+
+    // Part 2 of barthlimit_lagsym_quad: tau, u, v, p coefficients
+    double al[5];
+    {
+        double unctr[4] = {1.0 / rhoct, uctr, vctr, pctr_sp};
+        double a[4] = {1.e300, 1.e300, 1.e300, 1.e300};
+        for (int iv = 0; iv < 4; ++iv) {
+            double rhov = 1.0 / un[iv][0];
+            double uv = un[iv][1], vv = un[iv][2], ev = un[iv][3];
+            double pv = dmax(EPS_SP6, g1 * rhov * (ev - 0.5 * (uv * uv + vv * vv)));
+            double vals[4] = {1.0 / rhov, uv, vv, pv};
+            int p = ip[iv];
+            for (int q = 0; q < 4; ++q) {
+                double f = barthfct(d.nmm[q * np + p], d.nmm[(5 + q) * np + p], unctr[q], vals[q] - unctr[q]);
+                a[q] = dmin(a[q], f);
+            }
+        }
+        for (int q = 0; q < 4; ++q) al[q] = a[q];
+    }
+    // barthlimit_sympre_quadlb Part 4 and 6: velocity limiter in the frame
+    double af[4];
+    {
+        const double m0 = d.frame[0 * nc + ie], m1 = d.frame[1 * nc + ie];
+        const double m2 = d.frame[2 * nc + ie], m3 = d.frame[3 * nc + ie];
+        double uc2 = uctr * m0 + vctr * m1, uc3 = uctr * m2 + vctr * m3;
+        double fixi = 1.e300, fiet = 1.e300;
+        for (int iv = 0; iv < 4; ++iv) {
+            double un2 = un[iv][1] * m0 + un[iv][2] * m1;
+            double un3 = un[iv][1] * m2 + un[iv][2] * m3;
+            double mx2 = uc2, mx3 = uc3, mn2 = uc2, mn3 = uc3;
+            int p = ip[iv];
+            int col = d.ncase != 1 ? d.ncol[p] : 0;
+            for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
+                int je = d.esuv1[k];
+                double ucj = d.cavg[1 * nc + je], vcj = d.cavg[2 * nc + je];
+#define ACC(UU, VV)                                            \
+    {                                                          \
+        double ul = (UU) * m0 + (VV) * m1;                     \
+        double vl = (UU) * m2 + (VV) * m3;                     \
+        mx2 = dmax(mx2, ul); mx3 = dmax(mx3, vl);              \
+        mn2 = dmin(mn2, ul); mn3 = dmin(mn3, vl);              \
+    }
+                ACC(ucj, vcj);
+                if (col == 3) {
+                    ACC(ucj, -vcj);
+                } else if (col == 2) {
+                    ACC(-ucj, vcj);
+                } else if (col == 10) {
+                    ACC(ucj, -vcj);
+                    ACC(-ucj, vcj);
+                    ACC(-ucj, -vcj);
+                }
+#undef ACC
+            }
+            fixi = dmin(fixi, barthfct(mx2, mn2, uc2, un2 - uc2));
+            fiet = dmin(fiet, barthfct(mx3, mn3, uc3, un3 - uc3));
+        }
+        double fi11 = m0 * m0 * fixi + m2 * m2 * fiet;
+        double fi12 = m0 * m1 * fixi + m2 * m3 * fiet;
+        double fi22 = m1 * m1 * fixi + m3 * m3 * fiet;
+        af[0] = fi11; af[1] = fi12; af[2] = fi12; af[3] = fi22;  // col-major (1,1),(2,1),(1,2),(2,2)
+    }
+    // Part 3 of barthlimit_lagsym_quad: total-energy coefficient
+    {
+        double a5 = 1.e300;
+        for (int iv = 0; iv < 4; ++iv) {
+            double rhov = 1.0 / un[iv][0];
+            double uv = un[iv][1], vv = un[iv][2];
+            double pv = dmax(EPS_SP6, g1 * rhov * (un[iv][3] - 0.5 * (uv * uv + vv * vv)));
+            double t = tau + al[0] * (1.0 / rhov - tau);
+            double pl = pctr_sp + al[3] * (pv - pctr_sp);
+            double e5 = pl / g1 * t + 0.5 * (uv * uv + vv * vv);
+            int p = ip[iv];
+            a5 = dmin(a5, barthfct(d.nmm[4 * np + p], d.nmm[9 * np + p], ectr, e5 - ectr));
+        }
+        al[4] = a5;
+    }
+    for (int q = 0; q < 5; ++q) d.lim[q * nc + ie] = al[q];
+    for (int k = 0; k < 4; ++k) d.lim[(5 + k) * nc + ie] = af[k];
+
+    // Corner Riemann coefficients (getriem_quad_matrixsym, ndens=1, nlimi=6)
+    const double eps = 1.0e-6;
+    double gnx[4], gny[4], glen[4];
+    for (int e = 0; e < 4; ++e) {
+        int a = e, b = (e + 1) & 3;
+        double anx = yp[b] - yp[a];
+        double any = -(xp[b] - xp[a]);
+        gnx[e] = anx / sqrt(anx * anx + any * any);
+        gny[e] = any / sqrt(anx * anx + any * any);
+        glen[e] = sqrt(anx * anx + any * any);
+    }
+    const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+    const double dudr = af[0] * UI(1, 1) + af[2] * UI(2, 1);
+    const double duds = af[0] * UI(1, 2) + af[2] * UI(2, 2);
+    const double dvdr = af[1] * UI(1, 1) + af[3] * UI(2, 1);
+    const double dvds = af[1] * UI(1, 2) + af[3] * UI(2, 2);
+    const double sdctr = sqrt(dmax(eps, d.gamlg * pctr / rhoct));
+    for (int iv = 0; iv < 4; ++iv) {
+        double rhovt = 1.0 / un[iv][0];
+        double pv = dmax(eps, g1 * rhovt * (un[iv][3] - 0.5 * (un[iv][1] * un[iv][1] + un[iv][2] * un[iv][2])));
+        double uvx = uctr + dudr * bq[iv][1] + duds * bq[iv][2];
+        double vvx = vctr + dvdr * bq[iv][1] + dvds * bq[iv][2];
+        double pvx = pctr + al[3] * (pv - pctr);
+        int p = ip[iv];
+        double dux = d.usx[p] - uvx, duy = d.usy[p] - vvx;
+        double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+        for (int f = 0; f < 2; ++f) {
+            int e = f == 0 ? ((iv + 3) & 3) : iv;
+            double nx = gnx[e], ny = gny[e];
+            // corner area of the face, planar rcoef = 1: (2*1 + 1)/3*0.5*L
+            double area = (2.0 * 1.0 + 1.0) / 3.0 * 0.5 * glen[e];
+            double deltu = fabs(dux * nx + duy * ny);
+            double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
+            double m11;
+            if (a3 < 1.e12) {
+                m11 = mur * area * fabs(1.0);
+            } else {
+                double nrm = sqrt(dux * dux + duy * duy);
+                m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
+            }
+            double mu1 = m11 * uvx + 0.0 * vvx;
+            double mu2 = 0.0 * uvx + m11 * vvx;
+            double sn1 = -pvx * area * nx + 0.0 * area * ny;
+            double sn2 = 0.0 * area * nx + -pvx * area * ny;
+            int cf = f + 2 * iv;
+            d.rec[(0 * 8 + cf) * nc + ie] = m11;
+            d.rec[(1 * 8 + cf) * nc + ie] = mu1;
+            d.rec[(2 * 8 + cf) * nc + ie] = mu2;
+            d.rec[(3 * 8 + cf) * nc + ie] = sn1;
+            d.rec[(4 * 8 + cf) * nc + ie] = sn2;
+        }
+    }
+}
+
+// Nodal solve: gather the corner-face records of the cells around each node
+// in the reference's accumulation order (cell id ascending, then face), the
+// 2x2 solve u* = M^-1 (sum M u_c - sum A sigma n) and the wall projection
+// (Limiterlag.f90:526-651).
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q1(Dev d)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.np) return;
+    const int nc = d.nc;
+    double a11 = 0.0, a21 = 0.0, a12 = 0.0, a22 = 0.0, mu1 = 0.0, mu2 = 0.0, sn1 = 0.0, sn2 = 0.0;
+    for (int k = d.n2c_off[p]; k < d.n2c_off[p + 1]; ++k) {
+        int e = d.n2c[k];
+        int ie = e >> 2, iv = e & 3;
+        for (int f = 0; f < 2; ++f) {
+            int cf = f + 2 * iv;
+            double m = d.rec[(0 * 8 + cf) * nc + ie];
+            a11 = a11 + m;
+            a21 = a21 + 0.0;
+            a12 = a12 + 0.0;
+            a22 = a22 + m;
+            mu1 = mu1 + d.rec[(1 * 8 + cf) * nc + ie];
+            mu2 = mu2 + d.rec[(2 * 8 + cf) * nc + ie];
+            sn1 = sn1 + d.rec[(3 * 8 + cf) * nc + ie];
+            sn2 = sn2 + d.rec[(4 * 8 + cf) * nc + ie];
+        }
+    }
+    const uint8_t fl = d.nflag[p];
+    double ux = d.usx[p], uy = d.usy[p];
+    if (!(fl & 4)) {
+        double det = a11 * a22 - a21 * a12;
+        double i11 = a22 / det, i12 = -a12 / det, i21 = -a21 / det, i22 = a11 / det;
+        double r1 = mu1 - sn1, r2 = mu2 - sn2;
+        ux = i11 * r1 + i12 * r2;
+        uy = i21 * r1 + i22 * r2;
+    }
+    if (fl & 1) uy = 0.0;
+    if (fl & 2) ux = 0.0;
+    d.usx[p] = ux;
+    d.usy[p] = uy;
+}
+
+// Face integral (lumped corner weights), domain integral and the in-place
+// limiting of the stage DOFs of one quad.  With apply_update the mass-matrix
+// solve and the SSP-RK blend follow at once (stages after the first, where dt
+// is already known); otherwise R is stored for the dt reduction.
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_cellrhs_q1(Dev d, const double *__restrict__ Uin, double alfa,
+                                                            int apply_update)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc, nd = d.nd;
+    if (ie >= nc) return;
+    const double c13 = 0.3333333333333333, c16 = 0.1666666666666666;
+    const double g1 = d.gamlg - 1.0;
+    int ip[4];
+    double xp[4], yp[4], us[4], vs[4];
+    for (int k = 0; k < 4; ++k) {
+        ip[k] = d.ipq[k * nc + ie];
+        xp[k] = d.xs[ip[k]];
+        yp[k] = d.ys[ip[k]];
+        us[k] = d.usx[ip[k]];
+        vs[k] = d.usy[ip[k]];
+    }
+    const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
+    double gnx[4], gny[4], glen[4];
+    for (int e = 0; e < 4; ++e) {
+        int a = e, b = (e + 1) & 3;
+        double anx = yp[b] - yp[a];
+        double any = -(xp[b] - xp[a]);
+        gnx[e] = anx / sqrt(anx * anx + any * any);
+        gny[e] = any / sqrt(anx * anx + any * any);
+        glen[e] = sqrt(anx * anx + any * any);
+    }
+    double r[4][3];  // [q][id]
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 3; ++id) r[q][id] = 0.0;
+    for (int iv = 0; iv < 4; ++iv) {
+        double b[3] = {1.0, (xvq(iv) - rc) / 1.0, (yvq(iv) - sc) / 1.0};
+        double f[2][2];  // [face][dim]
+        for (int fa = 0; fa < 2; ++fa) {
+            int cf = fa + 2 * iv;
+            double m11 = d.rec[(0 * 8 + cf) * nc + ie];
+            double mu1 = d.rec[(1 * 8 + cf) * nc + ie], mu2 = d.rec[(2 * 8 + cf) * nc + ie];
+            double sn1 = d.rec[(3 * 8 + cf) * nc + ie], sn2 = d.rec[(4 * 8 + cf) * nc + ie];
+            f[fa][0] = sn1 + m11 * us[iv] + 0.0 * vs[iv] - mu1;
+            f[fa][1] = sn2 + 0.0 * us[iv] + m11 * vs[iv] - mu2;
+        }
+        int e0 = (iv + 3) & 3, e1 = iv;
+        for (int id = 0; id < 3; ++id) {
+            double w = 2.0 * b[id] + b[id];
+            double l11 = c16 * w * gnx[e0] * glen[e0], l21 = c16 * w * gny[e0] * glen[e0];
+            double l12 = c16 * w * gnx[e1] * glen[e1], l22 = c16 * w * gny[e1] * glen[e1];
+            r[0][id] = r[0][id] + us[iv] * l11 + vs[iv] * l21 + us[iv] * l12 + vs[iv] * l22;
+            r[1][id] = r[1][id] + f[0][0] * c13 * w + f[1][0] * c13 * w;
+            r[2][id] = r[2][id] + f[0][1] * c13 * w + f[1][1] * c13 * w;
+            r[3][id] = r[3][id] + us[iv] * f[0][0] * c13 * w + vs[iv] * f[0][1] * c13 * w +
+                       us[iv] * f[1][0] * c13 * w + vs[iv] * f[1][1] * c13 * w;
+        }
+    }
+    // domain integral
+    double al0 = d.lim[0 * nc + ie], al3 = d.lim[3 * nc + ie], al4 = d.lim[4 * nc + ie];
+    double af0 = d.lim[5 * nc + ie], af1 = d.lim[6 * nc + ie], af2 = d.lim[7 * nc + ie], af3 = d.lim[8 * nc + ie];
+    double u0[4][3];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 3; ++id) u0[q][id] = UI(q, id);
+    const double eps = 1.0e-6;
+    const double rhoct = 1.0 / u0[0][0];
+    const double uctr = u0[1][0], vctr = u0[2][0], ectr = u0[3][0];
+    const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+    const double dudr = af0 * u0[1][1] + af2 * u0[2][1];
+    const double duds = af0 * u0[1][2] + af2 * u0[2][2];
+    const double dvdr = af1 * u0[1][1] + af3 * u0[2][1];
+    const double dvds = af1 * u0[1][2] + af3 * u0[2][2];
+    const double pg[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
+                          0.5384693101056830910363144, 0.9061798459386639927976269};
+    const double wg[5] = {0.2369268850561890875142640, 0.4786286704993664680412915, 0.5688888888888888888888889,
+                          0.4786286704993664680412915, 0.2369268850561890875142640};
+    for (int gi = 0; gi < 5; ++gi) {
+        for (int gj = 0; gj < 5; ++gj) {
+            double rr = pg[gi], ss = pg[gj], wi = wg[gi] * wg[gj];
+            double rp = 1.0 + rr, rm = 1.0 - rr, sp = 1.0 + ss, sm = 1.0 - ss;
+            double dsr[4] = {-0.25 * sm, 0.25 * sm, 0.25 * sp, -0.25 * sp};
+            double dss[4] = {-0.25 * rm, -0.25 * rp, 0.25 * rp, 0.25 * rm};
+            double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+            for (int k = 0; k < 4; ++k) {
+                dxdr = dxdr + dsr[k] * xp[k];
+                dxds = dxds + dss[k] * xp[k];
+                dydr = dydr + dsr[k] * yp[k];
+                dyds = dyds + dss[k] * yp[k];
+            }
+            // planar: rcoef = 1 - alfrz + alfrz*y = 1
+            const double rcoef = 1.0 - 0.0 + 0.0 * 0.0;
+            double gd0[3] = {dyds * 0.0 + (-dydr) * 0.0, dyds * (1.0 / 1.0) + (-dydr) * 0.0,
+                             dyds * 0.0 + (-dydr) * (1.0 / 1.0)};
+            double gd1[3] = {(-dxds) * 0.0 + dxdr * 0.0, (-dxds) * (1.0 / 1.0) + dxdr * 0.0,
+                             (-dxds) * 0.0 + dxdr * (1.0 / 1.0)};
+            double b[3] = {1.0, (rr - rc) / 1.0, (ss - sc) / 1.0};
+            double un[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int id = 0; id < 3; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + u0[q][id] * b[id];
+            double rhoad = 1.0 / un[0];
+            double pres = g1 * rhoad * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]));
+            double uadv = uctr + dudr * b[1] + duds * b[2];
+            double vadv = vctr + dvdr * b[1] + dvds * b[2];
+            pres = pctr + al3 * (pres - pctr);
+            double s11 = -pres + 0.0, s12 = 0.0 + 0.0, s21 = 0.0 + 0.0, s22 = -pres + 0.0;
+            for (int id = 0; id < 3; ++id) {
+                double f0 = gd0[id] * uadv + gd1[id] * vadv;
+                double f1 = gd0[id] * s11 + gd1[id] * s12;
+                double f2 = gd0[id] * s21 + gd1[id] * s22;
+                double f3 = gd0[id] * (s11 * uadv + s12 * vadv) + gd1[id] * (s21 * uadv + s22 * vadv);
+                r[0][id] = r[0][id] - f0 * wi * rcoef;
+                r[1][id] = r[1][id] - f1 * wi * rcoef;
+                r[2][id] = r[2][id] - f2 * wi * rcoef;
+                r[3][id] = r[3][id] - f3 * wi * rcoef;
+            }
+        }
+    }
+    // in-place limiting of the stage DOFs (Getrhslagmc.f90:7283-7296)
+    double ul[4][3];
+    for (int q = 0; q < 4; ++q) ul[q][0] = u0[q][0];
+    ul[0][1] = u0[0][1] * al0;
+    ul[0][2] = u0[0][2] * al0;
+    ul[1][1] = dudr; ul[1][2] = duds;
+    ul[2][1] = dvdr; ul[2][2] = dvds;
+    ul[3][1] = u0[3][1] * al4;
+    ul[3][2] = u0[3][2] * al4;
+    if (apply_update) {
+        const double dt = d.scal[0];
+        double m[3][3];
+        m[0][0] = d.A[3 * nc + ie]; m[0][1] = 0.0; m[0][2] = 0.0;
+        m[1][0] = 0.0; m[1][1] = d.A[0 * nc + ie]; m[1][2] = d.A[1 * nc + ie];
+        m[2][0] = 0.0; m[2][1] = d.A[1 * nc + ie]; m[2][2] = d.A[2 * nc + ie];
+        double rs[4][3];
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < 3; ++id) rs[q][id] = dt * r[q][id];
+        for (int id = 0; id < 3; ++id) {
+            double s4[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < 3; ++k)
+                for (int q = 0; q < 4; ++q) s4[q] = s4[q] + m[id][k] * rs[q][k];
+            for (int q = 0; q < 4; ++q)
+                d.Us[(q * nd + id) * nc + ie] =
+                    alfa * d.Un[(q * nd + id) * nc + ie] + (1.0 - alfa) * (ul[q][id] + s4[q]);
+        }
+    } else {
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < 3; ++id) {
+                d.Us[(q * nd + id) * nc + ie] = ul[q][id];
+                d.R[(q * nd + id) * nc + ie] = r[q][id];
+            }
+    }
+}
+
+#undef UI
+
+}  // namespace lag2d

@@ -1,0 +1,174 @@
+"""ctypes binding of the lag2d C ABI (include/lag2d.h).
+
+The library is the in-tree lag2d_amd/liblag2d.so built by lag2d_amd/build.py;
+loading fails loudly when it is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .case import LagCase
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liblag2d.so")
+
+EXPORTS = ("lag2d_create", "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
+           "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
+           "lag2d_destroy", "lag2d_last_error")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Config(C.Structure):
+    _fields_ = [("ndegr", C.c_int), ("npoly", C.c_int), ("ncurv", C.c_int), ("nstag", C.c_int),
+                ("ndens", C.c_int), ("nlimi", C.c_int), ("ncase", C.c_int), ("nrz", C.c_int),
+                ("nfint", C.c_int), ("nriem", C.c_int), ("ndt", C.c_int),
+                ("gamlg", C.c_double), ("cfl", C.c_double), ("cimpd", C.c_double),
+                ("cdrho", C.c_double), ("tend", C.c_double), ("dtfix", C.c_double),
+                ("tacru", C.c_double)]
+
+
+class Lag2DError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load():
+    """Load liblag2d.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise Lag2DError(f"{LIBPATH} missing: build it with `python -m lag2d_amd.build`")
+    lib = C.CDLL(LIBPATH)
+    lib.lag2d_create.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip,
+                                 C.POINTER(C.c_void_p)]
+    lib.lag2d_upload.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.lag2d_step.argtypes = [C.c_void_p, C.c_int]
+    lib.lag2d_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip]
+    lib.lag2d_rhs.argtypes = [C.c_void_p, _dp]
+    lib.lag2d_download.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    lib.lag2d_get_time.argtypes = [C.c_void_p, _dp, _dp]
+    lib.lag2d_check.argtypes = [C.c_void_p]
+    lib.lag2d_sync.argtypes = [C.c_void_p]
+    lib.lag2d_kernel_time.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_char_p)]
+    lib.lag2d_kernel_stats.argtypes = [C.c_void_p, C.c_int, _dp, _ip, C.POINTER(C.c_char_p)]
+    lib.lag2d_destroy.argtypes = [C.c_void_p]
+    lib.lag2d_destroy.restype = None
+    lib.lag2d_last_error.restype = C.c_char_p
+    _LIB = lib
+    return lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().lag2d_last_error().decode()
+        raise Lag2DError(f"lag2d error {rc}: {msg}")
+
+
+def config_of(case: LagCase) -> Config:
+    c = case.ctl
+    return Config(ndegr=c.ndegr, npoly=c.npoly, ncurv=c.ncurv, nstag=c.nstag, ndens=c.ndens,
+                  nlimi=c.nlimi, ncase=c.ncase, nrz=c.nrz, nfint=c.nfint, nriem=c.nriem, ndt=c.ndt,
+                  gamlg=c.gamlg, cfl=c.cfl, cimpd=c.cimpd, cdrho=c.cdrho, tend=c.tend,
+                  dtfix=case.dtfix, tacru=case.tacru)
+
+
+class Lag2D:
+    """One problem resident on the GPU, driven through the C ABI."""
+
+    def __init__(self, case: LagCase):
+        lib = load()
+        self.case = case
+        self.itime = case.itime
+        cfg = config_of(case)
+        # the C ABI takes the reference's 1-based, column-major arrays
+        ipq = np.ascontiguousarray(case.ipqua + 1, dtype=np.int32)
+        bf = np.ascontiguousarray(case.bface, dtype=np.int32).copy()
+        bf[:, 0] += 1
+        bf[:, 1] += 1
+        bf[:, 4] += 1
+        e1 = np.ascontiguousarray(case.esuv1 + 1, dtype=np.int32)
+        e2 = np.ascontiguousarray(case.esuv2, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib.lag2d_create(C.byref(cfg), case.nquad, case.npoin, case.nbfac, _p(ipq, _ip), _p(bf, _ip),
+                                _p(e1, _ip), _p(e2, _ip), C.byref(h)))
+        self.h = h
+        self.upload(case)
+
+    def upload(self, case: LagCase):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (case.coord, case.coold, case.geoel, case.amatr, case.unkno, case.ustar)]
+        _check(load().lag2d_upload(self.h, *[_p(a) for a in arrs]))
+
+    def step(self, n=1):
+        taken = C.c_int(0)
+        _check(load().lag2d_run(self.h, self.itime + 1, n, C.byref(taken)))
+        self.itime += taken.value
+        return taken.value
+
+    def step_async(self):
+        self.itime += 1
+        _check(load().lag2d_step(self.h, self.itime))
+
+    def sync(self):
+        _check(load().lag2d_sync(self.h))
+
+    def rhs(self):
+        r = np.zeros((self.case.ncell, 4, self.case.ctl.ndegr))
+        _check(load().lag2d_rhs(self.h, _p(r)))
+        return r
+
+    def download(self):
+        c = self.case
+        coord = np.zeros((c.npoin, 2))
+        geoel = np.zeros((c.ncell, 24))
+        unkno = np.zeros((c.ncell, 4, c.ctl.ndegr))
+        ustar = np.zeros((c.npoin, 2))
+        _check(load().lag2d_download(self.h, _p(coord), _p(geoel), _p(unkno), _p(ustar)))
+        return {"coord": coord, "geoel": geoel, "unkno": unkno, "ustar": ustar}
+
+    def time(self):
+        dt, t = C.c_double(), C.c_double()
+        _check(load().lag2d_get_time(self.h, C.byref(dt), C.byref(t)))
+        return dt.value, t.value
+
+    def check(self):
+        return load().lag2d_check(self.h)
+
+    def timing(self, enable: bool):
+        _check(load().lag2d_kernel_time(self.h, 1 if enable else 0, None, None))
+
+    def kernel_report(self):
+        avg = C.c_double()
+        name = C.c_char_p()
+        _check(load().lag2d_kernel_time(self.h, -1, C.byref(avg), C.byref(name)))
+        stats = {}
+        for k in range(32):
+            tot, calls, nm = C.c_double(), C.c_int(), C.c_char_p()
+            if load().lag2d_kernel_stats(self.h, k, C.byref(tot), C.byref(calls), C.byref(nm)) != 0:
+                break
+            if calls.value:
+                stats[nm.value.decode()] = (tot.value, calls.value)
+        return name.value.decode(), avg.value, stats
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().lag2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -157,28 +157,37 @@ def quad_q2(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1, bulge=0.05) -> Mesh:
 SEDOV_ENERGY = 0.244816   # quadrant blast energy (Getrhslagmc.f90:5623-5639)
 
 
-def sedov_inject(unkno, geoel):
-    """Move the reference initialisation's blast energy to the corner cell.
+def sedov_inject(unkno, geoel, cells=(0,), energy=SEDOV_ENERGY):
+    """Place the blast energy in chosen cells of a reference initialisation.
 
     The reference deposits it in four hard-coded element ids
-    (Getrhslagmc.f90:5634-5639); the benchmark places the quadrant blast in
-    cell 1, the cell at the origin: its mass-averaged specific total energy
-    becomes E0/m (rho = 1, u = 0), every other cell keeps the ambient state.
+    (Getrhslagmc.f90:5634-5639), the centre of its 60x60 full-domain mesh;
+    the quadrant benchmark places it in cell 1, at the origin.  Each cell in
+    ``cells`` (0-based) gets mass-averaged specific total energy
+    ``energy``/m (rho = 1, u = 0); every other cell keeps the ambient state.
     ``unkno`` has shape (ncell, 4, ndegr); returns a modified copy."""
     u = unkno.copy()
     ncell = u.shape[0]
+    hard = (1769, 1770, 1829, 1830)
     amb = None
     for e in range(ncell):
-        if e not in (1769, 1770, 1829, 1830):
+        if e not in hard and e not in cells:
             amb = u[e, 3, 0]
             break
-    for e in (1769, 1770, 1829, 1830):
-        if e < ncell:
+    for e in hard:
+        if e < ncell and e not in cells:
             u[e, 3, :] = 0.0
             u[e, 3, 0] = amb
-    u[0, 3, :] = 0.0
-    u[0, 3, 0] = SEDOV_ENERGY / geoel[0, 3]
+    for e in cells:
+        u[e, 3, :] = 0.0
+        u[e, 3, 0] = energy / geoel[e, 3]
     return u
+
+
+def centre_cells(nx, ny):
+    """The four cells around the centre of an even nx x ny structured mesh."""
+    i, j = nx // 2 - 1, ny // 2 - 1
+    return (i + nx * j, i + 1 + nx * j, i + nx * (j + 1), i + 1 + nx * (j + 1))
 
 
 # --------------------------------------------------------------------------
@@ -226,7 +235,7 @@ def sedov_problem(nx=120, ny=None, seed=2024, length=1.2):
     ny = nx if ny is None else ny
     mesh = quad_q2(nx, ny, 0.0, length, 0.0, length * ny / nx, seed=seed)
     ctl = Control(cfl=0.25, tend=1.0, ndegr=6, npoly=2, ncurv=1, nstag=3, ncase=7,
-                  nfint=4, cdrho=0.0)
+                  nfint=4, cdrho=1.0)
     return mesh, ctl
 
 

@@ -41,6 +41,11 @@ typedef struct orc_case {
     /* per-stage outputs kept for inspection */
     double *rhsel;                  /* (ndegr, nq, ncell) */
     double *aflim, *afvec;          /* (nq+1, ncell), (2,2,ncell) */
+    /* curved P2 path */
+    const int *esqua;               /* face neighbours [f + 4*ie]; >= ncell: boundary */
+    int weno_strict;                /* 1: fail where the reference stops (WENO.f90:7699) */
+    int nweno_skipped;              /* troubled cells skipped for their neighbourhood */
+    double *troubled;               /* Persson flag of the last stage (0 / 100), or NULL */
 } orc_case;
 
 /* getcellvol_lag (Limiterlag.f90:1869-2164), quads */
@@ -51,6 +56,8 @@ int orc_rhs(orc_case *c);
 /* getsolvdg_lag (Getrhslag.f90:1553-2035), one full SSP-RK step followed by
  * the driver's copy-back (dgflo2d.f90:408-422). */
 int orc_step(orc_case *c);
+/* curved P2 sub-cell right-hand side (lag2d_oracle_q2.c) */
+int orc_rhs_q2(orc_case *c);
 /* operation counter (FP64 flops) of the last orc_step, when built with
  * ORC_COUNT_FLOPS */
 double orc_flops(void);

@@ -31,6 +31,7 @@ class _Case(C.Structure):
         ("geoel", _dp), ("amatr", _dp),
         ("unkno", _dp), ("unold", _dp), ("ustar", _dp),
         ("rhsel", _dp), ("aflim", _dp), ("afvec", _dp),
+        ("esqua", _ip), ("weno_strict", C.c_int), ("nweno_skipped", C.c_int), ("troubled", _dp),
     ]
 
 
@@ -54,7 +55,7 @@ def _p(a, t=_dp):
 class Oracle:
     """Holds one problem's state and advances it with the C restatement."""
 
-    def __init__(self, case):
+    def __init__(self, case, strict=False):
         c = case.ctl
         self.case = case
         self.ipqua = np.ascontiguousarray(case.ipqua, dtype=np.int32)
@@ -89,6 +90,10 @@ class Oracle:
         s.geoel = _p(self.geoel); s.amatr = _p(self.amatr)
         s.unkno = _p(self.unkno); s.unold = _p(self.unold); s.ustar = _p(self.ustar)
         s.rhsel = _p(self.rhsel); s.aflim = _p(self.aflim); s.afvec = _p(self.afvec)
+        from lag2d_amd.case import face_neighbours
+        self.esqua = np.ascontiguousarray(face_neighbours(case.ipqua, case.esuv1, case.esuv2), dtype=np.int32)
+        self.troubled = np.zeros(n)
+        s.esqua = _p(self.esqua, _ip); s.weno_strict = 1 if strict else 0; s.troubled = _p(self.troubled)
         self.s = s
 
     def rhs(self):

@@ -41,11 +41,16 @@ def _sedov_q2():
     return mg.sedov_problem(12, seed=11), "sedov"
 
 
+def _sedov_q2_centre():
+    return mg.sedov_problem(16, seed=5), "sedov_centre"
+
+
 SPECS = {
     # name: (builder, steps, dump-every)
     "sod_q1_100x4": (_sod, 400, 25),
     "sedov_q1_16x16": (_sedov_q1, 60, 20),
-    "sedov_q2_12x12": (_sedov_q2, 20, 5),
+    "sedov_q2_12x12": (_sedov_q2, 0, 1),
+    "sedov_q2c_16x16": (_sedov_q2_centre, 30, 10),
 }
 
 
@@ -70,10 +75,11 @@ def make(name):
         nd = ctl.ndegr
         _run(work, 0, 1, False)
         ncell, npoin, nbfac, nafac = (int(v) for v in _rd(work, "init_sizes.bin"))
-        if ic == "sedov":
+        if ic in ("sedov", "sedov_centre"):
             un = _rd(work, "init_unkno.bin").reshape(-1, 4, nd)[:ncell]
             ge = _rd(work, "init_geoel.bin").reshape(-1, 24)[:ncell]
-            inj = mg.sedov_inject(un, ge)
+            cells = (0,) if ic == "sedov" else mg.centre_cells(mesh.nx, mesh.ny)
+            inj = mg.sedov_inject(un, ge, cells)
             inj.astype(np.float64).tofile(os.path.join(work, "inject.bin"))
         _run(work, nsteps, every, ic is not None)
         nvqua = 4 if ctl.ncurv == 0 else 9

@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point include/lag2d.h
+declares (no device calls)."""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "lag2d.h")).read()
+    return sorted(set(re.findall(r"\b(lag2d_[a-z_]+)\s*\(", src)))
+
+
+def test_header_lists_entry_points():
+    names = declared()
+    assert "lag2d_step" in names and "lag2d_rhs" in names and "lag2d_create" in names
+
+
+def test_library_exports_all_declared_symbols():
+    import ctypes
+    from lag2d_amd import lib
+    if not os.path.exists(lib.LIBPATH):
+        pytest.fail("liblag2d.so not built (python -m lag2d_amd.build)")
+    so = ctypes.CDLL(lib.LIBPATH)
+    for n in declared():
+        assert hasattr(so, n), n
+    assert set(declared()) <= set(lib.EXPORTS)
+
+
+def test_unsupported_config_is_rejected_without_device():
+    """lag2d_create validates switches before touching the device."""
+    import ctypes
+    import numpy as np
+    from lag2d_amd import lib
+    L = lib.load()
+    cfg = lib.Config(ndegr=3, npoly=1, ncurv=0, nstag=2, ndens=2, nlimi=6, ncase=6, nrz=0, nfint=1,
+                     nriem=1, ndt=1, gamlg=1.4, cfl=0.3, cimpd=1.0, cdrho=0.0, tend=0.2)
+    ipq = np.array([1, 2, 3, 4], dtype=np.int32)
+    bf = np.zeros(5, dtype=np.int32)
+    e1 = np.array([1, 1, 1, 1], dtype=np.int32)
+    e2 = np.array([0, 1, 2, 3, 4], dtype=np.int32)
+    h = ctypes.c_void_p()
+    P = ctypes.POINTER(ctypes.c_int)
+    rc = L.lag2d_create(ctypes.byref(cfg), 1, 4, 0, ipq.ctypes.data_as(P), bf.ctypes.data_as(P),
+                        e1.ctypes.data_as(P), e2.ctypes.data_as(P), ctypes.byref(h))
+    assert rc == 2
+    assert b"ndens" in L.lag2d_last_error()
