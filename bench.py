@@ -1,0 +1,116 @@
+"""Benchmark: FP64 cell-DOF updates per second of the Lagrangian DG step.
+
+Workload: planar Sedov blast, P2 DG on a jittered curved Q2 mesh of 120x120
+quads on [0,1.2]^2, SSP-RK3, CFL 0.25, WENO on troubled cells (the
+benchmark's Sedov configuration), synthetic data generated in-process.
+A step is one full SSP-RK3 step (getsolvdg_lag) with the state resident in
+HBM; throughput = ncell * ndegr * 4 * steps / time.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 120]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=120)
+    ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the Lagrangian step runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from lag2d_amd import init
+    from lag2d_amd.lib import Lag2D
+
+    case = init.sedov_case(args.nx, seed=2024)
+    nc, nd = case.ncell, case.ctl.ndegr
+    dofs_per_step = nc * nd * 4
+    g = Lag2D(case)
+    for _ in range(args.warmup):
+        g.step_async()
+    g.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.step_async()
+    g.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    if g.check() != 0:
+        raise SystemExit("non-finite or non-positive-volume state after the timed steps")
+    dt, tnow = g.time()
+    value = dofs_per_step * args.steps / elapsed
+
+    # per-kernel device times (HIP events on the library's stream), short pass
+    g.timing(True)
+    for _ in range(10):
+        g.step_async()
+    g.sync()
+    name, avg_ms, stats = g.kernel_report()
+    g.timing(False)
+
+    # algorithmic traffic of the dominant kernel per launch (see the design notes)
+    from lag2d_amd import roofline
+    rf = roofline.figures(name, case, avg_ms)
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle  # the C restatement, timed as the CPU baseline
+        ocase = init.sedov_case(args.nx, seed=2024)
+        o = Oracle(ocase)
+        n_cpu = args.cpu_steps or 4
+        c0 = time.perf_counter()
+        o.step(n_cpu)
+        c1 = time.perf_counter()
+        cpu = {"value": dofs_per_step * n_cpu / (c1 - c0), "unit": "cell-DOF updates/s", "cores": 1,
+               "kind": "port",
+               "sample": f"first {n_cpu} SSP-RK3 steps of the same {args.nx}x{args.nx} Sedov P2 problem, "
+                         f"C restatement of the reference step (gcc -O2), 1 thread"}
+
+    line = {
+        "metric": "FP64 cell-DOF updates/s, Sedov P2 curved, 1 GPU",
+        "value": value,
+        "unit": "cell-DOF updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered curved mesh, Sedov initial state)",
+        "config": {"workload": f"sedov_p2_curved_{args.nx}x{args.nx}", "cells": nc, "ndegr": nd,
+                   "nodes": case.npoin, "rk_stages": 3, "cfl": case.ctl.cfl, "limiter": "Persson+WENO",
+                   "sim_time_reached": tnow},
+        "roofline": rf,
+        "cpu_baseline": cpu,
+        "kernels_ms": {k: {"total": v[0], "calls": v[1]} for k, v in stats.items()},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

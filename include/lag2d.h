@@ -25,7 +25,8 @@ enum {
     LAG2D_EUNSUPPORTED = 2,  /* switch combination not implemented */
     LAG2D_EDEVICE = 3,       /* HIP runtime failure */
     LAG2D_ENONFINITE = 4,    /* non-finite state detected by lag2d_check */
-    LAG2D_EVOLUME = 5        /* non-positive cell volume detected by lag2d_check */
+    LAG2D_EVOLUME = 5,       /* non-positive cell volume detected by lag2d_check */
+    LAG2D_ETOPOLOGY = 6      /* WENO stencil topology the reference stops on (strict) */
 };
 
 /* Switches and driver scalars of module `constant` (dgflo2d.f90:4-57) that the
@@ -45,6 +46,10 @@ typedef struct lag2d_config {
     int ndt;      /* 1: CFL-controlled dt, 0: fixed dtfix */
     double gamlg, cfl, cimpd, cdrho, tend;
     double dtfix, tacru;
+    /* 1: report troubled cells whose vertex neighbourhood the reference's
+     * WENO stencil builder cannot classify (it stops there, WENO.f90:7699)
+     * as LAG2D_ETOPOLOGY; 0: treat them like boundary stencils (skip). */
+    int strict;
 } lag2d_config;
 
 typedef struct lag2d_ctx lag2d_ctx;

@@ -77,7 +77,6 @@ extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int n
     if (c.nrz != 0 || c.ndens != 1 || c.nlimi != 6 || c.nriem != 1)
         return fail(LAG2D_EUNSUPPORTED, "supported: nrz=0, ndens=1, nlimi=6, nriem=1");
     if (c.nstag != 2 && c.nstag != 3) return fail(LAG2D_EUNSUPPORTED, "nstag must be 2 or 3");
-    if (q2) return fail(LAG2D_EUNSUPPORTED, "curved P2 path not built in this library");
 
     lag2d_ctx *x = new lag2d_ctx();
     x->cfg = c;
@@ -127,10 +126,13 @@ extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int n
             return bad(fail(LAG2D_EINVAL, "bface node out of range"));
         // wall projection acts on the face's two end nodes (Limiterlag.f90:608-619);
         // fixed walls exclude all face nodes from the solve (:476-481)
+        // (the sub-cell solver of the curved path projects all three face
+        // nodes, Getrhslaggauss.f90:4515-4545)
+        const int nw = nv == 9 ? 3 : 2;
         if (b[2] == 22) {
-            for (int k = 0; k < 2; ++k) nflag[nodes[k]] |= (b[3] == 221) ? 1 : (b[3] == 222 ? 2 : 0);
+            for (int k = 0; k < nw; ++k) nflag[nodes[k]] |= (b[3] == 221) ? 1 : (b[3] == 222 ? 2 : 0);
         } else if (b[2] == 25) {
-            for (int k = 0; k < 2; ++k) nflag[nodes[k]] |= 3;
+            for (int k = 0; k < nw; ++k) nflag[nodes[k]] |= 3;
             for (int k = 0; k < (nv == 9 ? 3 : 2); ++k) nflag[nodes[k]] |= 4;
         }
     }
@@ -160,6 +162,7 @@ extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int n
     uint8_t *t_fl, *t_col;
     const size_t nU = (size_t)4 * c.ndegr * nquad;
     x->nblk_dt = std::min(nblocks(nquad), 1024);
+    if (q2) x->nrec = 160;
     if ((rc = x->alloc(&t_ipq, ipq.size())) || (rc = x->alloc(&t_e1, e1.size())) ||
         (rc = x->alloc(&t_e2, e2.size())) || (rc = x->alloc(&t_n2c, n2c.size())) ||
         (rc = x->alloc(&t_n2co, n2c_off.size())) || (rc = x->alloc(&t_fl, (size_t)npoin)) ||
@@ -174,6 +177,38 @@ extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int n
         (rc = x->alloc(&d.rec, (size_t)x->nrec * nquad)) || (rc = x->alloc(&d.red, (size_t)2 * x->nblk_dt)) ||
         (rc = x->alloc(&d.scal, 4)))
         return bad(rc);
+    if (q2) {
+        // face neighbours for the WENO stencils: neighbour id, or nc + f on
+        // a boundary face (the reference keeps a ghost-cell id there)
+        std::vector<int> esq((size_t)4 * nquad);
+        for (int ie = 0; ie < nquad; ++ie)
+            for (int f = 0; f < 4; ++f) {
+                int a = ipq[(size_t)f * nquad + ie], b = ipq[(size_t)((f + 1) & 3) * nquad + ie];
+                int nb = nquad + f;
+                for (int k = e2[a]; k < e2[a + 1] && nb >= nquad; ++k) {
+                    int je = e1[k];
+                    if (je == ie) continue;
+                    for (int l = e2[b]; l < e2[b + 1]; ++l)
+                        if (e1[l] == je) {
+                            nb = je;
+                            break;
+                        }
+                }
+                esq[(size_t)f * nquad + ie] = nb;
+            }
+        int *t_esq;
+        if ((rc = x->alloc(&d.msub, (size_t)4 * nquad)) || (rc = x->alloc(&d.Ul, nU)) ||
+            (rc = x->alloc(&d.gq, (size_t)48 * nquad)) || (rc = x->alloc(&d.flag, (size_t)nquad)) ||
+            (rc = x->alloc(&d.tlist, (size_t)nquad)) || (rc = x->alloc(&d.tcount, 2)) ||
+            (rc = x->alloc(&t_esq, esq.size())))
+            return bad(rc);
+        d.esqua = t_esq;
+        d.wenobad = d.tcount + 1;
+        d.weno_strict = c.strict;
+        if (hipMemcpy(t_esq, esq.data(), esq.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(d.tcount, 0, 2 * sizeof(int)) != hipSuccess)
+            return bad(fail(LAG2D_EDEVICE, "curved-path table upload"));
+    }
     d.ipq = t_ipq; d.esuv1 = t_e1; d.esuv2 = t_e2; d.n2c = t_n2c; d.n2c_off = t_n2co;
     d.nflag = t_fl; d.ncol = t_col; d.A = x->A;
     d.x0 = x->x0; d.y0 = x->x0 + npoin;
@@ -340,6 +375,15 @@ extern "C" int lag2d_run(lag2d_ctx *x, int itime0, int nsteps, int *taken)
         int rc = step_impl(x, itime0 + k);
         if (rc) return rc;
         ++n;
+        if (x->cfg.strict && x->cfg.ncurv == 1) {
+            int bad = 0;
+            HIPCHK(hipMemcpyAsync(&bad, x->d.wenobad, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+            HIPCHK(hipStreamSynchronize(x->stream));
+            if (bad)
+                return fail(LAG2D_ETOPOLOGY, "WENO stencil: troubled cell with a vertex neighbourhood the "
+                                             "reference stops on (WENO.f90:7699) at step " +
+                                                 std::to_string(itime0 + k));
+        }
         if (watch) {
             HIPCHK(hipMemcpyAsync(sc, x->d.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, x->stream));
             HIPCHK(hipStreamSynchronize(x->stream));

@@ -69,8 +69,16 @@ struct Dev {
     double *rec;             // corner-face records
     double *red;             // reduction scratch
     double *scal;            // device scalars: 0 dtfix, 1 tacru
-    // sub-cell (Q2) data
-    double *msub;            // [4][nc] initial sub-cell masses
+    // curved P2 sub-cell path
+    double *msub;            // [4][nc] initial sub-cell masses (time-invariant)
+    double *Ul;              // limited stage state (after WENO) [(q*nd+id)*nc+ie]
+    double *gq;              // face geometry [48][nc]: slot j -> (nx, ny, |J|) rows 3j..3j+2
+    uint8_t *flag;           // Persson troubled flag per cell
+    int *tlist;              // compacted troubled-cell list
+    int *tcount;             // its length (device counter)
+    const int *esqua;        // face neighbours [4][nc]; >= nc marks a boundary face
+    int weno_strict;         // 1: report cells the reference's stencil builder stops on
+    int *wenobad;            // count of such cells
 };
 
 }  // namespace lag2d

@@ -25,8 +25,16 @@ int build_node_corners(lag2d_ctx &x, const std::vector<int> &ipq, const std::vec
             for (int j = 0; j < (nv == 4 ? 4 : 9); ++j)
                 if (ipq[(size_t)j * nc + ie] == p) iv = j;
             if (iv < 0) return LAG2D_EINVAL;
-            if (nv == 4) n2c.push_back(ie * 4 + iv);
-            else n2c.push_back(ie * 16 + iv);
+            if (nv == 4) {
+                n2c.push_back(ie * 4 + iv);
+            } else {
+                // sub-cell corners of this cell at node p, in the order the
+                // sub-cell solver scatters them (Getrhslaggauss.f90:10639-10858)
+                static const int ipqsg[4][4] = {{0, 4, 8, 7}, {4, 1, 5, 8}, {8, 5, 2, 6}, {7, 8, 6, 3}};
+                for (int isg = 0; isg < 4; ++isg)
+                    for (int ivsg = 0; ivsg < 4; ++ivsg)
+                        if (ipqsg[isg][ivsg] == iv) n2c.push_back(ie * 16 + isg * 4 + ivsg);
+            }
         }
         n2c_off[p + 1] = (int)n2c.size();
     }

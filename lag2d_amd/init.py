@@ -1,0 +1,181 @@
+"""Driver-side initialisation of a Lagrangian DG problem (host, numpy).
+
+The reference computes these arrays before its time loop and hands them to
+the step (dgflo2d.f90:165-282): cell geometry in ``geoel``
+(getgeoel_initial_lag, GetInitial.f90:1278-1697; P2 offsets
+getgeoelho_initial_lag, :1704-2096), the inverse mass matrices ``amatr``
+(getamatr_initial_lag, :697-1274) and the initial DOFs ``unkno``
+(inifield_mcnew, Getrhslagmc.f90:3293-3942, and getunkno_initial_lag).
+This module rebuilds them for meshes of any size with the same definitions
+(mass-centred Taylor basis in reference coordinates, P2 functions with zero
+mass-weighted mean, M^-1 of the degree >= 1 block), vectorised over cells.
+It feeds the benchmark and the large-mesh parity tests; on the small golden
+meshes it reproduces the reference's arrays to round-off.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import meshgen as mg
+from .case import LagCase, build_esuv, mesh_arrays
+
+_G6 = np.array([0.2386191860831969086305017, -0.2386191860831969086305017,
+                0.6612093864662645136613996, -0.6612093864662645136613996,
+                -0.9324695142031520278123016, 0.9324695142031520278123016])
+_W6 = np.array([0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
+                0.3607615730481386, 0.1713244923791704, 0.1713244923791704])
+_G5 = np.array([-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
+                0.5384693101056830910363144, 0.9061798459386639927976269])
+_W5 = np.array([0.2369268850561890875142640, 0.4786286704993664680412915,
+                0.5688888888888888888888889, 0.4786286704993664680412915,
+                0.2369268850561890875142640])
+
+
+def _rule(pg, wg):
+    r = np.repeat(pg, len(pg))
+    s = np.tile(pg, len(pg))
+    w = np.repeat(wg, len(pg)) * np.tile(wg, len(pg))
+    return r, s, w
+
+
+def _shape9(r, s):
+    """Q2 shape functions and reference derivatives at points (r, s)
+    (GetFE.f90:48-204 ordering); returns arrays (npt, 9)."""
+    sh = np.stack([0.25 * (1 - r) * r * s * (1 - s), -0.25 * (1 + r) * (1 - s) * r * s,
+                   0.25 * (1 + r) * (1 + s) * r * s, -0.25 * (1 - r) * (1 + s) * r * s,
+                   -0.5 * (1 - r * r) * (1 - s) * s, 0.5 * (1 + r) * (1 - s * s) * r,
+                   0.5 * (1 - r * r) * (1 + s) * s, -0.5 * (1 - r) * (1 - s * s) * r,
+                   (1 - r * r) * (1 - s * s)], axis=-1)
+    dr = np.stack([0.25 * (2 * r - 1) * s * (s - 1), 0.25 * (2 * r + 1) * s * (s - 1),
+                   0.25 * (2 * r + 1) * s * (s + 1), 0.25 * (2 * r - 1) * s * (s + 1),
+                   -r * s * (s - 1), 0.5 * (2 * r + 1) * (1 - s * s), -r * s * (s + 1),
+                   0.5 * (2 * r - 1) * (1 - s * s), -2 * r * (1 - s * s)], axis=-1)
+    ds = np.stack([0.25 * (2 * s - 1) * r * (r - 1), 0.25 * (2 * s - 1) * r * (r + 1),
+                   0.25 * (2 * s + 1) * r * (r + 1), 0.25 * (2 * s + 1) * r * (r - 1),
+                   0.5 * (1 - r * r) * (2 * s - 1), -r * s * (r + 1), 0.5 * (1 - r * r) * (2 * s + 1),
+                   -r * s * (r - 1), -2 * s * (1 - r * r)], axis=-1)
+    return sh, dr, ds
+
+
+def geometry_points(coord, ipqua0, ncurv):
+    """The 9 geometry points of every quad, (nc, 9, 2): Q2 nodes, or for
+    straight quads the corners, chord midpoints and centre
+    (Limiterlag.f90:2008-2021)."""
+    if ncurv == 1:
+        return coord[ipqua0]
+    c = coord[ipqua0[:, :4]]
+    m = np.stack([0.5 * (c[:, 0] + c[:, 1]), 0.5 * (c[:, 1] + c[:, 2]),
+                  0.5 * (c[:, 2] + c[:, 3]), 0.5 * (c[:, 3] + c[:, 0])], axis=1)
+    ctr = 0.5 * (m[:, 0] + m[:, 2])
+    return np.concatenate([c, m, ctr[:, None, :]], axis=1)
+
+
+def _jac(xp, r, s):
+    sh, dr, ds = _shape9(r, s)
+    x = np.einsum("gk,ekd->egd", sh, xp)
+    dxdr = np.einsum("gk,ek->eg", dr, xp[..., 0])
+    dxds = np.einsum("gk,ek->eg", ds, xp[..., 0])
+    dydr = np.einsum("gk,ek->eg", dr, xp[..., 1])
+    dyds = np.einsum("gk,ek->eg", ds, xp[..., 1])
+    return x, dxdr * dyds - dydr * dxds
+
+
+def initialise(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) -> LagCase:
+    """Initial arrays of the Lagrangian step.
+
+    ``density(xc, yc)`` gives the initial density of each cell from its
+    centre node (the piecewise-constant problems of the reference set cell
+    states by centre, Getrhslagmc.f90:5527-5841); ``state(xc, yc)`` returns
+    (u, v, p) per cell.  ``blast`` = (cells, energy) puts total energy
+    ``energy`` into each listed cell (Sedov)."""
+    ipq, bf, e1, e2 = mesh_arrays(mesh)
+    coord = mesh.coord.astype(np.float64)
+    nc = ipq.shape[0]
+    xp = geometry_points(coord, ipq, ctl.ncurv)
+    xc, yc = xp[:, 8, 0], xp[:, 8, 1]
+    rho = np.asarray(density(xc, yc), dtype=np.float64) * np.ones(nc)
+    u0, v0, p0 = (np.asarray(a, dtype=np.float64) * np.ones(nc) for a in state(xc, yc))
+
+    r6, s6, w6 = _rule(_G6, _W6)
+    _, dj6 = _jac(xp, r6, s6)
+    dj6 = dj6 * w6
+    vol = dj6.sum(axis=1)
+    mass = (rho[:, None] * dj6).sum(axis=1)
+    rc = (rho[:, None] * dj6 * r6).sum(axis=1) / mass
+    sc = (rho[:, None] * dj6 * s6).sum(axis=1) / mass
+    geoel = np.zeros((nc, 24))
+    geoel[:, 0], geoel[:, 1], geoel[:, 2], geoel[:, 3] = rc, sc, vol, mass
+    geoel[:, 4] = (dj6 * r6).sum(axis=1) / vol
+    geoel[:, 5] = (dj6 * s6).sum(axis=1) / vol
+    geoel[:, 6], geoel[:, 7] = geoel[:, 4], geoel[:, 5]
+    b2 = r6[None, :] - rc[:, None]
+    b3 = s6[None, :] - sc[:, None]
+    if ctl.npoly == 2:
+        geoel[:, 18] = (0.5 * rho[:, None] * b2 * b2 * dj6).sum(axis=1) / mass
+        geoel[:, 19] = (0.5 * rho[:, None] * b3 * b3 * dj6).sum(axis=1) / mass
+        geoel[:, 20] = (rho[:, None] * b2 * b3 * dj6).sum(axis=1) / mass
+        c2 = r6[None, :] - geoel[:, 6:7]
+        c3 = s6[None, :] - geoel[:, 7:8]
+        geoel[:, 21] = (0.5 * c2 * c2 * dj6).sum(axis=1) / vol
+        geoel[:, 22] = (0.5 * c3 * c3 * dj6).sum(axis=1) / vol
+        geoel[:, 23] = (c2 * c3 * dj6).sum(axis=1) / vol
+
+    # inverse mass matrix of the degree >= 1 block, 5x5 Gauss
+    r5, s5, w5 = _rule(_G5, _W5)
+    _, dj5 = _jac(xp, r5, s5)
+    dj5 = dj5 * w5
+    q2 = r5[None, :] - rc[:, None]
+    q3 = s5[None, :] - sc[:, None]
+    basis = [q2, q3]
+    if ctl.npoly == 2:
+        basis += [0.5 * q2 * q2 - geoel[:, 18:19], 0.5 * q3 * q3 - geoel[:, 19:20], q2 * q3 - geoel[:, 20:21]]
+    nb = len(basis)
+    M = np.empty((nc, nb, nb))
+    for i in range(nb):
+        for j in range(nb):
+            M[:, i, j] = (rho[:, None] * basis[i] * basis[j] * dj5).sum(axis=1)
+    Mi = np.linalg.inv(M)
+    if ctl.npoly == 1:
+        amatr = np.stack([Mi[:, 0, 0], Mi[:, 0, 1], Mi[:, 1, 1], 1.0 / mass], axis=1)
+    else:
+        iu = np.triu_indices(5)
+        amatr = np.concatenate([Mi[:, iu[0], iu[1]], (1.0 / mass)[:, None]], axis=1)
+
+    g1 = ctl.gamlg - 1.0
+    vol25 = dj5.sum(axis=1)
+    unkno = np.zeros((nc, 4, ctl.ndegr))
+    unkno[:, 0, 0] = vol25 / mass
+    unkno[:, 1, 0] = u0
+    unkno[:, 2, 0] = v0
+    unkno[:, 3, 0] = p0 / (g1 * rho) + 0.5 * (u0 * u0 + v0 * v0)
+    if blast is not None:
+        cells, energy = blast
+        for e in cells:
+            unkno[e, 3, 0] = energy / mass[e]
+    return LagCase(ctl=ctl, ipqua=ipq, bface=bf, esuv1=e1, esuv2=e2, coord=coord.copy(),
+                   coold=coord.copy(), geoel=geoel, amatr=amatr, unkno=unkno,
+                   ustar=np.zeros_like(coord))
+
+
+def sedov_case(nx=120, seed=2024, centre=False) -> LagCase:
+    """Planar Sedov blast on a jittered curved Q2 mesh, P2: rho = 1, u = 0,
+    p = 1e-6, blast energy 0.244816 in the corner cell (or in the four
+    centre cells with ``centre``)."""
+    mesh, ctl = mg.sedov_problem(nx, seed=seed)
+    cells = mg.centre_cells(nx, nx) if centre else (0,)
+    return initialise(mesh, ctl, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6),
+                      blast=(cells, mg.SEDOV_ENERGY))
+
+
+def sod_case(nx=100, ny=4) -> LagCase:
+    """Planar Sod tube on straight quads, P1."""
+    mesh, ctl = mg.sod_problem(nx, ny)
+
+    def rho(x, y):
+        return np.where(np.sqrt(x * x + y * y) <= 0.5, 1.0, 0.125)
+
+    def st(x, y):
+        inside = np.sqrt(x * x + y * y) <= 0.5
+        return 0.0, 0.0, np.where(inside, 1.0, 0.1)
+
+    return initialise(mesh, ctl, rho, st)

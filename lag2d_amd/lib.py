@@ -29,7 +29,7 @@ class Config(C.Structure):
                 ("nfint", C.c_int), ("nriem", C.c_int), ("ndt", C.c_int),
                 ("gamlg", C.c_double), ("cfl", C.c_double), ("cimpd", C.c_double),
                 ("cdrho", C.c_double), ("tend", C.c_double), ("dtfix", C.c_double),
-                ("tacru", C.c_double)]
+                ("tacru", C.c_double), ("strict", C.c_int)]
 
 
 class Lag2DError(RuntimeError):
@@ -76,22 +76,22 @@ def _check(rc):
         raise Lag2DError(f"lag2d error {rc}: {msg}")
 
 
-def config_of(case: LagCase) -> Config:
+def config_of(case: LagCase, strict: bool = False) -> Config:
     c = case.ctl
     return Config(ndegr=c.ndegr, npoly=c.npoly, ncurv=c.ncurv, nstag=c.nstag, ndens=c.ndens,
                   nlimi=c.nlimi, ncase=c.ncase, nrz=c.nrz, nfint=c.nfint, nriem=c.nriem, ndt=c.ndt,
                   gamlg=c.gamlg, cfl=c.cfl, cimpd=c.cimpd, cdrho=c.cdrho, tend=c.tend,
-                  dtfix=case.dtfix, tacru=case.tacru)
+                  dtfix=case.dtfix, tacru=case.tacru, strict=1 if strict else 0)
 
 
 class Lag2D:
     """One problem resident on the GPU, driven through the C ABI."""
 
-    def __init__(self, case: LagCase):
+    def __init__(self, case: LagCase, strict: bool = False):
         lib = load()
         self.case = case
         self.itime = case.itime
-        cfg = config_of(case)
+        cfg = config_of(case, strict)
         # the C ABI takes the reference's 1-based, column-major arrays
         ipq = np.ascontiguousarray(case.ipqua + 1, dtype=np.int32)
         bf = np.ascontiguousarray(case.bface, dtype=np.int32).copy()
