@@ -157,11 +157,15 @@ def initialise(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) -> La
                    ustar=np.zeros_like(coord))
 
 
-def sedov_case(nx=120, seed=2024, centre=False) -> LagCase:
+def sedov_case(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None) -> LagCase:
     """Planar Sedov blast on a jittered curved Q2 mesh, P2: rho = 1, u = 0,
-    p = 1e-6, blast energy 0.244816 in the corner cell (or in the four
-    centre cells with ``centre``)."""
+    p = 1e-6, blast energy 0.244816 in the corner cell (or in each of the
+    four centre cells with ``centre``)."""
     mesh, ctl = mg.sedov_problem(nx, seed=seed)
+    if cdrho is not None:
+        ctl.cdrho = cdrho
+    if cimpd is not None:
+        ctl.cimpd = cimpd
     cells = mg.centre_cells(nx, nx) if centre else (0,)
     return initialise(mesh, ctl, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6),
                       blast=(cells, mg.SEDOV_ENERGY))
