@@ -194,9 +194,11 @@ __device__ __forceinline__ void rotf(const double m[4], double &a, double &b)
     b = y;
 }
 
-// Matinv225, n = 5: inverse = adjugate / det with the Leibniz terms summed in
-// the reference's order (tables PERM5 / PERM4). a, b column-major.
-__device__ void inv5(const double *a, double *b)
+// Matinv225, n = 5, split for a workgroup: the inverse is adjugate / det,
+// each Leibniz expansion summed in the reference's term order (tables PERM5
+// / PERM4).  Matrices are column-major; the inverse holds cof5(a, i, j) /
+// det5(a) at (j, i).
+__device__ double det5(const double *a)
 {
     double det = 0.0;
     for (int t = 0; t < 120; ++t) {
@@ -206,77 +208,24 @@ __device__ void inv5(const double *a, double *b)
         if (p[5] < 0) v = -v;
         det = (t == 0) ? v : det + v;
     }
-    for (int i = 0; i < 5; ++i)
-        for (int j = 0; j < 5; ++j) {
-            int rows[4], cols[4], kk = 0, ll = 0;
-            for (int x = 0; x < 5; ++x)
-                if (x != i) rows[kk++] = x;
-            for (int x = 0; x < 5; ++x)
-                if (x != j) cols[ll++] = x;
-            double cof = 0.0;
-            for (int t = 0; t < 24; ++t) {
-                const signed char *p = PERM4[t];
-                double v = a[rows[0] + 5 * cols[p[0]]];
-                for (int k = 1; k < 4; ++k) v = v * a[rows[k] + 5 * cols[p[k]]];
-                if (p[4] < 0) v = -v;
-                cof = (t == 0) ? v : cof + v;
-            }
-            if ((i + j) & 1) cof = -cof;
-            b[j + 5 * i] = cof / det;
-        }
+    return det;
 }
 
-struct CellPts {
-    double djacoi[25], rhogi[25], xl[25], yl[25];
-};
-
-__device__ void cell_points(const Dev &d, int je, const double m[4], CellPts &P)
+__device__ double cof5(const double *a, int i, int j)
 {
-    const int nc = d.nc;
-    double xi[9], yi[9], xl[9], yl[9];
-    load9(d, d.x0, d.y0, je, xi, yi);
-    load9(d, d.xs, d.ys, je, xl, yl);
-    for (int k = 0; k < 9; ++k) rotf(m, xl[k], yl[k]);
-    double xcr, ycr;
-    map9(xi, yi, d.G[0 * nc + je], d.G[1 * nc + je], &xcr, &ycr);
-    const double rho = rho0(d.ncase, xcr, ycr);
-    for (int i = 0; i < 5; ++i)
-        for (int j = 0; j < 5; ++j) {
-            const int g = 5 * i + j;
-            double r = PG5[i], s = PG5[j], wi = WG5[i] * WG5[j];
-            double J[4];
-            jac9(xi, yi, r, s, J);
-            P.djacoi[g] = wi * (J[0] * J[3] - J[2] * J[1]);
-            map9(xl, yl, r, s, &P.xl[g], &P.yl[g]);
-            P.rhogi[g] = rho;
+    double cof = 0.0;
+    for (int t = 0; t < 24; ++t) {
+        const signed char *p = PERM4[t];
+        double v = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            const int row = k < i ? k : k + 1;
+            const int col = p[k] < j ? p[k] : p[k] + 1;
+            v = (k == 0) ? a[row + 5 * col] : v * a[row + 5 * col];
         }
-}
-
-// Physical mass centre of a cell (weno_char_quad_orthgl, WENO.f90:5983-6064)
-__device__ void mass_centre(const Dev &d, int ie, double &xc, double &yc)
-{
-    const int nc = d.nc;
-    double xp[9], yp[9], xi[9], yi[9];
-    load9(d, d.xs, d.ys, ie, xp, yp);
-    load9(d, d.x0, d.y0, ie, xi, yi);
-    double xcr, ycr;
-    map9(xi, yi, d.G[0 * nc + ie], d.G[1 * nc + ie], &xcr, &ycr);
-    const double rho = rho0(d.ncase, xcr, ycr);
-    double xm = 0.0, ym = 0.0;
-    for (int i = 0; i < 5; ++i)
-        for (int j = 0; j < 5; ++j) {
-            double r = PG5[i], s = PG5[j], wi = WG5[i] * WG5[j];
-            double J[4];
-            jac9(xi, yi, r, s, J);
-            double djacoi = wi * (J[0] * J[3] - J[2] * J[1]);
-            double xg, yg;
-            map9(xp, yp, r, s, &xg, &yg);
-            xm = xm + rho * djacoi * xg;
-            ym = ym + rho * djacoi * yg;
-        }
-    const double masel = d.G[3 * nc + ie];
-    xc = xm / masel;
-    yc = ym / masel;
+        if (p[4] < 0) v = -v;
+        cof = (t == 0) ? v : cof + v;
+    }
+    return ((i + j) & 1) ? -cof : cof;
 }
 
 __device__ void char_mats(double gam, double dnx, double dny, const double ul[4], const double ur[4], double q[4][4],
@@ -336,335 +285,459 @@ __device__ void char_mats(double gam, double dnx, double dny, const double ul[4]
     q[3][3] = -pf + rhosd * vedn;
 }
 
-// One troubled cell: 9-cell stencil, local orthogonal Taylor bases,
-// 1 + 4 Lagrange + 4 Hermite candidates, characteristic nonlinear weights
-// per real face, back-projection (weno_stncl_quad_orthol3 and helpers).
-__global__ void __launch_bounds__(64) k_q2_weno(Dev d, const double *__restrict__ Uin)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= *d.tcount) return;
-    const int nc = d.nc;
-    const int ie = d.tlist[t];
-    // stencil topology
-    int ielaj[9];
-    ielaj[0] = ie;
-    for (int f = 0; f < 4; ++f) ielaj[1 + f] = d.esqua[f * nc + ie];
-    for (int k = 5; k < 9; ++k) ielaj[k] = -2;
-    bool mismatch = false;
-    for (int iv = 0; iv < 4; ++iv) {
-        int jel[3] = {-1, -1, -1}, nv = 0;
-        int p = d.ipq[iv * nc + ie];
-        for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
-            int je = d.esuv1[k];
-            if (je != ie && nv < 3) jel[nv++] = je;
-        }
-        int idxod = 1, idxiv[3] = {0, 0, 0};
-        for (int a = 0; a < 3; ++a)
-            for (int f = 0; f < 4; ++f)
-                if (jel[a] == ielaj[1 + f]) {
-                    idxod = idxod * (f + 1);
-                    idxiv[a] = 1;
-                }
-        for (int a = 0; a < 3; ++a) {
-            if (idxiv[a]) continue;
-            if (idxod == 2) ielaj[5] = jel[a];
-            else if (idxod == 6) ielaj[6] = jel[a];
-            else if (idxod == 12) ielaj[7] = jel[a];
-            else if (idxod == 4) ielaj[8] = jel[a];
-            else mismatch = true;
-        }
-    }
-    if (mismatch) {
-        atomicAdd(d.wenobad, 1);
-        return;
-    }
-    for (int k = 0; k < 9; ++k)
-        if (ielaj[k] < 0) return;
+// One troubled cell per workgroup: 9-cell stencil, local orthogonal Taylor
+// bases, 1 + 4 Lagrange + 4 Hermite candidates, characteristic nonlinear
+// weights per real face, back-projection (weno_stncl_quad_orthol3 and
+// helpers).  The work of a cell is split into phases of independent tasks
+// (quadrature points, matrix entries, cofactors, stencil x variable), with
+// the stencil staged in LDS; every sum keeps the reference's sequential
+// order, so the result does not depend on the split.  Blocks stride over
+// the compacted troubled-cell list.
+constexpr int WENO_WG = 256;
+constexpr int WENO_GRID = 1024;
 
-    // velocity-aligned frame of the troubled cell
+struct WenoShared {
     double m[4];
-    {
-        double a = UD(Uin, 1, 0, ie), b = UD(Uin, 2, 0, ie);
-        if (sqrt(a * a + b * b) > 1.e-10) {
-            m[0] = a / sqrt(a * a + b * b);
-            m[1] = b / sqrt(a * a + b * b);
-            m[2] = -b / sqrt(a * a + b * b);
-            m[3] = a / sqrt(a * a + b * b);
-        } else {
-            m[0] = m[1] = m[2] = m[3] = 0.0;
-        }
-    }
-    double unkrl[9][4][6];
-    for (int s = 0; s < 9; ++s)
-        for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < 6; ++id) unkrl[s][q][id] = UD(Uin, q, id, ielaj[s]);
-    for (int s = 0; s < 9; ++s)
-        for (int id = 0; id < 6; ++id) rotf(m, unkrl[s][1][id], unkrl[s][2][id]);
+    double xl[9][25], yl[9][25], rd[9][25];  // rotated Gauss points, rho * |J| * w
+    double tx[9][25], ty[9][25];             // mass-centre terms (unrotated)
+    double bl[9][25][5];                     // basis about the cell's own mass centre
+    double bl0[9][25][5];                    // basis about the troubled cell's mass centre
+    double geopl[9][5], dxcj[9], masel[9];
+    double unkrl[9][4][6], unkpl[9][4][6], unksl[9][4][6];
+    double bb[9][25], bbq[9][25];
+    double inv[10][25], det[10];             // bbq^-1 of the 9 cells, bb^-1 of the troubled cell
+    double s1[9][5], mh2[9][5][2], rh2[9][4][2];
+    double qm[4][4][4], qi[4][4][4];
+    union {
+        double bq[9][25][5];                 // reference-coordinate basis, until bb is built
+        struct {
+            double rhs[9][4][5];
+            double ms[8][25], mi[8][25], mdet[8];  // 0-3 Lagrange, 4-7 Hermite normal matrices
+            double rs[4][4][5];
+            double ws[4][9];
+            double fch[4][4][6], uf[4][4][6];
+            double phy[4][6], rhsuk[4][5];
+        } l;
+    };
+    int ielaj[9];
+    int skip;
+    int nface;
+    int face[4], fnei[4];
+};
 
-    double geopl[9][5], dxcj[9];
-    CellPts P[9];
-    for (int s = 0; s < 9; ++s) {
-        int je = ielaj[s];
-        double xcl, ycl;
-        mass_centre(d, je, xcl, ycl);
-        rotf(m, xcl, ycl);
-        geopl[s][0] = xcl;
-        geopl[s][1] = ycl;
-        dxcj[s] = sqrt(d.G[2 * nc + je]);
-        cell_points(d, je, m, P[s]);
-        double v1 = 0.0, v2 = 0.0, v3 = 0.0;
-        for (int g = 0; g < 25; ++g) {
-            double b2 = (P[s].xl[g] - xcl) / dxcj[s];
-            double b3 = (P[s].yl[g] - ycl) / dxcj[s];
-            double rd = P[s].rhogi[g] * P[s].djacoi[g];
-            v1 = v1 + rd * b2 * b2;
-            v2 = v2 + rd * b3 * b3;
-            v3 = v3 + rd * b2 * b3;
-        }
-        double masel = d.G[3 * nc + je];
-        geopl[s][2] = 0.5 * (v1 / masel);
-        geopl[s][3] = 0.5 * (v2 / masel);
-        geopl[s][4] = v3 / masel;
-    }
+__constant__ static const int ST1[4][5] = {{1, 4, 5, 8, 7}, {2, 1, 6, 5, 8}, {3, 2, 7, 6, 5}, {4, 3, 8, 7, 6}};
+__constant__ static const int ST2[4][3] = {{1, 5, 8}, {2, 5, 6}, {3, 7, 6}, {4, 7, 8}};
 
-    double unkpl[9][4][6];
-    double mtbbq[25], mtbbi[25];
-    for (int s = 0; s < 9; ++s) {
-        int je = ielaj[s];
-        double rcj = d.G[0 * nc + je], scj = d.G[1 * nc + je];
-        double g0 = d.G[18 * nc + je], g1 = d.G[19 * nc + je], g2 = d.G[20 * nc + je];
-        double bb[25], bbq[25];
-        for (int k = 0; k < 25; ++k) bb[k] = bbq[k] = 0.0;
-        for (int i = 0; i < 5; ++i)
-            for (int j = 0; j < 5; ++j) {
-                const int g = 5 * i + j;
-                double bq[6], bl[6];
-                basis6(PG5[i], PG5[j], rcj, scj, g0, g1, g2, bq);
-                bl[0] = 1.0;
-                bl[1] = (P[s].xl[g] - geopl[s][0]) / dxcj[s];
-                bl[2] = (P[s].yl[g] - geopl[s][1]) / dxcj[s];
-                bl[3] = 0.5 * bl[1] * bl[1] - geopl[s][2];
-                bl[4] = 0.5 * bl[2] * bl[2] - geopl[s][3];
-                bl[5] = bl[1] * bl[2] - geopl[s][4];
-                double rd = P[s].rhogi[g] * P[s].djacoi[g];
-                for (int a = 1; a < 6; ++a)
-                    for (int b = 1; b < 6; ++b) {
-                        bb[(a - 1) + 5 * (b - 1)] = bb[(a - 1) + 5 * (b - 1)] + rd * bq[a] * bl[b];
-                        bbq[(a - 1) + 5 * (b - 1)] = bbq[(a - 1) + 5 * (b - 1)] + rd * bl[a] * bl[b];
-                    }
-            }
-        double rhs[4][6];
-        for (int q = 0; q < 4; ++q)
-            for (int a = 0; a < 6; ++a) rhs[q][a] = 0.0;
-        for (int a = 1; a < 6; ++a)
-            for (int b = 1; b < 6; ++b)
-                for (int q = 0; q < 4; ++q) rhs[q][a] = rhs[q][a] + unkrl[s][q][b] * bb[(b - 1) + 5 * (a - 1)];
-        double bbqi[25], bbi[25];
-        inv5(bbq, bbqi);
-        inv5(bb, bbi);
-        for (int q = 0; q < 4; ++q)
-            for (int a = 0; a < 6; ++a) unkpl[s][q][a] = 0.0;
-        for (int a = 1; a < 6; ++a)
-            for (int b = 1; b < 6; ++b)
-                for (int q = 0; q < 4; ++q) unkpl[s][q][a] = unkpl[s][q][a] + rhs[q][b] * bbqi[(b - 1) + 5 * (a - 1)];
-        if (s == 0) {
-            for (int k = 0; k < 25; ++k) {
-                mtbbq[k] = bbq[k];
-                mtbbi[k] = bbi[k];
-            }
-        }
-        for (int q = 0; q < 4; ++q) unkpl[s][q][0] = unkrl[s][q][0];
-    }
+// Row k of the Hermite least-squares matrix of stencil is, column a
+// (rows 0-2: the cell-average constraints, 3-8: the two gradient
+// constraints of each neighbour).
+__device__ __forceinline__ double mh_entry(const WenoShared &S, int is, int k, int a)
+{
+    if (k < 3) return S.s1[ST2[is][k]][a];
+    const int e = (k - 3) >> 1;
+    return S.mh2[ST2[is][e]][a][(k - 3) & 1];
+}
 
-    const int ST1[4][5] = {{1, 4, 5, 8, 7}, {2, 1, 6, 5, 8}, {3, 2, 7, 6, 5}, {4, 3, 8, 7, 6}};
-    const int ST2[4][3] = {{1, 5, 8}, {2, 5, 6}, {3, 7, 6}, {4, 7, 8}};
-    double unksl[9][4][6];
-    for (int s = 0; s < 9; ++s)
-        for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < 6; ++id) unksl[s][q][id] = 0.0;
-    for (int s = 0; s < 9; ++s)
-        for (int q = 0; q < 4; ++q) unksl[s][q][0] = unkpl[0][q][0];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) unksl[0][q][id] = unkpl[0][q][id];
-    const double dxcl = sqrt(d.G[2 * nc + ie]);
-    const double xmcl = geopl[0][0], ymcl = geopl[0][1];
-    int isten = 0;
-    for (int is = 0; is < 4; ++is) {
-        ++isten;
-        double mat[25], rhsl[4][5];
-        for (int k = 0; k < 25; ++k) mat[k] = 0.0;
-        for (int e = 0; e < 5; ++e) {
-            int sj = ST1[is][e];
-            double masej = d.G[3 * nc + ielaj[sj]];
-            for (int g = 0; g < 25; ++g) {
-                double bl[6];
-                bl[1] = (P[sj].xl[g] - xmcl) / dxcl;
-                bl[2] = (P[sj].yl[g] - ymcl) / dxcl;
-                bl[3] = 0.5 * bl[1] * bl[1] - geopl[0][2];
-                bl[4] = 0.5 * bl[2] * bl[2] - geopl[0][3];
-                bl[5] = bl[1] * bl[2] - geopl[0][4];
-                double rd = P[sj].rhogi[g] * P[sj].djacoi[g];
-                for (int k = 0; k < 5; ++k) mat[e + 5 * k] = mat[e + 5 * k] + rd * bl[k + 1] / masej;
-            }
-            for (int q = 0; q < 4; ++q) rhsl[q][e] = unkpl[sj][q][0] - unkpl[0][q][0];
-        }
-        double mi[25];
-        inv5(mat, mi);
-        for (int a = 1; a < 6; ++a)
-            for (int e = 0; e < 5; ++e)
-                for (int q = 0; q < 4; ++q) unksl[isten][q][a] = unksl[isten][q][a] + mi[(a - 1) + 5 * e] * rhsl[q][e];
-    }
-    for (int is = 0; is < 4; ++is) {
-        ++isten;
-        double mh[45], rh[4][9];
-        for (int k = 0; k < 45; ++k) mh[k] = 0.0;
-        for (int q = 0; q < 4; ++q)
-            for (int k = 0; k < 9; ++k) rh[q][k] = 0.0;
-        for (int e = 0; e < 3; ++e) {
-            int sj = ST2[is][e];
-            double masej = d.G[3 * nc + ielaj[sj]];
-            double dxj = dxcj[sj];
-            double xclj = geopl[sj][0], yclj = geopl[sj][1];
-            const int row = 3 + 2 * e;
-            for (int g = 0; g < 25; ++g) {
-                double bl[6], bj[6];
-                bl[1] = (P[sj].xl[g] - xmcl) / dxcl;
-                bl[2] = (P[sj].yl[g] - ymcl) / dxcl;
-                bl[3] = 0.5 * bl[1] * bl[1] - geopl[0][2];
-                bl[4] = 0.5 * bl[2] * bl[2] - geopl[0][3];
-                bl[5] = bl[1] * bl[2] - geopl[0][4];
-                double rd = P[sj].rhogi[g] * P[sj].djacoi[g];
-                for (int k = 0; k < 5; ++k) mh[e + 9 * k] = mh[e + 9 * k] + rd * bl[k + 1] / masej;
-                bj[1] = (P[sj].xl[g] - xclj) / dxj;
-                bj[2] = (P[sj].yl[g] - yclj) / dxj;
-                bj[3] = 0.5 * bj[1] * bj[1] - geopl[sj][2];
-                bj[4] = 0.5 * bj[2] * bj[2] - geopl[sj][3];
-                bj[5] = bj[1] * bj[2] - geopl[sj][4];
-                for (int a = 1; a < 6; ++a) {
-                    mh[row + 9 * (a - 1)] = mh[row + 9 * (a - 1)] + rd * bl[a] * bj[1] / masej;
-                    mh[row + 1 + 9 * (a - 1)] = mh[row + 1 + 9 * (a - 1)] + rd * bl[a] * bj[2] / masej;
+__global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__restrict__ Uin)
+{
+    __shared__ WenoShared S;
+    const int tid = threadIdx.x;
+    const int nc = d.nc;
+    const int ntr = *d.tcount;
+    for (int t = blockIdx.x; t < ntr; t += gridDim.x) {
+        __syncthreads();  // the previous cell's staging is dead
+        const int ie = d.tlist[t];
+        if (tid == 0) {
+            // stencil topology: face neighbours, then the vertex neighbour
+            // opposite each corner, identified by the product of the face
+            // numbers it shares
+            int ielaj[9];
+            ielaj[0] = ie;
+            for (int f = 0; f < 4; ++f) ielaj[1 + f] = d.esqua[f * nc + ie];
+            for (int k = 5; k < 9; ++k) ielaj[k] = -2;
+            bool mismatch = false;
+            for (int iv = 0; iv < 4; ++iv) {
+                int jel[3] = {-1, -1, -1}, nv = 0;
+                const int p = d.ipq[iv * nc + ie];
+                for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
+                    const int je = d.esuv1[k];
+                    if (je != ie && nv < 3) jel[nv++] = je;
                 }
-                for (int a = 1; a < 6; ++a)
+                int idxod = 1, idxiv[3] = {0, 0, 0};
+                for (int a = 0; a < 3; ++a)
+                    for (int f = 0; f < 4; ++f)
+                        if (jel[a] == ielaj[1 + f]) {
+                            idxod = idxod * (f + 1);
+                            idxiv[a] = 1;
+                        }
+                for (int a = 0; a < 3; ++a) {
+                    if (idxiv[a]) continue;
+                    if (idxod == 2) ielaj[5] = jel[a];
+                    else if (idxod == 6) ielaj[6] = jel[a];
+                    else if (idxod == 12) ielaj[7] = jel[a];
+                    else if (idxod == 4) ielaj[8] = jel[a];
+                    else mismatch = true;
+                }
+            }
+            int skip = 0;
+            if (mismatch) {
+                atomicAdd(d.wenobad, 1);
+                skip = 1;
+            }
+            for (int k = 0; k < 9; ++k) {
+                if (ielaj[k] < 0) skip = 1;
+                S.ielaj[k] = ielaj[k];
+            }
+            S.skip = skip;
+            int nf = 0;
+            for (int f = 0; f < 4; ++f) {
+                const int je = d.esqua[f * nc + ie];
+                if (je >= nc) continue;
+                S.face[nf] = f;
+                S.fnei[nf] = je;
+                ++nf;
+            }
+            S.nface = nf;
+            // velocity-aligned frame of the troubled cell
+            const double a = UD(Uin, 1, 0, ie), b = UD(Uin, 2, 0, ie);
+            if (sqrt(a * a + b * b) > 1.e-10) {
+                S.m[0] = a / sqrt(a * a + b * b);
+                S.m[1] = b / sqrt(a * a + b * b);
+                S.m[2] = -b / sqrt(a * a + b * b);
+                S.m[3] = a / sqrt(a * a + b * b);
+            } else {
+                S.m[0] = S.m[1] = S.m[2] = S.m[3] = 0.0;
+            }
+        }
+        __syncthreads();
+        if (S.skip) continue;
+        const double m[4] = {S.m[0], S.m[1], S.m[2], S.m[3]};
+
+        // 1: Gauss points of the stencil, rotated DOFs, face characteristic matrices
+        for (int k = tid; k < 225 + 54 + 4; k += WENO_WG) {
+            if (k < 225) {
+                const int s = k / 25, g = k - 25 * s, i = g / 5, j = g - 5 * i;
+                const int je = S.ielaj[s];
+                double xi[9], yi[9], xp[9], yp[9];
+                load9(d, d.x0, d.y0, je, xi, yi);
+                load9(d, d.xs, d.ys, je, xp, yp);
+                double xcr, ycr;
+                map9(xi, yi, d.G[0 * nc + je], d.G[1 * nc + je], &xcr, &ycr);
+                const double rho = rho0(d.ncase, xcr, ycr);
+                const double r = PG5[i], sg = PG5[j], wi = WG5[i] * WG5[j];
+                double J[4];
+                jac9(xi, yi, r, sg, J);
+                const double djacoi = wi * (J[0] * J[3] - J[2] * J[1]);
+                const double rdv = rho * djacoi;
+                double xg, yg;
+                map9(xp, yp, r, sg, &xg, &yg);
+                S.tx[s][g] = rdv * xg;
+                S.ty[s][g] = rdv * yg;
+                for (int q = 0; q < 9; ++q) rotf(m, xp[q], yp[q]);
+                map9(xp, yp, r, sg, &xg, &yg);
+                S.xl[s][g] = xg;
+                S.yl[s][g] = yg;
+                S.rd[s][g] = rdv;
+                double b[6];
+                basis6(r, sg, d.G[0 * nc + je], d.G[1 * nc + je], d.G[18 * nc + je], d.G[19 * nc + je],
+                       d.G[20 * nc + je], b);
+                for (int a = 0; a < 5; ++a) S.bq[s][g][a] = b[a + 1];
+            } else if (k < 225 + 54) {
+                const int u = k - 225, s = u / 6, id = u - 6 * s;
+                const int je = S.ielaj[s];
+                double v[4];
+                for (int q = 0; q < 4; ++q) v[q] = UD(Uin, q, id, je);
+                rotf(m, v[1], v[2]);
+                for (int q = 0; q < 4; ++q) S.unkrl[s][q][id] = v[q];
+            } else {
+                const int a = k - 225 - 54;
+                if (a < S.nface) {
+                    const int f = S.face[a], je = S.fnei[a];
+                    const int a0 = f, a1 = (f + 1) & 3;
+                    const int p0 = d.ipq[a0 * nc + ie], p1 = d.ipq[a1 * nc + ie];
+                    const double dtx = d.xs[p1] - d.xs[p0];
+                    const double dty = d.ys[p1] - d.ys[p0];
+                    const double dlgt = sqrt(dtx * dtx + dty * dty);
+                    double dnx = dty / dlgt, dny = -dtx / dlgt;
+                    rotf(m, dnx, dny);
+                    double ul[4], ur[4];
                     for (int q = 0; q < 4; ++q) {
-                        rh[q][row] = rh[q][row] + rd * bj[a] * bj[1] * unkpl[sj][q][a] / masej;
-                        rh[q][row + 1] = rh[q][row + 1] + rd * bj[a] * bj[2] * unkpl[sj][q][a] / masej;
+                        ul[q] = UD(Uin, q, 0, ie);
+                        ur[q] = UD(Uin, q, 0, je);
                     }
-            }
-            for (int q = 0; q < 4; ++q) rh[q][e] = unkpl[sj][q][0] - unkpl[0][q][0];
-        }
-        double ms[25], rs[4][5];
-        for (int a = 0; a < 5; ++a)
-            for (int b = 0; b < 5; ++b) {
-                double v = 0.0;
-                for (int k = 0; k < 9; ++k) v = v + mh[k + 9 * a] * mh[k + 9 * b];
-                ms[a + 5 * b] = v;
-            }
-        for (int q = 0; q < 4; ++q)
-            for (int a = 0; a < 5; ++a) {
-                double v = 0.0;
-                for (int k = 0; k < 9; ++k) v = v + mh[k + 9 * a] * rh[q][k];
-                rs[q][a] = v;
-            }
-        double mi[25];
-        inv5(ms, mi);
-        for (int a = 1; a < 6; ++a)
-            for (int e = 0; e < 5; ++e)
-                for (int q = 0; q < 4; ++q) unksl[isten][q][a] = unksl[isten][q][a] + mi[(a - 1) + 5 * e] * rs[q][e];
-    }
-    const int nst = isten + 1;
-
-    double xp[9], yp[9];
-    load9(d, d.xs, d.ys, ie, xp, yp);
-    const double volel = d.G[2 * nc + ie];
-    const double dxc = sqrt(volel), dyc = dxc;
-    double unk_phy[4][6];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) unk_phy[q][id] = 0.0;
-    double uf[4][4][6];
-    int ifprj = 0;
-    for (int f = 0; f < 4; ++f) {
-        int je = d.esqua[f * nc + ie];
-        if (je >= nc) continue;
-        int a0 = f, a1 = (f + 1) & 3;
-        double dtx = xp[a1] - xp[a0];
-        double dty = yp[a1] - yp[a0];
-        double dlgt = sqrt(dtx * dtx + dty * dty);
-        double dnx = dty / dlgt, dny = -dtx / dlgt;
-        rotf(m, dnx, dny);
-        double ul[4], ur[4];
-        for (int q = 0; q < 4; ++q) {
-            ul[q] = UD(Uin, q, 0, ie);
-            ur[q] = UD(Uin, q, 0, je);
-        }
-        rotf(m, ul[1], ul[2]);
-        rotf(m, ur[1], ur[2]);
-        double qm[4][4], qi[4][4];
-        char_mats(d.gamlg, dnx, dny, ul, ur, qm, qi);
-        double os1[9];
-        for (int s = 0; s < nst; ++s) {
-            double o[4];
-            for (int k = 0; k < 4; ++k) {
-                double c1 = 0.0, c2 = 0.0;
-                for (int q = 0; q < 4; ++q) {
-                    c1 = c1 + qi[k][q] * unksl[s][q][1];
-                    c2 = c2 + qi[k][q] * unksl[s][q][2];
+                    rotf(m, ul[1], ul[2]);
+                    rotf(m, ur[1], ur[2]);
+                    char_mats(d.gamlg, dnx, dny, ul, ur, S.qm[a], S.qi[a]);
                 }
-                o[k] = (c1 / dxc) * (c1 / dxc) + (c2 / dyc) * (c2 / dyc);
             }
-            os1[s] = dmax(dmax(dmax(o[0], o[1]), o[2]), o[3]);
         }
-        double wl[9], wh[9], wt = 0.0;
-        wl[0] = .2;
-        for (int s = 1; s < nst; ++s) wl[s] = 0.8 / ((double)nst - 1.0);
-        for (int s = 0; s < nst; ++s) {
-            double tt = 1.e-6 + os1[s];
-            wh[s] = wl[s] / (tt * tt);
+        __syncthreads();
+        // 2: physical mass centres
+        if (tid < 9) {
+            const int s = tid, je = S.ielaj[s];
+            double xm = 0.0, ym = 0.0;
+            for (int g = 0; g < 25; ++g) {
+                xm = xm + S.tx[s][g];
+                ym = ym + S.ty[s][g];
+            }
+            const double masel = d.G[3 * nc + je];
+            double xc = xm / masel, yc = ym / masel;
+            rotf(m, xc, yc);
+            S.geopl[s][0] = xc;
+            S.geopl[s][1] = yc;
+            S.dxcj[s] = sqrt(d.G[2 * nc + je]);
+            S.masel[s] = masel;
         }
-        for (int s = 0; s < nst; ++s) wt = wt + wh[s];
-        double fch[4][6];
-        for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < 6; ++id) fch[q][id] = 0.0;
-        for (int s = 0; s < nst; ++s) {
-            double ws = wh[s] / wt;
-            for (int k = 0; k < 4; ++k)
-                for (int id = 0; id < 6; ++id) {
-                    double c = 0.0;
-                    for (int q = 0; q < 4; ++q) c = c + qi[k][q] * unksl[s][q][id];
-                    fch[k][id] = fch[k][id] + ws * c;
+        __syncthreads();
+        // 3: linear physical basis about each cell's own centre
+        for (int k = tid; k < 225; k += WENO_WG) {
+            const int s = k / 25, g = k - 25 * s;
+            S.bl[s][g][0] = (S.xl[s][g] - S.geopl[s][0]) / S.dxcj[s];
+            S.bl[s][g][1] = (S.yl[s][g] - S.geopl[s][1]) / S.dxcj[s];
+        }
+        __syncthreads();
+        // 4: second moments
+        if (tid < 27) {
+            const int s = tid / 3, c = tid - 3 * s;
+            double v = 0.0;
+            for (int g = 0; g < 25; ++g) {
+                const double b2 = S.bl[s][g][0], b3 = S.bl[s][g][1], rd = S.rd[s][g];
+                const double x = (c == 1) ? b3 : b2, y = (c == 0) ? b2 : b3;
+                v = v + rd * x * y;
+            }
+            S.geopl[s][2 + c] = (c < 2) ? 0.5 * (v / S.masel[s]) : v / S.masel[s];
+        }
+        __syncthreads();
+        // 5: quadratic terms; the basis about the troubled cell's centre
+        for (int k = tid; k < 225; k += WENO_WG) {
+            const int s = k / 25, g = k - 25 * s;
+            const double b1 = S.bl[s][g][0], b2 = S.bl[s][g][1];
+            S.bl[s][g][2] = 0.5 * b1 * b1 - S.geopl[s][2];
+            S.bl[s][g][3] = 0.5 * b2 * b2 - S.geopl[s][3];
+            S.bl[s][g][4] = b1 * b2 - S.geopl[s][4];
+            const double c1 = (S.xl[s][g] - S.geopl[0][0]) / S.dxcj[0];
+            const double c2 = (S.yl[s][g] - S.geopl[0][1]) / S.dxcj[0];
+            S.bl0[s][g][0] = c1;
+            S.bl0[s][g][1] = c2;
+            S.bl0[s][g][2] = 0.5 * c1 * c1 - S.geopl[0][2];
+            S.bl0[s][g][3] = 0.5 * c2 * c2 - S.geopl[0][3];
+            S.bl0[s][g][4] = c1 * c2 - S.geopl[0][4];
+        }
+        __syncthreads();
+        // 6: projection matrices bb (reference -> physical basis) and bbq
+        //    (physical mass matrix); stencil-matrix moments of the neighbours
+        for (int k = tid; k < 450 + 40 + 80; k += WENO_WG) {
+            if (k < 450) {
+                const int s = k / 50, r = k - 50 * s, which = r / 25, e = r - 25 * which;
+                const int b = e / 5, a = e - 5 * b;
+                double v = 0.0;
+                if (which == 0) {
+                    for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bq[s][g][a] * S.bl[s][g][b];
+                    S.bb[s][e] = v;
+                } else {
+                    for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl[s][g][a] * S.bl[s][g][b];
+                    S.bbq[s][e] = v;
                 }
+            } else if (k < 490) {
+                const int u = k - 450, s = 1 + u / 5, c = u - 5 * (u / 5);
+                double v = 0.0;
+                for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl0[s][g][c] / S.masel[s];
+                S.s1[s][c] = v;
+            } else {
+                const int u = k - 490, s = 1 + u / 10, r = u - 10 * (u / 10), a = r >> 1, c = r & 1;
+                double v = 0.0;
+                for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl0[s][g][a] * S.bl[s][g][c] / S.masel[s];
+                S.mh2[s][a][c] = v;
+            }
         }
-        for (int k = 0; k < 4; ++k)
-            for (int id = 0; id < 6; ++id) uf[ifprj][k][id] = 0.0;
-        for (int q = 0; q < 4; ++q)
-            for (int id = 1; id < 6; ++id)
-                for (int k = 0; k < 4; ++k) uf[ifprj][k][id] = uf[ifprj][k][id] + qm[k][q] * fch[q][id];
-        ++ifprj;
+        __syncthreads();
+        // 7: inverses of bbq (9 cells) and bb (troubled cell): determinants
+        //    and cofactors; rhs of the projections; stencil matrices
+        for (int k = tid; k < 10 + 250 + 180 + 100 + 100; k += WENO_WG) {
+            if (k < 10) {
+                S.det[k] = det5(k < 9 ? S.bbq[k] : S.bb[0]);
+            } else if (k < 260) {
+                const int u = k - 10, mi = u / 25, e = u - 25 * mi, i = e / 5, j = e - 5 * i;
+                S.inv[mi][j + 5 * i] = cof5(mi < 9 ? S.bbq[mi] : S.bb[0], i, j);
+            } else if (k < 440) {
+                const int u = k - 260, s = u / 20, q = (u / 5) & 3, a = u - 5 * (u / 5);
+                double v = 0.0;
+                for (int b = 0; b < 5; ++b) v = v + S.unkrl[s][q][b + 1] * S.bb[s][b + 5 * a];
+                S.l.rhs[s][q][a] = v;
+            } else if (k < 540) {
+                const int u = k - 440, is = u / 25, e = u - 25 * is, kk = e / 5, r = e - 5 * kk;
+                S.l.ms[is][r + 5 * kk] = S.s1[ST1[is][r]][kk];
+            } else {
+                const int u = k - 540, is = u / 25, e = u - 25 * is, b = e / 5, a = e - 5 * b;
+                double v = 0.0;
+                for (int kk = 0; kk < 9; ++kk) v = v + mh_entry(S, is, kk, a) * mh_entry(S, is, kk, b);
+                S.l.ms[4 + is][a + 5 * b] = v;
+            }
+        }
+        __syncthreads();
+        // 8: divide by the determinants; determinants and cofactors of the
+        //    stencil matrices
+        for (int k = tid; k < 8 + 250 + 200; k += WENO_WG) {
+            if (k < 8) {
+                S.l.mdet[k] = det5(S.l.ms[k]);
+            } else if (k < 258) {
+                const int u = k - 8, mi = u / 25, e = u - 25 * mi;
+                S.inv[mi][e] = S.inv[mi][e] / S.det[mi];
+            } else {
+                const int u = k - 258, mi = u / 25, e = u - 25 * mi, i = e / 5, j = e - 5 * i;
+                S.l.mi[mi][j + 5 * i] = cof5(S.l.ms[mi], i, j);
+            }
+        }
+        __syncthreads();
+        // 9: stencil DOFs in the physical bases; divide the stencil inverses
+        for (int k = tid; k < 180 + 36 + 200; k += WENO_WG) {
+            if (k < 180) {
+                const int s = k / 20, q = (k / 5) & 3, a = k - 5 * (k / 5);
+                double v = 0.0;
+                for (int b = 0; b < 5; ++b) v = v + S.l.rhs[s][q][b] * S.inv[s][b + 5 * a];
+                S.unkpl[s][q][a + 1] = v;
+            } else if (k < 216) {
+                const int u = k - 180, s = u >> 2, q = u & 3;
+                S.unkpl[s][q][0] = S.unkrl[s][q][0];
+            } else {
+                const int u = k - 216, mi = u / 25, e = u - 25 * mi;
+                S.l.mi[mi][e] = S.l.mi[mi][e] / S.l.mdet[mi];
+            }
+        }
+        __syncthreads();
+        // 10: Hermite gradient moments; Lagrange candidates; candidate averages
+        for (int k = tid; k < 64 + 80 + 36 + 20; k += WENO_WG) {
+            if (k < 64) {
+                const int s = 1 + (k >> 3), q = (k >> 1) & 3, c = k & 1;
+                double v = 0.0;
+                for (int g = 0; g < 25; ++g) {
+                    const double rd = S.rd[s][g];
+                    for (int a = 0; a < 5; ++a)
+                        v = v + rd * S.bl[s][g][a] * S.bl[s][g][c] * S.unkpl[s][q][a + 1] / S.masel[s];
+                }
+                S.rh2[s][q][c] = v;
+            } else if (k < 144) {
+                const int u = k - 64, is = u / 20, q = (u / 5) & 3, a = u - 5 * (u / 5);
+                double v = 0.0;
+                for (int e = 0; e < 5; ++e)
+                    v = v + S.l.mi[is][a + 5 * e] * (S.unkpl[ST1[is][e]][q][0] - S.unkpl[0][q][0]);
+                S.unksl[1 + is][q][a + 1] = v;
+            } else if (k < 180) {
+                const int u = k - 144, s = u >> 2, q = u & 3;
+                S.unksl[s][q][0] = S.unkpl[0][q][0];
+            } else {
+                const int u = k - 180, q = u / 5, a = u - 5 * q;
+                S.unksl[0][q][a + 1] = S.unkpl[0][q][a + 1];
+            }
+        }
+        __syncthreads();
+        // 11: Hermite normal-equation right-hand sides
+        if (tid < 80) {
+            const int is = tid / 20, q = (tid / 5) & 3, a = tid - 5 * (tid / 5);
+            double v = 0.0;
+            for (int kk = 0; kk < 9; ++kk) {
+                double rh;
+                if (kk < 3) {
+                    rh = S.unkpl[ST2[is][kk]][q][0] - S.unkpl[0][q][0];
+                } else {
+                    const int e = (kk - 3) >> 1;
+                    rh = S.rh2[ST2[is][e]][q][(kk - 3) & 1];
+                }
+                v = v + mh_entry(S, is, kk, a) * rh;
+            }
+            S.l.rs[is][q][a] = v;
+        }
+        __syncthreads();
+        // 12: Hermite candidates
+        if (tid < 80) {
+            const int is = tid / 20, q = (tid / 5) & 3, a = tid - 5 * (tid / 5);
+            double v = 0.0;
+            for (int e = 0; e < 5; ++e) v = v + S.l.mi[4 + is][a + 5 * e] * S.l.rs[is][q][e];
+            S.unksl[5 + is][q][a + 1] = v;
+        }
+        __syncthreads();
+        // 13: smoothness indicators and nonlinear weights per real face
+        const int nst = 9;
+        if (tid < S.nface) {
+            const int a = tid;
+            const double dxc = S.dxcj[0], dyc = dxc;
+            double os1[9];
+            for (int s = 0; s < nst; ++s) {
+                double o[4];
+                for (int k = 0; k < 4; ++k) {
+                    double c1 = 0.0, c2 = 0.0;
+                    for (int q = 0; q < 4; ++q) {
+                        c1 = c1 + S.qi[a][k][q] * S.unksl[s][q][1];
+                        c2 = c2 + S.qi[a][k][q] * S.unksl[s][q][2];
+                    }
+                    o[k] = (c1 / dxc) * (c1 / dxc) + (c2 / dyc) * (c2 / dyc);
+                }
+                os1[s] = dmax(dmax(dmax(o[0], o[1]), o[2]), o[3]);
+            }
+            double wl[9], wh[9], wt = 0.0;
+            wl[0] = .2;
+            for (int s = 1; s < nst; ++s) wl[s] = 0.8 / ((double)nst - 1.0);
+            for (int s = 0; s < nst; ++s) {
+                const double tt = 1.e-6 + os1[s];
+                wh[s] = wl[s] / (tt * tt);
+            }
+            for (int s = 0; s < nst; ++s) wt = wt + wh[s];
+            for (int s = 0; s < nst; ++s) S.l.ws[a][s] = wh[s] / wt;
+        }
+        __syncthreads();
+        // 14: weighted characteristic polynomial per real face
+        for (int k = tid; k < S.nface * 24; k += WENO_WG) {
+            const int a = k / 24, r = k - 24 * a, kk = r / 6, id = r - 6 * kk;
+            double v = 0.0;
+            for (int s = 0; s < nst; ++s) {
+                double c = 0.0;
+                for (int q = 0; q < 4; ++q) c = c + S.qi[a][kk][q] * S.unksl[s][q][id];
+                v = v + S.l.ws[a][s] * c;
+            }
+            S.l.fch[a][kk][id] = v;
+        }
+        __syncthreads();
+        // 15: back to conserved variables
+        for (int k = tid; k < S.nface * 20; k += WENO_WG) {
+            const int a = k / 20, r = k - 20 * a, kk = r / 5, id = 1 + r - 5 * (r / 5);
+            double v = 0.0;
+            for (int q = 0; q < 4; ++q) v = v + S.qm[a][kk][q] * S.l.fch[a][q][id];
+            S.l.uf[a][kk][id] = v;
+        }
+        __syncthreads();
+        // 16: average over the real faces
+        if (tid < 20) {
+            const int q = tid / 5, id = 1 + tid - 5 * (tid / 5);
+            const double wf = 1.0 / (double)S.nface;
+            double v = 0.0;
+            for (int a = 0; a < S.nface; ++a) v = v + wf * S.l.uf[a][q][id];
+            S.l.phy[q][id] = v;
+        }
+        __syncthreads();
+        // 17: project back onto the reference-coordinate basis
+        if (tid < 20) {
+            const int q = tid / 5, a = tid - 5 * (tid / 5);
+            double v = 0.0;
+            for (int b = 0; b < 5; ++b) v = v + S.l.phy[q][b + 1] * S.bbq[0][b + 5 * a];
+            S.l.rhsuk[q][a] = v;
+        }
+        __syncthreads();
+        if (tid < 5) {
+            const int a = tid;
+            double out[4];
+            for (int q = 0; q < 4; ++q) {
+                double v = 0.0;
+                for (int b = 0; b < 5; ++b) v = v + S.l.rhsuk[q][b] * S.inv[9][b + 5 * a];
+                out[q] = v;
+            }
+            const double i11 = m[3], i12 = -m[1], i21 = -m[2], i22 = m[0];
+            const double x = out[1] * i11 + out[2] * i12;
+            const double y = out[1] * i21 + out[2] * i22;
+            out[1] = x;
+            out[2] = y;
+            for (int q = 0; q < 4; ++q) UD(d.Ul, q, a + 1, ie) = out[q];
+        }
     }
-    const double wf = 1.0 / (double)ifprj;
-    for (int a = 0; a < ifprj; ++a)
-        for (int q = 0; q < 4; ++q)
-            for (int id = 1; id < 6; ++id) unk_phy[q][id] = unk_phy[q][id] + wf * uf[a][q][id];
-    double rhsuk[4][5];
-    for (int q = 0; q < 4; ++q)
-        for (int a = 0; a < 5; ++a) rhsuk[q][a] = 0.0;
-    for (int a = 1; a < 6; ++a)
-        for (int b = 1; b < 6; ++b)
-            for (int q = 0; q < 4; ++q) rhsuk[q][a - 1] = rhsuk[q][a - 1] + unk_phy[q][b] * mtbbq[(b - 1) + 5 * (a - 1)];
-    double out[4][6];
-    for (int q = 0; q < 4; ++q)
-        for (int a = 0; a < 6; ++a) out[q][a] = 0.0;
-    for (int a = 1; a < 6; ++a)
-        for (int b = 1; b < 6; ++b)
-            for (int q = 0; q < 4; ++q) out[q][a] = out[q][a] + rhsuk[q][b - 1] * mtbbi[(b - 1) + 5 * (a - 1)];
-    const double i11 = m[3], i12 = -m[1], i21 = -m[2], i22 = m[0];
-    for (int a = 1; a < 6; ++a) {
-        double x = out[1][a] * i11 + out[2][a] * i12;
-        double y = out[1][a] * i21 + out[2][a] * i22;
-        out[1][a] = x;
-        out[2][a] = y;
-    }
-    for (int q = 0; q < 4; ++q)
-        for (int a = 1; a < 6; ++a) UD(d.Ul, q, a, ie) = out[q][a];
 }
 
 // ------------------------------------------------------------- corners --
@@ -1145,7 +1218,7 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
     hipLaunchKernelGGL(k_q2_prep, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
     x->toc(KID_Q2_PREP);
     x->tic(KID_Q2_WENO);
-    hipLaunchKernelGGL(k_q2_weno, dim3((nc + 63) / 64), dim3(64), 0, x->stream, ds, Uin);
+    hipLaunchKernelGGL(k_q2_weno, dim3(nc < WENO_GRID ? nc : WENO_GRID), dim3(WENO_WG), 0, x->stream, ds, Uin);
     x->toc(KID_Q2_WENO);
     x->tic(KID_Q2_CORNER);
     hipLaunchKernelGGL(k_q2_corner, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
