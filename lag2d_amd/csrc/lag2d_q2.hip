@@ -17,7 +17,7 @@
 //   rhsdomndg_lagmc_curvquad      Getrhslag.f90:4716-5013
 // Work split: one thread per cell (prep, corner, cell RHS), per node (nodal
 // solve), per troubled cell (WENO, over a compacted list).
-#include <vector>
+#include <utility>
 
 #include "lag2d_dev.h"
 #include "lag2d_host.h"
@@ -39,10 +39,6 @@ __constant__ static double WFR[4][4][2];
 // face-integral weights: momentum/energy {0,1} and volume times 1/4 (:8573-8621)
 __constant__ static double WFM[4][4][2];
 __constant__ static double WFV[4][4][2];
-// Leibniz-expansion term tables in the order of Matinv225 (LinearAlgebra.f90:130-):
-// 5x5: 120 permutations; 4x4 minors: 24 permutations; sign in the last entry
-__constant__ static signed char PERM5[120][6];
-__constant__ static signed char PERM4[24][5];
 // 5x5 Gauss rule of ruqope (dgflo2d.f90:2734-2743 ordering)
 __constant__ static const double PG5[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
                                            0.5384693101056830910363144, 0.9061798459386639927976269};
@@ -195,36 +191,108 @@ __device__ __forceinline__ void rotf(const double m[4], double &a, double &b)
 }
 
 // Matinv225, n = 5, split for a workgroup: the inverse is adjugate / det,
-// each Leibniz expansion summed in the reference's term order (tables PERM5
-// / PERM4).  Matrices are column-major; the inverse holds cof5(a, i, j) /
-// det5(a) at (j, i).
-__device__ double det5(const double *a)
+// each Leibniz expansion summed in the reference's term order.  The term
+// tables are generated at compile time: the column taken by the last row
+// ascends slowest, then the row before, ...; the first row takes the
+// remaining column; the sign is the parity of the permutation.  Matrices are
+// column-major; the inverse holds cof5(a, i, j) / det5(a) at (j, i).
+template <int M>
+struct Perms {
+    static constexpr int N = (M == 5) ? 120 : 24;
+    signed char p[N][M];
+    bool neg[N];
+};
+
+template <int M>
+constexpr Perms<M> make_perms()
+{
+    Perms<M> T{};
+    int q[M] = {};  // q = (p[M-1], ..., p[0]) in lexicographic order
+    for (int k = 0; k < M; ++k) q[k] = k;
+    for (int t = 0; t < Perms<M>::N; ++t) {
+        for (int k = 0; k < M; ++k) T.p[t][k] = (signed char)q[M - 1 - k];
+        int inv = 0;
+        for (int x = 0; x < M; ++x)
+            for (int y = x + 1; y < M; ++y)
+                if (T.p[t][x] > T.p[t][y]) ++inv;
+        T.neg[t] = (inv & 1) != 0;
+        int i = M - 2;
+        while (i >= 0 && q[i] >= q[i + 1]) --i;
+        if (i < 0) break;
+        int j = M - 1;
+        while (q[j] <= q[i]) --j;
+        int tmp = q[i];
+        q[i] = q[j];
+        q[j] = tmp;
+        for (int a = i + 1, b = M - 1; a < b; ++a, --b) {
+            tmp = q[a];
+            q[a] = q[b];
+            q[b] = tmp;
+        }
+    }
+    return T;
+}
+
+constexpr Perms<5> kP5 = make_perms<5>();
+constexpr Perms<4> kP4 = make_perms<4>();
+static_assert(kP5.p[0][4] == 0 && kP5.p[0][3] == 1 && kP5.p[0][0] == 4 && !kP5.neg[0] &&
+                  kP5.p[119][4] == 4 && kP5.p[119][0] == 0,
+              "Leibniz term order");
+
+template <int T>
+__device__ __forceinline__ double det5_term(const double (&m)[25])
+{
+    double v = m[0 + 5 * kP5.p[T][0]];
+    v = v * m[1 + 5 * kP5.p[T][1]];
+    v = v * m[2 + 5 * kP5.p[T][2]];
+    v = v * m[3 + 5 * kP5.p[T][3]];
+    v = v * m[4 + 5 * kP5.p[T][4]];
+    if constexpr (kP5.neg[T]) v = -v;
+    return v;
+}
+
+template <int... T>
+__device__ __forceinline__ double det5_sum(const double (&m)[25], std::integer_sequence<int, T...>)
 {
     double det = 0.0;
-    for (int t = 0; t < 120; ++t) {
-        const signed char *p = PERM5[t];
-        double v = a[0 + 5 * p[0]];
-        for (int k = 1; k < 5; ++k) v = v * a[k + 5 * p[k]];
-        if (p[5] < 0) v = -v;
-        det = (t == 0) ? v : det + v;
-    }
+    ((det = (T == 0) ? det5_term<T>(m) : det + det5_term<T>(m)), ...);
     return det;
 }
 
+template <int T>
+__device__ __forceinline__ double cof4_term(const double (&m)[16])
+{
+    double v = m[0 + 4 * kP4.p[T][0]];
+    v = v * m[1 + 4 * kP4.p[T][1]];
+    v = v * m[2 + 4 * kP4.p[T][2]];
+    v = v * m[3 + 4 * kP4.p[T][3]];
+    if constexpr (kP4.neg[T]) v = -v;
+    return v;
+}
+
+template <int... T>
+__device__ __forceinline__ double det4_sum(const double (&m)[16], std::integer_sequence<int, T...>)
+{
+    double det = 0.0;
+    ((det = (T == 0) ? cof4_term<T>(m) : det + cof4_term<T>(m)), ...);
+    return det;
+}
+
+__device__ double det5(const double *a)
+{
+    double m[25];
+    for (int k = 0; k < 25; ++k) m[k] = a[k];
+    return det5_sum(m, std::make_integer_sequence<int, 120>{});
+}
+
+// signed cofactor (i, j): Leibniz expansion of the minor without row i and
+// column j
 __device__ double cof5(const double *a, int i, int j)
 {
-    double cof = 0.0;
-    for (int t = 0; t < 24; ++t) {
-        const signed char *p = PERM4[t];
-        double v = 0.0;
-        for (int k = 0; k < 4; ++k) {
-            const int row = k < i ? k : k + 1;
-            const int col = p[k] < j ? p[k] : p[k] + 1;
-            v = (k == 0) ? a[row + 5 * col] : v * a[row + 5 * col];
-        }
-        if (p[4] < 0) v = -v;
-        cof = (t == 0) ? v : cof + v;
-    }
+    double m[16];
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) m[r + 4 * c] = a[(r < i ? r : r + 1) + 5 * (c < j ? c : c + 1)];
+    const double cof = det4_sum(m, std::make_integer_sequence<int, 24>{});
     return ((i + j) & 1) ? -cof : cof;
 }
 
@@ -320,6 +388,7 @@ struct WenoShared {
         } l;
     };
     int ielaj[9];
+    int jel[4][3];
     int skip;
     int nface;
     int face[4], fnei[4];
@@ -342,40 +411,61 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
 {
     __shared__ WenoShared S;
     const int tid = threadIdx.x;
+    // task index of this thread: consecutive tasks go to different waves, so
+    // phases with few tasks still use all four SIMDs
+    const int tsp = ((tid & 63) << 2) | (tid >> 6);
     const int nc = d.nc;
     const int ntr = *d.tcount;
     for (int t = blockIdx.x; t < ntr; t += gridDim.x) {
         __syncthreads();  // the previous cell's staging is dead
         const int ie = d.tlist[t];
+        // stencil topology: face neighbours, then the vertex neighbour
+        // opposite each corner, identified by the product of the face
+        // numbers it shares
+        if (tid < 4) {
+            S.ielaj[1 + tid] = d.esqua[tid * nc + ie];
+        } else if (tid >= 64 && tid < 68) {
+            const int iv = tid - 64;
+            int jel[3] = {-1, -1, -1}, nv = 0;
+            const int p = d.ipq[iv * nc + ie];
+            for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
+                const int je = d.esuv1[k];
+                if (je != ie && nv < 3) jel[nv++] = je;
+            }
+            for (int a = 0; a < 3; ++a) S.jel[iv][a] = jel[a];
+        } else if (tid == 128) {
+            // velocity-aligned frame of the troubled cell
+            const double a = UD(Uin, 1, 0, ie), b = UD(Uin, 2, 0, ie);
+            if (sqrt(a * a + b * b) > 1.e-10) {
+                S.m[0] = a / sqrt(a * a + b * b);
+                S.m[1] = b / sqrt(a * a + b * b);
+                S.m[2] = -b / sqrt(a * a + b * b);
+                S.m[3] = a / sqrt(a * a + b * b);
+            } else {
+                S.m[0] = S.m[1] = S.m[2] = S.m[3] = 0.0;
+            }
+        }
+        __syncthreads();
         if (tid == 0) {
-            // stencil topology: face neighbours, then the vertex neighbour
-            // opposite each corner, identified by the product of the face
-            // numbers it shares
             int ielaj[9];
             ielaj[0] = ie;
-            for (int f = 0; f < 4; ++f) ielaj[1 + f] = d.esqua[f * nc + ie];
+            for (int f = 0; f < 4; ++f) ielaj[1 + f] = S.ielaj[1 + f];
             for (int k = 5; k < 9; ++k) ielaj[k] = -2;
             bool mismatch = false;
             for (int iv = 0; iv < 4; ++iv) {
-                int jel[3] = {-1, -1, -1}, nv = 0;
-                const int p = d.ipq[iv * nc + ie];
-                for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
-                    const int je = d.esuv1[k];
-                    if (je != ie && nv < 3) jel[nv++] = je;
-                }
                 int idxod = 1, idxiv[3] = {0, 0, 0};
                 for (int a = 0; a < 3; ++a)
                     for (int f = 0; f < 4; ++f)
-                        if (jel[a] == ielaj[1 + f]) {
+                        if (S.jel[iv][a] == ielaj[1 + f]) {
                             idxod = idxod * (f + 1);
                             idxiv[a] = 1;
                         }
                 for (int a = 0; a < 3; ++a) {
                     if (idxiv[a]) continue;
-                    if (idxod == 2) ielaj[5] = jel[a];
-                    else if (idxod == 6) ielaj[6] = jel[a];
-                    else if (idxod == 12) ielaj[7] = jel[a];
-                    else if (idxod == 4) ielaj[8] = jel[a];
+                    if (idxod == 2) ielaj[5] = S.jel[iv][a];
+                    else if (idxod == 6) ielaj[6] = S.jel[iv][a];
+                    else if (idxod == 12) ielaj[7] = S.jel[iv][a];
+                    else if (idxod == 4) ielaj[8] = S.jel[iv][a];
                     else mismatch = true;
                 }
             }
@@ -391,30 +481,20 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
             S.skip = skip;
             int nf = 0;
             for (int f = 0; f < 4; ++f) {
-                const int je = d.esqua[f * nc + ie];
+                const int je = ielaj[1 + f];
                 if (je >= nc) continue;
                 S.face[nf] = f;
                 S.fnei[nf] = je;
                 ++nf;
             }
             S.nface = nf;
-            // velocity-aligned frame of the troubled cell
-            const double a = UD(Uin, 1, 0, ie), b = UD(Uin, 2, 0, ie);
-            if (sqrt(a * a + b * b) > 1.e-10) {
-                S.m[0] = a / sqrt(a * a + b * b);
-                S.m[1] = b / sqrt(a * a + b * b);
-                S.m[2] = -b / sqrt(a * a + b * b);
-                S.m[3] = a / sqrt(a * a + b * b);
-            } else {
-                S.m[0] = S.m[1] = S.m[2] = S.m[3] = 0.0;
-            }
         }
         __syncthreads();
         if (S.skip) continue;
         const double m[4] = {S.m[0], S.m[1], S.m[2], S.m[3]};
 
         // 1: Gauss points of the stencil, rotated DOFs, face characteristic matrices
-        for (int k = tid; k < 225 + 54 + 4; k += WENO_WG) {
+        for (int k = tsp; k < 225 + 54 + 4; k += WENO_WG) {
             if (k < 225) {
                 const int s = k / 25, g = k - 25 * s, i = g / 5, j = g - 5 * i;
                 const int je = S.ielaj[s];
@@ -473,8 +553,8 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 2: physical mass centres
-        if (tid < 9) {
-            const int s = tid, je = S.ielaj[s];
+        if (tsp < 9) {
+            const int s = tsp, je = S.ielaj[s];
             double xm = 0.0, ym = 0.0;
             for (int g = 0; g < 25; ++g) {
                 xm = xm + S.tx[s][g];
@@ -490,15 +570,15 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 3: linear physical basis about each cell's own centre
-        for (int k = tid; k < 225; k += WENO_WG) {
+        for (int k = tsp; k < 225; k += WENO_WG) {
             const int s = k / 25, g = k - 25 * s;
             S.bl[s][g][0] = (S.xl[s][g] - S.geopl[s][0]) / S.dxcj[s];
             S.bl[s][g][1] = (S.yl[s][g] - S.geopl[s][1]) / S.dxcj[s];
         }
         __syncthreads();
         // 4: second moments
-        if (tid < 27) {
-            const int s = tid / 3, c = tid - 3 * s;
+        if (tsp < 27) {
+            const int s = tsp / 3, c = tsp - 3 * s;
             double v = 0.0;
             for (int g = 0; g < 25; ++g) {
                 const double b2 = S.bl[s][g][0], b3 = S.bl[s][g][1], rd = S.rd[s][g];
@@ -509,7 +589,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 5: quadratic terms; the basis about the troubled cell's centre
-        for (int k = tid; k < 225; k += WENO_WG) {
+        for (int k = tsp; k < 225; k += WENO_WG) {
             const int s = k / 25, g = k - 25 * s;
             const double b1 = S.bl[s][g][0], b2 = S.bl[s][g][1];
             S.bl[s][g][2] = 0.5 * b1 * b1 - S.geopl[s][2];
@@ -526,7 +606,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         __syncthreads();
         // 6: projection matrices bb (reference -> physical basis) and bbq
         //    (physical mass matrix); stencil-matrix moments of the neighbours
-        for (int k = tid; k < 450 + 40 + 80; k += WENO_WG) {
+        for (int k = tsp; k < 450 + 40 + 80; k += WENO_WG) {
             if (k < 450) {
                 const int s = k / 50, r = k - 50 * s, which = r / 25, e = r - 25 * which;
                 const int b = e / 5, a = e - 5 * b;
@@ -553,7 +633,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         __syncthreads();
         // 7: inverses of bbq (9 cells) and bb (troubled cell): determinants
         //    and cofactors; rhs of the projections; stencil matrices
-        for (int k = tid; k < 10 + 250 + 180 + 100 + 100; k += WENO_WG) {
+        for (int k = tsp; k < 10 + 250 + 180 + 100 + 100; k += WENO_WG) {
             if (k < 10) {
                 S.det[k] = det5(k < 9 ? S.bbq[k] : S.bb[0]);
             } else if (k < 260) {
@@ -577,7 +657,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         __syncthreads();
         // 8: divide by the determinants; determinants and cofactors of the
         //    stencil matrices
-        for (int k = tid; k < 8 + 250 + 200; k += WENO_WG) {
+        for (int k = tsp; k < 8 + 250 + 200; k += WENO_WG) {
             if (k < 8) {
                 S.l.mdet[k] = det5(S.l.ms[k]);
             } else if (k < 258) {
@@ -590,7 +670,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 9: stencil DOFs in the physical bases; divide the stencil inverses
-        for (int k = tid; k < 180 + 36 + 200; k += WENO_WG) {
+        for (int k = tsp; k < 180 + 36 + 200; k += WENO_WG) {
             if (k < 180) {
                 const int s = k / 20, q = (k / 5) & 3, a = k - 5 * (k / 5);
                 double v = 0.0;
@@ -606,7 +686,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 10: Hermite gradient moments; Lagrange candidates; candidate averages
-        for (int k = tid; k < 64 + 80 + 36 + 20; k += WENO_WG) {
+        for (int k = tsp; k < 64 + 80 + 36 + 20; k += WENO_WG) {
             if (k < 64) {
                 const int s = 1 + (k >> 3), q = (k >> 1) & 3, c = k & 1;
                 double v = 0.0;
@@ -632,8 +712,8 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 11: Hermite normal-equation right-hand sides
-        if (tid < 80) {
-            const int is = tid / 20, q = (tid / 5) & 3, a = tid - 5 * (tid / 5);
+        if (tsp < 80) {
+            const int is = tsp / 20, q = (tsp / 5) & 3, a = tsp - 5 * (tsp / 5);
             double v = 0.0;
             for (int kk = 0; kk < 9; ++kk) {
                 double rh;
@@ -649,36 +729,38 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 12: Hermite candidates
-        if (tid < 80) {
-            const int is = tid / 20, q = (tid / 5) & 3, a = tid - 5 * (tid / 5);
+        if (tsp < 80) {
+            const int is = tsp / 20, q = (tsp / 5) & 3, a = tsp - 5 * (tsp / 5);
             double v = 0.0;
             for (int e = 0; e < 5; ++e) v = v + S.l.mi[4 + is][a + 5 * e] * S.l.rs[is][q][e];
             S.unksl[5 + is][q][a + 1] = v;
         }
         __syncthreads();
-        // 13: smoothness indicators and nonlinear weights per real face
+        // 13: smoothness indicator of each candidate per real face, then the
+        //     nonlinear weights
         const int nst = 9;
-        if (tid < S.nface) {
-            const int a = tid;
+        for (int k = tsp; k < S.nface * nst; k += WENO_WG) {
+            const int a = k / nst, s = k - nst * a;
             const double dxc = S.dxcj[0], dyc = dxc;
-            double os1[9];
-            for (int s = 0; s < nst; ++s) {
-                double o[4];
-                for (int k = 0; k < 4; ++k) {
-                    double c1 = 0.0, c2 = 0.0;
-                    for (int q = 0; q < 4; ++q) {
-                        c1 = c1 + S.qi[a][k][q] * S.unksl[s][q][1];
-                        c2 = c2 + S.qi[a][k][q] * S.unksl[s][q][2];
-                    }
-                    o[k] = (c1 / dxc) * (c1 / dxc) + (c2 / dyc) * (c2 / dyc);
+            double o[4];
+            for (int kk = 0; kk < 4; ++kk) {
+                double c1 = 0.0, c2 = 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    c1 = c1 + S.qi[a][kk][q] * S.unksl[s][q][1];
+                    c2 = c2 + S.qi[a][kk][q] * S.unksl[s][q][2];
                 }
-                os1[s] = dmax(dmax(dmax(o[0], o[1]), o[2]), o[3]);
+                o[kk] = (c1 / dxc) * (c1 / dxc) + (c2 / dyc) * (c2 / dyc);
             }
+            S.l.ws[a][s] = dmax(dmax(dmax(o[0], o[1]), o[2]), o[3]);
+        }
+        __syncthreads();
+        if (tsp < S.nface) {
+            const int a = tsp;
             double wl[9], wh[9], wt = 0.0;
             wl[0] = .2;
             for (int s = 1; s < nst; ++s) wl[s] = 0.8 / ((double)nst - 1.0);
             for (int s = 0; s < nst; ++s) {
-                const double tt = 1.e-6 + os1[s];
+                const double tt = 1.e-6 + S.l.ws[a][s];
                 wh[s] = wl[s] / (tt * tt);
             }
             for (int s = 0; s < nst; ++s) wt = wt + wh[s];
@@ -686,7 +768,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 14: weighted characteristic polynomial per real face
-        for (int k = tid; k < S.nface * 24; k += WENO_WG) {
+        for (int k = tsp; k < S.nface * 24; k += WENO_WG) {
             const int a = k / 24, r = k - 24 * a, kk = r / 6, id = r - 6 * kk;
             double v = 0.0;
             for (int s = 0; s < nst; ++s) {
@@ -698,7 +780,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 15: back to conserved variables
-        for (int k = tid; k < S.nface * 20; k += WENO_WG) {
+        for (int k = tsp; k < S.nface * 20; k += WENO_WG) {
             const int a = k / 20, r = k - 20 * a, kk = r / 5, id = 1 + r - 5 * (r / 5);
             double v = 0.0;
             for (int q = 0; q < 4; ++q) v = v + S.qm[a][kk][q] * S.l.fch[a][q][id];
@@ -706,8 +788,8 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 16: average over the real faces
-        if (tid < 20) {
-            const int q = tid / 5, id = 1 + tid - 5 * (tid / 5);
+        if (tsp < 20) {
+            const int q = tsp / 5, id = 1 + tsp - 5 * (tsp / 5);
             const double wf = 1.0 / (double)S.nface;
             double v = 0.0;
             for (int a = 0; a < S.nface; ++a) v = v + wf * S.l.uf[a][q][id];
@@ -715,15 +797,15 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 17: project back onto the reference-coordinate basis
-        if (tid < 20) {
-            const int q = tid / 5, a = tid - 5 * (tid / 5);
+        if (tsp < 20) {
+            const int q = tsp / 5, a = tsp - 5 * (tsp / 5);
             double v = 0.0;
             for (int b = 0; b < 5; ++b) v = v + S.l.phy[q][b + 1] * S.bbq[0][b + 5 * a];
             S.l.rhsuk[q][a] = v;
         }
         __syncthreads();
-        if (tid < 5) {
-            const int a = tid;
+        if (tsp < 5) {
+            const int a = tsp;
             double out[4];
             for (int q = 0; q < 4; ++q) {
                 double v = 0.0;
@@ -745,23 +827,40 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
 // specific volume, vertex states and the 32 corner-face Riemann records.
 // Records: rec[(r*32 + cf)*nc + ie], cf = f + 2*(ivsg + 4*isg),
 // r = 0: m = mu*A, 1-2: m u_c, 3-4: A sigma n.
-__global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_corner(Dev d)
+// 16 lanes per cell: first lane k evaluates face slot k and a quarter of
+// the 25 quadrature points of sub-cell k/4 (terms staged in LDS), then lane
+// (isg, ivsg) sums its sub-cell's terms in point order and writes the two
+// records of its sub-cell corner.
+constexpr int CORNER_CPB = 16;
+
+__global__ void __launch_bounds__(256) k_q2_corner(Dev d)
 {
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sgq[CORNER_CPB][16][3];
+    __shared__ double sta[CORNER_CPB][4][25], stm[CORNER_CPB][4][25], stv[CORNER_CPB][4][25];
+    const int cl = threadIdx.x >> 4, lane = threadIdx.x & 15;
+    const int ie = blockIdx.x * CORNER_CPB + cl;
     const int nc = d.nc;
-    if (ie >= nc) return;
+    const bool active = ie < nc;
+    const int isg = lane >> 2, part = lane & 3;
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
     double xp[9], yp[9];
-    load9(d, d.xs, d.ys, ie, xp, yp);
-    // ---- face geometry
-    double gq[16][3];
-    {
-        const double pq[4][2] = {{0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
-        for (int k = 0; k < 4; ++k) {
+    double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
+    if (active) {
+        load9(d, d.xs, d.ys, ie, xp, yp);
+        rc = d.G[0 * nc + ie];
+        sc = d.G[1 * nc + ie];
+        gg0 = d.G[18 * nc + ie];
+        gg1 = d.G[19 * nc + ie];
+        gg2 = d.G[20 * nc + ie];
+        // ---- face slot `lane`: 8-11 interior half-edges, others boundary points
+        double gx, gy, ga;
+        if (lane >= 8 && lane < 12) {
+            const int k = lane - 8;
+            const double pq[4][2] = {{0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
             double xv, yv;
             map9(xp, yp, pq[k][0], pq[k][1], &xv, &yv);
-            double fx[3] = {xp[4 + k], xp[8], xv};
-            double fy[3] = {yp[4 + k], yp[8], yv};
+            const double fx[3] = {xp[4 + k], xp[8], xv};
+            const double fy[3] = {yp[4 + k], yp[8], yv};
             const double r = -1.0;
             const double dd[3] = {-0.5 + r, 0.5 + r, -2.0 * r};
             double dxdr = 0.0, dydr = 0.0;
@@ -769,137 +868,127 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_corner(Dev d)
                 dxdr = dxdr + dd[i] * fx[i];
                 dydr = dydr + dd[i] * fy[i];
             }
-            double djaco = sqrt(dxdr * dxdr + dydr * dydr);
-            double anx = -dydr, any = dxdr;
-            gq[8 + k][0] = anx / sqrt(anx * anx + any * any);
-            gq[8 + k][1] = any / sqrt(anx * anx + any * any);
-            gq[8 + k][2] = djaco * 2.0;
-        }
-        const double pi[12][2] = {{-1, -1}, {1, -1}, {1, -1}, {1, 1}, {1, 1}, {-1, 1},
-                                  {-1, 1},  {-1, -1}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
-        const double vn[12][2] = {{0, -1}, {0, -1}, {1, 0}, {1, 0}, {0, 1}, {0, 1},
-                                  {-1, 0}, {-1, 0}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
-        const int slot[12] = {0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 14, 15};
-        for (int ig = 0; ig < 12; ++ig) {
+            const double djaco = sqrt(dxdr * dxdr + dydr * dydr);
+            const double anx = -dydr, any = dxdr;
+            gx = anx / sqrt(anx * anx + any * any);
+            gy = any / sqrt(anx * anx + any * any);
+            ga = djaco * 2.0;
+        } else {
+            const int ig = lane < 8 ? lane : lane - 4;
+            const double pi[12][2] = {{-1, -1}, {1, -1}, {1, -1}, {1, 1}, {1, 1}, {-1, 1},
+                                      {-1, 1},  {-1, -1}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
+            const double vn[12][2] = {{0, -1}, {0, -1}, {1, 0}, {1, 0}, {0, 1}, {0, 1},
+                                      {-1, 0}, {-1, 0}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
             double J[4];
             jac9(xp, yp, pi[ig][0], pi[ig][1], J);
-            double c11 = J[3], c12 = -J[2], c21 = -J[1], c22 = J[0];
-            double anx = c11 * vn[ig][0] + c12 * vn[ig][1];
-            double any = c21 * vn[ig][0] + c22 * vn[ig][1];
-            double w1 = anx * 2.0, w2 = any * 2.0;
-            double farea = sqrt(w1 * w1 + w2 * w2);
-            gq[slot[ig]][0] = anx * 2.0 / farea;
-            gq[slot[ig]][1] = any * 2.0 / farea;
-            gq[slot[ig]][2] = farea;
+            const double c11 = J[3], c12 = -J[2], c21 = -J[1], c22 = J[0];
+            const double anx = c11 * vn[ig][0] + c12 * vn[ig][1];
+            const double any = c21 * vn[ig][0] + c22 * vn[ig][1];
+            const double w1 = anx * 2.0, w2 = any * 2.0;
+            const double farea = sqrt(w1 * w1 + w2 * w2);
+            gx = anx * 2.0 / farea;
+            gy = any * 2.0 / farea;
+            ga = farea;
         }
-        for (int j = 0; j < 16; ++j)
-            for (int k = 0; k < 3; ++k) d.gq[(3 * j + k) * nc + ie] = gq[j][k];
-    }
-    double u[4][6];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) u[q][id] = UD(d.Ul, q, id, ie);
-    const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
-    const double gg0 = d.G[18 * nc + ie], gg1 = d.G[19 * nc + ie], gg2 = d.G[20 * nc + ie];
-    // ---- sub-cell density: initial mass / current area
-    double rsub[4];
-    {
+        sgq[cl][lane][0] = gx;
+        sgq[cl][lane][1] = gy;
+        sgq[cl][lane][2] = ga;
+        d.gq[(3 * lane + 0) * nc + ie] = gx;
+        d.gq[(3 * lane + 1) * nc + ie] = gy;
+        d.gq[(3 * lane + 2) * nc + ie] = ga;
+        // ---- quadrature terms of sub-cell isg: area of its Q2 map and the
+        //      mass-weighted mean specific volume of the DG polynomial
+        double u0[6];
+        for (int id = 0; id < 6; ++id) u0[id] = UD(d.Ul, 0, id, ie);
         const double pc[12][2] = {{-0.5, -1.0}, {0.5, -1.0}, {1.0, -.5}, {1.0, .5}, {0.5, 1.0}, {-0.5, 1.0},
                                   {-1.0, .5},   {-1.0, -.5}, {0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
         const int ipsc[4][4] = {{0, 8, 11, 7}, {1, 2, 9, 8}, {9, 3, 4, 10}, {11, 10, 5, 6}};
-        double xc[12], yc[12];
-        for (int k = 0; k < 12; ++k) map9(xp, yp, pc[k][0], pc[k][1], &xc[k], &yc[k]);
-        for (int isg = 0; isg < 4; ++isg) {
-            double xs[9], ys[9];
-            for (int k = 0; k < 4; ++k) {
-                xs[k] = xp[IPQSG[isg][k]];
-                ys[k] = yp[IPQSG[isg][k]];
-                xs[4 + k] = xc[ipsc[isg][k]];
-                ys[4 + k] = yc[ipsc[isg][k]];
-            }
-            xs[8] = -0.25 * (xs[0] + xs[1] + xs[2] + xs[3]) + 0.5 * (xs[4] + xs[5] + xs[6] + xs[7]);
-            ys[8] = -0.25 * (ys[0] + ys[1] + ys[2] + ys[3]) + 0.5 * (ys[4] + ys[5] + ys[6] + ys[7]);
-            double a = 0.0;
-            for (int i = 0; i < 5; ++i)
-                for (int j = 0; j < 5; ++j) {
-                    double J[4];
-                    jac9(xs, ys, PG5[i], PG5[j], J);
-                    a = a + WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
-                }
-            rsub[isg] = 0.0 + (1.0 / a) * d.msub[isg * nc + ie];
+        double xs[9], ys[9];
+        for (int k = 0; k < 4; ++k) {
+            xs[k] = xp[IPQSG[isg][k]];
+            ys[k] = yp[IPQSG[isg][k]];
+            const int c = ipsc[isg][k];
+            map9(xp, yp, pc[c][0], pc[c][1], &xs[4 + k], &ys[4 + k]);
         }
-    }
-    // ---- sub-cell mean specific volume (aflim(1) = 1 on this path)
-    double uq[4];
-    {
+        xs[8] = -0.25 * (xs[0] + xs[1] + xs[2] + xs[3]) + 0.5 * (xs[4] + xs[5] + xs[6] + xs[7]);
+        ys[8] = -0.25 * (ys[0] + ys[1] + ys[2] + ys[3]) + 0.5 * (ys[4] + ys[5] + ys[6] + ys[7]);
         const double sr[4] = {-1.0, 1.0, 1.0, -1.0}, ss[4] = {-1.0, -1.0, 1.0, 1.0};
-        const double rhomc = u[0][0];
-        for (int isg = 0; isg < 4; ++isg) {
-            double masel = 0.0, volel = 0.0;
-            for (int i = 0; i < 5; ++i)
-                for (int j = 0; j < 5; ++j) {
-                    double r = (PG5[i] + sr[isg]) / 2.0;
-                    double s = (PG5[j] + ss[isg]) / 2.0;
-                    double b[6];
-                    basis6(r, s, rc, sc, gg0, gg1, gg2, b);
-                    double t = 0.0;
-                    for (int id = 0; id < 6; ++id) t = t + u[0][id] * b[id];
-                    t = rhomc + 1.0 * (t - rhomc);
-                    double J[4];
-                    jac9(xp, yp, r, s, J);
-                    double djaco = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]) * 0.25;
-                    masel = masel + 1.0 / t * djaco;
-                    volel = volel + djaco;
-                }
-            uq[isg] = volel / masel;
+        const double rhomc = u0[0];
+        for (int g = part; g < 25; g += 4) {
+            const int i = g / 5, j = g - 5 * i;
+            double J[4];
+            jac9(xs, ys, PG5[i], PG5[j], J);
+            sta[cl][isg][g] = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
+            const double r = (PG5[i] + sr[isg]) / 2.0;
+            const double s = (PG5[j] + ss[isg]) / 2.0;
+            double b[6];
+            basis6(r, s, rc, sc, gg0, gg1, gg2, b);
+            double t = 0.0;
+            for (int id = 0; id < 6; ++id) t = t + u0[id] * b[id];
+            t = rhomc + 1.0 * (t - rhomc);
+            jac9(xp, yp, r, s, J);
+            const double djaco = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]) * 0.25;
+            stm[cl][isg][g] = 1.0 / t * djaco;
+            stv[cl][isg][g] = djaco;
         }
     }
-    // ---- vertex states of the P2 polynomial at the 9 nodes
-    double un[9][4];
-    for (int iv = 0; iv < 9; ++iv) {
+    __syncthreads();
+    if (!active) return;
+    // ---- sub-cell corner (isg, iv)
+    const int iv = part;
+    double a = 0.0, masel = 0.0, volel = 0.0;
+    for (int g = 0; g < 25; ++g) {
+        a = a + sta[cl][isg][g];
+        masel = masel + stm[cl][isg][g];
+        volel = volel + stv[cl][isg][g];
+    }
+    const double rsub = 0.0 + (1.0 / a) * d.msub[isg * nc + ie];
+    const double uq = volel / masel;
+    double u[4][6];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 6; ++id) u[q][id] = UD(d.Ul, q, id, ie);
+    const int ln = IPQSG[isg][iv];
+    double un[4];
+    {
         double b[6];
-        basis6(XV9[iv], YV9[iv], rc, sc, gg0, gg1, gg2, b);
-        for (int q = 0; q < 4; ++q) un[iv][q] = 0.0;
+        basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, b);
+        for (int q = 0; q < 4; ++q) un[q] = 0.0;
         for (int id = 0; id < 6; ++id)
-            for (int q = 0; q < 4; ++q) un[iv][q] = un[iv][q] + u[q][id] * b[id];
+            for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
     }
     const double rhomc = u[0][0], uctr = u[1][0], vctr = u[2][0], ectr = u[3][0];
     const double rhoct = 1.0 / rhomc;
     const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
     const double sdctr = sqrt(dmax(eps, d.gamlg * pctr / rhoct));
-    for (int isg = 0; isg < 4; ++isg) {
-        const double dcor = d.cdrho * (rsub[isg] - 1.0 / uq[isg]);
-        for (int iv = 0; iv < 4; ++iv) {
-            const int ln = IPQSG[isg][iv];
-            const int p = d.ipq[ln * nc + ie];
-            const double rhovt = 1.0 / un[ln][0];
-            const double rhovsg = rhovt + dcor;
-            const double uv = un[ln][1], vv = un[ln][2], ev = un[ln][3];
-            const double pv = dmax(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)));
-            const double dux = d.usx[p] - uv, duy = d.usy[p] - vv;
-            const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
-            for (int f = 0; f < 2; ++f) {
-                int tt = FNQSG[isg][2 * iv + f];
-                int sl = (tt > 0 ? tt : -tt) - 1;
-                double sg = tt > 0 ? 1.0 : -1.0;
-                double nx = sg * gq[sl][0], ny = sg * gq[sl][1];
-                double area = gq[sl][2] * WFR[isg][iv][f];
-                double deltu = fabs(dux * nx + duy * ny);
-                double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
-                double m11;
-                if (a3 < 1.e12) {
-                    m11 = mur * area * fabs(1.0);
-                } else {
-                    double nrm = sqrt(dux * dux + duy * duy);
-                    m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
-                }
-                const int cf = f + 2 * (iv + 4 * isg);
-                d.rec[(0 * 32 + cf) * nc + ie] = m11;
-                d.rec[(1 * 32 + cf) * nc + ie] = m11 * uv + 0.0 * vv;
-                d.rec[(2 * 32 + cf) * nc + ie] = 0.0 * uv + m11 * vv;
-                d.rec[(3 * 32 + cf) * nc + ie] = -pv * area * nx + 0.0 * area * ny;
-                d.rec[(4 * 32 + cf) * nc + ie] = 0.0 * area * nx + -pv * area * ny;
-            }
+    const double dcor = d.cdrho * (rsub - 1.0 / uq);
+    const int p = d.ipq[ln * nc + ie];
+    const double rhovt = 1.0 / un[0];
+    const double rhovsg = rhovt + dcor;
+    const double uv = un[1], vv = un[2], ev = un[3];
+    const double pv = dmax(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)));
+    const double dux = d.usx[p] - uv, duy = d.usy[p] - vv;
+    const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+    for (int f = 0; f < 2; ++f) {
+        const int tt = FNQSG[isg][2 * iv + f];
+        const int sl = (tt > 0 ? tt : -tt) - 1;
+        const double sg = tt > 0 ? 1.0 : -1.0;
+        const double nx = sg * sgq[cl][sl][0], ny = sg * sgq[cl][sl][1];
+        const double area = sgq[cl][sl][2] * WFR[isg][iv][f];
+        const double deltu = fabs(dux * nx + duy * ny);
+        const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
+        double m11;
+        if (a3 < 1.e12) {
+            m11 = mur * area * fabs(1.0);
+        } else {
+            const double nrm = sqrt(dux * dux + duy * duy);
+            m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
         }
+        const int cf = f + 2 * (iv + 4 * isg);
+        d.rec[(0 * 32 + cf) * nc + ie] = m11;
+        d.rec[(1 * 32 + cf) * nc + ie] = m11 * uv + 0.0 * vv;
+        d.rec[(2 * 32 + cf) * nc + ie] = 0.0 * uv + m11 * vv;
+        d.rec[(3 * 32 + cf) * nc + ie] = -pv * area * nx + 0.0 * area * ny;
+        d.rec[(4 * 32 + cf) * nc + ie] = 0.0 * area * nx + -pv * area * ny;
     }
 }
 
@@ -945,131 +1034,163 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
 
 // Sub-cell face integral, Gauss-Lobatto domain integral, the (identity)
 // limiter update and, with apply_update, the mass-matrix solve and RK blend.
-__global__ void __launch_bounds__(LAG2D_BLOCK) k_cellrhs_q2(Dev d, double alfa, int apply_update)
+// 32 lanes per cell: lanes 0-15 stage the 16 sub-cell corners (nodal
+// velocity, normals, areas, corner forces), lanes 16-31 and 0-8 the 25
+// Lobatto points (Jacobian, velocity, pressure); then lane (q, id) sums its
+// residual entry over corners and points in the sequential order, and
+// applies the update.
+constexpr int RHS_CPB = 8;
+
+__global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int apply_update)
 {
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double scn[RHS_CPB][16][12];
+    __shared__ double spt[RHS_CPB][25][8];
+    __shared__ double sres[RHS_CPB][4][6];
+    const int cl = threadIdx.x >> 5, lane = threadIdx.x & 31;
+    const int ie = blockIdx.x * RHS_CPB + cl;
     const int nc = d.nc;
-    if (ie == 0) *d.tcount = 0;  // WENO list of this stage consumed; reset for the next
-    if (ie >= nc) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d.tcount = 0;  // WENO list of this stage consumed
+    const bool active = ie < nc;
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
-    double u[4][6];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) u[q][id] = UD(d.Ul, q, id, ie);
-    const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
-    const double gg0 = d.G[18 * nc + ie], gg1 = d.G[19 * nc + ie], gg2 = d.G[20 * nc + ie];
-    double r[4][6];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) r[q][id] = 0.0;
-    // ---- face integral
-    {
-        double bq[9][6];
-        for (int iv = 0; iv < 9; ++iv) basis6(XV9[iv], YV9[iv], rc, sc, gg0, gg1, gg2, bq[iv]);
-        for (int isg = 0; isg < 4; ++isg) {
-            for (int iv = 0; iv < 4; ++iv) {
-                const int ln = IPQSG[isg][iv];
-                const int p = d.ipq[ln * nc + ie];
-                const double us1 = d.usx[p], us2 = d.usy[p];
-                double nx[2], ny[2], ar[2], fx[2], fy[2];
-                for (int f = 0; f < 2; ++f) {
-                    int tt = FNQSG[isg][2 * iv + f];
-                    int sl = (tt > 0 ? tt : -tt) - 1;
-                    double sg = tt > 0 ? 1.0 : -1.0;
-                    nx[f] = sg * d.gq[(3 * sl + 0) * nc + ie];
-                    ny[f] = sg * d.gq[(3 * sl + 1) * nc + ie];
-                    ar[f] = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
-                    const int cf = f + 2 * (iv + 4 * isg);
-                    double m11 = d.rec[(0 * 32 + cf) * nc + ie];
-                    double m1 = d.rec[(1 * 32 + cf) * nc + ie], m2 = d.rec[(2 * 32 + cf) * nc + ie];
-                    double s1 = d.rec[(3 * 32 + cf) * nc + ie], s2 = d.rec[(4 * 32 + cf) * nc + ie];
-                    fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
-                    fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
-                }
-                const double w1 = WFM[isg][iv][0], w2 = WFM[isg][iv][1];
-                for (int id = 0; id < 6; ++id) {
-                    const double b = bq[ln][id];
-                    r[0][id] = r[0][id] + us1 * nx[0] * ar[0] * b + us2 * ny[0] * ar[0] * b + us1 * nx[1] * ar[1] * b +
-                               us2 * ny[1] * ar[1] * b;
-                    r[1][id] = r[1][id] + fx[0] * b * w1 + fx[1] * b * w2;
-                    r[2][id] = r[2][id] + fy[0] * b * w1 + fy[1] * b * w2;
-                    r[3][id] = r[3][id] + us1 * fx[0] * b * w1 + us2 * fy[0] * b * w1 + us1 * fx[1] * b * w2 +
-                               us2 * fy[1] * b * w2;
-                }
+    double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
+    if (active) {
+        rc = d.G[0 * nc + ie];
+        sc = d.G[1 * nc + ie];
+        gg0 = d.G[18 * nc + ie];
+        gg1 = d.G[19 * nc + ie];
+        gg2 = d.G[20 * nc + ie];
+        if (lane < 16) {
+            const int isg = lane >> 2, iv = lane & 3;
+            const int ln = IPQSG[isg][iv];
+            const int p = d.ipq[ln * nc + ie];
+            const double us1 = d.usx[p], us2 = d.usy[p];
+            double *c = scn[cl][lane];
+            c[0] = us1;
+            c[1] = us2;
+            for (int f = 0; f < 2; ++f) {
+                const int tt = FNQSG[isg][2 * iv + f];
+                const int sl = (tt > 0 ? tt : -tt) - 1;
+                const double sg = tt > 0 ? 1.0 : -1.0;
+                const double nx = sg * d.gq[(3 * sl + 0) * nc + ie];
+                const double ny = sg * d.gq[(3 * sl + 1) * nc + ie];
+                const double ar = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
+                const int cf = f + 2 * (iv + 4 * isg);
+                const double m11 = d.rec[(0 * 32 + cf) * nc + ie];
+                const double m1 = d.rec[(1 * 32 + cf) * nc + ie], m2 = d.rec[(2 * 32 + cf) * nc + ie];
+                const double s1 = d.rec[(3 * 32 + cf) * nc + ie], s2 = d.rec[(4 * 32 + cf) * nc + ie];
+                c[2 + 3 * f] = nx;
+                c[3 + 3 * f] = ny;
+                c[4 + 3 * f] = ar;
+                c[8 + 2 * f] = s1 + m11 * us1 + 0.0 * us2 - m1;
+                c[9 + 2 * f] = s2 + 0.0 * us1 + m11 * us2 - m2;
             }
         }
+        const int pt = lane >= 16 ? lane - 16 : (lane < 9 ? 16 + lane : -1);
+        if (pt >= 0) {
+            const int i = pt / 5, j = pt - 5 * i;
+            const double rr = PL5[i], ss = PL5[j], wi = WL5[i] * WL5[j];
+            double xp[9], yp[9];
+            load9(d, d.xs, d.ys, ie, xp, yp);
+            double J[4];
+            jac9(xp, yp, rr, ss, J);
+            double b[6];
+            basis6(rr, ss, rc, sc, gg0, gg1, gg2, b);
+            double un[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int id = 0; id < 6; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + UD(d.Ul, q, id, ie) * b[id];
+            const double rhoad = 1.0 / un[0];
+            const double uu = un[1], vv = un[2];
+            const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
+            double *s = spt[cl][pt];
+            s[0] = J[0];
+            s[1] = J[1];
+            s[2] = J[2];
+            s[3] = J[3];
+            s[4] = uu;
+            s[5] = vv;
+            s[6] = pres;
+            s[7] = wi;
+        }
     }
-    // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
-    {
-        double xp[9], yp[9];
-        load9(d, d.xs, d.ys, ie, xp, yp);
-        for (int i = 0; i < 5; ++i)
-            for (int j = 0; j < 5; ++j) {
-                const double rr = PL5[i], ss = PL5[j], wi = WL5[i] * WL5[j];
-                double J[4];
-                jac9(xp, yp, rr, ss, J);
-                const double dxdr = J[0], dxds = J[1], dydr = J[2], dyds = J[3];
-                const double dbdr[6] = {0.0, 1.0 / 1.0, 0.0, (rr - rc) / 1.0, 0.0, (ss - sc) / 1.0 / 1.0};
-                const double dbds[6] = {0.0, 0.0, 1.0 / 1.0, 0.0, (ss - sc) / 1.0, (rr - rc) / 1.0 / 1.0};
-                double b[6];
-                basis6(rr, ss, rc, sc, gg0, gg1, gg2, b);
-                double un[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int id = 0; id < 6; ++id)
-                    for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
-                const double rhoad = 1.0 / un[0];
-                const double uu = un[1], vv = un[2];
-                const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
-                for (int id = 0; id < 6; ++id) {
-                    const double a = dyds * dbdr[id] + (-dydr) * dbds[id];
-                    const double c = (-dxds) * dbdr[id] + dxdr * dbds[id];
-                    const double f0 = a * uu + c * vv;
-                    r[0][id] = r[0][id] - f0 * wi;
-                    r[1][id] = r[1][id] - a * (-pres) * wi;
-                    r[2][id] = r[2][id] - c * (-pres) * wi;
-                    r[3][id] = r[3][id] - (a * uu + c * vv) * (-pres) * wi;
-                }
+    __syncthreads();
+    const int q = lane / 6, id = lane - 6 * (lane / 6);
+    if (active && lane < 24) {
+        double r = 0.0;
+        // ---- face integral, corners in (isg, iv) order
+        for (int k = 0; k < 16; ++k) {
+            const int isg = k >> 2, iv = k & 3;
+            const double *c = scn[cl][k];
+            double bb[6];
+            const int ln = IPQSG[isg][iv];
+            basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, bb);
+            const double b = bb[id];
+            const double us1 = c[0], us2 = c[1];
+            const double nx0 = c[2], ny0 = c[3], ar0 = c[4], nx1 = c[5], ny1 = c[6], ar1 = c[7];
+            const double fx0 = c[8], fy0 = c[9], fx1 = c[10], fy1 = c[11];
+            const double w1 = WFM[isg][iv][0], w2 = WFM[isg][iv][1];
+            if (q == 0)
+                r = r + us1 * nx0 * ar0 * b + us2 * ny0 * ar0 * b + us1 * nx1 * ar1 * b + us2 * ny1 * ar1 * b;
+            else if (q == 1)
+                r = r + fx0 * b * w1 + fx1 * b * w2;
+            else if (q == 2)
+                r = r + fy0 * b * w1 + fy1 * b * w2;
+            else
+                r = r + us1 * fx0 * b * w1 + us2 * fy0 * b * w1 + us1 * fx1 * b * w2 + us2 * fy1 * b * w2;
+        }
+        // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
+        for (int k = 0; k < 25; ++k) {
+            const double *s = spt[cl][k];
+            const int i = k / 5, j = k - 5 * i;
+            const double rr = PL5[i], ss = PL5[j];
+            const double dxdr = s[0], dxds = s[1], dydr = s[2], dyds = s[3];
+            const double uu = s[4], vv = s[5], pres = s[6], wi = s[7];
+            const double dbdr[6] = {0.0, 1.0 / 1.0, 0.0, (rr - rc) / 1.0, 0.0, (ss - sc) / 1.0 / 1.0};
+            const double dbds[6] = {0.0, 0.0, 1.0 / 1.0, 0.0, (ss - sc) / 1.0, (rr - rc) / 1.0 / 1.0};
+            const double a = dyds * dbdr[id] + (-dydr) * dbds[id];
+            const double c = (-dxds) * dbdr[id] + dxdr * dbds[id];
+            if (q == 0) {
+                const double f0 = a * uu + c * vv;
+                r = r - f0 * wi;
+            } else if (q == 1) {
+                r = r - a * (-pres) * wi;
+            } else if (q == 2) {
+                r = r - c * (-pres) * wi;
+            } else {
+                r = r - (a * uu + c * vv) * (-pres) * wi;
             }
+        }
+        sres[cl][q][id] = r;
     }
+    __syncthreads();
+    if (!active || lane >= 24) return;
     // ---- derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708)
-    double ul[4][6];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) ul[q][id] = u[q][id];
-    ul[0][1] = u[0][1] * 1.0;
-    ul[0][2] = u[0][2] * 1.0;
-    ul[1][1] = 1.0 * u[1][1] + 0.0 * u[2][1];
-    ul[1][2] = 1.0 * u[1][2] + 0.0 * u[2][2];
-    ul[2][1] = 0.0 * u[1][1] + 1.0 * u[2][1];
-    ul[2][2] = 0.0 * u[1][2] + 1.0 * u[2][2];
-    ul[3][1] = u[3][1] * 1.0;
-    ul[3][2] = u[3][2] * 1.0;
+    const double uqi = UD(d.Ul, q, id, ie);
+    double ul = uqi;
+    if (id == 1 || id == 2) {
+        if (q == 0 || q == 3) ul = uqi * 1.0;
+        else if (q == 1) ul = 1.0 * uqi + 0.0 * UD(d.Ul, 2, id, ie);
+        else ul = 0.0 * UD(d.Ul, 1, id, ie) + 1.0 * uqi;
+    }
     if (apply_update) {
         const double dt = d.scal[0];
-        double m[6][6];
-        for (int a = 0; a < 6; ++a)
-            for (int b = 0; b < 6; ++b) m[a][b] = 0.0;
-        m[0][0] = d.A[15 * nc + ie];
-        int k = 0;
-        for (int a = 1; a < 6; ++a)
-            for (int b = a; b < 6; ++b) {
-                double v = d.A[k * nc + ie];
-                m[a][b] = v;
-                m[b][a] = v;
-                ++k;
+        // row id of the unpacked inverse mass matrix (degree 0 decoupled;
+        // upper triangle of the 5x5 block packed row-wise, then 1/mass)
+        const int off[6] = {0, 0, 5, 9, 12, 14};
+        double mrow[6];
+        for (int kk = 0; kk < 6; ++kk) {
+            if (id == 0 || kk == 0) {
+                mrow[kk] = (id == 0 && kk == 0) ? d.A[15 * nc + ie] : 0.0;
+            } else {
+                const int a = id < kk ? id : kk, b = id < kk ? kk : id;
+                mrow[kk] = d.A[(off[a] + (b - a)) * nc + ie];
             }
-        double rs[4][6];
-        for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < 6; ++id) rs[q][id] = dt * r[q][id];
-        for (int id = 0; id < 6; ++id) {
-            double s4[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int kk = 0; kk < 6; ++kk)
-                for (int q = 0; q < 4; ++q) s4[q] = s4[q] + m[id][kk] * rs[q][kk];
-            for (int q = 0; q < 4; ++q)
-                UD(d.Us, q, id, ie) = alfa * UD(d.Un, q, id, ie) + (1.0 - alfa) * (ul[q][id] + s4[q]);
         }
+        double s4 = 0.0;
+        for (int kk = 0; kk < 6; ++kk) s4 = s4 + mrow[kk] * (dt * sres[cl][q][kk]);
+        UD(d.Us, q, id, ie) = alfa * UD(d.Un, q, id, ie) + (1.0 - alfa) * (ul + s4);
     } else {
-        for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < 6; ++id) {
-                UD(d.Us, q, id, ie) = ul[q][id];
-                UD(d.R, q, id, ie) = r[q][id];
-            }
+        UD(d.Us, q, id, ie) = ul;
+        UD(d.R, q, id, ie) = sres[cl][q][id];
     }
 }
 
@@ -1115,56 +1236,10 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_submass(Dev d)
 #undef UD
 
 // ---- host-side table set-up ------------------------------------------
-// Enumeration of the Leibniz terms in Matinv225's order: the column of the
-// last row ascends slowest, then the row before, ...; the first row takes
-// the remaining column.
-static void perm_table(int m, std::vector<std::vector<signed char>> &out)
-{
-    out.clear();
-    int p[5];
-    bool used[5] = {false, false, false, false, false};
-    // recursive lambda over levels m-1 .. 1
-    struct Rec {
-        static void go(int level, int m, int *p, bool *used, std::vector<std::vector<signed char>> &out)
-        {
-            if (level == 0) {
-                int j = 0;
-                while (used[j]) ++j;
-                p[0] = j;
-                int inv = 0;
-                for (int x = 0; x < m; ++x)
-                    for (int y = x + 1; y < m; ++y)
-                        if (p[x] > p[y]) ++inv;
-                std::vector<signed char> v(m + 1);
-                for (int k = 0; k < m; ++k) v[k] = (signed char)p[k];
-                v[m] = (inv & 1) ? -1 : 1;
-                out.push_back(v);
-                return;
-            }
-            for (int j = 0; j < m; ++j) {
-                if (used[j]) continue;
-                used[j] = true;
-                p[level] = j;
-                go(level - 1, m, p, used, out);
-                used[j] = false;
-            }
-        }
-    };
-    Rec::go(m - 1, m, p, used, out);
-}
-
 static int q2_tables()
 {
     static bool done = false;
     if (done) return 0;
-    std::vector<std::vector<signed char>> t5, t4;
-    perm_table(5, t5);
-    perm_table(4, t4);
-    signed char p5[120][6], p4[24][5];
-    for (int i = 0; i < 120; ++i)
-        for (int k = 0; k < 6; ++k) p5[i][k] = t5[i][k];
-    for (int i = 0; i < 24; ++i)
-        for (int k = 0; k < 5; ++k) p4[i][k] = t4[i][k];
     const double rt[4][4][2] = {
         {{4.0 / 6.0, 4.0 / 6.0}, {4.0 / 3.0, 8.0 / 3.0}, {1.0, 1.0}, {8.0 / 3.0, 4.0 / 3.0}},
         {{8.0 / 3.0, 4.0 / 3.0}, {4.0 / 6.0, 4.0 / 6.0}, {4.0 / 3.0, 8.0 / 3.0}, {1.0, 1.0}},
@@ -1187,8 +1262,6 @@ static int q2_tables()
                 wfm[a][b][f] = mt[a][b][f];
                 wfv[a][b][f] = 0.25 * vt[a][b][f];
             }
-    if (hipMemcpyToSymbol(HIP_SYMBOL(PERM5), p5, sizeof(p5)) != hipSuccess) return LAG2D_EDEVICE;
-    if (hipMemcpyToSymbol(HIP_SYMBOL(PERM4), p4, sizeof(p4)) != hipSuccess) return LAG2D_EDEVICE;
     if (hipMemcpyToSymbol(HIP_SYMBOL(WFR), wfr, sizeof(wfr)) != hipSuccess) return LAG2D_EDEVICE;
     if (hipMemcpyToSymbol(HIP_SYMBOL(WFM), wfm, sizeof(wfm)) != hipSuccess) return LAG2D_EDEVICE;
     if (hipMemcpyToSymbol(HIP_SYMBOL(WFV), wfv, sizeof(wfv)) != hipSuccess) return LAG2D_EDEVICE;
@@ -1221,13 +1294,13 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
     hipLaunchKernelGGL(k_q2_weno, dim3(nc < WENO_GRID ? nc : WENO_GRID), dim3(WENO_WG), 0, x->stream, ds, Uin);
     x->toc(KID_Q2_WENO);
     x->tic(KID_Q2_CORNER);
-    hipLaunchKernelGGL(k_q2_corner, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+    hipLaunchKernelGGL(k_q2_corner, dim3((nc + CORNER_CPB - 1) / CORNER_CPB), dim3(256), 0, x->stream, ds);
     x->toc(KID_Q2_CORNER);
     x->tic(KID_NODE_Q2);
     hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
     x->toc(KID_NODE_Q2);
     x->tic(KID_CELLRHS_Q2);
-    hipLaunchKernelGGL(k_cellrhs_q2, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, alfa, apply_update);
+    hipLaunchKernelGGL(k_cellrhs_q2, dim3((nc + RHS_CPB - 1) / RHS_CPB), dim3(256), 0, x->stream, ds, alfa, apply_update);
     x->toc(KID_CELLRHS_Q2);
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
 }
