@@ -1034,11 +1034,11 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
 
 // Sub-cell face integral, Gauss-Lobatto domain integral, the (identity)
 // limiter update and, with apply_update, the mass-matrix solve and RK blend.
-// 32 lanes per cell: lanes 0-15 stage the 16 sub-cell corners (nodal
-// velocity, normals, areas, corner forces), lanes 16-31 and 0-8 the 25
-// Lobatto points (Jacobian, velocity, pressure); then lane (q, id) sums its
-// residual entry over corners and points in the sequential order, and
-// applies the update.
+// 8 cells per 256-thread block.  Staging: 128 corner tasks (nodal velocity,
+// normals, areas, corner forces of the 16 sub-cell corners) and 200 point
+// tasks (Jacobian, velocity, pressure at the 25 Lobatto points).  Sums: wave
+// q owns variable q of all 8 cells, lane (cell, id) sums its residual entry
+// over corners and points in the sequential order, then applies the update.
 constexpr int RHS_CPB = 8;
 
 __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int apply_update)
@@ -1046,25 +1046,20 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
     __shared__ double scn[RHS_CPB][16][12];
     __shared__ double spt[RHS_CPB][25][8];
     __shared__ double sres[RHS_CPB][4][6];
-    const int cl = threadIdx.x >> 5, lane = threadIdx.x & 31;
-    const int ie = blockIdx.x * RHS_CPB + cl;
+    const int tid = threadIdx.x;
     const int nc = d.nc;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *d.tcount = 0;  // WENO list of this stage consumed
-    const bool active = ie < nc;
+    const int ie0 = blockIdx.x * RHS_CPB;
+    if (blockIdx.x == 0 && tid == 0) *d.tcount = 0;  // WENO list of this stage consumed
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
-    double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
-    if (active) {
-        rc = d.G[0 * nc + ie];
-        sc = d.G[1 * nc + ie];
-        gg0 = d.G[18 * nc + ie];
-        gg1 = d.G[19 * nc + ie];
-        gg2 = d.G[20 * nc + ie];
-        if (lane < 16) {
-            const int isg = lane >> 2, iv = lane & 3;
+    // corners on threads 0-127, points on threads 128-255 and 0-71
+    {
+        const int cl = tid >> 4, corner = tid & 15, ie = ie0 + cl;
+        if (tid < RHS_CPB * 16 && ie < nc) {
+            const int isg = corner >> 2, iv = corner & 3;
             const int ln = IPQSG[isg][iv];
             const int p = d.ipq[ln * nc + ie];
             const double us1 = d.usx[p], us2 = d.usy[p];
-            double *c = scn[cl][lane];
+            double *c = scn[cl][corner];
             c[0] = us1;
             c[1] = us2;
             for (int f = 0; f < 2; ++f) {
@@ -1085,38 +1080,49 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
                 c[9 + 2 * f] = s2 + 0.0 * us1 + m11 * us2 - m2;
             }
         }
-        const int pt = lane >= 16 ? lane - 16 : (lane < 9 ? 16 + lane : -1);
-        if (pt >= 0) {
-            const int i = pt / 5, j = pt - 5 * i;
-            const double rr = PL5[i], ss = PL5[j], wi = WL5[i] * WL5[j];
-            double xp[9], yp[9];
-            load9(d, d.xs, d.ys, ie, xp, yp);
-            double J[4];
-            jac9(xp, yp, rr, ss, J);
-            double b[6];
-            basis6(rr, ss, rc, sc, gg0, gg1, gg2, b);
-            double un[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int id = 0; id < 6; ++id)
-                for (int q = 0; q < 4; ++q) un[q] = un[q] + UD(d.Ul, q, id, ie) * b[id];
-            const double rhoad = 1.0 / un[0];
-            const double uu = un[1], vv = un[2];
-            const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
-            double *s = spt[cl][pt];
-            s[0] = J[0];
-            s[1] = J[1];
-            s[2] = J[2];
-            s[3] = J[3];
-            s[4] = uu;
-            s[5] = vv;
-            s[6] = pres;
-            s[7] = wi;
-        }
+    }
+    for (int u = (tid + 256 - RHS_CPB * 16) & 255; u < RHS_CPB * 25; u += 256) {
+        const int cl = u / 25, pt = u - 25 * cl, ie = ie0 + cl;
+        if (ie >= nc) continue;
+        const int i = pt / 5, j = pt - 5 * i;
+        const double rr = PL5[i], ss = PL5[j], wi = WL5[i] * WL5[j];
+        double xp[9], yp[9];
+        load9(d, d.xs, d.ys, ie, xp, yp);
+        double J[4];
+        jac9(xp, yp, rr, ss, J);
+        double b[6];
+        basis6(rr, ss, d.G[0 * nc + ie], d.G[1 * nc + ie], d.G[18 * nc + ie], d.G[19 * nc + ie],
+               d.G[20 * nc + ie], b);
+        double un[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int id = 0; id < 6; ++id)
+            for (int q = 0; q < 4; ++q) un[q] = un[q] + UD(d.Ul, q, id, ie) * b[id];
+        const double rhoad = 1.0 / un[0];
+        const double uu = un[1], vv = un[2];
+        const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
+        double *s = spt[cl][pt];
+        s[0] = J[0];
+        s[1] = J[1];
+        s[2] = J[2];
+        s[3] = J[3];
+        s[4] = uu;
+        s[5] = vv;
+        s[6] = pres;
+        s[7] = wi;
     }
     __syncthreads();
-    const int q = lane / 6, id = lane - 6 * (lane / 6);
-    if (active && lane < 24) {
+    const int q = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+    const int l = tid & 63, cl = l / 6, id = l - 6 * cl, ie = ie0 + cl;
+    const bool active = l < 6 * RHS_CPB && ie < nc;
+    double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
+    if (active) {
+        rc = d.G[0 * nc + ie];
+        sc = d.G[1 * nc + ie];
+        gg0 = d.G[18 * nc + ie];
+        gg1 = d.G[19 * nc + ie];
+        gg2 = d.G[20 * nc + ie];
         double r = 0.0;
         // ---- face integral, corners in (isg, iv) order
+#pragma unroll 2
         for (int k = 0; k < 16; ++k) {
             const int isg = k >> 2, iv = k & 3;
             const double *c = scn[cl][k];
@@ -1138,6 +1144,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
                 r = r + us1 * fx0 * b * w1 + us2 * fy0 * b * w1 + us1 * fx1 * b * w2 + us2 * fy1 * b * w2;
         }
         // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
+#pragma unroll 1
         for (int k = 0; k < 25; ++k) {
             const double *s = spt[cl][k];
             const int i = k / 5, j = k - 5 * i;
@@ -1162,7 +1169,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         sres[cl][q][id] = r;
     }
     __syncthreads();
-    if (!active || lane >= 24) return;
+    if (!active) return;
     // ---- derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708)
     const double uqi = UD(d.Ul, q, id, ie);
     double ul = uqi;
