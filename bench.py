@@ -35,6 +35,8 @@ def main():
     ap.add_argument("--nx", type=int, default=120)
     ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="finish with one device-to-device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the Lagrangian step runs on a single MI355X "
@@ -114,6 +116,14 @@ def main():
         "kernels_ms": {k: {"total": v[0], "calls": v[1]} for k, v in stats.items()},
     }
     print(json.dumps(line))
+    if args.copy_calib:
+        # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE readings
+        # calibrate the per-dispatch counters of the kernels above
+        n = args.copy_calib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

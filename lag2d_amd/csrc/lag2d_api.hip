@@ -262,7 +262,7 @@ extern "C" int lag2d_upload(lag2d_ctx *x, const double *coord, const double *coo
     for (int ie = 0; ie < nc; ++ie)
         for (int k = 0; k < nm; ++k) h[(size_t)k * nc + ie] = amatr[k + (size_t)nm * ie];
     HIPCHK(up(x->A, (size_t)nm * nc));
-    hipLaunchKernelGGL(k_cellvol, dim3(nblocks(nc)), dim3(LAG2D_BLOCK), 0, x->stream, d, d.xn, d.yn);
+    hipLaunchKernelGGL(k_cellvol, dim3((nc + CV_CELLS - 1) / CV_CELLS), dim3(LAG2D_BLOCK), 0, x->stream, d, d.xn, d.yn, 0);
     HIPCHK(hipGetLastError());
     int rc = upload_extra(*x);
     if (rc) return rc;
@@ -297,7 +297,7 @@ void lag2d_ctx::toc(int k)
 
 const char *const lag2d_kernel_names[KID_COUNT] = {
     "k_cellavg_q1", "k_nodeminmax", "k_limit_corner_q1", "k_node_q1", "k_cellrhs_q1", "k_dt_partial",
-    "k_dt_final",   "k_update",     "k_nodemove",        "k_center_q2", "k_cellvol",
+    "k_dt_final",   "k_update",     "k_nodemove",        "k_cellvol",
     "k_q2_prep",    "k_q2_weno",    "k_q2_corner",       "k_node_q2", "k_cellrhs_q2"};
 
 static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
@@ -339,13 +339,14 @@ static int step_impl(lag2d_ctx *x, int itime)
         if (s == 0) {
             LAUNCH(KID_DT_PARTIAL, k_dt_partial, x->nblk_dt, ds, Uin, x->nblk_dt);
             LAUNCH(KID_DT_FINAL, k_dt_final, 1, d, itime, x->nblk_dt);
-            LAUNCH(KID_UPDATE, k_update, nblocks(nc), d, alfa);
+            LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
         }
         LAUNCH(KID_NODEMOVE, k_nodemove, nblocks(np), d, alfa, s == 0 ? d.xn : d.xs, s == 0 ? d.yn : d.ys);
-        if (x->cfg.ncurv == 1) LAUNCH(KID_CENTER_Q2, k_center_q2, nblocks(nc), d);
-        if (x->cfg.ncurv == 1 && s + 1 < ns) LAUNCH(KID_CELLVOL, k_cellvol, nblocks(nc), d, d.xs, d.ys);
+        if (x->cfg.ncurv == 1 && s + 1 < ns)
+            LAUNCH(KID_CELLVOL, k_cellvol, (nc + CV_CELLS - 1) / CV_CELLS, d, d.xs, d.ys, 1);
     }
-    LAUNCH(KID_CELLVOL, k_cellvol, nblocks(nc), d, d.xs, d.ys);
+    // final volumes; curved cells reset their centre node here for the last stage
+    LAUNCH(KID_CELLVOL, k_cellvol, (nc + CV_CELLS - 1) / CV_CELLS, d, d.xs, d.ys, x->cfg.ncurv == 1);
     HIPCHK(hipGetLastError());
     std::swap(d.Un, d.Us);
     std::swap(d.xn, d.xs);

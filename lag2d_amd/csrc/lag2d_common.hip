@@ -1,8 +1,8 @@
 // Kernels shared by the straight and curved paths: time-step reduction
 // (getdelt_lag, Getrhslag.f90:20600-20713), mass-matrix solve with the
-// SSP-RK blend (Getrhslag.f90:1804-1904), node motion and the Q2 centre-node
-// reset (:1954-1971), and the cell volume / volume centre (getcellvol_lag,
-// Limiterlag.f90:1999-2162).
+// SSP-RK blend (Getrhslag.f90:1804-1904), node motion (:1954-1956), and the
+// cell volume / volume centre (getcellvol_lag, Limiterlag.f90:1999-2162)
+// fused with the Q2 centre-node reset (Getrhslag.f90:1962-1971).
 #include "lag2d_dev.h"
 #include "lag2d_kernels.h"
 
@@ -93,44 +93,42 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_dt_final(Dev d, int itime, int 
     }
 }
 
-// Mass-matrix solve and SSP-RK blend of one cell (Getrhslag.f90:1822-1904):
-// U_s = alfa U^n + (1 - alfa)(U_s + M^-1 dt R).
+// Mass-matrix solve and SSP-RK blend (Getrhslag.f90:1822-1904):
+// U_s = alfa U^n + (1 - alfa)(U_s + M^-1 dt R), one thread per DOF (q, id)
+// of a cell, cells fastest so that every access is coalesced.
 __global__ void __launch_bounds__(LAG2D_BLOCK) k_update(Dev d, double alfa)
 {
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc, nd = d.nd;
-    if (ie >= nc) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * nd * nc) return;
+    const int ie = t % nc, qi = t / nc, q = qi / nd, id = qi - nd * q;
     const double dt = d.scal[0];
-    double m[6][6];
+    // row id of the inverse mass matrix: degree 0 decoupled (1/mass), the
+    // rest packed as the upper triangle row-wise
+    double mrow[6];
     if (nd == 3) {
-        m[0][0] = d.A[3 * nc + ie]; m[0][1] = 0.0; m[0][2] = 0.0;
-        m[1][0] = 0.0; m[1][1] = d.A[0 * nc + ie]; m[1][2] = d.A[1 * nc + ie];
-        m[2][0] = 0.0; m[2][1] = d.A[1 * nc + ie]; m[2][2] = d.A[2 * nc + ie];
+        if (id == 0) {
+            mrow[0] = d.A[3 * nc + ie]; mrow[1] = 0.0; mrow[2] = 0.0;
+        } else if (id == 1) {
+            mrow[0] = 0.0; mrow[1] = d.A[0 * nc + ie]; mrow[2] = d.A[1 * nc + ie];
+        } else {
+            mrow[0] = 0.0; mrow[1] = d.A[1 * nc + ie]; mrow[2] = d.A[2 * nc + ie];
+        }
     } else {
-        for (int i = 0; i < 6; ++i)
-            for (int j = 0; j < 6; ++j) m[i][j] = 0.0;
-        m[0][0] = d.A[15 * nc + ie];
-        int k = 0;
-        for (int i = 1; i < 6; ++i)
-            for (int j = i; j < 6; ++j) {
-                double a = d.A[k * nc + ie];
-                m[i][j] = a;
-                m[j][i] = a;
-                ++k;
+        const int off[6] = {0, 0, 5, 9, 12, 14};
+        for (int k = 0; k < 6; ++k) {
+            if (id == 0 || k == 0) {
+                mrow[k] = (id == 0 && k == 0) ? d.A[15 * nc + ie] : 0.0;
+            } else {
+                const int a = id < k ? id : k, b = id < k ? k : id;
+                mrow[k] = d.A[(off[a] + (b - a)) * nc + ie];
             }
-    }
-    double rs[4][6];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < nd; ++id) rs[q][id] = dt * d.R[(q * nd + id) * nc + ie];
-    for (int id = 0; id < nd; ++id) {
-        double s4[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int k = 0; k < nd; ++k)
-            for (int q = 0; q < 4; ++q) s4[q] = s4[q] + m[id][k] * rs[q][k];
-        for (int q = 0; q < 4; ++q) {
-            const int o = (q * nd + id) * nc + ie;
-            d.Us[o] = alfa * d.Un[o] + (1.0 - alfa) * (d.Us[o] + s4[q]);
         }
     }
+    double s4 = 0.0;
+    for (int k = 0; k < nd; ++k) s4 = s4 + mrow[k] * (dt * d.R[(q * nd + k) * nc + ie]);
+    const int o = (q * nd + id) * nc + ie;
+    d.Us[o] = alfa * d.Un[o] + (1.0 - alfa) * (d.Us[o] + s4);
 }
 
 // x_s = alfa x^n + (1 - alfa)(x_src + dt u*)  (Getrhslag.f90:1954-1956)
@@ -144,22 +142,6 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_nodemove(Dev d, double alfa, co
     double y = alfa * d.yn[p] + (1.0 - alfa) * (ysrc[p] + dt * d.usy[p]);
     d.xs[p] = x;
     d.ys[p] = y;
-}
-
-// Q2 centre node := -1/4 sum(corners) + 1/2 sum(mid-nodes) (Getrhslag.f90:1962-1971)
-__global__ void __launch_bounds__(LAG2D_BLOCK) k_center_q2(Dev d)
-{
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nc = d.nc;
-    if (ie >= nc) return;
-    int ip[9];
-    for (int k = 0; k < 9; ++k) ip[k] = d.ipq[k * nc + ie];
-    double cx = d.xs[ip[0]] + d.xs[ip[1]] + d.xs[ip[2]] + d.xs[ip[3]];
-    double mx = d.xs[ip[4]] + d.xs[ip[5]] + d.xs[ip[6]] + d.xs[ip[7]];
-    double cy = d.ys[ip[0]] + d.ys[ip[1]] + d.ys[ip[2]] + d.ys[ip[3]];
-    double my = d.ys[ip[4]] + d.ys[ip[5]] + d.ys[ip[6]] + d.ys[ip[7]];
-    d.xs[ip[8]] = -0.25 * cx + 0.5 * mx;
-    d.ys[ip[8]] = -0.25 * cy + 0.5 * my;
 }
 
 __device__ __forceinline__ void shape_q2_d(double r, double s, double *sh, double *dr_, double *ds_)
@@ -202,35 +184,55 @@ __constant__ static const double PG6[6] = {0.2386191860831969086305017, -0.23861
 __constant__ static const double WG6[6] = {0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
                                            0.3607615730481386, 0.1713244923791704, 0.1713244923791704};
 
-__global__ void __launch_bounds__(LAG2D_BLOCK) k_cellvol(Dev d, const double *__restrict__ x,
-                                                         const double *__restrict__ y)
+// Cell volume and volume centre on the 6x6 Gauss rule, 8 lanes per cell:
+// the lanes evaluate the Jacobians, lane 0 sums them in point order.  With
+// set_centre (curved cells) the Q2 centre node is first reset to
+// -1/4 sum(corners) + 1/2 sum(mid-nodes) (Getrhslag.f90:1962-1971); it
+// belongs to this cell only, so the reset and the volume fuse.
+__global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y,
+                                                 int set_centre)
 {
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sdj[CV_CELLS][36 + 1];
+    const int cl = threadIdx.x >> 3, lane = threadIdx.x & 7;
+    const int ie = blockIdx.x * CV_CELLS + cl;
     const int nc = d.nc;
-    if (ie >= nc) return;
-    double xp[9], yp[9];
-    if (d.ncurv == 0) {
-        for (int k = 0; k < 4; ++k) {
-            int p = d.ipq[k * nc + ie];
-            xp[k] = x[p];
-            yp[k] = y[p];
+    if (ie < nc) {
+        double xp[9], yp[9];
+        if (d.ncurv == 0) {
+            for (int k = 0; k < 4; ++k) {
+                int p = d.ipq[k * nc + ie];
+                xp[k] = x[p];
+                yp[k] = y[p];
+            }
+            xp[4] = 0.5 * (xp[0] + xp[1]); yp[4] = 0.5 * (yp[0] + yp[1]);
+            xp[5] = 0.5 * (xp[1] + xp[2]); yp[5] = 0.5 * (yp[1] + yp[2]);
+            xp[6] = 0.5 * (xp[2] + xp[3]); yp[6] = 0.5 * (yp[2] + yp[3]);
+            xp[7] = 0.5 * (xp[3] + xp[0]); yp[7] = 0.5 * (yp[3] + yp[0]);
+            xp[8] = 0.5 * (xp[4] + xp[6]); yp[8] = 0.5 * (yp[4] + yp[6]);
+        } else {
+            for (int k = 0; k < 8; ++k) {
+                int p = d.ipq[k * nc + ie];
+                xp[k] = x[p];
+                yp[k] = y[p];
+            }
+            const int pc = d.ipq[8 * nc + ie];
+            if (set_centre) {
+                const double cx = xp[0] + xp[1] + xp[2] + xp[3], mx = xp[4] + xp[5] + xp[6] + xp[7];
+                const double cy = yp[0] + yp[1] + yp[2] + yp[3], my = yp[4] + yp[5] + yp[6] + yp[7];
+                xp[8] = -0.25 * cx + 0.5 * mx;
+                yp[8] = -0.25 * cy + 0.5 * my;
+                if (lane == 0) {
+                    x[pc] = xp[8];
+                    y[pc] = yp[8];
+                }
+            } else {
+                xp[8] = x[pc];
+                yp[8] = y[pc];
+            }
         }
-        xp[4] = 0.5 * (xp[0] + xp[1]); yp[4] = 0.5 * (yp[0] + yp[1]);
-        xp[5] = 0.5 * (xp[1] + xp[2]); yp[5] = 0.5 * (yp[1] + yp[2]);
-        xp[6] = 0.5 * (xp[2] + xp[3]); yp[6] = 0.5 * (yp[2] + yp[3]);
-        xp[7] = 0.5 * (xp[3] + xp[0]); yp[7] = 0.5 * (yp[3] + yp[0]);
-        xp[8] = 0.5 * (xp[4] + xp[6]); yp[8] = 0.5 * (yp[4] + yp[6]);
-    } else {
-        for (int k = 0; k < 9; ++k) {
-            int p = d.ipq[k * nc + ie];
-            xp[k] = x[p];
-            yp[k] = y[p];
-        }
-    }
-    double rcv = 0.0, scv = 0.0, vol = 0.0;
-    for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 6; ++j) {
-            double r = PG6[i], s = PG6[j], wi = WG6[i] * WG6[j];
+        for (int g = lane; g < 36; g += 8) {
+            const int i = g / 6, j = g - 6 * i;
+            const double r = PG6[i], s = PG6[j], wi = WG6[i] * WG6[j];
             double sh[9], dr_[9], ds_[9];
             shape_q2_d(r, s, sh, dr_, ds_);
             double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
@@ -240,12 +242,20 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_cellvol(Dev d, const double *__
                 dydr = dydr + dr_[k] * yp[k];
                 dyds = dyds + ds_[k] * yp[k];
             }
-            // planar: rcoef = 1 - alfrz + alfrz*y = 1
-            double djaco = wi * (dxdr * dyds - dydr * dxds);
-            vol = vol + djaco * 1.0;
-            rcv = rcv + r * djaco * 1.0;
-            scv = scv + s * djaco * 1.0;
+            sdj[cl][g] = wi * (dxdr * dyds - dydr * dxds);
         }
+    }
+    __syncthreads();
+    if (ie >= nc || lane != 0) return;
+    // planar: rcoef = 1 - alfrz + alfrz*y = 1
+    double rcv = 0.0, scv = 0.0, vol = 0.0;
+    for (int g = 0; g < 36; ++g) {
+        const int i = g / 6, j = g - 6 * i;
+        const double r = PG6[i], s = PG6[j], djaco = sdj[cl][g];
+        vol = vol + djaco * 1.0;
+        rcv = rcv + r * djaco * 1.0;
+        scv = scv + s * djaco * 1.0;
+    }
     d.G[2 * nc + ie] = vol;
     d.G[4 * nc + ie] = rcv / vol;
     d.G[5 * nc + ie] = scv / vol;

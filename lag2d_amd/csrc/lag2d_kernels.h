@@ -16,7 +16,7 @@ __global__ void k_dt_partial(Dev d, const double *__restrict__ Uin, int nblk);
 __global__ void k_dt_final(Dev d, int itime, int nblk);
 __global__ void k_update(Dev d, double alfa);
 __global__ void k_nodemove(Dev d, double alfa, const double *__restrict__ xsrc, const double *__restrict__ ysrc);
-__global__ void k_center_q2(Dev d);
-__global__ void k_cellvol(Dev d, const double *__restrict__ x, const double *__restrict__ y);
+__global__ void k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y, int set_centre);
+constexpr int CV_CELLS = 32;  // cells per block of k_cellvol
 
 }  // namespace lag2d

@@ -141,37 +141,48 @@ __device__ __forceinline__ double rho0(int ncase, double xc, double yc)
 
 // ---------------------------------------------------------------- prep --
 // Copies the stage DOFs into the limited buffer, evaluates the Persson
-// indicator on tau and appends troubled cells to the WENO list.
-__global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_prep(Dev d, const double *__restrict__ Uin)
+// indicator on tau and appends troubled cells to the WENO list.  8 lanes
+// per cell evaluate the 25 Gauss points; lane 0 sums them in point order.
+constexpr int PREP_CELLS = 32;
+
+__global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict__ Uin)
 {
-    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nc = d.nc;
-    if (ie >= nc) return;
-    double u[4][6];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) {
-            u[q][id] = UD(Uin, q, id, ie);
-            UD(d.Ul, q, id, ie) = u[q][id];
-        }
-    double xp[9], yp[9];
-    load9(d, d.xs, d.ys, ie, xp, yp);
-    const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
-    const double g0 = d.G[18 * nc + ie], g1 = d.G[19 * nc + ie], g2 = d.G[20 * nc + ie];
-    double v1 = 0.0, v2 = 0.0;
-    for (int i = 0; i < 5; ++i)
-        for (int j = 0; j < 5; ++j) {
-            double r = PG5[i], s = PG5[j], wi = WG5[i] * WG5[j];
+    __shared__ double sv[PREP_CELLS][2 * 25 + 1];
+    const int nc = d.nc, ie0 = blockIdx.x * PREP_CELLS;
+    for (int e = threadIdx.x; e < 24 * PREP_CELLS; e += 256) {
+        const int qi = e / PREP_CELLS, c = e - PREP_CELLS * qi, ie = ie0 + c;
+        if (ie < nc) d.Ul[qi * nc + ie] = Uin[qi * nc + ie];
+    }
+    const int cl = threadIdx.x >> 3, lane = threadIdx.x & 7, ie = ie0 + cl;
+    if (ie < nc) {
+        double u0[6];
+        for (int id = 0; id < 6; ++id) u0[id] = UD(Uin, 0, id, ie);
+        double xp[9], yp[9];
+        load9(d, d.xs, d.ys, ie, xp, yp);
+        const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
+        const double g0 = d.G[18 * nc + ie], g1 = d.G[19 * nc + ie], g2 = d.G[20 * nc + ie];
+        for (int g = lane; g < 25; g += 8) {
+            const int i = g / 5, j = g - 5 * i;
+            const double r = PG5[i], s = PG5[j], wi = WG5[i] * WG5[j];
             double J[4];
             jac9(xp, yp, r, s, J);
-            double djaco = wi * (J[0] * J[3] - J[2] * J[1]);
+            const double djaco = wi * (J[0] * J[3] - J[2] * J[1]);
             double b[6];
             basis6(r, s, rc, sc, g0, g1, g2, b);
             double un = 0.0, u1 = 0.0;
-            for (int id = 0; id < 6; ++id) un = un + u[0][id] * b[id];
-            for (int id = 0; id < 3; ++id) u1 = u1 + u[0][id] * b[id];
-            v2 = v2 + (u1 - un) * (u1 - un) * djaco;
-            v1 = v1 + un * un * djaco;
+            for (int id = 0; id < 6; ++id) un = un + u0[id] * b[id];
+            for (int id = 0; id < 3; ++id) u1 = u1 + u0[id] * b[id];
+            sv[cl][g] = un * un * djaco;
+            sv[cl][25 + g] = (u1 - un) * (u1 - un) * djaco;
         }
+    }
+    __syncthreads();
+    if (ie >= nc || lane != 0) return;
+    double v1 = 0.0, v2 = 0.0;
+    for (int g = 0; g < 25; ++g) {
+        v2 = v2 + sv[cl][25 + g];
+        v1 = v1 + sv[cl][g];
+    }
     const double thr = -3.0 * log10(4.0) - 4.0;
     const bool troubled = !(log10(v2 / v1) <= thr);
     d.flag[ie] = troubled ? 1 : 0;
@@ -835,7 +846,7 @@ constexpr int CORNER_CPB = 16;
 
 __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
 {
-    __shared__ double sgq[CORNER_CPB][16][3];
+    __shared__ double sgq[CORNER_CPB][16 * 3 + 1];  // padded: cells of a wave in different banks
     __shared__ double sta[CORNER_CPB][4][25], stm[CORNER_CPB][4][25], stv[CORNER_CPB][4][25];
     const int cl = threadIdx.x >> 4, lane = threadIdx.x & 15;
     const int ie = blockIdx.x * CORNER_CPB + cl;
@@ -890,9 +901,9 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
             gy = any * 2.0 / farea;
             ga = farea;
         }
-        sgq[cl][lane][0] = gx;
-        sgq[cl][lane][1] = gy;
-        sgq[cl][lane][2] = ga;
+        sgq[cl][3 * lane + 0] = gx;
+        sgq[cl][3 * lane + 1] = gy;
+        sgq[cl][3 * lane + 2] = ga;
         d.gq[(3 * lane + 0) * nc + ie] = gx;
         d.gq[(3 * lane + 1) * nc + ie] = gy;
         d.gq[(3 * lane + 2) * nc + ie] = ga;
@@ -972,8 +983,8 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
         const int tt = FNQSG[isg][2 * iv + f];
         const int sl = (tt > 0 ? tt : -tt) - 1;
         const double sg = tt > 0 ? 1.0 : -1.0;
-        const double nx = sg * sgq[cl][sl][0], ny = sg * sgq[cl][sl][1];
-        const double area = sgq[cl][sl][2] * WFR[isg][iv][f];
+        const double nx = sg * sgq[cl][3 * sl + 0], ny = sg * sgq[cl][3 * sl + 1];
+        const double area = sgq[cl][3 * sl + 2] * WFR[isg][iv][f];
         const double deltu = fabs(dux * nx + duy * ny);
         const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
         double m11;
@@ -1043,8 +1054,10 @@ constexpr int RHS_CPB = 8;
 
 __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int apply_update)
 {
-    __shared__ double scn[RHS_CPB][16][12];
-    __shared__ double spt[RHS_CPB][25][8];
+    // per-cell strides padded by one double so that the cells a wave reads
+    // together fall in different LDS banks
+    __shared__ double scn[RHS_CPB][16 * 12 + 1];
+    __shared__ double spt[RHS_CPB][25 * 8 + 1];
     __shared__ double sres[RHS_CPB][4][6];
     const int tid = threadIdx.x;
     const int nc = d.nc;
@@ -1059,7 +1072,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             const int ln = IPQSG[isg][iv];
             const int p = d.ipq[ln * nc + ie];
             const double us1 = d.usx[p], us2 = d.usy[p];
-            double *c = scn[cl][corner];
+            double *c = &scn[cl][12 * corner];
             c[0] = us1;
             c[1] = us2;
             for (int f = 0; f < 2; ++f) {
@@ -1099,7 +1112,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         const double rhoad = 1.0 / un[0];
         const double uu = un[1], vv = un[2];
         const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
-        double *s = spt[cl][pt];
+        double *s = &spt[cl][8 * pt];
         s[0] = J[0];
         s[1] = J[1];
         s[2] = J[2];
@@ -1125,7 +1138,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
 #pragma unroll 2
         for (int k = 0; k < 16; ++k) {
             const int isg = k >> 2, iv = k & 3;
-            const double *c = scn[cl][k];
+            const double *c = &scn[cl][12 * k];
             double bb[6];
             const int ln = IPQSG[isg][iv];
             basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, bb);
@@ -1146,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
 #pragma unroll 1
         for (int k = 0; k < 25; ++k) {
-            const double *s = spt[cl][k];
+            const double *s = &spt[cl][8 * k];
             const int i = k / 5, j = k - 5 * i;
             const double rr = PL5[i], ss = PL5[j];
             const double dxdr = s[0], dxds = s[1], dydr = s[2], dyds = s[3];
@@ -1295,7 +1308,7 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
 {
     const int nc = x->nc, np = x->np;
     x->tic(KID_Q2_PREP);
-    hipLaunchKernelGGL(k_q2_prep, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
+    hipLaunchKernelGGL(k_q2_prep, dim3((nc + PREP_CELLS - 1) / PREP_CELLS), dim3(256), 0, x->stream, ds, Uin);
     x->toc(KID_Q2_PREP);
     x->tic(KID_Q2_WENO);
     hipLaunchKernelGGL(k_q2_weno, dim3(nc < WENO_GRID ? nc : WENO_GRID), dim3(WENO_WG), 0, x->stream, ds, Uin);
