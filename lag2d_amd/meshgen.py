@@ -215,6 +215,11 @@ class Control:
     nriem: int = 1
     nfint: int = 1
     nrz: int = 0
+    # output interval of the driver's animation files; 0 = none.  The
+    # reference's main program evaluates mod(itime, ndump) even when ndump = 0
+    # (dgflo2d.f90:430, no short-circuit), so a run of that program needs a
+    # non-zero value.
+    ndump: int = 0
 
 
 def sod_problem(nx=100, ny=4):
@@ -253,7 +258,7 @@ def write_ctrl(path, c: Control):
         "ndt nstag", f"{c.ndt} {c.nstag}",
         "ncase nriem nfint", f"{c.ncase} {c.nriem} {c.nfint}",
         "nrz", f"{c.nrz}",
-        "nout ndump", "0 0",
+        "nout ndump", f"0 {c.ndump}",
     ]
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -23,6 +23,8 @@ OBJS   := $(addprefix $(OUT)/obj/,$(addsuffix .o,$(UNITS)))
 
 all: $(OUT)/refobjs.a $(OUT)/lag2d_refdump
 
+drivers: $(OUT)/lag2d_drv_cpu $(OUT)/lag2d_drv_gpu
+
 $(OUT)/obj/dgflo2d.o: $(REFSRC)/dgflo2d.f90
 	@mkdir -p $(OUT)/obj $(OUT)/mod
 	$(FC) $(FFLAGS) -c $< -o $@
@@ -40,4 +42,25 @@ $(OUT)/refobjs.a: $(OBJS)
 $(OUT)/lag2d_refdump: oracle/refdump.f90 $(OUT)/refobjs.a
 	$(FC) $(FFLAGS) -o $@ oracle/refdump.f90 $(OUT)/refobjs.a
 
-.PHONY: all
+# The reference's own program twice: as it is (CPU), and with its
+# getsolvdg_lag demoted to a weak symbol so that the drop-in of
+# lag2d_amd/fortran/getsolvdg_lag_shim.f90 (GPU, through liblag2d.so) takes
+# the call.  Used by the integration test of the Fortran boundary.
+SHIM := lag2d_amd/fortran
+REST := $(filter-out $(OUT)/obj/Getrhslag.o,$(OBJS))
+
+$(OUT)/lag2d_drv_cpu: $(OBJS)
+	$(FC) -o $@ $(OBJS)
+
+$(OUT)/obj/lag2d_c.o: $(SHIM)/lag2d_c.f90 $(OUT)/obj/dgflo2d.o
+	$(FC) $(FFLAGS) -c $< -o $@
+
+$(OUT)/obj/getsolvdg_lag_shim.o: $(SHIM)/getsolvdg_lag_shim.f90 $(OUT)/obj/lag2d_c.o
+	$(FC) $(FFLAGS) -c $< -o $@
+
+$(OUT)/lag2d_drv_gpu: $(OUT)/obj/getsolvdg_lag_shim.o $(OUT)/obj/lag2d_c.o $(OBJS) lag2d_amd/liblag2d.so
+	objcopy --weaken-symbol=getsolvdg_lag_ $(OUT)/obj/Getrhslag.o $(OUT)/obj/Getrhslag_weak.o
+	$(FC) -o $@ $(OUT)/obj/getsolvdg_lag_shim.o $(OUT)/obj/lag2d_c.o $(OUT)/obj/Getrhslag_weak.o $(REST) \
+	    -L lag2d_amd -llag2d -Wl,-rpath,'$$ORIGIN/../../lag2d_amd'
+
+.PHONY: all drivers

@@ -47,3 +47,18 @@ def test_unsupported_config_is_rejected_without_device():
                         e1.ctypes.data_as(P), e2.ctypes.data_as(P), ctypes.byref(h))
     assert rc == 2
     assert b"ndens" in L.lag2d_last_error()
+
+
+def test_fortran_interface_compiles(tmp_path):
+    """The ISO_C_BINDING module (lag2d_amd/fortran/lag2d_c.f90) compiles on
+    its own: it depends on nothing of the reference."""
+    import shutil
+    import subprocess
+    fc = shutil.which("amdflang") or shutil.which("flang") or shutil.which("gfortran")
+    if fc is None:
+        import pytest
+        pytest.skip("no Fortran compiler")
+    src = os.path.join(ROOT, "lag2d_amd", "fortran", "lag2d_c.f90")
+    r = subprocess.run([fc, "-c", src, "-o", str(tmp_path / "lag2d_c.o")], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
