@@ -27,6 +27,83 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 
 
+def _reference_rate(nx, nsteps, dofs_per_step):
+    """Cell-DOF updates/s of the reference itself (oracle/_ref/lag2d_refdump,
+    the reference's own subroutines built by oracle/ref.mk) on the first
+    nsteps steps of the same problem, 1 core: time(init + nsteps) minus
+    time(init)."""
+    import resource
+    import shutil
+    import subprocess
+    import tempfile
+    import numpy as np
+    from lag2d_amd import meshgen as mg
+    exe = os.path.join(ROOT, "oracle", "_ref", "lag2d_refdump")
+    if not os.path.exists(exe):
+        return None
+
+    def unlimited_stack():
+        # the reference keeps large automatic arrays on the stack
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+    def run(work, n):
+        with open(os.path.join(work, "refdump.in"), "w") as f:
+            f.write(f"{n} {max(n, 1)} 1\n")
+        t0 = time.perf_counter()
+        r = subprocess.run([exe], cwd=work, capture_output=True, preexec_fn=unlimited_stack)
+        t1 = time.perf_counter()
+        if r.returncode != 0:
+            raise RuntimeError(f"reference run failed ({r.returncode})")
+        return t1 - t0
+
+    mesh, ctl = mg.sedov_problem(nx, seed=2024)
+    ctl.cdrho = 1.0
+    work = tempfile.mkdtemp(prefix="lag2d_ref_")
+    try:
+        mg.write_case(work, mesh, ctl)
+        with open(os.path.join(work, "refdump.in"), "w") as f:
+            f.write("0 1 0\n")
+        r = subprocess.run([exe], cwd=work, capture_output=True, preexec_fn=unlimited_stack)
+        if r.returncode != 0:
+            return None
+        ncell = int(np.fromfile(os.path.join(work, "init_sizes.bin"))[0])
+        un = np.fromfile(os.path.join(work, "init_unkno.bin")).reshape(-1, 4, ctl.ndegr)[:ncell]
+        ge = np.fromfile(os.path.join(work, "init_geoel.bin")).reshape(-1, 24)[:ncell]
+        mg.sedov_inject(un, ge, mg.centre_cells(nx, nx)).astype(np.float64).tofile(os.path.join(work, "inject.bin"))
+        t_init = run(work, 0)
+        t_all = run(work, nsteps)
+        return dofs_per_step * nsteps / (t_all - t_init)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def cpu_baseline(args, dofs_per_step):
+    """The reference on the host's cores when its build is present, else the C
+    restatement of the step (oracle/), both single-threaded."""
+    n_cpu = args.cpu_steps or 12
+    try:
+        rate = _reference_rate(args.nx, n_cpu, dofs_per_step)
+    except Exception as e:  # noqa: BLE001 - fall back to the port below
+        print(f"reference CPU baseline unavailable: {e}", file=sys.stderr)
+        rate = None
+    if rate is not None:
+        return {"value": rate, "unit": "cell-DOF updates/s", "cores": 1, "kind": "reference",
+                "sample": f"first {n_cpu} SSP-RK3 steps of the same {args.nx}x{args.nx} Sedov P2 problem, "
+                          f"the reference's own subroutines (amdflang -O2 -ffp-contract=off build, "
+                          f"oracle/ref.mk), 1 thread; time(init + steps) - time(init)"}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle  # the C restatement, timed as the CPU baseline
+    from lag2d_amd import init
+    o = Oracle(init.sedov_case(args.nx, seed=2024, centre=True, cdrho=1.0))
+    c0 = time.perf_counter()
+    o.step(n_cpu)
+    c1 = time.perf_counter()
+    return {"value": dofs_per_step * n_cpu / (c1 - c0), "unit": "cell-DOF updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"first {n_cpu} SSP-RK3 steps of the same {args.nx}x{args.nx} Sedov P2 problem, "
+                      f"C restatement of the reference step (oracle/, gcc -O2), 1 thread"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,18 +159,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        from oracle import Oracle  # the C restatement, timed as the CPU baseline
-        ocase = init.sedov_case(args.nx, seed=2024, centre=True, cdrho=1.0)
-        o = Oracle(ocase)
-        n_cpu = args.cpu_steps or 4
-        c0 = time.perf_counter()
-        o.step(n_cpu)
-        c1 = time.perf_counter()
-        cpu = {"value": dofs_per_step * n_cpu / (c1 - c0), "unit": "cell-DOF updates/s", "cores": 1,
-               "kind": "port",
-               "sample": f"first {n_cpu} SSP-RK3 steps of the same {args.nx}x{args.nx} Sedov P2 problem, "
-                         f"C restatement of the reference step (gcc -O2), 1 thread"}
+        cpu = cpu_baseline(args, dofs_per_step)
 
     line = {
         "metric": "FP64 cell-DOF updates/s, Sedov P2 curved, 1 GPU",

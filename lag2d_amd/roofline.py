@@ -36,6 +36,23 @@ BYTES_Q2 = {
 }
 NODE_KERNELS = {"k_node_q2", "k_nodemove", "k_node_q1", "k_nodeminmax"}
 
+# HBM traffic per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (KiB per dispatch, mean over the dispatches of a 3-step run of the 120x120
+# benchmark), scaled by what the same counters read for a plain 256 MiB
+# device-to-device copy: FETCH_SIZE reads half the copied bytes (factor 2.0),
+# WRITE_SIZE all of them (factor 1.0).  Source: profiles/counters_sedov_p2_120x120.txt
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+MEASURED_KIB = {}  # kernel: (FETCH_SIZE, WRITE_SIZE), 120x120 benchmark only
+
+
+def traffic(kernel: str, case):
+    """Measured HBM bytes per launch (scaled counters), or None when the
+    counters were not taken for this kernel and mesh."""
+    if case.ncell != 14400 or kernel not in MEASURED_KIB:
+        return None
+    f, w = MEASURED_KIB[kernel]
+    return (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0
+
 
 def figures(kernel: str, case, avg_ms: float) -> dict:
     units = case.npoin if kernel in NODE_KERNELS else case.ncell
@@ -47,10 +64,10 @@ def figures(kernel: str, case, avg_ms: float) -> dict:
     if t_flop >= t_mem:
         achieved = flops / t / 1e12
         return {"kernel": kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic(kernel, case),
                 "algorithmic_flops": flops, "algorithmic_bytes": nbytes, "avg_ms": avg_ms,
                 "note": "FP64 vector peak (78.6 TFLOP/s, same as FP64 matrix); kernel uses VALU FP64"}
     achieved = nbytes / t / 1e9
     return {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "traffic": None, "algorithmic_flops": flops,
+            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic(kernel, case), "algorithmic_flops": flops,
             "algorithmic_bytes": nbytes, "avg_ms": avg_ms}
