@@ -249,6 +249,7 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
     if (ie >= nc || lane != 0) return;
     // planar: rcoef = 1 - alfrz + alfrz*y = 1
     double rcv = 0.0, scv = 0.0, vol = 0.0;
+#pragma unroll
     for (int g = 0; g < 36; ++g) {
         const int i = g / 6, j = g - 6 * i;
         const double r = PG6[i], s = PG6[j], djaco = sdj[cl][g];

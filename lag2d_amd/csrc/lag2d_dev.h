@@ -81,4 +81,16 @@ struct Dev {
     int *wenobad;            // count of such cells
 };
 
+// Logical block index for an XCD-aware order.  Workgroups are dispatched to
+// the 8 XCDs round-robin (block b runs on XCD b % 8, each XCD with its own
+// L2); this maps the blocks of one XCD to a contiguous range of logical
+// blocks, so that neighbouring cells -- which share cache lines of every
+// cell-indexed array -- are processed under the same L2.  A bijection on
+// [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 }  // namespace lag2d

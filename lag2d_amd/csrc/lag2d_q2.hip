@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict
     __syncthreads();
     if (ie >= nc || lane != 0) return;
     double v1 = 0.0, v2 = 0.0;
+#pragma unroll
     for (int g = 0; g < 25; ++g) {
         v2 = v2 + sv[cl][25 + g];
         v1 = v1 + sv[cl][g];
@@ -377,6 +378,8 @@ constexpr int WENO_GRID = 1024;
 
 struct WenoShared {
     double m[4];
+    double cx0[9][9], cy0[9][9], cxs[9][9], cys[9][9];  // initial and stage nodes of the stencil cells
+    double gc[9][7];                                     // rc, sc, offsets 19-21, volume, mass
     double xl[9][25], yl[9][25], rd[9][25];  // rotated Gauss points, rho * |J| * w
     double tx[9][25], ty[9][25];             // mass-centre terms (unrotated)
     double bl[9][25][5];                     // basis about the cell's own mass centre
@@ -423,8 +426,9 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
     __shared__ WenoShared S;
     const int tid = threadIdx.x;
     // task index of this thread: consecutive tasks go to different waves, so
-    // phases with few tasks still use all four SIMDs
+    // phases with few tasks still use every SIMD
     const int tsp = ((tid & 63) << 2) | (tid >> 6);
+    static_assert(WENO_WG == 256, "task spreading assumes 4 waves");
     const int nc = d.nc;
     const int ntr = *d.tcount;
     for (int t = blockIdx.x; t < ntr; t += gridDim.x) {
@@ -504,16 +508,36 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         if (S.skip) continue;
         const double m[4] = {S.m[0], S.m[1], S.m[2], S.m[3]};
 
+        // 0: nodes and geometry rows of the stencil cells into LDS
+        for (int k = tsp; k < 81 + 63; k += WENO_WG) {
+            if (k < 81) {
+                const int s = k / 9, q = k - 9 * s;
+                const int p = d.ipq[q * nc + S.ielaj[s]];
+                S.cx0[s][q] = d.x0[p];
+                S.cy0[s][q] = d.y0[p];
+                S.cxs[s][q] = d.xs[p];
+                S.cys[s][q] = d.ys[p];
+            } else {
+                const int u = k - 81, s = u / 7, r = u - 7 * s;
+                const int row[7] = {0, 1, 18, 19, 20, 2, 3};
+                S.gc[s][r] = d.G[row[r] * nc + S.ielaj[s]];
+            }
+        }
+        __syncthreads();
+
         // 1: Gauss points of the stencil, rotated DOFs, face characteristic matrices
         for (int k = tsp; k < 225 + 54 + 4; k += WENO_WG) {
             if (k < 225) {
                 const int s = k / 25, g = k - 25 * s, i = g / 5, j = g - 5 * i;
-                const int je = S.ielaj[s];
                 double xi[9], yi[9], xp[9], yp[9];
-                load9(d, d.x0, d.y0, je, xi, yi);
-                load9(d, d.xs, d.ys, je, xp, yp);
+                for (int q = 0; q < 9; ++q) {
+                    xi[q] = S.cx0[s][q];
+                    yi[q] = S.cy0[s][q];
+                    xp[q] = S.cxs[s][q];
+                    yp[q] = S.cys[s][q];
+                }
                 double xcr, ycr;
-                map9(xi, yi, d.G[0 * nc + je], d.G[1 * nc + je], &xcr, &ycr);
+                map9(xi, yi, S.gc[s][0], S.gc[s][1], &xcr, &ycr);
                 const double rho = rho0(d.ncase, xcr, ycr);
                 const double r = PG5[i], sg = PG5[j], wi = WG5[i] * WG5[j];
                 double J[4];
@@ -530,8 +554,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
                 S.yl[s][g] = yg;
                 S.rd[s][g] = rdv;
                 double b[6];
-                basis6(r, sg, d.G[0 * nc + je], d.G[1 * nc + je], d.G[18 * nc + je], d.G[19 * nc + je],
-                       d.G[20 * nc + je], b);
+                basis6(r, sg, S.gc[s][0], S.gc[s][1], S.gc[s][2], S.gc[s][3], S.gc[s][4], b);
                 for (int a = 0; a < 5; ++a) S.bq[s][g][a] = b[a + 1];
             } else if (k < 225 + 54) {
                 const int u = k - 225, s = u / 6, id = u - 6 * s;
@@ -565,18 +588,19 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         __syncthreads();
         // 2: physical mass centres
         if (tsp < 9) {
-            const int s = tsp, je = S.ielaj[s];
+            const int s = tsp;
             double xm = 0.0, ym = 0.0;
+#pragma unroll
             for (int g = 0; g < 25; ++g) {
                 xm = xm + S.tx[s][g];
                 ym = ym + S.ty[s][g];
             }
-            const double masel = d.G[3 * nc + je];
+            const double masel = S.gc[s][6];
             double xc = xm / masel, yc = ym / masel;
             rotf(m, xc, yc);
             S.geopl[s][0] = xc;
             S.geopl[s][1] = yc;
-            S.dxcj[s] = sqrt(d.G[2 * nc + je]);
+            S.dxcj[s] = sqrt(S.gc[s][5]);
             S.masel[s] = masel;
         }
         __syncthreads();
@@ -591,6 +615,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         if (tsp < 27) {
             const int s = tsp / 3, c = tsp - 3 * s;
             double v = 0.0;
+#pragma unroll
             for (int g = 0; g < 25; ++g) {
                 const double b2 = S.bl[s][g][0], b3 = S.bl[s][g][1], rd = S.rd[s][g];
                 const double x = (c == 1) ? b3 : b2, y = (c == 0) ? b2 : b3;
@@ -623,20 +648,24 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
                 const int b = e / 5, a = e - 5 * b;
                 double v = 0.0;
                 if (which == 0) {
+#pragma unroll
                     for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bq[s][g][a] * S.bl[s][g][b];
                     S.bb[s][e] = v;
                 } else {
+#pragma unroll
                     for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl[s][g][a] * S.bl[s][g][b];
                     S.bbq[s][e] = v;
                 }
             } else if (k < 490) {
                 const int u = k - 450, s = 1 + u / 5, c = u - 5 * (u / 5);
                 double v = 0.0;
+#pragma unroll
                 for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl0[s][g][c] / S.masel[s];
                 S.s1[s][c] = v;
             } else {
                 const int u = k - 490, s = 1 + u / 10, r = u - 10 * (u / 10), a = r >> 1, c = r & 1;
                 double v = 0.0;
+#pragma unroll
                 for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl0[s][g][a] * S.bl[s][g][c] / S.masel[s];
                 S.mh2[s][a][c] = v;
             }
@@ -948,6 +977,7 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
     // ---- sub-cell corner (isg, iv)
     const int iv = part;
     double a = 0.0, masel = 0.0, volel = 0.0;
+#pragma unroll
     for (int g = 0; g < 25; ++g) {
         a = a + sta[cl][isg][g];
         masel = masel + stm[cl][isg][g];
@@ -1045,23 +1075,30 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
 
 // Sub-cell face integral, Gauss-Lobatto domain integral, the (identity)
 // limiter update and, with apply_update, the mass-matrix solve and RK blend.
-// 8 cells per 256-thread block.  Staging: 128 corner tasks (nodal velocity,
-// normals, areas, corner forces of the 16 sub-cell corners) and 200 point
-// tasks (Jacobian, velocity, pressure at the 25 Lobatto points).  Sums: wave
-// q owns variable q of all 8 cells, lane (cell, id) sums its residual entry
-// over corners and points in the sequential order, then applies the update.
+// 8 cells per 256-thread block.  Staging: 128 corner tasks compute the
+// corner-only factors of every face-integral term and the basis at the
+// corner's node; 200 Lobatto-point tasks compute velocity, pressure and the
+// cofactor-form basis gradients of every DOF.  Sums: wave q owns variable q
+// of the 8 cells, lane (cell, id) accumulates its residual entry over the
+// corners and points in the reference's sequential order (each term is the
+// staged factor times the basis value, rounded as the reference rounds it),
+// then applies the update.
 constexpr int RHS_CPB = 8;
+// per-cell LDS strides, padded by one double so that the cells a wave reads
+// together fall in different banks
+constexpr int RHS_CN = 16 * 12 + 1;  // corner factors
+constexpr int RHS_BN = 16 * 6 + 1;   // basis at the corner's node
+constexpr int RHS_PT = 25 * 16 + 1;  // point values: a[6], c[6], u, v, p, w
 
 __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int apply_update)
 {
-    // per-cell strides padded by one double so that the cells a wave reads
-    // together fall in different LDS banks
-    __shared__ double scn[RHS_CPB][16 * 12 + 1];
-    __shared__ double spt[RHS_CPB][25 * 8 + 1];
+    __shared__ double scn[RHS_CPB][RHS_CN];
+    __shared__ double sbn[RHS_CPB][RHS_BN];
+    __shared__ double spt[RHS_CPB][RHS_PT];
     __shared__ double sres[RHS_CPB][4][6];
     const int tid = threadIdx.x;
     const int nc = d.nc;
-    const int ie0 = blockIdx.x * RHS_CPB;
+    const int ie0 = xcd_block(blockIdx.x, gridDim.x) * RHS_CPB;
     if (blockIdx.x == 0 && tid == 0) *d.tcount = 0;  // WENO list of this stage consumed
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
     // corners on threads 0-127, points on threads 128-255 and 0-71
@@ -1072,26 +1109,38 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             const int ln = IPQSG[isg][iv];
             const int p = d.ipq[ln * nc + ie];
             const double us1 = d.usx[p], us2 = d.usy[p];
-            double *c = &scn[cl][12 * corner];
-            c[0] = us1;
-            c[1] = us2;
+            double nx[2], ny[2], ar[2], fx[2], fy[2];
             for (int f = 0; f < 2; ++f) {
                 const int tt = FNQSG[isg][2 * iv + f];
                 const int sl = (tt > 0 ? tt : -tt) - 1;
                 const double sg = tt > 0 ? 1.0 : -1.0;
-                const double nx = sg * d.gq[(3 * sl + 0) * nc + ie];
-                const double ny = sg * d.gq[(3 * sl + 1) * nc + ie];
-                const double ar = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
+                nx[f] = sg * d.gq[(3 * sl + 0) * nc + ie];
+                ny[f] = sg * d.gq[(3 * sl + 1) * nc + ie];
+                ar[f] = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
                 const int cf = f + 2 * (iv + 4 * isg);
                 const double m11 = d.rec[(0 * 32 + cf) * nc + ie];
                 const double m1 = d.rec[(1 * 32 + cf) * nc + ie], m2 = d.rec[(2 * 32 + cf) * nc + ie];
                 const double s1 = d.rec[(3 * 32 + cf) * nc + ie], s2 = d.rec[(4 * 32 + cf) * nc + ie];
-                c[2 + 3 * f] = nx;
-                c[3 + 3 * f] = ny;
-                c[4 + 3 * f] = ar;
-                c[8 + 2 * f] = s1 + m11 * us1 + 0.0 * us2 - m1;
-                c[9 + 2 * f] = s2 + 0.0 * us1 + m11 * us2 - m2;
+                fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
+                fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
             }
+            double *c = &scn[cl][12 * corner];
+            c[0] = us1 * nx[0] * ar[0];  // tau: (u*.n A) b
+            c[1] = us2 * ny[0] * ar[0];
+            c[2] = us1 * nx[1] * ar[1];
+            c[3] = us2 * ny[1] * ar[1];
+            c[4] = fx[0];  // momentum: f* b w
+            c[5] = fx[1];
+            c[6] = fy[0];
+            c[7] = fy[1];
+            c[8] = us1 * fx[0];  // energy: (u*.f*) b w
+            c[9] = us2 * fy[0];
+            c[10] = us1 * fx[1];
+            c[11] = us2 * fy[1];
+            double bb[6];
+            basis6(XV9[ln], YV9[ln], d.G[0 * nc + ie], d.G[1 * nc + ie], d.G[18 * nc + ie], d.G[19 * nc + ie],
+                   d.G[20 * nc + ie], bb);
+            for (int id = 0; id < 6; ++id) sbn[cl][6 * corner + id] = bb[id];
         }
     }
     for (int u = (tid + 256 - RHS_CPB * 16) & 255; u < RHS_CPB * 25; u += 256) {
@@ -1099,84 +1148,99 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         if (ie >= nc) continue;
         const int i = pt / 5, j = pt - 5 * i;
         const double rr = PL5[i], ss = PL5[j], wi = WL5[i] * WL5[j];
+        const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
         double xp[9], yp[9];
         load9(d, d.xs, d.ys, ie, xp, yp);
         double J[4];
         jac9(xp, yp, rr, ss, J);
         double b[6];
-        basis6(rr, ss, d.G[0 * nc + ie], d.G[1 * nc + ie], d.G[18 * nc + ie], d.G[19 * nc + ie],
-               d.G[20 * nc + ie], b);
+        basis6(rr, ss, rc, sc, d.G[18 * nc + ie], d.G[19 * nc + ie], d.G[20 * nc + ie], b);
         double un[4] = {0.0, 0.0, 0.0, 0.0};
         for (int id = 0; id < 6; ++id)
             for (int q = 0; q < 4; ++q) un[q] = un[q] + UD(d.Ul, q, id, ie) * b[id];
         const double rhoad = 1.0 / un[0];
         const double uu = un[1], vv = un[2];
         const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
-        double *s = &spt[cl][8 * pt];
-        s[0] = J[0];
-        s[1] = J[1];
-        s[2] = J[2];
-        s[3] = J[3];
-        s[4] = uu;
-        s[5] = vv;
-        s[6] = pres;
-        s[7] = wi;
+        const double dxdr = J[0], dxds = J[1], dydr = J[2], dyds = J[3];
+        const double dbdr[6] = {0.0, 1.0 / 1.0, 0.0, (rr - rc) / 1.0, 0.0, (ss - sc) / 1.0 / 1.0};
+        const double dbds[6] = {0.0, 0.0, 1.0 / 1.0, 0.0, (ss - sc) / 1.0, (rr - rc) / 1.0 / 1.0};
+        double *s = &spt[cl][16 * pt];
+        for (int id = 0; id < 6; ++id) {
+            s[id] = dyds * dbdr[id] + (-dydr) * dbds[id];
+            s[6 + id] = (-dxds) * dbdr[id] + dxdr * dbds[id];
+        }
+        s[12] = uu;
+        s[13] = vv;
+        s[14] = pres;
+        s[15] = wi;
     }
     __syncthreads();
     const int q = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int l = tid & 63, cl = l / 6, id = l - 6 * cl, ie = ie0 + cl;
     const bool active = l < 6 * RHS_CPB && ie < nc;
-    double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
+    // operands of the update, loaded ahead of the sums
+    double uqi = 0.0, uo = 0.0, un = 0.0, mrow[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (active) {
-        rc = d.G[0 * nc + ie];
-        sc = d.G[1 * nc + ie];
-        gg0 = d.G[18 * nc + ie];
-        gg1 = d.G[19 * nc + ie];
-        gg2 = d.G[20 * nc + ie];
-        double r = 0.0;
-        // ---- face integral, corners in (isg, iv) order
-#pragma unroll 2
-        for (int k = 0; k < 16; ++k) {
-            const int isg = k >> 2, iv = k & 3;
-            const double *c = &scn[cl][12 * k];
-            double bb[6];
-            const int ln = IPQSG[isg][iv];
-            basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, bb);
-            const double b = bb[id];
-            const double us1 = c[0], us2 = c[1];
-            const double nx0 = c[2], ny0 = c[3], ar0 = c[4], nx1 = c[5], ny1 = c[6], ar1 = c[7];
-            const double fx0 = c[8], fy0 = c[9], fx1 = c[10], fy1 = c[11];
-            const double w1 = WFM[isg][iv][0], w2 = WFM[isg][iv][1];
-            if (q == 0)
-                r = r + us1 * nx0 * ar0 * b + us2 * ny0 * ar0 * b + us1 * nx1 * ar1 * b + us2 * ny1 * ar1 * b;
-            else if (q == 1)
-                r = r + fx0 * b * w1 + fx1 * b * w2;
-            else if (q == 2)
-                r = r + fy0 * b * w1 + fy1 * b * w2;
-            else
-                r = r + us1 * fx0 * b * w1 + us2 * fy0 * b * w1 + us1 * fx1 * b * w2 + us2 * fy1 * b * w2;
+        uqi = UD(d.Ul, q, id, ie);
+        if ((id == 1 || id == 2) && (q == 1 || q == 2)) uo = UD(d.Ul, 3 - q, id, ie);
+        if (apply_update) {
+            un = UD(d.Un, q, id, ie);
+            // row id of the unpacked inverse mass matrix (degree 0 decoupled;
+            // upper triangle of the 5x5 block packed row-wise, then 1/mass)
+            const int off[6] = {0, 0, 5, 9, 12, 14};
+            for (int kk = 0; kk < 6; ++kk) {
+                if (id == 0 || kk == 0) {
+                    mrow[kk] = (id == 0 && kk == 0) ? d.A[15 * nc + ie] : 0.0;
+                } else {
+                    const int a = id < kk ? id : kk, b = id < kk ? kk : id;
+                    mrow[kk] = d.A[(off[a] + (b - a)) * nc + ie];
+                }
+            }
         }
-        // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
-#pragma unroll 1
-        for (int k = 0; k < 25; ++k) {
-            const double *s = &spt[cl][8 * k];
-            const int i = k / 5, j = k - 5 * i;
-            const double rr = PL5[i], ss = PL5[j];
-            const double dxdr = s[0], dxds = s[1], dydr = s[2], dyds = s[3];
-            const double uu = s[4], vv = s[5], pres = s[6], wi = s[7];
-            const double dbdr[6] = {0.0, 1.0 / 1.0, 0.0, (rr - rc) / 1.0, 0.0, (ss - sc) / 1.0 / 1.0};
-            const double dbds[6] = {0.0, 0.0, 1.0 / 1.0, 0.0, (ss - sc) / 1.0, (rr - rc) / 1.0 / 1.0};
-            const double a = dyds * dbdr[id] + (-dydr) * dbds[id];
-            const double c = (-dxds) * dbdr[id] + dxdr * dbds[id];
-            if (q == 0) {
-                const double f0 = a * uu + c * vv;
-                r = r - f0 * wi;
-            } else if (q == 1) {
-                r = r - a * (-pres) * wi;
-            } else if (q == 2) {
-                r = r - c * (-pres) * wi;
-            } else {
-                r = r - (a * uu + c * vv) * (-pres) * wi;
+        double r = 0.0;
+        const double *cn = scn[cl];
+        const double *bn = &sbn[cl][id];
+        const double *pt = spt[cl];
+        // ---- face integral, corners in (isg, iv) order; then the domain
+        //      integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
+        if (q == 0) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const double b = bn[6 * k];
+                r = r + cn[12 * k + 0] * b + cn[12 * k + 1] * b + cn[12 * k + 2] * b + cn[12 * k + 3] * b;
+            }
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                const double *s = &pt[16 * k];
+                const double f0 = s[id] * s[12] + s[6 + id] * s[13];
+                r = r - f0 * s[15];
+            }
+        } else if (q == 1 || q == 2) {
+            const int o = (q == 1) ? 4 : 6;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const double b = bn[6 * k];
+                const double w1 = WFM[k >> 2][k & 3][0], w2 = WFM[k >> 2][k & 3][1];
+                r = r + cn[12 * k + o] * b * w1 + cn[12 * k + o + 1] * b * w2;
+            }
+            const int ac = (q == 1) ? 0 : 6;
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                const double *s = &pt[16 * k];
+                r = r - s[ac + id] * (-s[14]) * s[15];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const double b = bn[6 * k];
+                const double w1 = WFM[k >> 2][k & 3][0], w2 = WFM[k >> 2][k & 3][1];
+                r = r + cn[12 * k + 8] * b * w1 + cn[12 * k + 9] * b * w1 + cn[12 * k + 10] * b * w2 +
+                    cn[12 * k + 11] * b * w2;
+            }
+#pragma unroll
+            for (int k = 0; k < 25; ++k) {
+                const double *s = &pt[16 * k];
+                r = r - (s[id] * s[12] + s[6 + id] * s[13]) * (-s[14]) * s[15];
             }
         }
         sres[cl][q][id] = r;
@@ -1184,30 +1248,17 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
     __syncthreads();
     if (!active) return;
     // ---- derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708)
-    const double uqi = UD(d.Ul, q, id, ie);
     double ul = uqi;
     if (id == 1 || id == 2) {
         if (q == 0 || q == 3) ul = uqi * 1.0;
-        else if (q == 1) ul = 1.0 * uqi + 0.0 * UD(d.Ul, 2, id, ie);
-        else ul = 0.0 * UD(d.Ul, 1, id, ie) + 1.0 * uqi;
+        else if (q == 1) ul = 1.0 * uqi + 0.0 * uo;
+        else ul = 0.0 * uo + 1.0 * uqi;
     }
     if (apply_update) {
         const double dt = d.scal[0];
-        // row id of the unpacked inverse mass matrix (degree 0 decoupled;
-        // upper triangle of the 5x5 block packed row-wise, then 1/mass)
-        const int off[6] = {0, 0, 5, 9, 12, 14};
-        double mrow[6];
-        for (int kk = 0; kk < 6; ++kk) {
-            if (id == 0 || kk == 0) {
-                mrow[kk] = (id == 0 && kk == 0) ? d.A[15 * nc + ie] : 0.0;
-            } else {
-                const int a = id < kk ? id : kk, b = id < kk ? kk : id;
-                mrow[kk] = d.A[(off[a] + (b - a)) * nc + ie];
-            }
-        }
         double s4 = 0.0;
         for (int kk = 0; kk < 6; ++kk) s4 = s4 + mrow[kk] * (dt * sres[cl][q][kk]);
-        UD(d.Us, q, id, ie) = alfa * UD(d.Un, q, id, ie) + (1.0 - alfa) * (ul + s4);
+        UD(d.Us, q, id, ie) = alfa * un + (1.0 - alfa) * (ul + s4);
     } else {
         UD(d.Us, q, id, ie) = ul;
         UD(d.R, q, id, ie) = sres[cl][q][id];
