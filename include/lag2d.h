@@ -6,7 +6,8 @@
  * which calls
  *   getrhsdg_lag_mcnew(...)   once per stage  (Getrhslag.f90:1747-1750).
  * The entry points below replace those two subroutines.  A Fortran caller
- * binds them with ISO_C_BINDING (lag2d_amd/fortran/lag2d_shim.f90); arrays are
+ * binds them with ISO_C_BINDING (lag2d_amd/fortran/lag2d_c.f90; a drop-in
+ * getsolvdg_lag is lag2d_amd/fortran/getsolvdg_lag_shim.f90); arrays are
  * the reference's own column-major arrays, connectivity 1-based, sizes and
  * switches those of module `constant` (dgflo2d.f90:4-57).
  *
@@ -63,6 +64,25 @@ int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
                  const int *ipqua, const int *bface, const int *esuv1,
                  const int *esuv2, lag2d_ctx **ctx);
 
+/* Ensemble of independent runs (batched mode; curved P2 path only).  No
+ * counterpart in the reference, which runs one problem per process: the
+ * runs' meshes are concatenated in run order into one mesh -- the cells,
+ * nodes and boundary faces of run r follow those of run r-1, connectivity
+ * holds global 1-based ids, esuv1/esuv2 the CSR of the whole mesh -- and
+ * every kernel launch of a stage covers all runs.  Run r owns cells
+ * [cell_off[r], cell_off[r+1]) and nodes [node_off[r], node_off[r+1]); a
+ * cell may use only nodes of its own run.  Each run keeps its own dt
+ * (CFL-limited over its own cells, as getdelt_lag does for one problem),
+ * time tacru[r] and end time tend[r]; a run that has reached its end time
+ * is left unchanged by later steps.  cfg holds the switches shared by all
+ * runs; its tend, dtfix and tacru are not used.  Upload, download and check
+ * work on the concatenated arrays. */
+int lag2d_create_ensemble(const lag2d_config *cfg, int nrun, const int *cell_off,
+                          const int *node_off, const double *tend, const double *dtfix,
+                          const double *tacru, int nquad, int npoin, int nbfac,
+                          const int *ipqua, const int *bface, const int *esuv1,
+                          const int *esuv2, lag2d_ctx **ctx);
+
 /* Upload the solution state (reference layouts, nsize >= ncell):
  *   coord(2,npoin) x^n, coold(2,npoin) x^0, geoel(24,ncell), amatr(nmatr,ncell),
  *   unkno(ndegr,4,ncell), ustar(2,npoin).  Recomputes geoel(3,5,6) from coord
@@ -71,6 +91,15 @@ int lag2d_upload(lag2d_ctx *ctx, const double *coord, const double *coold,
                  const double *geoel, const double *amatr, const double *unkno,
                  const double *ustar);
 
+/* Initial density of every cell (ncell values), for initial states other
+ * than the reference's built-in problems.  No counterpart in the reference:
+ * its initial sub-cell masses (getrhosubg_averg, Getrhslaggauss.f90:7293-7374)
+ * and WENO mass centres (WENO.f90:5989-6061) evaluate the initial density
+ * of problem ncase (getrhoig_quadcurv2, Getrhslag.f90:17761-17948); after
+ * this call they use the given values.  Curved P2 path; call after
+ * lag2d_upload and before the first step. */
+int lag2d_set_initial_density(lag2d_ctx *ctx, const double *rho);
+
 /* One SSP-RK time step: getsolvdg_lag (Getrhslag.f90:1553-2035) followed by
  * the driver's copy of the stage state into unkno/coord
  * (dgflo2d.f90:408-422).  itime is the driver's 1-based step counter.
@@ -78,7 +107,8 @@ int lag2d_upload(lag2d_ctx *ctx, const double *coord, const double *coold,
 int lag2d_step(lag2d_ctx *ctx, int itime);
 
 /* nsteps steps (itime0, itime0+1, ...), stopping after the step that reaches
- * tend, as the driver does (dgflo2d.f90:458-463).  Returns the number taken
+ * tend, as the driver does (dgflo2d.f90:458-463); an ensemble stops when
+ * every run has reached its end time.  Returns the number taken
  * in *taken (synchronises). */
 int lag2d_run(lag2d_ctx *ctx, int itime0, int nsteps, int *taken);
 
@@ -95,8 +125,13 @@ int lag2d_rhs(lag2d_ctx *ctx, double *rhsel);
 int lag2d_download(lag2d_ctx *ctx, double *coord, double *geoel, double *unkno,
                    double *ustar);
 
-/* Driver scalars after the last step (module constant dtfix, tacru). */
+/* Driver scalars after the last step (module constant dtfix, tacru); of
+ * run 0 for an ensemble. */
 int lag2d_get_time(lag2d_ctx *ctx, double *dtfix, double *tacru);
+
+/* dt and time of every run of an ensemble (arrays of nrun; one entry for a
+ * single problem).  No counterpart in the reference. */
+int lag2d_get_run_times(lag2d_ctx *ctx, double *dtfix, double *tacru);
 
 /* Scan the resident state for non-finite DOFs or non-positive volumes:
  * returns LAG2D_ENONFINITE / LAG2D_EVOLUME (the reference has no such check

@@ -34,6 +34,7 @@ class LagCase:
     dtfix: float = 0.0
     tacru: float = 0.0
     itime: int = 0
+    rho0: np.ndarray | None = None  # (ncell,) initial density per cell, when known
 
     @property
     def ncell(self):

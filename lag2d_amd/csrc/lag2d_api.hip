@@ -64,8 +64,15 @@ int lag2d_ctx::alloc(T **p, size_t n)
     return rc;
 }
 
-extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac, const int *ipqua,
-                            const int *bface, const int *esuv1, const int *esuv2, lag2d_ctx **out)
+// Runs of an ensemble (one run for lag2d_create).
+struct RunSpec {
+    int nrun = 1;
+    std::vector<int> coff, noff;
+    std::vector<double> tend, dtfix, tacru;
+};
+
+static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac, const int *ipqua,
+                       const int *bface, const int *esuv1, const int *esuv2, const RunSpec &rs, lag2d_ctx **out)
 {
     if (!cfg || !out || !ipqua || !bface || !esuv1 || !esuv2) return fail(LAG2D_EINVAL, "null argument");
     if (nquad <= 0 || npoin <= 0 || nbfac < 0) return fail(LAG2D_EINVAL, "bad sizes");
@@ -94,6 +101,8 @@ extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int n
     };
     if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess)
         return bad(fail(LAG2D_EDEVICE, "hipStreamCreate failed"));
+    const int nrun = rs.nrun;
+    x->nrun = nrun;
 
     // ---- host-side mesh preprocessing (0-based, SoA)
     std::vector<int> ipq((size_t)nv * nquad);
@@ -161,7 +170,7 @@ extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int n
     int *t_ipq, *t_e1, *t_e2, *t_n2c, *t_n2co;
     uint8_t *t_fl, *t_col;
     const size_t nU = (size_t)4 * c.ndegr * nquad;
-    x->nblk_dt = std::min(nblocks(nquad), 1024);
+    x->nblk_dt = nrun == 1 ? std::min(nblocks(nquad), 1024) : 4;
     if (q2) x->nrec = 160;
     if ((rc = x->alloc(&t_ipq, ipq.size())) || (rc = x->alloc(&t_e1, e1.size())) ||
         (rc = x->alloc(&t_e2, e2.size())) || (rc = x->alloc(&t_n2c, n2c.size())) ||
@@ -174,9 +183,38 @@ extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int n
         (rc = x->alloc(&d.usx, (size_t)npoin)) || (rc = x->alloc(&d.usy, (size_t)npoin)) ||
         (rc = x->alloc(&d.cavg, (size_t)5 * nquad)) || (rc = x->alloc(&d.frame, (size_t)4 * nquad)) ||
         (rc = x->alloc(&d.nmm, (size_t)10 * npoin)) || (rc = x->alloc(&d.lim, (size_t)9 * nquad)) ||
-        (rc = x->alloc(&d.rec, (size_t)x->nrec * nquad)) || (rc = x->alloc(&d.red, (size_t)2 * x->nblk_dt)) ||
-        (rc = x->alloc(&d.scal, 4)))
+        (rc = x->alloc(&d.rec, (size_t)x->nrec * nquad)) ||
+        (rc = x->alloc(&d.red, (size_t)2 * nrun * x->nblk_dt)) || (rc = x->alloc(&d.scal, (size_t)2 * nrun + 2)))
         return bad(rc);
+    d.nrun = nrun;
+    if (nrun > 1) {
+        // run index of every cell and node
+        std::vector<int> runc(nquad), runn(npoin);
+        for (int r = 0; r < nrun; ++r) {
+            for (int ie = rs.coff[r]; ie < rs.coff[r + 1]; ++ie) runc[ie] = r;
+            for (int p = rs.noff[r]; p < rs.noff[r + 1]; ++p) runn[p] = r;
+        }
+        for (int ie = 0; ie < nquad; ++ie)
+            for (int k = 0; k < nv; ++k) {
+                const int p = ipq[(size_t)k * nquad + ie];
+                if (runn[p] != runc[ie]) return bad(fail(LAG2D_EINVAL, "a cell uses a node of another run"));
+            }
+        int *t_runc, *t_runn, *t_coff, *t_act;
+        double *t_tend;
+        if ((rc = x->alloc(&t_runc, (size_t)nquad)) || (rc = x->alloc(&t_runn, (size_t)npoin)) ||
+            (rc = x->alloc(&t_coff, (size_t)nrun + 1)) || (rc = x->alloc(&t_act, (size_t)nrun)) ||
+            (rc = x->alloc(&t_tend, (size_t)nrun)))
+            return bad(rc);
+        std::vector<int> ones(nrun, 1);
+        if (hipMemcpy(t_runc, runc.data(), runc.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(t_runn, runn.data(), runn.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(t_coff, rs.coff.data(), rs.coff.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(t_act, ones.data(), ones.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(t_tend, rs.tend.data(), rs.tend.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return bad(fail(LAG2D_EDEVICE, "ensemble table upload"));
+        d.runc = t_runc; d.runn = t_runn; d.coff = t_coff; d.act = t_act; d.tendv = t_tend;
+        x->tend_host = rs.tend;
+    }
     if (q2) {
         // face neighbours for the WENO stencils: neighbour id, or nc + f on
         // a boundary face (the reference keeps a ghost-cell id there)
@@ -221,12 +259,47 @@ extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int n
     if (e == hipSuccess) e = hipMemcpy(t_fl, nflag.data(), npoin, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(t_col, ncol.data(), npoin, hipMemcpyHostToDevice);
     if (e != hipSuccess) return bad(fail(LAG2D_EDEVICE, std::string("mesh upload: ") + hipGetErrorString(e)));
-    double sc[4] = {c.dtfix, c.tacru, 0.0, 0.0};
-    if (hipMemcpy(d.scal, sc, sizeof(sc), hipMemcpyHostToDevice) != hipSuccess)
+    std::vector<double> sc((size_t)2 * nrun + 2, 0.0);
+    for (int r = 0; r < nrun; ++r) {
+        sc[2 * r] = nrun == 1 ? c.dtfix : rs.dtfix[r];
+        sc[2 * r + 1] = nrun == 1 ? c.tacru : rs.tacru[r];
+    }
+    if (hipMemcpy(d.scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return bad(fail(LAG2D_EDEVICE, "scalar upload"));
     x->ipq_host = std::move(ipq);
     *out = x;
     return 0;
+}
+
+extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac, const int *ipqua,
+                            const int *bface, const int *esuv1, const int *esuv2, lag2d_ctx **out)
+{
+    return create_impl(cfg, nquad, npoin, nbfac, ipqua, bface, esuv1, esuv2, RunSpec(), out);
+}
+
+extern "C" int lag2d_create_ensemble(const lag2d_config *cfg, int nrun, const int *cell_off, const int *node_off,
+                                     const double *tend, const double *dtfix, const double *tacru, int nquad,
+                                     int npoin, int nbfac, const int *ipqua, const int *bface, const int *esuv1,
+                                     const int *esuv2, lag2d_ctx **out)
+{
+    if (!cfg || !cell_off || !node_off || !tend || !dtfix || !tacru) return fail(LAG2D_EINVAL, "null argument");
+    if (nrun < 1) return fail(LAG2D_EINVAL, "nrun < 1");
+    if (cfg->ncurv != 1) return fail(LAG2D_EUNSUPPORTED, "ensembles run the curved P2 path (ncurv=1)");
+    if (cell_off[0] != 0 || cell_off[nrun] != nquad || node_off[0] != 0 || node_off[nrun] != npoin)
+        return fail(LAG2D_EINVAL, "run offsets must start at 0 and end at nquad / npoin");
+    RunSpec rs;
+    rs.nrun = nrun;
+    for (int r = 0; r < nrun; ++r) {
+        if (cell_off[r + 1] <= cell_off[r] || node_off[r + 1] <= node_off[r])
+            return fail(LAG2D_EINVAL, "empty run or decreasing offsets");
+        if (!(tend[r] > 0.0)) return fail(LAG2D_EINVAL, "tend must be positive");
+    }
+    rs.coff.assign(cell_off, cell_off + nrun + 1);
+    rs.noff.assign(node_off, node_off + nrun + 1);
+    rs.tend.assign(tend, tend + nrun);
+    rs.dtfix.assign(dtfix, dtfix + nrun);
+    rs.tacru.assign(tacru, tacru + nrun);
+    return create_impl(cfg, nquad, npoin, nbfac, ipqua, bface, esuv1, esuv2, rs, out);
 }
 
 extern "C" int lag2d_upload(lag2d_ctx *x, const double *coord, const double *coold, const double *geoel,
@@ -298,7 +371,7 @@ void lag2d_ctx::toc(int k)
 const char *const lag2d_kernel_names[KID_COUNT] = {
     "k_cellavg_q1", "k_nodeminmax", "k_limit_corner_q1", "k_node_q1", "k_cellrhs_q1", "k_dt_partial",
     "k_dt_final",   "k_update",     "k_nodemove",        "k_cellvol",
-    "k_q2_prep",    "k_q2_weno",    "k_q2_corner",       "k_node_q2", "k_cellrhs_q2"};
+    "k_q2_prep",    "k_q2_weno",    "k_q2_corner",       "k_node_q2", "k_cellrhs_q2", "k_run_flags"};
 
 static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
 
@@ -326,6 +399,7 @@ static int step_impl(lag2d_ctx *x, int itime)
     Dev &d = x->d;
     const int nc = x->nc, np = x->np;
     const int ns = x->cfg.nstag;
+    if (x->nrun > 1) LAUNCH(KID_RUN_FLAGS, k_run_flags, nblocks(x->nrun), d);
     for (int s = 0; s < ns; ++s) {
         const double alfa = RKLAG[ns - 2][s];
         Dev ds = d;
@@ -337,8 +411,8 @@ static int step_impl(lag2d_ctx *x, int itime)
         int rc = launch_rhs(x, ds, Uin, alfa, s > 0);
         if (rc) return rc;
         if (s == 0) {
-            LAUNCH(KID_DT_PARTIAL, k_dt_partial, x->nblk_dt, ds, Uin, x->nblk_dt);
-            LAUNCH(KID_DT_FINAL, k_dt_final, 1, d, itime, x->nblk_dt);
+            LAUNCH(KID_DT_PARTIAL, k_dt_partial, x->nrun * x->nblk_dt, ds, Uin, x->nblk_dt);
+            LAUNCH(KID_DT_FINAL, k_dt_final, x->nrun, d, itime, x->nblk_dt);
             LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
         }
         LAUNCH(KID_NODEMOVE, k_nodemove, nblocks(np), d, alfa, s == 0 ? d.xn : d.xs, s == 0 ? d.yn : d.ys);
@@ -385,10 +459,20 @@ extern "C" int lag2d_run(lag2d_ctx *x, int itime0, int nsteps, int *taken)
                                              "reference stops on (WENO.f90:7699) at step " +
                                                  std::to_string(itime0 + k));
         }
-        if (watch) {
+        if (watch && x->nrun == 1) {
             HIPCHK(hipMemcpyAsync(sc, x->d.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, x->stream));
             HIPCHK(hipStreamSynchronize(x->stream));
             if (std::fabs(sc[1] - x->cfg.tend) / x->cfg.tend <= 1e-11) break;
+        } else if (x->nrun > 1) {
+            // an ensemble ends when every run has reached its end time
+            std::vector<double> all((size_t)2 * x->nrun);
+            HIPCHK(hipMemcpyAsync(all.data(), x->d.scal, all.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  x->stream));
+            HIPCHK(hipStreamSynchronize(x->stream));
+            bool done = true;
+            for (int r = 0; r < x->nrun && done; ++r)
+                done = std::fabs(all[2 * r + 1] - x->tend_host[r]) / x->tend_host[r] <= 1e-11;
+            if (done) break;
         }
     }
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -457,6 +541,36 @@ extern "C" int lag2d_download(lag2d_ctx *x, double *coord, double *geoel, double
                 bool live = (k == 2 || k == 4 || k == 5);
                 geoel[k + 24 * (size_t)ie] = live ? h[(size_t)k * nc + ie] : x->geo_host[k + 24 * (size_t)ie];
             }
+    }
+    return 0;
+}
+
+extern "C" int lag2d_set_initial_density(lag2d_ctx *x, const double *rho)
+{
+    if (!x || !rho) return fail(LAG2D_EINVAL, "null argument");
+    if (x->cfg.ncurv != 1) return fail(LAG2D_EUNSUPPORTED, "the initial density is used by the curved P2 path");
+    Dev &d = x->d;
+    if (!d.rhoi) {
+        double *t;
+        int rc = x->alloc(&t, (size_t)x->nc);
+        if (rc) return rc;
+        d.rhoi = t;
+    }
+    HIPCHK(hipMemcpy((void *)d.rhoi, rho, (size_t)x->nc * sizeof(double), hipMemcpyHostToDevice));
+    int rc = upload_extra(*x);  // initial sub-cell masses from this density
+    if (rc) return fail(rc, "initial sub-cell masses");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+extern "C" int lag2d_get_run_times(lag2d_ctx *x, double *dtfix, double *tacru)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    std::vector<double> sc((size_t)2 * x->nrun);
+    HIPCHK(hipMemcpy(sc.data(), x->d.scal, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int r = 0; r < x->nrun; ++r) {
+        if (dtfix) dtfix[r] = sc[2 * r];
+        if (tacru) tacru[r] = sc[2 * r + 1];
     }
     return 0;
 }

@@ -10,14 +10,17 @@ namespace lag2d {
 
 // Per-block minima of the CFL time (edge length / sound speed) and of the
 // volume-change time V/|dV/dt| over the quads, with dV/dt the degree-0
-// tau right-hand side before it is scaled by dt.
-__global__ void __launch_bounds__(LAG2D_BLOCK) k_dt_partial(Dev d, const double *__restrict__ Uin, int nblk)
+// tau right-hand side before it is scaled by dt.  Block b covers part
+// b % nbpr of run b / nbpr (one run: nbpr blocks over all cells).
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_dt_partial(Dev d, const double *__restrict__ Uin, int nbpr)
 {
     __shared__ double s_cfl[LAG2D_BLOCK], s_vol[LAG2D_BLOCK];
     const int nc = d.nc, nd = d.nd;
     const double eps = 1.0e-6, g1 = d.gamlg - 1.0;
+    const int r = blockIdx.x / nbpr, part = blockIdx.x - nbpr * r;
+    const int c0 = d.nrun > 1 ? d.coff[r] : 0, c1 = d.nrun > 1 ? d.coff[r + 1] : nc;
     double mcfl = 1e10, mvol = 1e10;
-    for (int ie = blockIdx.x * blockDim.x + threadIdx.x; ie < nc; ie += gridDim.x * blockDim.x) {
+    for (int ie = c0 + part * blockDim.x + threadIdx.x; ie < c1; ie += nbpr * blockDim.x) {
         double xp[9], yp[9];
         const int nv = d.ncurv == 0 ? 4 : 9;
         for (int k = 0; k < nv; ++k) {
@@ -57,17 +60,20 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_dt_partial(Dev d, const double 
     }
     if (threadIdx.x == 0) {
         d.red[blockIdx.x] = s_cfl[0];
-        d.red[nblk + blockIdx.x] = s_vol[0];
+        d.red[gridDim.x + blockIdx.x] = s_vol[0];
     }
 }
 
-__global__ void __launch_bounds__(LAG2D_BLOCK) k_dt_final(Dev d, int itime, int nblk)
+// One block per run: the run's dt (Getrhslag.f90:20705-20709, clamped to its
+// end time, :1768-1770) and time.  Minima do not depend on the order.
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_dt_final(Dev d, int itime, int nbpr)
 {
     __shared__ double s_cfl[LAG2D_BLOCK], s_vol[LAG2D_BLOCK];
+    const int r = blockIdx.x, nall = gridDim.x * nbpr;
     double mcfl = 1e10, mvol = 1e10;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        mcfl = dmin(mcfl, d.red[b]);
-        mvol = dmin(mvol, d.red[nblk + b]);
+    for (int b = threadIdx.x; b < nbpr; b += blockDim.x) {
+        mcfl = dmin(mcfl, d.red[r * nbpr + b]);
+        mvol = dmin(mvol, d.red[nall + r * nbpr + b]);
     }
     s_cfl[threadIdx.x] = mcfl;
     s_vol[threadIdx.x] = mvol;
@@ -79,18 +85,29 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_dt_final(Dev d, int itime, int 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double dt = d.scal[0], t = d.scal[1];
+    if (threadIdx.x == 0 && run_active(d, r)) {
+        const double tend = d.nrun > 1 ? d.tendv[r] : d.tend;
+        double dt = d.scal[2 * r], t = d.scal[2 * r + 1];
         if (d.ndt == 1) {
             if (itime == 1)
                 dt = dmin(d.cfl * s_cfl[0], 0.1 * s_vol[0]);
             else
                 dt = dmin(dmin(d.cfl * s_cfl[0], 0.1 * s_vol[0]), 1.01 * dt);
-            if (t + dt > d.tend) dt = d.tend - t;
+            if (t + dt > tend) dt = tend - t;
         }
-        d.scal[0] = dt;
-        d.scal[1] = t + dt;
+        d.scal[2 * r] = dt;
+        d.scal[2 * r + 1] = t + dt;
     }
+}
+
+// Start of a step of an ensemble: a run advances unless it has reached its
+// end time (the driver's exit test, dgflo2d.f90:458).
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_run_flags(Dev d)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d.nrun) return;
+    const double t = d.scal[2 * r + 1], tend = d.tendv[r];
+    d.act[r] = (fabs(t - tend) / tend <= 1e-11) ? 0 : 1;
 }
 
 // Mass-matrix solve and SSP-RK blend (Getrhslag.f90:1822-1904):
@@ -102,7 +119,13 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_update(Dev d, double alfa)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 4 * nd * nc) return;
     const int ie = t % nc, qi = t / nc, q = qi / nd, id = qi - nd * q;
-    const double dt = d.scal[0];
+    const int run = cell_run(d, ie);
+    const int o = (q * nd + id) * nc + ie;
+    if (!run_active(d, run)) {
+        d.Us[o] = d.Un[o];
+        return;
+    }
+    const double dt = d.scal[2 * run];
     // row id of the inverse mass matrix: degree 0 decoupled (1/mass), the
     // rest packed as the upper triangle row-wise
     double mrow[6];
@@ -127,7 +150,6 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_update(Dev d, double alfa)
     }
     double s4 = 0.0;
     for (int k = 0; k < nd; ++k) s4 = s4 + mrow[k] * (dt * d.R[(q * nd + k) * nc + ie]);
-    const int o = (q * nd + id) * nc + ie;
     d.Us[o] = alfa * d.Un[o] + (1.0 - alfa) * (d.Us[o] + s4);
 }
 
@@ -137,7 +159,13 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_nodemove(Dev d, double alfa, co
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.np) return;
-    const double dt = d.scal[0];
+    const int run = node_run(d, p);
+    if (!run_active(d, run)) {
+        d.xs[p] = d.xn[p];
+        d.ys[p] = d.yn[p];
+        return;
+    }
+    const double dt = d.scal[2 * run];
     double x = alfa * d.xn[p] + (1.0 - alfa) * (xsrc[p] + dt * d.usx[p]);
     double y = alfa * d.yn[p] + (1.0 - alfa) * (ysrc[p] + dt * d.usy[p]);
     d.xs[p] = x;

@@ -79,7 +79,23 @@ struct Dev {
     const int *esqua;        // face neighbours [4][nc]; >= nc marks a boundary face
     int weno_strict;         // 1: report cells the reference's stencil builder stops on
     int *wenobad;            // count of such cells
+    // ensemble of independent runs (nrun > 1): run r owns cells
+    // [coff[r], coff[r+1]); scal[2r], scal[2r+1] are its dt and time
+    int nrun;
+    const int *runc;         // run of each cell
+    const int *runn;         // run of each node
+    const int *coff;         // cell offsets of the runs [nrun+1]
+    const double *tendv;     // end time of each run
+    int *act;                // 1: the run advances in this step
+    // initial density per cell, for initial states other than the
+    // reference's built-in problems (null: the density of ncase)
+    const double *rhoi;
 };
+
+__device__ __forceinline__ int cell_run(const Dev &d, int ie) { return d.nrun > 1 ? d.runc[ie] : 0; }
+__device__ __forceinline__ int node_run(const Dev &d, int p) { return d.nrun > 1 ? d.runn[p] : 0; }
+// A run that has reached its end time is frozen: later steps leave its state unchanged.
+__device__ __forceinline__ bool run_active(const Dev &d, int r) { return d.nrun <= 1 || d.act[r] != 0; }
 
 // Logical block index for an XCD-aware order.  Workgroups are dispatched to
 // the 8 XCDs round-robin (block b runs on XCD b % 8, each XCD with its own

@@ -11,7 +11,7 @@
 enum {
     KID_CELLAVG_Q1 = 0, KID_NODEMINMAX, KID_LIMIT_Q1, KID_NODE_Q1, KID_CELLRHS_Q1, KID_DT_PARTIAL,
     KID_DT_FINAL, KID_UPDATE, KID_NODEMOVE, KID_CELLVOL,
-    KID_Q2_PREP, KID_Q2_WENO, KID_Q2_CORNER, KID_NODE_Q2, KID_CELLRHS_Q2, KID_COUNT
+    KID_Q2_PREP, KID_Q2_WENO, KID_Q2_CORNER, KID_NODE_Q2, KID_CELLRHS_Q2, KID_RUN_FLAGS, KID_COUNT
 };
 extern const char *const lag2d_kernel_names[KID_COUNT];
 
@@ -23,7 +23,9 @@ struct lag2d_ctx {
     lag2d_config cfg{};
     int nc = 0, np = 0, nb = 0, nd = 0, nvq = 0, nmatr = 0;
     int nrec = 40;          // corner-face record doubles per cell
-    int nblk_dt = 1;
+    int nblk_dt = 1;        // blocks per run of the dt reduction
+    int nrun = 1;           // runs of an ensemble
+    std::vector<double> tend_host;
     hipStream_t stream = nullptr;
     lag2d::Dev d{};
     double *A = nullptr;    // inverse mass matrix (SoA)

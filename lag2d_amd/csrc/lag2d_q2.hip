@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict
         v1 = v1 + sv[cl][g];
     }
     const double thr = -3.0 * log10(4.0) - 4.0;
-    const bool troubled = !(log10(v2 / v1) <= thr);
+    const bool troubled = !(log10(v2 / v1) <= thr) && run_active(d, cell_run(d, ie));
     d.flag[ie] = troubled ? 1 : 0;
     if (troubled) {
         int pos = atomicAdd(d.tcount, 1);
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
                 }
                 double xcr, ycr;
                 map9(xi, yi, S.gc[s][0], S.gc[s][1], &xcr, &ycr);
-                const double rho = rho0(d.ncase, xcr, ycr);
+                const double rho = d.rhoi ? d.rhoi[S.ielaj[s]] : rho0(d.ncase, xcr, ycr);
                 const double r = PG5[i], sg = PG5[j], wi = WG5[i] * WG5[j];
                 double J[4];
                 jac9(xi, yi, r, sg, J);
@@ -1040,6 +1040,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= d.np) return;
+    if (!run_active(d, node_run(d, p))) return;  // frozen run: keep u*
     const int nc = d.nc;
     double a11 = 0.0, a21 = 0.0, a12 = 0.0, a22 = 0.0, mu1 = 0.0, mu2 = 0.0, sn1 = 0.0, sn2 = 0.0;
     for (int k = d.n2c_off[p]; k < d.n2c_off[p + 1]; ++k) {
@@ -1255,7 +1256,12 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         else ul = 0.0 * uo + 1.0 * uqi;
     }
     if (apply_update) {
-        const double dt = d.scal[0];
+        const int run = cell_run(d, ie);
+        if (!run_active(d, run)) {  // frozen run: keep U^n
+            UD(d.Us, q, id, ie) = un;
+            return;
+        }
+        const double dt = d.scal[2 * run];
         double s4 = 0.0;
         for (int kk = 0; kk < 6; ++kk) s4 = s4 + mrow[kk] * (dt * sres[cl][q][kk]);
         UD(d.Us, q, id, ie) = alfa * un + (1.0 - alfa) * (ul + s4);
@@ -1281,7 +1287,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_submass(Dev d)
     for (int k = 0; k < 12; ++k) map9(xi, yi, pc[k][0], pc[k][1], &xc[k], &yc[k]);
     double xcc, ycc;
     map9(xi, yi, 0.0, 0.0, &xcc, &ycc);
-    const double rho = (d.ncase == 6) ? rho0(6, xcc, ycc) : 1.0;
+    const double rho = d.rhoi ? d.rhoi[ie] : ((d.ncase == 6) ? rho0(6, xcc, ycc) : 1.0);
     for (int isg = 0; isg < 4; ++isg) {
         double xs[9], ys[9];
         for (int k = 0; k < 4; ++k) {

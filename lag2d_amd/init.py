@@ -154,7 +154,7 @@ def initialise(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) -> La
             unkno[e, 3, 0] = energy / mass[e]
     return LagCase(ctl=ctl, ipqua=ipq, bface=bf, esuv1=e1, esuv2=e2, coord=coord.copy(),
                    coold=coord.copy(), geoel=geoel, amatr=amatr, unkno=unkno,
-                   ustar=np.zeros_like(coord))
+                   ustar=np.zeros_like(coord), rho0=rho.copy())
 
 
 def sedov_case(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None) -> LagCase:

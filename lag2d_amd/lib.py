@@ -15,7 +15,8 @@ from .case import LagCase
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liblag2d.so")
 
-EXPORTS = ("lag2d_create", "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
+EXPORTS = ("lag2d_create", "lag2d_create_ensemble", "lag2d_get_run_times", "lag2d_set_initial_density",
+           "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
            "lag2d_destroy", "lag2d_last_error")
 
@@ -49,6 +50,10 @@ def load():
     lib = C.CDLL(LIBPATH)
     lib.lag2d_create.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip,
                                  C.POINTER(C.c_void_p)]
+    lib.lag2d_create_ensemble.argtypes = [C.POINTER(Config), C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_int,
+                                          C.c_int, _ip, _ip, _ip, _ip, C.POINTER(C.c_void_p)]
+    lib.lag2d_get_run_times.argtypes = [C.c_void_p, _dp, _dp]
+    lib.lag2d_set_initial_density.argtypes = [C.c_void_p, _dp]
     lib.lag2d_upload.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.lag2d_step.argtypes = [C.c_void_p, C.c_int]
     lib.lag2d_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip]
@@ -87,7 +92,7 @@ def config_of(case: LagCase, strict: bool = False) -> Config:
 class Lag2D:
     """One problem resident on the GPU, driven through the C ABI."""
 
-    def __init__(self, case: LagCase, strict: bool = False):
+    def __init__(self, case: LagCase, strict: bool = False, rho0=None):
         lib = load()
         self.case = case
         self.itime = case.itime
@@ -105,6 +110,9 @@ class Lag2D:
                                 _p(e1, _ip), _p(e2, _ip), C.byref(h)))
         self.h = h
         self.upload(case)
+        if rho0 is not None:
+            r = np.ascontiguousarray(rho0, dtype=np.float64)
+            _check(lib.lag2d_set_initial_density(self.h, _p(r)))
 
     def upload(self, case: LagCase):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in

@@ -110,47 +110,33 @@ def quad_q2(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1, bulge=0.05) -> Mesh:
         d = rng.uniform(-jitter * h, jitter * h, size=(ny - 1, nx - 1, 2))
         P[2:-2:2, 2:-2:2, :] += d
     # mid-nodes of horizontal edges (even J, odd I) and vertical edges (odd J, even I)
-    for J in range(0, Ny, 2):
-        for I in range(1, Nx, 2):
-            a, b = P[J, I - 1], P[J, I + 1]
-            P[J, I] = 0.5 * (a + b)
-    for J in range(1, Ny, 2):
-        for I in range(0, Nx, 2):
-            a, b = P[J - 1, I], P[J + 1, I]
-            P[J, I] = 0.5 * (a + b)
+    P[0::2, 1::2] = 0.5 * (P[0::2, 0:-1:2] + P[0::2, 2::2])
+    P[1::2, 0::2] = 0.5 * (P[0:-1:2, 0::2] + P[2::2, 0::2])
     if rng is not None:
-        for J in range(0, Ny, 2):
-            for I in range(1, Nx, 2):
-                if J in (0, Ny - 1):
-                    continue
-                t = P[J, I + 1] - P[J, I - 1]
-                n = np.array([-t[1], t[0]]) / np.hypot(t[0], t[1])
-                P[J, I] += rng.uniform(-bulge * h, bulge * h) * n
-        for J in range(1, Ny, 2):
-            for I in range(0, Nx, 2):
-                if I in (0, Nx - 1):
-                    continue
-                t = P[J + 1, I] - P[J - 1, I]
-                n = np.array([-t[1], t[0]]) / np.hypot(t[0], t[1])
-                P[J, I] += rng.uniform(-bulge * h, bulge * h) * n
-    for j in range(ny):
-        for i in range(nx):
-            I, J = 2 * i, 2 * j
-            c = P[J, I] + P[J, I + 2] + P[J + 2, I + 2] + P[J + 2, I]
-            m = P[J, I + 1] + P[J + 1, I + 2] + P[J + 2, I + 1] + P[J + 1, I]
-            P[J + 1, I + 1] = -0.25 * c + 0.5 * m
+        # interior mid-nodes move along the edge normal; draws in row-major
+        # (J, I) order, horizontal edges first
+        def bulge_nodes(rows, cols, t):
+            if t.size == 0:
+                return
+            hyp = np.hypot(t[..., 0], t[..., 1])
+            u = rng.uniform(-bulge * h, bulge * h, size=t.shape[:2])
+            P[rows, cols, 0] += u * (-t[..., 1] / hyp)
+            P[rows, cols, 1] += u * (t[..., 0] / hyp)
+
+        bulge_nodes(slice(2, Ny - 2, 2), slice(1, Nx, 2), P[2:Ny - 2:2, 2::2] - P[2:Ny - 2:2, 0:-1:2])
+        bulge_nodes(slice(1, Ny, 2), slice(2, Nx - 2, 2), P[2::2, 2:Nx - 2:2] - P[0:-1:2, 2:Nx - 2:2])
+    c = P[0:-1:2, 0:-1:2] + P[0:-1:2, 2::2] + P[2::2, 2::2] + P[2::2, 0:-1:2]
+    m = P[0:-1:2, 1::2] + P[1::2, 2::2] + P[2::2, 1::2] + P[1::2, 0:-1:2]
+    P[1::2, 1::2] = -0.25 * c + 0.5 * m
     coord = P.reshape(-1, 2).copy()
 
     def nid(I, J):
         return I + Nx * J + 1
 
-    ipq = np.empty((nx * ny, 9), dtype=np.int32)
-    for j in range(ny):
-        for i in range(nx):
-            I, J = 2 * i, 2 * j
-            ipq[i + nx * j] = (nid(I, J), nid(I + 2, J), nid(I + 2, J + 2), nid(I, J + 2),
-                               nid(I + 1, J), nid(I + 2, J + 1), nid(I + 1, J + 2), nid(I, J + 1),
-                               nid(I + 1, J + 1))
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    I, J = (2 * ii).ravel(), (2 * jj).ravel()
+    ipq = np.stack([nid(I, J), nid(I + 2, J), nid(I + 2, J + 2), nid(I, J + 2), nid(I + 1, J), nid(I + 2, J + 1),
+                    nid(I + 1, J + 2), nid(I, J + 1), nid(I + 1, J + 1)], axis=1).astype(np.int32)
     return Mesh(coord, ipq, _boundary_faces(nid, nx, ny, True), 1, nx, ny)
 
 
