@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -267,6 +268,14 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
     if (hipMemcpy(d.scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return bad(fail(LAG2D_EDEVICE, "scalar upload"));
     x->ipq_host = std::move(ipq);
+    if (const char *v = std::getenv("LAG2D_CORNER_VARIANT")) {
+        const int k = std::atoi(v);
+        if (k == 1 || k == 2) x->corner_variant = k;
+    }
+    if (const char *v = std::getenv("LAG2D_RHS_VARIANT")) {
+        const int k = std::atoi(v);
+        if (k == 1 || k == 2) x->rhs_variant = k;
+    }
     *out = x;
     return 0;
 }

@@ -90,6 +90,8 @@ struct Dev {
     // initial density per cell, for initial states other than the
     // reference's built-in problems (null: the density of ncase)
     const double *rhoi;
+    // shape-function derivatives at the curved path's fixed quadrature points
+    const double *shp;
 };
 
 __device__ __forceinline__ int cell_run(const Dev &d, int ie) { return d.nrun > 1 ? d.runc[ie] : 0; }

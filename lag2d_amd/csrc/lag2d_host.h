@@ -25,6 +25,10 @@ struct lag2d_ctx {
     int nrec = 40;          // corner-face record doubles per cell
     int nblk_dt = 1;        // blocks per run of the dt reduction
     int nrun = 1;           // runs of an ensemble
+    int corner_variant = 0; // curved-path corner kernel: 0 = not tuned yet, 1 = lanes per cell, 2 = lane per sub-cell
+    float corner_ms[2] = {0.f, 0.f};  // tuning timings of the two variants
+    int rhs_variant = 0;    // curved-path cell RHS: 0 = not tuned yet, 1 = 8 cells per block, 2 = thread per cell
+    float rhs_ms[2] = {0.f, 0.f};
     std::vector<double> tend_host;
     hipStream_t stream = nullptr;
     lag2d::Dev d{};

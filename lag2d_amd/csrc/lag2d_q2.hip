@@ -1033,6 +1033,214 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
     }
 }
 
+// The same corner kernel organised for throughput on large meshes: one lane
+// per (cell, sub-cell), 64 cells per block.  Each lane sums its sub-cell's
+// 25 quadrature terms in registers in point order, so all lanes of a wave
+// evaluate the same point at the same time; the shape-function derivatives
+// at the quadrature points are read from a table (built by
+// k_q2_shape_tables with the same arithmetic) staged in LDS.  Face slots
+// are shared through LDS.  Results are identical to k_q2_corner's.
+constexpr int CSC_CELLS = 64;
+constexpr int SHP_POINTS = 150;  // 25 Gauss points, 4 x 25 sub-cell points, 25 Lobatto points
+constexpr int SHP_STRIDE = 18;   // dN/dr[9], dN/ds[9]
+
+__global__ void __launch_bounds__(256) k_q2_shape_tables(double *tab)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= SHP_POINTS) return;
+    const double sr[4] = {-1.0, 1.0, 1.0, -1.0}, ss[4] = {-1.0, -1.0, 1.0, 1.0};
+    double r, s;
+    if (k < 25) {
+        r = PG5[k / 5];
+        s = PG5[k % 5];
+    } else if (k < 125) {
+        const int isg = (k - 25) / 25, g = (k - 25) % 25;
+        r = (PG5[g / 5] + sr[isg]) / 2.0;
+        s = (PG5[g % 5] + ss[isg]) / 2.0;
+    } else {
+        r = PL5[(k - 125) / 5];
+        s = PL5[(k - 125) % 5];
+    }
+    double sh[9], dr_[9], ds_[9];
+    shape9(r, s, sh, dr_, ds_);
+    for (int q = 0; q < 9; ++q) {
+        tab[k * SHP_STRIDE + q] = dr_[q];
+        tab[k * SHP_STRIDE + 9 + q] = ds_[q];
+    }
+}
+
+__device__ __forceinline__ void jac9_tab(const double *xp, const double *yp, const double *t, double J[4])
+{
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        dxdr = dxdr + t[k] * xp[k];
+        dxds = dxds + t[9 + k] * xp[k];
+        dydr = dydr + t[k] * yp[k];
+        dyds = dyds + t[9 + k] * yp[k];
+    }
+    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
+}
+
+__global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
+{
+    __shared__ double stab[125 * SHP_STRIDE];
+    __shared__ double sgq[CSC_CELLS][16 * 3 + 1];
+    for (int k = threadIdx.x; k < 125 * SHP_STRIDE; k += 256) stab[k] = d.shp[k];
+    const int cl = threadIdx.x >> 2, isg = threadIdx.x & 3;
+    const int ie = blockIdx.x * CSC_CELLS + cl;
+    const int nc = d.nc;
+    const bool active = ie < nc;
+    const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
+    double xp[9], yp[9];
+    double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
+    double a = 0.0, masel = 0.0, volel = 0.0;
+    if (active) {
+        load9(d, d.xs, d.ys, ie, xp, yp);
+        rc = d.G[0 * nc + ie];
+        sc = d.G[1 * nc + ie];
+        gg0 = d.G[18 * nc + ie];
+        gg1 = d.G[19 * nc + ie];
+        gg2 = d.G[20 * nc + ie];
+        // ---- face slots 4 isg .. 4 isg + 3: 8-11 interior half-edges, others boundary points
+        for (int u = 0; u < 4; ++u) {
+            const int slot = 4 * isg + u;
+            double gx, gy, ga;
+            if (slot >= 8 && slot < 12) {
+                const int k = slot - 8;
+                const double pq[4][2] = {{0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
+                double xv, yv;
+                map9(xp, yp, pq[k][0], pq[k][1], &xv, &yv);
+                const double fx[3] = {xp[4 + k], xp[8], xv};
+                const double fy[3] = {yp[4 + k], yp[8], yv};
+                const double r = -1.0;
+                const double dd[3] = {-0.5 + r, 0.5 + r, -2.0 * r};
+                double dxdr = 0.0, dydr = 0.0;
+                for (int i = 0; i < 3; ++i) {
+                    dxdr = dxdr + dd[i] * fx[i];
+                    dydr = dydr + dd[i] * fy[i];
+                }
+                const double djaco = sqrt(dxdr * dxdr + dydr * dydr);
+                const double anx = -dydr, any = dxdr;
+                gx = anx / sqrt(anx * anx + any * any);
+                gy = any / sqrt(anx * anx + any * any);
+                ga = djaco * 2.0;
+            } else {
+                const int ig = slot < 8 ? slot : slot - 4;
+                const double pi[12][2] = {{-1, -1}, {1, -1}, {1, -1}, {1, 1}, {1, 1}, {-1, 1},
+                                          {-1, 1},  {-1, -1}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
+                const double vn[12][2] = {{0, -1}, {0, -1}, {1, 0}, {1, 0}, {0, 1}, {0, 1},
+                                          {-1, 0}, {-1, 0}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
+                double J[4];
+                jac9(xp, yp, pi[ig][0], pi[ig][1], J);
+                const double c11 = J[3], c12 = -J[2], c21 = -J[1], c22 = J[0];
+                const double anx = c11 * vn[ig][0] + c12 * vn[ig][1];
+                const double any = c21 * vn[ig][0] + c22 * vn[ig][1];
+                const double w1 = anx * 2.0, w2 = any * 2.0;
+                const double farea = sqrt(w1 * w1 + w2 * w2);
+                gx = anx * 2.0 / farea;
+                gy = any * 2.0 / farea;
+                ga = farea;
+            }
+            sgq[cl][3 * slot + 0] = gx;
+            sgq[cl][3 * slot + 1] = gy;
+            sgq[cl][3 * slot + 2] = ga;
+            d.gq[(3 * slot + 0) * nc + ie] = gx;
+            d.gq[(3 * slot + 1) * nc + ie] = gy;
+            d.gq[(3 * slot + 2) * nc + ie] = ga;
+        }
+    }
+    __syncthreads();  // shape table staged, face slots shared
+    if (!active) return;
+    {
+        // ---- sub-cell isg: area of its Q2 map and the mass-weighted mean
+        //      specific volume of the DG polynomial, summed in point order
+        double u0[6];
+        for (int id = 0; id < 6; ++id) u0[id] = UD(d.Ul, 0, id, ie);
+        const double pc[12][2] = {{-0.5, -1.0}, {0.5, -1.0}, {1.0, -.5}, {1.0, .5}, {0.5, 1.0}, {-0.5, 1.0},
+                                  {-1.0, .5},   {-1.0, -.5}, {0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
+        const int ipsc[4][4] = {{0, 8, 11, 7}, {1, 2, 9, 8}, {9, 3, 4, 10}, {11, 10, 5, 6}};
+        double xs[9], ys[9];
+        for (int k = 0; k < 4; ++k) {
+            xs[k] = xp[IPQSG[isg][k]];
+            ys[k] = yp[IPQSG[isg][k]];
+            const int c = ipsc[isg][k];
+            map9(xp, yp, pc[c][0], pc[c][1], &xs[4 + k], &ys[4 + k]);
+        }
+        xs[8] = -0.25 * (xs[0] + xs[1] + xs[2] + xs[3]) + 0.5 * (xs[4] + xs[5] + xs[6] + xs[7]);
+        ys[8] = -0.25 * (ys[0] + ys[1] + ys[2] + ys[3]) + 0.5 * (ys[4] + ys[5] + ys[6] + ys[7]);
+        const double sr[4] = {-1.0, 1.0, 1.0, -1.0}, ss[4] = {-1.0, -1.0, 1.0, 1.0};
+        const double rhomc = u0[0];
+        for (int g = 0; g < 25; ++g) {
+            const int i = g / 5, j = g - 5 * i;
+            double J[4];
+            jac9_tab(xs, ys, &stab[g * SHP_STRIDE], J);
+            a = a + WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
+            const double r = (PG5[i] + sr[isg]) / 2.0;
+            const double s = (PG5[j] + ss[isg]) / 2.0;
+            double b[6];
+            basis6(r, s, rc, sc, gg0, gg1, gg2, b);
+            double t = 0.0;
+            for (int id = 0; id < 6; ++id) t = t + u0[id] * b[id];
+            t = rhomc + 1.0 * (t - rhomc);
+            jac9_tab(xp, yp, &stab[(25 + 25 * isg + g) * SHP_STRIDE], J);
+            const double djaco = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]) * 0.25;
+            masel = masel + 1.0 / t * djaco;
+            volel = volel + djaco;
+        }
+    }
+    const double rsub = 0.0 + (1.0 / a) * d.msub[isg * nc + ie];
+    const double uq = volel / masel;
+    double u[4][6];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 6; ++id) u[q][id] = UD(d.Ul, q, id, ie);
+    const double rhomc = u[0][0], uctr = u[1][0], vctr = u[2][0], ectr = u[3][0];
+    const double rhoct = 1.0 / rhomc;
+    const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+    const double sdctr = sqrt(dmax(eps, d.gamlg * pctr / rhoct));
+    const double dcor = d.cdrho * (rsub - 1.0 / uq);
+    for (int iv = 0; iv < 4; ++iv) {
+        // ---- sub-cell corner (isg, iv)
+        const int ln = IPQSG[isg][iv];
+        double un[4];
+        {
+            double b[6];
+            basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, b);
+            for (int q = 0; q < 4; ++q) un[q] = 0.0;
+            for (int id = 0; id < 6; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
+        }
+        const int p = d.ipq[ln * nc + ie];
+        const double rhovt = 1.0 / un[0];
+        const double rhovsg = rhovt + dcor;
+        const double uv = un[1], vv = un[2], ev = un[3];
+        const double pv = dmax(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)));
+        const double dux = d.usx[p] - uv, duy = d.usy[p] - vv;
+        const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+        for (int f = 0; f < 2; ++f) {
+            const int tt = FNQSG[isg][2 * iv + f];
+            const int sl = (tt > 0 ? tt : -tt) - 1;
+            const double sg = tt > 0 ? 1.0 : -1.0;
+            const double nx = sg * sgq[cl][3 * sl + 0], ny = sg * sgq[cl][3 * sl + 1];
+            const double area = sgq[cl][3 * sl + 2] * WFR[isg][iv][f];
+            const double deltu = fabs(dux * nx + duy * ny);
+            const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
+            double m11;
+            if (a3 < 1.e12) {
+                m11 = mur * area * fabs(1.0);
+            } else {
+                const double nrm = sqrt(dux * dux + duy * duy);
+                m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
+            }
+            const int cf = f + 2 * (iv + 4 * isg);
+            d.rec[(0 * 32 + cf) * nc + ie] = m11;
+            d.rec[(1 * 32 + cf) * nc + ie] = m11 * uv + 0.0 * vv;
+            d.rec[(2 * 32 + cf) * nc + ie] = 0.0 * uv + m11 * vv;
+            d.rec[(3 * 32 + cf) * nc + ie] = -pv * area * nx + 0.0 * area * ny;
+            d.rec[(4 * 32 + cf) * nc + ie] = 0.0 * area * nx + -pv * area * ny;
+        }
+    }
+}
+
 // Nodal solve over the sub-cell corners: contributions gathered in the
 // reference's accumulation order (cell, sub-cell, vertex, face); walls on
 // all three face nodes (Getrhslaggauss.f90:4494-4545).
@@ -1085,11 +1293,17 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
 // staged factor times the basis value, rounded as the reference rounds it),
 // then applies the update.
 constexpr int RHS_CPB = 8;
-// per-cell LDS strides, padded by one double so that the cells a wave reads
-// together fall in different banks
+// LDS strides, chosen against bank conflicts (32 banks of 4 bytes): in the
+// sums a wave reads 8 cells x 6 DOFs -- the 6 lanes of a cell read 6
+// consecutive doubles -- and in the staging 64 lanes write the values of 64
+// different (cell, point) pairs.  Per-point stride 17 and per-cell stride
+// 425 keep both at most 4-way; the basis array's per-cell stride (102) puts
+// the 8 cells' 6-double windows 12 banks apart (3-way); the corner factors
+// are read per cell (broadcast).
 constexpr int RHS_CN = 16 * 12 + 1;  // corner factors
-constexpr int RHS_BN = 16 * 6 + 1;   // basis at the corner's node
-constexpr int RHS_PT = 25 * 16 + 1;  // point values: a[6], c[6], u, v, p, w
+constexpr int RHS_BN = 16 * 6 + 6;   // basis at the corner's node
+constexpr int RHS_PS = 17;           // point values: a[6], c[6], u, v, p, w (+1 pad)
+constexpr int RHS_PT = 25 * RHS_PS;
 
 __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int apply_update)
 {
@@ -1165,7 +1379,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         const double dxdr = J[0], dxds = J[1], dydr = J[2], dyds = J[3];
         const double dbdr[6] = {0.0, 1.0 / 1.0, 0.0, (rr - rc) / 1.0, 0.0, (ss - sc) / 1.0 / 1.0};
         const double dbds[6] = {0.0, 0.0, 1.0 / 1.0, 0.0, (ss - sc) / 1.0, (rr - rc) / 1.0 / 1.0};
-        double *s = &spt[cl][16 * pt];
+        double *s = &spt[cl][RHS_PS * pt];
         for (int id = 0; id < 6; ++id) {
             s[id] = dyds * dbdr[id] + (-dydr) * dbds[id];
             s[6 + id] = (-dxds) * dbdr[id] + dxdr * dbds[id];
@@ -1212,7 +1426,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             }
 #pragma unroll
             for (int k = 0; k < 25; ++k) {
-                const double *s = &pt[16 * k];
+                const double *s = &pt[RHS_PS * k];
                 const double f0 = s[id] * s[12] + s[6 + id] * s[13];
                 r = r - f0 * s[15];
             }
@@ -1227,7 +1441,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             const int ac = (q == 1) ? 0 : 6;
 #pragma unroll
             for (int k = 0; k < 25; ++k) {
-                const double *s = &pt[16 * k];
+                const double *s = &pt[RHS_PS * k];
                 r = r - s[ac + id] * (-s[14]) * s[15];
             }
         } else {
@@ -1240,7 +1454,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             }
 #pragma unroll
             for (int k = 0; k < 25; ++k) {
-                const double *s = &pt[16 * k];
+                const double *s = &pt[RHS_PS * k];
                 r = r - (s[id] * s[12] + s[6 + id] * s[13]) * (-s[14]) * s[15];
             }
         }
@@ -1268,6 +1482,144 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
     } else {
         UD(d.Us, q, id, ie) = ul;
         UD(d.R, q, id, ie) = sres[cl][q][id];
+    }
+}
+
+// The cell RHS organised for throughput on large meshes: one thread per
+// cell (coalesced structure-of-arrays loads, the residual in registers).
+// All lanes of a wave evaluate the same Lobatto point at the same time, so
+// the shape-function derivatives are read from the table as wave-uniform
+// loads.  Every residual entry accumulates in the same order as in
+// k_cellrhs_q2: identical results.
+__global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc;
+    if (ie == 0) *d.tcount = 0;  // WENO list of this stage consumed
+    if (ie >= nc) return;
+    const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
+    const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
+    const double gg0 = d.G[18 * nc + ie], gg1 = d.G[19 * nc + ie], gg2 = d.G[20 * nc + ie];
+    double r[4][6];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 6; ++id) r[q][id] = 0.0;
+    // ---- face integral, corners in (isg, iv) order
+#pragma unroll 1
+    for (int k = 0; k < 16; ++k) {
+        const int isg = k >> 2, iv = k & 3;
+        const int ln = IPQSG[isg][iv];
+        const int p = d.ipq[ln * nc + ie];
+        const double us1 = d.usx[p], us2 = d.usy[p];
+        double nx[2], ny[2], ar[2], fx[2], fy[2];
+        for (int f = 0; f < 2; ++f) {
+            const int tt = FNQSG[isg][2 * iv + f];
+            const int sl = (tt > 0 ? tt : -tt) - 1;
+            const double sg = tt > 0 ? 1.0 : -1.0;
+            nx[f] = sg * d.gq[(3 * sl + 0) * nc + ie];
+            ny[f] = sg * d.gq[(3 * sl + 1) * nc + ie];
+            ar[f] = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
+            const int cf = f + 2 * k;
+            const double m11 = d.rec[(0 * 32 + cf) * nc + ie];
+            const double m1 = d.rec[(1 * 32 + cf) * nc + ie], m2 = d.rec[(2 * 32 + cf) * nc + ie];
+            const double s1 = d.rec[(3 * 32 + cf) * nc + ie], s2 = d.rec[(4 * 32 + cf) * nc + ie];
+            fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
+            fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
+        }
+        const double w1 = WFM[isg][iv][0], w2 = WFM[isg][iv][1];
+        double bq[6];
+        basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, bq);
+        for (int id = 0; id < 6; ++id) {
+            const double b = bq[id];
+            r[0][id] = r[0][id] + us1 * nx[0] * ar[0] * b + us2 * ny[0] * ar[0] * b + us1 * nx[1] * ar[1] * b +
+                       us2 * ny[1] * ar[1] * b;
+            r[1][id] = r[1][id] + fx[0] * b * w1 + fx[1] * b * w2;
+            r[2][id] = r[2][id] + fy[0] * b * w1 + fy[1] * b * w2;
+            r[3][id] = r[3][id] + us1 * fx[0] * b * w1 + us2 * fy[0] * b * w1 + us1 * fx[1] * b * w2 +
+                       us2 * fy[1] * b * w2;
+        }
+    }
+    // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
+    {
+        double u[4][6];
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < 6; ++id) u[q][id] = UD(d.Ul, q, id, ie);
+        double xp[9], yp[9];
+        load9(d, d.xs, d.ys, ie, xp, yp);
+#pragma unroll 1
+        for (int k = 0; k < 25; ++k) {
+            const int i = k / 5, j = k - 5 * i;
+            const double rr = PL5[i], ss = PL5[j], wi = WL5[i] * WL5[j];
+            const double *t = d.shp + (125 + k) * SHP_STRIDE;
+            double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+            for (int n = 0; n < 9; ++n) {
+                dxdr = dxdr + t[n] * xp[n];
+                dxds = dxds + t[9 + n] * xp[n];
+                dydr = dydr + t[n] * yp[n];
+                dyds = dyds + t[9 + n] * yp[n];
+            }
+            const double dbdr[6] = {0.0, 1.0 / 1.0, 0.0, (rr - rc) / 1.0, 0.0, (ss - sc) / 1.0 / 1.0};
+            const double dbds[6] = {0.0, 0.0, 1.0 / 1.0, 0.0, (ss - sc) / 1.0, (rr - rc) / 1.0 / 1.0};
+            double b[6];
+            basis6(rr, ss, rc, sc, gg0, gg1, gg2, b);
+            double un[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int id = 0; id < 6; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
+            const double rhoad = 1.0 / un[0];
+            const double uu = un[1], vv = un[2];
+            const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
+            for (int id = 0; id < 6; ++id) {
+                const double a = dyds * dbdr[id] + (-dydr) * dbds[id];
+                const double c = (-dxds) * dbdr[id] + dxdr * dbds[id];
+                const double f0 = a * uu + c * vv;
+                r[0][id] = r[0][id] - f0 * wi;
+                r[1][id] = r[1][id] - a * (-pres) * wi;
+                r[2][id] = r[2][id] - c * (-pres) * wi;
+                r[3][id] = r[3][id] - (a * uu + c * vv) * (-pres) * wi;
+            }
+        }
+    }
+    // ---- derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708)
+    double ul[4][6];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 6; ++id) ul[q][id] = UD(d.Ul, q, id, ie);
+    for (int id = 1; id < 3; ++id) {
+        const double u1 = ul[1][id], u2 = ul[2][id];
+        ul[0][id] = ul[0][id] * 1.0;
+        ul[1][id] = 1.0 * u1 + 0.0 * u2;
+        ul[2][id] = 0.0 * u1 + 1.0 * u2;
+        ul[3][id] = ul[3][id] * 1.0;
+    }
+    if (apply_update) {
+        const int run = cell_run(d, ie);
+        if (!run_active(d, run)) {  // frozen run: keep U^n
+            for (int q = 0; q < 4; ++q)
+                for (int id = 0; id < 6; ++id) UD(d.Us, q, id, ie) = UD(d.Un, q, id, ie);
+            return;
+        }
+        const double dt = d.scal[2 * run];
+        const int off[6] = {0, 0, 5, 9, 12, 14};
+        for (int id = 0; id < 6; ++id) {
+            double mrow[6];
+            for (int kk = 0; kk < 6; ++kk) {
+                if (id == 0 || kk == 0) {
+                    mrow[kk] = (id == 0 && kk == 0) ? d.A[15 * nc + ie] : 0.0;
+                } else {
+                    const int a = id < kk ? id : kk, b = id < kk ? kk : id;
+                    mrow[kk] = d.A[(off[a] + (b - a)) * nc + ie];
+                }
+            }
+            for (int q = 0; q < 4; ++q) {
+                double s4 = 0.0;
+                for (int kk = 0; kk < 6; ++kk) s4 = s4 + mrow[kk] * (dt * r[q][kk]);
+                UD(d.Us, q, id, ie) = alfa * UD(d.Un, q, id, ie) + (1.0 - alfa) * (ul[q][id] + s4);
+            }
+        }
+    } else {
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < 6; ++id) {
+                UD(d.Us, q, id, ie) = ul[q][id];
+                UD(d.R, q, id, ie) = r[q][id];
+            }
     }
 }
 
@@ -1357,8 +1709,92 @@ int upload_extra(lag2d_ctx &x)
     if (x.cfg.ncurv != 1) return 0;
     int rc = q2_tables();
     if (rc) return rc;
+    if (!x.d.shp) {
+        double *t;
+        if ((rc = x.alloc(&t, (size_t)SHP_POINTS * SHP_STRIDE))) return rc;
+        x.d.shp = t;
+        hipLaunchKernelGGL(k_q2_shape_tables, dim3(1), dim3(256), 0, x.stream, t);
+    }
     hipLaunchKernelGGL(k_q2_submass, dim3(nblk(x.nc)), dim3(LAG2D_BLOCK), 0, x.stream, x.d);
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
+}
+
+// Corner kernel variants (identical results): 1 = 16 lanes per cell (short
+// latency, for small meshes), 2 = one lane per sub-cell with tabulated shape
+// derivatives (throughput, for large meshes).
+static void launch_corner(lag2d_ctx *x, Dev &ds, int variant)
+{
+    const int nc = x->nc;
+    if (variant == 2)
+        hipLaunchKernelGGL(k_q2_corner_sc, dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
+    else
+        hipLaunchKernelGGL(k_q2_corner, dim3((nc + CORNER_CPB - 1) / CORNER_CPB), dim3(256), 0, x->stream, ds);
+}
+
+// Autotuning on the context's own mesh: time both variants (each twice,
+// the second timing counts) and keep the faster.  Both write the same
+// values, so running both costs only time.  LAG2D_CORNER_VARIANT = 1 or 2
+// in the environment at context creation fixes the choice instead.
+static int tune_corner(lag2d_ctx *x, Dev &ds)
+{
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return LAG2D_EDEVICE;
+    float ms[3] = {0.f, 0.f, 0.f};
+    int rc = 0;
+    for (int rep = 0; rep < 2 && !rc; ++rep)
+        for (int v = 1; v <= 2 && !rc; ++v) {
+            if (hipEventRecord(a, x->stream) != hipSuccess) rc = LAG2D_EDEVICE;
+            launch_corner(x, ds, v);
+            if (!rc && hipEventRecord(b, x->stream) != hipSuccess) rc = LAG2D_EDEVICE;
+            if (!rc && hipEventSynchronize(b) != hipSuccess) rc = LAG2D_EDEVICE;
+            if (!rc && hipEventElapsedTime(&ms[v], a, b) != hipSuccess) rc = LAG2D_EDEVICE;
+        }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) return rc;
+    x->corner_variant = (ms[2] < ms[1]) ? 2 : 1;
+    x->corner_ms[0] = ms[1];
+    x->corner_ms[1] = ms[2];
+    return 0;
+}
+
+// Cell-RHS variants (identical results): 1 = 8 cells per block with staged
+// corner and point values (small meshes), 2 = one thread per cell (large
+// meshes).
+static void launch_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update, int variant)
+{
+    const int nc = x->nc;
+    if (variant == 2)
+        hipLaunchKernelGGL(k_cellrhs_q2_tpc, dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
+    else
+        hipLaunchKernelGGL(k_cellrhs_q2, dim3((nc + RHS_CPB - 1) / RHS_CPB), dim3(256), 0, x->stream, ds, alfa,
+                           apply_update);
+}
+
+// As tune_corner.  The kernel reads Ul, the corner records and u* and writes
+// the stage state (and R): running it again on the same inputs rewrites the
+// same values, and it does not read what it writes.
+static int tune_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update)
+{
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return LAG2D_EDEVICE;
+    float ms[3] = {0.f, 0.f, 0.f};
+    int rc = 0;
+    for (int rep = 0; rep < 2 && !rc; ++rep)
+        for (int v = 1; v <= 2 && !rc; ++v) {
+            if (hipEventRecord(a, x->stream) != hipSuccess) rc = LAG2D_EDEVICE;
+            launch_cellrhs(x, ds, alfa, apply_update, v);
+            if (!rc && hipEventRecord(b, x->stream) != hipSuccess) rc = LAG2D_EDEVICE;
+            if (!rc && hipEventSynchronize(b) != hipSuccess) rc = LAG2D_EDEVICE;
+            if (!rc && hipEventElapsedTime(&ms[v], a, b) != hipSuccess) rc = LAG2D_EDEVICE;
+        }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc) return rc;
+    x->rhs_variant = (ms[2] < ms[1]) ? 2 : 1;
+    x->rhs_ms[0] = ms[1];
+    x->rhs_ms[1] = ms[2];
+    return 0;
 }
 
 int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int apply_update)
@@ -1370,14 +1806,22 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
     x->tic(KID_Q2_WENO);
     hipLaunchKernelGGL(k_q2_weno, dim3(nc < WENO_GRID ? nc : WENO_GRID), dim3(WENO_WG), 0, x->stream, ds, Uin);
     x->toc(KID_Q2_WENO);
+    if (x->corner_variant == 0) {
+        int rc = tune_corner(x, ds);
+        if (rc) return rc;
+    }
     x->tic(KID_Q2_CORNER);
-    hipLaunchKernelGGL(k_q2_corner, dim3((nc + CORNER_CPB - 1) / CORNER_CPB), dim3(256), 0, x->stream, ds);
+    launch_corner(x, ds, x->corner_variant);
     x->toc(KID_Q2_CORNER);
     x->tic(KID_NODE_Q2);
     hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
     x->toc(KID_NODE_Q2);
+    if (x->rhs_variant == 0) {
+        int rc = tune_cellrhs(x, ds, alfa, apply_update);
+        if (rc) return rc;
+    }
     x->tic(KID_CELLRHS_Q2);
-    hipLaunchKernelGGL(k_cellrhs_q2, dim3((nc + RHS_CPB - 1) / RHS_CPB), dim3(256), 0, x->stream, ds, alfa, apply_update);
+    launch_cellrhs(x, ds, alfa, apply_update, x->rhs_variant);
     x->toc(KID_CELLRHS_Q2);
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
 }

@@ -1,7 +1,7 @@
 """HIP curved-quad P2 path (sub-cell Riemann solver, Persson indicator,
 characteristic WENO) against the reference's fixtures and the oracle.
-Bar: 1e-12 relative L1 per variable group; the run also reports whether the
-states are bit-identical."""
+Bar: bit-identical states (same operation order, no contraction), for both
+variants of the corner and cell-RHS kernels."""
 import numpy as np
 import pytest
 
@@ -11,24 +11,33 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
+@pytest.fixture(params=[1, 2], ids=["small-mesh-kernels", "large-mesh-kernels"])
+def variant(request, monkeypatch):
+    """Kernel variants (corner and cell RHS; the library picks them by timing)
+    fixed for the contexts a test creates."""
+    monkeypatch.setenv("LAG2D_CORNER_VARIANT", str(request.param))
+    monkeypatch.setenv("LAG2D_RHS_VARIANT", str(request.param))
+    return request.param
+
+
 def _lag(case, strict=False):
     from lag2d_amd.lib import Lag2D
     return Lag2D(case, strict=strict)
 
 
 @pytest.mark.parametrize("name", ["sedov_q2_12x12", "sedov_q2c_16x16"])
-def test_rhs_matches_reference(name):
+def test_rhs_matches_reference(name, variant):
     case, z = load(name)
     g = _lag(case)
     r = g.rhs()
     st = g.download()
-    print(name, "rhs bit-exact:", np.array_equal(r, z["rhs_rhsel"]), np.array_equal(st["ustar"], z["rhs_ustar"]))
-    assert rel_l1(r, z["rhs_rhsel"]) <= TOL
-    assert rel_l1(st["ustar"], z["rhs_ustar"]) <= TOL
-    assert rel_l1(st["unkno"], z["rhs_unkno"]) <= TOL
+    print(name, "rhs", rel_l1(r, z["rhs_rhsel"]))
+    assert np.array_equal(r, z["rhs_rhsel"])
+    assert np.array_equal(st["ustar"], z["rhs_ustar"])
+    assert np.array_equal(st["unkno"], z["rhs_unkno"])
 
 
-def test_steps_with_weno_match_reference():
+def test_steps_with_weno_match_reference(variant):
     """Interior Sedov blast: troubled cells and WENO from the first step."""
     case, z = load("sedov_q2c_16x16")
     g = _lag(case)
@@ -41,12 +50,13 @@ def test_steps_with_weno_match_reference():
         err = state_errors(st["unkno"], st["coord"], z[f"s{s}_unkno"], z[f"s{s}_coord"])
         bit = np.array_equal(st["unkno"], z[f"s{s}_unkno"]) and np.array_equal(st["coord"], z[f"s{s}_coord"])
         print("sedov_q2c_16x16 step", s, "bit-exact", bit, err)
-        assert abs(t - z[f"s{s}_time"][1]) <= 1e-13 * abs(t)
-        for k, v in err.items():
-            assert v <= TOL, (k, v)
+        assert t == z[f"s{s}_time"][1] and dt == z[f"s{s}_time"][0]
+        assert bit
+        assert np.array_equal(st["ustar"], z[f"s{s}_ustar"])
+        assert np.array_equal(st["geoel"][:, [2, 4, 5]], z[f"s{s}_geoel"])
 
 
-def test_corner_blast_matches_oracle():
+def test_corner_blast_matches_oracle(variant):
     """Quadrant blast (the benchmark's geometry): the reference stops in its
     WENO stencil builder here; the HIP path and the oracle both treat those
     stencils like boundary stencils and must agree."""
@@ -58,8 +68,8 @@ def test_corner_blast_matches_oracle():
     o = Oracle(case)
     o.step(8)
     err = state_errors(st["unkno"], st["coord"], o.unkno, o.coord)
-    print("corner blast 8 steps vs oracle:", err, "bit-exact", np.array_equal(st["unkno"], o.unkno))
-    assert max(err.values()) <= TOL
+    print("corner blast 8 steps vs oracle:", err)
+    assert np.array_equal(st["unkno"], o.unkno) and np.array_equal(st["coord"], o.coord)
 
 
 def test_corner_blast_strict_reports_topology():
