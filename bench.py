@@ -199,17 +199,25 @@ def main():
     dt, tnow = g.time()
     value = dofs_per_step * args.steps / elapsed
 
-    # per-kernel device times (HIP events on the library's stream), short pass
+    # per-kernel device times: the same warmup + timed steps again on a fresh
+    # context, every launch timed with HIP events on the library's stream
+    # (a rocprofv3 kernel trace of this command averages over both passes,
+    # which run the same launches)
+    g.close()
+    g = Lag2D(case)
     g.timing(True)
-    for _ in range(10):
+    for _ in range(args.warmup + args.steps):
         g.step_async()
     g.sync()
     name, avg_ms, stats = g.kernel_report()
+    weno_cells = g.weno_cells()
     g.timing(False)
+    weno_launches = stats.get("k_q2_weno", (0.0, 0))[1]
 
-    # algorithmic traffic of the dominant kernel per launch (see the design notes)
+    # algorithmic flops / bytes of the dominant kernel per launch (per-unit
+    # counts from the op-counting oracle, units of this run)
     from lag2d_amd import roofline
-    rf = roofline.figures(name, case, avg_ms)
+    rf = roofline.figures(name, case, avg_ms, weno_cells / max(weno_launches, 1))
 
     cpu = None
     if not args.no_cpu:

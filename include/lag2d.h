@@ -141,14 +141,28 @@ int lag2d_check(lag2d_ctx *ctx);
 /* Wait for all work queued on the context's stream. */
 int lag2d_sync(lag2d_ctx *ctx);
 
-/* Kernel timing: average device time (ms) of the dominant kernel over the
- * last lag2d_run, measured with HIP events on the context's stream, and its
- * name.  Returns LAG2D_EINVAL if no timed run happened. */
+/* Kernel timing: enable > 0 starts timing every kernel launch with HIP
+ * events on the context's stream (and clears the WENO cell counter), 0 stops;
+ * enable < 0 reports the average device time (ms) of the dominant kernel
+ * over the timed launches, and its name.  Returns LAG2D_EINVAL if no timed
+ * run happened. */
 int lag2d_kernel_time(lag2d_ctx *ctx, int enable, double *avg_ms, const char **name);
 
 /* Per-kernel totals of the last timed run: kernel index k = 0, 1, ...;
  * returns LAG2D_EINVAL past the last kernel. */
 int lag2d_kernel_stats(lag2d_ctx *ctx, int k, double *total_ms, int *calls, const char **name);
+
+/* Troubled cells the WENO kernel processed since kernel timing was last
+ * enabled (the unit of that kernel's work; 0 on the straight path).  No
+ * counterpart in the reference. */
+int lag2d_weno_cells(lag2d_ctx *ctx, long long *total);
+
+/* Self-test of the library's division by a shared divisor: q[i] = a[i] / b[i]
+ * computed on the device by the reciprocal-and-correction sequence the
+ * kernels use where many quotients share one divisor; it must equal the IEEE
+ * quotient bit for bit.  No context needed; synchronises.  No counterpart in
+ * the reference. */
+int lag2d_division_check(int n, const double *a, const double *b, double *q);
 
 void lag2d_destroy(lag2d_ctx *ctx);
 const char *lag2d_last_error(void);

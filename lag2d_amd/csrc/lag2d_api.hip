@@ -239,13 +239,15 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
         if ((rc = x->alloc(&d.msub, (size_t)4 * nquad)) || (rc = x->alloc(&d.Ul, nU)) ||
             (rc = x->alloc(&d.gq, (size_t)48 * nquad)) || (rc = x->alloc(&d.flag, (size_t)nquad)) ||
             (rc = x->alloc(&d.tlist, (size_t)nquad)) || (rc = x->alloc(&d.tcount, 2)) ||
+            (rc = x->alloc(&d.tsum, 1)) ||
             (rc = x->alloc(&t_esq, esq.size())))
             return bad(rc);
         d.esqua = t_esq;
         d.wenobad = d.tcount + 1;
         d.weno_strict = c.strict;
         if (hipMemcpy(t_esq, esq.data(), esq.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(d.tcount, 0, 2 * sizeof(int)) != hipSuccess)
+            hipMemset(d.tcount, 0, 2 * sizeof(int)) != hipSuccess ||
+            hipMemset(d.tsum, 0, sizeof(unsigned long long)) != hipSuccess)
             return bad(fail(LAG2D_EDEVICE, "curved-path table upload"));
     }
     d.ipq = t_ipq; d.esuv1 = t_e1; d.esuv2 = t_e2; d.n2c = t_n2c; d.n2c_off = t_n2co;
@@ -572,6 +574,38 @@ extern "C" int lag2d_set_initial_density(lag2d_ctx *x, const double *rho)
     return 0;
 }
 
+extern "C" int lag2d_weno_cells(lag2d_ctx *x, long long *total)
+{
+    if (!x || !total) return fail(LAG2D_EINVAL, "null argument");
+    *total = 0;
+    if (!x->d.tsum) return 0;
+    unsigned long long v = 0;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemcpy(&v, x->d.tsum, sizeof(v), hipMemcpyDeviceToHost));
+    *total = (long long)v;
+    return 0;
+}
+
+extern "C" int lag2d_division_check(int n, const double *a, const double *b, double *q)
+{
+    if (n < 0 || (n > 0 && (!a || !b || !q))) return fail(LAG2D_EINVAL, "bad argument");
+    if (n == 0) return 0;
+    double *da = nullptr;
+    int rc = dalloc(&da, (size_t)3 * n);
+    if (rc) return rc;
+    const size_t bytes = (size_t)n * sizeof(double);
+    hipError_t e = hipMemcpy(da, a, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(da + n, b, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_division_check, dim3(nblocks(n)), dim3(LAG2D_BLOCK), 0, 0, da, da + n, da + 2 * (size_t)n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(q, da + 2 * (size_t)n, bytes, hipMemcpyDeviceToHost);
+    (void)hipFree(da);
+    if (e != hipSuccess) return fail(LAG2D_EDEVICE, std::string("division check: ") + hipGetErrorString(e));
+    return 0;
+}
+
 extern "C" int lag2d_get_run_times(lag2d_ctx *x, double *dtfix, double *tacru)
 {
     if (!x) return fail(LAG2D_EINVAL, "null context");
@@ -630,6 +664,7 @@ extern "C" int lag2d_kernel_time(lag2d_ctx *x, int enable, double *avg_ms, const
             t.ev.clear();
         }
         x->timing = enable != 0;
+        if (enable && x->d.tsum) HIPCHK(hipMemset(x->d.tsum, 0, sizeof(unsigned long long)));
         return 0;
     }
     // enable < 0: report

@@ -110,6 +110,15 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_run_flags(Dev d)
     d.act[r] = (fabs(t - tend) / tend <= 1e-11) ? 0 : 1;
 }
 
+// Quotients by the shared-divisor division the kernels use (qdiv), for the
+// library's self-test against the IEEE division.
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_division_check(const double *a, const double *b, double *q, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    q[i] = qdiv(a[i], recip(b[i]));
+}
+
 // Mass-matrix solve and SSP-RK blend (Getrhslag.f90:1822-1904):
 // U_s = alfa U^n + (1 - alfa)(U_s + M^-1 dt R), one thread per DOF (q, id)
 // of a cell, cells fastest so that every access is coalesced.

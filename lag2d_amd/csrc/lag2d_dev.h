@@ -76,6 +76,7 @@ struct Dev {
     uint8_t *flag;           // Persson troubled flag per cell
     int *tlist;              // compacted troubled-cell list
     int *tcount;             // its length (device counter)
+    unsigned long long *tsum;  // troubled cells processed since the counter was last cleared
     const int *esqua;        // face neighbours [4][nc]; >= nc marks a boundary face
     int weno_strict;         // 1: report cells the reference's stencil builder stops on
     int *wenobad;            // count of such cells
@@ -90,9 +91,63 @@ struct Dev {
     // initial density per cell, for initial states other than the
     // reference's built-in problems (null: the density of ncase)
     const double *rhoi;
-    // shape-function derivatives at the curved path's fixed quadrature points
-    const double *shp;
+    // shape-function derivatives at the curved path's fixed quadrature points,
+    // and shape values at the 25 Gauss points
+    const double *shp, *shv;
+    // time-invariant WENO stencil data per cell: topology [16][nc] (stencil
+    // cells, flags, real faces) and rho |J| w at the 25 Gauss points [25][nc]
+    const int *wtop;
+    const double *wrd;
 };
+
+// Quotients a / b by one divisor b, bit-identical to the IEEE division.
+// gfx950 divides FP64 with div_scale (operand scaling), rcp, two Newton
+// steps on the reciprocal, q0 = a*y, the correction fma(fma(-b, q0, a), y,
+// q0) (div_fmas) and div_fixup (special operands).  When both exponents lie
+// in [-380, 380] (or a is zero, whose quotient is q0 = a*y, a signed zero)
+// the scaling and the fix-up are identities, so the quotient is those
+// three operations on the reciprocal, which a shared divisor computes once.
+// Other operands take the division itself.
+struct Recip {
+    double b, y;
+    bool ok;
+};
+
+__device__ __forceinline__ bool div_exp_ok(double x)
+{
+    const unsigned e = (unsigned)(__double_as_longlong(x) >> 52) & 0x7ffu;
+    return e - (1023u - 380u) <= 760u;
+}
+
+__device__ __forceinline__ Recip recip(double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    return Recip{b, y, div_exp_ok(b)};
+}
+
+__device__ __forceinline__ double qdiv(double a, const Recip &r)
+{
+    const double q0 = a * r.y;
+    double q = __builtin_fma(__builtin_fma(-r.b, q0, a), r.y, q0);
+    if (a == 0.0) q = q0;
+    if (!(r.ok && (div_exp_ok(a) || a == 0.0))) q = a / r.b;
+    return q;
+}
+
+// The shortcut alone, for a term of a sum that starts at +0: clears ok when
+// a needs the division itself (the caller then recomputes the sum with it).
+// For a = -0 it returns +0 where the division gives -0; a sum started at +0
+// cannot tell them apart.
+__device__ __forceinline__ double qdiv_term(double a, const Recip &r, bool &ok)
+{
+    ok = ok & (div_exp_ok(a) | (a == 0.0));
+    const double q0 = a * r.y;
+    return __builtin_fma(__builtin_fma(-r.b, q0, a), r.y, q0);
+}
 
 __device__ __forceinline__ int cell_run(const Dev &d, int ie) { return d.nrun > 1 ? d.runc[ie] : 0; }
 __device__ __forceinline__ int node_run(const Dev &d, int p) { return d.nrun > 1 ? d.runn[p] : 0; }

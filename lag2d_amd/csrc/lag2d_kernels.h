@@ -15,6 +15,7 @@ __global__ void k_cellrhs_q1(Dev d, const double *__restrict__ Uin, double alfa,
 __global__ void k_dt_partial(Dev d, const double *__restrict__ Uin, int nbpr);
 __global__ void k_dt_final(Dev d, int itime, int nbpr);
 __global__ void k_run_flags(Dev d);
+__global__ void k_division_check(const double *a, const double *b, double *q, int n);
 __global__ void k_update(Dev d, double alfa);
 __global__ void k_nodemove(Dev d, double alfa, const double *__restrict__ xsrc, const double *__restrict__ ysrc);
 __global__ void k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y, int set_centre);

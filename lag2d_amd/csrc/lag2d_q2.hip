@@ -50,6 +50,12 @@ __constant__ static const double PL5[5] = {-1.0, -0.6546536707079771437983, 0.0,
 __constant__ static const double WL5[5] = {0.1, 0.544444444444444444444, 0.7111111111111111111111,
                                            0.544444444444444444444, 0.1};
 
+// Shape-function table of the fixed quadrature points (k_q2_shape_tables):
+// derivatives at 25 Gauss points, 4 x 25 sub-cell points and 25 Lobatto
+// points, then the shape values at the 25 Gauss points.
+constexpr int SHP_POINTS = 150;
+constexpr int SHP_STRIDE = 18;   // dN/dr[9], dN/ds[9]
+
 __device__ __forceinline__ void shape9(double r, double s, double *sh, double *dr_, double *ds_)
 {
     const double c10 = 1.0;
@@ -308,16 +314,15 @@ __device__ double cof5(const double *a, int i, int j)
     return ((i + j) & 1) ? -cof : cof;
 }
 
-__device__ void char_mats(double gam, double dnx, double dny, const double ul[4], const double ur[4], double q[4][4],
-                          double qi[4][4])
+// Characteristic matrices of a real face (Roe-averaged state of the two
+// cells, face normal in the velocity-aligned frame), in two parts so that a
+// workgroup can spread them: the face's averaged quantities, then one
+// matrix entry per call.  P = dnx, dny, uf, vf, rhosd, pf, gam1, vnm, vedn.
+__device__ __forceinline__ void char_face(double gam, double dnx, double dny, const double ul[4], const double ur[4], double P[9])
 {
     const double eps = 1.e-6;
     double rhoc1 = 1.0 / ul[0], u1 = ul[1], v1 = ul[2], e1 = ul[3];
-    double p1 = dmax(eps, (gam - 1.0) * rhoc1 * (e1 - 0.5 * (u1 * u1 + v1 * v1)));
     double rhoc2 = 1.0 / ur[0], u2 = ur[1], v2 = ur[2], e2 = ur[3];
-    double p2 = dmax(eps, (gam - 1.0) * rhoc2 * (e2 - 0.5 * (u2 * u2 + v2 * v2)));
-    (void)p1;
-    (void)p2;
     double rhof = sqrt(rhoc1 * rhoc2);
     double rhosq = sqrt(rhoc1) + sqrt(rhoc2);
     double uf = (sqrt(rhoc1) * u1 + sqrt(rhoc2) * u2) / rhosq;
@@ -326,44 +331,63 @@ __device__ void char_mats(double gam, double dnx, double dny, const double ul[4]
     double h2 = dmax(eps, gam * e2 - (gam - 1.0) * 0.5 * (u2 * u2 + v2 * v2));
     double hf = (sqrt(rhoc1) * h1 + sqrt(rhoc2) * h2) / rhosq;
     double sdf = sqrt(dmax(eps, (gam - 1.0) * (hf - 0.5 * (uf * uf + vf * vf))));
-    double rhosd = rhof * sdf;
-    double pf = sdf * sdf * rhof / gam;
-    double gam1 = (gam - 1.0) / gam;
-    double vnm = vf * dnx - uf * dny;
-    qi[0][0] = -.5 / gam;
-    qi[0][1] = .5 * dnx / rhosd - 0.5 * uf / pf * gam1;
-    qi[0][2] = .5 * dny / rhosd - 0.5 * vf / pf * gam1;
-    qi[0][3] = .5 / pf * gam1;
-    qi[1][0] = 1.0 / gam;
-    qi[1][1] = -dny * vnm / pf - uf / pf / gam;
-    qi[1][2] = dnx * vnm / pf - vf / pf / gam;
-    qi[1][3] = 1.0 / pf / gam;
-    qi[2][0] = 0.0;
-    qi[2][1] = dny / pf;
-    qi[2][2] = -dnx / pf;
-    qi[2][3] = 0.0;
-    qi[3][0] = .5 / gam;
-    qi[3][1] = .5 * dnx / rhosd + 0.5 * uf / pf * gam1;
-    qi[3][2] = .5 * dny / rhosd + 0.5 * vf / pf * gam1;
-    qi[3][3] = -.5 / pf * gam1;
-    double vedn = uf * dnx + vf * dny;
-    q[0][0] = -1.0;
-    q[0][1] = gam - 1.0;
-    q[0][2] = (gam - 1.0) * vnm;
-    q[0][3] = 1.0;
-    q[1][0] = rhosd * dnx;
-    q[1][1] = 0.0;
-    q[1][2] = pf * dny;
-    q[1][3] = rhosd * dnx;
-    q[2][0] = rhosd * dny;
-    q[2][1] = 0.0;
-    q[2][2] = -pf * dnx;
-    q[2][3] = rhosd * dny;
-    q[3][0] = pf + rhosd * vedn;
-    q[3][1] = pf;
-    q[3][2] = 0.0;
-    q[3][3] = -pf + rhosd * vedn;
+    P[0] = dnx;
+    P[1] = dny;
+    P[2] = uf;
+    P[3] = vf;
+    P[4] = rhof * sdf;                  // rhosd
+    P[5] = sdf * sdf * rhof / gam;      // pf
+    P[6] = (gam - 1.0) / gam;           // gam1
+    P[7] = vf * dnx - uf * dny;         // vnm
+    P[8] = uf * dnx + vf * dny;         // vedn
 }
+
+// Entry e of the face's matrices: e < 16 the left eigenvectors qi[e/4][e%4],
+// else the right eigenvectors q[(e-16)/4][(e-16)%4].
+__device__ __forceinline__ double char_entry(double gam, const double P[9], int e)
+{
+    const double dnx = P[0], dny = P[1], uf = P[2], vf = P[3], rhosd = P[4], pf = P[5], gam1 = P[6],
+                 vnm = P[7], vedn = P[8];
+    switch (e) {
+    case 0: return -.5 / gam;
+    case 1: return .5 * dnx / rhosd - 0.5 * uf / pf * gam1;
+    case 2: return .5 * dny / rhosd - 0.5 * vf / pf * gam1;
+    case 3: return .5 / pf * gam1;
+    case 4: return 1.0 / gam;
+    case 5: return -dny * vnm / pf - uf / pf / gam;
+    case 6: return dnx * vnm / pf - vf / pf / gam;
+    case 7: return 1.0 / pf / gam;
+    case 8: return 0.0;
+    case 9: return dny / pf;
+    case 10: return -dnx / pf;
+    case 11: return 0.0;
+    case 12: return .5 / gam;
+    case 13: return .5 * dnx / rhosd + 0.5 * uf / pf * gam1;
+    case 14: return .5 * dny / rhosd + 0.5 * vf / pf * gam1;
+    case 15: return -.5 / pf * gam1;
+    case 16: return -1.0;
+    case 17: return gam - 1.0;
+    case 18: return (gam - 1.0) * vnm;
+    case 19: return 1.0;
+    case 20: return rhosd * dnx;
+    case 21: return 0.0;
+    case 22: return pf * dny;
+    case 23: return rhosd * dnx;
+    case 24: return rhosd * dny;
+    case 25: return 0.0;
+    case 26: return -pf * dnx;
+    case 27: return rhosd * dny;
+    case 28: return pf + rhosd * vedn;
+    case 29: return pf;
+    case 30: return 0.0;
+    default: return -pf + rhosd * vedn;
+    }
+}
+
+// Compiler fence: the LDS loads written above it are issued together and
+// waited for once.  Without it the scheduler places each LDS load next to
+// its use, which exposes the full LDS latency per load at one wave per SIMD.
+#define LDS_FENCE() asm volatile("" ::: "memory")
 
 // One troubled cell per workgroup: 9-cell stencil, local orthogonal Taylor
 // bases, 1 + 4 Lagrange + 4 Hermite candidates, characteristic nonlinear
@@ -374,14 +398,18 @@ __device__ void char_mats(double gam, double dnx, double dny, const double ul[4]
 // order, so the result does not depend on the split.  Blocks stride over
 // the compacted troubled-cell list.
 constexpr int WENO_WG = 256;
-constexpr int WENO_GRID = 1024;
+constexpr int WENO_GRID = 512;  // two resident blocks per CU
 
 struct WenoShared {
     double m[4];
-    double cx0[9][9], cy0[9][9], cxs[9][9], cys[9][9];  // initial and stage nodes of the stencil cells
+    double shv[25][9];                                   // shape values at the Gauss points
+    double cxs[9][9], cys[9][9];                         // stage nodes of the stencil cells
+    double cxr[9][9], cyr[9][9];                         // the same in the velocity-aligned frame
     double gc[9][7];                                     // rc, sc, offsets 19-21, volume, mass
     double xl[9][25], yl[9][25], rd[9][25];  // rotated Gauss points, rho * |J| * w
-    double tx[9][25], ty[9][25];             // mass-centre terms (unrotated)
+    double xg[9][25], yg[9][25];             // unrotated Gauss points
+    double fdat[4][10];                      // per real face: edge vector, left and right states
+    double fpro[4][9];                       // per real face: averaged quantities (char_face)
     double bl[9][25][5];                     // basis about the cell's own mass centre
     double bl0[9][25][5];                    // basis about the troubled cell's mass centre
     double geopl[9][5], dxcj[9], masel[9];
@@ -401,15 +429,26 @@ struct WenoShared {
             double phy[4][6], rhsuk[4][5];
         } l;
     };
+    int top[16];  // stencil topology of the cell (k_q2_weno_setup)
     int ielaj[9];
-    int jel[4][3];
-    int skip;
     int nface;
     int face[4], fnei[4];
 };
 
 __constant__ static const int ST1[4][5] = {{1, 4, 5, 8, 7}, {2, 1, 6, 5, 8}, {3, 2, 7, 6, 5}, {4, 3, 8, 7, 6}};
 __constant__ static const int ST2[4][3] = {{1, 5, 8}, {2, 5, 6}, {3, 7, 6}, {4, 7, 8}};
+
+// Value of lane J of this lane's quad (lanes 4i..4i+3), by DPP quad_perm;
+// every lane of the quad must be active.
+template <int J>
+__device__ __forceinline__ double quad_bcast(double v)
+{
+    constexpr int ctrl = J | (J << 2) | (J << 4) | (J << 6);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 
 // Row k of the Hermite least-squares matrix of stencil is, column a
 // (rows 0-2: the cell-average constraints, 3-8: the two gradient
@@ -424,32 +463,27 @@ __device__ __forceinline__ double mh_entry(const WenoShared &S, int is, int k, i
 __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__restrict__ Uin)
 {
     __shared__ WenoShared S;
-    const int tid = threadIdx.x;
-    // task index of this thread: consecutive tasks go to different waves, so
-    // phases with few tasks still use every SIMD
-    const int tsp = ((tid & 63) << 2) | (tid >> 6);
     static_assert(WENO_WG == 256, "task spreading assumes 4 waves");
     const int nc = d.nc;
     const int ntr = *d.tcount;
+    if (blockIdx.x >= ntr) return;
+    for (int k = threadIdx.x; k < 225; k += WENO_WG) (&S.shv[0][0])[k] = d.shv[k];
     for (int t = blockIdx.x; t < ntr; t += gridDim.x) {
+        // The thread index passes through an empty asm statement, so the
+        // compiler cannot hoist the phases' index arithmetic out of the
+        // cell loop and keep it live in registers the phases need.
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        // task index of this thread: consecutive tasks go to different waves,
+        // so phases with few tasks still use every SIMD
+        const int tsp = ((tid & 63) << 2) | (tid >> 6);
         __syncthreads();  // the previous cell's staging is dead
         const int ie = d.tlist[t];
-        // stencil topology: face neighbours, then the vertex neighbour
-        // opposite each corner, identified by the product of the face
-        // numbers it shares
-        if (tid < 4) {
-            S.ielaj[1 + tid] = d.esqua[tid * nc + ie];
-        } else if (tid >= 64 && tid < 68) {
-            const int iv = tid - 64;
-            int jel[3] = {-1, -1, -1}, nv = 0;
-            const int p = d.ipq[iv * nc + ie];
-            for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
-                const int je = d.esuv1[k];
-                if (je != ie && nv < 3) jel[nv++] = je;
-            }
-            for (int a = 0; a < 3; ++a) S.jel[iv][a] = jel[a];
-        } else if (tid == 128) {
-            // velocity-aligned frame of the troubled cell
+        // 0a: stencil topology (time-invariant, from k_q2_weno_setup) and the
+        //     velocity-aligned frame of the troubled cell
+        if (tid < 16) {
+            S.top[tid] = d.wtop[tid * nc + ie];
+        } else if (tid == 64) {
             const double a = UD(Uin, 1, 0, ie), b = UD(Uin, 2, 0, ie);
             if (sqrt(a * a + b * b) > 1.e-10) {
                 S.m[0] = a / sqrt(a * a + b * b);
@@ -461,146 +495,125 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
             }
         }
         __syncthreads();
-        if (tid == 0) {
-            int ielaj[9];
-            ielaj[0] = ie;
-            for (int f = 0; f < 4; ++f) ielaj[1 + f] = S.ielaj[1 + f];
-            for (int k = 5; k < 9; ++k) ielaj[k] = -2;
-            bool mismatch = false;
-            for (int iv = 0; iv < 4; ++iv) {
-                int idxod = 1, idxiv[3] = {0, 0, 0};
-                for (int a = 0; a < 3; ++a)
-                    for (int f = 0; f < 4; ++f)
-                        if (S.jel[iv][a] == ielaj[1 + f]) {
-                            idxod = idxod * (f + 1);
-                            idxiv[a] = 1;
-                        }
-                for (int a = 0; a < 3; ++a) {
-                    if (idxiv[a]) continue;
-                    if (idxod == 2) ielaj[5] = S.jel[iv][a];
-                    else if (idxod == 6) ielaj[6] = S.jel[iv][a];
-                    else if (idxod == 12) ielaj[7] = S.jel[iv][a];
-                    else if (idxod == 4) ielaj[8] = S.jel[iv][a];
-                    else mismatch = true;
-                }
-            }
-            int skip = 0;
-            if (mismatch) {
-                atomicAdd(d.wenobad, 1);
-                skip = 1;
-            }
-            for (int k = 0; k < 9; ++k) {
-                if (ielaj[k] < 0) skip = 1;
-                S.ielaj[k] = ielaj[k];
-            }
-            S.skip = skip;
-            int nf = 0;
-            for (int f = 0; f < 4; ++f) {
-                const int je = ielaj[1 + f];
-                if (je >= nc) continue;
-                S.face[nf] = f;
-                S.fnei[nf] = je;
-                ++nf;
-            }
-            S.nface = nf;
+        {
+            const int flags = S.top[9];
+            if (tid == 0 && (flags & 2)) atomicAdd(d.wenobad, 1);
+            if (flags & 1) continue;
         }
-        __syncthreads();
-        if (S.skip) continue;
         const double m[4] = {S.m[0], S.m[1], S.m[2], S.m[3]};
+        int ielaj[9];
+        for (int k = 0; k < 9; ++k) ielaj[k] = S.top[k];
+        if (tid < 9) {
+            S.ielaj[tid] = ielaj[tid];
+        } else if (tid == 64) {
+            const int nf = S.top[10];
+            S.nface = nf;
+            for (int a = 0; a < 4; ++a) {
+                const int f = S.top[11 + a];
+                S.face[a] = f;
+                S.fnei[a] = (a < nf) ? ielaj[1 + f] : 0;
+            }
+        }
 
-        // 0: nodes and geometry rows of the stencil cells into LDS
-        for (int k = tsp; k < 81 + 63; k += WENO_WG) {
+        // 0b: everything the cell reads from HBM, one batch per thread:
+        //     rho |J| w at the stencil's Gauss points (time-invariant), stage
+        //     nodes, geometry rows, DOFs, and the real faces' edge vectors and
+        //     neighbouring states
+        {
+            const int k = tsp;
+            double rdv = 0.0;
+            if (k < 225) {
+                const int s = k / 25, g = k - 25 * s;
+                rdv = d.wrd[g * nc + ielaj[s]];
+            }
             if (k < 81) {
                 const int s = k / 9, q = k - 9 * s;
-                const int p = d.ipq[q * nc + S.ielaj[s]];
-                S.cx0[s][q] = d.x0[p];
-                S.cy0[s][q] = d.y0[p];
-                S.cxs[s][q] = d.xs[p];
-                S.cys[s][q] = d.ys[p];
-            } else {
+                const int p = d.ipq[q * nc + ielaj[s]];
+                double x = d.xs[p], y = d.ys[p];
+                S.cxs[s][q] = x;
+                S.cys[s][q] = y;
+                rotf(m, x, y);
+                S.cxr[s][q] = x;
+                S.cyr[s][q] = y;
+            } else if (k < 81 + 63) {
                 const int u = k - 81, s = u / 7, r = u - 7 * s;
                 const int row[7] = {0, 1, 18, 19, 20, 2, 3};
-                S.gc[s][r] = d.G[row[r] * nc + S.ielaj[s]];
-            }
-        }
-        __syncthreads();
-
-        // 1: Gauss points of the stencil, rotated DOFs, face characteristic matrices
-        for (int k = tsp; k < 225 + 54 + 4; k += WENO_WG) {
-            if (k < 225) {
-                const int s = k / 25, g = k - 25 * s, i = g / 5, j = g - 5 * i;
-                double xi[9], yi[9], xp[9], yp[9];
-                for (int q = 0; q < 9; ++q) {
-                    xi[q] = S.cx0[s][q];
-                    yi[q] = S.cy0[s][q];
-                    xp[q] = S.cxs[s][q];
-                    yp[q] = S.cys[s][q];
-                }
-                double xcr, ycr;
-                map9(xi, yi, S.gc[s][0], S.gc[s][1], &xcr, &ycr);
-                const double rho = d.rhoi ? d.rhoi[S.ielaj[s]] : rho0(d.ncase, xcr, ycr);
-                const double r = PG5[i], sg = PG5[j], wi = WG5[i] * WG5[j];
-                double J[4];
-                jac9(xi, yi, r, sg, J);
-                const double djacoi = wi * (J[0] * J[3] - J[2] * J[1]);
-                const double rdv = rho * djacoi;
-                double xg, yg;
-                map9(xp, yp, r, sg, &xg, &yg);
-                S.tx[s][g] = rdv * xg;
-                S.ty[s][g] = rdv * yg;
-                for (int q = 0; q < 9; ++q) rotf(m, xp[q], yp[q]);
-                map9(xp, yp, r, sg, &xg, &yg);
-                S.xl[s][g] = xg;
-                S.yl[s][g] = yg;
-                S.rd[s][g] = rdv;
-                double b[6];
-                basis6(r, sg, S.gc[s][0], S.gc[s][1], S.gc[s][2], S.gc[s][3], S.gc[s][4], b);
-                for (int a = 0; a < 5; ++a) S.bq[s][g][a] = b[a + 1];
-            } else if (k < 225 + 54) {
-                const int u = k - 225, s = u / 6, id = u - 6 * s;
-                const int je = S.ielaj[s];
+                S.gc[s][r] = d.G[row[r] * nc + ielaj[s]];
+            } else if (k < 81 + 63 + 54) {
+                const int u = k - 81 - 63, s = u / 6, id = u - 6 * s;
+                const int je = ielaj[s];
                 double v[4];
                 for (int q = 0; q < 4; ++q) v[q] = UD(Uin, q, id, je);
                 rotf(m, v[1], v[2]);
                 for (int q = 0; q < 4; ++q) S.unkrl[s][q][id] = v[q];
-            } else {
-                const int a = k - 225 - 54;
-                if (a < S.nface) {
-                    const int f = S.face[a], je = S.fnei[a];
-                    const int a0 = f, a1 = (f + 1) & 3;
-                    const int p0 = d.ipq[a0 * nc + ie], p1 = d.ipq[a1 * nc + ie];
-                    const double dtx = d.xs[p1] - d.xs[p0];
-                    const double dty = d.ys[p1] - d.ys[p0];
-                    const double dlgt = sqrt(dtx * dtx + dty * dty);
-                    double dnx = dty / dlgt, dny = -dtx / dlgt;
-                    rotf(m, dnx, dny);
-                    double ul[4], ur[4];
+            } else if (k < 81 + 63 + 54 + 4) {
+                const int a = k - 81 - 63 - 54;
+                if (a < S.top[10]) {
+                    const int f = S.top[11 + a], je = ielaj[1 + f];
+                    const int p0 = d.ipq[f * nc + ie], p1 = d.ipq[((f + 1) & 3) * nc + ie];
+                    S.fdat[a][0] = d.xs[p1] - d.xs[p0];
+                    S.fdat[a][1] = d.ys[p1] - d.ys[p0];
                     for (int q = 0; q < 4; ++q) {
-                        ul[q] = UD(Uin, q, 0, ie);
-                        ur[q] = UD(Uin, q, 0, je);
+                        S.fdat[a][2 + q] = UD(Uin, q, 0, ie);
+                        S.fdat[a][6 + q] = UD(Uin, q, 0, je);
                     }
-                    rotf(m, ul[1], ul[2]);
-                    rotf(m, ur[1], ur[2]);
-                    char_mats(d.gamlg, dnx, dny, ul, ur, S.qm[a], S.qi[a]);
                 }
             }
+            if (k < 225) S.rd[k / 25][k - 25 * (k / 25)] = rdv;
+        }
+        __syncthreads();
+
+        // 1: Gauss points of the stencil (unrotated and rotated) and the
+        //    reference-coordinate basis there
+        for (int k = tsp; k < 225; k += WENO_WG) {
+            const int s = k / 25, g = k - 25 * s, i = g / 5, j = g - 5 * i;
+            double sh[9], xs[9], ys[9], xr[9], yr[9];
+            for (int q = 0; q < 9; ++q) {
+                sh[q] = S.shv[g][q];
+                xs[q] = S.cxs[s][q];
+                ys[q] = S.cys[s][q];
+                xr[q] = S.cxr[s][q];
+                yr[q] = S.cyr[s][q];
+            }
+            const double g0 = S.gc[s][0], g1 = S.gc[s][1], g2 = S.gc[s][2], g3 = S.gc[s][3], g4 = S.gc[s][4];
+            LDS_FENCE();
+            double a = 0.0, b = 0.0, ar = 0.0, br = 0.0;
+            for (int q = 0; q < 9; ++q) {
+                a = a + sh[q] * xs[q];
+                b = b + sh[q] * ys[q];
+                ar = ar + sh[q] * xr[q];
+                br = br + sh[q] * yr[q];
+            }
+            S.xg[s][g] = a;
+            S.yg[s][g] = b;
+            S.xl[s][g] = ar;
+            S.yl[s][g] = br;
+            double bas[6];
+            basis6(PG5[i], PG5[j], g0, g1, g2, g3, g4, bas);
+            for (int c = 0; c < 5; ++c) S.bq[s][g][c] = bas[c + 1];
         }
         __syncthreads();
         // 2: physical mass centres
         if (tsp < 9) {
             const int s = tsp;
-            double xm = 0.0, ym = 0.0;
-#pragma unroll
+            double rd[25], xg[25], yg[25];
             for (int g = 0; g < 25; ++g) {
-                xm = xm + S.tx[s][g];
-                ym = ym + S.ty[s][g];
+                rd[g] = S.rd[s][g];
+                xg[g] = S.xg[s][g];
+                yg[g] = S.yg[s][g];
             }
-            const double masel = S.gc[s][6];
+            const double masel = S.gc[s][6], vol = S.gc[s][5];
+            LDS_FENCE();
+            double xm = 0.0, ym = 0.0;
+            for (int g = 0; g < 25; ++g) {
+                xm = xm + rd[g] * xg[g];
+                ym = ym + rd[g] * yg[g];
+            }
             double xc = xm / masel, yc = ym / masel;
             rotf(m, xc, yc);
             S.geopl[s][0] = xc;
             S.geopl[s][1] = yc;
-            S.dxcj[s] = sqrt(S.gc[s][5]);
+            S.dxcj[s] = sqrt(vol);
             S.masel[s] = masel;
         }
         __syncthreads();
@@ -611,17 +624,40 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
             S.bl[s][g][1] = (S.yl[s][g] - S.geopl[s][1]) / S.dxcj[s];
         }
         __syncthreads();
-        // 4: second moments
-        if (tsp < 27) {
-            const int s = tsp / 3, c = tsp - 3 * s;
-            double v = 0.0;
-#pragma unroll
-            for (int g = 0; g < 25; ++g) {
-                const double b2 = S.bl[s][g][0], b3 = S.bl[s][g][1], rd = S.rd[s][g];
-                const double x = (c == 1) ? b3 : b2, y = (c == 0) ? b2 : b3;
-                v = v + rd * x * y;
+        // 4: second moments (waves 0-2); the real faces' averaged states for
+        //    the characteristic matrices (wave 3)
+        const int t3 = (tid & 63) * 3 + (tid >> 6);  // task index over waves 0-2 (tid < 192)
+        if (tid < 192) {
+            if (t3 < 27) {
+                const int s = t3 / 3, c = t3 - 3 * s;
+                double x[25], y[25], rd[25];
+                for (int g = 0; g < 25; ++g) {
+                    const double b2 = S.bl[s][g][0], b3 = S.bl[s][g][1];
+                    rd[g] = S.rd[s][g];
+                    x[g] = (c == 1) ? b3 : b2;
+                    y[g] = (c == 0) ? b2 : b3;
+                }
+                const double masel = S.masel[s];
+                LDS_FENCE();
+                double v = 0.0;
+                for (int g = 0; g < 25; ++g) v = v + rd[g] * x[g] * y[g];
+                S.geopl[s][2 + c] = (c < 2) ? 0.5 * (v / masel) : v / masel;
             }
-            S.geopl[s][2 + c] = (c < 2) ? 0.5 * (v / S.masel[s]) : v / S.masel[s];
+        } else if (tid - 192 < S.nface) {
+            const int a = tid - 192;
+            const double dtx = S.fdat[a][0], dty = S.fdat[a][1];
+            const double dlgt = sqrt(dtx * dtx + dty * dty);
+            double dnx = dty / dlgt, dny = -dtx / dlgt;
+            rotf(m, dnx, dny);
+            double ul[4], ur[4], P[9];
+            for (int q = 0; q < 4; ++q) {
+                ul[q] = S.fdat[a][2 + q];
+                ur[q] = S.fdat[a][6 + q];
+            }
+            rotf(m, ul[1], ul[2]);
+            rotf(m, ur[1], ur[2]);
+            char_face(d.gamlg, dnx, dny, ul, ur, P);
+            for (int k = 0; k < 9; ++k) S.fpro[a][k] = P[k];
         }
         __syncthreads();
         // 5: quadratic terms; the basis about the troubled cell's centre
@@ -641,71 +677,125 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         }
         __syncthreads();
         // 6: projection matrices bb (reference -> physical basis) and bbq
-        //    (physical mass matrix); stencil-matrix moments of the neighbours
-        for (int k = tsp; k < 450 + 40 + 80; k += WENO_WG) {
-            if (k < 450) {
-                const int s = k / 50, r = k - 50 * s, which = r / 25, e = r - 25 * which;
-                const int b = e / 5, a = e - 5 * b;
-                double v = 0.0;
-                if (which == 0) {
+        //    (physical mass matrix), entry e of both per task; then the
+        //    stencil-matrix moments of the neighbours (quotients by the
+        //    cell mass through its reciprocal, bit-identical to division)
+        if (tsp < 225) {
+            const int s = tsp / 25, e = tsp - 25 * s, b = e / 5, a = e - 5 * b;
+            double v = 0.0, w = 0.0;
 #pragma unroll
-                    for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bq[s][g][a] * S.bl[s][g][b];
-                    S.bb[s][e] = v;
-                } else {
+            for (int g0 = 0; g0 < 25; g0 += 13) {
+                constexpr int NB = 13;
+                double rd[NB], lb[NB], qa[NB], la[NB];
 #pragma unroll
-                    for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl[s][g][a] * S.bl[s][g][b];
-                    S.bbq[s][e] = v;
+                for (int u = 0; u < NB; ++u) {
+                    const int g = (g0 + u < 25) ? g0 + u : 24;
+                    rd[u] = S.rd[s][g];
+                    lb[u] = S.bl[s][g][b];
+                    qa[u] = S.bq[s][g][a];
+                    la[u] = S.bl[s][g][a];
                 }
-            } else if (k < 490) {
-                const int u = k - 450, s = 1 + u / 5, c = u - 5 * (u / 5);
-                double v = 0.0;
+                LDS_FENCE();
 #pragma unroll
-                for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl0[s][g][c] / S.masel[s];
-                S.s1[s][c] = v;
-            } else {
-                const int u = k - 490, s = 1 + u / 10, r = u - 10 * (u / 10), a = r >> 1, c = r & 1;
-                double v = 0.0;
-#pragma unroll
-                for (int g = 0; g < 25; ++g) v = v + S.rd[s][g] * S.bl0[s][g][a] * S.bl[s][g][c] / S.masel[s];
-                S.mh2[s][a][c] = v;
+                for (int u = 0; u < NB; ++u) {
+                    if (g0 + u < 25) {
+                        v = v + rd[u] * qa[u] * lb[u];
+                        w = w + rd[u] * la[u] * lb[u];
+                    }
+                }
             }
+            S.bb[s][e] = v;
+            S.bbq[s][e] = w;
+        }
+        for (int k = tsp; k < 40 + 80; k += WENO_WG) {
+            // s1: rd * bl0[c] / masel; mh2: rd * bl0[a] * bl[c] / masel
+            int s, a, c;
+            bool lin;
+            if (k < 40) {
+                s = 1 + k / 5;
+                a = k - 5 * (k / 5);
+                c = 0;
+                lin = true;
+            } else {
+                const int u = k - 40, r = u - 10 * (u / 10);
+                s = 1 + u / 10;
+                a = r >> 1;
+                c = r & 1;
+                lin = false;
+            }
+            const Recip rm = recip(S.masel[s]);
+            bool ok = rm.ok;
+            double rd[25], b0[25], bc[25];
+#pragma unroll
+            for (int g = 0; g < 25; ++g) {
+                rd[g] = S.rd[s][g];
+                b0[g] = S.bl0[s][g][a];
+                bc[g] = S.bl[s][g][c];
+            }
+            LDS_FENCE();
+            double v = 0.0;
+#pragma unroll
+            for (int g = 0; g < 25; ++g) {
+                double x = rd[g] * b0[g];
+                if (!lin) x = x * bc[g];
+                v = v + qdiv_term(x, rm, ok);
+            }
+            if (!ok) {
+                v = 0.0;
+                for (int g = 0; g < 25; ++g) {
+                    double x = S.rd[s][g] * S.bl0[s][g][a];
+                    if (!lin) x = x * S.bl[s][g][c];
+                    v = v + x / S.masel[s];
+                }
+            }
+            if (lin) S.s1[s][a] = v;
+            else S.mh2[s][a][c] = v;
         }
         __syncthreads();
         // 7: inverses of bbq (9 cells) and bb (troubled cell): determinants
-        //    and cofactors; rhs of the projections; stencil matrices
-        for (int k = tsp; k < 10 + 250 + 180 + 100 + 100; k += WENO_WG) {
-            if (k < 10) {
-                S.det[k] = det5(k < 9 ? S.bbq[k] : S.bb[0]);
-            } else if (k < 260) {
-                const int u = k - 10, mi = u / 25, e = u - 25 * mi, i = e / 5, j = e - 5 * i;
-                S.inv[mi][j + 5 * i] = cof5(mi < 9 ? S.bbq[mi] : S.bb[0], i, j);
-            } else if (k < 440) {
-                const int u = k - 260, s = u / 20, q = (u / 5) & 3, a = u - 5 * (u / 5);
-                double v = 0.0;
-                for (int b = 0; b < 5; ++b) v = v + S.unkrl[s][q][b + 1] * S.bb[s][b + 5 * a];
-                S.l.rhs[s][q][a] = v;
-            } else if (k < 540) {
-                const int u = k - 440, is = u / 25, e = u - 25 * is, kk = e / 5, r = e - 5 * kk;
+        //    and cofactors; rhs of the projections; stencil matrices.  The
+        //    ten 120-term determinants run on wave 0 with the copies, the
+        //    cofactors and sums on waves 1-3, so no wave executes both.
+        if (tid < 64) {
+            if (tid < 10) S.det[tid] = det5(tid < 9 ? S.bbq[tid] : S.bb[0]);
+            for (int k = tid - 10; k >= 0 && k < 100; k += 54) {
+                const int is = k / 25, e = k - 25 * is, kk = e / 5, r = e - 5 * kk;
                 S.l.ms[is][r + 5 * kk] = S.s1[ST1[is][r]][kk];
-            } else {
-                const int u = k - 540, is = u / 25, e = u - 25 * is, b = e / 5, a = e - 5 * b;
-                double v = 0.0;
-                for (int kk = 0; kk < 9; ++kk) v = v + mh_entry(S, is, kk, a) * mh_entry(S, is, kk, b);
-                S.l.ms[4 + is][a + 5 * b] = v;
+            }
+        } else {
+            const int w = ((tid & 63) * 3) + (tid >> 6) - 1;  // 0..191 spread over waves 1-3
+            for (int k = w; k < 250 + 180 + 100; k += 192) {
+                if (k < 250) {
+                    const int mi = k / 25, e = k - 25 * mi, i = e / 5, j = e - 5 * i;
+                    S.inv[mi][j + 5 * i] = cof5(mi < 9 ? S.bbq[mi] : S.bb[0], i, j);
+                } else if (k < 430) {
+                    const int u = k - 250, s = u / 20, q = (u / 5) & 3, a = u - 5 * (u / 5);
+                    double v = 0.0;
+                    for (int b = 0; b < 5; ++b) v = v + S.unkrl[s][q][b + 1] * S.bb[s][b + 5 * a];
+                    S.l.rhs[s][q][a] = v;
+                } else {
+                    const int u = k - 430, is = u / 25, e = u - 25 * is, b = e / 5, a = e - 5 * b;
+                    double v = 0.0;
+                    for (int kk = 0; kk < 9; ++kk) v = v + mh_entry(S, is, kk, a) * mh_entry(S, is, kk, b);
+                    S.l.ms[4 + is][a + 5 * b] = v;
+                }
             }
         }
         __syncthreads();
         // 8: divide by the determinants; determinants and cofactors of the
-        //    stencil matrices
-        for (int k = tsp; k < 8 + 250 + 200; k += WENO_WG) {
-            if (k < 8) {
-                S.l.mdet[k] = det5(S.l.ms[k]);
-            } else if (k < 258) {
-                const int u = k - 8, mi = u / 25, e = u - 25 * mi;
-                S.inv[mi][e] = S.inv[mi][e] / S.det[mi];
-            } else {
-                const int u = k - 258, mi = u / 25, e = u - 25 * mi, i = e / 5, j = e - 5 * i;
-                S.l.mi[mi][j + 5 * i] = cof5(S.l.ms[mi], i, j);
+        //    stencil matrices (determinants on wave 0, the rest on waves 1-3)
+        if (tid < 64) {
+            if (tid < 8) S.l.mdet[tid] = det5(S.l.ms[tid]);
+        } else {
+            const int w = ((tid & 63) * 3) + (tid >> 6) - 1;
+            for (int k = w; k < 250 + 200; k += 192) {
+                if (k < 250) {
+                    const int mi = k / 25, e = k - 25 * mi;
+                    S.inv[mi][e] = S.inv[mi][e] / S.det[mi];
+                } else {
+                    const int u = k - 250, mi = u / 25, e = u - 25 * mi, i = e / 5, j = e - 5 * i;
+                    S.l.mi[mi][j + 5 * i] = cof5(S.l.ms[mi], i, j);
+                }
             }
         }
         __syncthreads();
@@ -725,35 +815,81 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
             }
         }
         __syncthreads();
-        // 10: Hermite gradient moments; Lagrange candidates; candidate averages
-        for (int k = tsp; k < 64 + 80 + 36 + 20; k += WENO_WG) {
-            if (k < 64) {
-                const int s = 1 + (k >> 3), q = (k >> 1) & 3, c = k & 1;
-                double v = 0.0;
-                for (int g = 0; g < 25; ++g) {
-                    const double rd = S.rd[s][g];
+        // 10: Hermite gradient moments rh2 (8 cells x 4 variables x 2
+        //     directions; 125 terms each, in the order g, a): lane j of a
+        //     quad computes the terms of the Gauss points g = j, j+4, ...,
+        //     then every lane of the quad sums all 125 in order, taking them
+        //     from the quad by DPP broadcast.
+        {
+            const int task = tid >> 2, j = tid & 3;
+            const int s = 1 + (task >> 3), q = (task >> 1) & 3, c = task & 1;
+            const Recip rm = recip(S.masel[s]);
+            bool ok = rm.ok;
+            double u[5];
+            for (int a = 0; a < 5; ++a) u[a] = S.unkpl[s][q][a + 1];
+            double t[7][5];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const int g = (4 * k + j < 25) ? 4 * k + j : 24;
+                double bl[5];
+                for (int a = 0; a < 5; ++a) bl[a] = S.bl[s][g][a];
+                const double rd = S.rd[s][g], bc = S.bl[s][g][c];
+                LDS_FENCE();
+#pragma unroll
+                for (int a = 0; a < 5; ++a) t[k][a] = qdiv_term(rd * bl[a] * bc * u[a], rm, ok);
+            }
+            if (!ok) {
+                for (int k = 0; k < 7; ++k) {
+                    const int g = (4 * k + j < 25) ? 4 * k + j : 24;
                     for (int a = 0; a < 5; ++a)
-                        v = v + rd * S.bl[s][g][a] * S.bl[s][g][c] * S.unkpl[s][q][a + 1] / S.masel[s];
+                        t[k][a] = S.rd[s][g] * S.bl[s][g][a] * S.bl[s][g][c] * S.unkpl[s][q][a + 1] / S.masel[s];
                 }
-                S.rh2[s][q][c] = v;
-            } else if (k < 144) {
-                const int u = k - 64, is = u / 20, q = (u / 5) & 3, a = u - 5 * (u / 5);
+            }
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+#pragma unroll
+                for (int a = 0; a < 5; ++a) v = v + quad_bcast<0>(t[k][a]);
+#pragma unroll
+                for (int a = 0; a < 5; ++a) v = v + quad_bcast<1>(t[k][a]);
+#pragma unroll
+                for (int a = 0; a < 5; ++a) v = v + quad_bcast<2>(t[k][a]);
+#pragma unroll
+                for (int a = 0; a < 5; ++a) v = v + quad_bcast<3>(t[k][a]);
+            }
+#pragma unroll
+            for (int a = 0; a < 5; ++a) v = v + quad_bcast<0>(t[6][a]);
+            if (j == 0) S.rh2[s][q][c] = v;
+        }
+        //     Lagrange candidates; candidate averages
+        for (int k = tsp; k < 80 + 36 + 20; k += WENO_WG) {
+            if (k < 80) {
+                const int is = k / 20, q = (k / 5) & 3, a = k - 5 * (k / 5);
                 double v = 0.0;
                 for (int e = 0; e < 5; ++e)
                     v = v + S.l.mi[is][a + 5 * e] * (S.unkpl[ST1[is][e]][q][0] - S.unkpl[0][q][0]);
                 S.unksl[1 + is][q][a + 1] = v;
-            } else if (k < 180) {
-                const int u = k - 144, s = u >> 2, q = u & 3;
+            } else if (k < 116) {
+                const int u = k - 80, s = u >> 2, q = u & 3;
                 S.unksl[s][q][0] = S.unkpl[0][q][0];
             } else {
-                const int u = k - 180, q = u / 5, a = u - 5 * q;
+                const int u = k - 116, q = u / 5, a = u - 5 * q;
                 S.unksl[0][q][a + 1] = S.unkpl[0][q][a + 1];
             }
         }
         __syncthreads();
-        // 11: Hermite normal-equation right-hand sides
-        if (tsp < 80) {
-            const int is = tsp / 20, q = (tsp / 5) & 3, a = tsp - 5 * (tsp / 5);
+        // 11: Hermite normal-equation right-hand sides (waves 0-2); the left
+        //     characteristic matrices of the real faces (wave 3)
+        if (tid >= 192) {
+            const int a = tid - 192;
+            if (a < S.nface) {
+                double P[9];
+                for (int k = 0; k < 9; ++k) P[k] = S.fpro[a][k];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) S.qi[a][e >> 2][e & 3] = char_entry(d.gamlg, P, e);
+            }
+        } else if (t3 < 80) {
+            const int is = t3 / 20, q = (t3 / 5) & 3, a = t3 - 5 * (t3 / 5);
             double v = 0.0;
             for (int kk = 0; kk < 9; ++kk) {
                 double rh;
@@ -768,9 +904,18 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
             S.l.rs[is][q][a] = v;
         }
         __syncthreads();
-        // 12: Hermite candidates
-        if (tsp < 80) {
-            const int is = tsp / 20, q = (tsp / 5) & 3, a = tsp - 5 * (tsp / 5);
+        // 12: Hermite candidates (waves 0-2); the right characteristic
+        //     matrices (wave 3)
+        if (tid >= 192) {
+            const int a = tid - 192;
+            if (a < S.nface) {
+                double P[9];
+                for (int k = 0; k < 9; ++k) P[k] = S.fpro[a][k];
+#pragma unroll
+                for (int e = 16; e < 32; ++e) S.qm[a][(e - 16) >> 2][e & 3] = char_entry(d.gamlg, P, e);
+            }
+        } else if (t3 < 80) {
+            const int is = t3 / 20, q = (t3 / 5) & 3, a = t3 - 5 * (t3 / 5);
             double v = 0.0;
             for (int e = 0; e < 5; ++e) v = v + S.l.mi[4 + is][a + 5 * e] * S.l.rs[is][q][e];
             S.unksl[5 + is][q][a + 1] = v;
@@ -1041,8 +1186,6 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
 // k_q2_shape_tables with the same arithmetic) staged in LDS.  Face slots
 // are shared through LDS.  Results are identical to k_q2_corner's.
 constexpr int CSC_CELLS = 64;
-constexpr int SHP_POINTS = 150;  // 25 Gauss points, 4 x 25 sub-cell points, 25 Lobatto points
-constexpr int SHP_STRIDE = 18;   // dN/dr[9], dN/ds[9]
 
 __global__ void __launch_bounds__(256) k_q2_shape_tables(double *tab)
 {
@@ -1066,6 +1209,7 @@ __global__ void __launch_bounds__(256) k_q2_shape_tables(double *tab)
     for (int q = 0; q < 9; ++q) {
         tab[k * SHP_STRIDE + q] = dr_[q];
         tab[k * SHP_STRIDE + 9 + q] = ds_[q];
+        if (k < 25) tab[SHP_POINTS * SHP_STRIDE + k * 9 + q] = sh[q];
     }
 }
 
@@ -1314,7 +1458,10 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
     const int tid = threadIdx.x;
     const int nc = d.nc;
     const int ie0 = xcd_block(blockIdx.x, gridDim.x) * RHS_CPB;
-    if (blockIdx.x == 0 && tid == 0) *d.tcount = 0;  // WENO list of this stage consumed
+    if (blockIdx.x == 0 && tid == 0) {  // WENO list of this stage consumed
+        *d.tsum += (unsigned long long)*d.tcount;
+        *d.tcount = 0;
+    }
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
     // corners on threads 0-127, points on threads 128-255 and 0-71
     {
@@ -1495,7 +1642,10 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc;
-    if (ie == 0) *d.tcount = 0;  // WENO list of this stage consumed
+    if (ie == 0) {  // WENO list of this stage consumed
+        *d.tsum += (unsigned long long)*d.tcount;
+        *d.tcount = 0;
+    }
     if (ie >= nc) return;
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
     const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
@@ -1704,6 +1854,85 @@ using namespace lag2d;
 
 static inline int nblk(int n) { return (n + LAG2D_BLOCK - 1) / LAG2D_BLOCK; }
 
+// Time-invariant data of every cell's WENO stencil (the mesh connectivity
+// and the initial geometry and density do not change), one thread per cell:
+//   wtop[0..8]   stencil cells: the cell, its face neighbours (>= nc on a
+//                boundary face) and the vertex neighbour opposite each corner,
+//                found by the product of the face numbers it shares
+//                (WENO.f90:7659-7700); -2 where none
+//   wtop[9]      bit 0: skip (boundary stencil or unclassified vertex
+//                neighbourhood), bit 1: a neighbourhood the reference's
+//                builder stops on (WENO.f90:7699)
+//   wtop[10]     number of real faces, wtop[11..14] their face numbers
+//   wrd[g]       rho0 |J| w at Gauss point g of the initial map, rho0 the
+//                initial density at the initial mass centre (WENO.f90:5989-6061)
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_weno_setup(Dev d, int *wtop, double *wrd)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc;
+    if (ie >= nc) return;
+    int ielaj[9];
+    ielaj[0] = ie;
+    for (int f = 0; f < 4; ++f) ielaj[1 + f] = d.esqua[f * nc + ie];
+    for (int k = 5; k < 9; ++k) ielaj[k] = -2;
+    int jel[4][3];
+    for (int iv = 0; iv < 4; ++iv) {
+        int nv = 0;
+        for (int a = 0; a < 3; ++a) jel[iv][a] = -1;
+        const int p = d.ipq[iv * nc + ie];
+        for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
+            const int je = d.esuv1[k];
+            if (je != ie && nv < 3) jel[iv][nv++] = je;
+        }
+    }
+    bool mismatch = false;
+    for (int iv = 0; iv < 4; ++iv) {
+        int idxod = 1, idxiv[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a)
+            for (int f = 0; f < 4; ++f)
+                if (jel[iv][a] == ielaj[1 + f]) {
+                    idxod = idxod * (f + 1);
+                    idxiv[a] = 1;
+                }
+        for (int a = 0; a < 3; ++a) {
+            if (idxiv[a]) continue;
+            if (idxod == 2) ielaj[5] = jel[iv][a];
+            else if (idxod == 6) ielaj[6] = jel[iv][a];
+            else if (idxod == 12) ielaj[7] = jel[iv][a];
+            else if (idxod == 4) ielaj[8] = jel[iv][a];
+            else mismatch = true;
+        }
+    }
+    int skip = mismatch ? 1 : 0;
+    for (int k = 0; k < 9; ++k)
+        if (ielaj[k] < 0) skip = 1;
+    int nf = 0, face[4] = {0, 0, 0, 0};
+    for (int f = 0; f < 4; ++f)
+        if (ielaj[1 + f] < nc) face[nf++] = f;
+    for (int k = 0; k < 9; ++k) wtop[k * nc + ie] = ielaj[k];
+    wtop[9 * nc + ie] = skip | (mismatch ? 2 : 0);
+    wtop[10 * nc + ie] = nf;
+    for (int a = 0; a < 4; ++a) wtop[(11 + a) * nc + ie] = face[a];
+    wtop[15 * nc + ie] = 0;
+
+    double xi[9], yi[9];
+    for (int q = 0; q < 9; ++q) {
+        const int p = d.ipq[q * nc + ie];
+        xi[q] = d.x0[p];
+        yi[q] = d.y0[p];
+    }
+    double xcr, ycr;
+    map9(xi, yi, d.G[0 * nc + ie], d.G[1 * nc + ie], &xcr, &ycr);
+    const double rho = d.rhoi ? d.rhoi[ie] : rho0(d.ncase, xcr, ycr);
+    for (int g = 0; g < 25; ++g) {
+        const int i = g / 5, j = g - 5 * i;
+        double J[4];
+        jac9(xi, yi, PG5[i], PG5[j], J);
+        const double djacoi = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
+        wrd[g * nc + ie] = rho * djacoi;
+    }
+}
+
 int upload_extra(lag2d_ctx &x)
 {
     if (x.cfg.ncurv != 1) return 0;
@@ -1711,11 +1940,22 @@ int upload_extra(lag2d_ctx &x)
     if (rc) return rc;
     if (!x.d.shp) {
         double *t;
-        if ((rc = x.alloc(&t, (size_t)SHP_POINTS * SHP_STRIDE))) return rc;
+        if ((rc = x.alloc(&t, (size_t)SHP_POINTS * SHP_STRIDE + 25 * 9))) return rc;
         x.d.shp = t;
+        x.d.shv = t + SHP_POINTS * SHP_STRIDE;
         hipLaunchKernelGGL(k_q2_shape_tables, dim3(1), dim3(256), 0, x.stream, t);
     }
     hipLaunchKernelGGL(k_q2_submass, dim3(nblk(x.nc)), dim3(LAG2D_BLOCK), 0, x.stream, x.d);
+    if (!x.d.wtop) {
+        int *t;
+        double *r;
+        if ((rc = x.alloc(&t, (size_t)16 * x.nc))) return rc;
+        if ((rc = x.alloc(&r, (size_t)25 * x.nc))) return rc;
+        x.d.wtop = t;
+        x.d.wrd = r;
+    }
+    hipLaunchKernelGGL(k_q2_weno_setup, dim3(nblk(x.nc)), dim3(LAG2D_BLOCK), 0, x.stream, x.d, (int *)x.d.wtop,
+                       (double *)x.d.wrd);
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
 }
 

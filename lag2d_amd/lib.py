@@ -15,10 +15,10 @@ from .case import LagCase
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liblag2d.so")
 
-EXPORTS = ("lag2d_create", "lag2d_create_ensemble", "lag2d_get_run_times", "lag2d_set_initial_density",
+EXPORTS = ("lag2d_create", "lag2d_create_ensemble", "lag2d_get_run_times", "lag2d_set_initial_density", "lag2d_weno_cells",
            "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
-           "lag2d_destroy", "lag2d_last_error")
+           "lag2d_division_check", "lag2d_destroy", "lag2d_last_error")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -54,6 +54,8 @@ def load():
                                           C.c_int, _ip, _ip, _ip, _ip, C.POINTER(C.c_void_p)]
     lib.lag2d_get_run_times.argtypes = [C.c_void_p, _dp, _dp]
     lib.lag2d_set_initial_density.argtypes = [C.c_void_p, _dp]
+    lib.lag2d_weno_cells.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    lib.lag2d_division_check.argtypes = [C.c_int, _dp, _dp, _dp]
     lib.lag2d_upload.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.lag2d_step.argtypes = [C.c_void_p, C.c_int]
     lib.lag2d_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip]
@@ -169,6 +171,12 @@ class Lag2D:
             if calls.value:
                 stats[nm.value.decode()] = (tot.value, calls.value)
         return name.value.decode(), avg.value, stats
+
+    def weno_cells(self):
+        """Troubled cells WENO processed since timing was last enabled."""
+        n = C.c_longlong()
+        _check(load().lag2d_weno_cells(self.h, C.byref(n)))
+        return n.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -1,40 +1,51 @@
-"""Roofline figures of the dominant kernel of a step.
+"""Roofline figures of the dominant kernel of a step (curved P2 path).
 
-Algorithmic bytes per launch count every array a kernel must read or write
-once (DOFs, coordinates gathered through the connectivity, geometry rows,
-corner records); algorithmic flops per launch are the FP64 operations of the
-kernel's arithmetic per cell (or node) times the cells (nodes) of a launch.
-The per-cell operation counts below are those of the kernels' source loops
-(shape functions evaluated at every quadrature point, no contraction); they
-agree with the executed FP64 VALU instruction counts rocprofv3 reports for the
-same kernels (profiles/ holds those summaries).
+Algorithmic flops per unit are the FP64 operations (+, -, *, /, sqrt) the
+reference's algorithm performs for the work a kernel does, counted by the
+op-counting build of the oracle (oracle/opcount/count.py, which prints this
+table; profiles/opcount_sedov_p2_120x120.txt holds its output).  The units:
+a cell (per stage), a node, or -- for WENO -- a troubled cell, whose count
+per launch the library reports (lag2d_weno_cells).
+
+Algorithmic bytes per unit count the compulsory HBM traffic of a launch:
+every cell-indexed array the kernel reads or writes, once; node arrays at
+their per-cell share (np/nc nodes per cell); connectivity at 4 bytes.
+
+Peaks: MI355X HBM3E 8 TB/s; FP64 78.6 TFLOP/s (vector and matrix, counting
+a fused multiply-add as two flops).  The reference's rounding rules out
+fused multiply-adds, so add/multiply code can execute at most half the
+FP64 peak; frac is quoted against the published peak all the same.
 """
 from __future__ import annotations
 
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 
-# FP64 flops per cell (per node for the node kernels), curved P2 path
+# FP64 operations per unit, from oracle/opcount/count.py (120x120 Sedov P2)
 FLOPS_Q2 = {
-    "k_q2_prep": 25 * (189 + 5 + 10 + 12 + 6 + 7) + 30,
-    "k_q2_corner": 720 + 2400 + 1836 + 4 * (14 + 25 * 194) + 8 + 100 * 227 + 522 + 16 * 15 + 32 * 20,
-    "k_node_q2": 8 * 10 + 16,
-    "k_cellrhs_q2": 9 * 10 + 16 * 6 * 2 * 14 + 25 * (189 + 10 + 48 + 10 + 6 * 18) + 6 * 6 * 4 * 2 + 24 * 4,
-    "k_cellvol": 36 * (189 + 10),
-    "k_update": 6 * 6 * 4 * 2 + 24 * 4,
-    "k_nodemove": 8,
+    "k_q2_prep": 6477.0,                   # per cell: Persson indicator, 25 Gauss points
+    "k_q2_weno": 434244.6,                 # per troubled cell, incl. its stencil's 9 mass centres
+    "k_q2_corner": 55633.0,                # per cell: face geometry, sub-cell density and mean tau, records
+    "k_node_q2": 78.5,                     # per node: gather and 2x2 solve
+    "k_cellrhs_q2": 15614.0 + 432.0 * 2 / 3,  # per cell: face + domain integral, limiter; M^-1 update in 2 of 3 stages
+    "k_update": 432.0,                     # per cell (stage 1)
+    "k_nodemove": 12.0,                    # per node
+    "k_cellvol": 9015.5,                   # per cell and call, incl. the centre-node reset
 }
-# algorithmic bytes per cell (per node for node kernels)
+
+# compulsory bytes per unit (see the module docstring)
+_NODE_SHARE = 4.03  # nodes per cell of a Q2 quad mesh (58081 / 14400)
 BYTES_Q2 = {
-    "k_q2_prep": 8 * (24 + 24 + 18 + 5) + 4 * 9,
-    "k_q2_corner": 8 * (24 + 18 + 5 + 4 + 18 + 48 + 160) + 4 * 9,
-    "k_node_q2": 8 * (8 * 5 + 4) + 4 * 8 + 1,
-    "k_cellrhs_q2": 8 * (24 + 48 + 160 + 18 + 18 + 5 + 16 + 24 + 24) + 4 * 9,
-    "k_cellvol": 8 * (18 + 3) + 4 * 9,
-    "k_update": 8 * (24 * 3 + 16 + 24),
+    "k_q2_prep": 8 * (24 + 24 + 5) + 4 * 9 + 16 * _NODE_SHARE + 1,
+    "k_q2_weno": 9 * (8 * (24 + 7) + 4 * 9 + 32 * 9) + 8 * 20,
+    "k_q2_corner": 8 * (24 + 5 + 4 + 48 + 160) + 4 * 9 + 32 * _NODE_SHARE,
+    "k_node_q2": 8 * 2 * 5 * 4 + 4 * 5 + 1 + 32,
+    "k_cellrhs_q2": 8 * (24 + 160 + 48 + 5 + 16 + 24 + 24) + 4 * 9 + 32 * _NODE_SHARE,
+    "k_update": 8 * (16 + 24 * 4),
     "k_nodemove": 8 * 8,
+    "k_cellvol": 8 * 3 + 4 * 9 + 16 * _NODE_SHARE + 2,
 }
-NODE_KERNELS = {"k_node_q2", "k_nodemove", "k_node_q1", "k_nodeminmax"}
+NODE_KERNELS = {"k_node_q2", "k_nodemove"}
 
 # HBM traffic per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (KiB per dispatch, mean over the dispatches of a 3-step run of the 120x120
@@ -54,20 +65,27 @@ def traffic(kernel: str, case):
     return (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0
 
 
-def figures(kernel: str, case, avg_ms: float) -> dict:
-    units = case.npoin if kernel in NODE_KERNELS else case.ncell
-    flops = FLOPS_Q2.get(kernel, 0) * units
-    nbytes = BYTES_Q2.get(kernel, 0) * units
+def units(kernel: str, case, weno_cells_per_launch: float = 0.0) -> float:
+    if kernel == "k_q2_weno":
+        return weno_cells_per_launch
+    return case.npoin if kernel in NODE_KERNELS else case.ncell
+
+
+def figures(kernel: str, case, avg_ms: float, weno_cells_per_launch: float = 0.0) -> dict:
+    n = units(kernel, case, weno_cells_per_launch)
+    flops = FLOPS_Q2.get(kernel, 0.0) * n
+    nbytes = BYTES_Q2.get(kernel, 0.0) * n
     t = avg_ms * 1e-3
     t_flop = flops / (PEAK_FP64_TFLOPS * 1e12)
     t_mem = nbytes / (PEAK_HBM_GBS * 1e9)
+    common = {"kernel": kernel, "traffic": traffic(kernel, case), "algorithmic_flops": flops,
+              "algorithmic_bytes": nbytes, "units_per_launch": n, "avg_ms": avg_ms}
     if t_flop >= t_mem:
         achieved = flops / t / 1e12
-        return {"kernel": kernel, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic(kernel, case),
-                "algorithmic_flops": flops, "algorithmic_bytes": nbytes, "avg_ms": avg_ms,
-                "note": "FP64 vector peak (78.6 TFLOP/s, same as FP64 matrix); kernel uses VALU FP64"}
+        return {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, **common,
+                "note": "FP64-bound (VALU; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s, "
+                        "FMA counted as 2; no contraction allowed)"}
     achieved = nbytes / t / 1e9
-    return {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic(kernel, case), "algorithmic_flops": flops,
-            "algorithmic_bytes": nbytes, "avg_ms": avg_ms}
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, **common}

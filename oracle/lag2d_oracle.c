@@ -726,6 +726,7 @@ int orc_step(orc_case *c)
 {
     static const double rklag[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
     const int nd = c->ndegr, nq = c->nq;
+    ORC_PHASE(ORC_PH_CELLVOL);
     orc_cellvol(c, c->coord);
     double *dvdt = (double *)malloc(sizeof(double) * c->ncell);
     double m[6][6];
@@ -733,6 +734,7 @@ int orc_step(orc_case *c)
         double alfa = rklag[c->nstag - 2][istag];
         int rc = orc_rhs(c);
         if (rc) { free(dvdt); return rc; }
+        ORC_PHASE(ORC_PH_DT);
         if (c->ndt == 1) {
             if (istag == 0) {
                 for (int ie = 0; ie < c->ncell; ++ie) dvdt[ie] = U(c, c->rhsel, 0, 0, ie);
@@ -743,6 +745,7 @@ int orc_step(orc_case *c)
         } else if (istag == 0) {
             c->tacru = c->tacru + c->dtfix;
         }
+        ORC_PHASE(ORC_PH_UPDATE);
         for (int i = 0; i < nd * nq * c->ncell; ++i) c->rhsel[i] = c->dtfix * c->rhsel[i];
         for (int ie = 0; ie < c->ncell; ++ie) {
             const double *a = c->amatr + c->nmatr * ie;
@@ -771,10 +774,12 @@ int orc_step(orc_case *c)
                                                 (1.0 - alfa) * (U(c, c->unold, id, q, ie) + un[q]);
             }
         }
+        ORC_PHASE(ORC_PH_NODEMOVE);
         for (int p = 0; p < c->npoin; ++p)
             for (int d = 0; d < 2; ++d)
                 X(c->cooro, d, p) = alfa * X(c->coord, d, p) +
                                     (1.0 - alfa) * (X(c->cooro, d, p) + c->dtfix * X(c->ustar, d, p));
+        ORC_PHASE(ORC_PH_CELLVOL);
         if (c->ncurv == 1) {
             for (int ie = 0; ie < c->nquad; ++ie) {
                 const int *ip = c->ipqua + c->nvqua * ie;
@@ -796,4 +801,3 @@ int orc_step(orc_case *c)
     return 0;
 }
 
-double orc_flops(void) { return 0.0; }

@@ -58,9 +58,23 @@ int orc_rhs(orc_case *c);
 int orc_step(orc_case *c);
 /* curved P2 sub-cell right-hand side (lag2d_oracle_q2.c) */
 int orc_rhs_q2(orc_case *c);
-/* operation counter (FP64 flops) of the last orc_step, when built with
- * ORC_COUNT_FLOPS */
-double orc_flops(void);
+/* Op counting (oracle/opcount/): the arithmetic after a phase marker is
+ * attributed to that phase, named for the GPU kernel that does the same
+ * work (WENO_MC: the physical mass centres the reference computes for every
+ * cell, of which a troubled cell's stencil needs 9).  No-op in the ordinary
+ * build. */
+enum orc_phase {
+    ORC_PH_OTHER = 0, ORC_PH_PREP, ORC_PH_WENO_MC, ORC_PH_WENO, ORC_PH_CORNER, ORC_PH_NODE,
+    ORC_PH_CELLRHS, ORC_PH_UPDATE, ORC_PH_NODEMOVE, ORC_PH_CELLVOL, ORC_PH_DT,
+    ORC_PH_HOISTED,  /* time-invariant work the GPU path does once: initial sub-cell masses */
+    ORC_PH_COUNT
+};
+#ifndef ORC_PHASE
+#define ORC_PHASE(k) ((void)0)
+#endif
+#ifndef ORC_WENO_CELL
+#define ORC_WENO_CELL() ((void)0)
+#endif
 
 #ifdef __cplusplus
 }

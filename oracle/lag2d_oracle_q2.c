@@ -735,6 +735,7 @@ static int weno(orc_case *c, const double *flag, const int *esqua)
     const int nc = c->ncell, nd = c->ndegr;
     double *geoph = (double *)malloc(sizeof(double) * 2 * nc);
     double *mapmt = (double *)malloc(sizeof(double) * 4 * nc);
+    ORC_PHASE(ORC_PH_WENO_MC);
     for (int ie = 0; ie < nc; ++ie) {
         double xp[2][9], xi[2][9];
         q2_nodes(c, c->cooro, ie, xp);
@@ -772,8 +773,10 @@ static int weno(orc_case *c, const double *flag, const int *esqua)
     E.geoph = geoph;
     E.unknr = unknr;
     int bad = 0;
+    ORC_PHASE(ORC_PH_WENO);
     for (int ie = 0; ie < nc; ++ie) {
         if (flag[ie] < 10.0) continue;
+        ORC_WENO_CELL();
         int r = weno_cell(&E, ie, mapmt + 4 * ie, esqua, c->unold);
         if (r) {
             ++bad;
@@ -802,7 +805,9 @@ static void subcell_density(const orc_case *c, int ie, double rsub[4])
     double xc[2][12], xic[2][12];
     for (int k = 0; k < 12; ++k) {
         q2_map(xp, posqc[k][0], posqc[k][1], &xc[0][k], &xc[1][k]);
+        ORC_PHASE(ORC_PH_HOISTED);
         q2_map(xi, posqc[k][0], posqc[k][1], &xic[0][k], &xic[1][k]);
+        ORC_PHASE(ORC_PH_CORNER);
     }
     double f0[4];
     for (int isg = 0; isg < 4; ++isg) {
@@ -820,6 +825,7 @@ static void subcell_density(const orc_case *c, int ie, double rsub[4])
         }
         f0[isg] = 1.0 / a;
     }
+    ORC_PHASE(ORC_PH_HOISTED);
     for (int isg = 0; isg < 4; ++isg) {
         double xs[2][9];
         for (int d = 0; d < 2; ++d) {
@@ -837,8 +843,11 @@ static void subcell_density(const orc_case *c, int ie, double rsub[4])
             double rhoi = (c->ncase == 6) ? rho_initial(c, xcc, ycc) : 1.0;
             ms = ms + rhoi * 1.0 * djaco;
         }
+        ORC_PHASE(ORC_PH_CORNER);
         rsub[isg] = 0.0 + f0[isg] * ms;
+        ORC_PHASE(ORC_PH_HOISTED);
     }
+    ORC_PHASE(ORC_PH_CORNER);
 }
 
 /* getrhosubg_daverg (Getrhslaggauss.f90:10949-11109): mass-weighted mean of
@@ -883,6 +892,7 @@ static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
     double *mn = (double *)calloc((size_t)4 * np, sizeof(double));
     double *mu = (double *)calloc((size_t)2 * np, sizeof(double));
     double *sn = (double *)calloc((size_t)2 * np, sizeof(double));
+    ORC_PHASE(ORC_PH_CORNER);
     double *mcl = (double *)malloc(sizeof(double) * 32 * nc);  /* m = mu*A per corner face */
     double *mul = (double *)malloc(sizeof(double) * 64 * nc);  /* m u_c */
     double *snl = (double *)malloc(sizeof(double) * 64 * nc);  /* A sigma n */
@@ -940,6 +950,7 @@ static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
                     double mu1 = m11 * uv + 0.0 * vv, mu2 = 0.0 * uv + m11 * vv;
                     double sn1 = -pv * area * vn[iv][f][0] + 0.0 * area * vn[iv][f][1];
                     double sn2 = 0.0 * area * vn[iv][f][0] + -pv * area * vn[iv][f][1];
+                    ORC_PHASE(ORC_PH_NODE);
                     mn[0 + 4 * p] += m11;
                     mn[1 + 4 * p] += 0.0;
                     mn[2 + 4 * p] += 0.0;
@@ -948,6 +959,7 @@ static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
                     mu[2 * p + 1] += mu2;
                     sn[2 * p] += sn1;
                     sn[2 * p + 1] += sn2;
+                    ORC_PHASE(ORC_PH_CORNER);
                     int o = f + 2 * (iv + 4 * isg);
                     mcl[o + 32 * ie] = m11;
                     mul[2 * o + 64 * ie] = mu1;
@@ -958,6 +970,7 @@ static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
             }
         }
     }
+    ORC_PHASE(ORC_PH_NODE);
     int *indnd = (int *)calloc((size_t)np, sizeof(int));
     for (int f = 0; f < c->nbfac; ++f) {
         const int *b = c->bface + 5 * f;
@@ -986,6 +999,7 @@ static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
             }
         }
     }
+    ORC_PHASE(ORC_PH_CELLRHS);
     for (int ie = 0; ie < nc; ++ie) {
         const int *ip = c->ipqua + 9 * ie;
         for (int isg = 0; isg < 4; ++isg)
@@ -1109,13 +1123,16 @@ int orc_rhs_q2(orc_case *c)
     double *gq = (double *)malloc(sizeof(double) * 48 * nc);
     double *flag = (double *)malloc(sizeof(double) * nc);
     double *fstar = (double *)malloc(sizeof(double) * 64 * nc);
+    ORC_PHASE(ORC_PH_CORNER);
     fnds_q2(c, c->cooro, gq);
+    ORC_PHASE(ORC_PH_PREP);
     persson(c, c->cooro, flag);
     int rc = weno(c, flag, c->esqua);
     if (rc) {
         free(gq); free(flag); free(fstar);
         return rc;
     }
+    ORC_PHASE(ORC_PH_OTHER);
     for (int ie = 0; ie < nc; ++ie) {
         GE(c, 9, ie) = 0.0;
         if (c->troubled) c->troubled[ie] = flag[ie];
@@ -1126,6 +1143,7 @@ int orc_rhs_q2(orc_case *c)
         c->afvec[4 * ie + 3] = 1.0;
     }
     ndvelo_q2(c, gq, fstar);
+    ORC_PHASE(ORC_PH_CELLRHS);
     iface_q2(c, gq, fstar);
     domn_q2(c);
     /* derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708) */

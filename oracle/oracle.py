@@ -48,6 +48,29 @@ def lib():
     return _LIB
 
 
+_OPLIB = None
+
+
+def opcount_lib():
+    """The op-counting build of the same sources (oracle/opcount/)."""
+    global _OPLIB
+    if _OPLIB is None:
+        path = os.path.join(_HERE, "opcount", "liblag2d_opcount.so")
+        if not os.path.exists(path):
+            raise RuntimeError("op-counting oracle not built: run `make -C oracle/opcount`")
+        _OPLIB = C.CDLL(path)
+        _OPLIB.orc_rhs.argtypes = [C.POINTER(_Case)]
+        _OPLIB.orc_step.argtypes = [C.POINTER(_Case)]
+        _OPLIB.orc_cellvol.argtypes = [C.POINTER(_Case), _dp]
+        _OPLIB.orc_opcount.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_longlong), C.c_int]
+    return _OPLIB
+
+
+# phases of the op counts (enum orc_phase in lag2d_oracle.h)
+PHASES = ("other", "prep", "weno_mc", "weno", "corner", "node", "cellrhs", "update", "nodemove", "cellvol", "dt",
+          "hoisted")
+
+
 def _p(a, t=_dp):
     return a.ctypes.data_as(t)
 
@@ -55,8 +78,9 @@ def _p(a, t=_dp):
 class Oracle:
     """Holds one problem's state and advances it with the C restatement."""
 
-    def __init__(self, case, strict=False):
+    def __init__(self, case, strict=False, counting=False):
         c = case.ctl
+        self.lib = opcount_lib() if counting else lib()
         self.case = case
         self.ipqua = np.ascontiguousarray(case.ipqua, dtype=np.int32)
         self.bface = np.ascontiguousarray(case.bface, dtype=np.int32)
@@ -98,9 +122,9 @@ class Oracle:
 
     def rhs(self):
         """One right-hand side on the current state (stage 1 of step 1)."""
-        lib().orc_cellvol(C.byref(self.s), _p(self.coord))
+        self.lib.orc_cellvol(C.byref(self.s), _p(self.coord))
         self.s.itime = max(self.s.itime, 1)
-        rc = lib().orc_rhs(C.byref(self.s))
+        rc = self.lib.orc_rhs(C.byref(self.s))
         if rc:
             raise RuntimeError(f"oracle rhs: unsupported configuration ({rc})")
         return self.rhsel
@@ -108,11 +132,19 @@ class Oracle:
     def step(self, n=1):
         for _ in range(n):
             self.s.itime += 1
-            rc = lib().orc_step(C.byref(self.s))
+            rc = self.lib.orc_step(C.byref(self.s))
             if rc:
                 raise RuntimeError(f"oracle step: unsupported configuration ({rc})")
             if abs(self.s.tacru - self.case.ctl.tend) / self.case.ctl.tend <= 1e-11:
                 break
+
+    def op_counts(self, reset=True):
+        """FP64 operations per phase since the last reset, and the number of
+        troubled cells WENO processed (op-counting build only)."""
+        ops = (C.c_ulonglong * 16)()
+        nw = C.c_longlong()
+        self.lib.orc_opcount(ops, C.byref(nw), 1 if reset else 0)
+        return {k: int(ops[i]) for i, k in enumerate(PHASES)}, int(nw.value)
 
     @property
     def dtfix(self):
