@@ -445,8 +445,8 @@ __device__ __forceinline__ double quad_bcast(double v)
 {
     constexpr int ctrl = J | (J << 2) | (J << 4) | (J << 6);
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, ctrl, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, ctrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), ctrl, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
