@@ -35,10 +35,15 @@ class LagCase:
     tacru: float = 0.0
     itime: int = 0
     rho0: np.ndarray | None = None  # (ncell,) initial density per cell, when known
+    iptri: np.ndarray | None = None  # (ntria, 3) int32, 0-based; cells 0..ntria-1 are triangles
+
+    @property
+    def ntria(self):
+        return 0 if self.iptri is None else self.iptri.shape[0]
 
     @property
     def ncell(self):
-        return self.ipqua.shape[0]
+        return self.ntria + self.ipqua.shape[0]
 
     @property
     def nquad(self):

@@ -28,6 +28,11 @@ class Mesh:
     ncurv: int             # 0 straight Q1, 1 quadratic Q2
     nx: int
     ny: int
+    iptri: np.ndarray = None  # (ntria, 3) int32, 1-based, counter-clockwise; None: no triangles
+
+    @property
+    def ntria(self) -> int:
+        return 0 if self.iptri is None else self.iptri.shape[0]
 
     @property
     def npoin(self) -> int:
@@ -87,6 +92,31 @@ def quad_q1(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1) -> Mesh:
         for i in range(nx):
             ipq[i + nx * j] = (nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1))
     return Mesh(coord, ipq, _boundary_faces(nid, nx, ny, False), 0, nx, ny)
+
+
+def split_quads(mesh: Mesh, fraction=0.3, seed=0) -> Mesh:
+    """Hybrid mesh: a seeded ``fraction`` of the straight quads is split into
+    two counter-clockwise triangles along a seeded diagonal (1-3 or 2-4).
+    The reference numbers triangles before quads (Preprocess.f90:168-198):
+    element e < ntria is a triangle, element ntria + q is the q-th remaining
+    quad.  Boundary faces are unchanged."""
+    if mesh.ncurv != 0:
+        raise ValueError("only straight quads are split")
+    rng = np.random.default_rng(seed)
+    nq = mesh.nquad
+    split = rng.random(nq) < fraction
+    diag = rng.integers(0, 2, nq)
+    tris, quads = [], []
+    for e in range(nq):
+        a, b, c, d = (int(v) for v in mesh.ipqua[e])
+        if not split[e]:
+            quads.append((a, b, c, d))
+        elif diag[e] == 0:
+            tris += [(a, b, c), (a, c, d)]
+        else:
+            tris += [(a, b, d), (b, c, d)]
+    return Mesh(mesh.coord, np.asarray(quads, dtype=np.int32).reshape(-1, 4), mesh.bface, 0, mesh.nx, mesh.ny,
+                iptri=np.asarray(tris, dtype=np.int32).reshape(-1, 3))
 
 
 def quad_q2(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1, bulge=0.05) -> Mesh:
@@ -222,6 +252,20 @@ def sedov_q1_problem(nx=16, ny=None, seed=7, length=1.2):
     return mesh, ctl
 
 
+def sod_hybrid_problem(nx=40, ny=8, fraction=0.3, seed=3):
+    """The planar Sod tube on a hybrid mesh (a seeded share of the quads split
+    into triangles), P1 / Barth limiter."""
+    mesh = split_quads(quad_q1(nx, ny, 0.0, 1.0, 0.0, 0.04 * ny / 4.0), fraction, seed)
+    return mesh, Control()
+
+
+def sedov_hybrid_problem(nx=16, ny=None, fraction=0.3, seed=9, length=1.2):
+    """Planar Sedov blast on a jittered hybrid mesh, P1 / Barth limiter."""
+    ny = nx if ny is None else ny
+    mesh = split_quads(quad_q1(nx, ny, 0.0, length, 0.0, length * ny / nx, seed=seed), fraction, seed + 1)
+    return mesh, Control(cfl=0.25, tend=1.0, ncase=7)
+
+
 def sedov_problem(nx=120, ny=None, seed=2024, length=1.2):
     ny = nx if ny is None else ny
     mesh = quad_q2(nx, ny, 0.0, length, 0.0, length * ny / nx, seed=seed)
@@ -256,10 +300,12 @@ def write_domn(path, m: Mesh):
         f.write("1\nLagrangian DG mesh\n")
         f.write(" ndimn ntype\n 2 1\n")
         f.write(" ntria npoin nbfac tinpt nquad\n")
-        f.write(f" 0 {m.npoin} {m.nbfac} 0.0 {m.nquad}\n")
+        f.write(f" {m.ntria} {m.npoin} {m.nbfac} 0.0 {m.nquad}\n")
         f.write(" element connectivity\n")
+        for e in range(m.ntria):
+            f.write(f"{e + 1} " + " ".join(str(int(v)) for v in m.iptri[e]) + "\n")
         for e in range(m.nquad):
-            f.write(f"{e + 1} " + " ".join(str(int(v)) for v in m.ipqua[e]) + "\n")
+            f.write(f"{m.ntria + e + 1} " + " ".join(str(int(v)) for v in m.ipqua[e]) + "\n")
         f.write(" coordinates\n")
         for p in range(m.npoin):
             f.write(f"{p + 1} {float(m.coord[p, 0])!r} {float(m.coord[p, 1])!r}\n")

@@ -8,8 +8,8 @@
  * subroutine (file:line) whose arithmetic it restates.
  *
  * Paths covered: planar Lagrangian DG with nriem=1, nlimi=6, ndens=1 and
- *   - ncurv=0, nfint=1, P1 (straight quads, Barth + symmetry-preserving
- *     velocity limiter), Getrhslagmc.f90:6810-7405.
+ *   - ncurv=0, nfint=1, P1 on straight quads and triangles (hybrid meshes),
+ *     Barth + symmetry-preserving velocity limiter, Getrhslagmc.f90:6810-7405.
  */
 #include "lag2d_oracle.h"
 
@@ -21,9 +21,9 @@
 
 /* The 9 geometry points of a quad: Q2 nodes, or for straight quads the
  * corners plus chord midpoints (Limiterlag.f90:2008-2021). */
-static void quad_points9(const orc_case *c, const double *x, int ie, double xp[2][9])
+static void quad_points9(const orc_case *c, const double *x, int iq, double xp[2][9])
 {
-    const int *ip = c->ipqua + c->nvqua * ie;
+    const int *ip = c->ipqua + c->nvqua * iq;
     if (c->ncurv == 0) {
         for (int k = 0; k < 4; ++k) {
             xp[0][k] = X(x, 0, ip[k]);
@@ -44,16 +44,101 @@ static void quad_points9(const orc_case *c, const double *x, int ie, double xp[2
     }
 }
 
+/* rutope, 7-point area rule of the reference triangle r, s in [0, 1]
+ * (dgflo2d.f90:2234-2255), its literal values. */
+static const double PT7[7][2] = {
+    {0.333333333333333, 0.333333333333333}, {0.059715871789770, 0.470142064105115},
+    {0.470142064105115, 0.059715871789770}, {0.470142064105115, 0.470142064105115},
+    {0.797426985353087, 0.101286507323456}, {0.101286507323456, 0.797426985353087},
+    {0.101286507323456, 0.101286507323456}};
+static const double WT7[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506, 0.132394152788506,
+                              0.125939180544827, 0.125939180544827, 0.125939180544827};
+
+/* Reference-triangle vertices (0,0), (1,0), (0,1) and the basis scaling
+ * dr = ds = 0.5 of the P1 triangle basis (Getrhslagmctria.f90:464-470). */
+static const double XVT[3] = {0.0, 1.0, 0.0};
+static const double YVT[3] = {0.0, 0.0, 1.0};
+
 /* ------------------------------------------------------------------------ */
-/* getcellvol_lag, quad loop (Limiterlag.f90:1999-2162): current volume
- * geoel(3) and volume centre geoel(5:6) by the 6x6 Gauss rule. */
+/* getcellvol_lag (Limiterlag.f90:1869-2164): current volume geoel(3) and
+ * volume centre geoel(5:6); triangles by the 7-point rule on the quadratic
+ * triangle map with chord-midpoint mid-nodes (:1898-2036), quads by the 6x6
+ * Gauss rule (:2037-2162). */
 void orc_cellvol(orc_case *c, const double *x)
 {
+    const double c10 = 1.0;
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        double xp[2][6];
+        for (int k = 0; k < 3; ++k) {
+            xp[0][k] = X(x, 0, ip[k]);
+            xp[1][k] = X(x, 1, ip[k]);
+        }
+        for (int d = 0; d < 2; ++d) {
+            xp[d][3] = 0.5 * (xp[d][0] + xp[d][1]);
+            xp[d][4] = 0.5 * (xp[d][1] + xp[d][2]);
+            xp[d][5] = 0.5 * (xp[d][0] + xp[d][2]);
+        }
+        double rcv = 0.0, scv = 0.0, volel = 0.0, areel = 0.0, rcvrz = 0.0, scvrz = 0.0;
+        for (int g = 0; g < 7; ++g) {
+            const double r = PT7[g][0], s = PT7[g][1], wi = WT7[g];
+            double sh[6], dr_[6], ds_[6];
+            sh[0] = -(c10 - r - s) * (c10 - 2.0 * (c10 - r - s));
+            sh[1] = -r * (c10 - 2.0 * r);
+            sh[2] = -s * (c10 - 2.0 * s);
+            sh[3] = 4.0 * r * (c10 - r - s);
+            sh[4] = 4.0 * r * s;
+            sh[5] = 4.0 * s * (c10 - r - s);
+            dr_[0] = c10 - 4.0 * (c10 - r - s);
+            dr_[1] = -1.0 + 4.0 * r;
+            dr_[2] = 0.0;
+            dr_[3] = 4.0 * (1.0 - 2.0 * r - s);
+            dr_[4] = 4.0 * s;
+            dr_[5] = -4.0 * s;
+            ds_[0] = c10 - 4.0 * (c10 - r - s);
+            ds_[1] = 0.0;
+            ds_[2] = -1.0 + 4.0 * s;
+            ds_[3] = -4.0 * r;
+            ds_[4] = 4.0 * r;
+            ds_[5] = 4.0 * (1.0 - r - 2.0 * s);
+            double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+            for (int k = 0; k < 6; ++k) {
+                dxdr = dxdr + dr_[k] * xp[0][k];
+                dxds = dxds + ds_[k] * xp[0][k];
+                dydr = dydr + dr_[k] * xp[1][k];
+                dyds = dyds + ds_[k] * xp[1][k];
+            }
+            double yg = 0.0;
+            for (int k = 0; k < 6; ++k) yg = yg + sh[k] * xp[1][k];
+            const double rcoef = 1.0 - c->alfrz + c->alfrz * yg;
+            const double djaco = 0.5 * wi * (dxdr * dyds - dydr * dxds);
+            volel = volel + djaco * rcoef;
+            rcv = rcv + r * djaco * rcoef;
+            scv = scv + s * djaco * rcoef;
+            if (c->nrz == 2) {
+                areel = areel + djaco;
+                rcvrz = rcvrz + r * djaco;
+                scvrz = scvrz + s * djaco;
+            }
+        }
+        GE(c, 2, it) = volel;
+        GE(c, 4, it) = rcv / volel;
+        GE(c, 5, it) = scv / volel;
+        if (c->nrz == 2) {
+            GE(c, 4, it) = rcvrz / areel;
+            GE(c, 5, it) = scvrz / areel;
+            GE(c, 10, it) = volel / areel;
+            GE(c, 11, it) = areel;
+            GE(c, 14, it) = rcvrz / areel;
+            GE(c, 15, it) = scvrz / areel;
+        }
+    }
     double pos[72], w[36];
     ruqope2(6, pos, w);
-    for (int ie = 0; ie < c->nquad; ++ie) {
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int ie = c->ntria + iq;
         double xp[2][9];
-        quad_points9(c, x, ie, xp);
+        quad_points9(c, x, iq, xp);
         double rcv = 0.0, scv = 0.0, volel = 0.0, areel = 0.0, rcvrz = 0.0, scvrz = 0.0;
         for (int g = 0; g < 36; ++g) {
             double r = pos[2 * g], s = pos[2 * g + 1], wi = w[g];
@@ -112,6 +197,24 @@ static void fnds_q1(const orc_case *c, const double *x, double *gq)
     }
 }
 
+/* getfnds_lag_mc_hybrid, triangle loop (Getrhslagmc.f90:2163-2188): edges
+ * 1-2, 2-3, 3-1.  gt[(k + 3*edge) + 9*it]. */
+static void fnds_tri(const orc_case *c, const double *x, double *gt)
+{
+    static const int ea[3] = {0, 1, 2}, eb[3] = {1, 2, 0};
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        for (int e = 0; e < 3; ++e) {
+            double anx = X(x, 1, ip[eb[e]]) - X(x, 1, ip[ea[e]]);
+            double any = -(X(x, 0, ip[eb[e]]) - X(x, 0, ip[ea[e]]));
+            double *g = gt + 3 * e + 9 * it;
+            g[0] = anx / sqrt(anx * anx + any * any);
+            g[1] = any / sqrt(anx * anx + any * any);
+            g[2] = sqrt(anx * anx + any * any);
+        }
+    }
+}
+
 /* barthlimit_lag_vtxunknew, quad loop (Getrhslagmctria.f90:2522-2636): per
  * node max/min over surrounding cells of (tau, u, v, p, E, div u).
  * unmax[q + 6*ip]. The divergence row uses dr, ds that the reference only
@@ -123,8 +226,27 @@ static void vtx_minmax(const orc_case *c, double *unmax, double *unmin)
         unmax[i] = -1.e10;
         unmin[i] = 1.e10;
     }
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
+    /* triangle loop (Getrhslagmctria.f90:2421-2521) */
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        double rhom = U(c, c->unold, 0, 0, it);
+        double uctr = U(c, c->unold, 0, 1, it);
+        double vctr = U(c, c->unold, 0, 2, it);
+        double ectr = U(c, c->unold, 0, 3, it);
+        double rhoct = 1.0 / rhom;
+        double pctr = dmax(EPS_SP6, (c->gamlg - 1.0) * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        double unctr[5] = {U(c, c->unold, 0, 0, it), uctr, vctr, pctr, ectr};
+        for (int iv = 0; iv < 3; ++iv) {
+            int p = ip[iv];
+            for (int q = 0; q < 5; ++q) {
+                if (unctr[q] > unmax[q + 6 * p]) unmax[q + 6 * p] = unctr[q];
+                if (unctr[q] < unmin[q + 6 * p]) unmin[q + 6 * p] = unctr[q];
+            }
+        }
+    }
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
         double rhom = U(c, c->unold, 0, 0, ie);
         double uctr = U(c, c->unold, 0, 1, ie);
         double vctr = U(c, c->unold, 0, 2, ie);
@@ -167,8 +289,9 @@ static void sympre_quad(orc_case *c, const int *indbd, double *afvec)
 {
     const int nq = c->nq;
     double *mapmt = (double *)calloc((size_t)4 * c->nquad, sizeof(double));
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
         double xp[2][4];
         for (int k = 0; k < 4; ++k) {
             xp[0][k] = X(c->cooro, 0, ip[k]);
@@ -202,7 +325,7 @@ static void sympre_quad(orc_case *c, const int *indbd, double *afvec)
         double mapd = matra * matrd - matrb * matrc;
         double lamda1 = 0.5 * mapt + sqrt(0.25 * (mapt * mapt) - mapd);
         double lamda2 = 0.5 * mapt - sqrt(0.25 * (mapt * mapt) - mapd);
-        double *m = mapmt + 4 * ie; /* m[0]=(1,1) m[1]=(1,2) m[2]=(2,1) m[3]=(2,2) */
+        double *m = mapmt + 4 * iq; /* m[0]=(1,1) m[1]=(1,2) m[2]=(2,1) m[3]=(2,2) */
         if (fabs(matrc) > 1.e-8) {
             double l1 = sqrt((lamda1 - matrd) * (lamda1 - matrd) + matrc * matrc);
             double l2 = sqrt((lamda2 - matrd) * (lamda2 - matrd) + matrc * matrc);
@@ -217,9 +340,10 @@ static void sympre_quad(orc_case *c, const int *indbd, double *afvec)
             m[0] = m[1] = m[2] = m[3] = 0.0;
         }
     }
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
-        const double *m = mapmt + 4 * ie;
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
+        const double *m = mapmt + 4 * iq;
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double alfa[2][4];
         double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie);
@@ -282,6 +406,172 @@ static void sympre_quad(orc_case *c, const int *indbd, double *afvec)
     free(mapmt);
 }
 
+/* Node colouring of the symmetry-preserving velocity limiters
+ * (barthlimit_sympre_quadlb, Limiterlag.f90:8212-8250; the triangle one,
+ * barthlimit_sympre_trialb, Getrhslagmctria.f90:2711-2746, colours the same
+ * way): 3 on a v=0 wall, 2 on a u=0 wall, 10 where they meet. */
+static void sym_colour(const orc_case *c, int *indbd)
+{
+    if (c->ncase == 1) return;
+    for (int f = 0; f < c->nbfac; ++f) {
+        const int *b = c->bface + 5 * f;
+        if (b[2] == 22 && b[3] == 221) {
+            indbd[b[0]] = 3; indbd[b[1]] = 3;
+            if (c->ncurv == 1) indbd[b[4]] = 3;
+        }
+    }
+    for (int f = 0; f < c->nbfac; ++f) {
+        const int *b = c->bface + 5 * f;
+        if (b[2] == 22 && b[3] == 222) {
+            int nn = c->ncurv == 1 ? 3 : 2;
+            for (int k = 0; k < nn; ++k) {
+                int p = (k < 2) ? b[k] : b[4];
+                indbd[p] = (indbd[p] == 3) ? 10 : 2;
+            }
+        }
+    }
+}
+
+/* barthlimit_sympre_trialb (Getrhslagmctria.f90:2644-3258): the triangle
+ * velocity limiter works in the frame of the cell-average velocity
+ * (:2810-2822), with the same neighbour max/min and symmetry mirrors as the
+ * quad limiter; afvec = R^T diag(phi_xi, phi_eta) R (:3236-3255).  Its
+ * divergence and Mach factors (:3164-3233) feed nothing and are omitted. */
+static void sympre_tri(orc_case *c, const int *indbd, double *afvec)
+{
+    const double dr = 0.5, ds = 0.5;
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        const double ma = U(c, c->unold, 0, 1, it), mb = U(c, c->unold, 0, 2, it);
+        double m[4];
+        if (sqrt(ma * ma + mb * mb) > 1.e-10) {
+            m[0] = ma / sqrt(ma * ma + mb * mb);
+            m[1] = mb / sqrt(ma * ma + mb * mb);
+            m[2] = -mb / sqrt(ma * ma + mb * mb);
+            m[3] = ma / sqrt(ma * ma + mb * mb);
+        } else {
+            m[0] = m[1] = m[2] = m[3] = 0.0;
+        }
+        const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+        const double uctr = U(c, c->unold, 0, 1, it), vctr = U(c, c->unold, 0, 2, it);
+        const double uc2 = uctr * m[0] + vctr * m[1];
+        const double uc3 = uctr * m[2] + vctr * m[3];
+        double fixi = 1.e300, fiet = 1.e300;
+        for (int iv = 0; iv < 3; ++iv) {
+            double bt[3] = {1.0, (XVT[iv] - rc) / dr, (YVT[iv] - sc) / ds};
+            double uv = 0.0, vv = 0.0;
+            for (int id = 0; id < 3; ++id) {
+                uv = uv + U(c, c->unold, id, 1, it) * bt[id];
+                vv = vv + U(c, c->unold, id, 2, it) * bt[id];
+            }
+            const double un2 = uv * m[0] + vv * m[1];
+            const double un3 = uv * m[2] + vv * m[3];
+            double mx2 = uc2, mx3 = uc3, mn2 = uc2, mn3 = uc3;
+            const int p = ip[iv];
+            for (int k = c->esuv2[p]; k < c->esuv2[p + 1]; ++k) {
+                const int je = c->esuv1[k];
+                const double ucj = U(c, c->unold, 0, 1, je), vcj = U(c, c->unold, 0, 2, je);
+#define ACC(UU, VV)                                          \
+    do {                                                     \
+        double ul = (UU) * m[0] + (VV) * m[1];               \
+        double vl = (UU) * m[2] + (VV) * m[3];               \
+        mx2 = dmax(mx2, ul); mx3 = dmax(mx3, vl);            \
+        mn2 = dmin(mn2, ul); mn3 = dmin(mn3, vl);            \
+    } while (0)
+                ACC(ucj, vcj);
+                if (c->ncase != 1) {
+                    if (indbd[p] == 3) {
+                        ACC(ucj, -vcj);
+                    } else if (indbd[p] == 2) {
+                        ACC(-ucj, vcj);
+                    } else if (indbd[p] == 10) {
+                        ACC(ucj, -vcj);
+                        ACC(-ucj, vcj);
+                        ACC(-ucj, -vcj);
+                    }
+                }
+#undef ACC
+            }
+            fixi = dmin(fixi, barthfct(mx2, mn2, uc2, un2 - uc2));
+            fiet = dmin(fiet, barthfct(mx3, mn3, uc3, un3 - uc3));
+        }
+        const double xicx = m[0], xicy = m[1], etcx = m[2], etcy = m[3];
+        double *a = afvec + 4 * it; /* a[0]=(1,1) a[1]=(2,1) a[2]=(1,2) a[3]=(2,2) */
+        a[0] = xicx * xicx * fixi + etcx * etcx * fiet;
+        a[2] = xicx * xicy * fixi + etcx * etcy * fiet;
+        a[1] = a[2];
+        a[3] = xicy * xicy * fixi + etcy * etcy * fiet;
+    }
+}
+
+/* barthlimit_lagsym_tria (Getrhslagmctria.f90:422-842) for ndens=1, P1:
+ * aflim(1:4) from the vertex values against the node max/min, the velocity
+ * limiter, then the total-energy coefficient aflim(5).  The boundary-node
+ * overrides of ncase 1, 2, 3 and -7 (:539-582) are not restated; those
+ * problems are rejected on hybrid meshes. */
+static void barth_tri(orc_case *c, const double *unmax, const double *unmin, double *aflim, double *afvec)
+{
+    const double g1 = c->gamlg - 1.0, dr = 0.5, ds = 0.5;
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+        double unk[4][3];
+        for (int iv = 0; iv < 3; ++iv) {
+            double bt[3] = {1.0, (XVT[iv] - rc) / dr, (YVT[iv] - sc) / ds};
+            for (int q = 0; q < 4; ++q) unk[q][iv] = 0.0;
+            for (int id = 0; id < 3; ++id)
+                for (int q = 0; q < 4; ++q) unk[q][iv] = unk[q][iv] + U(c, c->unold, id, q, it) * bt[id];
+            double rhov = 1.0 / unk[0][iv];
+            double uv = unk[1][iv], vv = unk[2][iv], ev = unk[3][iv];
+            double pv = dmax(EPS_SP6, g1 * rhov * (ev - 0.5 * (uv * uv + vv * vv)));
+            unk[0][iv] = 1.0 / rhov;
+            unk[3][iv] = pv;
+        }
+        double rhom = U(c, c->unold, 0, 0, it);
+        double uctr = U(c, c->unold, 0, 1, it), vctr = U(c, c->unold, 0, 2, it);
+        double ectr = U(c, c->unold, 0, 3, it);
+        double rhoct = 1.0 / rhom;
+        double pctr = dmax(EPS_SP6, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        double unctr[4] = {1.0 / rhoct, uctr, vctr, pctr};
+        for (int q = 0; q < 4; ++q) {
+            double a = 1.e300;
+            for (int iv = 0; iv < 3; ++iv) {
+                int p = ip[iv];
+                a = dmin(a, barthfct(unmax[q + 6 * p], unmin[q + 6 * p], unctr[q], unk[q][iv] - unctr[q]));
+            }
+            aflim[q + 5 * it] = a;
+        }
+    }
+    int *indbd = (int *)calloc((size_t)c->npoin, sizeof(int));
+    sym_colour(c, indbd);
+    sympre_tri(c, indbd, afvec);
+    free(indbd);
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+        double rhom = U(c, c->unold, 0, 0, it);
+        double uctr = U(c, c->unold, 0, 1, it), vctr = U(c, c->unold, 0, 2, it);
+        double ectr = U(c, c->unold, 0, 3, it);
+        double rhoct = 1.0 / rhom;
+        double pctr = dmax(EPS_SP6, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        double a5 = 1.e300;
+        for (int iv = 0; iv < 3; ++iv) {
+            double bt[3] = {1.0, (XVT[iv] - rc) / dr, (YVT[iv] - sc) / ds};
+            double u[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int id = 0; id < 3; ++id)
+                for (int q = 0; q < 4; ++q) u[q] = u[q] + U(c, c->unold, id, q, it) * bt[id];
+            double rhov = 1.0 / u[0];
+            double pv = dmax(EPS_SP6, g1 * rhov * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2])));
+            double t = rhom + aflim[0 + 5 * it] * (1.0 / rhov - rhom);
+            double pl = pctr + aflim[3 + 5 * it] * (pv - pctr);
+            double e5 = pl / g1 * t + 0.5 * (u[1] * u[1] + u[2] * u[2]);
+            int p = ip[iv];
+            a5 = dmin(a5, barthfct(unmax[4 + 6 * p], unmin[4 + 6 * p], ectr, e5 - ectr));
+        }
+        aflim[4 + 5 * it] = a5;
+    }
+}
+
 /* barthlimit_lagsym_quad (Limiterlag.f90:4-404) for ndens=1, P1.
  * aflim[q + 5*ie] rows tau,u,v,p,E; afvec column-major 2x2 per cell. */
 static void barth_quad(orc_case *c, const double *unmax, const double *unmin,
@@ -289,28 +579,10 @@ static void barth_quad(orc_case *c, const double *unmax, const double *unmin,
 {
     const double g1 = c->gamlg - 1.0;
     int *indbd = (int *)calloc((size_t)c->npoin, sizeof(int));
-    /* node colouring of barthlimit_sympre_quadlb (Limiterlag.f90:8212-8250) */
-    if (c->ncase != 1) {
-        for (int f = 0; f < c->nbfac; ++f) {
-            const int *b = c->bface + 5 * f;
-            if (b[2] == 22 && b[3] == 221) {
-                indbd[b[0]] = 3; indbd[b[1]] = 3;
-                if (c->ncurv == 1) indbd[b[4]] = 3;
-            }
-        }
-        for (int f = 0; f < c->nbfac; ++f) {
-            const int *b = c->bface + 5 * f;
-            if (b[2] == 22 && b[3] == 222) {
-                int nn = c->ncurv == 1 ? 3 : 2;
-                for (int k = 0; k < nn; ++k) {
-                    int p = (k < 2) ? b[k] : b[4];
-                    indbd[p] = (indbd[p] == 3) ? 10 : 2;
-                }
-            }
-        }
-    }
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
+    sym_colour(c, indbd);
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double unk[4][4];
         for (int iv = 0; iv < 4; ++iv) {
@@ -342,8 +614,9 @@ static void barth_quad(orc_case *c, const double *unmax, const double *unmin,
     }
     sympre_quad(c, indbd, afvec);
     /* Part 3: total-energy coefficient (Limiterlag.f90:253-381) */
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double rhom = U(c, c->unold, 0, 0, ie);
         double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie);
@@ -373,8 +646,8 @@ static void barth_quad(orc_case *c, const double *unmax, const double *unmin,
 /* getriem_quad_matrixsym (Limiterlag.f90:708-1033) + getriecoef_matrixnew
  * (:1037-1076) and the nodal solve of getndvelo_lag_mc_matrixsym
  * (:408-704), straight P1 quads. fstarq[(d + 2*(f + 2*iv)) + 16*ie]. */
-static void ndvelo_q1(orc_case *c, const double *gq, const double *aflim, const double *afvec,
-                      double *fstarq)
+static void ndvelo_q1(orc_case *c, const double *gt, const double *gq, const double *aflim, const double *afvec,
+                      double *fstart, double *fstarq)
 {
     const int np = c->npoin;
     const double g1 = c->gamlg - 1.0;
@@ -382,21 +655,123 @@ static void ndvelo_q1(orc_case *c, const double *gq, const double *aflim, const 
     double *mn = (double *)calloc((size_t)4 * np, sizeof(double)); /* munacn(2,2,np) col-major */
     double *mu = (double *)calloc((size_t)2 * np, sizeof(double));
     double *sn = (double *)calloc((size_t)2 * np, sizeof(double));
-    double *mcl = (double *)malloc(sizeof(double) * 32 * c->nquad); /* munaclq(2,2,2,4) */
-    double *mul = (double *)malloc(sizeof(double) * 16 * c->nquad);
-    double *snl = (double *)malloc(sizeof(double) * 16 * c->nquad);
+    double *mcl = (double *)malloc(sizeof(double) * 32 * (c->nquad + 1)); /* munaclq(2,2,2,4) */
+    double *mul = (double *)malloc(sizeof(double) * 16 * (c->nquad + 1));
+    double *snl = (double *)malloc(sizeof(double) * 16 * (c->nquad + 1));
+    double *mct = (double *)malloc(sizeof(double) * 24 * (c->ntria + 1)); /* munaclt(2,2,2,3) */
+    double *mut = (double *)malloc(sizeof(double) * 12 * (c->ntria + 1));
+    double *snt = (double *)malloc(sizeof(double) * 12 * (c->ntria + 1));
+    /* triangle corners first (getriem_tria_matrixsym, Getrhslagmctria.f90:847-1204,
+     * called before the quad routine at Limiterlag.f90:535-536) */
+    {
+        static const int tfe[3][2] = {{2, 0}, {0, 1}, {1, 2}}; /* (edge before, edge after) */
+        static const int tnb[3][2] = {{2, 1}, {0, 2}, {1, 0}}; /* the other end of that edge */
+        const double dr = 0.5, ds = 0.5;
+        for (int it = 0; it < c->ntria; ++it) {
+            const int *ip = c->iptri + 3 * it;
+            const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+            double bt[3][3];
+            for (int iv = 0; iv < 3; ++iv) {
+                bt[iv][0] = 1.0;
+                bt[iv][1] = (XVT[iv] - rc) / dr;
+                bt[iv][2] = (YVT[iv] - sc) / ds;
+            }
+            double rco[3];
+            for (int k = 0; k < 3; ++k) rco[k] = 1.0 - c->alfrz + c->alfrz * X(c->cooro, 1, ip[k]);
+            double vn[3][2][3];
+            for (int iv = 0; iv < 3; ++iv)
+                for (int f = 0; f < 2; ++f) {
+                    const double *g = gt + 3 * tfe[iv][f] + 9 * it;
+                    vn[iv][f][0] = g[0];
+                    vn[iv][f][1] = g[1];
+                    vn[iv][f][2] = (2.0 * rco[iv] + rco[tnb[iv][f]]) / 3.0 * 0.5 * g[2];
+                }
+            const double rhomc = U(c, c->unold, 0, 0, it);
+            const double uctr = U(c, c->unold, 0, 1, it), vctr = U(c, c->unold, 0, 2, it);
+            const double ectr = U(c, c->unold, 0, 3, it);
+            const double rhoct = 1.0 / rhomc;
+            const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+            const double *af = afvec + 4 * it;
+            const double dudr = af[0] * U(c, c->unold, 1, 1, it) + af[2] * U(c, c->unold, 1, 2, it);
+            const double duds = af[0] * U(c, c->unold, 2, 1, it) + af[2] * U(c, c->unold, 2, 2, it);
+            const double dvdr = af[1] * U(c, c->unold, 1, 1, it) + af[3] * U(c, c->unold, 1, 2, it);
+            const double dvds = af[1] * U(c, c->unold, 2, 1, it) + af[3] * U(c, c->unold, 2, 2, it);
+            double uvx[3], vvx[3], pvx[3], aj[3][3];
+            for (int iv = 0; iv < 3; ++iv) {
+                double u[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int id = 0; id < 3; ++id)
+                    for (int q = 0; q < 4; ++q) u[q] = u[q] + U(c, c->unold, id, q, it) * bt[iv][id];
+                const double rhovt = 1.0 / u[0];
+                const double pv = dmax(eps, g1 * rhovt * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2])));
+                uvx[iv] = U(c, c->unold, 0, 1, it) + dudr * bt[iv][1] + duds * bt[iv][2];
+                vvx[iv] = U(c, c->unold, 0, 2, it) + dvdr * bt[iv][1] + dvds * bt[iv][2];
+                pvx[iv] = pctr + aflim[3 + 5 * it] * (pv - pctr);
+                const double acnx = X(c->ustar, 0, ip[iv]) - uvx[iv];
+                const double acny = X(c->ustar, 1, ip[iv]) - vvx[iv];
+                if (sqrt(acnx * acnx + acny * acny) < (double)1.e-11f) {
+                    aj[iv][0] = (double)1.e-11f;
+                    aj[iv][1] = (double)1.e-11f;
+                } else {
+                    aj[iv][0] = acnx / sqrt(acnx * acnx + acny * acny);
+                    aj[iv][1] = acny / sqrt(acnx * acnx + acny * acny);
+                }
+                aj[iv][2] = sqrt(acnx * acnx + acny * acny);
+            }
+            const double sdctr = sqrt(dmax(eps, c->gamlg * pctr / rhoct));
+            for (int iv = 0; iv < 3; ++iv) aj[iv][2] = aj[iv][2] / sdctr;
+            double murie[3];
+            for (int iv = 0; iv < 3; ++iv) {
+                const double dux = X(c->ustar, 0, ip[iv]) - uvx[iv];
+                const double duy = X(c->ustar, 1, ip[iv]) - vvx[iv];
+                const double deltu = sqrt(dux * dux + duy * duy);
+                murie[iv] = rhoct * sdctr + c->cimpd * rhoct * c->slpdu * deltu;
+            }
+            for (int iv = 0; iv < 3; ++iv) {
+                const int p = ip[iv];
+                for (int f = 0; f < 2; ++f) {
+                    const double area = vn[iv][f][2], nx = vn[iv][f][0], ny = vn[iv][f][1];
+                    const double vnau = (aj[iv][2] < 1.e12) ? 1.0 : nx * aj[iv][0] + ny * aj[iv][1];
+                    const double m11 = murie[iv] * area * fabs(vnau), m21 = 0.0, m12 = 0.0;
+                    const double m22 = murie[iv] * area * fabs(vnau);
+                    const double mu1 = m11 * uvx[iv] + m21 * vvx[iv];
+                    const double mu2 = m21 * uvx[iv] + m22 * vvx[iv];
+                    const double sxx = -pvx[iv], sxy = 0.0, syx = 0.0, syy = -pvx[iv];
+                    const double sn1 = sxx * area * nx + sxy * area * ny;
+                    const double sn2 = syx * area * nx + syy * area * ny;
+                    mn[0 + 4 * p] = mn[0 + 4 * p] + m11;
+                    mn[1 + 4 * p] = mn[1 + 4 * p] + m21;
+                    mn[2 + 4 * p] = mn[2 + 4 * p] + m12;
+                    mn[3 + 4 * p] = mn[3 + 4 * p] + m22;
+                    mu[0 + 2 * p] = mu[0 + 2 * p] + mu1;
+                    mu[1 + 2 * p] = mu[1 + 2 * p] + mu2;
+                    sn[0 + 2 * p] = sn[0 + 2 * p] + sn1;
+                    sn[1 + 2 * p] = sn[1 + 2 * p] + sn2;
+                    const int o = f + 2 * iv;
+                    mct[0 + 4 * o + 24 * it] = m11;
+                    mct[1 + 4 * o + 24 * it] = m21;
+                    mct[2 + 4 * o + 24 * it] = m12;
+                    mct[3 + 4 * o + 24 * it] = m22;
+                    mut[0 + 2 * o + 12 * it] = mu1;
+                    mut[1 + 2 * o + 12 * it] = mu2;
+                    snt[0 + 2 * o + 12 * it] = sn1;
+                    snt[1 + 2 * o + 12 * it] = sn2;
+                }
+            }
+        }
+    }
     /* corner faces: (edge before, edge after) of each vertex */
     static const int fe[4][2] = {{3, 0}, {0, 1}, {1, 2}, {2, 3}};
     static const int nb[4][2] = {{3, 1}, {0, 2}, {1, 3}, {2, 0}};
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double rco[4];
         for (int k = 0; k < 4; ++k) rco[k] = 1.0 - c->alfrz + c->alfrz * X(c->cooro, 1, ip[k]);
         double vn[4][2][3];
         for (int iv = 0; iv < 4; ++iv)
             for (int f = 0; f < 2; ++f) {
-                const double *g = gq + 3 * fe[iv][f] + 12 * ie;
+                const double *g = gq + 3 * fe[iv][f] + 12 * iq;
                 vn[iv][f][0] = g[0];
                 vn[iv][f][1] = g[1];
                 vn[iv][f][2] = (2.0 * rco[iv] + rco[nb[iv][f]]) / 3.0 * 0.5 * g[2];
@@ -462,14 +837,14 @@ static void ndvelo_q1(orc_case *c, const double *gq, const double *aflim, const 
                 sn[0 + 2 * p] = sn[0 + 2 * p] + sn1;
                 sn[1 + 2 * p] = sn[1 + 2 * p] + sn2;
                 int o = f + 2 * iv;
-                mcl[0 + 4 * o + 32 * ie] = m11;
-                mcl[1 + 4 * o + 32 * ie] = m21;
-                mcl[2 + 4 * o + 32 * ie] = m12;
-                mcl[3 + 4 * o + 32 * ie] = m22;
-                mul[0 + 2 * o + 16 * ie] = mu1;
-                mul[1 + 2 * o + 16 * ie] = mu2;
-                snl[0 + 2 * o + 16 * ie] = sn1;
-                snl[1 + 2 * o + 16 * ie] = sn2;
+                mcl[0 + 4 * o + 32 * iq] = m11;
+                mcl[1 + 4 * o + 32 * iq] = m21;
+                mcl[2 + 4 * o + 32 * iq] = m12;
+                mcl[3 + 4 * o + 32 * iq] = m22;
+                mul[0 + 2 * o + 16 * iq] = mu1;
+                mul[1 + 2 * o + 16 * iq] = mu2;
+                snl[0 + 2 * o + 16 * iq] = sn1;
+                snl[1 + 2 * o + 16 * iq] = sn2;
             }
         }
     }
@@ -503,7 +878,18 @@ static void ndvelo_q1(orc_case *c, const double *gq, const double *aflim, const 
             X(c->ustar, 0, b[0]) = 0.0; X(c->ustar, 0, b[1]) = 0.0;
         }
     }
-    /* corner forces (Limiterlag.f90:680-698) */
+    /* corner forces (Limiterlag.f90:663-698), triangles then quads */
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        for (int iv = 0; iv < 3; ++iv)
+            for (int f = 0; f < 2; ++f) {
+                int o = f + 2 * iv;
+                const double *m = mct + 4 * o + 24 * it;
+                double us1 = X(c->ustar, 0, ip[iv]), us2 = X(c->ustar, 1, ip[iv]);
+                fstart[0 + 2 * o + 12 * it] = snt[0 + 2 * o + 12 * it] + m[0] * us1 + m[1] * us2 - mut[0 + 2 * o + 12 * it];
+                fstart[1 + 2 * o + 12 * it] = snt[1 + 2 * o + 12 * it] + m[2] * us1 + m[3] * us2 - mut[1 + 2 * o + 12 * it];
+            }
+    }
     for (int ie = 0; ie < c->nquad; ++ie) {
         const int *ip = c->ipqua + c->nvqua * ie;
         for (int iv = 0; iv < 4; ++iv)
@@ -516,7 +902,7 @@ static void ndvelo_q1(orc_case *c, const double *gq, const double *aflim, const 
             }
     }
     free(indnd);
-    free(mn); free(mu); free(sn); free(mcl); free(mul); free(snl);
+    free(mn); free(mu); free(sn); free(mcl); free(mul); free(snl); free(mct); free(mut); free(snt);
 }
 
 /* rhsifacedg_lag_mc_hybridquad2 (Getrhslagmc.f90:14468-14618): lumped P1
@@ -526,8 +912,9 @@ static void iface_q1(orc_case *c, const double *gq, const double *fstarq)
     const double c13 = 0.3333333333333333, c16 = 0.1666666666666666;
     static const int fe[4][2] = {{3, 0}, {0, 1}, {1, 2}, {2, 3}};
     const int nd = c->ndegr;
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double bq[4][3];
         for (int iv = 0; iv < 4; ++iv) {
@@ -538,9 +925,9 @@ static void iface_q1(orc_case *c, const double *gq, const double *fstarq)
         double ul[3] = {0, 0, 0}, pl1[3] = {0, 0, 0}, pl2[3] = {0, 0, 0}, el[3] = {0, 0, 0};
         for (int iv = 0; iv < 4; ++iv) {
             double us1 = X(c->ustar, 0, ip[iv]), us2 = X(c->ustar, 1, ip[iv]);
-            const double *g0 = gq + 3 * fe[iv][0] + 12 * ie;
-            const double *g1 = gq + 3 * fe[iv][1] + 12 * ie;
-            const double *f = fstarq + 16 * ie + 4 * iv; /* f[d + 2*face] */
+            const double *g0 = gq + 3 * fe[iv][0] + 12 * iq;
+            const double *g1 = gq + 3 * fe[iv][1] + 12 * iq;
+            const double *f = fstarq + 16 * iq + 4 * iv; /* f[d + 2*face] */
             for (int id = 0; id < nd; ++id) {
                 double w = 2.0 * bq[iv][id] + bq[iv][id];
                 double l11 = c16 * w * g0[0] * g0[2], l21 = c16 * w * g0[1] * g0[2];
@@ -561,6 +948,119 @@ static void iface_q1(orc_case *c, const double *gq, const double *fstarq)
     }
 }
 
+/* rhsifacedg_lag_mc_hybridtria (Getrhslagmc.f90:14006-14160): P1 triangle
+ * face integral with consistent corner weights (2 b_i + b_j)/6 (u* n) and
+ * (2 b_i + b_j)/3 (forces); overwrites rhsel. */
+static void iface_tri(orc_case *c, const double *gt, const double *fstart)
+{
+    const double c13 = 0.3333333333333333, c16 = 0.1666666666666666;
+    static const int tfe[3][2] = {{2, 0}, {0, 1}, {1, 2}};
+    static const int tnb[3][2] = {{2, 1}, {0, 2}, {1, 0}};
+    const double dr = 0.5, ds = 0.5;
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+        double bt[3][3];
+        for (int iv = 0; iv < 3; ++iv) {
+            bt[iv][0] = 1.0;
+            bt[iv][1] = (XVT[iv] - rc) / dr;
+            bt[iv][2] = (YVT[iv] - sc) / ds;
+        }
+        double ul[3] = {0, 0, 0}, pl1[3] = {0, 0, 0}, pl2[3] = {0, 0, 0}, el[3] = {0, 0, 0};
+        for (int iv = 0; iv < 3; ++iv) {
+            const double us1 = X(c->ustar, 0, ip[iv]), us2 = X(c->ustar, 1, ip[iv]);
+            const double *g0 = gt + 3 * tfe[iv][0] + 9 * it;
+            const double *g1 = gt + 3 * tfe[iv][1] + 9 * it;
+            const double *f = fstart + 12 * it + 4 * iv; /* f[d + 2*face] */
+            for (int id = 0; id < 3; ++id) {
+                const double w0 = 2.0 * bt[iv][id] + bt[tnb[iv][0]][id];
+                const double w1 = 2.0 * bt[iv][id] + bt[tnb[iv][1]][id];
+                const double l10 = c16 * w0 * g0[0] * g0[2], l20 = c16 * w0 * g0[1] * g0[2];
+                const double l11 = c16 * w1 * g1[0] * g1[2], l21 = c16 * w1 * g1[1] * g1[2];
+                ul[id] = ul[id] + us1 * l10 + us2 * l20 + us1 * l11 + us2 * l21;
+                pl1[id] = pl1[id] + f[0] * c13 * w0 + f[2] * c13 * w1;
+                pl2[id] = pl2[id] + f[1] * c13 * w0 + f[3] * c13 * w1;
+                el[id] = el[id] + us1 * f[0] * c13 * w0 + us2 * f[1] * c13 * w0 + us1 * f[2] * c13 * w1 +
+                         us2 * f[3] * c13 * w1;
+            }
+        }
+        for (int id = 0; id < 3; ++id) {
+            U(c, c->rhsel, id, 0, it) = ul[id];
+            U(c, c->rhsel, id, 1, it) = pl1[id];
+            U(c, c->rhsel, id, 2, it) = pl2[id];
+            U(c, c->rhsel, id, 3, it) = el[id];
+        }
+    }
+}
+
+/* rhsdomndg_lag_mc_triasym (Getrhslagmctria.f90:1208-1510): -int grad(b).F
+ * on the 7-point rule (ngausd = 7 in the control file) with the cofactor
+ * form, djaco = w/2. */
+static void domn_tri(orc_case *c, const double *aflim, const double *afvec)
+{
+    const double g1 = c->gamlg - 1.0, eps = 1.0e-6, dr = 0.5, ds = 0.5;
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = c->iptri + 3 * it;
+        double xp[2][3];
+        for (int k = 0; k < 3; ++k) {
+            xp[0][k] = X(c->cooro, 0, ip[k]);
+            xp[1][k] = X(c->cooro, 1, ip[k]);
+        }
+        const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+        const double dbdr[3] = {0.0, 1.0 / dr, 0.0}, dbds[3] = {0.0, 0.0, 1.0 / ds};
+        const double rhomc = U(c, c->unold, 0, 0, it);
+        const double uctr = U(c, c->unold, 0, 1, it), vctr = U(c, c->unold, 0, 2, it);
+        const double ectr = U(c, c->unold, 0, 3, it);
+        const double rhoct = 1.0 / rhomc;
+        const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        const double *af = afvec + 4 * it;
+        const double dudr = af[0] * U(c, c->unold, 1, 1, it) + af[2] * U(c, c->unold, 1, 2, it);
+        const double duds = af[0] * U(c, c->unold, 2, 1, it) + af[2] * U(c, c->unold, 2, 2, it);
+        const double dvdr = af[1] * U(c, c->unold, 1, 1, it) + af[3] * U(c, c->unold, 1, 2, it);
+        const double dvds = af[1] * U(c, c->unold, 2, 1, it) + af[3] * U(c, c->unold, 2, 2, it);
+        const double dspr[3] = {-1.0, 1.0, 0.0}, dsps[3] = {-1.0, 0.0, 1.0};
+        double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            dxdr = dxdr + dspr[k] * xp[0][k];
+            dxds = dxds + dsps[k] * xp[0][k];
+            dydr = dydr + dspr[k] * xp[1][k];
+            dyds = dyds + dsps[k] * xp[1][k];
+        }
+        double gd[2][3];
+        for (int id = 0; id < 3; ++id) {
+            gd[0][id] = dyds * dbdr[id] + (-dydr) * dbds[id];
+            gd[1][id] = (-dxds) * dbdr[id] + dxdr * dbds[id];
+        }
+        for (int g = 0; g < 7; ++g) {
+            const double r = PT7[g][0], s = PT7[g][1], wi = WT7[g];
+            const double sh[3] = {1.0 - r - s, r, s};
+            const double djaco = 0.5 * wi;
+            double yg = 0.0;
+            for (int k = 0; k < 3; ++k) yg = yg + sh[k] * xp[1][k];
+            const double rcoef = 1.0 - c->alfrz + c->alfrz * yg;
+            const double b[3] = {1.0, (r - rc) / dr, (s - sc) / ds};
+            double un[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int id = 0; id < 3; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + U(c, c->unold, id, q, it) * b[id];
+            const double rhoad = 1.0 / un[0];
+            double pres = g1 * rhoad * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]));
+            const double uadv = U(c, c->unold, 0, 1, it) + dudr * b[1] + duds * b[2];
+            const double vadv = U(c, c->unold, 0, 2, it) + dvdr * b[1] + dvds * b[2];
+            pres = pctr + aflim[3 + 5 * it] * (pres - pctr);
+            double fl[3][4];
+            for (int id = 0; id < 3; ++id) {
+                fl[id][0] = gd[0][id] * uadv + gd[1][id] * vadv;
+                fl[id][1] = gd[0][id] * (-pres);
+                fl[id][2] = gd[1][id] * (-pres);
+                fl[id][3] = (gd[0][id] * uadv + gd[1][id] * vadv) * (-pres);
+            }
+            for (int id = 0; id < 3; ++id)
+                for (int q = 0; q < 4; ++q)
+                    U(c, c->rhsel, id, q, it) = U(c, c->rhsel, id, q, it) - fl[id][q] * djaco * rcoef;
+        }
+    }
+}
+
 /* rhsdomndg_lag_mc_quadsym (Limiterlag.f90:1108-1490): -int grad(b).F on the
  * 5x5 Gauss rule with the cofactor form (djaco = w); P1 terms only.  The
  * artificial stress of getAS_llnl is identically zero (Limiterlag.f90:15701). */
@@ -570,8 +1070,9 @@ static void domn_q1(orc_case *c, const double *aflim, const double *afvec)
     ruqope2(5, pos, w);
     const double g1 = c->gamlg - 1.0;
     const double eps = 1.0e-6;
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
         double xp[2][4];
         for (int k = 0; k < 4; ++k) {
             xp[0][k] = X(c->cooro, 0, ip[k]);
@@ -642,15 +1143,22 @@ static void domn_q1(orc_case *c, const double *aflim, const double *afvec)
 static int rhs_q1(orc_case *c)
 {
     if (c->npoly != 1 || c->nrz != 0 || c->ndens != 1 || c->nlimi != 6) return 2;
-    double *gq = (double *)malloc(sizeof(double) * 12 * c->nquad);
+    if (c->ntria > 0 && (c->ncase == 1 || c->ncase == 2 || c->ncase == 3 || c->ncase == -7)) return 2;
+    double *gq = (double *)malloc(sizeof(double) * 12 * (c->nquad + 1));
+    double *gt = (double *)malloc(sizeof(double) * 9 * (c->ntria + 1));
     double *unmax = (double *)malloc(sizeof(double) * 6 * c->npoin);
     double *unmin = (double *)malloc(sizeof(double) * 6 * c->npoin);
-    double *fstarq = (double *)malloc(sizeof(double) * 16 * c->nquad);
+    double *fstarq = (double *)malloc(sizeof(double) * 16 * (c->nquad + 1));
+    double *fstart = (double *)malloc(sizeof(double) * 12 * (c->ntria + 1));
+    fnds_tri(c, c->cooro, gt);
     fnds_q1(c, c->cooro, gq);
     vtx_minmax(c, unmax, unmin);
+    barth_tri(c, unmax, unmin, c->aflim, c->afvec);
     barth_quad(c, unmax, unmin, c->aflim, c->afvec);
-    ndvelo_q1(c, gq, c->aflim, c->afvec, fstarq);
+    ndvelo_q1(c, gt, gq, c->aflim, c->afvec, fstart, fstarq);
+    iface_tri(c, gt, fstart);
     iface_q1(c, gq, fstarq);
+    domn_tri(c, c->aflim, c->afvec);
     domn_q1(c, c->aflim, c->afvec);
     for (int ie = 0; ie < c->ncell; ++ie) {
         const double *al = c->aflim + 5 * ie, *af = c->afvec + 4 * ie;
@@ -667,7 +1175,7 @@ static int rhs_q1(orc_case *c)
         U(c, c->unold, 1, 3, ie) = U(c, c->unold, 1, 3, ie) * al[4];
         U(c, c->unold, 2, 3, ie) = U(c, c->unold, 2, 3, ie) * al[4];
     }
-    free(gq); free(unmax); free(unmin); free(fstarq);
+    free(gq); free(gt); free(unmax); free(unmin); free(fstarq); free(fstart);
     return 0;
 }
 
@@ -684,8 +1192,9 @@ static void delt(orc_case *c, const double *dvdt)
 {
     const double eps = 1.0e-6, g1 = c->gamlg - 1.0;
     double dtcfl = 1e10, dtvol = 1e10;
-    for (int ie = 0; ie < c->nquad; ++ie) {
-        const int *ip = c->ipqua + c->nvqua * ie;
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int *ip = c->ipqua + c->nvqua * iq;
+        const int ie = c->ntria + iq;
         double xp[2][9];
         int nv = c->ncurv == 0 ? 4 : 9;
         for (int k = 0; k < nv; ++k) {
@@ -713,7 +1222,10 @@ static void delt(orc_case *c, const double *dvdt)
         for (int k = 0; k < 4; ++k) dlen = dmin(dlen, dl[k]);
         dtcfl = dmin(dtcfl, dlen / sdctr);
     }
-    for (int ie = 0; ie < c->nquad; ++ie) dtvol = dmin(dtvol, GE(c, 2, ie) / fabs(dvdt[ie]));
+    for (int iq = 0; iq < c->nquad; ++iq) {
+        const int ie = c->ntria + iq;
+        dtvol = dmin(dtvol, GE(c, 2, ie) / fabs(dvdt[ie]));
+    }
     if (c->itime == 1)
         c->dtfix = dmin(c->cfl * dtcfl, 0.1 * dtvol);
     else

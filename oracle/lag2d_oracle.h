@@ -10,7 +10,8 @@
  *   geoel[k + 24*ie]                    geoel(24,nsize)        dgflo2d.f90:119
  *   amatr[k + nmatr*ie]                 amatr(nmatr,ncell)     dgflo2d.f90:238-245
  *   coord[d + 2*ip]                     coord(2,npoin)         dgflo2d.f90:118
- *   ipqua[iv + nvqua*ie]   0-based nodes
+ *   iptri[iv + 3*it], ipqua[iv + nvqua*iq]   0-based nodes; cell it is
+ *                          triangle it, cell ntria + iq is quad iq
  *   bface[k + 5*ifa]       nodes (k=0,1,4) 0-based; k=2 BC code, k=3 subcode
  *   esuv2[npoin+1] offsets, esuv1[] 0-based element ids   Preprocess.f90:720-766
  */
@@ -31,7 +32,10 @@ typedef struct orc_case {
     /* mutable driver scalars (module constant) */
     double dtfix, tacru;
     int itime;
-    /* mesh */
+    /* mesh: ntria triangles (cells 0..ntria-1) then the quads (cell ntria+q),
+     * as the reference numbers them (Preprocess.f90:168-198) */
+    int ntria;
+    const int *iptri;               /* iptri[iv + 3*it], 0-based */
     const int *ipqua, *bface, *esuv1, *esuv2;
     /* state: coord = x^n, cooro = stage x, coold = x^0 */
     double *coord, *cooro, *coold;

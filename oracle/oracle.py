@@ -26,6 +26,7 @@ class _Case(C.Structure):
         ("gamlg", C.c_double), ("cfl", C.c_double), ("cimpd", C.c_double), ("cdrho", C.c_double),
         ("tend", C.c_double), ("slpdu", C.c_double), ("alfrz", C.c_double),
         ("dtfix", C.c_double), ("tacru", C.c_double), ("itime", C.c_int),
+        ("ntria", C.c_int), ("iptri", _ip),
         ("ipqua", _ip), ("bface", _ip), ("esuv1", _ip), ("esuv2", _ip),
         ("coord", _dp), ("cooro", _dp), ("coold", _dp),
         ("geoel", _dp), ("amatr", _dp),
@@ -83,6 +84,7 @@ class Oracle:
         self.lib = opcount_lib() if counting else lib()
         self.case = case
         self.ipqua = np.ascontiguousarray(case.ipqua, dtype=np.int32)
+        self.iptri = np.ascontiguousarray(case.iptri if case.ntria else np.zeros((1, 3)), dtype=np.int32)
         self.bface = np.ascontiguousarray(case.bface, dtype=np.int32)
         self.esuv1 = np.ascontiguousarray(case.esuv1, dtype=np.int32)
         self.esuv2 = np.ascontiguousarray(case.esuv2, dtype=np.int32)
@@ -108,6 +110,7 @@ class Oracle:
         s.slpdu = (c.gamlg + 1.0) / 2.0
         s.alfrz = 1.0 if c.nrz in (1, 2) else 0.0
         s.dtfix = case.dtfix; s.tacru = case.tacru; s.itime = case.itime
+        s.ntria = case.ntria; s.iptri = _p(self.iptri, _ip)
         s.ipqua = _p(self.ipqua, _ip); s.bface = _p(self.bface, _ip)
         s.esuv1 = _p(self.esuv1, _ip); s.esuv2 = _p(self.esuv2, _ip)
         s.coord = _p(self.coord); s.cooro = _p(self.cooro); s.coold = _p(self.coold)
@@ -115,7 +118,10 @@ class Oracle:
         s.unkno = _p(self.unkno); s.unold = _p(self.unold); s.ustar = _p(self.ustar)
         s.rhsel = _p(self.rhsel); s.aflim = _p(self.aflim); s.afvec = _p(self.afvec)
         from lag2d_amd.case import face_neighbours
-        self.esqua = np.ascontiguousarray(face_neighbours(case.ipqua, case.esuv1, case.esuv2), dtype=np.int32)
+        if case.ntria:
+            self.esqua = np.zeros((1, 4), dtype=np.int32)  # WENO (curved path) only
+        else:
+            self.esqua = np.ascontiguousarray(face_neighbours(case.ipqua, case.esuv1, case.esuv2), dtype=np.int32)
         self.troubled = np.zeros(n)
         s.esqua = _p(self.esqua, _ip); s.weno_strict = 1 if strict else 0; s.troubled = _p(self.troubled)
         self.s = s

@@ -22,10 +22,11 @@ def load(name):
     bf[:, 1] -= 1
     bf[:, 4] = np.where(bf[:, 4] > 0, bf[:, 4] - 1, 0)
     coord = z["coord"].copy()
+    iptri = (z["iptri"] - 1).astype(np.int32) if "iptri" in z.files else None
     case = LagCase(ctl=ctl, ipqua=(z["ipqua"] - 1).astype(np.int32), bface=bf,
                    esuv1=(z["esuv1"] - 1).astype(np.int32), esuv2=z["esuv2"].astype(np.int32),
                    coord=coord, coold=coord.copy(), geoel=z["geoel"].copy(), amatr=z["amatr"].copy(),
-                   unkno=z["unkno"].copy(), ustar=np.zeros_like(coord))
+                   unkno=z["unkno"].copy(), ustar=np.zeros_like(coord), iptri=iptri)
     return case, z
 
 

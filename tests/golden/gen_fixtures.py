@@ -45,12 +45,22 @@ def _sedov_q2_centre():
     return mg.sedov_problem(16, seed=5), "sedov_centre"
 
 
+def _sod_hybrid():
+    return mg.sod_hybrid_problem(40, 8), None
+
+
+def _sedov_hybrid():
+    return mg.sedov_hybrid_problem(16), "sedov_origin"
+
+
 SPECS = {
     # name: (builder, steps, dump-every)
     "sod_q1_100x4": (_sod, 400, 25),
     "sedov_q1_16x16": (_sedov_q1, 60, 20),
     "sedov_q2_12x12": (_sedov_q2, 0, 1),
     "sedov_q2c_16x16": (_sedov_q2_centre, 30, 10),
+    "sod_hyb_40x8": (_sod_hybrid, 200, 50),
+    "sedov_hyb_16x16": (_sedov_hybrid, 60, 20),
 }
 
 
@@ -75,10 +85,17 @@ def make(name):
         nd = ctl.ndegr
         _run(work, 0, 1, False)
         ncell, npoin, nbfac, nafac = (int(v) for v in _rd(work, "init_sizes.bin"))
-        if ic in ("sedov", "sedov_centre"):
+        if ic in ("sedov", "sedov_centre", "sedov_origin"):
             un = _rd(work, "init_unkno.bin").reshape(-1, 4, nd)[:ncell]
             ge = _rd(work, "init_geoel.bin").reshape(-1, 24)[:ncell]
-            cells = (0,) if ic == "sedov" else mg.centre_cells(mesh.nx, mesh.ny)
+            if ic == "sedov":
+                cells = (0,)
+            elif ic == "sedov_centre":
+                cells = mg.centre_cells(mesh.nx, mesh.ny)
+            else:  # every cell at the origin node (node 1), triangles numbered first
+                elems = ([e for e in range(mesh.ntria) if 1 in mesh.iptri[e]] +
+                         [mesh.ntria + e for e in range(mesh.nquad) if 1 in mesh.ipqua[e]])
+                cells = tuple(elems)
             inj = mg.sedov_inject(un, ge, cells)
             inj.astype(np.float64).tofile(os.path.join(work, "inject.bin"))
         _run(work, nsteps, every, ic is not None)
@@ -86,7 +103,7 @@ def make(name):
         nmatr = 4 if ctl.npoly == 1 else 16
         d = {
             "ctl": np.array(json.dumps(dataclasses.asdict(ctl))),
-            "ipqua": _rd(work, "init_ipqua.bin", np.int32).reshape(ncell, nvqua),
+            "ipqua": _rd(work, "init_ipqua.bin", np.int32).reshape(mesh.nquad, nvqua),
             "bface": _rd(work, "init_bface.bin", np.int32).reshape(nbfac, 5),
             "esuv1": _rd(work, "init_esuv1.bin", np.int32),
             "esuv2": _rd(work, "init_esuv2.bin", np.int32),
@@ -99,6 +116,8 @@ def make(name):
             "rhs_unkno": _rd(work, "rhs_unkno.bin").reshape(-1, 4, nd)[:ncell],
             "rhs_geoel": _rd(work, "rhs_geoel.bin").reshape(-1, 24)[:ncell],
         }
+        if mesh.ntria:
+            d["iptri"] = _rd(work, "init_iptri.bin", np.int32).reshape(mesh.ntria, 3)
         steps = sorted(int(f[1:].split("_")[0]) for f in os.listdir(work)
                        if f.startswith("s") and f.endswith("_time.bin"))
         d["steps"] = np.array(steps, dtype=np.int32)
