@@ -59,10 +59,23 @@ typedef struct lag2d_ctx lag2d_ctx;
  *   ipqua(nvqua, nquad)  quad nodes, 1-based, nvqua = 4 or 9
  *   bface(5, nbfac)      n1, n2, BC code, BC sub-code, mid node (1-based)
  *   esuv1(esuv2(npoin+1)), esuv2(npoin+1)  elements around points (CSR)
- * Quads only (ntria = 0). */
+ * Quads only (ntria = 0; lag2d_create_hybrid takes triangles too). */
 int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
                  const int *ipqua, const int *bface, const int *esuv1,
                  const int *esuv2, lag2d_ctx **ctx);
+
+/* Hybrid mesh of triangles and quads (straight P1 path, ncurv=0, nfint=1):
+ * the reference numbers triangles first, so cell ie < ntria is triangle ie
+ * (iptri(3, ntria), counter-clockwise, 1-based) and cell ntria + q is quad q
+ * (Preprocess.f90:168-198); esuv1 holds those cell ids.  Each cell type runs
+ * the reference's routines for it (Getrhslagmc.f90:6810-6939: the triangle
+ * limiter, Riemann coefficients, consistent-weight face integral and
+ * 7-point domain integral of Getrhslagmctria.f90 / Getrhslagmc.f90:14006-14160).
+ * The problems ncase = 1, 2, 3, -7, whose triangle limiter overrides
+ * boundary nodes (Getrhslagmctria.f90:539-582), return LAG2D_EUNSUPPORTED. */
+int lag2d_create_hybrid(const lag2d_config *cfg, int ntria, int nquad, int npoin, int nbfac,
+                        const int *iptri, const int *ipqua, const int *bface, const int *esuv1,
+                        const int *esuv2, lag2d_ctx **ctx);
 
 /* Ensemble of independent runs (batched mode; curved P2 path only).  No
  * counterpart in the reference, which runs one problem per process: the

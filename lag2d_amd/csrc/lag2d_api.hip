@@ -72,11 +72,13 @@ struct RunSpec {
     std::vector<double> tend, dtfix, tacru;
 };
 
-static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac, const int *ipqua,
-                       const int *bface, const int *esuv1, const int *esuv2, const RunSpec &rs, lag2d_ctx **out)
+static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin, int nbfac, const int *iptri,
+                       const int *ipqua, const int *bface, const int *esuv1, const int *esuv2, const RunSpec &rs,
+                       lag2d_ctx **out)
 {
-    if (!cfg || !out || !ipqua || !bface || !esuv1 || !esuv2) return fail(LAG2D_EINVAL, "null argument");
-    if (nquad <= 0 || npoin <= 0 || nbfac < 0) return fail(LAG2D_EINVAL, "bad sizes");
+    if (!cfg || !out || (nquad > 0 && !ipqua) || (ntria > 0 && !iptri) || !bface || !esuv1 || !esuv2)
+        return fail(LAG2D_EINVAL, "null argument");
+    if (ntria < 0 || nquad < 0 || ntria + nquad <= 0 || npoin <= 0 || nbfac < 0) return fail(LAG2D_EINVAL, "bad sizes");
     const lag2d_config &c = *cfg;
     const bool q1 = c.ncurv == 0 && c.nfint == 1 && c.npoly == 1 && c.ndegr == 3;
     const bool q2 = c.ncurv == 1 && c.nfint == 4 && c.npoly == 2 && c.ndegr == 6;
@@ -85,11 +87,17 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
     if (c.nrz != 0 || c.ndens != 1 || c.nlimi != 6 || c.nriem != 1)
         return fail(LAG2D_EUNSUPPORTED, "supported: nrz=0, ndens=1, nlimi=6, nriem=1");
     if (c.nstag != 2 && c.nstag != 3) return fail(LAG2D_EUNSUPPORTED, "nstag must be 2 or 3");
+    if (ntria > 0 && !q1) return fail(LAG2D_EUNSUPPORTED, "triangles: straight P1 path only (ncurv=0, nfint=1)");
+    if (ntria > 0 && (c.ncase == 1 || c.ncase == 2 || c.ncase == 3 || c.ncase == -7))
+        return fail(LAG2D_EUNSUPPORTED, "triangles: the boundary-node limiter overrides of ncase 1, 2, 3, -7 "
+                                        "(Getrhslagmctria.f90:539-582) are not implemented");
 
     lag2d_ctx *x = new lag2d_ctx();
     x->cfg = c;
     const int nv = c.ncurv == 0 ? 4 : 9;
-    x->nc = nquad;
+    const int nc = ntria + nquad;
+    x->nc = nc;
+    x->ntria = ntria;
     x->np = npoin;
     x->nb = nbfac;
     x->nd = c.ndegr;
@@ -105,20 +113,27 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
     const int nrun = rs.nrun;
     x->nrun = nrun;
 
-    // ---- host-side mesh preprocessing (0-based, SoA)
-    std::vector<int> ipq((size_t)nv * nquad);
-    for (int ie = 0; ie < nquad; ++ie)
+    // ---- host-side mesh preprocessing (0-based, SoA; triangles first, their
+    // unused fourth node slot repeats the third node)
+    std::vector<int> ipq((size_t)nv * nc);
+    for (int it = 0; it < ntria; ++it)
         for (int k = 0; k < nv; ++k) {
-            int p = ipqua[(size_t)k + (size_t)nv * ie] - 1;
+            int p = iptri[(size_t)(k < 3 ? k : 2) + (size_t)3 * it] - 1;
+            if (p < 0 || p >= npoin) return bad(fail(LAG2D_EINVAL, "iptri node out of range"));
+            ipq[(size_t)k * nc + it] = p;
+        }
+    for (int iq = 0; iq < nquad; ++iq)
+        for (int k = 0; k < nv; ++k) {
+            int p = ipqua[(size_t)k + (size_t)nv * iq] - 1;
             if (p < 0 || p >= npoin) return bad(fail(LAG2D_EINVAL, "ipqua node out of range"));
-            ipq[(size_t)k * nquad + ie] = p;
+            ipq[(size_t)k * nc + ntria + iq] = p;
         }
     const int ne = esuv2[npoin];
-    if (ne != nv * nquad) return bad(fail(LAG2D_EINVAL, "esuv2(npoin+1) != nvqua*nquad"));
+    if (ne != 3 * ntria + nv * nquad) return bad(fail(LAG2D_EINVAL, "esuv2(npoin+1) != 3*ntria + nvqua*nquad"));
     std::vector<int> e1(ne), e2(esuv2, esuv2 + npoin + 1);
     for (int k = 0; k < ne; ++k) {
         e1[k] = esuv1[k] - 1;
-        if (e1[k] < 0 || e1[k] >= nquad) return bad(fail(LAG2D_EINVAL, "esuv1 element out of range"));
+        if (e1[k] < 0 || e1[k] >= nc) return bad(fail(LAG2D_EINVAL, "esuv1 element out of range"));
     }
     for (int p = 0; p < npoin; ++p)
         if (e2[p + 1] < e2[p]) return bad(fail(LAG2D_EINVAL, "esuv2 not monotone"));
@@ -164,14 +179,14 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
 
     Dev &d = x->d;
     memset(&d, 0, sizeof(d));
-    d.nc = nquad; d.np = npoin; d.nb = nbfac; d.nd = c.ndegr; d.nvq = nv; d.nmatr = x->nmatr;
+    d.nc = nc; d.ntria = ntria; d.np = npoin; d.nb = nbfac; d.nd = c.ndegr; d.nvq = nv; d.nmatr = x->nmatr;
     d.nstag = c.nstag; d.ncase = c.ncase; d.ncurv = c.ncurv; d.npoly = c.npoly; d.ndt = c.ndt;
     d.gamlg = c.gamlg; d.cfl = c.cfl; d.cimpd = c.cimpd; d.cdrho = c.cdrho; d.tend = c.tend;
     d.slpdu = (c.gamlg + 1.0) / 2.0;
     int *t_ipq, *t_e1, *t_e2, *t_n2c, *t_n2co;
     uint8_t *t_fl, *t_col;
-    const size_t nU = (size_t)4 * c.ndegr * nquad;
-    x->nblk_dt = nrun == 1 ? std::min(nblocks(nquad), 1024) : 4;
+    const size_t nU = (size_t)4 * c.ndegr * nc;
+    x->nblk_dt = nrun == 1 ? std::min(nblocks(nc), 1024) : 4;
     if (q2) x->nrec = 160;
     if ((rc = x->alloc(&t_ipq, ipq.size())) || (rc = x->alloc(&t_e1, e1.size())) ||
         (rc = x->alloc(&t_e2, e2.size())) || (rc = x->alloc(&t_n2c, n2c.size())) ||
@@ -180,29 +195,29 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
         (rc = x->alloc(&d.yn, (size_t)npoin)) || (rc = x->alloc(&d.xs, (size_t)npoin)) ||
         (rc = x->alloc(&d.ys, (size_t)npoin)) || (rc = x->alloc(&x->x0, (size_t)2 * npoin)) ||
         (rc = x->alloc(&d.Un, nU)) || (rc = x->alloc(&d.Us, nU)) || (rc = x->alloc(&d.R, nU)) ||
-        (rc = x->alloc(&x->A, (size_t)x->nmatr * nquad)) || (rc = x->alloc(&d.G, (size_t)24 * nquad)) ||
+        (rc = x->alloc(&x->A, (size_t)x->nmatr * nc)) || (rc = x->alloc(&d.G, (size_t)24 * nc)) ||
         (rc = x->alloc(&d.usx, (size_t)npoin)) || (rc = x->alloc(&d.usy, (size_t)npoin)) ||
-        (rc = x->alloc(&d.cavg, (size_t)5 * nquad)) || (rc = x->alloc(&d.frame, (size_t)4 * nquad)) ||
-        (rc = x->alloc(&d.nmm, (size_t)10 * npoin)) || (rc = x->alloc(&d.lim, (size_t)9 * nquad)) ||
-        (rc = x->alloc(&d.rec, (size_t)x->nrec * nquad)) ||
+        (rc = x->alloc(&d.cavg, (size_t)5 * nc)) || (rc = x->alloc(&d.frame, (size_t)4 * nc)) ||
+        (rc = x->alloc(&d.nmm, (size_t)10 * npoin)) || (rc = x->alloc(&d.lim, (size_t)9 * nc)) ||
+        (rc = x->alloc(&d.rec, (size_t)x->nrec * nc)) ||
         (rc = x->alloc(&d.red, (size_t)2 * nrun * x->nblk_dt)) || (rc = x->alloc(&d.scal, (size_t)2 * nrun + 2)))
         return bad(rc);
     d.nrun = nrun;
     if (nrun > 1) {
         // run index of every cell and node
-        std::vector<int> runc(nquad), runn(npoin);
+        std::vector<int> runc(nc), runn(npoin);
         for (int r = 0; r < nrun; ++r) {
             for (int ie = rs.coff[r]; ie < rs.coff[r + 1]; ++ie) runc[ie] = r;
             for (int p = rs.noff[r]; p < rs.noff[r + 1]; ++p) runn[p] = r;
         }
-        for (int ie = 0; ie < nquad; ++ie)
+        for (int ie = 0; ie < nc; ++ie)
             for (int k = 0; k < nv; ++k) {
-                const int p = ipq[(size_t)k * nquad + ie];
+                const int p = ipq[(size_t)k * nc + ie];
                 if (runn[p] != runc[ie]) return bad(fail(LAG2D_EINVAL, "a cell uses a node of another run"));
             }
         int *t_runc, *t_runn, *t_coff, *t_act;
         double *t_tend;
-        if ((rc = x->alloc(&t_runc, (size_t)nquad)) || (rc = x->alloc(&t_runn, (size_t)npoin)) ||
+        if ((rc = x->alloc(&t_runc, (size_t)nc)) || (rc = x->alloc(&t_runn, (size_t)npoin)) ||
             (rc = x->alloc(&t_coff, (size_t)nrun + 1)) || (rc = x->alloc(&t_act, (size_t)nrun)) ||
             (rc = x->alloc(&t_tend, (size_t)nrun)))
             return bad(rc);
@@ -219,12 +234,12 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
     if (q2) {
         // face neighbours for the WENO stencils: neighbour id, or nc + f on
         // a boundary face (the reference keeps a ghost-cell id there)
-        std::vector<int> esq((size_t)4 * nquad);
-        for (int ie = 0; ie < nquad; ++ie)
+        std::vector<int> esq((size_t)4 * nc);
+        for (int ie = 0; ie < nc; ++ie)
             for (int f = 0; f < 4; ++f) {
-                int a = ipq[(size_t)f * nquad + ie], b = ipq[(size_t)((f + 1) & 3) * nquad + ie];
-                int nb = nquad + f;
-                for (int k = e2[a]; k < e2[a + 1] && nb >= nquad; ++k) {
+                int a = ipq[(size_t)f * nc + ie], b = ipq[(size_t)((f + 1) & 3) * nc + ie];
+                int nb = nc + f;
+                for (int k = e2[a]; k < e2[a + 1] && nb >= nc; ++k) {
                     int je = e1[k];
                     if (je == ie) continue;
                     for (int l = e2[b]; l < e2[b + 1]; ++l)
@@ -233,12 +248,12 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
                             break;
                         }
                 }
-                esq[(size_t)f * nquad + ie] = nb;
+                esq[(size_t)f * nc + ie] = nb;
             }
         int *t_esq;
-        if ((rc = x->alloc(&d.msub, (size_t)4 * nquad)) || (rc = x->alloc(&d.Ul, nU)) ||
-            (rc = x->alloc(&d.gq, (size_t)48 * nquad)) || (rc = x->alloc(&d.flag, (size_t)nquad)) ||
-            (rc = x->alloc(&d.tlist, (size_t)nquad)) || (rc = x->alloc(&d.tcount, 2)) ||
+        if ((rc = x->alloc(&d.msub, (size_t)4 * nc)) || (rc = x->alloc(&d.Ul, nU)) ||
+            (rc = x->alloc(&d.gq, (size_t)48 * nc)) || (rc = x->alloc(&d.flag, (size_t)nc)) ||
+            (rc = x->alloc(&d.tlist, (size_t)nc)) || (rc = x->alloc(&d.tcount, 2)) ||
             (rc = x->alloc(&d.tsum, 1)) ||
             (rc = x->alloc(&t_esq, esq.size())))
             return bad(rc);
@@ -285,7 +300,14 @@ static int create_impl(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
 extern "C" int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac, const int *ipqua,
                             const int *bface, const int *esuv1, const int *esuv2, lag2d_ctx **out)
 {
-    return create_impl(cfg, nquad, npoin, nbfac, ipqua, bface, esuv1, esuv2, RunSpec(), out);
+    return create_impl(cfg, 0, nquad, npoin, nbfac, nullptr, ipqua, bface, esuv1, esuv2, RunSpec(), out);
+}
+
+extern "C" int lag2d_create_hybrid(const lag2d_config *cfg, int ntria, int nquad, int npoin, int nbfac,
+                                   const int *iptri, const int *ipqua, const int *bface, const int *esuv1,
+                                   const int *esuv2, lag2d_ctx **out)
+{
+    return create_impl(cfg, ntria, nquad, npoin, nbfac, iptri, ipqua, bface, esuv1, esuv2, RunSpec(), out);
 }
 
 extern "C" int lag2d_create_ensemble(const lag2d_config *cfg, int nrun, const int *cell_off, const int *node_off,
@@ -310,7 +332,7 @@ extern "C" int lag2d_create_ensemble(const lag2d_config *cfg, int nrun, const in
     rs.tend.assign(tend, tend + nrun);
     rs.dtfix.assign(dtfix, dtfix + nrun);
     rs.tacru.assign(tacru, tacru + nrun);
-    return create_impl(cfg, nquad, npoin, nbfac, ipqua, bface, esuv1, esuv2, rs, out);
+    return create_impl(cfg, 0, nquad, npoin, nbfac, nullptr, ipqua, bface, esuv1, esuv2, rs, out);
 }
 
 extern "C" int lag2d_upload(lag2d_ctx *x, const double *coord, const double *coold, const double *geoel,

@@ -21,6 +21,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_dt_partial(Dev d, const double 
     const int c0 = d.nrun > 1 ? d.coff[r] : 0, c1 = d.nrun > 1 ? d.coff[r + 1] : nc;
     double mcfl = 1e10, mvol = 1e10;
     for (int ie = c0 + part * blockDim.x + threadIdx.x; ie < c1; ie += nbpr * blockDim.x) {
+        if (ie < d.ntria) continue;  // the reference's dt loops over quads only (Getrhslag.f90:20624-20698)
         double xp[9], yp[9];
         const int nv = d.ncurv == 0 ? 4 : 9;
         for (int k = 0; k < nv; ++k) {
@@ -226,6 +227,17 @@ __constant__ static const double WG6[6] = {0.4679139345726910, 0.467913934572691
 // set_centre (curved cells) the Q2 centre node is first reset to
 // -1/4 sum(corners) + 1/2 sum(mid-nodes) (Getrhslag.f90:1962-1971); it
 // belongs to this cell only, so the reset and the volume fuse.
+// rutope 7-point area rule of the reference triangle (dgflo2d.f90:2234-2255)
+__constant__ static const double PT7R_C[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115,
+                                              0.470142064105115, 0.797426985353087, 0.101286507323456,
+                                              0.101286507323456};
+__constant__ static const double PT7S_C[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770,
+                                              0.470142064105115, 0.101286507323456, 0.797426985353087,
+                                              0.101286507323456};
+__constant__ static const double WT7_C[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506,
+                                             0.132394152788506, 0.125939180544827, 0.125939180544827,
+                                             0.125939180544827};
+
 __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y,
                                                  int set_centre)
 {
@@ -233,7 +245,7 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
     const int cl = threadIdx.x >> 3, lane = threadIdx.x & 7;
     const int ie = blockIdx.x * CV_CELLS + cl;
     const int nc = d.nc;
-    if (ie < nc) {
+    if (ie < nc && ie >= d.ntria) {
         double xp[9], yp[9];
         if (d.ncurv == 0) {
             for (int k = 0; k < 4; ++k) {
@@ -284,6 +296,43 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
     }
     __syncthreads();
     if (ie >= nc || lane != 0) return;
+    if (ie < d.ntria) {
+        // triangle: 7-point rule on the quadratic triangle map with chord
+        // midpoints as mid-nodes (Limiterlag.f90:1898-2036)
+        double xp[6], yp[6];
+        for (int k = 0; k < 3; ++k) {
+            const int p = d.ipq[k * nc + ie];
+            xp[k] = x[p];
+            yp[k] = y[p];
+        }
+        xp[3] = 0.5 * (xp[0] + xp[1]); yp[3] = 0.5 * (yp[0] + yp[1]);
+        xp[4] = 0.5 * (xp[1] + xp[2]); yp[4] = 0.5 * (yp[1] + yp[2]);
+        xp[5] = 0.5 * (xp[0] + xp[2]); yp[5] = 0.5 * (yp[0] + yp[2]);
+        const double c10 = 1.0;
+        double rcv = 0.0, scv = 0.0, vol = 0.0;
+        for (int g = 0; g < 7; ++g) {
+            const double r = PT7R_C[g], s = PT7S_C[g], wi = WT7_C[g];
+            const double dr_[6] = {c10 - 4.0 * (c10 - r - s), -1.0 + 4.0 * r, 0.0, 4.0 * (1.0 - 2.0 * r - s),
+                                   4.0 * s, -4.0 * s};
+            const double ds_[6] = {c10 - 4.0 * (c10 - r - s), 0.0, -1.0 + 4.0 * s, -4.0 * r, 4.0 * r,
+                                   4.0 * (1.0 - r - 2.0 * s)};
+            double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+            for (int k = 0; k < 6; ++k) {
+                dxdr = dxdr + dr_[k] * xp[k];
+                dxds = dxds + ds_[k] * xp[k];
+                dydr = dydr + dr_[k] * yp[k];
+                dyds = dyds + ds_[k] * yp[k];
+            }
+            const double djaco = 0.5 * wi * (dxdr * dyds - dydr * dxds);
+            vol = vol + djaco * 1.0;
+            rcv = rcv + r * djaco * 1.0;
+            scv = scv + s * djaco * 1.0;
+        }
+        d.G[2 * nc + ie] = vol;
+        d.G[4 * nc + ie] = rcv / vol;
+        d.G[5 * nc + ie] = scv / vol;
+        return;
+    }
     // planar: rcoef = 1 - alfrz + alfrz*y = 1
     double rcv = 0.0, scv = 0.0, vol = 0.0;
 #pragma unroll

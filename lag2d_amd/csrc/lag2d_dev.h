@@ -6,7 +6,7 @@
 //   A[k*nc + ie]             packed inverse mass matrix (amatr row k)
 //   G[k*nc + ie]             geometry rows of geoel (row k, 0-based)
 //   X[ip], Y[ip]             node coordinates
-//   IPQ[iv*nc + ie]          quad nodes (0-based)
+//   IPQ[iv*nc + ie]          cell nodes (0-based); a triangle uses iv = 0..2
 // Arithmetic is FP64 throughout and is built without FMA contraction so the
 // results round exactly where the reference's do.
 #pragma once
@@ -44,6 +44,7 @@ __constant__ static const double YVQ[4] = {-1.0, -1.0, 1.0, 1.0};
 
 struct Dev {
     int nc, np, nb, nd, nvq, nmatr;
+    int ntria;               // cells 0..ntria-1 are triangles (straight P1 path), the rest quads
     int nstag, ncase, ncurv, npoly, ndt;
     double gamlg, cfl, cimpd, cdrho, tend, slpdu;
     // mesh

@@ -22,6 +22,7 @@ struct lag2d_timer {
 struct lag2d_ctx {
     lag2d_config cfg{};
     int nc = 0, np = 0, nb = 0, nd = 0, nvq = 0, nmatr = 0;
+    int ntria = 0;          // triangles first in the cell numbering (straight P1 path)
     int nrec = 40;          // corner-face record doubles per cell
     int nblk_dt = 1;        // blocks per run of the dt reduction
     int nrun = 1;           // runs of an ensemble

@@ -5,10 +5,11 @@
 
 #include "lag2d_host.h"
 
-// Straight quads: each node appears at exactly one corner of each of its
+// Straight cells: each node appears at exactly one corner of each of its
 // cells, and the reference scatters corner contributions cell by cell in
-// increasing cell id (Limiterlag.f90:1011-1016), which is the order of the
-// elements-around-points lists.  Entry = cell*4 + local vertex.
+// increasing cell id -- triangles (Getrhslagmctria.f90:1186-1199), then quads
+// (Limiterlag.f90:1011-1016), as the cells are numbered -- which is the order
+// of the elements-around-points lists.  Entry = cell*4 + local vertex.
 int build_node_corners(lag2d_ctx &x, const std::vector<int> &ipq, const std::vector<int> &esuv1,
                        const std::vector<int> &esuv2, std::vector<int> &n2c, std::vector<int> &n2c_off)
 {
@@ -22,7 +23,8 @@ int build_node_corners(lag2d_ctx &x, const std::vector<int> &ipq, const std::vec
             if (ie < prev) return LAG2D_EINVAL;  // element lists must be ascending
             prev = ie;
             int iv = -1;
-            for (int j = 0; j < (nv == 4 ? 4 : 9); ++j)
+            const int nvc = ie < x.ntria ? 3 : (nv == 4 ? 4 : 9);
+            for (int j = 0; j < nvc; ++j)
                 if (ipq[(size_t)j * nc + ie] == p) iv = j;
             if (iv < 0) return LAG2D_EINVAL;
             if (nv == 4) {

@@ -13,6 +13,12 @@
 //   getriecoef_matrixnew         Limiterlag.f90:1037-1076
 //   rhsifacedg_lag_mc_hybridquad2 Getrhslagmc.f90:14468-14618
 //   rhsdomndg_lag_mc_quadsym     Limiterlag.f90:1108-1490
+// and for the triangles of a hybrid mesh (cells 0..ntria-1):
+//   barthlimit_lagsym_tria       Getrhslagmctria.f90:422-842
+//   barthlimit_sympre_trialb     Getrhslagmctria.f90:2644-3258
+//   getriem_tria_matrixsym       Getrhslagmctria.f90:847-1204
+//   rhsifacedg_lag_mc_hybridtria Getrhslagmc.f90:14006-14160
+//   rhsdomndg_lag_mc_triasym     Getrhslagmctria.f90:1208-1510
 #include "lag2d_dev.h"
 #include "lag2d_kernels.h"
 
@@ -22,6 +28,339 @@ __device__ __forceinline__ double xvq(int iv) { return (iv == 1 || iv == 2) ? 1.
 __device__ __forceinline__ double yvq(int iv) { return (iv >= 2) ? 1.0 : -1.0; }
 
 #define UI(q, id) Uin[((q) * nd + (id)) * nc + ie]
+
+// ---- triangles (cells 0..ntria-1 of a hybrid mesh)
+// Reference triangle r, s in [0, 1] with vertices (0,0), (1,0), (0,1); the P1
+// basis is scaled by dr = ds = 0.5 (Getrhslagmctria.f90:464-470).
+__device__ __forceinline__ double xvt(int iv) { return iv == 1 ? 1.0 : 0.0; }
+__device__ __forceinline__ double yvt(int iv) { return iv == 2 ? 1.0 : 0.0; }
+// corner faces of vertex iv: (edge before, edge after), edge e joins nodes e
+// and e+1; and the far end of each of those edges
+__constant__ static const int TFE[3][2] = {{2, 0}, {0, 1}, {1, 2}};
+__constant__ static const int TNB[3][2] = {{2, 1}, {0, 2}, {1, 0}};
+// rutope 7-point area rule (dgflo2d.f90:2234-2255), its literal values
+__constant__ static const double PT7R[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115,
+                                            0.470142064105115, 0.797426985353087, 0.101286507323456,
+                                            0.101286507323456};
+__constant__ static const double PT7S[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770,
+                                            0.470142064105115, 0.101286507323456, 0.797426985353087,
+                                            0.101286507323456};
+__constant__ static const double WT7[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506,
+                                           0.132394152788506, 0.125939180544827, 0.125939180544827,
+                                           0.125939180544827};
+
+// Limiter coefficients and corner-face Riemann records of one triangle:
+// barthlimit_lagsym_tria (Getrhslagmctria.f90:422-842) with the velocity
+// limiter of barthlimit_sympre_trialb (:2644-3258) in the frame of the
+// cell-average velocity, and getriem_tria_matrixsym (:847-1204), whose
+// impedance uses |u* - u_v| (not its normal component) at both faces of a
+// corner.  Records as for quads, corner faces cf = f + 2*iv, iv = 0..2.
+__device__ void limit_corner_tri(const Dev &d, const double *__restrict__ Uin, int ie)
+{
+    const int nc = d.nc, nd = d.nd, np = d.np;
+    const double g1 = d.gamlg - 1.0, dr = 0.5, ds = 0.5;
+    int ip[3];
+    double xp[3], yp[3];
+    for (int k = 0; k < 3; ++k) {
+        ip[k] = d.ipq[k * nc + ie];
+        xp[k] = d.xs[ip[k]];
+        yp[k] = d.ys[ip[k]];
+    }
+    const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
+    double bt[3][3];
+    for (int iv = 0; iv < 3; ++iv) {
+        bt[iv][0] = 1.0;
+        bt[iv][1] = (xvt(iv) - rc) / dr;
+        bt[iv][2] = (yvt(iv) - sc) / ds;
+    }
+    double un[3][4];
+    for (int iv = 0; iv < 3; ++iv) {
+        for (int q = 0; q < 4; ++q) un[iv][q] = 0.0;
+        for (int id = 0; id < 3; ++id)
+            for (int q = 0; q < 4; ++q) un[iv][q] = un[iv][q] + UI(q, id) * bt[iv][id];
+    }
+    const double tau = UI(0, 0), uctr = UI(1, 0), vctr = UI(2, 0), ectr = UI(3, 0);
+    const double rhoct = 1.0 / tau;
+    const double pctr_sp = dmax(EPS_SP6, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+    double al[5];
+    {
+        double unctr[4] = {1.0 / rhoct, uctr, vctr, pctr_sp};
+        double a[4] = {1.e300, 1.e300, 1.e300, 1.e300};
+        for (int iv = 0; iv < 3; ++iv) {
+            double rhov = 1.0 / un[iv][0];
+            double uv = un[iv][1], vv = un[iv][2], ev = un[iv][3];
+            double pv = dmax(EPS_SP6, g1 * rhov * (ev - 0.5 * (uv * uv + vv * vv)));
+            double vals[4] = {1.0 / rhov, uv, vv, pv};
+            int p = ip[iv];
+            for (int q = 0; q < 4; ++q) {
+                double f = barthfct(d.nmm[q * np + p], d.nmm[(5 + q) * np + p], unctr[q], vals[q] - unctr[q]);
+                a[q] = dmin(a[q], f);
+            }
+        }
+        for (int q = 0; q < 4; ++q) al[q] = a[q];
+    }
+    double af[4];
+    {
+        const double m0 = d.frame[0 * nc + ie], m1 = d.frame[1 * nc + ie];
+        const double m2 = d.frame[2 * nc + ie], m3 = d.frame[3 * nc + ie];
+        double uc2 = uctr * m0 + vctr * m1, uc3 = uctr * m2 + vctr * m3;
+        double fixi = 1.e300, fiet = 1.e300;
+        for (int iv = 0; iv < 3; ++iv) {
+            double un2 = un[iv][1] * m0 + un[iv][2] * m1;
+            double un3 = un[iv][1] * m2 + un[iv][2] * m3;
+            double mx2 = uc2, mx3 = uc3, mn2 = uc2, mn3 = uc3;
+            int p = ip[iv];
+            int col = d.ncase != 1 ? d.ncol[p] : 0;
+            for (int k = d.esuv2[p]; k < d.esuv2[p + 1]; ++k) {
+                int je = d.esuv1[k];
+                double ucj = d.cavg[1 * nc + je], vcj = d.cavg[2 * nc + je];
+#define ACC(UU, VV)                                            \
+    {                                                          \
+        double ul = (UU) * m0 + (VV) * m1;                     \
+        double vl = (UU) * m2 + (VV) * m3;                     \
+        mx2 = dmax(mx2, ul); mx3 = dmax(mx3, vl);              \
+        mn2 = dmin(mn2, ul); mn3 = dmin(mn3, vl);              \
+    }
+                ACC(ucj, vcj);
+                if (col == 3) {
+                    ACC(ucj, -vcj);
+                } else if (col == 2) {
+                    ACC(-ucj, vcj);
+                } else if (col == 10) {
+                    ACC(ucj, -vcj);
+                    ACC(-ucj, vcj);
+                    ACC(-ucj, -vcj);
+                }
+#undef ACC
+            }
+            fixi = dmin(fixi, barthfct(mx2, mn2, uc2, un2 - uc2));
+            fiet = dmin(fiet, barthfct(mx3, mn3, uc3, un3 - uc3));
+        }
+        af[0] = m0 * m0 * fixi + m2 * m2 * fiet;
+        af[2] = m0 * m1 * fixi + m2 * m3 * fiet;
+        af[1] = af[2];
+        af[3] = m1 * m1 * fixi + m3 * m3 * fiet;
+    }
+    {
+        double a5 = 1.e300;
+        for (int iv = 0; iv < 3; ++iv) {
+            double rhov = 1.0 / un[iv][0];
+            double uv = un[iv][1], vv = un[iv][2];
+            double pv = dmax(EPS_SP6, g1 * rhov * (un[iv][3] - 0.5 * (uv * uv + vv * vv)));
+            double t = tau + al[0] * (1.0 / rhov - tau);
+            double pl = pctr_sp + al[3] * (pv - pctr_sp);
+            double e5 = pl / g1 * t + 0.5 * (uv * uv + vv * vv);
+            int p = ip[iv];
+            a5 = dmin(a5, barthfct(d.nmm[4 * np + p], d.nmm[9 * np + p], ectr, e5 - ectr));
+        }
+        al[4] = a5;
+    }
+    for (int q = 0; q < 5; ++q) d.lim[q * nc + ie] = al[q];
+    for (int k = 0; k < 4; ++k) d.lim[(5 + k) * nc + ie] = af[k];
+
+    const double eps = 1.0e-6;
+    double gnx[3], gny[3], glen[3];
+    for (int e = 0; e < 3; ++e) {
+        int a = e, b = e == 2 ? 0 : e + 1;
+        double anx = yp[b] - yp[a];
+        double any = -(xp[b] - xp[a]);
+        gnx[e] = anx / sqrt(anx * anx + any * any);
+        gny[e] = any / sqrt(anx * anx + any * any);
+        glen[e] = sqrt(anx * anx + any * any);
+    }
+    const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+    const double dudr = af[0] * UI(1, 1) + af[2] * UI(2, 1);
+    const double duds = af[0] * UI(1, 2) + af[2] * UI(2, 2);
+    const double dvdr = af[1] * UI(1, 1) + af[3] * UI(2, 1);
+    const double dvds = af[1] * UI(1, 2) + af[3] * UI(2, 2);
+    const double sdctr = sqrt(dmax(eps, d.gamlg * pctr / rhoct));
+    for (int iv = 0; iv < 3; ++iv) {
+        double rhovt = 1.0 / un[iv][0];
+        double pv = dmax(eps, g1 * rhovt * (un[iv][3] - 0.5 * (un[iv][1] * un[iv][1] + un[iv][2] * un[iv][2])));
+        double uvx = uctr + dudr * bt[iv][1] + duds * bt[iv][2];
+        double vvx = vctr + dvdr * bt[iv][1] + dvds * bt[iv][2];
+        double pvx = pctr + al[3] * (pv - pctr);
+        int p = ip[iv];
+        double acnx = d.usx[p] - uvx, acny = d.usy[p] - vvx;
+        double aj0, aj1;
+        if (sqrt(acnx * acnx + acny * acny) < (double)1.e-11f) {
+            aj0 = (double)1.e-11f;
+            aj1 = (double)1.e-11f;
+        } else {
+            aj0 = acnx / sqrt(acnx * acnx + acny * acny);
+            aj1 = acny / sqrt(acnx * acnx + acny * acny);
+        }
+        const double a3 = sqrt(acnx * acnx + acny * acny) / sdctr;
+        const double deltu = sqrt(acnx * acnx + acny * acny);
+        const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
+        for (int f = 0; f < 2; ++f) {
+            int e = TFE[iv][f];
+            double nx = gnx[e], ny = gny[e];
+            // corner area of the face, planar rcoef = 1: (2*1 + 1)/3*0.5*L
+            double area = (2.0 * 1.0 + 1.0) / 3.0 * 0.5 * glen[e];
+            double vnau = (a3 < 1.e12) ? 1.0 : nx * aj0 + ny * aj1;
+            double m11 = mur * area * fabs(vnau);
+            double mu1 = m11 * uvx + 0.0 * vvx;
+            double mu2 = 0.0 * uvx + m11 * vvx;
+            double sn1 = -pvx * area * nx + 0.0 * area * ny;
+            double sn2 = 0.0 * area * nx + -pvx * area * ny;
+            int cf = f + 2 * iv;
+            d.rec[(0 * 8 + cf) * nc + ie] = m11;
+            d.rec[(1 * 8 + cf) * nc + ie] = mu1;
+            d.rec[(2 * 8 + cf) * nc + ie] = mu2;
+            d.rec[(3 * 8 + cf) * nc + ie] = sn1;
+            d.rec[(4 * 8 + cf) * nc + ie] = sn2;
+        }
+    }
+}
+
+// Face integral with consistent corner weights (rhsifacedg_lag_mc_hybridtria,
+// Getrhslagmc.f90:14006-14160), domain integral on the 7-point rule
+// (rhsdomndg_lag_mc_triasym, Getrhslagmctria.f90:1208-1510), in-place
+// limiting and, with apply_update, the mass-matrix solve and RK blend of one
+// triangle.
+__device__ void cellrhs_tri(const Dev &d, const double *__restrict__ Uin, double alfa, int apply_update, int ie)
+{
+    const int nc = d.nc, nd = d.nd;
+    const double c13 = 0.3333333333333333, c16 = 0.1666666666666666;
+    const double g1 = d.gamlg - 1.0, dr = 0.5, ds = 0.5;
+    int ip[3];
+    double xp[3], yp[3], us[3], vs[3];
+    for (int k = 0; k < 3; ++k) {
+        ip[k] = d.ipq[k * nc + ie];
+        xp[k] = d.xs[ip[k]];
+        yp[k] = d.ys[ip[k]];
+        us[k] = d.usx[ip[k]];
+        vs[k] = d.usy[ip[k]];
+    }
+    const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
+    double gnx[3], gny[3], glen[3];
+    for (int e = 0; e < 3; ++e) {
+        int a = e, b = e == 2 ? 0 : e + 1;
+        double anx = yp[b] - yp[a];
+        double any = -(xp[b] - xp[a]);
+        gnx[e] = anx / sqrt(anx * anx + any * any);
+        gny[e] = any / sqrt(anx * anx + any * any);
+        glen[e] = sqrt(anx * anx + any * any);
+    }
+    double bt[3][3];
+    for (int iv = 0; iv < 3; ++iv) {
+        bt[iv][0] = 1.0;
+        bt[iv][1] = (xvt(iv) - rc) / dr;
+        bt[iv][2] = (yvt(iv) - sc) / ds;
+    }
+    double r[4][3];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 3; ++id) r[q][id] = 0.0;
+    for (int iv = 0; iv < 3; ++iv) {
+        double f[2][2];  // [face][dim]
+        for (int fa = 0; fa < 2; ++fa) {
+            int cf = fa + 2 * iv;
+            double m11 = d.rec[(0 * 8 + cf) * nc + ie];
+            double mu1 = d.rec[(1 * 8 + cf) * nc + ie], mu2 = d.rec[(2 * 8 + cf) * nc + ie];
+            double sn1 = d.rec[(3 * 8 + cf) * nc + ie], sn2 = d.rec[(4 * 8 + cf) * nc + ie];
+            f[fa][0] = sn1 + m11 * us[iv] + 0.0 * vs[iv] - mu1;
+            f[fa][1] = sn2 + 0.0 * us[iv] + m11 * vs[iv] - mu2;
+        }
+        const int e0 = TFE[iv][0], e1 = TFE[iv][1], j0 = TNB[iv][0], j1 = TNB[iv][1];
+        for (int id = 0; id < 3; ++id) {
+            double w0 = 2.0 * bt[iv][id] + bt[j0][id];
+            double w1 = 2.0 * bt[iv][id] + bt[j1][id];
+            double l10 = c16 * w0 * gnx[e0] * glen[e0], l20 = c16 * w0 * gny[e0] * glen[e0];
+            double l11 = c16 * w1 * gnx[e1] * glen[e1], l21 = c16 * w1 * gny[e1] * glen[e1];
+            r[0][id] = r[0][id] + us[iv] * l10 + vs[iv] * l20 + us[iv] * l11 + vs[iv] * l21;
+            r[1][id] = r[1][id] + f[0][0] * c13 * w0 + f[1][0] * c13 * w1;
+            r[2][id] = r[2][id] + f[0][1] * c13 * w0 + f[1][1] * c13 * w1;
+            r[3][id] = r[3][id] + us[iv] * f[0][0] * c13 * w0 + vs[iv] * f[0][1] * c13 * w0 +
+                       us[iv] * f[1][0] * c13 * w1 + vs[iv] * f[1][1] * c13 * w1;
+        }
+    }
+    double al0 = d.lim[0 * nc + ie], al3 = d.lim[3 * nc + ie], al4 = d.lim[4 * nc + ie];
+    double af0 = d.lim[5 * nc + ie], af1 = d.lim[6 * nc + ie], af2 = d.lim[7 * nc + ie], af3 = d.lim[8 * nc + ie];
+    double u0[4][3];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 3; ++id) u0[q][id] = UI(q, id);
+    const double eps = 1.0e-6;
+    const double rhoct = 1.0 / u0[0][0];
+    const double uctr = u0[1][0], vctr = u0[2][0], ectr = u0[3][0];
+    const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+    const double dudr = af0 * u0[1][1] + af2 * u0[2][1];
+    const double duds = af0 * u0[1][2] + af2 * u0[2][2];
+    const double dvdr = af1 * u0[1][1] + af3 * u0[2][1];
+    const double dvds = af1 * u0[1][2] + af3 * u0[2][2];
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    {
+        const double dspr[3] = {-1.0, 1.0, 0.0}, dsps[3] = {-1.0, 0.0, 1.0};
+        for (int k = 0; k < 3; ++k) {
+            dxdr = dxdr + dspr[k] * xp[k];
+            dxds = dxds + dsps[k] * xp[k];
+            dydr = dydr + dspr[k] * yp[k];
+            dyds = dyds + dsps[k] * yp[k];
+        }
+    }
+    const double dbdr[3] = {0.0, 1.0 / dr, 0.0}, dbds[3] = {0.0, 0.0, 1.0 / ds};
+    double gd0[3], gd1[3];
+    for (int id = 0; id < 3; ++id) {
+        gd0[id] = dyds * dbdr[id] + (-dydr) * dbds[id];
+        gd1[id] = (-dxds) * dbdr[id] + dxdr * dbds[id];
+    }
+    for (int g = 0; g < 7; ++g) {
+        const double rr = PT7R[g], ss = PT7S[g], djaco = 0.5 * WT7[g];
+        // planar: rcoef = 1 - alfrz + alfrz*y = 1
+        const double rcoef = 1.0 - 0.0 + 0.0 * 0.0;
+        double b[3] = {1.0, (rr - rc) / dr, (ss - sc) / ds};
+        double un[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int id = 0; id < 3; ++id)
+            for (int q = 0; q < 4; ++q) un[q] = un[q] + u0[q][id] * b[id];
+        double rhoad = 1.0 / un[0];
+        double pres = g1 * rhoad * (un[3] - 0.5 * (un[1] * un[1] + un[2] * un[2]));
+        double uadv = uctr + dudr * b[1] + duds * b[2];
+        double vadv = vctr + dvdr * b[1] + dvds * b[2];
+        pres = pctr + al3 * (pres - pctr);
+        for (int id = 0; id < 3; ++id) {
+            double f0 = gd0[id] * uadv + gd1[id] * vadv;
+            double f1 = gd0[id] * (-pres);
+            double f2 = gd1[id] * (-pres);
+            double f3 = (gd0[id] * uadv + gd1[id] * vadv) * (-pres);
+            r[0][id] = r[0][id] - f0 * djaco * rcoef;
+            r[1][id] = r[1][id] - f1 * djaco * rcoef;
+            r[2][id] = r[2][id] - f2 * djaco * rcoef;
+            r[3][id] = r[3][id] - f3 * djaco * rcoef;
+        }
+    }
+    double ul[4][3];
+    for (int q = 0; q < 4; ++q) ul[q][0] = u0[q][0];
+    ul[0][1] = u0[0][1] * al0;
+    ul[0][2] = u0[0][2] * al0;
+    ul[1][1] = dudr; ul[1][2] = duds;
+    ul[2][1] = dvdr; ul[2][2] = dvds;
+    ul[3][1] = u0[3][1] * al4;
+    ul[3][2] = u0[3][2] * al4;
+    if (apply_update) {
+        const double dt = d.scal[0];
+        double m[3][3];
+        m[0][0] = d.A[3 * nc + ie]; m[0][1] = 0.0; m[0][2] = 0.0;
+        m[1][0] = 0.0; m[1][1] = d.A[0 * nc + ie]; m[1][2] = d.A[1 * nc + ie];
+        m[2][0] = 0.0; m[2][1] = d.A[1 * nc + ie]; m[2][2] = d.A[2 * nc + ie];
+        double rs[4][3];
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < 3; ++id) rs[q][id] = dt * r[q][id];
+        for (int id = 0; id < 3; ++id) {
+            double s4[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int k = 0; k < 3; ++k)
+                for (int q = 0; q < 4; ++q) s4[q] = s4[q] + m[id][k] * rs[q][k];
+            for (int q = 0; q < 4; ++q)
+                d.Us[(q * nd + id) * nc + ie] =
+                    alfa * d.Un[(q * nd + id) * nc + ie] + (1.0 - alfa) * (ul[q][id] + s4[q]);
+        }
+    } else {
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < 3; ++id) {
+                d.Us[(q * nd + id) * nc + ie] = ul[q][id];
+                d.R[(q * nd + id) * nc + ie] = r[q][id];
+            }
+    }
+}
 
 // Cell averages (tau, u, v, p, E) and the principal-strain frame of
 // barthlimit_sympre_quadlb Part 1 (Limiterlag.f90:8262-8387).
@@ -39,6 +378,22 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_cellavg_q1(Dev d, const double 
     d.cavg[2 * nc + ie] = v;
     d.cavg[3 * nc + ie] = pctr;
     d.cavg[4 * nc + ie] = e;
+    if (ie < d.ntria) {
+        // triangles: frame of the cell-average velocity
+        // (barthlimit_sympre_trialb, Getrhslagmctria.f90:2810-2822)
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+        if (sqrt(u * u + v * v) > 1.e-10) {
+            m0 = u / sqrt(u * u + v * v);
+            m1 = v / sqrt(u * u + v * v);
+            m2 = -v / sqrt(u * u + v * v);
+            m3 = u / sqrt(u * u + v * v);
+        }
+        d.frame[0 * nc + ie] = m0;
+        d.frame[1 * nc + ie] = m1;
+        d.frame[2 * nc + ie] = m2;
+        d.frame[3 * nc + ie] = m3;
+        return;
+    }
 
     double xp[4], yp[4];
     for (int k = 0; k < 4; ++k) {
@@ -118,6 +473,10 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_limit_corner_q1(Dev d, const do
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc, nd = d.nd, np = d.np;
     if (ie >= nc) return;
+    if (ie < d.ntria) {
+        limit_corner_tri(d, Uin, ie);
+        return;
+    }
     const double g1 = d.gamlg - 1.0;
     int ip[4];
     double xp[4], yp[4];
@@ -327,6 +686,10 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_cellrhs_q1(Dev d, const double 
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc, nd = d.nd;
     if (ie >= nc) return;
+    if (ie < d.ntria) {
+        cellrhs_tri(d, Uin, alfa, apply_update, ie);
+        return;
+    }
     const double c13 = 0.3333333333333333, c16 = 0.1666666666666666;
     const double g1 = d.gamlg - 1.0;
     int ip[4];

@@ -15,7 +15,7 @@ from .case import LagCase
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liblag2d.so")
 
-EXPORTS = ("lag2d_create", "lag2d_create_ensemble", "lag2d_get_run_times", "lag2d_set_initial_density", "lag2d_weno_cells",
+EXPORTS = ("lag2d_create", "lag2d_create_hybrid", "lag2d_create_ensemble", "lag2d_get_run_times", "lag2d_set_initial_density", "lag2d_weno_cells",
            "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
            "lag2d_division_check", "lag2d_destroy", "lag2d_last_error")
@@ -50,6 +50,8 @@ def load():
     lib = C.CDLL(LIBPATH)
     lib.lag2d_create.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip,
                                  C.POINTER(C.c_void_p)]
+    lib.lag2d_create_hybrid.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip,
+                                        _ip, C.POINTER(C.c_void_p)]
     lib.lag2d_create_ensemble.argtypes = [C.POINTER(Config), C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_int,
                                           C.c_int, _ip, _ip, _ip, _ip, C.POINTER(C.c_void_p)]
     lib.lag2d_get_run_times.argtypes = [C.c_void_p, _dp, _dp]
@@ -108,8 +110,14 @@ class Lag2D:
         e1 = np.ascontiguousarray(case.esuv1 + 1, dtype=np.int32)
         e2 = np.ascontiguousarray(case.esuv2, dtype=np.int32)
         h = C.c_void_p()
-        _check(lib.lag2d_create(C.byref(cfg), case.nquad, case.npoin, case.nbfac, _p(ipq, _ip), _p(bf, _ip),
-                                _p(e1, _ip), _p(e2, _ip), C.byref(h)))
+        if case.ntria:
+            ipt = np.ascontiguousarray(case.iptri + 1, dtype=np.int32)
+            _check(lib.lag2d_create_hybrid(C.byref(cfg), case.ntria, case.nquad, case.npoin, case.nbfac,
+                                           _p(ipt, _ip), _p(ipq, _ip), _p(bf, _ip), _p(e1, _ip), _p(e2, _ip),
+                                           C.byref(h)))
+        else:
+            _check(lib.lag2d_create(C.byref(cfg), case.nquad, case.npoin, case.nbfac, _p(ipq, _ip), _p(bf, _ip),
+                                    _p(e1, _ip), _p(e2, _ip), C.byref(h)))
         self.h = h
         self.upload(case)
         if rho0 is not None:
