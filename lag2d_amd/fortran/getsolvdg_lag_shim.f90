@@ -51,10 +51,6 @@ type(c_ptr), save :: ctx = c_null_ptr
 type(lag2d_config) :: cfg
 real(c_double) :: dt, t
 
-if (ntria /= 0) then
-  write(*,*) 'lag2d: getsolvdg_lag on the GPU takes quad meshes (ntria = 0)'
-  error stop 1
-end if
 if (itime == 1 .and. c_associated(ctx)) then
   call lag2d_destroy(ctx)
   ctx = c_null_ptr
@@ -65,7 +61,12 @@ if (.not. c_associated(ctx)) then
   cfg%nfint = nfint; cfg%nriem = nriem; cfg%ndt = ndt
   cfg%gamlg = gamlg; cfg%cfl = cfl; cfg%cimpd = cimpd; cfg%cdrho = cdrho; cfg%tend = tend
   cfg%dtfix = dtfix; cfg%tacru = tacru; cfg%strict = 1
-  call lag2d_require(lag2d_create(cfg, nquad, npoin, nbfac, ipqua, bface, esuv1, esuv2, ctx), 'create')
+  if (ntria > 0) then
+    call lag2d_require(lag2d_create_hybrid(cfg, ntria, nquad, npoin, nbfac, iptri, ipqua, bface, esuv1, esuv2, &
+                                           ctx), 'create')
+  else
+    call lag2d_require(lag2d_create(cfg, nquad, npoin, nbfac, ipqua, bface, esuv1, esuv2, ctx), 'create')
+  end if
   call lag2d_require(lag2d_upload(ctx, coord, coold, geoel, amatr, unkno, ustar), 'upload')
 end if
 call lag2d_require(lag2d_step(ctx, itime), 'step')

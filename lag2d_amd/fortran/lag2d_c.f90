@@ -28,6 +28,15 @@ module lag2d_c
       type(c_ptr), intent(out) :: ctx
     end function lag2d_create
 
+    integer(c_int) function lag2d_create_hybrid(cfg, ntria, nquad, npoin, nbfac, iptri, ipqua, bface, &
+        esuv1, esuv2, ctx) bind(c, name='lag2d_create_hybrid')
+      import :: c_int, c_ptr, lag2d_config
+      type(lag2d_config), intent(in) :: cfg
+      integer(c_int), value :: ntria, nquad, npoin, nbfac
+      integer(c_int), intent(in) :: iptri(*), ipqua(*), bface(*), esuv1(*), esuv2(*)
+      type(c_ptr), intent(out) :: ctx
+    end function lag2d_create_hybrid
+
     integer(c_int) function lag2d_upload(ctx, coord, coold, geoel, amatr, unkno, ustar) &
         bind(c, name='lag2d_upload')
       import :: c_int, c_ptr, c_double
