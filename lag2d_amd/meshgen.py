@@ -266,6 +266,16 @@ def sedov_hybrid_problem(nx=16, ny=None, fraction=0.3, seed=9, length=1.2):
     return mesh, Control(cfl=0.25, tend=1.0, ncase=7)
 
 
+def sedov_rz_hybrid_problem(nx=16, ny=None, fraction=0.3, seed=13, length=1.2):
+    """Sedov blast in cylindrical (r = y, z = x) geometry with the area-weighted
+    RZ scheme (nrz = 2) on a jittered hybrid mesh, P1 / Barth limiter.  The
+    axis is the wall y = 0.  dtfix is the reference's fixed RZ Sedov step
+    (Getrhslag.f90:1673-1676), which the first right-hand side also uses."""
+    ny = nx if ny is None else ny
+    mesh = split_quads(quad_q1(nx, ny, 0.0, length, 0.0, length * ny / nx, seed=seed), fraction, seed + 1)
+    return mesh, Control(cfl=0.25, tend=1.0, ncase=7, nrz=2, dtfix=1e-6)
+
+
 def sedov_problem(nx=120, ny=None, seed=2024, length=1.2):
     ny = nx if ny is None else ny
     mesh = quad_q2(nx, ny, 0.0, length, 0.0, length * ny / nx, seed=seed)

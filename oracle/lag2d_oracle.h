@@ -50,6 +50,10 @@ typedef struct orc_case {
     int weno_strict;                /* 1: fail where the reference stops (WENO.f90:7699) */
     int nweno_skipped;              /* troubled cells skipped for their neighbourhood */
     double *troubled;               /* Persson flag of the last stage (0 / 100), or NULL */
+    /* RZ area weighting (nrz=2): module scalars getsolvdg_lag sets per stage
+     * and per step (Getrhslag.f90:1633-1692, :1745) */
+    int rkstg;
+    double epsaw;
 } orc_case;
 
 /* getcellvol_lag (Limiterlag.f90:1869-2164), quads */
@@ -60,6 +64,8 @@ int orc_rhs(orc_case *c);
 /* getsolvdg_lag (Getrhslag.f90:1553-2035), one full SSP-RK step followed by
  * the driver's copy-back (dgflo2d.f90:408-422). */
 int orc_step(orc_case *c);
+/* epsaw (and with set_dt, dtfix) of step c->itime, Getrhslag.f90:1663-1737 */
+void orc_schedule(orc_case *c, int set_dt);
 /* curved P2 sub-cell right-hand side (lag2d_oracle_q2.c) */
 int orc_rhs_q2(orc_case *c);
 /* Op counting (oracle/opcount/): the arithmetic after a phase marker is

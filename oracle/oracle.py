@@ -33,6 +33,7 @@ class _Case(C.Structure):
         ("unkno", _dp), ("unold", _dp), ("ustar", _dp),
         ("rhsel", _dp), ("aflim", _dp), ("afvec", _dp),
         ("esqua", _ip), ("weno_strict", C.c_int), ("nweno_skipped", C.c_int), ("troubled", _dp),
+        ("rkstg", C.c_int), ("epsaw", C.c_double),
     ]
 
 
@@ -46,6 +47,7 @@ def lib():
         _LIB.orc_rhs.argtypes = [C.POINTER(_Case)]
         _LIB.orc_step.argtypes = [C.POINTER(_Case)]
         _LIB.orc_cellvol.argtypes = [C.POINTER(_Case), _dp]
+        _LIB.orc_schedule.argtypes = [C.POINTER(_Case), C.c_int]
     return _LIB
 
 
@@ -130,6 +132,8 @@ class Oracle:
         """One right-hand side on the current state (stage 1 of step 1)."""
         self.lib.orc_cellvol(C.byref(self.s), _p(self.coord))
         self.s.itime = max(self.s.itime, 1)
+        self.s.rkstg = 1
+        self.lib.orc_schedule(C.byref(self.s), 0)
         rc = self.lib.orc_rhs(C.byref(self.s))
         if rc:
             raise RuntimeError(f"oracle rhs: unsupported configuration ({rc})")

@@ -116,6 +116,12 @@ w_geoel = geoel; w_unkno = unkno; w_ustar = ustar; w_unkgd = unkgd; w_coora = co
 w_amatr = amatr; w_unkaw = unkno
 call getcellvol_lag(iptri, ipqua, w_geoel, coord)
 rkstg = 1
+! epsaw of the first step (Getrhslag.f90:1663-1737): the RZ area weighting
+! (nrz = 2) reads it in the right-hand side
+epsaw = 1.d-1
+if(nrz.ge.1.and.ncase.eq.7) epsaw = 1.d0
+if(nrz.ge.1.and.ncase.eq.11) epsaw = 1.d-3
+if(ncase.eq.-14) epsaw = 1.d-2
 cooro = coord
 call getrhsdg_lag_mcnew(w_ustar,uchar,bface,w_unkno,w_unkaw,w_unkgd,strnq_devtp,rhsel,rhsgd,intfac,inpoel,iptri,ipqua,&
                         geofa,w_geoel,gesgt0,gesgq0,cooro,coold,w_coora,esuv1,esuv2,ltelem,estri,esqua,w_amatr,1)

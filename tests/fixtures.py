@@ -26,7 +26,7 @@ def load(name):
     case = LagCase(ctl=ctl, ipqua=(z["ipqua"] - 1).astype(np.int32), bface=bf,
                    esuv1=(z["esuv1"] - 1).astype(np.int32), esuv2=z["esuv2"].astype(np.int32),
                    coord=coord, coold=coord.copy(), geoel=z["geoel"].copy(), amatr=z["amatr"].copy(),
-                   unkno=z["unkno"].copy(), ustar=np.zeros_like(coord), iptri=iptri)
+                   unkno=z["unkno"].copy(), ustar=np.zeros_like(coord), iptri=iptri, dtfix=ctl.dtfix)
     return case, z
 
 

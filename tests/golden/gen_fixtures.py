@@ -53,6 +53,10 @@ def _sedov_hybrid():
     return mg.sedov_hybrid_problem(16), "sedov_origin"
 
 
+def _sedov_rz_hybrid():
+    return mg.sedov_rz_hybrid_problem(16), "sedov_origin"
+
+
 SPECS = {
     # name: (builder, steps, dump-every)
     "sod_q1_100x4": (_sod, 400, 25),
@@ -61,6 +65,7 @@ SPECS = {
     "sedov_q2c_16x16": (_sedov_q2_centre, 30, 10),
     "sod_hyb_40x8": (_sod_hybrid, 200, 50),
     "sedov_hyb_16x16": (_sedov_hybrid, 60, 20),
+    "sedov_rz_16x16": (_sedov_rz_hybrid, 40, 10),
 }
 
 
