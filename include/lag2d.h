@@ -134,13 +134,21 @@ int lag2d_rhs(lag2d_ctx *ctx, double *rhsel);
 
 /* Download the state (same layouts as lag2d_upload); NULL pointers skip.
  * geoel receives rows 3, 5, 6 (current volume and volume centre) updated and
- * the uploaded values elsewhere. */
+ * the uploaded values elsewhere; on the RZ path (nrz=2) also rows 1, 2
+ * (mass centre), 11, 12 (mean radius, area), 13, 14 (predicted mass centre)
+ * and 15, 16 (area centre), which it updates every stage. */
 int lag2d_download(lag2d_ctx *ctx, double *coord, double *geoel, double *unkno,
                    double *ustar);
 
 /* Driver scalars after the last step (module constant dtfix, tacru); of
  * run 0 for an ensemble. */
 int lag2d_get_time(lag2d_ctx *ctx, double *dtfix, double *tacru);
+
+/* The resident inverse mass matrix amatr(nmatr, ncell), as lag2d_upload takes
+ * it.  The RZ path recomputes it every stage (getamatr_lagmc_rzawrev,
+ * Getrhslagrz.f90:3633-4221, writing the amatr argument of getsolvdg_lag,
+ * Getrhslag.f90:1747); the planar paths leave it as uploaded. */
+int lag2d_get_mass_matrix(lag2d_ctx *ctx, double *amatr);
 
 /* dt and time of every run of an ensemble (arrays of nrun; one entry for a
  * single problem).  No counterpart in the reference. */

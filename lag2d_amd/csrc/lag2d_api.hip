@@ -44,6 +44,7 @@ static int dalloc(T **p, size_t n)
 }
 
 static inline int nblocks(int n) { return (n + LAG2D_BLOCK - 1) / LAG2D_BLOCK; }
+static void launch_cellvol(lag2d_ctx *x, double *xs, double *ys, int set_centre, bool timed);
 
 lag2d_ctx::~lag2d_ctx()
 {
@@ -84,8 +85,18 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     const bool q2 = c.ncurv == 1 && c.nfint == 4 && c.npoly == 2 && c.ndegr == 6;
     if (!(q1 || q2))
         return fail(LAG2D_EUNSUPPORTED, "supported: ncurv=0/nfint=1/P1 or ncurv=1/nfint=4/P2");
-    if (c.nrz != 0 || c.ndens != 1 || c.nlimi != 6 || c.nriem != 1)
-        return fail(LAG2D_EUNSUPPORTED, "supported: nrz=0, ndens=1, nlimi=6, nriem=1");
+    if ((c.nrz != 0 && c.nrz != 2) || c.ndens != 1 || c.nlimi != 6 || c.nriem != 1)
+        return fail(LAG2D_EUNSUPPORTED, "supported: nrz=0 or 2, ndens=1, nlimi=6, nriem=1");
+    if (c.nrz == 2) {
+        if (!q1) return fail(LAG2D_EUNSUPPORTED, "nrz=2: straight P1 path only (ncurv=0, nfint=1)");
+        // ncase 1, 2, 12 place wall normals by node position and 3, 4 have
+        // pressure boundaries (GetBClag.f90:84-165); -7 overrides the limiter
+        if (c.ncase == 1 || c.ncase == 2 || c.ncase == 3 || c.ncase == 4 || c.ncase == 12 || c.ncase == -7)
+            return fail(LAG2D_EUNSUPPORTED, "nrz=2: ncase 1, 2, 3, 4, 12, -7 are not implemented");
+        for (int f = 0; f < nbfac; ++f)
+            if (bface[5 * (size_t)f + 2] != 22)
+                return fail(LAG2D_EUNSUPPORTED, "nrz=2: slip walls (boundary code 22) only");
+    }
     if (c.nstag != 2 && c.nstag != 3) return fail(LAG2D_EUNSUPPORTED, "nstag must be 2 or 3");
     if (ntria > 0 && !q1) return fail(LAG2D_EUNSUPPORTED, "triangles: straight P1 path only (ncurv=0, nfint=1)");
     if (ntria > 0 && (c.ncase == 1 || c.ncase == 2 || c.ncase == 3 || c.ncase == -7))
@@ -177,8 +188,30 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         }
     }
 
+    // RZ slip-wall forces: the (at most two) wall faces of each node, in
+    // face order, as other node*4 + (1: subcode 221, 2: 222, 0: other)
+    std::vector<int> wall;
+    if (c.nrz == 2) {
+        wall.assign((size_t)2 * npoin, -1);
+        for (int f = 0; f < nbfac; ++f) {
+            const int *b = bface + 5 * (size_t)f;
+            if (b[2] != 22) continue;
+            const int typ = b[3] == 221 ? 1 : (b[3] == 222 ? 2 : 0);
+            for (int k = 0; k < 2; ++k) {
+                const int p = b[k] - 1, o = b[1 - k] - 1;
+                nflag[p] |= 8;
+                const int j = wall[p] < 0 ? 0 : 1;
+                if (j == 1 && wall[(size_t)npoin + p] >= 0)
+                    return bad(fail(LAG2D_EINVAL, "nrz=2: a node on more than two wall faces"));
+                wall[(size_t)j * npoin + p] = o * 4 + typ;
+            }
+        }
+    }
+
     Dev &d = x->d;
     memset(&d, 0, sizeof(d));
+    d.nrz = c.nrz;
+    d.alfrz = (c.nrz == 1 || c.nrz == 2) ? 1.0 : 0.0;
     d.nc = nc; d.ntria = ntria; d.np = npoin; d.nb = nbfac; d.nd = c.ndegr; d.nvq = nv; d.nmatr = x->nmatr;
     d.nstag = c.nstag; d.ncase = c.ncase; d.ncurv = c.ncurv; d.npoly = c.npoly; d.ndt = c.ndt;
     d.gamlg = c.gamlg; d.cfl = c.cfl; d.cimpd = c.cimpd; d.cdrho = c.cdrho; d.tend = c.tend;
@@ -264,6 +297,13 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
             hipMemset(d.tcount, 0, 2 * sizeof(int)) != hipSuccess ||
             hipMemset(d.tsum, 0, sizeof(unsigned long long)) != hipSuccess)
             return bad(fail(LAG2D_EDEVICE, "curved-path table upload"));
+    }
+    if (c.nrz == 2) {
+        int *t_wall;
+        if ((rc = x->alloc(&t_wall, wall.size()))) return bad(rc);
+        if (hipMemcpy(t_wall, wall.data(), wall.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+            return bad(fail(LAG2D_EDEVICE, "wall table upload"));
+        d.wall = t_wall;
     }
     d.ipq = t_ipq; d.esuv1 = t_e1; d.esuv2 = t_e2; d.n2c = t_n2c; d.n2c_off = t_n2co;
     d.nflag = t_fl; d.ncol = t_col; d.A = x->A;
@@ -368,7 +408,7 @@ extern "C" int lag2d_upload(lag2d_ctx *x, const double *coord, const double *coo
     for (int ie = 0; ie < nc; ++ie)
         for (int k = 0; k < nm; ++k) h[(size_t)k * nc + ie] = amatr[k + (size_t)nm * ie];
     HIPCHK(up(x->A, (size_t)nm * nc));
-    hipLaunchKernelGGL(k_cellvol, dim3((nc + CV_CELLS - 1) / CV_CELLS), dim3(LAG2D_BLOCK), 0, x->stream, d, d.xn, d.yn, 0);
+    launch_cellvol(x, d.xn, d.yn, 0, false);
     HIPCHK(hipGetLastError());
     int rc = upload_extra(*x);
     if (rc) return rc;
@@ -404,16 +444,72 @@ void lag2d_ctx::toc(int k)
 const char *const lag2d_kernel_names[KID_COUNT] = {
     "k_cellavg_q1", "k_nodeminmax", "k_limit_corner_q1", "k_node_q1", "k_cellrhs_q1", "k_dt_partial",
     "k_dt_final",   "k_update",     "k_nodemove",        "k_cellvol",
-    "k_q2_prep",    "k_q2_weno",    "k_q2_corner",       "k_node_q2", "k_cellrhs_q2", "k_run_flags"};
+    "k_q2_prep",    "k_q2_weno",    "k_q2_corner",       "k_node_q2", "k_cellrhs_q2", "k_run_flags",
+    "k_limit_corner_rz", "k_node_rz", "k_cellrhs_rz",    "k_schedule"};
 
 static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
 
-// RHS of one stage on state Uin and stage coordinates (ds.xs/ds.ys); leaves
-// the limited DOFs in Us.  apply_update fuses the M^-1 solve and RK blend.
-static int launch_rhs(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int apply_update)
+// The dtfix and epsaw that getsolvdg_lag sets at the start of step itime
+// (Getrhslag.f90:1663-1737) for the problems that fix them; returns whether
+// dt is fixed.  epsaw (the RZ basis-motion lead) is 0.1 otherwise.
+static bool schedule(const lag2d_config &c, int itime, double *dt, double *epsaw)
+{
+    *epsaw = 1.e-1;
+    if (c.nrz >= 1 && c.ncase == 7) {
+        if (itime <= 2000) { *dt = 1e-6; *epsaw = 1.0; }
+        else if (itime <= 6000) { *dt = 2.5e-6; *epsaw = 4.e-1; }
+        else if (itime <= 10000) { *dt = 5.e-6; *epsaw = 2.e-1; }
+        else { *dt = 5.e-5; *epsaw = 2.e-2; }
+        return true;
+    }
+    if (c.nrz >= 1 && c.ncase == 11) {
+        if (itime <= 7500) { *dt = 1.e-4; *epsaw = 1.e-3; }
+        else { *dt = 2.5e-6; *epsaw = 2.e-2; }
+        return true;
+    }
+    if (c.ncase == -4) { *dt = 2.e-6; return true; }
+    if (c.ncase == 13) {
+        *dt = itime <= 7500 ? 1.e-4 : (itime <= 17500 ? 2.e-5 : 1e-5);
+        return true;
+    }
+    if (c.ncase == -14) {
+        if (itime <= 1000) { *dt = 5.e-4; *epsaw = 1.e-2; }
+        else *dt = 1.e-4;
+        return true;
+    }
+    return false;
+}
+
+static void launch_cellvol(lag2d_ctx *x, double *xs, double *ys, int set_centre, bool timed)
+{
+    const int grid = (x->nc + CV_CELLS - 1) / CV_CELLS;
+    if (timed) x->tic(KID_CELLVOL);
+    if (x->cfg.nrz == 2)
+        hipLaunchKernelGGL(k_cellvol<1>, dim3(grid), dim3(LAG2D_BLOCK), 0, x->stream, x->d, xs, ys, set_centre);
+    else
+        hipLaunchKernelGGL(k_cellvol<0>, dim3(grid), dim3(LAG2D_BLOCK), 0, x->stream, x->d, xs, ys, set_centre);
+    if (timed) x->toc(KID_CELLVOL);
+}
+
+// RHS of one stage (0-based) of step itime on state Uin and stage
+// coordinates (ds.xs/ds.ys); leaves the limited DOFs in Us.  apply_update
+// fuses the M^-1 solve and RK blend.
+static int launch_rhs(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int apply_update, int itime,
+                      int stage)
 {
     const int nc = x->nc, np = x->np;
-    if (x->cfg.ncurv == 0) {
+    if (x->cfg.nrz == 2) {
+        double dt, epsaw;
+        schedule(x->cfg, itime, &dt, &epsaw);
+        // the stage's mass centre and mass matrix are recomputed from the
+        // second stage of the first step on (Getrhslagmc.f90:6882-6885)
+        const int rev = (itime > 1 || stage > 0) ? 1 : 0;
+        LAUNCH(KID_CELLAVG_Q1, k_cellavg_q1, nblocks(nc), ds, Uin);
+        LAUNCH(KID_NODEMINMAX, k_nodeminmax, nblocks(np), ds);
+        LAUNCH(KID_LIMIT_RZ, k_limit_corner_rz, nblocks(nc), ds, Uin, rev);
+        LAUNCH(KID_NODE_RZ, k_node_rz, nblocks(np), ds);
+        LAUNCH(KID_CELLRHS_RZ, k_cellrhs_rz, nblocks(nc), ds, Uin, alfa, apply_update, epsaw);
+    } else if (x->cfg.ncurv == 0) {
         LAUNCH(KID_CELLAVG_Q1, k_cellavg_q1, nblocks(nc), ds, Uin);
         LAUNCH(KID_NODEMINMAX, k_nodeminmax, nblocks(np), ds);
         LAUNCH(KID_LIMIT_Q1, k_limit_corner_q1, nblocks(nc), ds, Uin);
@@ -433,6 +529,8 @@ static int step_impl(lag2d_ctx *x, int itime)
     const int nc = x->nc, np = x->np;
     const int ns = x->cfg.nstag;
     if (x->nrun > 1) LAUNCH(KID_RUN_FLAGS, k_run_flags, nblocks(x->nrun), d);
+    double dtfix, epsaw;
+    if (schedule(x->cfg, itime, &dtfix, &epsaw)) LAUNCH(KID_SCHEDULE, k_schedule, nblocks(x->nrun), d, dtfix);
     for (int s = 0; s < ns; ++s) {
         const double alfa = RKLAG[ns - 2][s];
         Dev ds = d;
@@ -441,7 +539,7 @@ static int step_impl(lag2d_ctx *x, int itime)
             ds.ys = d.yn;
         }
         const double *Uin = (s == 0) ? d.Un : d.Us;
-        int rc = launch_rhs(x, ds, Uin, alfa, s > 0);
+        int rc = launch_rhs(x, ds, Uin, alfa, s > 0, itime, s);
         if (rc) return rc;
         if (s == 0) {
             LAUNCH(KID_DT_PARTIAL, k_dt_partial, x->nrun * x->nblk_dt, ds, Uin, x->nblk_dt);
@@ -449,11 +547,12 @@ static int step_impl(lag2d_ctx *x, int itime)
             LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
         }
         LAUNCH(KID_NODEMOVE, k_nodemove, nblocks(np), d, alfa, s == 0 ? d.xn : d.xs, s == 0 ? d.yn : d.ys);
-        if (x->cfg.ncurv == 1 && s + 1 < ns)
-            LAUNCH(KID_CELLVOL, k_cellvol, (nc + CV_CELLS - 1) / CV_CELLS, d, d.xs, d.ys, 1);
+        // the volumes of the stage coordinates: the curved path's centre
+        // nodes, and the RZ path's mean radii, are read by the next stage
+        if ((x->cfg.ncurv == 1 || x->cfg.nrz == 2) && s + 1 < ns) launch_cellvol(x, d.xs, d.ys, x->cfg.ncurv == 1, true);
     }
     // final volumes; curved cells reset their centre node here for the last stage
-    LAUNCH(KID_CELLVOL, k_cellvol, (nc + CV_CELLS - 1) / CV_CELLS, d, d.xs, d.ys, x->cfg.ncurv == 1);
+    launch_cellvol(x, d.xs, d.ys, x->cfg.ncurv == 1, true);
     HIPCHK(hipGetLastError());
     std::swap(d.Un, d.Us);
     std::swap(d.xn, d.xs);
@@ -520,7 +619,7 @@ extern "C" int lag2d_rhs(lag2d_ctx *x, double *rhsel)
     Dev ds = d;
     ds.xs = d.xn;
     ds.ys = d.yn;
-    int rc = launch_rhs(x, ds, d.Un, 0.0, 0);
+    int rc = launch_rhs(x, ds, d.Un, 0.0, 0, 1, 0);
     if (rc) return rc;
     const int nc = x->nc, nd = x->nd;
     const size_t nU = (size_t)4 * nd * nc;
@@ -571,10 +670,23 @@ extern "C" int lag2d_download(lag2d_ctx *x, double *coord, double *geoel, double
         HIPCHK(hipMemcpy(h.data(), d.G, (size_t)24 * nc * sizeof(double), hipMemcpyDeviceToHost));
         for (int ie = 0; ie < nc; ++ie)
             for (int k = 0; k < 24; ++k) {
-                bool live = (k == 2 || k == 4 || k == 5);
+                bool live = (k == 2 || k == 4 || k == 5) ||
+                            (x->cfg.nrz == 2 && (k == 0 || k == 1 || (k >= 10 && k <= 15)));
                 geoel[k + 24 * (size_t)ie] = live ? h[(size_t)k * nc + ie] : x->geo_host[k + 24 * (size_t)ie];
             }
     }
+    return 0;
+}
+
+extern "C" int lag2d_get_mass_matrix(lag2d_ctx *x, double *amatr)
+{
+    if (!x || !amatr) return fail(LAG2D_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(x->stream));
+    const int nc = x->nc, nm = x->nmatr;
+    std::vector<double> h((size_t)nm * nc);
+    HIPCHK(hipMemcpy(h.data(), x->A, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int ie = 0; ie < nc; ++ie)
+        for (int k = 0; k < nm; ++k) amatr[k + (size_t)nm * ie] = h[(size_t)k * nc + ie];
     return 0;
 }
 

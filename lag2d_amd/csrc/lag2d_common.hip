@@ -228,6 +228,19 @@ __constant__ static const double WG6[6] = {0.4679139345726910, 0.467913934572691
 // -1/4 sum(corners) + 1/2 sum(mid-nodes) (Getrhslag.f90:1962-1971); it
 // belongs to this cell only, so the reset and the volume fuse.
 // rutope 7-point area rule of the reference triangle (dgflo2d.f90:2234-2255)
+// The area-weighted rows of a cell from its radius-weighted volume, area
+// and area moments.
+__device__ __forceinline__ void rz_rows(const Dev &d, int ie, double vol, double area, double rca, double sca)
+{
+    const int nc = d.nc;
+    d.G[4 * nc + ie] = rca / area;
+    d.G[5 * nc + ie] = sca / area;
+    d.G[10 * nc + ie] = vol / area;
+    d.G[11 * nc + ie] = area;
+    d.G[14 * nc + ie] = rca / area;
+    d.G[15 * nc + ie] = sca / area;
+}
+
 __constant__ static const double PT7R_C[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115,
                                               0.470142064105115, 0.797426985353087, 0.101286507323456,
                                               0.101286507323456};
@@ -238,10 +251,16 @@ __constant__ static const double WT7_C[7] = {0.225000000000000, 0.13239415278850
                                              0.132394152788506, 0.125939180544827, 0.125939180544827,
                                              0.125939180544827};
 
+// With RZ (nrz=2) the volume is the radius-weighted one (rcoef = 1 - alfrz
+// + alfrz*y at each point) and the area-weighted rows follow
+// (Limiterlag.f90:2001-2035, :2140-2160): geoel(11) mean radius = volume /
+// area, geoel(12) area, and the area centre in geoel(5:6) and geoel(15:16).
+template <int RZ>
 __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y,
                                                  int set_centre)
 {
     __shared__ double sdj[CV_CELLS][36 + 1];
+    __shared__ double syg[RZ ? CV_CELLS : 1][36 + 1];
     const int cl = threadIdx.x >> 3, lane = threadIdx.x & 7;
     const int ie = blockIdx.x * CV_CELLS + cl;
     const int nc = d.nc;
@@ -292,6 +311,11 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
                 dyds = dyds + ds_[k] * yp[k];
             }
             sdj[cl][g] = wi * (dxdr * dyds - dydr * dxds);
+            if (RZ) {
+                double yg = 0.0;
+                for (int k = 0; k < 9; ++k) yg = yg + sh[k] * yp[k];
+                syg[RZ ? cl : 0][g] = yg;
+            }
         }
     }
     __syncthreads();
@@ -309,7 +333,7 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
         xp[4] = 0.5 * (xp[1] + xp[2]); yp[4] = 0.5 * (yp[1] + yp[2]);
         xp[5] = 0.5 * (xp[0] + xp[2]); yp[5] = 0.5 * (yp[0] + yp[2]);
         const double c10 = 1.0;
-        double rcv = 0.0, scv = 0.0, vol = 0.0;
+        double rcv = 0.0, scv = 0.0, vol = 0.0, area = 0.0, rca = 0.0, sca = 0.0;
         for (int g = 0; g < 7; ++g) {
             const double r = PT7R_C[g], s = PT7S_C[g], wi = WT7_C[g];
             const double dr_[6] = {c10 - 4.0 * (c10 - r - s), -1.0 + 4.0 * r, 0.0, 4.0 * (1.0 - 2.0 * r - s),
@@ -324,13 +348,45 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
                 dyds = dyds + ds_[k] * yp[k];
             }
             const double djaco = 0.5 * wi * (dxdr * dyds - dydr * dxds);
-            vol = vol + djaco * 1.0;
-            rcv = rcv + r * djaco * 1.0;
-            scv = scv + s * djaco * 1.0;
+            if (RZ) {
+                const double sh[6] = {-(c10 - r - s) * (c10 - 2.0 * (c10 - r - s)), -r * (c10 - 2.0 * r),
+                                      -s * (c10 - 2.0 * s), 4.0 * r * (c10 - r - s), 4.0 * r * s,
+                                      4.0 * s * (c10 - r - s)};
+                double yg = 0.0;
+                for (int k = 0; k < 6; ++k) yg = yg + sh[k] * yp[k];
+                const double rcoef = 1.0 - d.alfrz + d.alfrz * yg;
+                vol = vol + djaco * rcoef;
+                area = area + djaco;
+                rca = rca + r * djaco;
+                sca = sca + s * djaco;
+            } else {
+                vol = vol + djaco * 1.0;
+                rcv = rcv + r * djaco * 1.0;
+                scv = scv + s * djaco * 1.0;
+            }
         }
         d.G[2 * nc + ie] = vol;
-        d.G[4 * nc + ie] = rcv / vol;
-        d.G[5 * nc + ie] = scv / vol;
+        if (RZ) {
+            rz_rows(d, ie, vol, area, rca, sca);
+        } else {
+            d.G[4 * nc + ie] = rcv / vol;
+            d.G[5 * nc + ie] = scv / vol;
+        }
+        return;
+    }
+    if (RZ) {
+        double vol = 0.0, area = 0.0, rca = 0.0, sca = 0.0;
+        for (int g = 0; g < 36; ++g) {
+            const int i = g / 6, j = g - 6 * i;
+            const double r = PG6[i], s = PG6[j], djaco = sdj[cl][g];
+            const double rcoef = 1.0 - d.alfrz + d.alfrz * syg[RZ ? cl : 0][g];
+            vol = vol + djaco * rcoef;
+            area = area + djaco;
+            rca = rca + r * djaco;
+            sca = sca + s * djaco;
+        }
+        d.G[2 * nc + ie] = vol;
+        rz_rows(d, ie, vol, area, rca, sca);
         return;
     }
     // planar: rcoef = 1 - alfrz + alfrz*y = 1
@@ -347,5 +403,8 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
     d.G[4 * nc + ie] = rcv / vol;
     d.G[5 * nc + ie] = scv / vol;
 }
+
+template __global__ void k_cellvol<0>(Dev, double *__restrict__, double *__restrict__, int);
+template __global__ void k_cellvol<1>(Dev, double *__restrict__, double *__restrict__, int);
 
 }  // namespace lag2d

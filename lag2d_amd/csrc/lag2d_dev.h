@@ -45,6 +45,8 @@ __constant__ static const double YVQ[4] = {-1.0, -1.0, 1.0, 1.0};
 struct Dev {
     int nc, np, nb, nd, nvq, nmatr;
     int ntria;               // cells 0..ntria-1 are triangles (straight P1 path), the rest quads
+    int nrz;                 // 0 planar, 2 area-weighted RZ (r = y)
+    double alfrz;            // 1 for RZ: rcoef = 1 - alfrz + alfrz*y
     int nstag, ncase, ncurv, npoly, ndt;
     double gamlg, cfl, cimpd, cdrho, tend, slpdu;
     // mesh
@@ -52,14 +54,16 @@ struct Dev {
     const int *esuv1, *esuv2;
     const int *n2c;          // node -> corner CSR entries (ie*NCORN + corner), offsets n2c_off
     const int *n2c_off;
-    const uint8_t *nflag;    // bit0: v=0 wall, bit1: u=0 wall, bit2: fixed wall (no solve)
+    const uint8_t *nflag;    // bit0: v=0 wall, bit1: u=0 wall, bit2: fixed wall (no solve),
+                             // bit3: node of a slip-wall face (code 22)
+    const int *wall;         // RZ: [2][np] slip-wall faces of a node, other node*4 + (1: 221, 2: 222), -1
     const uint8_t *ncol;     // symmetry colouring for the velocity limiter (0,2,3,10)
     // state
     double *xn, *yn;         // x^n
     double *xs, *ys;         // stage coordinates
     const double *x0, *y0;   // initial coordinates
     double *Un, *Us;         // U^n and stage state
-    const double *A;         // inverse mass matrix
+    double *A;               // inverse mass matrix (updated per stage on the RZ path)
     double *G;               // geoel rows [24][nc]
     double *usx, *usy;       // nodal velocity
     double *R;               // right-hand side [(q*nd+id)*nc + ie]

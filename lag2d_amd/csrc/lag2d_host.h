@@ -11,7 +11,8 @@
 enum {
     KID_CELLAVG_Q1 = 0, KID_NODEMINMAX, KID_LIMIT_Q1, KID_NODE_Q1, KID_CELLRHS_Q1, KID_DT_PARTIAL,
     KID_DT_FINAL, KID_UPDATE, KID_NODEMOVE, KID_CELLVOL,
-    KID_Q2_PREP, KID_Q2_WENO, KID_Q2_CORNER, KID_NODE_Q2, KID_CELLRHS_Q2, KID_RUN_FLAGS, KID_COUNT
+    KID_Q2_PREP, KID_Q2_WENO, KID_Q2_CORNER, KID_NODE_Q2, KID_CELLRHS_Q2, KID_RUN_FLAGS,
+    KID_LIMIT_RZ, KID_NODE_RZ, KID_CELLRHS_RZ, KID_SCHEDULE, KID_COUNT
 };
 extern const char *const lag2d_kernel_names[KID_COUNT];
 

@@ -18,7 +18,14 @@ __global__ void k_run_flags(Dev d);
 __global__ void k_division_check(const double *a, const double *b, double *q, int n);
 __global__ void k_update(Dev d, double alfa);
 __global__ void k_nodemove(Dev d, double alfa, const double *__restrict__ xsrc, const double *__restrict__ ysrc);
+template <int RZ>
 __global__ void k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y, int set_centre);
+
+// area-weighted RZ path (nrz=2), straight P1 cells
+__global__ void k_limit_corner_rz(Dev d, const double *__restrict__ Uin, int rev);
+__global__ void k_node_rz(Dev d);
+__global__ void k_cellrhs_rz(Dev d, const double *__restrict__ Uin, double alfa, int apply_update, double epsaw);
+__global__ void k_schedule(Dev d, double dt);
 constexpr int CV_CELLS = 32;  // cells per block of k_cellvol
 
 }  // namespace lag2d

@@ -9,9 +9,12 @@
 ! when the driver restarts at itime = 1) and stay resident on the GPU; each
 ! call advances one step and copies the results back into the arrays the
 ! driver reads afterwards -- unold (new DOFs), cooro (new coordinates),
-! ustar, geoel rows 3, 5, 6 -- and the module scalars dtfix and tacru, as
-! the reference routine leaves them.  The driver then copies unold into
-! unkno and cooro into coord (dgflo2d.f90:408-422) as before.
+! ustar, geoel rows 3, 5, 6 (on the RZ path nrz=2 also its mass-centre,
+! radius and area rows, and amatr, which the reference's RZ stages rewrite
+! through this routine's amatr argument) -- and the module scalars dtfix,
+! tacru and epsaw, as the reference routine leaves them.  The driver then
+! copies unold into unkno and cooro into coord (dgflo2d.f90:408-422) as
+! before.
 subroutine getsolvdg_lag(intfac,inpoel,iptri,ipqua,bface,cooro,coord,coora,geofa,geoel,gesgt0,gesgq0,uchar,&
                          unkno,unold,unkaw,unkgd,unogd,strnq_devtp,&
                          ltelem,estri,esqua,itime,amatr,amagd,ustar,coold, esuv1, esuv2)
@@ -35,7 +38,9 @@ real*8,dimension(1:ndegr,1:nq,1:nsize),intent(in)::unkno
 integer*4,dimension(1:3,1:ncell)::ltelem
 integer, dimension(1:nftri,1:ntria), intent(in)::estri
 integer, dimension(1:nfqua,1:nquad), intent(in)::esqua
-real*8,dimension(nmatr,ncell), intent(in)::amatr
+! declared intent(in) by the reference, whose RZ stages nevertheless
+! rewrite it (Getrhslagmc.f90:6724); no intent here so it can be returned
+real*8,dimension(nmatr,ncell)::amatr
 real*8,dimension(1:ndimn,1:npoin)     ::ustar
 integer*4,dimension(1:nbfai,1:nbfac)::bface
 integer*4, intent(in)::esuv1(npoin1),esuv2(npoin2)
@@ -74,5 +79,27 @@ call lag2d_require(lag2d_download(ctx, cooro, geoel, unold, ustar), 'download')
 call lag2d_require(lag2d_get_time(ctx, dt, t), 'get_time')
 dtfix = dt
 tacru = t
+if (nrz == 2) then
+  call lag2d_require(lag2d_get_mass_matrix(ctx, amatr), 'get_mass_matrix')
+  ! the step's epsaw (Getrhslag.f90:1663-1692)
+  epsaw = 1.d-1
+  if (ncase == 7) then
+    if (itime <= 2000) then
+      epsaw = 1.d0
+    elseif (itime <= 6000) then
+      epsaw = 4.d-1
+    elseif (itime <= 10000) then
+      epsaw = 2.d-1
+    else
+      epsaw = 2.d-2
+    end if
+  elseif (ncase == 11) then
+    if (itime <= 7500) then
+      epsaw = 1.d-3
+    else
+      epsaw = 2.d-2
+    end if
+  end if
+end if
 unogd = unkgd
 end subroutine getsolvdg_lag

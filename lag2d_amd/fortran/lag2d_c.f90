@@ -75,6 +75,12 @@ module lag2d_c
       real(c_double), intent(out) :: dtfix, tacru
     end function lag2d_get_time
 
+    integer(c_int) function lag2d_get_mass_matrix(ctx, amatr) bind(c, name='lag2d_get_mass_matrix')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: amatr(*)
+    end function lag2d_get_mass_matrix
+
     integer(c_int) function lag2d_check(ctx) bind(c, name='lag2d_check')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

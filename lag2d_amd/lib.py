@@ -18,7 +18,7 @@ LIBPATH = os.path.join(HERE, "liblag2d.so")
 EXPORTS = ("lag2d_create", "lag2d_create_hybrid", "lag2d_create_ensemble", "lag2d_get_run_times", "lag2d_set_initial_density", "lag2d_weno_cells",
            "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
-           "lag2d_division_check", "lag2d_destroy", "lag2d_last_error")
+           "lag2d_division_check", "lag2d_get_mass_matrix", "lag2d_destroy", "lag2d_last_error")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -64,6 +64,7 @@ def load():
     lib.lag2d_rhs.argtypes = [C.c_void_p, _dp]
     lib.lag2d_download.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.lag2d_get_time.argtypes = [C.c_void_p, _dp, _dp]
+    lib.lag2d_get_mass_matrix.argtypes = [C.c_void_p, _dp]
     lib.lag2d_check.argtypes = [C.c_void_p]
     lib.lag2d_sync.argtypes = [C.c_void_p]
     lib.lag2d_kernel_time.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_char_p)]
@@ -155,6 +156,11 @@ class Lag2D:
         ustar = np.zeros((c.npoin, 2))
         _check(load().lag2d_download(self.h, _p(coord), _p(geoel), _p(unkno), _p(ustar)))
         return {"coord": coord, "geoel": geoel, "unkno": unkno, "ustar": ustar}
+
+    def mass_matrix(self):
+        a = np.zeros((self.case.ncell, self.case.nmatr))
+        _check(load().lag2d_get_mass_matrix(self.h, _p(a)))
+        return a
 
     def time(self):
         dt, t = C.c_double(), C.c_double()

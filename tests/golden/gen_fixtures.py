@@ -53,6 +53,10 @@ def _sedov_hybrid():
     return mg.sedov_hybrid_problem(16), "sedov_origin"
 
 
+def _sod_rz_hybrid():
+    return mg.sod_rz_hybrid_problem(40, 8), None
+
+
 def _sedov_rz_hybrid():
     return mg.sedov_rz_hybrid_problem(16), "sedov_origin"
 
@@ -66,6 +70,7 @@ SPECS = {
     "sod_hyb_40x8": (_sod_hybrid, 200, 50),
     "sedov_hyb_16x16": (_sedov_hybrid, 60, 20),
     "sedov_rz_16x16": (_sedov_rz_hybrid, 40, 10),
+    "sod_rz_40x8": (_sod_rz_hybrid, 120, 30),
 }
 
 

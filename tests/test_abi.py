@@ -49,6 +49,36 @@ def test_unsupported_config_is_rejected_without_device():
     assert b"ndens" in L.lag2d_last_error()
 
 
+@pytest.mark.parametrize("change,needle", [
+    ({"nrz": 1}, b"nrz=0 or 2"),
+    ({"nrz": 2, "ncurv": 1, "nfint": 4, "npoly": 2, "ndegr": 6}, b"nrz=2: straight P1"),
+    ({"nrz": 2, "ncase": 12}, b"ncase"),
+    ({"nrz": 2, "code": 25}, b"slip walls"),
+])
+def test_unsupported_rz_config_is_rejected_without_device(change, needle):
+    """The RZ path takes straight P1 cells with slip walls; other RZ
+    switches are refused before the device is touched."""
+    import ctypes
+    import numpy as np
+    from lag2d_amd import lib
+    L = lib.load()
+    kw = dict(ndegr=3, npoly=1, ncurv=0, nstag=2, ndens=1, nlimi=6, ncase=7, nrz=0, nfint=1,
+              nriem=1, ndt=1, gamlg=1.4, cfl=0.3, cimpd=1.0, cdrho=0.0, tend=0.2)
+    code = change.pop("code", 22)
+    kw.update(change)
+    cfg = lib.Config(**kw)
+    ipq = np.array([1, 2, 3, 4], dtype=np.int32)
+    bf = np.array([1, 2, code, 221, 0], dtype=np.int32)
+    e1 = np.array([1, 1, 1, 1], dtype=np.int32)
+    e2 = np.array([0, 1, 2, 3, 4], dtype=np.int32)
+    h = ctypes.c_void_p()
+    P = ctypes.POINTER(ctypes.c_int)
+    rc = L.lag2d_create(ctypes.byref(cfg), 1, 4, 1, ipq.ctypes.data_as(P), bf.ctypes.data_as(P),
+                        e1.ctypes.data_as(P), e2.ctypes.data_as(P), ctypes.byref(h))
+    assert rc == 2
+    assert needle in L.lag2d_last_error()
+
+
 def test_fortran_interface_compiles(tmp_path):
     """The ISO_C_BINDING module (lag2d_amd/fortran/lag2d_c.f90) compiles on
     its own: it depends on nothing of the reference."""

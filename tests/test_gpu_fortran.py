@@ -32,9 +32,10 @@ def _run(exe, work):
         return f.read()
 
 
-@pytest.mark.parametrize("hybrid", [False, True], ids=["quads", "hybrid"])
-def test_reference_driver_with_gpu_step(tmp_path, hybrid):
-    mesh, ctl = mg.sod_hybrid_problem(40, 8) if hybrid else mg.sod_problem(100, 4)
+@pytest.mark.parametrize("kind", ["quads", "hybrid", "rz"])
+def test_reference_driver_with_gpu_step(tmp_path, kind):
+    mesh, ctl = {"quads": lambda: mg.sod_problem(100, 4), "hybrid": lambda: mg.sod_hybrid_problem(40, 8),
+                 "rz": lambda: mg.sod_rz_hybrid_problem(40, 8)}[kind]()
     ctl = dataclasses.replace(ctl, ntime=40, ndump=1000000)  # 40 SSP-RK2 steps, t_end not reached
     for sub in ("cpu", "gpu"):
         mg.write_case(str(tmp_path / sub), mesh, ctl)

@@ -7,7 +7,7 @@ from fixtures import load, state_errors
 from oracle import Oracle
 
 CASES = ["sod_q1_100x4", "sedov_q1_16x16", "sedov_q2_12x12", "sedov_q2c_16x16", "sod_hyb_40x8", "sedov_hyb_16x16",
-         "sedov_rz_16x16"]
+         "sedov_rz_16x16", "sod_rz_40x8"]
 
 
 @pytest.mark.parametrize("name", CASES)
