@@ -93,9 +93,11 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         // pressure boundaries (GetBClag.f90:84-165); -7 overrides the limiter
         if (c.ncase == 1 || c.ncase == 2 || c.ncase == 3 || c.ncase == 4 || c.ncase == 12 || c.ncase == -7)
             return fail(LAG2D_EUNSUPPORTED, "nrz=2: ncase 1, 2, 3, 4, 12, -7 are not implemented");
+        // slip walls (22) and free boundaries (21: no force unless ncase is
+        // 3 or 4, GetBClag.f90:112-127)
         for (int f = 0; f < nbfac; ++f)
-            if (bface[5 * (size_t)f + 2] != 22)
-                return fail(LAG2D_EUNSUPPORTED, "nrz=2: slip walls (boundary code 22) only");
+            if (bface[5 * (size_t)f + 2] != 22 && bface[5 * (size_t)f + 2] != 21)
+                return fail(LAG2D_EUNSUPPORTED, "nrz=2: slip walls (code 22) and free boundaries (21) only");
     }
     if (c.nstag != 2 && c.nstag != 3) return fail(LAG2D_EUNSUPPORTED, "nstag must be 2 or 3");
     if (ntria > 0 && !q1) return fail(LAG2D_EUNSUPPORTED, "triangles: straight P1 path only (ncurv=0, nfint=1)");

@@ -157,6 +157,159 @@ def initialise(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) -> La
                    ustar=np.zeros_like(coord), rho0=rho.copy())
 
 
+# rutope 7-point rule of the reference triangle r, s in [0, 1] (dgflo2d.f90:2234-2255)
+_T7 = np.array([[0.333333333333333, 0.333333333333333], [0.059715871789770, 0.470142064105115],
+                [0.470142064105115, 0.059715871789770], [0.470142064105115, 0.470142064105115],
+                [0.797426985353087, 0.101286507323456], [0.101286507323456, 0.797426985353087],
+                [0.101286507323456, 0.101286507323456]])
+_T7W = np.array([0.225000000000000, 0.132394152788506, 0.132394152788506, 0.132394152788506,
+                 0.125939180544827, 0.125939180544827, 0.125939180544827])
+
+
+def _shape6(r, s):
+    """Quadratic-triangle shape functions and derivatives (nodes 1-3, then
+    mid-nodes 12, 23, 13; GetInitial.f90:1384-1405); arrays (npt, 6)."""
+    t = 1.0 - r - s
+    sh = np.stack([-t * (1 - 2 * t), -r * (1 - 2 * r), -s * (1 - 2 * s), 4 * r * t, 4 * r * s, 4 * s * t], axis=-1)
+    z = np.zeros_like(r)
+    dr = np.stack([1 - 4 * t, -1 + 4 * r, z, 4 * (1 - 2 * r - s), 4 * s, -4 * s], axis=-1)
+    ds = np.stack([1 - 4 * t, z, -1 + 4 * s, -4 * r, 4 * r, 4 * (1 - r - 2 * s)], axis=-1)
+    return sh, dr, ds
+
+
+def build_esuv_hybrid(iptri0: np.ndarray, ipqua0: np.ndarray, npoin: int):
+    """Elements-surrounding-points CSR with triangles numbered first
+    (element e < ntria) and quads after them, each node listing its elements
+    in increasing id (Preprocess.f90:720-766)."""
+    nt = iptri0.shape[0]
+    elem = np.concatenate([np.repeat(np.arange(nt), 3), nt + np.repeat(np.arange(ipqua0.shape[0]), 4)])
+    nodes = np.concatenate([iptri0.ravel(), ipqua0.ravel()])
+    counts = np.bincount(nodes, minlength=npoin)
+    esuv2 = np.zeros(npoin + 1, dtype=np.int32)
+    np.cumsum(counts, out=esuv2[1:])
+    order = np.lexsort((elem, nodes))
+    return elem[order].astype(np.int32), esuv2
+
+
+def _cell_integrals(xp, pts, w, shape, half, density, alfrz):
+    """Per-cell sums over a quadrature rule on the quadratic map of the
+    geometry points xp (nc, nn, 2): weighted areas djaco (nc, ng), radius
+    factors rcoef and densities at the points."""
+    r, s = pts[:, 0], pts[:, 1]
+    sh, dr, ds = shape(r, s)
+    x = np.einsum("gk,ekd->egd", sh, xp)
+    dxdr = np.einsum("gk,ek->eg", dr, xp[..., 0])
+    dxds = np.einsum("gk,ek->eg", ds, xp[..., 0])
+    dydr = np.einsum("gk,ek->eg", dr, xp[..., 1])
+    dyds = np.einsum("gk,ek->eg", ds, xp[..., 1])
+    dj = (0.5 if half else 1.0) * w * (dxdr * dyds - dydr * dxds)
+    rcoef = 1.0 - alfrz + alfrz * x[..., 1]
+    rho = np.asarray(density(x[..., 0], x[..., 1]), dtype=np.float64) * np.ones_like(dj)
+    return dj, rcoef, rho
+
+
+def initialise_p1(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) -> LagCase:
+    """Initial arrays of the straight P1 path on a hybrid mesh (triangles
+    first), planar (nrz=0) or area-weighted RZ (nrz=2): getgeoel_initial_lag
+    (GetInitial.f90:1278-1697) for the mass centre, volume, mass and, for
+    RZ, the area mass, mean radius geoel(11) and area geoel(12);
+    getamatr_initial_lag (:697-1274), RZ divided by geoel(11) (:420, :675,
+    :996, :1270).  ``density(x, y)`` is evaluated at the quadrature points
+    (getrhog_initial); ``state(xc, yc)`` gives (u, v, p) per cell from its
+    centroid.  ``blast`` = (cells, energy) deposits total energy ``energy``
+    spread over the listed cells by mass."""
+    if ctl.ncurv != 0 or ctl.npoly != 1:
+        raise ValueError("straight P1 meshes only")
+    rz = ctl.nrz == 2
+    alfrz = 1.0 if ctl.nrz in (1, 2) else 0.0
+    ipt = (mesh.iptri - 1).astype(np.int32) if mesh.ntria else np.zeros((0, 3), dtype=np.int32)
+    ipq, bf, _, _ = mesh_arrays(mesh)
+    e1, e2 = build_esuv_hybrid(ipt, ipq, mesh.npoin)
+    coord = mesh.coord.astype(np.float64)
+    nt, nq = ipt.shape[0], ipq.shape[0]
+    nc = nt + nq
+    geoel = np.zeros((nc, 24))
+    amatr = np.zeros((nc, 4))
+    tau = np.zeros(nc)
+    xcen = np.zeros((nc, 2))
+    blocks = []
+    if nt:
+        c = coord[ipt]
+        xp = np.concatenate([c, 0.5 * (c[:, 0:1] + c[:, 1:2]), 0.5 * (c[:, 1:2] + c[:, 2:3]),
+                             0.5 * (c[:, 0:1] + c[:, 2:3])], axis=1)
+        blocks.append((slice(0, nt), xp, (_T7, _T7W), (_T7, _T7W), _shape6, True, 0.5))
+    if nq:
+        xp = geometry_points(coord, ipq, 0)
+        r6, s6, w6 = _rule(_G6, _W6)
+        r5, s5, w5 = _rule(_G5, _W5)
+        blocks.append((slice(nt, nc), xp, (np.stack([r6, s6], 1), w6), (np.stack([r5, s5], 1), w5),
+                       lambda r, s: _shape9(r, s), False, 1.0))
+    for sl, xp, geo, mas, shape, half, d in blocks:
+        dj, rcoef, rho = _cell_integrals(xp, geo[0], geo[1], shape, half, density, alfrz)
+        r, s = geo[0][:, 0], geo[0][:, 1]
+        vol = (dj * rcoef).sum(axis=1)
+        if rz:
+            mass = (rho * dj).sum(axis=1)
+            area = dj.sum(axis=1)
+            rc = (rho * r * dj).sum(axis=1) / mass
+            sc = (rho * s * dj).sum(axis=1) / mass
+            geoel[sl, 6] = (dj * r).sum(axis=1) / area
+            geoel[sl, 7] = (dj * s).sum(axis=1) / area
+            geoel[sl, 10] = vol / area
+            geoel[sl, 11] = area
+        else:
+            mass = (rho * dj * rcoef).sum(axis=1)
+            rc = (rho * r * dj * rcoef).sum(axis=1) / mass
+            sc = (rho * s * dj * rcoef).sum(axis=1) / mass
+            geoel[sl, 6] = (dj * r * rcoef).sum(axis=1) / vol
+            geoel[sl, 7] = (dj * s * rcoef).sum(axis=1) / vol
+        geoel[sl, 0], geoel[sl, 1], geoel[sl, 2], geoel[sl, 3] = rc, sc, vol, mass
+        # inverse mass matrix of the slopes (area-weighted for RZ)
+        dj5, rcoef5, rho5 = _cell_integrals(xp, mas[0], mas[1], shape, half, density, alfrz)
+        wgt = rho5 * dj5 * (1.0 if rz else rcoef5)
+        b2 = (mas[0][None, :, 0] - rc[:, None]) / d
+        b3 = (mas[0][None, :, 1] - sc[:, None]) / d
+        f0 = wgt.sum(axis=1)
+        f1 = (wgt * b2 * b2).sum(axis=1)
+        f2 = (wgt * b2 * b3).sum(axis=1)
+        f3 = (wgt * b3 * b3).sum(axis=1)
+        det = f1 * f3 - f2 * f2
+        am = np.stack([f3 / det, -f2 / det, f1 / det, 1.0 / f0], axis=1)
+        if rz:
+            am = am / geoel[sl, 10:11]
+        amatr[sl] = am
+        tau[sl] = (dj5 * (1.0 if rz else rcoef5)).sum(axis=1) / f0
+        xcen[sl] = xp[:, :3 if half else 4].mean(axis=1)
+    g1 = ctl.gamlg - 1.0
+    u0, v0, p0 = (np.asarray(a, dtype=np.float64) * np.ones(nc) for a in state(xcen[:, 0], xcen[:, 1]))
+    unkno = np.zeros((nc, 4, ctl.ndegr))
+    unkno[:, 0, 0] = tau
+    unkno[:, 1, 0] = u0
+    unkno[:, 2, 0] = v0
+    unkno[:, 3, 0] = p0 * tau / g1 + 0.5 * (u0 * u0 + v0 * v0)
+    if blast is not None:
+        cells, energy = blast
+        cells = np.asarray(cells)
+        unkno[cells, 3, 0] = energy / geoel[cells, 3].sum()
+    return LagCase(ctl=ctl, ipqua=ipq, bface=bf, esuv1=e1, esuv2=e2, coord=coord.copy(), coold=coord.copy(),
+                   geoel=geoel, amatr=amatr, unkno=unkno, ustar=np.zeros_like(coord),
+                   iptri=ipt if nt else None, dtfix=ctl.dtfix)
+
+
+def rz_sedov_case(n=196, seed=2024, fraction=0.3, radius=0.02, energy=0.5) -> LagCase:
+    """Sedov blast in cylindrical coordinates on a hybrid quarter disk of
+    radius 1 (z = x, r = y): gamma = 5/3, rho = 1, p = 1e-6, total energy
+    ``energy`` in the cells whose centroid lies within ``radius`` of the
+    origin, area-weighted RZ scheme (nrz = 2), P1 / Barth limiter."""
+    mesh, ctl = mg.rz_quarter_disk_problem(n, seed=seed, fraction=fraction)
+    case = initialise_p1(mesh, ctl, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6))
+    cen = np.concatenate([case.coord[case.iptri].mean(axis=1) if case.ntria else np.zeros((0, 2)),
+                          case.coord[case.ipqua].mean(axis=1)])
+    cells = np.nonzero(np.hypot(cen[:, 0], cen[:, 1]) < radius)[0]
+    case.unkno[cells, 3, 0] = energy / case.geoel[cells, 3].sum()
+    return case
+
+
 def sedov_case(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None) -> LagCase:
     """Planar Sedov blast on a jittered curved Q2 mesh, P2: rho = 1, u = 0,
     p = 1e-6, blast energy 0.244816 in the corner cell (or in each of the

@@ -285,6 +285,45 @@ def sedov_rz_hybrid_problem(nx=16, ny=None, fraction=0.3, seed=13, length=1.2):
     return mesh, Control(cfl=0.25, tend=1.0, ncase=7, nrz=2, dtfix=1e-6)
 
 
+def quarter_disk_q1(n) -> Mesh:
+    """Straight quads on the quarter disk x, y >= 0, x^2 + y^2 <= 1: the
+    n x n lattice of the unit square mapped by the concentric map (the
+    lattice squares of side k/n go to the circles of radius k/n, the angle
+    growing linearly along each), which keeps every cell convex and of
+    bounded aspect ratio, the origin included.  Boundaries: y = 0 (axis,
+    slip wall 221), x = 0 (symmetry, slip wall 222), the arc free (code 21)."""
+    u = np.linspace(0.0, 1.0, n + 1)
+    U, V = np.meshgrid(u, u, indexing="xy")
+    R = np.maximum(U, V)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        th = np.where(U >= V, 0.25 * np.pi * np.where(U > 0, V / U, 0.0),
+                      0.5 * np.pi - 0.25 * np.pi * np.where(V > 0, U / V, 0.0))
+    X, Y = R * np.cos(th), R * np.sin(th)
+    X[:, 0] = 0.0   # exactly on the symmetry line and the axis
+    Y[0, :] = 0.0
+    coord = np.stack([X.ravel(), Y.ravel()], axis=1)
+
+    def nid(I, J):
+        return I + (n + 1) * J + 1
+
+    ipq = np.empty((n * n, 4), dtype=np.int32)
+    for j in range(n):
+        for i in range(n):
+            ipq[i + n * j] = (nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1))
+    f = _boundary_faces(nid, n, n, False)
+    f[n:3 * n, 2] = 21   # right and top edges of the square: the arc
+    f[n:3 * n, 3] = 0
+    return Mesh(coord, ipq, f, 0, n, n)
+
+
+def rz_quarter_disk_problem(n=196, seed=2024, fraction=0.3):
+    """The RZ Sedov configuration: hybrid quarter disk (a seeded share of
+    the quads split along a seeded diagonal), gamma = 5/3, CFL 0.25, area-
+    weighted RZ scheme, t_end 0.8."""
+    mesh = split_quads(quarter_disk_q1(n), fraction, seed)
+    return mesh, Control(cfl=0.25, tend=0.8, gamlg=5.0 / 3.0, ncase=7, nrz=2, dtfix=1e-6)
+
+
 def sedov_problem(nx=120, ny=None, seed=2024, length=1.2):
     ny = nx if ny is None else ny
     mesh = quad_q2(nx, ny, 0.0, length, 0.0, length * ny / nx, seed=seed)

@@ -1891,8 +1891,10 @@ static int rhs_rz(orc_case *c)
      * 84-110); 3, 4: pressure boundaries; -7: limiter overrides */
     if (c->ncase == 1 || c->ncase == 2 || c->ncase == 3 || c->ncase == 4 || c->ncase == 12 || c->ncase == -7)
         return 2;
+    /* slip walls (22) and free boundaries (21, which adds no force unless
+     * ncase is 3 or 4, GetBClag.f90:112-127) */
     for (int f = 0; f < c->nbfac; ++f)
-        if (c->bface[5 * f + 2] != 22) return 2;
+        if (c->bface[5 * f + 2] != 22 && c->bface[5 * f + 2] != 21) return 2;
     const int nd = c->ndegr, nq = c->nq, nt = nd * nq * c->ncell;
     double *gq = (double *)malloc(sizeof(double) * 12 * (c->nquad + 1));
     double *gt = (double *)malloc(sizeof(double) * 9 * (c->ntria + 1));
