@@ -41,7 +41,7 @@ typedef struct lag2d_config {
     int ndens;    /* 1: specific volume evolved as a DG variable */
     int nlimi;    /* 6 */
     int ncase;    /* test-problem id (only boundary-colouring rules read it) */
-    int nrz;      /* 0 planar */
+    int nrz;      /* 0 planar, 2 area-weighted RZ (r = y; straight P1 only) */
     int nfint;    /* 1 (ncurv=0) or 4 (ncurv=1 sub-cell scheme) */
     int nriem;    /* 1 */
     int ndt;      /* 1: CFL-controlled dt, 0: fixed dtfix */
@@ -72,7 +72,11 @@ int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
  * limiter, Riemann coefficients, consistent-weight face integral and
  * 7-point domain integral of Getrhslagmctria.f90 / Getrhslagmc.f90:14006-14160).
  * The problems ncase = 1, 2, 3, -7, whose triangle limiter overrides
- * boundary nodes (Getrhslagmctria.f90:539-582), return LAG2D_EUNSUPPORTED. */
+ * boundary nodes (Getrhslagmctria.f90:539-582), return LAG2D_EUNSUPPORTED.
+ * With cfg->nrz = 2 the cells take the area-weighted RZ routines instead
+ * (Getrhslagrz.f90, Getrhslagmc.f90:6876-6971, :7299-7392); lag2d_create
+ * accepts nrz = 2 on quad meshes as well.  RZ boundaries are slip walls
+ * (code 22) or free (21). */
 int lag2d_create_hybrid(const lag2d_config *cfg, int ntria, int nquad, int npoin, int nbfac,
                         const int *iptri, const int *ipqua, const int *bface, const int *esuv1,
                         const int *esuv2, lag2d_ctx **ctx);
