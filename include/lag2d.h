@@ -189,6 +189,33 @@ int lag2d_weno_cells(lag2d_ctx *ctx, long long *total);
  * the reference. */
 int lag2d_division_check(int n, const double *a, const double *b, double *q);
 
+/* Discrete adjoint of the step (straight P1 planar path: ncurv=0, nrz=0,
+ * quads, SSP-RK2).  No counterpart in the reference.  The limiter
+ * coefficients, the impedance's |du.n| (the shock switch of the corner
+ * Riemann coefficients, Limiterlag.f90:987-996) and the time step are
+ * frozen at their forward values, so the gradient is that of the map the
+ * recorded forward computes with them held fixed.
+ *
+ * lag2d_adj_record: nsteps plain steps (itime0, itime0+1, ...) from the
+ * resident state, as lag2d_step, recording every stage's limiter
+ * coefficients and every step's dt, and checkpointing U, x and u* every
+ * `every` steps.  Device memory: 34*ncell doubles per step plus a checkpoint
+ * of 12*ncell + 4*npoin doubles per `every` steps plus the stage data of
+ * `every` steps (104*ncell + 10*npoin doubles each).
+ *
+ * lag2d_adj_replay: nsteps steps from the resident state with the recorded
+ * limiter coefficients and dt (upload a perturbed initial state first): the
+ * frozen forward map whose derivative the adjoint computes.
+ *
+ * lag2d_adj_gradient: given lam_coord(2,npoin) = dJ/dx at the end of the
+ * recorded steps, returns grad(ndegr,4,ncell) = dJ/dunkno at their start.
+ * Each checkpointed segment is recomputed forward (stage data kept), then
+ * reversed step by step.  Leaves the resident state unspecified: upload
+ * before stepping again. */
+int lag2d_adj_record(lag2d_ctx *ctx, int itime0, int nsteps, int every);
+int lag2d_adj_replay(lag2d_ctx *ctx, int nsteps);
+int lag2d_adj_gradient(lag2d_ctx *ctx, const double *lam_coord, double *grad);
+
 void lag2d_destroy(lag2d_ctx *ctx);
 const char *lag2d_last_error(void);
 

@@ -447,7 +447,8 @@ const char *const lag2d_kernel_names[KID_COUNT] = {
     "k_cellavg_q1", "k_nodeminmax", "k_limit_corner_q1", "k_node_q1", "k_cellrhs_q1", "k_dt_partial",
     "k_dt_final",   "k_update",     "k_nodemove",        "k_cellvol",
     "k_q2_prep",    "k_q2_weno",    "k_q2_corner",       "k_node_q2", "k_cellrhs_q2", "k_run_flags",
-    "k_limit_corner_rz", "k_node_rz", "k_cellrhs_rz",    "k_schedule"};
+    "k_limit_corner_rz", "k_node_rz", "k_cellrhs_rz",    "k_schedule",
+    "k_adj_cellrhs", "k_adj_corner", "k_adj_node+gather+nodemove", "k_limit_corner_q1<frozen>"};
 
 static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
 
@@ -514,7 +515,7 @@ static int launch_rhs(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int
     } else if (x->cfg.ncurv == 0) {
         LAUNCH(KID_CELLAVG_Q1, k_cellavg_q1, nblocks(nc), ds, Uin);
         LAUNCH(KID_NODEMINMAX, k_nodeminmax, nblocks(np), ds);
-        LAUNCH(KID_LIMIT_Q1, k_limit_corner_q1, nblocks(nc), ds, Uin);
+        LAUNCH(KID_LIMIT_Q1, k_limit_corner_q1<0>, nblocks(nc), ds, Uin);
         LAUNCH(KID_NODE_Q1, k_node_q1, nblocks(np), ds);
         LAUNCH(KID_CELLRHS_Q1, k_cellrhs_q1, nblocks(nc), ds, Uin, alfa, apply_update);
     } else {
@@ -559,6 +560,239 @@ static int step_impl(lag2d_ctx *x, int itime)
     std::swap(d.Un, d.Us);
     std::swap(d.xn, d.xs);
     std::swap(d.yn, d.ys);
+    return 0;
+}
+
+// ---- discrete adjoint (straight P1 planar quads, SSP-RK2)
+
+enum { ADJ_RECORD = 0, ADJ_REPLAY = 1, ADJ_SEGMENT = 2 };
+
+// Segment slot layout: U^n, U^(1) [12 nc each]; x^n, x^(1), u*_prev, u*_0,
+// u*_1 [2 np each]; corner records of both stages [40 nc each].
+static size_t seg_stride(const lag2d_ctx *x) { return (size_t)104 * x->nc + (size_t)10 * x->np; }
+
+// One step of the recorded forward (RECORD: the plain step, whose limiter
+// coefficients and dt are written to the tape), of the frozen replay
+// (REPLAY: limiter and dt read from the tape), or of a recomputed segment
+// (SEGMENT: as REPLAY, keeping the stage data of slot k for the reverse).
+static int adj_fwd_step(lag2d_ctx *x, int itime, int n, int mode, int k)
+{
+    Dev &d = x->d;
+    lag2d_adj &A = x->adj;
+    const int nc = x->nc, np = x->np;
+    const size_t nU = (size_t)12 * nc, lim = (size_t)17 * nc;
+    double *slot = A.seg + (size_t)k * A.seg_stride;
+    auto cp = [&](double *dst, const double *src, size_t n) {
+        return hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, x->stream);
+    };
+    for (int s = 0; s < 2; ++s) {
+        const double alfa = RKLAG[0][s];
+        Dev ds = d;
+        if (s == 0) {
+            ds.xs = d.xn;
+            ds.ys = d.yn;
+        }
+        const double *Uin = (s == 0) ? d.Un : d.Us;
+        if (mode == ADJ_SEGMENT) {
+            if (s == 0) {
+                HIPCHK(cp(slot, d.Un, nU));
+                HIPCHK(cp(slot + 2 * nU, d.xn, np));
+                HIPCHK(cp(slot + 2 * nU + np, d.yn, np));
+                HIPCHK(cp(slot + 2 * nU + 4 * (size_t)np, d.usx, np));
+                HIPCHK(cp(slot + 2 * nU + 5 * (size_t)np, d.usy, np));
+            } else {
+                HIPCHK(cp(slot + nU, d.Us, nU));
+                HIPCHK(cp(slot + 2 * nU + 2 * (size_t)np, d.xs, np));
+                HIPCHK(cp(slot + 2 * nU + 3 * (size_t)np, d.ys, np));
+            }
+        }
+        // the stage's limiter coefficients and impedance |du.n| live in the
+        // tape: written by the recorded pass, read by the frozen ones
+        ds.lim = A.lim_tape + ((size_t)n * 2 + s) * lim;
+        ds.dlt = ds.lim + 9 * (size_t)nc;
+        if (mode == ADJ_RECORD) {
+            LAUNCH(KID_CELLAVG_Q1, k_cellavg_q1, nblocks(nc), ds, Uin);
+            LAUNCH(KID_NODEMINMAX, k_nodeminmax, nblocks(np), ds);
+            LAUNCH(KID_LIMIT_Q1, k_limit_corner_q1<0>, nblocks(nc), ds, Uin);
+        } else {
+            LAUNCH(KID_LIMIT_FROZEN, k_limit_corner_q1<1>, nblocks(nc), ds, Uin);
+        }
+        if (mode == ADJ_SEGMENT) HIPCHK(cp(slot + 2 * nU + 10 * (size_t)np + (size_t)s * 40 * nc, d.rec, (size_t)40 * nc));
+        LAUNCH(KID_NODE_Q1, k_node_q1, nblocks(np), ds);
+        if (mode == ADJ_SEGMENT) {
+            HIPCHK(cp(slot + 2 * nU + (6 + 2 * s) * (size_t)np, d.usx, np));
+            HIPCHK(cp(slot + 2 * nU + (7 + 2 * s) * (size_t)np, d.usy, np));
+        }
+        LAUNCH(KID_CELLRHS_Q1, k_cellrhs_q1, nblocks(nc), ds, Uin, alfa, s > 0);
+        if (s == 0) {
+            if (mode == ADJ_RECORD) {
+                LAUNCH(KID_DT_PARTIAL, k_dt_partial, x->nblk_dt, ds, Uin, x->nblk_dt);
+                LAUNCH(KID_DT_FINAL, k_dt_final, 1, d, itime, x->nblk_dt);
+                HIPCHK(cp(A.dt_tape + n, d.scal, 1));
+            } else {
+                LAUNCH(KID_DT_FINAL, k_replay_dt, 1, d, A.dt_tape + n);
+            }
+            LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
+        }
+        LAUNCH(KID_NODEMOVE, k_nodemove, nblocks(np), d, alfa, s == 0 ? d.xn : d.xs, s == 0 ? d.yn : d.ys);
+    }
+    if (mode == ADJ_RECORD) launch_cellvol(x, d.xs, d.ys, 0, true);
+    HIPCHK(hipGetLastError());
+    std::swap(d.Un, d.Us);
+    std::swap(d.xn, d.xs);
+    std::swap(d.yn, d.ys);
+    return 0;
+}
+
+static bool adj_supported(const lag2d_ctx *x)
+{
+    const lag2d_config &c = x->cfg;
+    return c.ncurv == 0 && c.npoly == 1 && c.nrz == 0 && c.nstag == 2 && x->ntria == 0 && x->nrun == 1;
+}
+
+extern "C" int lag2d_adj_record(lag2d_ctx *x, int itime0, int nsteps, int every)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    if (!adj_supported(x))
+        return fail(LAG2D_EUNSUPPORTED, "the adjoint covers the straight P1 planar path on quads with SSP-RK2");
+    if (nsteps < 1 || every < 1 || itime0 < 1) return fail(LAG2D_EINVAL, "nsteps, every and itime0 must be >= 1");
+    const int nc = x->nc, np = x->np;
+    lag2d_adj &A = x->adj;
+    const int nckpt = (nsteps + every - 1) / every;
+    if (A.nsteps < nsteps || A.every < every || A.nckpt < nckpt) {
+        int rc;
+        if ((rc = x->alloc(&A.lim_tape, (size_t)nsteps * 34 * nc)) || (rc = x->alloc(&A.dt_tape, (size_t)nsteps)) ||
+            (rc = x->alloc(&A.ck, (size_t)nckpt * (12 * (size_t)nc + 4 * (size_t)np))))
+            return rc;
+        A.seg_stride = seg_stride(x);
+        if ((rc = x->alloc(&A.seg, (size_t)every * A.seg_stride)) ||
+            (rc = x->alloc(&A.buf, (size_t)(36 + 8 * 2 + 40) * nc + (size_t)13 * np)))
+            return rc;
+    }
+    A.nsteps = nsteps;
+    A.every = every;
+    A.nckpt = nckpt;
+    A.itime0 = itime0;
+    Dev &d = x->d;
+    const size_t ckw = 12 * (size_t)nc + 4 * (size_t)np;
+    for (int n = 0; n < nsteps; ++n) {
+        if (n % every == 0) {
+            double *c = A.ck + (size_t)(n / every) * ckw;
+            HIPCHK(hipMemcpyAsync(c, d.Un, 12 * (size_t)nc * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+            HIPCHK(hipMemcpyAsync(c + 12 * (size_t)nc, d.xn, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+            HIPCHK(hipMemcpyAsync(c + 12 * (size_t)nc + np, d.yn, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+            HIPCHK(hipMemcpyAsync(c + 12 * (size_t)nc + 2 * (size_t)np, d.usx, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                  x->stream));
+            HIPCHK(hipMemcpyAsync(c + 12 * (size_t)nc + 3 * (size_t)np, d.usy, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                  x->stream));
+        }
+        int rc = adj_fwd_step(x, itime0 + n, n, ADJ_RECORD, 0);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+extern "C" int lag2d_adj_replay(lag2d_ctx *x, int nsteps)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    if (x->adj.nsteps < 1) return fail(LAG2D_EINVAL, "nothing recorded (lag2d_adj_record)");
+    if (nsteps < 0 || nsteps > x->adj.nsteps) return fail(LAG2D_EINVAL, "more steps than recorded");
+    for (int n = 0; n < nsteps; ++n) {
+        int rc = adj_fwd_step(x, x->adj.itime0 + n, n, ADJ_REPLAY, 0);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+// One step reversed from the stage data of segment slot k.
+static int adj_rev_step(lag2d_ctx *x, int n, int k, AdjStage &a)
+{
+    Dev &d = x->d;
+    lag2d_adj &A = x->adj;
+    const int nc = x->nc, np = x->np;
+    const size_t nU = (size_t)12 * nc;
+    const double *slot = A.seg + (size_t)k * A.seg_stride;
+    const double *Xn = slot + 2 * nU, *X1 = Xn + 2 * (size_t)np, *usp = X1 + 2 * (size_t)np;
+    const double *us0 = usp + 2 * (size_t)np, *us1 = us0 + 2 * (size_t)np;
+    const double *rec0 = us1 + 2 * (size_t)np, *rec1 = rec0 + 40 * (size_t)nc;
+    a.dt = A.dt_tape + n;
+    for (int s = 1; s >= 0; --s) {
+        const double alfa = RKLAG[0][s];
+        a.Uin = s == 1 ? slot + nU : slot;
+        a.xin = s == 1 ? X1 : Xn;
+        a.yin = a.xin + np;
+        a.usx = s == 1 ? us1 : us0;
+        a.usy = a.usx + np;
+        a.rec = s == 1 ? rec1 : rec0;
+        a.lim = A.lim_tape + ((size_t)n * 2 + s) * 17 * nc;
+        a.dlt = a.lim + 9 * (size_t)nc;
+        LAUNCH(KID_ADJ_NODES, k_adj_nodemove, nblocks(np), d, a, alfa);
+        LAUNCH(KID_ADJ_CELLRHS, k_adj_cellrhs, nblocks(nc), d, a, alfa);
+        LAUNCH(KID_ADJ_NODES, k_adj_node, nblocks(np), d, a);
+        LAUNCH(KID_ADJ_CORNER, k_adj_corner, nblocks(nc), d, a);
+        LAUNCH(KID_ADJ_NODES, k_adj_gather, nblocks(np), d, a);
+        if (s == 1) std::swap(a.LU, a.LUI);
+    }
+    LAUNCH(KID_ADJ_NODES, k_adj_step_end, nblocks(std::max(12 * nc, 2 * np)), d, a);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int lag2d_adj_gradient(lag2d_ctx *x, const double *lam_coord, double *grad)
+{
+    if (!x || !lam_coord || !grad) return fail(LAG2D_EINVAL, "null argument");
+    lag2d_adj &A = x->adj;
+    if (A.nsteps < 1) return fail(LAG2D_EINVAL, "nothing recorded (lag2d_adj_record)");
+    Dev &d = x->d;
+    const int nc = x->nc, np = x->np;
+    const size_t nU = (size_t)12 * nc;
+    AdjStage a{};
+    double *b = A.buf;
+    a.LU = b; b += nU;
+    a.LUN = b; b += nU;
+    a.LUI = b; b += nU;
+    a.CX = b; b += 8 * (size_t)nc;
+    a.CU = b; b += 8 * (size_t)nc;
+    a.LREC = b; b += 40 * (size_t)nc;
+    a.LX = b; b += 2 * (size_t)np;
+    a.LXN = b; b += 2 * (size_t)np;
+    a.LXI = b; b += 2 * (size_t)np;
+    a.LUS = b; b += 2 * (size_t)np;
+    a.LUSN = b; b += 2 * (size_t)np;
+    a.NG = b; b += 3 * (size_t)np;
+    HIPCHK(hipMemsetAsync(A.buf, 0, (size_t)(b - A.buf) * sizeof(double), x->stream));
+    std::vector<double> h(2 * (size_t)np);
+    for (int p = 0; p < np; ++p) {
+        h[p] = lam_coord[2 * (size_t)p];
+        h[np + p] = lam_coord[2 * (size_t)p + 1];
+    }
+    HIPCHK(hipMemcpyAsync(a.LX, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    const size_t ckw = 12 * (size_t)nc + 4 * (size_t)np;
+    for (int sg = A.nckpt - 1; sg >= 0; --sg) {
+        const int n0 = sg * A.every, n1 = std::min(n0 + A.every, A.nsteps);
+        const double *c = A.ck + (size_t)sg * ckw;
+        HIPCHK(hipMemcpyAsync(d.Un, c, nU * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(d.xn, c + nU, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(d.yn, c + nU + np, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(d.usx, c + nU + 2 * (size_t)np, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(d.usy, c + nU + 3 * (size_t)np, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        for (int n = n0; n < n1; ++n) {
+            int rc = adj_fwd_step(x, A.itime0 + n, n, ADJ_SEGMENT, n - n0);
+            if (rc) return rc;
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            int rc = adj_rev_step(x, n, n - n0, a);
+            if (rc) return rc;
+        }
+    }
+    std::vector<double> g(nU);
+    HIPCHK(hipMemcpyAsync(g.data(), a.LU, nU * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int ie = 0; ie < nc; ++ie)
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < 3; ++id) grad[id + 3 * (q + 4 * (size_t)ie)] = g[((size_t)q * 3 + id) * nc + ie];
     return 0;
 }
 

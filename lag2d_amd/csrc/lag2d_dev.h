@@ -71,6 +71,8 @@ struct Dev {
     double *frame;           // [4][nc]  principal-strain frame (row-major 2x2)
     double *nmm;             // [10][np] node max (0..4) and min (5..9)
     double *lim;             // [9][nc]  aflim(1..5), afvec (col-major 2x2)
+    double *dlt;             // [8][nc]  |du.n| of each corner face's impedance, recorded for the
+                             //          adjoint's frozen forward (null otherwise; straight P1 quads)
     double *rec;             // corner-face records
     double *red;             // reduction scratch
     double *scal;            // device scalars: 0 dtfix, 1 tacru
@@ -153,6 +155,25 @@ __device__ __forceinline__ double qdiv_term(double a, const Recip &r, bool &ok)
     const double q0 = a * r.y;
     return __builtin_fma(__builtin_fma(-r.b, q0, a), r.y, q0);
 }
+
+// One stage of the discrete adjoint (straight P1 planar quads): the stage's
+// forward inputs and outputs, and the adjoint buffers.  Cell arrays are
+// [k*nc + ie], node arrays [c*np + p], corner arrays [(c*4 + iv)*nc + ie].
+struct AdjStage {
+    const double *Uin;           // stage input DOFs [12][nc]
+    const double *xin, *yin;     // stage input coordinates
+    const double *usx, *usy;     // this stage's nodal velocity
+    const double *rec;           // this stage's corner records [40][nc]
+    const double *lim;           // this stage's (frozen) limiter coefficients [9][nc]
+    const double *dlt;           // this stage's (frozen) impedance |du.n| per corner face [8][nc]
+    const double *dt;            // the step's dt (device scalar)
+    double *LU, *LUN, *LUI;      // lam of U_out, of U^n (accumulated), of U_in
+    double *LX, *LXN, *LXI;      // lam of x_out, of x^n (accumulated), direct part of x_in
+    double *LUS, *LUSN;          // lam of this stage's u*, of the next stage's u*_prev
+    double *CX, *CU;             // per-corner lam of x_in, u* [2][4][nc]
+    double *LREC;                // lam of the corner records [40][nc]
+    double *NG;                  // node coefficients of the reversed solve [3][np]
+};
 
 __device__ __forceinline__ int cell_run(const Dev &d, int ie) { return d.nrun > 1 ? d.runc[ie] : 0; }
 __device__ __forceinline__ int node_run(const Dev &d, int p) { return d.nrun > 1 ? d.runn[p] : 0; }

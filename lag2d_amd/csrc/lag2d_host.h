@@ -12,12 +12,24 @@ enum {
     KID_CELLAVG_Q1 = 0, KID_NODEMINMAX, KID_LIMIT_Q1, KID_NODE_Q1, KID_CELLRHS_Q1, KID_DT_PARTIAL,
     KID_DT_FINAL, KID_UPDATE, KID_NODEMOVE, KID_CELLVOL,
     KID_Q2_PREP, KID_Q2_WENO, KID_Q2_CORNER, KID_NODE_Q2, KID_CELLRHS_Q2, KID_RUN_FLAGS,
-    KID_LIMIT_RZ, KID_NODE_RZ, KID_CELLRHS_RZ, KID_SCHEDULE, KID_COUNT
+    KID_LIMIT_RZ, KID_NODE_RZ, KID_CELLRHS_RZ, KID_SCHEDULE,
+    KID_ADJ_CELLRHS, KID_ADJ_CORNER, KID_ADJ_NODES, KID_LIMIT_FROZEN, KID_COUNT
 };
 extern const char *const lag2d_kernel_names[KID_COUNT];
 
 struct lag2d_timer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+// Recording and buffers of the discrete adjoint (straight P1 planar quads).
+struct lag2d_adj {
+    int nsteps = 0, every = 0, itime0 = 0, nckpt = 0;
+    double *lim_tape = nullptr;  // [nsteps][2][9*nc] limiter coefficients of every stage
+    double *dt_tape = nullptr;   // [nsteps]
+    double *ck = nullptr;        // [nckpt][12*nc + 4*np] checkpoints: U, x, y, u*
+    double *seg = nullptr;       // [every][seg_stride] stage data of a recomputed segment
+    size_t seg_stride = 0;
+    double *buf = nullptr;       // adjoint buffers
 };
 
 struct lag2d_ctx {
@@ -34,6 +46,7 @@ struct lag2d_ctx {
     std::vector<double> tend_host;
     hipStream_t stream = nullptr;
     lag2d::Dev d{};
+    lag2d_adj adj;
     double *A = nullptr;    // inverse mass matrix (SoA)
     double *x0 = nullptr;   // initial coordinates [2][np]
     std::vector<void *> allocs;

@@ -7,6 +7,7 @@ namespace lag2d {
 // straight-quad P1 path
 __global__ void k_cellavg_q1(Dev d, const double *__restrict__ Uin);
 __global__ void k_nodeminmax(Dev d);
+template <int FROZEN>
 __global__ void k_limit_corner_q1(Dev d, const double *__restrict__ Uin);
 __global__ void k_node_q1(Dev d);
 __global__ void k_cellrhs_q1(Dev d, const double *__restrict__ Uin, double alfa, int apply_update);
@@ -26,6 +27,15 @@ __global__ void k_limit_corner_rz(Dev d, const double *__restrict__ Uin, int rev
 __global__ void k_node_rz(Dev d);
 __global__ void k_cellrhs_rz(Dev d, const double *__restrict__ Uin, double alfa, int apply_update, double epsaw);
 __global__ void k_schedule(Dev d, double dt);
+
+// discrete adjoint of the straight P1 planar step (quads)
+__global__ void k_adj_nodemove(Dev d, AdjStage a, double alfa);
+__global__ void k_adj_cellrhs(Dev d, AdjStage a, double alfa);
+__global__ void k_adj_node(Dev d, AdjStage a);
+__global__ void k_adj_corner(Dev d, AdjStage a);
+__global__ void k_adj_gather(Dev d, AdjStage a);
+__global__ void k_adj_step_end(Dev d, AdjStage a);
+__global__ void k_replay_dt(Dev d, const double *dt);
 constexpr int CV_CELLS = 32;  // cells per block of k_cellvol
 
 }  // namespace lag2d

@@ -397,12 +397,17 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_nodeminmax(Dev d)
 // one quad.  Records per corner face cf = f + 2*iv (face f=0 is the edge
 // entering vertex iv, f=1 the edge leaving it): m = mu*A (the isotropic
 // corner matrix), m*u_c (2), A*sigma*n (2).
+// With FROZEN the limiter coefficients and the impedance's |du.n| are not
+// computed: the kernel reads them from d.lim and d.dlt, where a recorded
+// pass left them (the adjoint's frozen forward; quads only).  Without
+// FROZEN, a non-null d.dlt records |du.n| of every corner face.
+template <int FROZEN>
 __global__ void __launch_bounds__(LAG2D_BLOCK) k_limit_corner_q1(Dev d, const double *__restrict__ Uin)
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc, nd = d.nd, np = d.np;
     if (ie >= nc) return;
-    if (ie < d.ntria) {
+    if (!FROZEN && ie < d.ntria) {
         limit_corner_tri(d, Uin, ie);
         return;
     }
@@ -431,9 +436,14 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_limit_corner_q1(Dev d, const do
     const double rhoct = 1.0 / tau;
     // barthlimit_lagsym_quad Parts 2-3 and barthlimit_sympre_quadlb Parts 4, 6
     double al[5], af[4];
-    barth_limit<4>(d, Uin, ie, ip, un, al, af);
-    for (int q = 0; q < 5; ++q) d.lim[q * nc + ie] = al[q];
-    for (int k = 0; k < 4; ++k) d.lim[(5 + k) * nc + ie] = af[k];
+    if (FROZEN) {
+        for (int q = 0; q < 5; ++q) al[q] = d.lim[q * nc + ie];
+        for (int k = 0; k < 4; ++k) af[k] = d.lim[(5 + k) * nc + ie];
+    } else {
+        barth_limit<4>(d, Uin, ie, ip, un, al, af);
+        for (int q = 0; q < 5; ++q) d.lim[q * nc + ie] = al[q];
+        for (int k = 0; k < 4; ++k) d.lim[(5 + k) * nc + ie] = af[k];
+    }
 
     // Corner Riemann coefficients (getriem_quad_matrixsym, ndens=1, nlimi=6)
     const double eps = 1.0e-6;
@@ -466,7 +476,13 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_limit_corner_q1(Dev d, const do
             double nx = gnx[e], ny = gny[e];
             // corner area of the face, planar rcoef = 1: (2*1 + 1)/3*0.5*L
             double area = (2.0 * 1.0 + 1.0) / 3.0 * 0.5 * glen[e];
-            double deltu = fabs(dux * nx + duy * ny);
+            double deltu;
+            if (FROZEN) {
+                deltu = d.dlt[(f + 2 * iv) * nc + ie];
+            } else {
+                deltu = fabs(dux * nx + duy * ny);
+                if (d.dlt) d.dlt[(f + 2 * iv) * nc + ie] = deltu;
+            }
             double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
             double m11;
             if (a3 < 1.e12) {
@@ -686,5 +702,8 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_cellrhs_q1(Dev d, const double 
 }
 
 #undef UI
+
+template __global__ void k_limit_corner_q1<0>(Dev, const double *__restrict__);
+template __global__ void k_limit_corner_q1<1>(Dev, const double *__restrict__);
 
 }  // namespace lag2d

@@ -81,6 +81,25 @@ module lag2d_c
       real(c_double), intent(out) :: amatr(*)
     end function lag2d_get_mass_matrix
 
+    integer(c_int) function lag2d_adj_record(ctx, itime0, nsteps, every) bind(c, name='lag2d_adj_record')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: itime0, nsteps, every
+    end function lag2d_adj_record
+
+    integer(c_int) function lag2d_adj_replay(ctx, nsteps) bind(c, name='lag2d_adj_replay')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+    end function lag2d_adj_replay
+
+    integer(c_int) function lag2d_adj_gradient(ctx, lam_coord, grad) bind(c, name='lag2d_adj_gradient')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: lam_coord(*)
+      real(c_double), intent(out) :: grad(*)
+    end function lag2d_adj_gradient
+
     integer(c_int) function lag2d_check(ctx) bind(c, name='lag2d_check')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

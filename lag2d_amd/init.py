@@ -310,6 +310,28 @@ def rz_sedov_case(n=196, seed=2024, fraction=0.3, radius=0.02, energy=0.5) -> La
     return case
 
 
+def sod_square_case(n=200, seed=2024, bump=None) -> LagCase:
+    """Sod-type tube on the unit square, P1 on n x n straight quads with
+    vertices jittered by up to 0.1h (seed), SSP-RK2, CFL 0.3, slip walls:
+    (rho, u, v, p) = (1, 0, 0, 1) for x < 0.5, (0.125, 0, 0, 0.1) beyond.
+    ``bump`` = (amplitude, x0, y0, width) multiplies the initial density by
+    1 + amplitude exp(-|x - (x0, y0)|^2 / width^2)."""
+    mesh = mg.quad_q1(n, n, 0.0, 1.0, 0.0, 1.0, seed=seed)
+    ctl = mg.Control(cfl=0.3, tend=0.2)
+
+    def rho(x, y):
+        r = np.where(x < 0.5, 1.0, 0.125)
+        if bump is not None:
+            a, x0, y0, w = bump
+            r = r * (1.0 + a * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (w * w)))
+        return r
+
+    def st(x, y):
+        return 0.0, 0.0, np.where(x < 0.5, 1.0, 0.1)
+
+    return initialise_p1(mesh, ctl, rho, st)
+
+
 def sedov_case(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None) -> LagCase:
     """Planar Sedov blast on a jittered curved Q2 mesh, P2: rho = 1, u = 0,
     p = 1e-6, blast energy 0.244816 in the corner cell (or in each of the

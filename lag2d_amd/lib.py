@@ -18,7 +18,8 @@ LIBPATH = os.path.join(HERE, "liblag2d.so")
 EXPORTS = ("lag2d_create", "lag2d_create_hybrid", "lag2d_create_ensemble", "lag2d_get_run_times", "lag2d_set_initial_density", "lag2d_weno_cells",
            "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
-           "lag2d_division_check", "lag2d_get_mass_matrix", "lag2d_destroy", "lag2d_last_error")
+           "lag2d_division_check", "lag2d_get_mass_matrix", "lag2d_adj_record", "lag2d_adj_replay",
+           "lag2d_adj_gradient", "lag2d_destroy", "lag2d_last_error")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -65,6 +66,9 @@ def load():
     lib.lag2d_download.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.lag2d_get_time.argtypes = [C.c_void_p, _dp, _dp]
     lib.lag2d_get_mass_matrix.argtypes = [C.c_void_p, _dp]
+    lib.lag2d_adj_record.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    lib.lag2d_adj_replay.argtypes = [C.c_void_p, C.c_int]
+    lib.lag2d_adj_gradient.argtypes = [C.c_void_p, _dp, _dp]
     lib.lag2d_check.argtypes = [C.c_void_p]
     lib.lag2d_sync.argtypes = [C.c_void_p]
     lib.lag2d_kernel_time.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_char_p)]
@@ -156,6 +160,25 @@ class Lag2D:
         ustar = np.zeros((c.npoin, 2))
         _check(load().lag2d_download(self.h, _p(coord), _p(geoel), _p(unkno), _p(ustar)))
         return {"coord": coord, "geoel": geoel, "unkno": unkno, "ustar": ustar}
+
+    def adj_record(self, nsteps, every):
+        """nsteps plain steps recording the limiter and dt, checkpoints every
+        ``every`` steps (the discrete adjoint's forward pass)."""
+        _check(load().lag2d_adj_record(self.h, self.itime + 1, nsteps, every))
+        self.itime += nsteps
+
+    def adj_replay(self, nsteps):
+        """The recorded steps again from the resident state with the limiter
+        and dt frozen at their recorded values."""
+        _check(load().lag2d_adj_replay(self.h, nsteps))
+
+    def adj_gradient(self, lam_coord):
+        """dJ/dunkno (ncell, 4, ndegr) at the start of the recording, from
+        dJ/dcoord (npoin, 2) at its end."""
+        lam = np.ascontiguousarray(lam_coord, dtype=np.float64)
+        g = np.zeros((self.case.ncell, 4, self.case.ctl.ndegr))
+        _check(load().lag2d_adj_gradient(self.h, _p(lam), _p(g)))
+        return g
 
     def mass_matrix(self):
         a = np.zeros((self.case.ncell, self.case.nmatr))
