@@ -808,7 +808,7 @@ static void ndvelo_q1(orc_case *c, const double *gt, const double *gq, const dou
             double dux = X(c->ustar, 0, p) - uvx[iv];
             double duy = X(c->ustar, 1, p) - vvx[iv];
             for (int f = 0; f < 2; ++f) {
-                double deltu = fabs(dux * vn[iv][f][0] + duy * vn[iv][f][1]);
+                double deltu = ORC_FROZEN(fabs(dux * vn[iv][f][0] + duy * vn[iv][f][1]));
                 double mur = rhoct * sdctr + c->cimpd * rhoct * c->slpdu * deltu;
                 double area = vn[iv][f][2];
                 /* getriecoef_matrixnew: aujmp(3) < 1e12 gives vnau = 1 */
@@ -1157,6 +1157,8 @@ static int rhs_q1(orc_case *c)
     vtx_minmax(c, unmax, unmin);
     barth_tri(c, unmax, unmin, c->aflim, c->afvec);
     barth_quad(c, unmax, unmin, c->aflim, c->afvec);
+    for (int i = 0; i < 5 * c->ncell; ++i) c->aflim[i] = ORC_FROZEN(c->aflim[i]);
+    for (int i = 0; i < 4 * c->ncell; ++i) c->afvec[i] = ORC_FROZEN(c->afvec[i]);
     ndvelo_q1(c, gt, gq, c->aflim, c->afvec, fstart, fstarq);
     iface_tri(c, gt, fstart);
     iface_q1(c, gq, fstarq);
@@ -2085,6 +2087,7 @@ int orc_step(orc_case *c)
                 for (int ie = 0; ie < c->ncell; ++ie) dvdt[ie] = U(c, c->rhsel, 0, 0, ie);
                 delt(c, dvdt);
                 if (c->tacru + c->dtfix > c->tend) c->dtfix = c->tend - c->tacru;
+                c->dtfix = ORC_FROZEN(c->dtfix);
                 c->tacru = c->tacru + c->dtfix;
             }
         } else if (istag == 0) {

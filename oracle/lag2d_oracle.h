@@ -85,6 +85,13 @@ enum orc_phase {
 #ifndef ORC_WENO_CELL
 #define ORC_WENO_CELL() ((void)0)
 #endif
+/* Tangent build (oracle/tangent/): marks the values the discrete adjoint
+ * holds at their forward values -- the limiter coefficients, the corner
+ * impedance's |du.n| and the time step -- whose derivative is dropped.  The
+ * identity in the ordinary build. */
+#ifndef ORC_FROZEN
+#define ORC_FROZEN(x) (x)
+#endif
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,21 @@ def opcount_lib():
     return _OPLIB
 
 
+_TLIB = None
+
+
+def tangent_lib():
+    """The tangent (forward-mode) build of the same sources (oracle/tangent/)."""
+    global _TLIB
+    if _TLIB is None:
+        path = os.path.join(_HERE, "tangent", "liblag2d_tangent.so")
+        if not os.path.exists(path):
+            raise RuntimeError("tangent oracle not built: run `make -C oracle/tangent`")
+        _TLIB = C.CDLL(path)
+        _TLIB.orc_tangent_J.argtypes = [C.POINTER(_Case), _dp, C.c_int, _dp, _dp, _dp]
+    return _TLIB
+
+
 # phases of the op counts (enum orc_phase in lag2d_oracle.h)
 PHASES = ("other", "prep", "weno_mc", "weno", "corner", "node", "cellrhs", "update", "nodemove", "cellvol", "dt",
           "hoisted")
@@ -147,6 +162,19 @@ class Oracle:
                 raise RuntimeError(f"oracle step: unsupported configuration ({rc})")
             if abs(self.s.tacru - self.case.ctl.tend) / self.case.ctl.tend <= 1e-11:
                 break
+
+    def tangent_J(self, dunkno, nsteps, xt):
+        """J = |x - xt|^2 after nsteps steps from the current state and its
+        derivative along dunkno (ncell, 4, ndegr) of the DOFs, from the
+        tangent build, with the limiter, the corner impedance's |du.n| and dt
+        frozen at their forward values.  The state is not advanced."""
+        v = np.ascontiguousarray(dunkno, dtype=np.float64)
+        x = np.ascontiguousarray(xt, dtype=np.float64)
+        J, dJ = C.c_double(), C.c_double()
+        rc = tangent_lib().orc_tangent_J(C.byref(self.s), _p(v), nsteps, _p(x), C.byref(J), C.byref(dJ))
+        if rc:
+            raise RuntimeError(f"oracle tangent: unsupported configuration ({rc})")
+        return J.value, dJ.value
 
     def op_counts(self, reset=True):
         """FP64 operations per phase since the last reset, and the number of
