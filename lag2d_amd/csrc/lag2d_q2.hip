@@ -1018,6 +1018,20 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
 // records of its sub-cell corner.
 constexpr int CORNER_CPB = 16;
 
+// Jacobian of a Q2 map from tabulated shape-function derivatives (the table
+// is built by k_q2_shape_tables with shape9, so the sums equal jac9's).
+__device__ __forceinline__ void jac9_tab(const double *xp, const double *yp, const double *t, double J[4])
+{
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        dxdr = dxdr + t[k] * xp[k];
+        dxds = dxds + t[9 + k] * xp[k];
+        dydr = dydr + t[k] * yp[k];
+        dyds = dyds + t[9 + k] * yp[k];
+    }
+    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
+}
+
 __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
 {
     __shared__ double sgq[CORNER_CPB][16 * 3 + 1];  // padded: cells of a wave in different banks
@@ -1213,17 +1227,6 @@ __global__ void __launch_bounds__(256) k_q2_shape_tables(double *tab)
     }
 }
 
-__device__ __forceinline__ void jac9_tab(const double *xp, const double *yp, const double *t, double J[4])
-{
-    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
-    for (int k = 0; k < 9; ++k) {
-        dxdr = dxdr + t[k] * xp[k];
-        dxds = dxds + t[9 + k] * xp[k];
-        dydr = dydr + t[k] * yp[k];
-        dyds = dyds + t[9 + k] * yp[k];
-    }
-    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
-}
 
 __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
 {
@@ -1445,6 +1448,7 @@ constexpr int RHS_CPB = 8;
 // the 8 cells' 6-double windows 12 banks apart (3-way); the corner factors
 // are read per cell (broadcast).
 constexpr int RHS_CN = 16 * 12 + 1;  // corner factors
+constexpr int RHS_CS = 49;           // per-cell inputs: x[9], y[9], Ul[24], rc, sc, g19, g20, g21 (odd stride)
 constexpr int RHS_BN = 16 * 6 + 6;   // basis at the corner's node
 constexpr int RHS_PS = 17;           // point values: a[6], c[6], u, v, p, w (+1 pad)
 constexpr int RHS_PT = 25 * RHS_PS;
@@ -1455,6 +1459,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
     __shared__ double sbn[RHS_CPB][RHS_BN];
     __shared__ double spt[RHS_CPB][RHS_PT];
     __shared__ double sres[RHS_CPB][4][6];
+    __shared__ double scl[RHS_CPB][RHS_CS];
     const int tid = threadIdx.x;
     const int nc = d.nc;
     const int ie0 = xcd_block(blockIdx.x, gridDim.x) * RHS_CPB;
@@ -1463,6 +1468,23 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         *d.tcount = 0;
     }
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
+    // the cells' node coordinates, limited DOFs and basis geometry, loaded
+    // once per cell for the 16 corner and 25 point tasks that read them
+    for (int u = tid; u < RHS_CPB * 47; u += 256) {
+        const int cl = u / 47, k = u - 47 * cl, ie = ie0 + cl;
+        if (ie >= nc) continue;
+        double v;
+        if (k < 9)
+            v = d.xs[d.ipq[k * nc + ie]];
+        else if (k < 18)
+            v = d.ys[d.ipq[(k - 9) * nc + ie]];
+        else if (k < 42)
+            v = d.Ul[(k - 18) * nc + ie];
+        else
+            v = d.G[(k == 42 ? 0 : (k == 43 ? 1 : 18 + (k - 44))) * nc + ie];
+        scl[cl][k] = v;
+    }
+    __syncthreads();
     // corners on threads 0-127, points on threads 128-255 and 0-71
     {
         const int cl = tid >> 4, corner = tid & 15, ie = ie0 + cl;
@@ -1500,8 +1522,8 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             c[10] = us1 * fx[1];
             c[11] = us2 * fy[1];
             double bb[6];
-            basis6(XV9[ln], YV9[ln], d.G[0 * nc + ie], d.G[1 * nc + ie], d.G[18 * nc + ie], d.G[19 * nc + ie],
-                   d.G[20 * nc + ie], bb);
+            const double *sc_ = scl[cl];
+            basis6(XV9[ln], YV9[ln], sc_[42], sc_[43], sc_[44], sc_[45], sc_[46], bb);
             for (int id = 0; id < 6; ++id) sbn[cl][6 * corner + id] = bb[id];
         }
     }
@@ -1510,16 +1532,15 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         if (ie >= nc) continue;
         const int i = pt / 5, j = pt - 5 * i;
         const double rr = PL5[i], ss = PL5[j], wi = WL5[i] * WL5[j];
-        const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
-        double xp[9], yp[9];
-        load9(d, d.xs, d.ys, ie, xp, yp);
+        const double *sc_ = scl[cl];
+        const double rc = sc_[42], sc = sc_[43];
         double J[4];
-        jac9(xp, yp, rr, ss, J);
+        jac9(sc_, sc_ + 9, rr, ss, J);
         double b[6];
-        basis6(rr, ss, rc, sc, d.G[18 * nc + ie], d.G[19 * nc + ie], d.G[20 * nc + ie], b);
+        basis6(rr, ss, rc, sc, sc_[44], sc_[45], sc_[46], b);
         double un[4] = {0.0, 0.0, 0.0, 0.0};
         for (int id = 0; id < 6; ++id)
-            for (int q = 0; q < 4; ++q) un[q] = un[q] + UD(d.Ul, q, id, ie) * b[id];
+            for (int q = 0; q < 4; ++q) un[q] = un[q] + sc_[18 + 6 * q + id] * b[id];
         const double rhoad = 1.0 / un[0];
         const double uu = un[1], vv = un[2];
         const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
