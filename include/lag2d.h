@@ -189,6 +189,15 @@ int lag2d_weno_cells(lag2d_ctx *ctx, long long *total);
  * the reference. */
 int lag2d_division_check(int n, const double *a, const double *b, double *q);
 
+/* With enable != 0, lag2d_step and lag2d_run replay each step from a HIP
+ * graph captured from the step's launches (one per buffer parity and fixed
+ * dt and epsaw), cutting the host's per-step launch cost; the results are
+ * the eager step's.  The GPU's own gaps between dependent kernels are not
+ * shortened, so small meshes bound by them gain nothing.  The first step of a run, steps with kernel timing on and
+ * curved-path steps before its kernel variants are tuned run eagerly.  No
+ * counterpart in the reference. */
+int lag2d_use_graphs(lag2d_ctx *ctx, int enable);
+
 /* Discrete adjoint of the step (straight P1 planar path: ncurv=0, nrz=0,
  * quads, SSP-RK2).  No counterpart in the reference.  The limiter
  * coefficients, the impedance's |du.n| (the shock switch of the corner

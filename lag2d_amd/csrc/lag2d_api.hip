@@ -48,6 +48,8 @@ static void launch_cellvol(lag2d_ctx *x, double *xs, double *ys, int set_centre,
 
 lag2d_ctx::~lag2d_ctx()
 {
+    for (auto &g : graphs)
+        if (g.exec) hipGraphExecDestroy(g.exec);
     for (void *p : allocs) hipFree(p);
     for (auto &t : timers) {
         for (auto &ev : t.ev) {
@@ -560,6 +562,7 @@ static int step_impl(lag2d_ctx *x, int itime)
     std::swap(d.Un, d.Us);
     std::swap(d.xn, d.xs);
     std::swap(d.yn, d.ys);
+    x->parity ^= 1;
     return 0;
 }
 
@@ -641,6 +644,7 @@ static int adj_fwd_step(lag2d_ctx *x, int itime, int n, int mode, int k)
     std::swap(d.Un, d.Us);
     std::swap(d.xn, d.xs);
     std::swap(d.yn, d.ys);
+    x->parity ^= 1;
     return 0;
 }
 
@@ -796,11 +800,61 @@ extern "C" int lag2d_adj_gradient(lag2d_ctx *x, const double *lam_coord, double 
     return 0;
 }
 
+// A step through a captured HIP graph: the first step of each key (swap
+// parity, fixed dt and epsaw) is captured from step_impl's launches, later
+// ones replay it.  The first step of a run (dt from the CFL alone), timed
+// steps and curved-path steps before the variants are tuned run eagerly.
+static int step_graph(lag2d_ctx *x, int itime)
+{
+    const bool tuned = x->cfg.ncurv != 1 || (x->corner_variant != 0 && x->rhs_variant != 0);
+    if (!x->use_graphs || itime <= 1 || x->timing || !tuned) return step_impl(x, itime);
+    double dt = 0.0, epsaw = 0.0;
+    const int sched = schedule(x->cfg, itime, &dt, &epsaw) ? 1 : 0;
+    Dev &d = x->d;
+    for (auto &g : x->graphs) {
+        if (g.parity == x->parity && g.sched == sched && g.dt == dt && g.epsaw == epsaw) {
+            HIPCHK(hipGraphLaunch(g.exec, x->stream));
+            std::swap(d.Un, d.Us);
+            std::swap(d.xn, d.xs);
+            std::swap(d.yn, d.ys);
+            x->parity ^= 1;
+            return 0;
+        }
+    }
+    lag2d_graph g;
+    g.parity = x->parity;
+    g.sched = sched;
+    g.dt = dt;
+    g.epsaw = epsaw;
+    HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = step_impl(x, itime);  // records the launches and swaps the buffers
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(x->stream, &graph);
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    HIPCHK(e);
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    HIPCHK(ei);
+    x->graphs.push_back(g);
+    HIPCHK(hipGraphLaunch(g.exec, x->stream));
+    return 0;
+}
+
+extern "C" int lag2d_use_graphs(lag2d_ctx *x, int enable)
+{
+    if (!x) return fail(LAG2D_EINVAL, "null context");
+    x->use_graphs = enable != 0;
+    return 0;
+}
+
 extern "C" int lag2d_step(lag2d_ctx *x, int itime)
 {
     if (!x) return fail(LAG2D_EINVAL, "null context");
     if (itime < 1) return fail(LAG2D_EINVAL, "itime is 1-based");
-    return step_impl(x, itime);
+    return step_graph(x, itime);
 }
 
 extern "C" int lag2d_run(lag2d_ctx *x, int itime0, int nsteps, int *taken)
@@ -815,7 +869,7 @@ extern "C" int lag2d_run(lag2d_ctx *x, int itime0, int nsteps, int *taken)
     HIPCHK(hipStreamSynchronize(x->stream));
     bool watch = x->cfg.ndt == 1;
     for (int k = 0; k < nsteps; ++k) {
-        int rc = step_impl(x, itime0 + k);
+        int rc = step_graph(x, itime0 + k);
         if (rc) return rc;
         ++n;
         if (x->cfg.strict && x->cfg.ncurv == 1) {

@@ -32,6 +32,14 @@ struct lag2d_adj {
     double *buf = nullptr;       // adjoint buffers
 };
 
+// A captured step: its launches depend on the parity of the state-buffer
+// swaps and, where the problem fixes dt, on the step's dt and epsaw.
+struct lag2d_graph {
+    int parity = 0, sched = 0;
+    double dt = 0.0, epsaw = 0.0;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct lag2d_ctx {
     lag2d_config cfg{};
     int nc = 0, np = 0, nb = 0, nd = 0, nvq = 0, nmatr = 0;
@@ -47,6 +55,9 @@ struct lag2d_ctx {
     hipStream_t stream = nullptr;
     lag2d::Dev d{};
     lag2d_adj adj;
+    int parity = 0;                   // state buffers swapped an odd number of times
+    bool use_graphs = false;          // replay captured steps (lag2d_use_graphs)
+    std::vector<lag2d_graph> graphs;
     double *A = nullptr;    // inverse mass matrix (SoA)
     double *x0 = nullptr;   // initial coordinates [2][np]
     std::vector<void *> allocs;

@@ -103,6 +103,10 @@ class Ensemble:
         self.itime += taken.value
         return taken.value
 
+    def use_graphs(self, enable=True):
+        """Replay steps from captured HIP graphs (results unchanged)."""
+        _check(load().lag2d_use_graphs(self.h, 1 if enable else 0))
+
     def sync(self):
         _check(load().lag2d_sync(self.h))
 

@@ -81,6 +81,12 @@ module lag2d_c
       real(c_double), intent(out) :: amatr(*)
     end function lag2d_get_mass_matrix
 
+    integer(c_int) function lag2d_use_graphs(ctx, enable) bind(c, name='lag2d_use_graphs')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function lag2d_use_graphs
+
     integer(c_int) function lag2d_adj_record(ctx, itime0, nsteps, every) bind(c, name='lag2d_adj_record')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

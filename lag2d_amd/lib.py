@@ -19,7 +19,7 @@ EXPORTS = ("lag2d_create", "lag2d_create_hybrid", "lag2d_create_ensemble", "lag2
            "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
            "lag2d_division_check", "lag2d_get_mass_matrix", "lag2d_adj_record", "lag2d_adj_replay",
-           "lag2d_adj_gradient", "lag2d_destroy", "lag2d_last_error")
+           "lag2d_adj_gradient", "lag2d_use_graphs", "lag2d_destroy", "lag2d_last_error")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -69,6 +69,7 @@ def load():
     lib.lag2d_adj_record.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     lib.lag2d_adj_replay.argtypes = [C.c_void_p, C.c_int]
     lib.lag2d_adj_gradient.argtypes = [C.c_void_p, _dp, _dp]
+    lib.lag2d_use_graphs.argtypes = [C.c_void_p, C.c_int]
     lib.lag2d_check.argtypes = [C.c_void_p]
     lib.lag2d_sync.argtypes = [C.c_void_p]
     lib.lag2d_kernel_time.argtypes = [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_char_p)]
@@ -160,6 +161,10 @@ class Lag2D:
         ustar = np.zeros((c.npoin, 2))
         _check(load().lag2d_download(self.h, _p(coord), _p(geoel), _p(unkno), _p(ustar)))
         return {"coord": coord, "geoel": geoel, "unkno": unkno, "ustar": ustar}
+
+    def use_graphs(self, enable=True):
+        """Replay steps from captured HIP graphs (results unchanged)."""
+        _check(load().lag2d_use_graphs(self.h, 1 if enable else 0))
 
     def adj_record(self, nsteps, every):
         """nsteps plain steps recording the limiter and dt, checkpoints every
