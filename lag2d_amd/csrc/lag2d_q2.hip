@@ -1103,11 +1103,24 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
                                   {-1.0, .5},   {-1.0, -.5}, {0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
         const int ipsc[4][4] = {{0, 8, 11, 7}, {1, 2, 9, 8}, {9, 3, 4, 10}, {11, 10, 5, 6}};
         double xs[9], ys[9];
+        {
+            // the sub-cell's four lanes form a DPP quad: lane `part` maps
+            // mid-node `part`, and the quad shares the four results
+            const int c = ipsc[isg][part];
+            double xm, ym;
+            map9(xp, yp, pc[c][0], pc[c][1], &xm, &ym);
+            xs[4] = quad_bcast<0>(xm);
+            ys[4] = quad_bcast<0>(ym);
+            xs[5] = quad_bcast<1>(xm);
+            ys[5] = quad_bcast<1>(ym);
+            xs[6] = quad_bcast<2>(xm);
+            ys[6] = quad_bcast<2>(ym);
+            xs[7] = quad_bcast<3>(xm);
+            ys[7] = quad_bcast<3>(ym);
+        }
         for (int k = 0; k < 4; ++k) {
             xs[k] = xp[IPQSG[isg][k]];
             ys[k] = yp[IPQSG[isg][k]];
-            const int c = ipsc[isg][k];
-            map9(xp, yp, pc[c][0], pc[c][1], &xs[4 + k], &ys[4 + k]);
         }
         xs[8] = -0.25 * (xs[0] + xs[1] + xs[2] + xs[3]) + 0.5 * (xs[4] + xs[5] + xs[6] + xs[7]);
         ys[8] = -0.25 * (ys[0] + ys[1] + ys[2] + ys[3]) + 0.5 * (ys[4] + ys[5] + ys[6] + ys[7]);
