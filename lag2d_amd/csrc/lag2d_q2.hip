@@ -1035,7 +1035,6 @@ __device__ __forceinline__ void jac9_tab(const double *xp, const double *yp, con
 __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
 {
     __shared__ double sgq[CORNER_CPB][16 * 3 + 1];  // padded: cells of a wave in different banks
-    __shared__ double sta[CORNER_CPB][4][25], stm[CORNER_CPB][4][25], stv[CORNER_CPB][4][25];
     const int cl = threadIdx.x >> 4, lane = threadIdx.x & 15;
     const int ie = blockIdx.x * CORNER_CPB + cl;
     const int nc = d.nc;
@@ -1044,6 +1043,7 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
     double xp[9], yp[9];
     double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
+    double a = 0.0, masel = 0.0, volel = 0.0;  // sub-cell area, mass and volume
     if (active) {
         load9(d, d.xs, d.ys, ie, xp, yp);
         rc = d.G[0 * nc + ie];
@@ -1126,11 +1126,13 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
         ys[8] = -0.25 * (ys[0] + ys[1] + ys[2] + ys[3]) + 0.5 * (ys[4] + ys[5] + ys[6] + ys[7]);
         const double sr[4] = {-1.0, 1.0, 1.0, -1.0}, ss[4] = {-1.0, -1.0, 1.0, 1.0};
         const double rhomc = u0[0];
-        for (int g = part; g < 25; g += 4) {
+        // terms of Gauss point g: the sub-cell map's area element, and the
+        // parent map's volume element with the mass element 1/tau |J| w
+        auto terms = [&](int g, double &ta, double &tm, double &tv) {
             const int i = g / 5, j = g - 5 * i;
             double J[4];
             jac9(xs, ys, PG5[i], PG5[j], J);
-            sta[cl][isg][g] = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
+            ta = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
             const double r = (PG5[i] + sr[isg]) / 2.0;
             const double s = (PG5[j] + ss[isg]) / 2.0;
             double b[6];
@@ -1140,21 +1142,37 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
             t = rhomc + 1.0 * (t - rhomc);
             jac9(xp, yp, r, s, J);
             const double djaco = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]) * 0.25;
-            stm[cl][isg][g] = 1.0 / t * djaco;
-            stv[cl][isg][g] = djaco;
+            tm = 1.0 / t * djaco;
+            tv = djaco;
+        };
+        // Lane `part` evaluates point 4k + part in round k (the last round
+        // has point 24 only); the quad adds each round's terms in point
+        // order, so every lane of the sub-cell ends with the sequential sums
+        // over points 0..24.
+#pragma unroll 1
+        for (int k = 0; k < 7; ++k) {
+            double ta, tm, tv;
+            terms(k < 6 ? 4 * k + part : 24, ta, tm, tv);
+            a = a + quad_bcast<0>(ta);
+            masel = masel + quad_bcast<0>(tm);
+            volel = volel + quad_bcast<0>(tv);
+            if (k < 6) {
+                a = a + quad_bcast<1>(ta);
+                masel = masel + quad_bcast<1>(tm);
+                volel = volel + quad_bcast<1>(tv);
+                a = a + quad_bcast<2>(ta);
+                masel = masel + quad_bcast<2>(tm);
+                volel = volel + quad_bcast<2>(tv);
+                a = a + quad_bcast<3>(ta);
+                masel = masel + quad_bcast<3>(tm);
+                volel = volel + quad_bcast<3>(tv);
+            }
         }
     }
-    __syncthreads();
+    __syncthreads();  // face slots shared
     if (!active) return;
     // ---- sub-cell corner (isg, iv)
     const int iv = part;
-    double a = 0.0, masel = 0.0, volel = 0.0;
-#pragma unroll
-    for (int g = 0; g < 25; ++g) {
-        a = a + sta[cl][isg][g];
-        masel = masel + stm[cl][isg][g];
-        volel = volel + stv[cl][isg][g];
-    }
     const double rsub = 0.0 + (1.0 / a) * d.msub[isg * nc + ie];
     const double uq = volel / masel;
     double u[4][6];
