@@ -1035,17 +1035,29 @@ __device__ __forceinline__ void jac9_tab(const double *xp, const double *yp, con
 __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
 {
     __shared__ double sgq[CORNER_CPB][16 * 3 + 1];  // padded: cells of a wave in different banks
+    // node coordinates of the cells and of their sub-cells (sub-cell isg at
+    // 9 isg), shared by the lanes that read them; strides keep the 4 cells x
+    // 4 sub-cells a wave reads at once in distinct banks
+    __shared__ double sxp[CORNER_CPB][9], syp[CORNER_CPB][9];
+    __shared__ double ssx[CORNER_CPB][36], ssy[CORNER_CPB][36];
     const int cl = threadIdx.x >> 4, lane = threadIdx.x & 15;
     const int ie = blockIdx.x * CORNER_CPB + cl;
     const int nc = d.nc;
     const bool active = ie < nc;
     const int isg = lane >> 2, part = lane & 3;
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
-    double xp[9], yp[9];
+    if (active && lane < 9) {
+        const int p = d.ipq[lane * nc + ie];
+        sxp[cl][lane] = d.xs[p];
+        syp[cl][lane] = d.ys[p];
+    }
+    __syncthreads();
+    const double *xp = sxp[cl], *yp = syp[cl];
+    double *xs = &ssx[cl][9 * isg], *ys = &ssy[cl][9 * isg];
     double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
     double a = 0.0, masel = 0.0, volel = 0.0;  // sub-cell area, mass and volume
+    double u0[6];
     if (active) {
-        load9(d, d.xs, d.ys, ie, xp, yp);
         rc = d.G[0 * nc + ie];
         sc = d.G[1 * nc + ie];
         gg0 = d.G[18 * nc + ie];
@@ -1097,39 +1109,47 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
         d.gq[(3 * lane + 2) * nc + ie] = ga;
         // ---- quadrature terms of sub-cell isg: area of its Q2 map and the
         //      mass-weighted mean specific volume of the DG polynomial
-        double u0[6];
         for (int id = 0; id < 6; ++id) u0[id] = UD(d.Ul, 0, id, ie);
+        // nodes of sub-cell isg: lane `part` stores corner `part` and maps
+        // mid-node `part`; the sub-cell's four lanes form a DPP quad, which
+        // shares the mid-nodes for the centre node
         const double pc[12][2] = {{-0.5, -1.0}, {0.5, -1.0}, {1.0, -.5}, {1.0, .5}, {0.5, 1.0}, {-0.5, 1.0},
                                   {-1.0, .5},   {-1.0, -.5}, {0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
         const int ipsc[4][4] = {{0, 8, 11, 7}, {1, 2, 9, 8}, {9, 3, 4, 10}, {11, 10, 5, 6}};
-        double xs[9], ys[9];
-        {
-            // the sub-cell's four lanes form a DPP quad: lane `part` maps
-            // mid-node `part`, and the quad shares the four results
-            const int c = ipsc[isg][part];
-            double xm, ym;
-            map9(xp, yp, pc[c][0], pc[c][1], &xm, &ym);
-            xs[4] = quad_bcast<0>(xm);
-            ys[4] = quad_bcast<0>(ym);
-            xs[5] = quad_bcast<1>(xm);
-            ys[5] = quad_bcast<1>(ym);
-            xs[6] = quad_bcast<2>(xm);
-            ys[6] = quad_bcast<2>(ym);
-            xs[7] = quad_bcast<3>(xm);
-            ys[7] = quad_bcast<3>(ym);
+        const int c = ipsc[isg][part];
+        double xm, ym;
+        map9(xp, yp, pc[c][0], pc[c][1], &xm, &ym);
+        xs[part] = xp[IPQSG[isg][part]];
+        ys[part] = yp[IPQSG[isg][part]];
+        xs[4 + part] = xm;
+        ys[4 + part] = ym;
+        const double mx0 = quad_bcast<0>(xm), mx1 = quad_bcast<1>(xm), mx2 = quad_bcast<2>(xm),
+                     mx3 = quad_bcast<3>(xm);
+        const double my0 = quad_bcast<0>(ym), my1 = quad_bcast<1>(ym), my2 = quad_bcast<2>(ym),
+                     my3 = quad_bcast<3>(ym);
+        if (part == 0) {
+            xs[8] = -0.25 * (xp[IPQSG[isg][0]] + xp[IPQSG[isg][1]] + xp[IPQSG[isg][2]] + xp[IPQSG[isg][3]]) +
+                    0.5 * (mx0 + mx1 + mx2 + mx3);
+            ys[8] = -0.25 * (yp[IPQSG[isg][0]] + yp[IPQSG[isg][1]] + yp[IPQSG[isg][2]] + yp[IPQSG[isg][3]]) +
+                    0.5 * (my0 + my1 + my2 + my3);
         }
-        for (int k = 0; k < 4; ++k) {
-            xs[k] = xp[IPQSG[isg][k]];
-            ys[k] = yp[IPQSG[isg][k]];
-        }
-        xs[8] = -0.25 * (xs[0] + xs[1] + xs[2] + xs[3]) + 0.5 * (xs[4] + xs[5] + xs[6] + xs[7]);
-        ys[8] = -0.25 * (ys[0] + ys[1] + ys[2] + ys[3]) + 0.5 * (ys[4] + ys[5] + ys[6] + ys[7]);
+    }
+    __syncthreads();  // sub-cell nodes staged, face slots shared
+    if (!active) return;
+    {
         const double sr[4] = {-1.0, 1.0, 1.0, -1.0}, ss[4] = {-1.0, -1.0, 1.0, 1.0};
         const double rhomc = u0[0];
         // terms of Gauss point g: the sub-cell map's area element, and the
         // parent map's volume element with the mass element 1/tau |J| w
         auto terms = [&](int g, double &ta, double &tm, double &tv) {
             const int i = g / 5, j = g - 5 * i;
+            // the node offsets pass through an empty asm statement, so the
+            // compiler reads the nodes from LDS here rather than holding
+            // them in registers across the loop
+            int oc = cl, os = 9 * isg;
+            asm volatile("" : "+v"(oc), "+v"(os));
+            const double *xs = &ssx[oc][os], *ys = &ssy[oc][os];
+            const double *xp = sxp[oc], *yp = syp[oc];
             double J[4];
             jac9(xs, ys, PG5[i], PG5[j], J);
             ta = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
@@ -1169,8 +1189,6 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
             }
         }
     }
-    __syncthreads();  // face slots shared
-    if (!active) return;
     // ---- sub-cell corner (isg, iv)
     const int iv = part;
     const double rsub = 0.0 + (1.0 / a) * d.msub[isg * nc + ie];
