@@ -1008,9 +1008,15 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
 }
 
 // ------------------------------------------------------------- corners --
+// Corner-face records, cell-major: the 5 records of corner face cf of cell
+// ie at (32 ie + cf) 5 + r, so the two faces of a sub-cell corner are 10
+// consecutive doubles for the corner kernel that writes them, the node
+// gather and the cell RHS that read them.
+#define RQ2(ie, cf, r) (((size_t)(ie) * 32 + (cf)) * 5 + (r))
+
 // Face geometry (stored for the face integral), sub-cell density and mean
 // specific volume, vertex states and the 32 corner-face Riemann records.
-// Records: rec[(r*32 + cf)*nc + ie], cf = f + 2*(ivsg + 4*isg),
+// Records: rec[RQ2(ie, cf, r)], cf = f + 2*(ivsg + 4*isg),
 // r = 0: m = mu*A, 1-2: m u_c, 3-4: A sigma n.
 // 16 lanes per cell: first lane k evaluates face slot k and a quarter of
 // the 25 quadrature points of sub-cell k/4 (terms staged in LDS), then lane
@@ -1233,11 +1239,11 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
             m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
         }
         const int cf = f + 2 * (iv + 4 * isg);
-        d.rec[(0 * 32 + cf) * nc + ie] = m11;
-        d.rec[(1 * 32 + cf) * nc + ie] = m11 * uv + 0.0 * vv;
-        d.rec[(2 * 32 + cf) * nc + ie] = 0.0 * uv + m11 * vv;
-        d.rec[(3 * 32 + cf) * nc + ie] = -pv * area * nx + 0.0 * area * ny;
-        d.rec[(4 * 32 + cf) * nc + ie] = 0.0 * area * nx + -pv * area * ny;
+        d.rec[RQ2(ie, cf, 0)] = m11;
+        d.rec[RQ2(ie, cf, 1)] = m11 * uv + 0.0 * vv;
+        d.rec[RQ2(ie, cf, 2)] = 0.0 * uv + m11 * vv;
+        d.rec[RQ2(ie, cf, 3)] = -pv * area * nx + 0.0 * area * ny;
+        d.rec[RQ2(ie, cf, 4)] = 0.0 * area * nx + -pv * area * ny;
     }
 }
 
@@ -1428,11 +1434,11 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
                 m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
             }
             const int cf = f + 2 * (iv + 4 * isg);
-            d.rec[(0 * 32 + cf) * nc + ie] = m11;
-            d.rec[(1 * 32 + cf) * nc + ie] = m11 * uv + 0.0 * vv;
-            d.rec[(2 * 32 + cf) * nc + ie] = 0.0 * uv + m11 * vv;
-            d.rec[(3 * 32 + cf) * nc + ie] = -pv * area * nx + 0.0 * area * ny;
-            d.rec[(4 * 32 + cf) * nc + ie] = 0.0 * area * nx + -pv * area * ny;
+            d.rec[RQ2(ie, cf, 0)] = m11;
+            d.rec[RQ2(ie, cf, 1)] = m11 * uv + 0.0 * vv;
+            d.rec[RQ2(ie, cf, 2)] = 0.0 * uv + m11 * vv;
+            d.rec[RQ2(ie, cf, 3)] = -pv * area * nx + 0.0 * area * ny;
+            d.rec[RQ2(ie, cf, 4)] = 0.0 * area * nx + -pv * area * ny;
         }
     }
 }
@@ -1452,15 +1458,15 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
         const int ie = e >> 4, slot = e & 15;
         for (int f = 0; f < 2; ++f) {
             const int cf = f + 2 * slot;
-            double m = d.rec[(0 * 32 + cf) * nc + ie];
+            double m = d.rec[RQ2(ie, cf, 0)];
             a11 = a11 + m;
             a21 = a21 + 0.0;
             a12 = a12 + 0.0;
             a22 = a22 + m;
-            mu1 = mu1 + d.rec[(1 * 32 + cf) * nc + ie];
-            mu2 = mu2 + d.rec[(2 * 32 + cf) * nc + ie];
-            sn1 = sn1 + d.rec[(3 * 32 + cf) * nc + ie];
-            sn2 = sn2 + d.rec[(4 * 32 + cf) * nc + ie];
+            mu1 = mu1 + d.rec[RQ2(ie, cf, 1)];
+            mu2 = mu2 + d.rec[RQ2(ie, cf, 2)];
+            sn1 = sn1 + d.rec[RQ2(ie, cf, 3)];
+            sn2 = sn2 + d.rec[RQ2(ie, cf, 4)];
         }
     }
     const uint8_t fl = d.nflag[p];
@@ -1551,9 +1557,9 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
                 ny[f] = sg * d.gq[(3 * sl + 1) * nc + ie];
                 ar[f] = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
                 const int cf = f + 2 * (iv + 4 * isg);
-                const double m11 = d.rec[(0 * 32 + cf) * nc + ie];
-                const double m1 = d.rec[(1 * 32 + cf) * nc + ie], m2 = d.rec[(2 * 32 + cf) * nc + ie];
-                const double s1 = d.rec[(3 * 32 + cf) * nc + ie], s2 = d.rec[(4 * 32 + cf) * nc + ie];
+                const double m11 = d.rec[RQ2(ie, cf, 0)];
+                const double m1 = d.rec[RQ2(ie, cf, 1)], m2 = d.rec[RQ2(ie, cf, 2)];
+                const double s1 = d.rec[RQ2(ie, cf, 3)], s2 = d.rec[RQ2(ie, cf, 4)];
                 fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
                 fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
             }
@@ -1739,9 +1745,9 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
             ny[f] = sg * d.gq[(3 * sl + 1) * nc + ie];
             ar[f] = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
             const int cf = f + 2 * k;
-            const double m11 = d.rec[(0 * 32 + cf) * nc + ie];
-            const double m1 = d.rec[(1 * 32 + cf) * nc + ie], m2 = d.rec[(2 * 32 + cf) * nc + ie];
-            const double s1 = d.rec[(3 * 32 + cf) * nc + ie], s2 = d.rec[(4 * 32 + cf) * nc + ie];
+            const double m11 = d.rec[RQ2(ie, cf, 0)];
+            const double m1 = d.rec[RQ2(ie, cf, 1)], m2 = d.rec[RQ2(ie, cf, 2)];
+            const double s1 = d.rec[RQ2(ie, cf, 3)], s2 = d.rec[RQ2(ie, cf, 4)];
             fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
             fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
         }
