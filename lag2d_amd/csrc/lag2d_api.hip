@@ -420,6 +420,75 @@ extern "C" int lag2d_upload(lag2d_ctx *x, const double *coord, const double *coo
     return 0;
 }
 
+// Driver-side initialisation on the device (curved P2 path): the arrays
+// the reference's start-up computes before the time loop -- geometry,
+// inverse mass matrices, DOFs -- from the coordinates and a piecewise
+// constant state per cell, in place of an upload of host-computed arrays.
+extern "C" int lag2d_init_state(lag2d_ctx *x, const double *coord, const double *rho, const double *u,
+                                const double *v, const double *p, int nblast, const int *blast,
+                                const double *energy)
+{
+    if (!x || !coord || !rho || !u || !v || !p || (nblast > 0 && (!blast || !energy)) || nblast < 0)
+        return fail(LAG2D_EINVAL, "null or negative argument");
+    const lag2d_config &c = x->cfg;
+    if (c.ncurv != 1 || c.npoly != 2 || c.nrz != 0)
+        return fail(LAG2D_EUNSUPPORTED, "device initialisation covers the planar curved P2 path (ncurv=1, npoly=2)");
+    const int nc = x->nc, np = x->np;
+    for (int k = 0; k < nblast; ++k)
+        if (blast[k] < 1 || blast[k] > nc) return fail(LAG2D_EINVAL, "blast cell out of range (1-based)");
+    for (int ie = 0; ie < nc; ++ie)
+        if (!(rho[ie] > 0.0)) return fail(LAG2D_EINVAL, "initial density must be positive");
+    Dev &d = x->d;
+    std::vector<double> h((size_t)std::max(2 * np, 6 * nc), 0.0);
+    auto up = [&](double *dst, size_t n) { return hipMemcpy(dst, h.data(), n * sizeof(double), hipMemcpyHostToDevice); };
+    for (int q = 0; q < 2; ++q) {
+        for (int k = 0; k < np; ++k) h[k] = coord[2 * (size_t)k + q];
+        HIPCHK(up(q == 0 ? d.xn : d.yn, np));
+    }
+    for (int k = 0; k < np; ++k) {
+        h[k] = coord[2 * (size_t)k];
+        h[np + k] = coord[2 * (size_t)k + 1];
+    }
+    HIPCHK(up(x->x0, 2 * (size_t)np));  // coold = coord
+    HIPCHK(hipMemset(d.usx, 0, np * sizeof(double)));
+    HIPCHK(hipMemset(d.usy, 0, np * sizeof(double)));
+    // the per-cell state, staged in the residual buffer (rewritten by the first step)
+    std::fill(h.begin(), h.end(), 0.0);
+    for (int ie = 0; ie < nc; ++ie) {
+        h[ie] = rho[ie];
+        h[nc + ie] = u[ie];
+        h[2 * (size_t)nc + ie] = v[ie];
+        h[3 * (size_t)nc + ie] = p[ie];
+    }
+    for (int k = 0; k < nblast; ++k) {
+        h[4 * (size_t)nc + blast[k] - 1] = energy[k];
+        h[5 * (size_t)nc + blast[k] - 1] = 1.0;
+    }
+    HIPCHK(up(d.R, 6 * (size_t)nc));
+    hipLaunchKernelGGL(k_init_q2, dim3(nblocks(nc)), dim3(LAG2D_BLOCK), 0, x->stream, d, (const double *)d.R, x->A);
+    HIPCHK(hipGetLastError());
+    // the initial density of the sub-cell masses
+    if (!d.rhoi) {
+        double *t;
+        int rc = x->alloc(&t, (size_t)nc);
+        if (rc) return rc;
+        d.rhoi = t;
+    }
+    HIPCHK(hipMemcpyAsync((void *)d.rhoi, rho, (size_t)nc * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    x->geo_host.resize((size_t)24 * nc);
+    std::vector<double> g((size_t)24 * nc);
+    HIPCHK(hipMemcpyAsync(g.data(), d.G, g.size() * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int ie = 0; ie < nc; ++ie)
+        for (int k = 0; k < 24; ++k) x->geo_host[k + 24 * (size_t)ie] = g[(size_t)k * nc + ie];
+    launch_cellvol(x, d.xn, d.yn, 0, false);
+    HIPCHK(hipGetLastError());
+    int rc = upload_extra(*x);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
 // ---- launch helpers with optional per-kernel event timing
 void lag2d_ctx::tic(int k)
 {

@@ -407,4 +407,162 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
 template __global__ void k_cellvol<0>(Dev, double *__restrict__, double *__restrict__, int);
 template __global__ void k_cellvol<1>(Dev, double *__restrict__, double *__restrict__, int);
 
+// 5x5 Gauss rule of ruqope (dgflo2d.f90:2734-2743 ordering)
+__constant__ static const double PG5I[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
+                                            0.5384693101056830910363144, 0.9061798459386639927976269};
+__constant__ static const double WG5I[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
+                                            0.5688888888888888888888889, 0.4786286704993664680412915,
+                                            0.2369268850561890875142640};
+
+// Weighted Jacobian determinant of the Q2 map at (r, s).
+__device__ __forceinline__ double djaco_q2(const double *xp, const double *yp, double r, double s, double wi)
+{
+    double sh[9], dr_[9], ds_[9];
+    shape_q2_d(r, s, sh, dr_, ds_);
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        dxdr = dxdr + dr_[k] * xp[k];
+        dxds = dxds + ds_[k] * xp[k];
+        dydr = dydr + dr_[k] * yp[k];
+        dyds = dyds + ds_[k] * yp[k];
+    }
+    return wi * (dxdr * dyds - dydr * dxds);
+}
+
+// Driver-side initialisation of the curved P2 path, one thread per cell,
+// with the reference's loops and summation orders (planar: the radius
+// coefficient 1 - alfrz + alfrz y is exactly 1 and its products drop):
+//   getgeoel_initial_lag (GetInitial.f90:1507-1693): mass, volume, mass and
+//     volume centres on the 6x6 rule;
+//   getgeoelho_initial_lag (:1917-2090): the second moments about both
+//     centres;
+//   getunkno_initial_lag (:478-534): the mass matrix of the degree >= 1
+//     block on the 5x5 rule, every entry accumulated as written there, and
+//     its inverse by getinvmat (dgflo2d.f90:2792-2908: ludeco without
+//     pivoting, then lureso per unit column), stored as amatr(1:15) = the
+//     inverse's upper triangle row by row, amatr(16) = 1 / mass.
+// The state is piecewise constant per cell: in = [rho; u; v; p; blast
+// energy; blast flag] x nc.  The DOFs are tau = (5x5 volume) / mass, u, v,
+// E = p / ((gamma - 1) rho) + |u|^2 / 2 (E = energy / mass in a blast
+// cell), higher DOFs 0, as the host initialisation sets them.
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_init_q2(Dev d, const double *__restrict__ in, double *A)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc;
+    if (ie >= nc) return;
+    const double rho = in[ie];
+    double xp[9], yp[9];
+    for (int k = 0; k < 9; ++k) {
+        const int p = d.ipq[k * nc + ie];
+        xp[k] = d.xn[p];
+        yp[k] = d.yn[p];
+    }
+    // ---- getgeoel_initial_lag
+    double masel = 0.0, volel = 0.0, xcent = 0.0, ycent = 0.0, xcel = 0.0, ycel = 0.0;
+    for (int g = 0; g < 36; ++g) {
+        const int i = g / 6, j = g - 6 * i;
+        const double r = PG6[i], s = PG6[j];
+        const double djaco = djaco_q2(xp, yp, r, s, WG6[i] * WG6[j]);
+        masel = masel + rho * djaco;
+        volel = volel + djaco;
+        xcent = xcent + rho * r * djaco;
+        ycent = ycent + rho * s * djaco;
+        xcel = xcel + djaco * r;
+        ycel = ycel + djaco * s;
+    }
+    const double rc = xcent / masel, sc = ycent / masel;
+    const double rci = xcel / volel, sci = ycel / volel;
+    // ---- getgeoelho_initial_lag
+    double bq4 = 0.0, bq5 = 0.0, bq6 = 0.0, bi4 = 0.0, bi5 = 0.0, bi6 = 0.0;
+    for (int g = 0; g < 36; ++g) {
+        const int i = g / 6, j = g - 6 * i;
+        const double r = PG6[i], s = PG6[j];
+        const double djaco = djaco_q2(xp, yp, r, s, WG6[i] * WG6[j]);
+        const double b2 = (r - rc) / 1.0, b3 = (s - sc) / 1.0;
+        bq4 = bq4 + 0.5 * rho * b2 * b2 * djaco;
+        bq5 = bq5 + 0.5 * rho * b3 * b3 * djaco;
+        bq6 = bq6 + rho * b2 * b3 * djaco;
+        const double c2 = (r - rci) / 1.0, c3 = (s - sci) / 1.0;
+        bi4 = bi4 + 0.5 * c2 * c2 * djaco;
+        bi5 = bi5 + 0.5 * c3 * c3 * djaco;
+        bi6 = bi6 + c2 * c3 * djaco;
+    }
+    const double g18 = bq4 / masel, g19 = bq5 / masel, g20 = bq6 / masel;
+    // ---- getunkno_initial_lag: mass matrix a(i, j) at a[i + 5 j]
+    double a[25], f0 = 0.0, vol25 = 0.0;
+    for (int k = 0; k < 25; ++k) a[k] = 0.0;
+    for (int g = 0; g < 25; ++g) {
+        const int i = g / 5, j = g - 5 * i;
+        const double r = PG5I[i], s = PG5I[j];
+        const double djaco = djaco_q2(xp, yp, r, s, WG5I[i] * WG5I[j]);
+        double b[5];
+        b[0] = (r - rc) / 1.0;
+        b[1] = (s - sc) / 1.0;
+        b[2] = 0.5 * b[0] * b[0] - g18;
+        b[3] = 0.5 * b[1] * b[1] - g19;
+        b[4] = b[0] * b[1] - g20;
+        f0 = f0 + rho * djaco;
+        for (int mx = 0; mx < 5; ++mx)
+            for (int my = 0; my < 5; ++my) a[mx + 5 * my] = a[mx + 5 * my] + rho * b[mx] * b[my] * djaco;
+        vol25 = vol25 + djaco;
+    }
+    // ludeco (dgflo2d.f90:2826-2864)
+    for (int k = 0; k < 5; ++k) {
+        for (int m = 0; m < k; ++m) a[k + 5 * k] = a[k + 5 * k] - a[k + 5 * m] * a[m + 5 * k];
+        const double adicv = 1.0 / a[k + 5 * k];
+        for (int i = k + 1; i < 5; ++i) {
+            for (int m = 0; m < k; ++m) {
+                a[k + 5 * i] = a[k + 5 * i] - a[k + 5 * m] * a[m + 5 * i];
+                a[i + 5 * k] = a[i + 5 * k] - a[i + 5 * m] * a[m + 5 * k];
+            }
+            a[i + 5 * k] = a[i + 5 * k] * adicv;
+        }
+    }
+    // lureso for each unit column (dgflo2d.f90:2872-2908); x5(row, col)
+    double x5[25];
+    for (int col = 0; col < 5; ++col) {
+        double *x = &x5[5 * col];
+        for (int i = 0; i < 5; ++i) {
+            double c = 0.0;
+            for (int j = 0; j < i; ++j) c = c + a[i + 5 * j] * x[j];
+            x[i] = (i == col ? 1.0 : 0.0) - c;
+        }
+        for (int i = 4; i >= 0; --i) {
+            double c = 0.0;
+            for (int j = i + 1; j < 5; ++j) c = c + a[i + 5 * j] * x[j];
+            x[i] = (x[i] - c) / a[i + 5 * i];
+        }
+    }
+    int k = 0;
+    for (int r = 0; r < 5; ++r)
+        for (int c = r; c < 5; ++c) A[(k++) * nc + ie] = x5[r + 5 * c];
+    A[15 * nc + ie] = 1.0 / f0;
+    // ---- geometry rows and DOFs
+    for (int row = 0; row < 24; ++row) d.G[row * nc + ie] = 0.0;
+    d.G[0 * nc + ie] = rc;
+    d.G[1 * nc + ie] = sc;
+    d.G[2 * nc + ie] = volel;
+    d.G[3 * nc + ie] = masel;
+    d.G[4 * nc + ie] = rci;
+    d.G[5 * nc + ie] = sci;
+    d.G[6 * nc + ie] = rci;
+    d.G[7 * nc + ie] = sci;
+    d.G[18 * nc + ie] = g18;
+    d.G[19 * nc + ie] = g19;
+    d.G[20 * nc + ie] = g20;
+    d.G[21 * nc + ie] = bi4 / volel;
+    d.G[22 * nc + ie] = bi5 / volel;
+    d.G[23 * nc + ie] = bi6 / volel;
+    const double u0 = in[nc + ie], v0 = in[2 * nc + ie], p0 = in[3 * nc + ie];
+    const double g1 = d.gamlg - 1.0;
+    double e0 = p0 / (g1 * rho) + 0.5 * (u0 * u0 + v0 * v0);
+    if (in[5 * nc + ie] != 0.0) e0 = in[4 * nc + ie] / masel;
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 6; ++id) d.Un[(q * 6 + id) * nc + ie] = 0.0;
+    d.Un[(0 * 6) * nc + ie] = vol25 / masel;
+    d.Un[(1 * 6) * nc + ie] = u0;
+    d.Un[(2 * 6) * nc + ie] = v0;
+    d.Un[(3 * 6) * nc + ie] = e0;
+}
+
 }  // namespace lag2d

@@ -55,7 +55,23 @@ def concatenate(cases):
 class Ensemble:
     """Several runs resident on the GPU, advanced together."""
 
-    def __init__(self, cases, tend=None):
+    @classmethod
+    def from_states(cls, cases, states, tend=None):
+        """An ensemble initialised on the device (curved P2 path): ``cases``
+        give the meshes, ``states`` the per-run (rho, u, v, p, blast_cells,
+        blast_energies) of ``init.initial_state``."""
+        ens = cls(cases, tend=tend, upload=False)
+        off = ens.cell_off
+        cat = [np.concatenate([st[k] for st in states]) for k in range(4)]
+        cells = np.concatenate([np.asarray(st[4], dtype=np.int64) + off[r] for r, st in enumerate(states)])
+        energy = np.concatenate([np.asarray(st[5], dtype=np.float64) for st in states])
+        a = [np.ascontiguousarray(t, dtype=np.float64) for t in (ens.case.coord, *cat)]
+        b = np.ascontiguousarray(cells + 1, dtype=np.int32)
+        e = np.ascontiguousarray(energy, dtype=np.float64)
+        _check(load().lag2d_init_state(ens.h, *[_p(t) for t in a], len(b), _p(b, _ip), _p(e)))
+        return ens
+
+    def __init__(self, cases, tend=None, upload=True):
         lib = load()
         self.cases = list(cases)
         self.nrun = len(self.cases)
@@ -82,6 +98,8 @@ class Ensemble:
                                          _p(bf, _ip), _p(e1, _ip), _p(e2, _ip), C.byref(h)))
         self.h = h
         self.itime = 0
+        if not upload:
+            return
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
                 (k.coord, k.coold, k.geoel, k.amatr, k.unkno, k.ustar)]
         _check(lib.lag2d_upload(self.h, *[_p(a) for a in arrs]))

@@ -44,6 +44,15 @@ module lag2d_c
       real(c_double), intent(in) :: coord(*), coold(*), geoel(*), amatr(*), unkno(*), ustar(*)
     end function lag2d_upload
 
+    integer(c_int) function lag2d_init_state(ctx, coord, rho, u, v, p, nblast, blast, energy) &
+        bind(c, name='lag2d_init_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: coord(*), rho(*), u(*), v(*), p(*), energy(*)
+      integer(c_int), value :: nblast
+      integer(c_int), intent(in) :: blast(*)
+    end function lag2d_init_state
+
     integer(c_int) function lag2d_step(ctx, itime) bind(c, name='lag2d_step')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

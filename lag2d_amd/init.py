@@ -80,6 +80,34 @@ def _jac(xp, r, s):
     return x, dxdr * dyds - dydr * dxds
 
 
+def _cell_states(mesh: mg.Mesh, ctl: mg.Control, density, state):
+    """Connectivity, coordinates, geometry points and the piecewise
+    constant state of every cell, evaluated at its centre node."""
+    ipq, bf, e1, e2 = mesh_arrays(mesh)
+    coord = mesh.coord.astype(np.float64)
+    nc = ipq.shape[0]
+    xp = geometry_points(coord, ipq, ctl.ncurv)
+    xc, yc = xp[:, 8, 0], xp[:, 8, 1]
+    rho = np.asarray(density(xc, yc), dtype=np.float64) * np.ones(nc)
+    u0, v0, p0 = (np.asarray(a, dtype=np.float64) * np.ones(nc) for a in state(xc, yc))
+    return ipq, bf, e1, e2, coord, xp, rho, u0, v0, p0
+
+
+def initial_state(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None):
+    """Inputs of the device initialisation (``Lag2D.from_state``, the C
+    entry lag2d_init_state): the case with its mesh and zero-filled state
+    arrays, and ``(rho, u, v, p, blast_cells, blast_energies)`` per cell as
+    ``initialise`` evaluates them (``blast`` = (cells, energy) as there)."""
+    ipq, bf, e1, e2, coord, xp, rho, u0, v0, p0 = _cell_states(mesh, ctl, density, state)
+    nc = ipq.shape[0]
+    case = LagCase(ctl=ctl, ipqua=ipq, bface=bf, esuv1=e1, esuv2=e2, coord=coord.copy(), coold=coord.copy(),
+                   geoel=np.zeros((nc, 24)), amatr=np.zeros((nc, 16 if ctl.npoly == 2 else 4)),
+                   unkno=np.zeros((nc, 4, ctl.ndegr)), ustar=np.zeros_like(coord), rho0=rho.copy())
+    cells = np.asarray(blast[0] if blast is not None else [], dtype=np.int64)
+    energy = np.full(len(cells), float(blast[1]) if blast is not None else 0.0)
+    return case, (rho, u0, v0, p0, cells, energy)
+
+
 def initialise(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) -> LagCase:
     """Initial arrays of the Lagrangian step.
 
@@ -88,13 +116,8 @@ def initialise(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) -> La
     states by centre, Getrhslagmc.f90:5527-5841); ``state(xc, yc)`` returns
     (u, v, p) per cell.  ``blast`` = (cells, energy) puts total energy
     ``energy`` into each listed cell (Sedov)."""
-    ipq, bf, e1, e2 = mesh_arrays(mesh)
-    coord = mesh.coord.astype(np.float64)
+    ipq, bf, e1, e2, coord, xp, rho, u0, v0, p0 = _cell_states(mesh, ctl, density, state)
     nc = ipq.shape[0]
-    xp = geometry_points(coord, ipq, ctl.ncurv)
-    xc, yc = xp[:, 8, 0], xp[:, 8, 1]
-    rho = np.asarray(density(xc, yc), dtype=np.float64) * np.ones(nc)
-    u0, v0, p0 = (np.asarray(a, dtype=np.float64) * np.ones(nc) for a in state(xc, yc))
 
     r6, s6, w6 = _rule(_G6, _W6)
     _, dj6 = _jac(xp, r6, s6)
@@ -344,6 +367,19 @@ def sedov_case(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None) -> LagCa
     cells = mg.centre_cells(nx, nx) if centre else (0,)
     return initialise(mesh, ctl, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6),
                       blast=(cells, mg.SEDOV_ENERGY))
+
+
+def sedov_state(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None):
+    """``sedov_case``'s problem as inputs of the device initialisation
+    (see ``initial_state``)."""
+    mesh, ctl = mg.sedov_problem(nx, seed=seed)
+    if cdrho is not None:
+        ctl.cdrho = cdrho
+    if cimpd is not None:
+        ctl.cimpd = cimpd
+    cells = mg.centre_cells(nx, nx) if centre else (0,)
+    return initial_state(mesh, ctl, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6),
+                         blast=(cells, mg.SEDOV_ENERGY))
 
 
 def sod_case(nx=100, ny=4) -> LagCase:

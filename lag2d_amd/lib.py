@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liblag2d.so")
 
 EXPORTS = ("lag2d_create", "lag2d_create_hybrid", "lag2d_create_ensemble", "lag2d_get_run_times", "lag2d_set_initial_density", "lag2d_weno_cells",
-           "lag2d_upload", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
+           "lag2d_upload", "lag2d_init_state", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
            "lag2d_division_check", "lag2d_get_mass_matrix", "lag2d_adj_record", "lag2d_adj_replay",
            "lag2d_adj_gradient", "lag2d_use_graphs", "lag2d_destroy", "lag2d_last_error")
@@ -60,6 +60,7 @@ def load():
     lib.lag2d_weno_cells.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     lib.lag2d_division_check.argtypes = [C.c_int, _dp, _dp, _dp]
     lib.lag2d_upload.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.lag2d_init_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _dp]
     lib.lag2d_step.argtypes = [C.c_void_p, C.c_int]
     lib.lag2d_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip]
     lib.lag2d_rhs.argtypes = [C.c_void_p, _dp]
@@ -102,7 +103,7 @@ def config_of(case: LagCase, strict: bool = False) -> Config:
 class Lag2D:
     """One problem resident on the GPU, driven through the C ABI."""
 
-    def __init__(self, case: LagCase, strict: bool = False, rho0=None):
+    def __init__(self, case: LagCase, strict: bool = False, rho0=None, upload: bool = True):
         lib = load()
         self.case = case
         self.itime = case.itime
@@ -125,10 +126,28 @@ class Lag2D:
             _check(lib.lag2d_create(C.byref(cfg), case.nquad, case.npoin, case.nbfac, _p(ipq, _ip), _p(bf, _ip),
                                     _p(e1, _ip), _p(e2, _ip), C.byref(h)))
         self.h = h
+        if not upload:
+            return
         self.upload(case)
         if rho0 is not None:
             r = np.ascontiguousarray(rho0, dtype=np.float64)
             _check(lib.lag2d_set_initial_density(self.h, _p(r)))
+
+    @classmethod
+    def from_state(cls, case: LagCase, states, strict: bool = False):
+        """A context initialised on the device (curved P2 path) from the
+        mesh of ``case`` and ``states`` = (rho, u, v, p, blast_cells,
+        blast_energies) per cell, as ``init.initial_state`` returns them."""
+        g = cls(case, strict=strict, upload=False)
+        g.init_state(states)
+        return g
+
+    def init_state(self, states):
+        rho, u, v, p, cells, energy = states
+        a = [np.ascontiguousarray(t, dtype=np.float64) for t in (self.case.coord, rho, u, v, p)]
+        b = np.ascontiguousarray(np.asarray(cells) + 1, dtype=np.int32)
+        e = np.ascontiguousarray(energy, dtype=np.float64)
+        _check(load().lag2d_init_state(self.h, *[_p(t) for t in a], len(b), _p(b, _ip), _p(e)))
 
     def upload(self, case: LagCase):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
