@@ -194,12 +194,14 @@ def save_checkpoint(path, case: LagCase, state: dict | None = None, time: tuple 
                     itime: int | None = None) -> None:
     """Everything a Lagrangian restart needs, coordinates included: the case
     (mesh, control, initial coordinates and density) and the current
-    ``state`` (``Lag2D.download()``) with ``time`` = (dt, t)."""
+    ``state`` (``Lag2D.download()``, with ``amatr`` = ``Lag2D.mass_matrix()``
+    where the RZ branch rewrites it) with ``time`` = (dt, t)."""
     st = state or {}
     dt, t = time if time is not None else (case.dtfix, case.tacru)
     arrs = dict(ipqua=case.ipqua, bface=case.bface, esuv1=case.esuv1, esuv2=case.esuv2,
                 coord=st.get("coord", case.coord), coold=case.coold, geoel=st.get("geoel", case.geoel),
-                amatr=case.amatr, unkno=st.get("unkno", case.unkno), ustar=st.get("ustar", case.ustar),
+                amatr=st.get("amatr", case.amatr), unkno=st.get("unkno", case.unkno),
+                ustar=st.get("ustar", case.ustar),
                 dtfix=np.float64(dt), tacru=np.float64(t), itime=np.int64(case.itime if itime is None else itime),
                 ctl=np.array(list(vars(case.ctl).items()), dtype=object).astype(str))
     if case.rho0 is not None:
