@@ -132,6 +132,39 @@ def batched_gain(args):
                         f"[1000, 40000], 5% perturbations of rho and p from 8 Fourier modes; each run its own dt"}
 
 
+def scaled_step(nx, steps=10, warmup=3):
+    """The same Sedov P2 problem on an nx x nx mesh, initialised on the
+    device: its step time and both roofline fractions of the whole step.  At
+    1200 x 1200 (1.44 M cells, about 5 GB) the state no longer fits the
+    256 MB Infinity Cache, so the HBM fraction means what it says."""
+    import torch
+    from lag2d_amd import init, roofline
+    from lag2d_amd.lib import Lag2D
+    case, states = init.sedov_state(nx, seed=2024, centre=True, cdrho=1.0)
+    g = Lag2D.from_state(case, states)
+    for _ in range(warmup):
+        g.step_async()
+    g.sync()
+    g.timing(True)   # resets the troubled-cell count ...
+    g.timing(False)  # ... and leaves the launches untimed
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step_async()
+    g.sync()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ok = g.check() == 0
+    weno = g.weno_cells() / (3.0 * steps)
+    g.close()
+    ms = 1e3 * t / steps
+    out = {"workload": f"sedov_p2_curved_{nx}x{nx}", "cells": case.ncell, "nodes": case.npoin, "steps": steps,
+           "warmup": warmup, "ms_per_step": ms, "value": case.ncell * 6 * 4 * steps / t,
+           "unit": "cell-DOF updates/s", "state_valid": ok}
+    out.update(roofline.step_figures(case, ms, weno))
+    return out
+
+
 def cpu_baseline(args, dofs_per_step):
     """The reference on the host's cores when its build is present, else the C
     restatement of the step (oracle/), both single-threaded."""
@@ -171,6 +204,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--scaled-nx", type=int, default=1200,
+                    help="mesh of the scaled step-roofline line (0: skip); device-initialised")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="finish with one device-to-device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -223,12 +258,14 @@ def main():
     # counts from the op-counting oracle, units of this run)
     from lag2d_amd import roofline
     rf = roofline.figures(name, case, avg_ms, weno_cells / max(weno_launches, 1))
+    step_rf = roofline.step_figures(case, 1e3 * elapsed / args.steps, weno_cells / max(weno_launches, 1))
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args, dofs_per_step)
     g.close()
     batched = None if args.no_batched else batched_gain(args)
+    scaled = scaled_step(args.scaled_nx) if args.scaled_nx else None
     adj = None
     if not args.no_adjoint:
         from lag2d_amd import adjoint
@@ -256,6 +293,8 @@ def main():
                    "nodes": case.npoin, "rk_stages": 3, "cfl": case.ctl.cfl, "limiter": "Persson+WENO",
                    "sim_time_reached": tnow},
         "roofline": rf,
+        "step_roofline": step_rf,
+        "scaled": scaled,
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adj,

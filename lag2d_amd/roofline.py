@@ -98,3 +98,33 @@ def figures(kernel: str, case, avg_ms: float, weno_cells_per_launch: float = 0.0
     achieved = nbytes / t / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS, **common}
+
+
+def step_figures(case, ms_per_step: float, weno_cells_per_stage: float) -> dict:
+    """Both roofline fractions of a whole step of the curved P2 path.
+
+    Compulsory bytes per step with S stages (constants read once per stage,
+    the time-invariant sub-cell masses precomputed):
+      S [nc 8 (2 nd nq + nmatr + 6 + 4) + np 8 (2 2 + 2 2) + nc 4 (9 + 9)]
+      + (S - 1) [nc 8 nd nq + np 16]
+    (U read and written, M^-1, 6 geometry constants, 4 sub-cell masses; x
+    read and written, u* written and read; Q2 connectivity and the node CSR;
+    the RK blend's reads of U^n and x^n).  Algorithmic flops per step: the
+    per-unit counts above over S stages, the troubled cells' WENO, and the
+    stage-1 update and dt.  achieved = T_roof / T_step with T_roof =
+    max(bytes / HBM peak, flops / FP64 peak)."""
+    S, nc, nd, nq, npn, nm = case.ctl.nstag, case.ncell, case.ctl.ndegr, 4, case.npoin, 16
+    nbytes = S * (nc * 8 * (2 * nd * nq + nm + 6 + 4) + npn * 8 * (2 * 2 + 2 * 2) + nc * 4 * (9 + 9)) \
+        + (S - 1) * (nc * 8 * nd * nq + npn * 16)
+    per_cell = sum(FLOPS_Q2[k] for k in ("k_q2_prep", "k_q2_corner", "k_cellrhs_q2", "k_cellvol"))
+    per_node = FLOPS_Q2["k_node_q2"] + FLOPS_Q2["k_nodemove"]
+    flops = S * (nc * per_cell + npn * per_node + weno_cells_per_stage * FLOPS_Q2["k_q2_weno"]) \
+        + nc * (FLOPS_Q2["k_update"] + 81.0)
+    t = ms_per_step * 1e-3
+    t_roof = max(nbytes / (PEAK_HBM_GBS * 1e9), flops / (PEAK_FP64_TFLOPS * 1e12))
+    return {"bytes_per_step": nbytes, "flops_per_step": flops,
+            "hbm_GBs": nbytes / t / 1e9, "hbm_frac": nbytes / t / (PEAK_HBM_GBS * 1e9),
+            "fp64_TFs": flops / t / 1e12, "fp64_frac": flops / t / (PEAK_FP64_TFLOPS * 1e12),
+            "achieved": t_roof / t, "bound": "hbm" if nbytes / (PEAK_HBM_GBS * 1e9) >= flops / (
+                PEAK_FP64_TFLOPS * 1e12) else "fp64",
+            "weno_cells_per_stage": weno_cells_per_stage}
