@@ -14,7 +14,11 @@ together, against the same runs advanced one after another.  The "adjoint"
 entry runs the discrete adjoint's configuration (Sod-type tube, P1 on
 200x200 quads, 400 SSP-RK2 steps, checkpoints every 20): the cost of the
 gradient against the forward run, and its agreement with central
-differences of the frozen forward.
+differences of the frozen forward.  "roofline" is the dominant kernel's;
+"step_roofline" gives both fractions of the whole step (compulsory bytes
+and algorithmic flops per step), and "scaled" the same problem at
+1200x1200, initialised on the device, where the state no longer fits the
+Infinity Cache.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 120]
 """
