@@ -333,6 +333,13 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         const int k = std::atoi(v);
         if (k == 1 || k == 2) x->corner_variant = k;
     }
+    // the update reads each cell's M^-1 and residual once where the state
+    // outgrows the caches (LAG2D_UPDATE_VARIANT = 1 per DOF, 2 per cell)
+    x->update_cells = nc >= (1 << 18);
+    if (const char *v = std::getenv("LAG2D_UPDATE_VARIANT")) {
+        const int k = std::atoi(v);
+        if (k == 1 || k == 2) x->update_cells = k == 2;
+    }
     if (const char *v = std::getenv("LAG2D_RHS_VARIANT")) {
         const int k = std::atoi(v);
         if (k == 1 || k == 2) x->rhs_variant = k;
@@ -618,7 +625,10 @@ static int step_impl(lag2d_ctx *x, int itime)
         if (s == 0) {
             LAUNCH(KID_DT_PARTIAL, k_dt_partial, x->nrun * x->nblk_dt, ds, Uin, x->nblk_dt);
             LAUNCH(KID_DT_FINAL, k_dt_final, x->nrun, d, itime, x->nblk_dt);
-            LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
+            if (x->update_cells)
+                LAUNCH(KID_UPDATE, k_update_cells, nblocks(4 * nc), d, alfa);
+            else
+                LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
         }
         LAUNCH(KID_NODEMOVE, k_nodemove, nblocks(np), d, alfa, s == 0 ? d.xn : d.xs, s == 0 ? d.yn : d.ys);
         // the volumes of the stage coordinates: the curved path's centre
@@ -704,7 +714,10 @@ static int adj_fwd_step(lag2d_ctx *x, int itime, int n, int mode, int k)
             } else {
                 LAUNCH(KID_DT_FINAL, k_replay_dt, 1, d, A.dt_tape + n);
             }
-            LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
+            if (x->update_cells)
+                LAUNCH(KID_UPDATE, k_update_cells, nblocks(4 * nc), d, alfa);
+            else
+                LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
         }
         LAUNCH(KID_NODEMOVE, k_nodemove, nblocks(np), d, alfa, s == 0 ? d.xn : d.xs, s == 0 ? d.yn : d.ys);
     }

@@ -122,7 +122,9 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_division_check(const double *a,
 
 // Mass-matrix solve and SSP-RK blend (Getrhslag.f90:1822-1904):
 // U_s = alfa U^n + (1 - alfa)(U_s + M^-1 dt R), one thread per DOF (q, id)
-// of a cell, cells fastest so that every access is coalesced.
+// of a cell, cells fastest so that every access is coalesced.  For meshes
+// that stay in the caches; k_update_cells below reads each cell's M^-1
+// and residual from HBM once, for meshes that do not.
 __global__ void __launch_bounds__(LAG2D_BLOCK) k_update(Dev d, double alfa)
 {
     const int nc = d.nc, nd = d.nd;
@@ -161,6 +163,61 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_update(Dev d, double alfa)
     double s4 = 0.0;
     for (int k = 0; k < nd; ++k) s4 = s4 + mrow[k] * (dt * d.R[(q * nd + k) * nc + ie]);
     d.Us[o] = alfa * d.Un[o] + (1.0 - alfa) * (d.Us[o] + s4);
+}
+
+
+// The same update, one thread per (cell,
+// variable), the four variables of a cell in adjacent lanes: a cell's M^-1
+// and residual are read from HBM once for all its DOFs (the lanes of a
+// cell share the lines), and each lane reads its variable's rows for 16
+// consecutive cells of a wave.
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_update_cells(Dev d, double alfa)
+{
+    const int nc = d.nc, nd = d.nd;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ie = t >> 2, q = t & 3;
+    if (ie >= nc) return;
+    const int run = cell_run(d, ie);
+    if (!run_active(d, run)) {
+        for (int id = 0; id < nd; ++id) d.Us[(q * nd + id) * nc + ie] = d.Un[(q * nd + id) * nc + ie];
+        return;
+    }
+    const double dt = d.scal[2 * run];
+    // the inverse mass matrix: degree 0 decoupled (1/mass), the rest packed
+    // as the upper triangle row-wise
+    const int nm = nd == 3 ? 4 : 16;
+    double a[16];
+    for (int k = 0; k < nm; ++k) a[k] = d.A[k * nc + ie];
+    {
+        double r[6];
+        for (int k = 0; k < nd; ++k) r[k] = d.R[(q * nd + k) * nc + ie];
+        for (int id = 0; id < nd; ++id) {
+            double mrow[6];
+            if (nd == 3) {
+                if (id == 0) {
+                    mrow[0] = a[3]; mrow[1] = 0.0; mrow[2] = 0.0;
+                } else if (id == 1) {
+                    mrow[0] = 0.0; mrow[1] = a[0]; mrow[2] = a[1];
+                } else {
+                    mrow[0] = 0.0; mrow[1] = a[1]; mrow[2] = a[2];
+                }
+            } else {
+                const int off[6] = {0, 0, 5, 9, 12, 14};
+                for (int k = 0; k < 6; ++k) {
+                    if (id == 0 || k == 0) {
+                        mrow[k] = (id == 0 && k == 0) ? a[15] : 0.0;
+                    } else {
+                        const int lo = id < k ? id : k, hi = id < k ? k : id;
+                        mrow[k] = a[off[lo] + (hi - lo)];
+                    }
+                }
+            }
+            double s4 = 0.0;
+            for (int k = 0; k < nd; ++k) s4 = s4 + mrow[k] * (dt * r[k]);
+            const int o = (q * nd + id) * nc + ie;
+            d.Us[o] = alfa * d.Un[o] + (1.0 - alfa) * (d.Us[o] + s4);
+        }
+    }
 }
 
 // x_s = alfa x^n + (1 - alfa)(x_src + dt u*)  (Getrhslag.f90:1954-1956)

@@ -51,6 +51,7 @@ struct lag2d_ctx {
     float corner_ms[2] = {0.f, 0.f};  // tuning timings of the two variants
     int rhs_variant = 0;    // curved-path cell RHS: 0 = not tuned yet, 1 = 8 cells per block, 2 = thread per cell
     float rhs_ms[2] = {0.f, 0.f};
+    bool update_cells = false;  // k_update_cells (large meshes) instead of k_update
     std::vector<double> tend_host;
     hipStream_t stream = nullptr;
     lag2d::Dev d{};

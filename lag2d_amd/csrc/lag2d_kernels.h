@@ -18,6 +18,7 @@ __global__ void k_dt_final(Dev d, int itime, int nbpr);
 __global__ void k_run_flags(Dev d);
 __global__ void k_division_check(const double *a, const double *b, double *q, int n);
 __global__ void k_update(Dev d, double alfa);
+__global__ void k_update_cells(Dev d, double alfa);
 __global__ void k_nodemove(Dev d, double alfa, const double *__restrict__ xsrc, const double *__restrict__ ysrc);
 template <int RZ>
 __global__ void k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y, int set_centre);
