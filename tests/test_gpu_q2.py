@@ -78,3 +78,32 @@ def test_corner_blast_strict_reports_topology():
     g = _lag(case, strict=True)
     with pytest.raises(Lag2DError, match="error 6"):
         g.step(2)
+
+
+def test_benchmark_size_matches_oracle_bitwise():
+    """The benchmark problem at its full size (120x120 curved P2 Sedov,
+    blast energy in the four centre cells, cdrho = 1; troubled cells and
+    WENO from the first step): 3 steps bit-identical to the oracle, then
+    150 more steps conserve the total energy sum_cells m E to round-off (slip
+    walls do no work) and keep the state finite with positive volumes."""
+    from oracle import Oracle
+    from lag2d_amd import init
+    case = init.sedov_case(120, seed=2024, centre=True, cdrho=1.0)
+    g = _lag(case)
+    g.step(3)
+    st = g.download()
+    o = Oracle(case)
+    o.step(3)
+    err = state_errors(st["unkno"], st["coord"], o.unkno, o.coord)
+    print("120x120 benchmark problem, 3 steps vs oracle:", err, flush=True)
+    assert np.array_equal(st["unkno"], o.unkno) and np.array_equal(st["coord"], o.coord)
+    assert np.array_equal(st["ustar"], o.ustar)
+    mass = 1.0 / case.amatr[:, 15]
+    e0 = float(np.sum(mass * case.unkno[:, 3, 0]))
+    g.step(150)
+    assert g.check() == 0
+    st = g.download()
+    e1 = float(np.sum(mass * st["unkno"][:, 3, 0]))
+    print(f"total energy {e0!r} -> {e1!r}, weno cells so far: {g.weno_cells()}", flush=True)
+    assert abs(e1 - e0) <= 1e-12 * abs(e0)
+    assert np.all(np.isfinite(st["unkno"])) and np.all(st["geoel"][:, 2] > 0)
