@@ -78,7 +78,8 @@ struct Dev {
     double *scal;            // device scalars: 0 dtfix, 1 tacru
     // curved P2 sub-cell path
     double *msub;            // [4][nc] initial sub-cell masses (time-invariant)
-    double *Ul;              // limited stage state (after WENO) [(q*nd+id)*nc+ie]
+    double *Ul;              // limited DOFs of the cells WENO limits (flag set) [(q*nd+id)*nc+ie]
+    const double *Ui;        // the stage's input state: the DOFs of every other cell
     double *gq;              // face geometry [48][nc]: slot j -> (nx, ny, |J|) rows 3j..3j+2
     uint8_t *flag;           // Persson troubled flag per cell
     int *tlist;              // compacted troubled-cell list

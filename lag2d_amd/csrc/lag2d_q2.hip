@@ -145,6 +145,13 @@ __device__ __forceinline__ double rho0(int ncase, double xc, double yc)
 
 #define UD(buf, q, id, ie) (buf)[((q) * 6 + (id)) * nc + (ie)]
 
+// The limited DOFs of cell ie: WENO's output where it limited the cell (the
+// troubled flag it leaves set), the stage input everywhere else.
+__device__ __forceinline__ const double *cell_dofs(const Dev &d, int ie)
+{
+    return d.flag[ie] ? d.Ul : d.Ui;
+}
+
 // ---------------------------------------------------------------- prep --
 // Copies the stage DOFs into the limited buffer, evaluates the Persson
 // indicator on tau and appends troubled cells to the WENO list.  8 lanes
@@ -155,10 +162,6 @@ __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict
 {
     __shared__ double sv[PREP_CELLS][2 * 25 + 1];
     const int nc = d.nc, ie0 = blockIdx.x * PREP_CELLS;
-    for (int e = threadIdx.x; e < 24 * PREP_CELLS; e += 256) {
-        const int qi = e / PREP_CELLS, c = e - PREP_CELLS * qi, ie = ie0 + c;
-        if (ie < nc) d.Ul[qi * nc + ie] = Uin[qi * nc + ie];
-    }
     const int cl = threadIdx.x >> 3, lane = threadIdx.x & 7, ie = ie0 + cl;
     if (ie < nc) {
         double u0[6];
@@ -498,7 +501,10 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         {
             const int flags = S.top[9];
             if (tid == 0 && (flags & 2)) atomicAdd(d.wenobad, 1);
-            if (flags & 1) continue;
+            if (flags & 1) {  // not limited: the later kernels read the stage input
+                if (tid == 0) d.flag[ie] = 0;
+                continue;
+            }
         }
         const double m[4] = {S.m[0], S.m[1], S.m[2], S.m[3]};
         int ielaj[9];
@@ -1003,6 +1009,8 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
             out[1] = x;
             out[2] = y;
             for (int q = 0; q < 4; ++q) UD(d.Ul, q, a + 1, ie) = out[q];
+            if (a == 0)
+                for (int q = 0; q < 4; ++q) UD(d.Ul, q, 0, ie) = UD(Uin, q, 0, ie);
         }
     }
 }
@@ -1115,7 +1123,7 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
         d.gq[(3 * lane + 2) * nc + ie] = ga;
         // ---- quadrature terms of sub-cell isg: area of its Q2 map and the
         //      mass-weighted mean specific volume of the DG polynomial
-        for (int id = 0; id < 6; ++id) u0[id] = UD(d.Ul, 0, id, ie);
+        for (int id = 0; id < 6; ++id) u0[id] = UD(cell_dofs(d, ie), 0, id, ie);
         // nodes of sub-cell isg: lane `part` stores corner `part` and maps
         // mid-node `part`; the sub-cell's four lanes form a DPP quad, which
         // shares the mid-nodes for the centre node
@@ -1201,7 +1209,7 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
     const double uq = volel / masel;
     double u[4][6];
     for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) u[q][id] = UD(d.Ul, q, id, ie);
+        for (int id = 0; id < 6; ++id) u[q][id] = UD(cell_dofs(d, ie), q, id, ie);
     const int ln = IPQSG[isg][iv];
     double un[4];
     {
@@ -1357,7 +1365,7 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
         // ---- sub-cell isg: area of its Q2 map and the mass-weighted mean
         //      specific volume of the DG polynomial, summed in point order
         double u0[6];
-        for (int id = 0; id < 6; ++id) u0[id] = UD(d.Ul, 0, id, ie);
+        for (int id = 0; id < 6; ++id) u0[id] = UD(cell_dofs(d, ie), 0, id, ie);
         const double pc[12][2] = {{-0.5, -1.0}, {0.5, -1.0}, {1.0, -.5}, {1.0, .5}, {0.5, 1.0}, {-0.5, 1.0},
                                   {-1.0, .5},   {-1.0, -.5}, {0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
         const int ipsc[4][4] = {{0, 8, 11, 7}, {1, 2, 9, 8}, {9, 3, 4, 10}, {11, 10, 5, 6}};
@@ -1394,7 +1402,7 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
     const double uq = volel / masel;
     double u[4][6];
     for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) u[q][id] = UD(d.Ul, q, id, ie);
+        for (int id = 0; id < 6; ++id) u[q][id] = UD(cell_dofs(d, ie), q, id, ie);
     const double rhomc = u[0][0], uctr = u[1][0], vctr = u[2][0], ectr = u[3][0];
     const double rhoct = 1.0 / rhomc;
     const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
@@ -1534,7 +1542,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
         else if (k < 18)
             v = d.ys[d.ipq[(k - 9) * nc + ie]];
         else if (k < 42)
-            v = d.Ul[(k - 18) * nc + ie];
+            v = cell_dofs(d, ie)[(k - 18) * nc + ie];
         else
             v = d.G[(k == 42 ? 0 : (k == 43 ? 1 : 18 + (k - 44))) * nc + ie];
         scl[cl][k] = v;
@@ -1619,8 +1627,9 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
     // operands of the update, loaded ahead of the sums
     double uqi = 0.0, uo = 0.0, un = 0.0, mrow[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (active) {
-        uqi = UD(d.Ul, q, id, ie);
-        if ((id == 1 || id == 2) && (q == 1 || q == 2)) uo = UD(d.Ul, 3 - q, id, ie);
+        const double *Ue = cell_dofs(d, ie);
+        uqi = UD(Ue, q, id, ie);
+        if ((id == 1 || id == 2) && (q == 1 || q == 2)) uo = UD(Ue, 3 - q, id, ie);
         if (apply_update) {
             un = UD(d.Un, q, id, ie);
             // row id of the unpacked inverse mass matrix (degree 0 decoupled;
@@ -1768,7 +1777,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
     {
         double u[4][6];
         for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < 6; ++id) u[q][id] = UD(d.Ul, q, id, ie);
+            for (int id = 0; id < 6; ++id) u[q][id] = UD(cell_dofs(d, ie), q, id, ie);
         double xp[9], yp[9];
         load9(d, d.xs, d.ys, ie, xp, yp);
 #pragma unroll 1
@@ -1807,7 +1816,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
     // ---- derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708)
     double ul[4][6];
     for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) ul[q][id] = UD(d.Ul, q, id, ie);
+        for (int id = 0; id < 6; ++id) ul[q][id] = UD(cell_dofs(d, ie), q, id, ie);
     for (int id = 1; id < 3; ++id) {
         const double u1 = ul[1][id], u2 = ul[2][id];
         ul[0][id] = ul[0][id] * 1.0;
@@ -2116,6 +2125,7 @@ static int tune_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update)
 int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int apply_update)
 {
     const int nc = x->nc, np = x->np;
+    ds.Ui = Uin;
     x->tic(KID_Q2_PREP);
     hipLaunchKernelGGL(k_q2_prep, dim3((nc + PREP_CELLS - 1) / PREP_CELLS), dim3(256), 0, x->stream, ds, Uin);
     x->toc(KID_Q2_PREP);
