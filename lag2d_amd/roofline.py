@@ -54,14 +54,14 @@ NODE_KERNELS = {"k_node_q2", "k_nodemove"}
 # WRITE_SIZE all of them (factor 1.0).  Source: profiles/counters_sedov_p2_120x120.txt
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 MEASURED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) KiB per dispatch, 120x120 benchmark only
-    "k_q2_prep": (2788.0, 2714.4),
-    "k_q2_weno": (617.6, 16.3),
-    "k_q2_corner": (4079.1, 23400.0),      # lanes-per-cell variant; lane-per-sub-cell: (3982.3, 25496.0)
-    "k_node_q2": (15197.7, 907.6),
-    "k_cellrhs_q2": (15978.2, 4053.8),     # 8-cells-per-block variant; thread-per-cell: (14910.1, 4050.1)
-    "k_update": (10317.0, 2700.0),
-    "k_nodemove": (1224.3, 907.6),
-    "k_cellvol": (1162.0, 768.5),
+    "k_q2_prep": (1773.9, 14.4),
+    "k_q2_weno": (623.1, 19.6),
+    "k_q2_corner": (4142.7, 23400.0),      # lanes-per-cell variant; lane-per-sub-cell: (4003.8, 25528.0)
+    "k_node_q2": (15197.1, 907.6),
+    "k_cellrhs_q2": (15993.6, 4052.8),     # 8-cells-per-block variant; thread-per-cell: (14909.7, 4050.1)
+    "k_update": (10316.5, 2700.0),
+    "k_nodemove": (1227.3, 907.6),
+    "k_cellvol": (1161.4, 768.5),
 }
 
 
