@@ -26,7 +26,7 @@ enum {
     LAG2D_EUNSUPPORTED = 2,  /* switch combination not implemented */
     LAG2D_EDEVICE = 3,       /* HIP runtime failure */
     LAG2D_ENONFINITE = 4,    /* non-finite state detected by lag2d_check */
-    LAG2D_EVOLUME = 5,       /* non-positive cell volume detected by lag2d_check */
+    LAG2D_EVOLUME = 5,       /* non-positive cell or specific volume detected by lag2d_check */
     LAG2D_ETOPOLOGY = 6      /* WENO stencil topology the reference stops on (strict) */
 };
 
@@ -138,7 +138,10 @@ int lag2d_set_initial_density(lag2d_ctx *ctx, const double *rho);
 /* One SSP-RK time step: getsolvdg_lag (Getrhslag.f90:1553-2035) followed by
  * the driver's copy of the stage state into unkno/coord
  * (dgflo2d.f90:408-422).  itime is the driver's 1-based step counter.
- * Asynchronous on the context's stream. */
+ * Asynchronous on the context's stream, except with cfg->strict on the
+ * curved path: that step synchronises and returns LAG2D_ETOPOLOGY if any
+ * troubled cell met a WENO neighbourhood the reference stops on (the flag is
+ * sticky: later steps of the context keep failing). */
 int lag2d_step(lag2d_ctx *ctx, int itime);
 
 /* nsteps steps (itime0, itime0+1, ...), stopping after the step that reaches
@@ -151,7 +154,8 @@ int lag2d_run(lag2d_ctx *ctx, int itime0, int nsteps, int *taken);
  * current state x^n / unkno, with ustar as the previous nodal velocity.
  * Writes rhsel(ndegr,4,ncell) and updates the resident ustar and the limited
  * DOFs (the reference limits its unkno argument in place,
- * Getrhslagmc.f90:7283-7296).  Synchronises. */
+ * Getrhslagmc.f90:7283-7296).  Synchronises.  Strict mode reports the WENO
+ * topology stop as lag2d_step does. */
 int lag2d_rhs(lag2d_ctx *ctx, double *rhsel);
 
 /* Download the state (same layouts as lag2d_upload); NULL pointers skip.
@@ -176,9 +180,11 @@ int lag2d_get_mass_matrix(lag2d_ctx *ctx, double *amatr);
  * single problem).  No counterpart in the reference. */
 int lag2d_get_run_times(lag2d_ctx *ctx, double *dtfix, double *tacru);
 
-/* Scan the resident state for non-finite DOFs or non-positive volumes:
- * returns LAG2D_ENONFINITE / LAG2D_EVOLUME (the reference has no such check
- * and would propagate them silently). */
+/* Scan the resident state: a non-finite DOF returns LAG2D_ENONFINITE; a
+ * non-positive cell volume (geoel(3), e.g. a tangled cell) or a non-positive
+ * cell-average specific volume tau returns LAG2D_EVOLUME; the message names
+ * the first such cell (1-based).  The reference has no such check and would
+ * propagate them silently.  Synchronises. */
 int lag2d_check(lag2d_ctx *ctx);
 
 /* Wait for all work queued on the context's stream. */

@@ -102,10 +102,20 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
                 return fail(LAG2D_EUNSUPPORTED, "nrz=2: slip walls (code 22) and free boundaries (21) only");
     }
     if (c.nstag != 2 && c.nstag != 3) return fail(LAG2D_EUNSUPPORTED, "nstag must be 2 or 3");
+    // ncase 8 lowers the CFL number once the time passes 0.6
+    // (Getrhslag.f90:1785); that switch is not implemented
+    if (c.ncase == 8) return fail(LAG2D_EUNSUPPORTED, "ncase 8: the mid-run CFL switch (Getrhslag.f90:1785) "
+                                                      "is not implemented");
     if (ntria > 0 && !q1) return fail(LAG2D_EUNSUPPORTED, "triangles: straight P1 path only (ncurv=0, nfint=1)");
     if (ntria > 0 && (c.ncase == 1 || c.ncase == 2 || c.ncase == 3 || c.ncase == -7))
         return fail(LAG2D_EUNSUPPORTED, "triangles: the boundary-node limiter overrides of ncase 1, 2, 3, -7 "
                                         "(Getrhslagmctria.f90:539-582) are not implemented");
+
+    // the elements-around-points CSR is checked before anything else reads
+    // it (the node->corner and WENO-neighbour builders index esuv1 by it)
+    if (esuv2[0] != 0) return fail(LAG2D_EINVAL, "esuv2(1) must be 0 (CSR offsets)");
+    for (int p = 0; p < npoin; ++p)
+        if (esuv2[p + 1] < esuv2[p]) return fail(LAG2D_EINVAL, "esuv2 not monotone");
 
     lag2d_ctx *x = new lag2d_ctx();
     x->cfg = c;
@@ -932,11 +942,30 @@ extern "C" int lag2d_use_graphs(lag2d_ctx *x, int enable)
     return 0;
 }
 
+// Strict mode on the curved path: the reference stops where its WENO stencil
+// builder meets a vertex neighbourhood it cannot classify (WENO.f90:7699);
+// the kernels flag such troubled cells in wenobad, read back here (this
+// synchronises the stream, so strict steps are not asynchronous).
+static int check_strict(lag2d_ctx *x, int itime)
+{
+    if (!(x->cfg.strict && x->cfg.ncurv == 1)) return 0;
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, x->d.wenobad, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    if (bad)
+        return fail(LAG2D_ETOPOLOGY, "WENO stencil: troubled cell with a vertex neighbourhood the "
+                                     "reference stops on (WENO.f90:7699) at step " +
+                                         std::to_string(itime));
+    return 0;
+}
+
 extern "C" int lag2d_step(lag2d_ctx *x, int itime)
 {
     if (!x) return fail(LAG2D_EINVAL, "null context");
     if (itime < 1) return fail(LAG2D_EINVAL, "itime is 1-based");
-    return step_graph(x, itime);
+    const int rc = step_graph(x, itime);
+    if (rc) return rc;
+    return check_strict(x, itime);
 }
 
 extern "C" int lag2d_run(lag2d_ctx *x, int itime0, int nsteps, int *taken)
@@ -954,15 +983,7 @@ extern "C" int lag2d_run(lag2d_ctx *x, int itime0, int nsteps, int *taken)
         int rc = step_graph(x, itime0 + k);
         if (rc) return rc;
         ++n;
-        if (x->cfg.strict && x->cfg.ncurv == 1) {
-            int bad = 0;
-            HIPCHK(hipMemcpyAsync(&bad, x->d.wenobad, sizeof(int), hipMemcpyDeviceToHost, x->stream));
-            HIPCHK(hipStreamSynchronize(x->stream));
-            if (bad)
-                return fail(LAG2D_ETOPOLOGY, "WENO stencil: troubled cell with a vertex neighbourhood the "
-                                             "reference stops on (WENO.f90:7699) at step " +
-                                                 std::to_string(itime0 + k));
-        }
+        if ((rc = check_strict(x, itime0 + k))) return rc;
         if (watch && x->nrun == 1) {
             HIPCHK(hipMemcpyAsync(sc, x->d.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, x->stream));
             HIPCHK(hipStreamSynchronize(x->stream));
@@ -993,6 +1014,7 @@ extern "C" int lag2d_rhs(lag2d_ctx *x, double *rhsel)
     ds.ys = d.yn;
     int rc = launch_rhs(x, ds, d.Un, 0.0, 0, 1, 0);
     if (rc) return rc;
+    if ((rc = check_strict(x, 1))) return rc;
     const int nc = x->nc, nd = x->nd;
     const size_t nU = (size_t)4 * nd * nc;
     HIPCHK(hipMemcpyAsync(d.Un, d.Us, nU * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
@@ -1143,10 +1165,15 @@ extern "C" int lag2d_check(lag2d_ctx *x)
     HIPCHK(hipStreamSynchronize(x->stream));
     HIPCHK(hipMemcpy(u.data(), x->d.Un, u.size() * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(g.data(), x->d.G + 2 * (size_t)nc, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost));
-    for (double v : u)
-        if (!std::isfinite(v)) return fail(LAG2D_ENONFINITE, "non-finite DOF in the state");
+    for (size_t k = 0; k < u.size(); ++k)
+        if (!std::isfinite(u[k]))
+            return fail(LAG2D_ENONFINITE, "non-finite DOF in the state at cell " + std::to_string(k % nc + 1));
     for (int ie = 0; ie < nc; ++ie)
         if (!(g[ie] > 0.0)) return fail(LAG2D_EVOLUME, "non-positive cell volume at cell " + std::to_string(ie + 1));
+    // the cell average of tau (specific volume, variable 1, degree 1)
+    for (int ie = 0; ie < nc; ++ie)
+        if (!(u[ie] > 0.0))
+            return fail(LAG2D_EVOLUME, "non-positive specific volume (mean tau) at cell " + std::to_string(ie + 1));
     return 0;
 }
 

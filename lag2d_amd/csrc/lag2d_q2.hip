@@ -153,9 +153,11 @@ __device__ __forceinline__ const double *cell_dofs(const Dev &d, int ie)
 }
 
 // ---------------------------------------------------------------- prep --
-// Copies the stage DOFs into the limited buffer, evaluates the Persson
-// indicator on tau and appends troubled cells to the WENO list.  8 lanes
-// per cell evaluate the 25 Gauss points; lane 0 sums them in point order.
+// Evaluates the Persson indicator on tau, sets each cell's troubled flag
+// and appends troubled cells to the WENO list (the stage DOFs are not
+// copied: WENO writes only the troubled cells' limited DOFs to Ul, and
+// cell_dofs reads the other cells' from the stage input).  8 lanes per
+// cell evaluate the 25 Gauss points; lane 0 sums them in point order.
 constexpr int PREP_CELLS = 32;
 
 __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict__ Uin)
@@ -2099,11 +2101,16 @@ static void launch_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update,
                            apply_update);
 }
 
-// As tune_corner.  The kernel reads Ul, the corner records and u* and writes
-// the stage state (and R): running it again on the same inputs rewrites the
-// same values, and it does not read what it writes.
+// As tune_corner.  The kernel reads the stage input (Ui, and Ul for troubled
+// cells), the corner records and u*, and writes R and the stage state Us.
+// Timing it several times is harmless only where it does not read what it
+// writes: without the fused RK update, and with an input other than Us.
+// Elsewhere (the stages after the first, where Ui == Us and the update is
+// applied) the default variant runs untimed and tuning waits for a call
+// that allows it.
 static int tune_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update)
 {
+    if (apply_update != 0 || ds.Ui == ds.Us) return 0;
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return LAG2D_EDEVICE;
     float ms[3] = {0.f, 1e30f, 1e30f};
@@ -2152,7 +2159,7 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
         if (rc) return rc;
     }
     x->tic(KID_CELLRHS_Q2);
-    launch_cellrhs(x, ds, alfa, apply_update, x->rhs_variant);
+    launch_cellrhs(x, ds, alfa, apply_update, x->rhs_variant ? x->rhs_variant : 1);
     x->toc(KID_CELLRHS_Q2);
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
 }

@@ -236,6 +236,9 @@ class Control:
     # (dgflo2d.f90:430, no short-circuit), so a run of that program needs a
     # non-zero value.
     ndump: int = 0
+    # 1: the driver's own initial condition; 2: the DOFs read from
+    # restartdg.dat (Getrhslagmc.f90:3923-3936)
+    nint: int = 1
 
 
 def sod_problem(nx=100, ny=4):
@@ -338,7 +341,7 @@ def write_ctrl(path, c: Control):
         "nmatel", "1",
         "gamlg cdrho tend cimpd", f"{c.gamlg!r} {c.cdrho!r} {c.tend!r} {c.cimpd!r}",
         "ntime ndegr mdegr npoly nreco", f"{c.ntime} {c.ndegr} {c.ndegr} {c.npoly} 0",
-        "nlimi ngaus ngausf ngausd nint", f"{c.nlimi} 2 2 7 1",
+        "nlimi ngaus ngausf ngausd nint", f"{c.nlimi} 2 2 7 {c.nint}",
         "ngausf_geo ngausd_geo", "2 7",
         "ncurv", f"{c.ncurv}",
         "nresi", "1000000",

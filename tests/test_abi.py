@@ -79,6 +79,54 @@ def test_unsupported_rz_config_is_rejected_without_device(change, needle):
     assert needle in L.lag2d_last_error()
 
 
+@pytest.mark.parametrize("e2,needle", [
+    ([1, 1, 2, 3, 4], b"esuv2(1)"),
+    ([-1, 1, 2, 3, 4], b"esuv2(1)"),
+    ([0, 2, 1, 3, 4], b"monotone"),
+])
+def test_bad_csr_is_rejected_without_device(e2, needle):
+    """A CSR whose first offset is not 0 or whose offsets decrease would
+    make the host-side neighbour builders index esuv1 out of bounds; it is
+    refused before any of them (or the device) runs."""
+    import ctypes
+    import numpy as np
+    from lag2d_amd import lib
+    L = lib.load()
+    cfg = lib.Config(ndegr=3, npoly=1, ncurv=0, nstag=2, ndens=1, nlimi=6, ncase=6, nrz=0, nfint=1,
+                     nriem=1, ndt=1, gamlg=1.4, cfl=0.3, cimpd=1.0, cdrho=0.0, tend=0.2)
+    ipq = np.array([1, 2, 3, 4], dtype=np.int32)
+    bf = np.zeros(5, dtype=np.int32)
+    e1 = np.array([1, 1, 1, 1], dtype=np.int32)
+    e2 = np.array(e2, dtype=np.int32)
+    h = ctypes.c_void_p()
+    P = ctypes.POINTER(ctypes.c_int)
+    rc = L.lag2d_create(ctypes.byref(cfg), 1, 4, 0, ipq.ctypes.data_as(P), bf.ctypes.data_as(P),
+                        e1.ctypes.data_as(P), e2.ctypes.data_as(P), ctypes.byref(h))
+    assert rc == 1
+    assert needle in L.lag2d_last_error()
+
+
+def test_ncase8_is_rejected_without_device():
+    """ncase 8 lowers the CFL number mid-run (Getrhslag.f90:1785), which the
+    library does not reproduce: it refuses the problem rather than drift."""
+    import ctypes
+    import numpy as np
+    from lag2d_amd import lib
+    L = lib.load()
+    cfg = lib.Config(ndegr=3, npoly=1, ncurv=0, nstag=2, ndens=1, nlimi=6, ncase=8, nrz=0, nfint=1,
+                     nriem=1, ndt=1, gamlg=1.4, cfl=0.3, cimpd=1.0, cdrho=0.0, tend=0.2)
+    ipq = np.array([1, 2, 3, 4], dtype=np.int32)
+    bf = np.zeros(5, dtype=np.int32)
+    e1 = np.array([1, 1, 1, 1], dtype=np.int32)
+    e2 = np.array([0, 1, 2, 3, 4], dtype=np.int32)
+    h = ctypes.c_void_p()
+    P = ctypes.POINTER(ctypes.c_int)
+    rc = L.lag2d_create(ctypes.byref(cfg), 1, 4, 0, ipq.ctypes.data_as(P), bf.ctypes.data_as(P),
+                        e1.ctypes.data_as(P), e2.ctypes.data_as(P), ctypes.byref(h))
+    assert rc == 2
+    assert b"ncase 8" in L.lag2d_last_error()
+
+
 def test_fortran_interface_compiles(tmp_path):
     """The ISO_C_BINDING module (lag2d_amd/fortran/lag2d_c.f90) compiles on
     its own: it depends on nothing of the reference."""

@@ -1,0 +1,141 @@
+"""Failure detection through the C ABI.  The reference has none: NaN,
+negative specific volumes and tangled cells propagate silently and its WENO
+stencil builder stops the program (WENO.f90:7699).  The library reports
+them as error codes instead.  Every case here is bad data injected from the
+host into normal kernels: nothing makes the GPU fault.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fixtures import load
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRV_CPU = os.path.join(ROOT, "oracle", "_ref", "lag2d_drv_cpu")
+DRV_GPU = os.path.join(ROOT, "oracle", "_ref", "lag2d_drv_gpu")
+ENONFINITE, EVOLUME, ETOPOLOGY = 4, 5, 6
+
+
+def _last_error():
+    from lag2d_amd import lib
+    return lib.load().lag2d_last_error().decode()
+
+
+@pytest.mark.parametrize("name", ["sedov_q1_16x16", "sedov_q2c_16x16"])
+def test_clean_state_passes_check(name):
+    from lag2d_amd.lib import Lag2D
+    case, _ = load(name)
+    g = Lag2D(case)
+    assert g.check() == 0
+    g.step(2)
+    assert g.check() == 0
+
+
+@pytest.mark.parametrize("name", ["sedov_q1_16x16", "sedov_q2c_16x16"])
+def test_nan_in_state_is_reported(name):
+    from lag2d_amd.lib import Lag2D
+    case, _ = load(name)
+    cell = case.ncell // 3
+    case.unkno[cell, 3, 1] = np.nan
+    g = Lag2D(case)
+    assert g.check() == ENONFINITE
+    assert f"cell {cell + 1}" in _last_error()
+
+
+def test_nan_spreads_and_is_reported_after_steps():
+    """A NaN injected into one cell's energy reaches the nodal velocities and
+    the neighbours within a step; the check after the steps still reports."""
+    from lag2d_amd.lib import Lag2D
+    case, _ = load("sedov_q1_16x16")
+    case.unkno[100, 3, 0] = np.nan
+    g = Lag2D(case)
+    g.step(2)
+    assert g.check() == ENONFINITE
+
+
+@pytest.mark.parametrize("name", ["sedov_q1_16x16", "sedov_q2c_16x16"])
+def test_negative_specific_volume_is_reported(name):
+    from lag2d_amd.lib import Lag2D
+    case, _ = load(name)
+    cell = case.ncell - 5
+    case.unkno[cell, 0, 0] = -abs(case.unkno[cell, 0, 0])
+    g = Lag2D(case)
+    assert g.check() == EVOLUME
+    assert "specific volume" in _last_error() and f"cell {cell + 1}" in _last_error()
+
+
+@pytest.mark.parametrize("name", ["sedov_q1_16x16", "sedov_q2c_16x16"])
+def test_tangled_cell_is_reported(name):
+    """Two opposite corners of one cell moved past each other: the cell's
+    Jacobian changes sign, the upload's getcellvol_lag pass gives it a
+    negative volume, and the check names it."""
+    from lag2d_amd.lib import Lag2D
+    case, _ = load(name)
+    nc = case.ncell
+    cell = nc // 2
+    ip = case.ipqua[cell]
+    x = case.coord.copy()
+    # swap the positions of corner nodes 1 and 3 of the cell (0-based 0, 2)
+    x[[ip[0], ip[2]]] = x[[ip[2], ip[0]]]
+    if ip.size == 9:  # curved cell: move the mid and centre nodes accordingly
+        x[ip[8]] = -0.25 * x[ip[:4]].sum(0) + 0.5 * x[ip[4:8]].sum(0)
+    case.coord[:] = x
+    g = Lag2D(case)
+    assert g.check() == EVOLUME
+    msg = _last_error()
+    assert "cell volume" in msg
+
+
+def test_strict_step_reports_topology():
+    """Quadrant blast on the curved P2 path: the reference stops in its WENO
+    stencil builder at the first stage (WENO.f90:7699).  In strict mode the
+    asynchronous lag2d_step reports it (as lag2d_run does), and so does
+    lag2d_rhs; without strict the step goes on (stencil skipped)."""
+    import ctypes
+    from lag2d_amd import lib
+    from lag2d_amd.lib import Lag2D
+    case, _ = load("sedov_q2_12x12")
+    g = Lag2D(case, strict=True)
+    rc = lib.load().lag2d_step(g.h, 1)
+    assert rc == ETOPOLOGY, rc
+    assert "WENO.f90:7699" in _last_error()
+    g2 = Lag2D(case, strict=True)
+    r = np.zeros((case.ncell, 4, case.ctl.ndegr))
+    rc = lib.load().lag2d_rhs(g2.h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == ETOPOLOGY, rc
+    g3 = Lag2D(case, strict=False)
+    assert lib.load().lag2d_step(g3.h, 1) == 0
+    g3.sync()
+
+
+@pytest.mark.skipif(not (os.path.exists(DRV_CPU) and os.path.exists(DRV_GPU)),
+                    reason="reference drivers not built (make -f oracle/ref.mk drivers)")
+def test_fortran_drop_in_stops_with_library_message(tmp_path):
+    """The reference's own driver with the GPU drop-in of getsolvdg_lag, on
+    the quadrant-blast case the reference itself stops on: the CPU program
+    stops in WENO, and the drop-in error-stops with the library's message
+    instead of running on."""
+    from lag2d_amd import meshgen as mg
+    import dataclasses
+    from lag2d_amd import io
+    mesh, ctl = mg.sedov_problem(12, seed=11)
+    # the driver's own Sedov set-up places the blast by element ids this mesh
+    # does not have, so it goes in through the restart file (nint = 2), as
+    # the fixture's blast did
+    ctl = dataclasses.replace(ctl, ntime=3, ndump=1000000, nint=2)
+    case, z = load("sedov_q2_12x12")
+    for sub in ("cpu", "gpu"):
+        mg.write_case(str(tmp_path / sub), mesh, ctl)
+        io.write_restart(str(tmp_path / sub / "restartdg.dat"), case.unkno)
+    print("running the reference driver (CPU) and the GPU drop-in", flush=True)
+    rc = subprocess.run([DRV_CPU], cwd=tmp_path / "cpu", capture_output=True, text=True, timeout=300)
+    rg = subprocess.run([DRV_GPU], cwd=tmp_path / "gpu", capture_output=True, text=True, timeout=300)
+    assert "Complete the solver" not in rc.stdout  # the reference stops
+    assert rg.returncode != 0
+    out = rg.stdout + rg.stderr
+    assert "lag2d" in out and "WENO.f90:7699" in out, out[-2000:]
+    assert "Complete the solver" not in rg.stdout
