@@ -70,43 +70,45 @@ def test_negative_specific_volume_is_reported(name):
 
 @pytest.mark.parametrize("name", ["sedov_q1_16x16", "sedov_q2c_16x16"])
 def test_tangled_cell_is_reported(name):
-    """Two opposite corners of one cell moved past each other: the cell's
-    Jacobian changes sign, the upload's getcellvol_lag pass gives it a
-    negative volume, and the check names it."""
+    """One cell's nodes mirrored in x about their mean: the cell turns
+    inside out (its Jacobian changes sign everywhere), the upload's
+    getcellvol_lag pass gives it a negative volume and the check names it.
+    (Checked on the host for these meshes: the mirrored cell is the only
+    one whose volume turns negative.)"""
     from lag2d_amd.lib import Lag2D
     case, _ = load(name)
-    nc = case.ncell
-    cell = nc // 2
+    cell = case.ncell // 2
     ip = case.ipqua[cell]
     x = case.coord.copy()
-    # swap the positions of corner nodes 1 and 3 of the cell (0-based 0, 2)
-    x[[ip[0], ip[2]]] = x[[ip[2], ip[0]]]
-    if ip.size == 9:  # curved cell: move the mid and centre nodes accordingly
-        x[ip[8]] = -0.25 * x[ip[:4]].sum(0) + 0.5 * x[ip[4:8]].sum(0)
+    xc = x[ip, 0].mean()
+    x[ip, 0] = 2.0 * xc - x[ip, 0]
     case.coord[:] = x
     g = Lag2D(case)
     assert g.check() == EVOLUME
     msg = _last_error()
-    assert "cell volume" in msg
+    assert "cell volume" in msg and f"cell {cell + 1}" in msg, msg
 
 
 def test_strict_step_reports_topology():
-    """Quadrant blast on the curved P2 path: the reference stops in its WENO
-    stencil builder at the first stage (WENO.f90:7699).  In strict mode the
-    asynchronous lag2d_step reports it (as lag2d_run does), and so does
-    lag2d_rhs; without strict the step goes on (stencil skipped)."""
+    """Quadrant blast on the curved P2 path: the reference computes the
+    right-hand side of the initial state (the fixture holds it) and then
+    stops in its WENO stencil builder within the first step
+    (WENO.f90:7699), once the corner cell is troubled.  In strict mode
+    lag2d_rhs on the initial state succeeds, as the reference does, and the
+    asynchronous lag2d_step reports the stop (as lag2d_run does); without
+    strict the step goes on (stencil skipped)."""
     import ctypes
     from lag2d_amd import lib
     from lag2d_amd.lib import Lag2D
     case, _ = load("sedov_q2_12x12")
+    g2 = Lag2D(case, strict=True)
+    r = np.zeros((case.ncell, 4, case.ctl.ndegr))
+    rc = lib.load().lag2d_rhs(g2.h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0, _last_error()
     g = Lag2D(case, strict=True)
     rc = lib.load().lag2d_step(g.h, 1)
     assert rc == ETOPOLOGY, rc
     assert "WENO.f90:7699" in _last_error()
-    g2 = Lag2D(case, strict=True)
-    r = np.zeros((case.ncell, 4, case.ctl.ndegr))
-    rc = lib.load().lag2d_rhs(g2.h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    assert rc == ETOPOLOGY, rc
     g3 = Lag2D(case, strict=False)
     assert lib.load().lag2d_step(g3.h, 1) == 0
     g3.sync()
