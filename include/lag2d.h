@@ -213,6 +213,20 @@ int lag2d_weno_cells(lag2d_ctx *ctx, long long *total);
  * the reference. */
 int lag2d_division_check(int n, const double *a, const double *b, double *q);
 
+/* lag2d_division_check without any device allocation: no hipMalloc, hipFree
+ * or allocating library call.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  The device copies
+ * of a, b and q live in a static device array of 3 * LAG2D_NOALLOC_MAX_N
+ * doubles (1.5 MiB) that each GPU the library runs on holds until the
+ * process ends; n > LAG2D_NOALLOC_MAX_N returns LAG2D_EINVAL before the
+ * device is touched.  Results equal lag2d_division_check's.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (from several
+ * host threads) race and must be serialised by the caller.  Works on the
+ * current HIP device; synchronises. */
+#define LAG2D_NOALLOC_MAX_N 65536
+int lag2d_division_check_noalloc(int n, const double *a, const double *b, double *q);
+
 /* With enable != 0, lag2d_step and lag2d_run replay each step from a HIP
  * graph captured from the step's launches (one per buffer parity and fixed
  * dt and epsaw), cutting the host's per-step launch cost; the results are

@@ -34,6 +34,9 @@ static int fail(int code, const std::string &msg)
 
 extern "C" const char *lag2d_last_error(void) { return g_err.c_str(); }
 
+// For the other translation units of the library (lag2d_noalloc.hip).
+int lag2d_set_error(int code, const std::string &msg) { return fail(code, msg); }
+
 template <class T>
 static int dalloc(T **p, size_t n)
 {

@@ -19,7 +19,8 @@ EXPORTS = ("lag2d_create", "lag2d_create_hybrid", "lag2d_create_ensemble", "lag2
            "lag2d_upload", "lag2d_init_state", "lag2d_step", "lag2d_run", "lag2d_rhs", "lag2d_download",
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
            "lag2d_division_check", "lag2d_get_mass_matrix", "lag2d_adj_record", "lag2d_adj_replay",
-           "lag2d_adj_gradient", "lag2d_use_graphs", "lag2d_destroy", "lag2d_last_error")
+           "lag2d_adj_gradient", "lag2d_use_graphs", "lag2d_destroy", "lag2d_last_error",
+           "lag2d_division_check_noalloc")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -59,6 +60,7 @@ def load():
     lib.lag2d_set_initial_density.argtypes = [C.c_void_p, _dp]
     lib.lag2d_weno_cells.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     lib.lag2d_division_check.argtypes = [C.c_int, _dp, _dp, _dp]
+    lib.lag2d_division_check_noalloc.argtypes = [C.c_int, _dp, _dp, _dp]
     lib.lag2d_upload.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.lag2d_init_state.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _dp]
     lib.lag2d_step.argtypes = [C.c_void_p, C.c_int]
