@@ -87,9 +87,11 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     if (ntria < 0 || nquad < 0 || ntria + nquad <= 0 || npoin <= 0 || nbfac < 0) return fail(LAG2D_EINVAL, "bad sizes");
     const lag2d_config &c = *cfg;
     const bool q1 = c.ncurv == 0 && c.nfint == 1 && c.npoly == 1 && c.ndegr == 3;
-    const bool q2 = c.ncurv == 1 && c.nfint == 4 && c.npoly == 2 && c.ndegr == 6;
+    // the curved sub-cell branch (Getrhslagmc.f90:7556-7708), P2 or P1
+    const bool q2 = c.ncurv == 1 && c.nfint == 4 &&
+                    ((c.npoly == 2 && c.ndegr == 6) || (c.npoly == 1 && c.ndegr == 3));
     if (!(q1 || q2))
-        return fail(LAG2D_EUNSUPPORTED, "supported: ncurv=0/nfint=1/P1 or ncurv=1/nfint=4/P2");
+        return fail(LAG2D_EUNSUPPORTED, "supported: ncurv=0/nfint=1/P1 or ncurv=1/nfint=4/P1 or P2");
     if ((c.nrz != 0 && c.nrz != 2) || c.ndens != 1 || c.nlimi != 6 || c.nriem != 1)
         return fail(LAG2D_EUNSUPPORTED, "supported: nrz=0 or 2, ndens=1, nlimi=6, nriem=1");
     if (c.nrz == 2) {
@@ -357,6 +359,10 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         const int k = std::atoi(v);
         if (k == 1 || k == 2) x->rhs_variant = k;
     }
+    if (q2 && c.npoly == 1) {  // curved P1 has one corner and one cell-RHS form
+        x->corner_variant = 2;
+        x->rhs_variant = 2;
+    }
     *out = x;
     return 0;
 }
@@ -381,7 +387,8 @@ extern "C" int lag2d_create_ensemble(const lag2d_config *cfg, int nrun, const in
 {
     if (!cfg || !cell_off || !node_off || !tend || !dtfix || !tacru) return fail(LAG2D_EINVAL, "null argument");
     if (nrun < 1) return fail(LAG2D_EINVAL, "nrun < 1");
-    if (cfg->ncurv != 1) return fail(LAG2D_EUNSUPPORTED, "ensembles run the curved P2 path (ncurv=1)");
+    if (cfg->ncurv != 1 || cfg->npoly != 2)
+        return fail(LAG2D_EUNSUPPORTED, "ensembles run the curved P2 path (ncurv=1, npoly=2)");
     if (cell_off[0] != 0 || cell_off[nrun] != nquad || node_off[0] != 0 || node_off[nrun] != npoin)
         return fail(LAG2D_EINVAL, "run offsets must start at 0 and end at nquad / npoin");
     RunSpec rs;

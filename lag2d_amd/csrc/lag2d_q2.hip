@@ -144,6 +144,8 @@ __device__ __forceinline__ double rho0(int ncase, double xc, double yc)
 }
 
 #define UD(buf, q, id, ie) (buf)[((q) * 6 + (id)) * nc + (ie)]
+// the same for kernels templated on the number of DOFs per variable ND
+#define UDN(buf, q, id, ie) (buf)[((q) * ND + (id)) * nc + (ie)]
 
 // The limited DOFs of cell ie: WENO's output where it limited the cell (the
 // troubled flag it leaves set), the stage input everywhere else.
@@ -503,7 +505,10 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
         {
             const int flags = S.top[9];
             if (tid == 0 && (flags & 2)) atomicAdd(d.wenobad, 1);
-            if (flags & 1) {  // not limited: the later kernels read the stage input
+            // a cell at rest has the zero frame and the reference's normal
+            // equations vanish (0/0); outside strict mode it keeps its DOFs
+            const bool rest = S.m[0] == 0.0 && S.m[1] == 0.0 && S.m[2] == 0.0 && S.m[3] == 0.0 && !d.weno_strict;
+            if ((flags & 1) || rest) {  // not limited: the later kernels read the stage input
                 if (tid == 0) d.flag[ie] = 0;
                 continue;
             }
@@ -1293,6 +1298,7 @@ __global__ void __launch_bounds__(256) k_q2_shape_tables(double *tab)
 }
 
 
+template <int ND>
 __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
 {
     __shared__ double stab[125 * SHP_STRIDE];
@@ -1366,8 +1372,8 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
     {
         // ---- sub-cell isg: area of its Q2 map and the mass-weighted mean
         //      specific volume of the DG polynomial, summed in point order
-        double u0[6];
-        for (int id = 0; id < 6; ++id) u0[id] = UD(cell_dofs(d, ie), 0, id, ie);
+        double u0[ND];
+        for (int id = 0; id < ND; ++id) u0[id] = UDN(cell_dofs(d, ie), 0, id, ie);
         const double pc[12][2] = {{-0.5, -1.0}, {0.5, -1.0}, {1.0, -.5}, {1.0, .5}, {0.5, 1.0}, {-0.5, 1.0},
                                   {-1.0, .5},   {-1.0, -.5}, {0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
         const int ipsc[4][4] = {{0, 8, 11, 7}, {1, 2, 9, 8}, {9, 3, 4, 10}, {11, 10, 5, 6}};
@@ -1392,7 +1398,7 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
             double b[6];
             basis6(r, s, rc, sc, gg0, gg1, gg2, b);
             double t = 0.0;
-            for (int id = 0; id < 6; ++id) t = t + u0[id] * b[id];
+            for (int id = 0; id < ND; ++id) t = t + u0[id] * b[id];
             t = rhomc + 1.0 * (t - rhomc);
             jac9_tab(xp, yp, &stab[(25 + 25 * isg + g) * SHP_STRIDE], J);
             const double djaco = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]) * 0.25;
@@ -1402,9 +1408,9 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
     }
     const double rsub = 0.0 + (1.0 / a) * d.msub[isg * nc + ie];
     const double uq = volel / masel;
-    double u[4][6];
+    double u[4][ND];
     for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) u[q][id] = UD(cell_dofs(d, ie), q, id, ie);
+        for (int id = 0; id < ND; ++id) u[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
     const double rhomc = u[0][0], uctr = u[1][0], vctr = u[2][0], ectr = u[3][0];
     const double rhoct = 1.0 / rhomc;
     const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
@@ -1418,7 +1424,7 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
             double b[6];
             basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, b);
             for (int q = 0; q < 4; ++q) un[q] = 0.0;
-            for (int id = 0; id < 6; ++id)
+            for (int id = 0; id < ND; ++id)
                 for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
         }
         const int p = d.ipq[ln * nc + ie];
@@ -1725,6 +1731,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
 // the shape-function derivatives are read from the table as wave-uniform
 // loads.  Every residual entry accumulates in the same order as in
 // k_cellrhs_q2: identical results.
+template <int ND>
 __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1737,9 +1744,9 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
     const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
     const double gg0 = d.G[18 * nc + ie], gg1 = d.G[19 * nc + ie], gg2 = d.G[20 * nc + ie];
-    double r[4][6];
+    double r[4][ND];
     for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) r[q][id] = 0.0;
+        for (int id = 0; id < ND; ++id) r[q][id] = 0.0;
     // ---- face integral, corners in (isg, iv) order
 #pragma unroll 1
     for (int k = 0; k < 16; ++k) {
@@ -1765,7 +1772,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
         const double w1 = WFM[isg][iv][0], w2 = WFM[isg][iv][1];
         double bq[6];
         basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, bq);
-        for (int id = 0; id < 6; ++id) {
+        for (int id = 0; id < ND; ++id) {
             const double b = bq[id];
             r[0][id] = r[0][id] + us1 * nx[0] * ar[0] * b + us2 * ny[0] * ar[0] * b + us1 * nx[1] * ar[1] * b +
                        us2 * ny[1] * ar[1] * b;
@@ -1777,9 +1784,9 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
     }
     // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
     {
-        double u[4][6];
+        double u[4][ND];
         for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < 6; ++id) u[q][id] = UD(cell_dofs(d, ie), q, id, ie);
+            for (int id = 0; id < ND; ++id) u[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
         double xp[9], yp[9];
         load9(d, d.xs, d.ys, ie, xp, yp);
 #pragma unroll 1
@@ -1799,12 +1806,12 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
             double b[6];
             basis6(rr, ss, rc, sc, gg0, gg1, gg2, b);
             double un[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int id = 0; id < 6; ++id)
+            for (int id = 0; id < ND; ++id)
                 for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
             const double rhoad = 1.0 / un[0];
             const double uu = un[1], vv = un[2];
             const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
-            for (int id = 0; id < 6; ++id) {
+            for (int id = 0; id < ND; ++id) {
                 const double a = dyds * dbdr[id] + (-dydr) * dbds[id];
                 const double c = (-dxds) * dbdr[id] + dxdr * dbds[id];
                 const double f0 = a * uu + c * vv;
@@ -1816,9 +1823,9 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
         }
     }
     // ---- derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708)
-    double ul[4][6];
+    double ul[4][ND];
     for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < 6; ++id) ul[q][id] = UD(cell_dofs(d, ie), q, id, ie);
+        for (int id = 0; id < ND; ++id) ul[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
     for (int id = 1; id < 3; ++id) {
         const double u1 = ul[1][id], u2 = ul[2][id];
         ul[0][id] = ul[0][id] * 1.0;
@@ -1830,32 +1837,47 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
         const int run = cell_run(d, ie);
         if (!run_active(d, run)) {  // frozen run: keep U^n
             for (int q = 0; q < 4; ++q)
-                for (int id = 0; id < 6; ++id) UD(d.Us, q, id, ie) = UD(d.Un, q, id, ie);
+                for (int id = 0; id < ND; ++id) UDN(d.Us, q, id, ie) = UDN(d.Un, q, id, ie);
             return;
         }
         const double dt = d.scal[2 * run];
-        const int off[6] = {0, 0, 5, 9, 12, 14};
-        for (int id = 0; id < 6; ++id) {
-            double mrow[6];
-            for (int kk = 0; kk < 6; ++kk) {
-                if (id == 0 || kk == 0) {
-                    mrow[kk] = (id == 0 && kk == 0) ? d.A[15 * nc + ie] : 0.0;
+        for (int id = 0; id < ND; ++id) {
+            // row id of the inverse mass matrix (Getrhslag.f90:1830-1881):
+            // P2 the upper triangle of the 5x5 block packed row-wise, then
+            // 1/mass; P1 a11, a12, a22, 1/mass; degree 0 decoupled
+            double mrow[ND];
+            if constexpr (ND == 6) {
+                const int off[6] = {0, 0, 5, 9, 12, 14};
+                for (int kk = 0; kk < 6; ++kk) {
+                    if (id == 0 || kk == 0) {
+                        mrow[kk] = (id == 0 && kk == 0) ? d.A[15 * nc + ie] : 0.0;
+                    } else {
+                        const int a = id < kk ? id : kk, b = id < kk ? kk : id;
+                        mrow[kk] = d.A[(off[a] + (b - a)) * nc + ie];
+                    }
+                }
+            } else {
+                if (id == 0) {
+                    mrow[0] = d.A[3 * nc + ie];
+                    mrow[1] = 0.0;
+                    mrow[2] = 0.0;
                 } else {
-                    const int a = id < kk ? id : kk, b = id < kk ? kk : id;
-                    mrow[kk] = d.A[(off[a] + (b - a)) * nc + ie];
+                    mrow[0] = 0.0;
+                    mrow[1] = d.A[(id == 1 ? 0 : 1) * nc + ie];
+                    mrow[2] = d.A[(id == 1 ? 1 : 2) * nc + ie];
                 }
             }
             for (int q = 0; q < 4; ++q) {
                 double s4 = 0.0;
-                for (int kk = 0; kk < 6; ++kk) s4 = s4 + mrow[kk] * (dt * r[q][kk]);
-                UD(d.Us, q, id, ie) = alfa * UD(d.Un, q, id, ie) + (1.0 - alfa) * (ul[q][id] + s4);
+                for (int kk = 0; kk < ND; ++kk) s4 = s4 + mrow[kk] * (dt * r[q][kk]);
+                UDN(d.Us, q, id, ie) = alfa * UDN(d.Un, q, id, ie) + (1.0 - alfa) * (ul[q][id] + s4);
             }
         }
     } else {
         for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < 6; ++id) {
-                UD(d.Us, q, id, ie) = ul[q][id];
-                UD(d.R, q, id, ie) = r[q][id];
+            for (int id = 0; id < ND; ++id) {
+                UDN(d.Us, q, id, ie) = ul[q][id];
+                UDN(d.R, q, id, ie) = r[q][id];
             }
     }
 }
@@ -1899,7 +1921,296 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_submass(Dev d)
     }
 }
 
+// ------------------------------------------------ curved P1 WENO (npoly=1) --
+// On the curved P1 branch the reference flags every cell troubled
+// (Getrhslagmc.f90:7607) and WENO reconstructs every cell whose stencil is
+// complete: weno_stncl_quad_orthol3 with npoly = 1 (WENO.f90:7541-8098),
+// getgeophy_local_Taylor / getunknp_local_Taylor with 2 x 2 matrices and
+// weno_cell_Taylor with 5 candidates.  Two quirks of that code decide its
+// results and are kept: each Lagrange stencil fits 3 equations in 2
+// unknowns and Matinv225 reads the first four elements of the 3 x 2 matrix
+// in memory order as the 2 x 2 to invert; the npoly = 1 smoothness
+// indicator is the P2 formula, reading the entries at flat offsets 3..5
+// past (1, iq, is) of an array dimensioned (3, nq, nsten).
+// One thread per cell; every sum in the reference's order.
+constexpr int P1_ND = 3;
+#define UD3(buf, q, id, ie) (buf)[((q) * P1_ND + (id)) * nc + (ie)]
+
+// Matinv225 for n = 2 (LinearAlgebra.f90:141-148, :457-458), column-major
+__device__ __forceinline__ void inv2(const double *a, double *b)
+{
+    const double det = a[0] * a[3] - a[2] * a[1];
+    b[0] = a[3] / det;
+    b[2] = -a[2] / det;
+    b[1] = -a[1] / det;
+    b[3] = a[0] / det;
+}
+
+__constant__ static const int ST1P1[4][3] = {{1, 4, 5}, {2, 1, 6}, {3, 2, 7}, {4, 3, 8}};
+
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__restrict__ Uin)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc;
+    bool limited = false;
+    if (ie < nc) {
+        const int flags = d.wtop[9 * nc + ie];
+        if (flags & 2) atomicAdd(d.wenobad, 1);
+        double m[4];
+        {
+            const double a = UD3(Uin, 1, 0, ie), b = UD3(Uin, 2, 0, ie);
+            if (sqrt(a * a + b * b) > 1.e-10) {
+                m[0] = a / sqrt(a * a + b * b);
+                m[1] = b / sqrt(a * a + b * b);
+                m[2] = -b / sqrt(a * a + b * b);
+                m[3] = a / sqrt(a * a + b * b);
+            } else {
+                m[0] = m[1] = m[2] = m[3] = 0.0;
+            }
+        }
+        // a cell at rest has the zero frame and the reference's normal
+        // equations vanish (0/0); outside strict mode it keeps its DOFs
+        const bool rest = m[0] == 0.0 && m[1] == 0.0 && m[2] == 0.0 && m[3] == 0.0 && !d.weno_strict;
+        if ((flags & 1) || rest || !run_active(d, cell_run(d, ie))) {
+            d.flag[ie] = 0;  // not limited: the later kernels read the stage input
+        } else {
+            limited = true;
+            int ielaj[9];
+            for (int k = 0; k < 9; ++k) ielaj[k] = d.wtop[k * nc + ie];
+            double unkrl[9][4][3], unkpl[9][4][3];
+            for (int s = 0; s < 9; ++s)
+                for (int id = 0; id < 3; ++id) {
+                    for (int q = 0; q < 4; ++q) unkrl[s][q][id] = UD3(Uin, q, id, ielaj[s]);
+                    rotf(m, unkrl[s][1][id], unkrl[s][2][id]);
+                }
+            double ct1 = 0.0, ct2 = 0.0, ct3 = 0.0, xmcl = 0.0, ymcl = 0.0, dxcl = 1.0;
+            double mtbbq[4], mtbbi[4], ml[9][2];
+            for (int s = 0; s < 9; ++s) {
+                const int je = ielaj[s];
+                double xn[9], yn[9], xr[9], yr[9];
+                for (int k = 0; k < 9; ++k) {
+                    const int p = d.ipq[k * nc + je];
+                    xn[k] = d.xs[p];
+                    yn[k] = d.ys[p];
+                    xr[k] = xn[k];
+                    yr[k] = yn[k];
+                    rotf(m, xr[k], yr[k]);
+                }
+                const double rcj = d.G[0 * nc + je], scj = d.G[1 * nc + je];
+                const double volj = d.G[2 * nc + je], masj = d.G[3 * nc + je];
+                // physical mass centre on the stage map, rho |J| w of the
+                // initial map (WENO.f90:5989-6061)
+                double xmc = 0.0, ymc = 0.0;
+                for (int g = 0; g < 25; ++g) {
+                    const double *sh = d.shv + 9 * g;
+                    double xg = 0.0, yg = 0.0;
+                    for (int k = 0; k < 9; ++k) {
+                        xg = xg + sh[k] * xn[k];
+                        yg = yg + sh[k] * yn[k];
+                    }
+                    const double rd = d.wrd[g * nc + je];
+                    xmc = xmc + rd * xg;
+                    ymc = ymc + rd * yg;
+                }
+                double xcl = xmc / masj, ycl = ymc / masj;
+                rotf(m, xcl, ycl);
+                const double dxcj = sqrt(volj);
+                if (s == 0) {
+                    xmcl = xcl;
+                    ymcl = ycl;
+                    dxcl = dxcj;
+                }
+                double v1 = 0.0, v2 = 0.0, v3 = 0.0, m0 = 0.0, m1 = 0.0;
+                double bb[4] = {0.0, 0.0, 0.0, 0.0}, bbq[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int g = 0; g < 25; ++g) {
+                    const int i = g / 5, j = g - 5 * i;
+                    const double *sh = d.shv + 9 * g;
+                    double xl = 0.0, yl = 0.0;
+                    for (int k = 0; k < 9; ++k) {
+                        xl = xl + sh[k] * xr[k];
+                        yl = yl + sh[k] * yr[k];
+                    }
+                    const double rd = d.wrd[g * nc + je];
+                    const double b2 = (xl - xcl) / dxcj, b3 = (yl - ycl) / dxcj;
+                    if (s == 0) {
+                        v1 = v1 + rd * b2 * b2;
+                        v2 = v2 + rd * b3 * b3;
+                        v3 = v3 + rd * b2 * b3;
+                    }
+                    const double bq[3] = {1.0, (PG5[i] - rcj) / 1.0, (PG5[j] - scj) / 1.0};
+                    const double bl[3] = {1.0, b2, b3};
+                    for (int a = 1; a < 3; ++a)
+                        for (int c = 1; c < 3; ++c) {
+                            bb[(a - 1) + 2 * (c - 1)] = bb[(a - 1) + 2 * (c - 1)] + rd * bq[a] * bl[c];
+                            bbq[(a - 1) + 2 * (c - 1)] = bbq[(a - 1) + 2 * (c - 1)] + rd * bl[a] * bl[c];
+                        }
+                    if (s > 0) {  // Lagrange-stencil moments in the troubled cell's basis
+                        m0 = m0 + rd * ((xl - xmcl) / dxcl) / masj;
+                        m1 = m1 + rd * ((yl - ymcl) / dxcl) / masj;
+                    }
+                }
+                ml[s][0] = m0;
+                ml[s][1] = m1;
+                if (s == 0) {
+                    ct1 = 0.5 * (v1 / masj);
+                    ct2 = 0.5 * (v2 / masj);
+                    ct3 = v3 / masj;
+                }
+                double rhs[4][3];
+                for (int q = 0; q < 4; ++q) rhs[q][0] = rhs[q][1] = rhs[q][2] = 0.0;
+                for (int a = 1; a < 3; ++a)
+                    for (int c = 1; c < 3; ++c)
+                        for (int q = 0; q < 4; ++q) rhs[q][a] = rhs[q][a] + unkrl[s][q][c] * bb[(c - 1) + 2 * (a - 1)];
+                double bbqi[4];
+                inv2(bbq, bbqi);
+                if (s == 0) {
+                    for (int k = 0; k < 4; ++k) mtbbq[k] = bbq[k];
+                    inv2(bb, mtbbi);
+                }
+                for (int q = 0; q < 4; ++q) unkpl[s][q][1] = unkpl[s][q][2] = 0.0;
+                for (int a = 1; a < 3; ++a)
+                    for (int c = 1; c < 3; ++c)
+                        for (int q = 0; q < 4; ++q)
+                            unkpl[s][q][a] = unkpl[s][q][a] + rhs[q][c] * bbqi[(c - 1) + 2 * (a - 1)];
+                for (int q = 0; q < 4; ++q) unkpl[s][q][0] = unkrl[s][q][0];
+            }
+            // candidates: the centre polynomial and the 4 Lagrange stencils
+            double unksl[5][4][3];
+            for (int c = 0; c < 5; ++c)
+                for (int q = 0; q < 4; ++q) {
+                    unksl[c][q][0] = unkpl[0][q][0];
+                    unksl[c][q][1] = unksl[c][q][2] = 0.0;
+                }
+            for (int q = 0; q < 4; ++q) {
+                unksl[0][q][1] = unkpl[0][q][1];
+                unksl[0][q][2] = unkpl[0][q][2];
+            }
+            for (int is = 0; is < 4; ++is) {
+                const int s0 = ST1P1[is][0], s1 = ST1P1[is][1], s2 = ST1P1[is][2];
+                // matlg(3, 2) read as a column-major 2 x 2 from its first four elements
+                const double a[4] = {ml[s0][0], ml[s1][0], ml[s2][0], ml[s0][1]};
+                double mi[4];
+                inv2(a, mi);
+                double rhsl[4][2];
+                for (int q = 0; q < 4; ++q) {
+                    rhsl[q][0] = unkpl[s0][q][0] - unkpl[0][q][0];
+                    rhsl[q][1] = unkpl[s1][q][0] - unkpl[0][q][0];
+                }
+                for (int i = 1; i < 3; ++i)
+                    for (int e = 0; e < 2; ++e)
+                        for (int q = 0; q < 4; ++q)
+                            unksl[1 + is][q][i] = unksl[1 + is][q][i] + mi[(i - 1) + 2 * e] * rhsl[q][e];
+            }
+            // weno_cell_Taylor, npoly = 1: characteristic weights per real face
+            double xp[4], yp[4];
+            for (int k = 0; k < 4; ++k) {
+                const int p = d.ipq[k * nc + ie];
+                xp[k] = d.xs[p];
+                yp[k] = d.ys[p];
+            }
+            const double volel = d.G[2 * nc + ie];
+            const double dxc = sqrt(volel), dyc = dxc;
+            // x**4 as ((x*x)*x)*x, as the reference's build evaluates it
+            const double dx2 = dxc * dxc, dy2 = dyc * dyc, dx4 = dx2 * dxc * dxc, dy4 = dy2 * dyc * dyc;
+            const double dxy = dyc * dxc, dxy2 = dxy * dxy;
+            const int nf = d.wtop[10 * nc + ie];
+            double uph[4][3];
+            for (int q = 0; q < 4; ++q) uph[q][0] = uph[q][1] = uph[q][2] = 0.0;
+            double ufp[4][4][2];
+            for (int a = 0; a < nf; ++a) {
+                const int f = d.wtop[(11 + a) * nc + ie], f1 = (f + 1) & 3;
+                const double dtx = xp[f1] - xp[f], dty = yp[f1] - yp[f];
+                const double dlgt = sqrt(dtx * dtx + dty * dty);
+                double dnx = dty / dlgt, dny = -dtx / dlgt;
+                rotf(m, dnx, dny);
+                double ul[4], ur[4];
+                for (int q = 0; q < 4; ++q) {
+                    ul[q] = unkrl[0][q][0];
+                    ur[q] = unkrl[1 + f][q][0];
+                }
+                double P[9];
+                char_face(d.gamlg, dnx, dny, ul, ur, P);
+                double qi[4][4];
+                for (int k = 0; k < 4; ++k)
+                    for (int q = 0; q < 4; ++q) qi[k][q] = char_entry(d.gamlg, P, 4 * k + q);
+                // unksf_cha(3, 4, 10), column-major, zeroed
+                double cha[120];
+                for (int k = 0; k < 120; ++k) cha[k] = 0.0;
+                for (int c = 0; c < 5; ++c)
+                    for (int q = 0; q < 4; ++q)
+                        for (int id = 0; id < 3; ++id)
+                            for (int k = 0; k < 4; ++k)
+                                cha[id + 3 * k + 12 * c] = cha[id + 3 * k + 12 * c] + qi[k][q] * unksl[c][q][id];
+                double wh[5], wt = 0.0;
+                for (int c = 0; c < 5; ++c) {
+                    double o[4];
+                    for (int q = 0; q < 4; ++q) {
+                        const double u2 = cha[1 + 3 * q + 12 * c], u3 = cha[2 + 3 * q + 12 * c];
+                        const double u4 = cha[3 + 3 * q + 12 * c], u5 = cha[4 + 3 * q + 12 * c],
+                                     u6 = cha[5 + 3 * q + 12 * c];
+                        const double sm1 =
+                            1.0 / dx2 * (u2 * u2 + 2.0 * ct1 * (u4 * u4) + 2.0 * ct2 * (u6 * u6) + 2.0 * ct3 * u4 * u6);
+                        const double sm2 =
+                            1.0 / dy2 * (u3 * u3 + 2.0 * ct2 * (u5 * u5) + 2.0 * ct1 * (u6 * u6) + 2.0 * ct3 * u5 * u6);
+                        const double sm3 = 1.0 / dx4 * (u4 * u4);
+                        const double sm4 = 1.0 / dy4 * (u5 * u5);
+                        const double sm5 = 1.0 / dxy2 * (u6 * u6);
+                        o[q] = (sm1 + sm2) + (sm3 + sm4 + sm5) * volel;
+                    }
+                    const double os1 = dmax(dmax(dmax(o[0], o[1]), o[2]), o[3]);
+                    const double wl = (c == 0) ? .2 : 0.8 / (5.0 - 1.0);
+                    const double t = 1.e-6 + os1;
+                    wh[c] = wl / (t * t);
+                }
+                for (int c = 0; c < 5; ++c) wt = wt + wh[c];
+                double fch[4][3];
+                for (int q = 0; q < 4; ++q) fch[q][0] = fch[q][1] = fch[q][2] = 0.0;
+                for (int c = 0; c < 5; ++c)
+                    for (int q = 0; q < 4; ++q)
+                        for (int id = 0; id < 3; ++id)
+                            fch[q][id] = fch[q][id] + (wh[c] / wt) * cha[id + 3 * q + 12 * c];
+                for (int k = 0; k < 4; ++k) ufp[a][k][0] = ufp[a][k][1] = 0.0;
+                for (int q = 0; q < 4; ++q)
+                    for (int id = 1; id < 3; ++id)
+                        for (int k = 0; k < 4; ++k)
+                            ufp[a][k][id - 1] = ufp[a][k][id - 1] + char_entry(d.gamlg, P, 16 + 4 * k + q) * fch[q][id];
+            }
+            const double wf = 1.0 / (double)nf;
+            for (int a = 0; a < nf; ++a)
+                for (int q = 0; q < 4; ++q)
+                    for (int id = 1; id < 3; ++id) uph[q][id] = uph[q][id] + wf * ufp[a][q][id - 1];
+            // back to the cell's Taylor basis, then the global frame
+            double rhsuk[4][2], out[4][3];
+            for (int q = 0; q < 4; ++q) rhsuk[q][0] = rhsuk[q][1] = 0.0;
+            for (int i = 1; i < 3; ++i)
+                for (int j = 1; j < 3; ++j)
+                    for (int q = 0; q < 4; ++q)
+                        rhsuk[q][i - 1] = rhsuk[q][i - 1] + uph[q][j] * mtbbq[(j - 1) + 2 * (i - 1)];
+            for (int q = 0; q < 4; ++q) out[q][1] = out[q][2] = 0.0;
+            for (int i = 1; i < 3; ++i)
+                for (int j = 1; j < 3; ++j)
+                    for (int q = 0; q < 4; ++q) out[q][i] = out[q][i] + rhsuk[q][j - 1] * mtbbi[(j - 1) + 2 * (i - 1)];
+            for (int i = 1; i < 3; ++i) {  // getvector_local2 with the inverse frame
+                const double a = out[1][i], b = out[2][i];
+                out[1][i] = a * m[3] + b * (-m[1]);
+                out[2][i] = a * (-m[2]) + b * m[0];
+            }
+            for (int q = 0; q < 4; ++q) {
+                UD3(d.Ul, q, 0, ie) = UD3(Uin, q, 0, ie);
+                UD3(d.Ul, q, 1, ie) = out[q][1];
+                UD3(d.Ul, q, 2, ie) = out[q][2];
+            }
+            d.flag[ie] = 1;
+        }
+    }
+    // cells reconstructed this stage (the WENO work counter)
+    const unsigned long long bal = __ballot(limited);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(d.tcount, (int)__popcll(bal));
+}
+#undef UD3
+
 #undef UD
+#undef UDN
 
 // ---- host-side table set-up ------------------------------------------
 static int q2_tables()
@@ -2053,7 +2364,7 @@ static void launch_corner(lag2d_ctx *x, Dev &ds, int variant)
 {
     const int nc = x->nc;
     if (variant == 2)
-        hipLaunchKernelGGL(k_q2_corner_sc, dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
+        hipLaunchKernelGGL(k_q2_corner_sc<6>, dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
     else
         hipLaunchKernelGGL(k_q2_corner, dim3((nc + CORNER_CPB - 1) / CORNER_CPB), dim3(256), 0, x->stream, ds);
 }
@@ -2095,7 +2406,7 @@ static void launch_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update,
 {
     const int nc = x->nc;
     if (variant == 2)
-        hipLaunchKernelGGL(k_cellrhs_q2_tpc, dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
+        hipLaunchKernelGGL(k_cellrhs_q2_tpc<6>, dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
     else
         hipLaunchKernelGGL(k_cellrhs_q2, dim3((nc + RHS_CPB - 1) / RHS_CPB), dim3(256), 0, x->stream, ds, alfa,
                            apply_update);
@@ -2138,6 +2449,21 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
 {
     const int nc = x->nc, np = x->np;
     ds.Ui = Uin;
+    if (x->nd == 3) {  // curved P1: WENO on every cell, then the sub-cell scheme
+        x->tic(KID_Q2_WENO);
+        hipLaunchKernelGGL(k_c1_weno, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
+        x->toc(KID_Q2_WENO);
+        x->tic(KID_Q2_CORNER);
+        hipLaunchKernelGGL(k_q2_corner_sc<3>, dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
+        x->toc(KID_Q2_CORNER);
+        x->tic(KID_NODE_Q2);
+        hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+        x->toc(KID_NODE_Q2);
+        x->tic(KID_CELLRHS_Q2);
+        hipLaunchKernelGGL(k_cellrhs_q2_tpc<3>, dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
+        x->toc(KID_CELLRHS_Q2);
+        return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
+    }
     x->tic(KID_Q2_PREP);
     hipLaunchKernelGGL(k_q2_prep, dim3((nc + PREP_CELLS - 1) / PREP_CELLS), dim3(256), 0, x->stream, ds, Uin);
     x->toc(KID_Q2_PREP);

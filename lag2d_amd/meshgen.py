@@ -170,6 +170,86 @@ def quad_q2(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1, bulge=0.05) -> Mesh:
     return Mesh(coord, ipq, _boundary_faces(nid, nx, ny, True), 1, nx, ny)
 
 
+def annulus_q2(nr, nt, r0=0.2, r1=1.0, seed=None, jitter=0.1, bulge=0.05, code=25) -> Mesh:
+    """Q2 quads (ncurv=1) on the annulus r0 <= r <= r1: nr cells across, nt
+    around, periodic in the angle, so no mesh vertex belongs to a single
+    cell.  Arc edges are curved by construction (their mid-nodes lie on the
+    arc); with ``seed`` set, interior vertices also move by U(-jitter*h,
+    jitter*h) per coordinate and interior mid-nodes along the edge normal by
+    U(-bulge*h, bulge*h), h the radial cell size.
+
+    Cell (i, j) lists its corners starting on its inner arc, counter-
+    clockwise: (t_{i+1}, r_j), (t_i, r_j), (t_i, r_{j+1}), (t_{i+1}, r_{j+1}).
+    Its face 1 is then the inner arc and face 3 the outer arc, so the wall
+    faces of the first and last rings are faces 1 and 3 -- the orientation
+    under which the reference's WENO stencil builder classifies every vertex
+    neighbourhood (WENO.f90:7641-7704; a wall on face 2 or 4 makes it stop).
+    Both circles are walls of BC ``code`` (25: fixed wall), oriented with
+    the domain on their left."""
+    NI, NJ = 2 * nt, 2 * nr + 1
+    h = (r1 - r0) / nr
+    th = 2.0 * np.pi * np.arange(NI) / NI
+    rr = r0 + 0.5 * h * np.arange(NJ)
+    P = np.zeros((NJ, NI, 2))
+    P[..., 0] = rr[:, None] * np.cos(th)[None, :]
+    P[..., 1] = rr[:, None] * np.sin(th)[None, :]
+    rng = np.random.default_rng(seed) if seed is not None else None
+    if rng is not None and nr > 1:
+        P[2:-2:2, 0::2, :] += rng.uniform(-jitter * h, jitter * h, size=(nr - 1, nt, 2))
+    # radial-edge mid-nodes at the chord midpoint; arc mid-nodes stay on the arc
+    P[1::2, 0::2] = 0.5 * (P[0:-1:2, 0::2] + P[2::2, 0::2])
+    if rng is not None and nr > 1:
+        t = P[2:-2:2, 2::2] - P[2:-2:2, 0:-2:2]
+        t = np.concatenate([t, (P[2:-2:2, 0] - P[2:-2:2, -2])[:, None]], axis=1)
+        u = rng.uniform(-bulge * h, bulge * h, size=t.shape[:2])
+        hyp = np.hypot(t[..., 0], t[..., 1])
+        P[2:-2:2, 1::2, 0] += u * (-t[..., 1] / hyp)
+        P[2:-2:2, 1::2, 1] += u * (t[..., 0] / hyp)
+        t2 = P[2::2, 0::2] - P[0:-1:2, 0::2]
+        u2 = rng.uniform(-bulge * h, bulge * h, size=t2.shape[:2])
+        hyp2 = np.hypot(t2[..., 0], t2[..., 1])
+        P[1::2, 0::2, 0] += u2 * (-t2[..., 1] / hyp2)
+        P[1::2, 0::2, 1] += u2 * (t2[..., 0] / hyp2)
+
+    def nid(I, J):
+        return (I % NI) + NI * J + 1
+
+    def at(I, J):
+        return P[J, I % NI]
+
+    # centre nodes by the reset formula (Getrhslag.f90:1962-1971)
+    for j in range(nr):
+        for i in range(nt):
+            I, J = 2 * i, 2 * j
+            c = at(I + 2, J) + at(I, J) + at(I, J + 2) + at(I + 2, J + 2)
+            m = at(I + 1, J) + at(I, J + 1) + at(I + 1, J + 2) + at(I + 2, J + 1)
+            P[J + 1, I + 1] = -0.25 * c + 0.5 * m
+    coord = P.reshape(-1, 2).copy()
+    ipq = []
+    for j in range(nr):
+        for i in range(nt):
+            I, J = 2 * i, 2 * j
+            ipq.append((nid(I + 2, J), nid(I, J), nid(I, J + 2), nid(I + 2, J + 2),
+                        nid(I + 1, J), nid(I, J + 1), nid(I + 1, J + 2), nid(I + 2, J + 1), nid(I + 1, J + 1)))
+    faces = []
+    for i in range(nt):   # inner circle: face 1 of the first ring, clockwise
+        faces.append((nid(2 * i + 2, 0), nid(2 * i, 0), code, 0, nid(2 * i + 1, 0)))
+    for i in range(nt):   # outer circle: face 3 of the last ring, counter-clockwise
+        J = 2 * nr
+        faces.append((nid(2 * i, J), nid(2 * i + 2, J), code, 0, nid(2 * i + 1, J)))
+    return Mesh(coord, np.asarray(ipq, dtype=np.int32), np.asarray(faces, dtype=np.int32), 1, nt, nr)
+
+
+def sod_annulus_p1_problem(nr=6, nt=24, seed=17):
+    """Cylindrical Sod problem on the annulus 0.2 <= r <= 1 between fixed
+    walls, P1 DG on curved Q2 cells (ncurv=1, nfint=4, npoly=1): the
+    reference's Sod set-up (ncase 6) splits the state at centroid radius 0.5
+    (Getrhslagmc.f90:5600-5614).  Every cell goes to WENO on this branch."""
+    mesh = annulus_q2(nr, nt, seed=seed)
+    ctl = Control(cfl=0.3, tend=0.2, ndegr=3, npoly=1, ncurv=1, nstag=2, ncase=6, nfint=4, cdrho=1.0)
+    return mesh, ctl
+
+
 SEDOV_ENERGY = 0.244816   # quadrant blast energy (Getrhslagmc.f90:5623-5639)
 
 

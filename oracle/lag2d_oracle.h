@@ -49,11 +49,16 @@ typedef struct orc_case {
     const int *esqua;               /* face neighbours [f + 4*ie]; >= ncell: boundary */
     int weno_strict;                /* 1: fail where the reference stops (WENO.f90:7699) */
     int nweno_skipped;              /* troubled cells skipped for their neighbourhood */
+    int nweno_rest;                 /* troubled cells at rest (zero frame) left unchanged */
     double *troubled;               /* Persson flag of the last stage (0 / 100), or NULL */
     /* RZ area weighting (nrz=2): module scalars getsolvdg_lag sets per stage
      * and per step (Getrhslag.f90:1633-1692, :1745) */
     int rkstg;
     double epsaw;
+    /* per-cell initial density in place of problem ncase's (the role of
+     * getrhoig_quadcurv2, Getrhslag.f90:17761-17948, and getrhog_initial,
+     * in the initial sub-cell masses and the WENO mass centres), or NULL */
+    const double *rho0;
 } orc_case;
 
 /* getcellvol_lag (Limiterlag.f90:1869-2164), quads */

@@ -1,7 +1,7 @@
 /* CPU restatement of the curved P2 sub-cell path of the reference's
  * Lagrangian DG right-hand side (the oracle; TEST INFRASTRUCTURE ONLY).
  *
- * ncurv=1, nfint=4, npoly=2, nlimi=6, ndens=1, planar
+ * ncurv=1, nfint=4, npoly=2 or 1, nlimi=6, ndens=1, planar
  * (getrhsdg_lag_mcnew branch Getrhslagmc.f90:7556-7708):
  *   getfnds_lagsmsif_hybrid      Getrhslaggauss.f90:3356-3598
  *   getfnds_lagsms_simpsonhybrid Getrhslaggauss.f90:9660-10017
@@ -13,7 +13,11 @@
  *   rhsifacedg_lagsubgq2         Getrhslaggauss.f90:8501-8745
  *   rhsdomndg_lagmc_curvquad     Getrhslag.f90:4716-5013
  * barthlimit_lagcurv is called by the reference on this branch but its
- * result (node max/min) is not read for P2; it is omitted.
+ * result (node max/min) is not read for P2; it is omitted.  For P1 the
+ * reference also runs barthlimit_lagcurv and barthlimit_lag_curvquadcb
+ * (:7598-7604), whose outputs (aflim, afvec, node max/min and geoel(10)) it
+ * overwrites before any use (:7607, :7650-7656): they are omitted too, and
+ * P1 takes every cell to WENO (weno_cell_p1) in place of the Persson flag.
  *
  * Sums follow the reference's loop order so the restatement rounds where
  * the reference rounds; the one exception is the 5x5 inverse of the WENO
@@ -292,6 +296,17 @@ static void inv_small(int n, const double *a, double *b)
         }
 }
 
+/* Matinv225 for n = 2 (LinearAlgebra.f90:141-148, :457-458): adjugate
+ * over det; a and b column-major */
+static void inv2(const double *a, double *b)
+{
+    const double det = a[0] * a[3] - a[2] * a[1];
+    b[0] = a[3] / det;
+    b[2] = -a[2] / det;
+    b[1] = -a[1] / det;
+    b[3] = a[0] / det;
+}
+
 /* rotate (a, b) by the 2x2 frame m (getvector_local, Limiterlag.f90:19009) */
 static inline void rot(const double m[4], double *a, double *b)
 {
@@ -313,6 +328,7 @@ static inline void rotinv(const double m[4], double *a, double *b)
 
 typedef struct {
     const orc_case *c;
+    const int *esqua;     /* face neighbours */
     const double *xc;     /* current coordinates */
     const double *x0;     /* initial coordinates */
     const double *geoph;  /* physical mass centres [2*ncell] */
@@ -349,7 +365,7 @@ static void cell_points(const weno_env *E, int je, const double m[4], cell_pts *
         }
         P->xl[g] = a;
         P->yl[g] = b;
-        P->rhogi[g] = rho_initial(c, xcr, ycr);
+        P->rhogi[g] = c->rho0 ? c->rho0[je] : rho_initial(c, xcr, ycr);
     }
 }
 
@@ -417,6 +433,48 @@ static void char_matrices(double gam, double dnx, double dny, const double ul[4]
  * skipped as a boundary stencil, 1 when its vertex neighbourhood does not
  * match the structured pattern the reference expects (the reference stops
  * there, WENO.f90:7699; here the cell is skipped like a boundary stencil). */
+/* The 9-cell stencil of cell ie in the reference's order (WENO.f90:7641-
+ * 7708): the cell, its 4 face neighbours, then the 4 diagonal cells placed by
+ * the product of the face indices they share a vertex with.  Returns 0 for
+ * a full stencil, 2 for a boundary stencil (the reference leaves the cell
+ * unchanged, :7706-7708), 1 when a vertex neighbourhood matches none of the
+ * expected products (the reference stops there, :7699). */
+static int stencil_cells(const orc_case *c, int ie, const int *esqua, int ielaj[9])
+{
+    ielaj[0] = ie;
+    for (int f = 0; f < 4; ++f) ielaj[1 + f] = esqua[f + 4 * ie];
+    for (int k = 5; k < 9; ++k) ielaj[k] = -2; /* unset */
+    const int *ip = c->ipqua + 9 * ie;
+    int mismatch = 0;
+    for (int iv = 0; iv < 4; ++iv) {
+        int jel[3] = {-1, -1, -1}, nv = 0;
+        int p = ip[iv];
+        for (int k = c->esuv2[p]; k < c->esuv2[p + 1]; ++k) {
+            int je = c->esuv1[k];
+            if (je != ie && nv < 3) jel[nv++] = je;
+        }
+        int idxod = 1, idxiv[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a)
+            for (int f = 0; f < 4; ++f)
+                if (jel[a] == ielaj[1 + f]) {
+                    idxod = idxod * (f + 1);
+                    idxiv[a] = 1;
+                }
+        for (int a = 0; a < 3; ++a) {
+            if (idxiv[a]) continue;
+            if (idxod == 2) ielaj[5] = jel[a];
+            else if (idxod == 6) ielaj[6] = jel[a];
+            else if (idxod == 12) ielaj[7] = jel[a];
+            else if (idxod == 4) ielaj[8] = jel[a];
+            else mismatch = 1;
+        }
+    }
+    if (mismatch) return 1;
+    for (int k = 0; k < 9; ++k)
+        if (ielaj[k] < 0) return 2;
+    return 0;
+}
+
 static int weno_cell(const weno_env *E, int ie, const double m[4], const int *esqua, double *unkno)
 {
     const orc_case *c = E->c;
@@ -724,6 +782,244 @@ static int weno_cell(const weno_env *E, int ie, const double m[4], const int *es
     return 0;
 }
 
+/* weno_stncl_quad_orthol3 (WENO.f90:7541-8098) for npoly = 1, with
+ * getgeophy_local_Taylor (:6750-7001), getunknp_local_Taylor (:7005-7215)
+ * and weno_cell_Taylor (:7219-7537), nbsis = 1.  Three quirks of the
+ * reference's P1 code decide its results and are reproduced:
+ *  - the Lagrange stencils fit 3 equations in 2 unknowns (matlg is 3 x 2),
+ *    and Matinv225 is called on matlg itself with n = 2, so its 2 x 2
+ *    explicit-shape dummy sees the first four elements of matlg in memory
+ *    (column-major): a = [matlg(1,1) matlg(3,1); matlg(2,1) matlg(1,2)]
+ *    (WENO.f90:7855-7858); the update then uses the first 2 equations;
+ *  - weno_cell_Taylor's npoly = 1 smoothness indicator is the P2 formula
+ *    (WENO.f90:7375-7392): it reads unksf_cha(4:6, iq, is) of an array
+ *    dimensioned (ndegr = 3, nq, nsten), i.e. the elements at the flat
+ *    offsets 3..5 past (1, iq, is) -- the next variables' (or the next
+ *    stencil's) leading entries, or zeros;
+ *  - there are 5 candidates (centre + 4 Lagrange), weights 0.2 and 0.2. */
+static int weno_cell_p1(const weno_env *E, int ie, const double m[4], const int *ielaj, double *unkno)
+{
+    const orc_case *c = E->c;
+    double unkrl[9][4][3];
+    for (int s = 0; s < 9; ++s)
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < 3; ++id) unkrl[s][q][id] = U(c, E->unknr, id, q, ielaj[s]);
+    for (int s = 0; s < 9; ++s)
+        for (int id = 0; id < 3; ++id) rot(m, &unkrl[s][1][id], &unkrl[s][2][id]);
+
+    /* getgeophy_local_Taylor: local centres, the P2 offsets (nbsis = 1) */
+    double geopl[9][5];
+    cell_pts P[9];
+    double dxcj[9];
+    for (int s = 0; s < 9; ++s) {
+        int je = ielaj[s];
+        double xcl = E->geoph[2 * je], ycl = E->geoph[2 * je + 1];
+        rot(m, &xcl, &ycl);
+        geopl[s][0] = xcl;
+        geopl[s][1] = ycl;
+        dxcj[s] = sqrt(GE(c, 2, je));
+        cell_points(E, je, m, &P[s]);
+        double v1 = 0.0, v2 = 0.0, v3 = 0.0;
+        for (int g = 0; g < 25; ++g) {
+            double b2 = (P[s].xl[g] - xcl) / dxcj[s];
+            double b3 = (P[s].yl[g] - ycl) / dxcj[s];
+            double rd = P[s].rhogi[g] * P[s].djacoi[g];
+            v1 = v1 + rd * b2 * b2;
+            v2 = v2 + rd * b3 * b3;
+            v3 = v3 + rd * b2 * b3;
+        }
+        double masel = GE(c, 3, je);
+        v1 = v1 / masel;
+        v2 = v2 / masel;
+        v3 = v3 / masel;
+        geopl[s][2] = 0.5 * v1;
+        geopl[s][3] = 0.5 * v2;
+        geopl[s][4] = v3;
+    }
+
+    /* getunknp_local_Taylor with 2 x 2 matrices */
+    double unkpl[9][4][3];
+    double mtbbq[4], mtbbi[4];
+    for (int s = 0; s < 9; ++s) {
+        int je = ielaj[s];
+        double rcj = GE(c, 0, je), scj = GE(c, 1, je);
+        double bb[4] = {0.0, 0.0, 0.0, 0.0}, bbq[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int g = 0; g < 25; ++g) {
+            double bq[3];
+            bq[0] = 1.0;
+            bq[1] = (E->pos[2 * g] - rcj) / 1.0;
+            bq[2] = (E->pos[2 * g + 1] - scj) / 1.0;
+            double bl[3];
+            bl[0] = 1.0;
+            bl[1] = (P[s].xl[g] - geopl[s][0]) / dxcj[s];
+            bl[2] = (P[s].yl[g] - geopl[s][1]) / dxcj[s];
+            double rd = P[s].rhogi[g] * P[s].djacoi[g];
+            for (int i = 1; i < 3; ++i)
+                for (int j = 1; j < 3; ++j) {
+                    bb[(i - 1) + 2 * (j - 1)] = bb[(i - 1) + 2 * (j - 1)] + rd * bq[i] * bl[j];
+                    bbq[(i - 1) + 2 * (j - 1)] = bbq[(i - 1) + 2 * (j - 1)] + rd * bl[i] * bl[j];
+                }
+        }
+        double rhs[4][3];
+        for (int q = 0; q < 4; ++q)
+            for (int i = 0; i < 3; ++i) rhs[q][i] = 0.0;
+        for (int i = 1; i < 3; ++i)
+            for (int j = 1; j < 3; ++j)
+                for (int q = 0; q < 4; ++q) rhs[q][i] = rhs[q][i] + unkrl[s][q][j] * bb[(j - 1) + 2 * (i - 1)];
+        double bbqi[4], bbi[4];
+        inv2(bbq, bbqi);
+        inv2(bb, bbi);
+        for (int q = 0; q < 4; ++q)
+            for (int i = 0; i < 3; ++i) unkpl[s][q][i] = 0.0;
+        for (int i = 1; i < 3; ++i)
+            for (int j = 1; j < 3; ++j)
+                for (int q = 0; q < 4; ++q) unkpl[s][q][i] = unkpl[s][q][i] + rhs[q][j] * bbqi[(j - 1) + 2 * (i - 1)];
+        if (s == 0) {
+            memcpy(mtbbq, bbq, sizeof(bbq));
+            memcpy(mtbbi, bbi, sizeof(bbi));
+        }
+        for (int q = 0; q < 4; ++q) unkpl[s][q][0] = unkrl[s][q][0];
+    }
+
+    /* candidates: the centre polynomial and 4 Lagrange stencils of 3 cells */
+    static const int ST1[4][3] = {{1, 4, 5}, {2, 1, 6}, {3, 2, 7}, {4, 3, 8}};
+    double unksl[5][4][3];
+    memset(unksl, 0, sizeof(unksl));
+    for (int s = 0; s < 5; ++s)
+        for (int q = 0; q < 4; ++q) unksl[s][q][0] = unkpl[0][q][0];
+    for (int q = 0; q < 4; ++q)
+        for (int i = 0; i < 3; ++i) unksl[0][q][i] = unkpl[0][q][i];
+    const double dxcl = sqrt(GE(c, 2, ie));
+    const double xmcl = geopl[0][0], ymcl = geopl[0][1];
+    for (int is = 0; is < 4; ++is) {
+        const int isten = is + 1;
+        double matlg[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}; /* matlg(3, 2), column-major */
+        double rhsl[4][3];
+        for (int e = 0; e < 3; ++e) {
+            int sj = ST1[is][e];
+            double masej = GE(c, 3, ielaj[sj]);
+            for (int g = 0; g < 25; ++g) {
+                double bl[3];
+                bl[1] = (P[sj].xl[g] - xmcl) / dxcl;
+                bl[2] = (P[sj].yl[g] - ymcl) / dxcl;
+                double rd = P[sj].rhogi[g] * P[sj].djacoi[g];
+                for (int k = 0; k < 2; ++k) matlg[e + 3 * k] = matlg[e + 3 * k] + rd * bl[k + 1] / masej;
+            }
+            for (int q = 0; q < 4; ++q) rhsl[q][e] = unkpl[sj][q][0] - unkpl[0][q][0];
+        }
+        /* Matinv225(matlg, matin, 2): the first four elements of matlg as a
+         * column-major 2 x 2 */
+        double mi[4];
+        inv2(matlg, mi);
+        for (int i = 1; i < 3; ++i)
+            for (int e = 0; e < 2; ++e)
+                for (int q = 0; q < 4; ++q)
+                    unksl[isten][q][i] = unksl[isten][q][i] + mi[(i - 1) + 2 * e] * rhsl[q][e];
+    }
+    const int nst = 5;
+
+    /* weno_cell_Taylor, npoly = 1 */
+    double xp[2][9];
+    q2_nodes(c, E->xc, ie, xp);
+    const double volel = GE(c, 2, ie);
+    const double dxc = sqrt(volel), dyc = dxc;
+    const double ct1 = geopl[0][2], ct2 = geopl[0][3], ct3 = geopl[0][4];
+    double unkf_phy[4][4][3];
+    memset(unkf_phy, 0, sizeof(unkf_phy));
+    int ifprj = 0;
+    static const int mapfe[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    for (int f = 0; f < 4; ++f) {
+        int je = E->esqua[f + 4 * ie];
+        if (je >= c->ncell) continue;
+        double dtx = xp[0][mapfe[f][1]] - xp[0][mapfe[f][0]];
+        double dty = xp[1][mapfe[f][1]] - xp[1][mapfe[f][0]];
+        double dlgt = sqrt(dtx * dtx + dty * dty);
+        double dnx = dty / dlgt, dny = -dtx / dlgt;
+        rot(m, &dnx, &dny);
+        double ul[4], ur[4];
+        for (int q = 0; q < 4; ++q) {
+            ul[q] = U(c, E->unknr, 0, q, ie);
+            ur[q] = U(c, E->unknr, 0, q, je);
+        }
+        rot(m, &ul[1], &ul[2]);
+        rot(m, &ur[1], &ur[2]);
+        double qm[4][4], qi[4][4];
+        char_matrices(c->gamlg, dnx, dny, ul, ur, qm, qi);
+        /* unksf_cha(ndegr = 3, nq = 4, nsten = 10), column-major, zeroed */
+        double cha[120];
+        memset(cha, 0, sizeof(cha));
+#define CHA(id, k, s) cha[(id) + 3 * (k) + 12 * (s)]
+        for (int s = 0; s < nst; ++s)
+            for (int q = 0; q < 4; ++q)
+                for (int id = 0; id < 3; ++id)
+                    for (int k = 0; k < 4; ++k) CHA(id, k, s) = CHA(id, k, s) + qi[k][q] * unksl[s][q][id];
+        double os1[5];
+        /* x**4 is evaluated as ((x*x)*x)*x by the reference's build (the
+         * squared form (x*x)*(x*x) differs in the last bit here) */
+        const double dx2 = dxc * dxc, dy2 = dyc * dyc, dx4 = dx2 * dxc * dxc, dy4 = dy2 * dyc * dyc;
+        const double dxy = dyc * dxc, dxy2 = dxy * dxy;
+        for (int s = 0; s < nst; ++s) {
+            double o[4];
+            for (int q = 0; q < 4; ++q) {
+                /* degrees 4..6 read past the first dimension (flat offsets) */
+                const double u2 = CHA(1, q, s), u3 = CHA(2, q, s);
+                const double u4 = cha[3 + 3 * q + 12 * s], u5 = cha[4 + 3 * q + 12 * s], u6 = cha[5 + 3 * q + 12 * s];
+                double sm1 = 1.0 / dx2 * (u2 * u2 + 2.0 * ct1 * (u4 * u4) + 2.0 * ct2 * (u6 * u6) + 2.0 * ct3 * u4 * u6);
+                double sm2 = 1.0 / dy2 * (u3 * u3 + 2.0 * ct2 * (u5 * u5) + 2.0 * ct1 * (u6 * u6) + 2.0 * ct3 * u5 * u6);
+                double sm3 = 1.0 / dx4 * (u4 * u4);
+                double sm4 = 1.0 / dy4 * (u5 * u5);
+                double sm5 = 1.0 / dxy2 * (u6 * u6);
+                o[q] = (sm1 + sm2) + (sm3 + sm4 + sm5) * volel;
+            }
+            os1[s] = dmax(dmax(dmax(o[0], o[1]), o[2]), o[3]);
+        }
+        double wl[5];
+        wl[0] = .2;
+        for (int s = 1; s < nst; ++s) wl[s] = 0.8 / ((double)nst - 1.0);
+        const double epsil = 1.e-6;
+        double wh[5], wt = 0.0;
+        for (int s = 0; s < nst; ++s) {
+            double t = epsil + os1[s];
+            wh[s] = wl[s] / (t * t);
+        }
+        for (int s = 0; s < nst; ++s) wt = wt + wh[s];
+        double fch[4][3];
+        memset(fch, 0, sizeof(fch));
+        for (int s = 0; s < nst; ++s)
+            for (int q = 0; q < 4; ++q)
+                for (int id = 0; id < 3; ++id) fch[q][id] = fch[q][id] + (wh[s] / wt) * CHA(id, q, s);
+#undef CHA
+        for (int q = 0; q < 4; ++q)
+            for (int id = 1; id < 3; ++id)
+                for (int k = 0; k < 4; ++k) unkf_phy[ifprj][k][id] = unkf_phy[ifprj][k][id] + qm[k][q] * fch[q][id];
+        ++ifprj;
+    }
+    double unk_phy[4][3];
+    memset(unk_phy, 0, sizeof(unk_phy));
+    const double wf = 1.0 / (double)ifprj;
+    for (int a = 0; a < ifprj; ++a)
+        for (int q = 0; q < 4; ++q)
+            for (int id = 1; id < 3; ++id) unk_phy[q][id] = unk_phy[q][id] + wf * unkf_phy[a][q][id];
+
+    /* back to the cell's Taylor basis and the global frame */
+    double rhsuk[4][2];
+    for (int q = 0; q < 4; ++q)
+        for (int i = 0; i < 2; ++i) rhsuk[q][i] = 0.0;
+    for (int i = 1; i < 3; ++i)
+        for (int j = 1; j < 3; ++j)
+            for (int q = 0; q < 4; ++q) rhsuk[q][i - 1] = rhsuk[q][i - 1] + unk_phy[q][j] * mtbbq[(j - 1) + 2 * (i - 1)];
+    double out[4][3];
+    for (int q = 0; q < 4; ++q)
+        for (int i = 0; i < 3; ++i) out[q][i] = 0.0;
+    for (int i = 1; i < 3; ++i)
+        for (int j = 1; j < 3; ++j)
+            for (int q = 0; q < 4; ++q) out[q][i] = out[q][i] + rhsuk[q][j - 1] * mtbbi[(j - 1) + 2 * (i - 1)];
+    for (int i = 1; i < 3; ++i) rotinv(m, &out[1][i], &out[2][i]);
+    for (int q = 0; q < 4; ++q)
+        for (int i = 1; i < 3; ++i) U(c, unkno, i, q, ie) = out[q][i];
+    return 0;
+}
+
 /* weno_char_quad_orthgl (WENO.f90:5927-6120) */
 static int weno(orc_case *c, const double *flag, const int *esqua)
 {
@@ -751,7 +1047,7 @@ static int weno(orc_case *c, const double *flag, const int *esqua)
             double djacoi = wi * (J[0] * J[3] - J[2] * J[1]);
             double xg, yg;
             q2_map(xp, r, s, &xg, &yg);
-            double rho = rho_initial(c, xcr, ycr);
+            double rho = c->rho0 ? c->rho0[ie] : rho_initial(c, xcr, ycr);
             xmc = xmc + rho * djacoi * xg;
             ymc = ymc + rho * djacoi * yg;
         }
@@ -774,10 +1070,29 @@ static int weno(orc_case *c, const double *flag, const int *esqua)
     E.unknr = unknr;
     int bad = 0;
     ORC_PHASE(ORC_PH_WENO);
+    E.esqua = esqua;
     for (int ie = 0; ie < nc; ++ie) {
         if (flag[ie] < 10.0) continue;
+        /* A cell at rest has the zero frame (WENO.f90:6066-6081): every
+         * stencil point maps to the origin, the normal equations vanish and
+         * Matinv225 returns 0/0.  Strict mode keeps that arithmetic (the
+         * reference's NaN); otherwise the cell keeps its DOFs, like a
+         * boundary stencil. */
+        const double *mz = mapmt + 4 * ie;
+        if (!c->weno_strict && mz[0] == 0.0 && mz[1] == 0.0 && mz[2] == 0.0 && mz[3] == 0.0) {
+            c->nweno_rest++;
+            continue;
+        }
         ORC_WENO_CELL();
-        int r = weno_cell(&E, ie, mapmt + 4 * ie, esqua, c->unold);
+        int r;
+        if (c->npoly == 1) {
+            int ielaj[9];
+            r = stencil_cells(c, ie, esqua, ielaj);
+            if (r == 0) weno_cell_p1(&E, ie, mapmt + 4 * ie, ielaj, c->unold);
+            if (r == 2) r = 0;
+        } else {
+            r = weno_cell(&E, ie, mapmt + 4 * ie, esqua, c->unold);
+        }
         if (r) {
             ++bad;
             c->nweno_skipped++;
@@ -840,7 +1155,7 @@ static void subcell_density(const orc_case *c, int ie, double rsub[4])
             double djaco = w[g] * (J[0] * J[3] - J[2] * J[1]);
             double xcc, ycc;
             q2_map(xi, 0.0, 0.0, &xcc, &ycc);
-            double rhoi = (c->ncase == 6) ? rho_initial(c, xcc, ycc) : 1.0;
+            double rhoi = c->rho0 ? c->rho0[ie] : ((c->ncase == 6) ? rho_initial(c, xcc, ycc) : 1.0);
             ms = ms + rhoi * 1.0 * djaco;
         }
         ORC_PHASE(ORC_PH_CORNER);
@@ -869,7 +1184,7 @@ static void subcell_tau(const orc_case *c, int ie, double uq[4])
             double b[6];
             basis_p2(r, s, rc, sc, &GE(c, 18, ie), b);
             double t = 0.0;
-            for (int id = 0; id < 6; ++id) t = t + U(c, c->unold, id, 0, ie) * b[id];
+            for (int id = 0; id < c->ndegr; ++id) t = t + U(c, c->unold, id, 0, ie) * b[id];
             t = rhomc + 1.0 * (t - rhomc); /* nlimi=6 with aflim(1)=1 */
             double J[4];
             q2_jac(xp, r, s, J, NULL, NULL);
@@ -1118,7 +1433,9 @@ static void domn_q2(orc_case *c)
 
 int orc_rhs_q2(orc_case *c)
 {
-    if (c->npoly != 2 || c->nrz != 0 || c->ndens != 1 || c->nlimi != 6 || !c->esqua) return 2;
+    if (!((c->npoly == 2 && c->ndegr == 6) || (c->npoly == 1 && c->ndegr == 3)) || c->nrz != 0 ||
+        c->ndens != 1 || c->nlimi != 6 || !c->esqua)
+        return 2;
     const int nc = c->ncell;
     double *gq = (double *)malloc(sizeof(double) * 48 * nc);
     double *flag = (double *)malloc(sizeof(double) * nc);
@@ -1126,7 +1443,15 @@ int orc_rhs_q2(orc_case *c)
     ORC_PHASE(ORC_PH_CORNER);
     fnds_q2(c, c->cooro, gq);
     ORC_PHASE(ORC_PH_PREP);
-    persson(c, c->cooro, flag);
+    if (c->npoly == 2) {
+        persson(c, c->cooro, flag);
+    } else {
+        /* P1: the Barth limiters run and are then discarded (aflim = 1,
+         * afvec = I below, Getrhslagmc.f90:7598-7604, :7650-7656); every cell
+         * is flagged troubled (geoel(10) = 100, :7607), so WENO reconstructs
+         * every cell whose stencil is complete */
+        for (int ie = 0; ie < nc; ++ie) flag[ie] = 100.0;
+    }
     int rc = weno(c, flag, c->esqua);
     if (rc) {
         free(gq); free(flag); free(fstar);

@@ -32,8 +32,10 @@ class _Case(C.Structure):
         ("geoel", _dp), ("amatr", _dp),
         ("unkno", _dp), ("unold", _dp), ("ustar", _dp),
         ("rhsel", _dp), ("aflim", _dp), ("afvec", _dp),
-        ("esqua", _ip), ("weno_strict", C.c_int), ("nweno_skipped", C.c_int), ("troubled", _dp),
+        ("esqua", _ip), ("weno_strict", C.c_int), ("nweno_skipped", C.c_int), ("nweno_rest", C.c_int),
+        ("troubled", _dp),
         ("rkstg", C.c_int), ("epsaw", C.c_double),
+        ("rho0", _dp),
     ]
 
 
@@ -96,7 +98,7 @@ def _p(a, t=_dp):
 class Oracle:
     """Holds one problem's state and advances it with the C restatement."""
 
-    def __init__(self, case, strict=False, counting=False):
+    def __init__(self, case, strict=False, counting=False, rho0=None):
         c = case.ctl
         self.lib = opcount_lib() if counting else lib()
         self.case = case
@@ -141,6 +143,10 @@ class Oracle:
             self.esqua = np.ascontiguousarray(face_neighbours(case.ipqua, case.esuv1, case.esuv2), dtype=np.int32)
         self.troubled = np.zeros(n)
         s.esqua = _p(self.esqua, _ip); s.weno_strict = 1 if strict else 0; s.troubled = _p(self.troubled)
+        # per-cell initial density in place of problem ncase's (as
+        # Lag2D(case, rho0=...) / lag2d_set_initial_density)
+        self.rho0 = None if rho0 is None else np.ascontiguousarray(rho0, dtype=np.float64).copy()
+        s.rho0 = _p(self.rho0) if self.rho0 is not None else None
         self.s = s
 
     def rhs(self):

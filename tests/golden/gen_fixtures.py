@@ -61,6 +61,17 @@ def _sedov_rz_hybrid():
     return mg.sedov_rz_hybrid_problem(16), "sedov_origin"
 
 
+ROTATION = 0.5
+
+
+def _sedov_q2_wall():
+    return mg.sedov_problem(16, seed=21), "sedov_wall"
+
+
+def _sod_annulus_p1():
+    return mg.sod_annulus_p1_problem(6, 24, seed=17), "rotation"
+
+
 SPECS = {
     # name: (builder, steps, dump-every)
     "sod_q1_100x4": (_sod, 400, 25),
@@ -71,6 +82,8 @@ SPECS = {
     "sedov_hyb_16x16": (_sedov_hybrid, 60, 20),
     "sedov_rz_16x16": (_sedov_rz_hybrid, 40, 10),
     "sod_rz_40x8": (_sod_rz_hybrid, 120, 30),
+    "sod_p1c_annulus": (_sod_annulus_p1, 20, 5),
+    "sedov_q2w_16x16": (_sedov_q2_wall, 60, 10),
 }
 
 
@@ -95,11 +108,26 @@ def make(name):
         nd = ctl.ndegr
         _run(work, 0, 1, False)
         ncell, npoin, nbfac, nafac = (int(v) for v in _rd(work, "init_sizes.bin"))
-        if ic in ("sedov", "sedov_centre", "sedov_origin"):
+        if ic == "rotation":
+            # the reference's own state plus a rigid rotation u = w (-y, x)
+            # about the origin at each cell's corner mean, added to the cell
+            # averages with the kinetic energy: every cell moves, so the
+            # velocity-aligned WENO frame is defined everywhere
+            # (WENO.f90:6066-6081 takes a zero frame, and 0/0, at rest)
+            un = _rd(work, "init_unkno.bin").reshape(-1, 4, nd)[:ncell].copy()
+            xy = mesh.coord[mesh.ipqua[:, :4] - 1].mean(axis=1)
+            u, v = -ROTATION * xy[:, 1], ROTATION * xy[:, 0]
+            un[:, 1, 0] += u
+            un[:, 2, 0] += v
+            un[:, 3, 0] += 0.5 * (u * u + v * v)
+            un.astype(np.float64).tofile(os.path.join(work, "inject.bin"))
+        if ic in ("sedov", "sedov_centre", "sedov_origin", "sedov_wall"):
             un = _rd(work, "init_unkno.bin").reshape(-1, 4, nd)[:ncell]
             ge = _rd(work, "init_geoel.bin").reshape(-1, 24)[:ncell]
             if ic == "sedov":
                 cells = (0,)
+            elif ic == "sedov_wall":  # the two middle cells of the bottom (slip-wall) row
+                cells = (mesh.nx // 2 - 1, mesh.nx // 2)
             elif ic == "sedov_centre":
                 cells = mg.centre_cells(mesh.nx, mesh.ny)
             else:  # every cell at the origin node (node 1), triangles numbered first

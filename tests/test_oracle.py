@@ -7,7 +7,7 @@ from fixtures import load, state_errors
 from oracle import Oracle
 
 CASES = ["sod_q1_100x4", "sedov_q1_16x16", "sedov_q2_12x12", "sedov_q2c_16x16", "sod_hyb_40x8", "sedov_hyb_16x16",
-         "sedov_rz_16x16", "sod_rz_40x8"]
+         "sedov_rz_16x16", "sod_rz_40x8", "sod_p1c_annulus", "sedov_q2w_16x16"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -91,3 +91,50 @@ def test_inverse5_is_an_inverse():
         L.test_inv(a.ctypes.data_as(P), b.ctypes.data_as(P))
         B = b.reshape(5, 5).T
         assert np.allclose(A @ B, np.eye(5), atol=1e-12)
+
+
+def test_curved_p1_weno_reconstructs_every_moving_cell():
+    """Curved P1 (ncurv=1, nfint=4, npoly=1): the reference flags every cell
+    (Getrhslagmc.f90:7607) and WENO reconstructs every cell with a full
+    stencil; on the rotating annulus every cell moves, so none is skipped,
+    and the oracle still matches the reference bit for bit
+    (test_steps_bit_exact)."""
+    case, z = load("sod_p1c_annulus")
+    o = Oracle(case)
+    o.step(1)
+    assert o.s.nweno_rest == 0 and o.s.nweno_skipped == 0
+
+
+def test_curved_p1_at_rest_is_nan_in_strict_mode():
+    """The reference's curved P1 WENO turns a cell at rest into NaN: its
+    velocity-aligned frame is the zero matrix (WENO.f90:6066-6081), every
+    stencil point maps to the origin and Matinv225 divides 0 by 0.  The
+    reference's own dump of the annulus started at rest shows it (the state
+    after one right-hand side is NaN in every interior cell).  Strict mode
+    keeps that arithmetic; the default keeps such cells' DOFs, like boundary
+    stencils, so the state stays finite."""
+    case, z = load("sod_p1c_annulus")
+    # the fixture's injected rotation removed again: the annulus at rest
+    xy = case.coord[case.ipqua[:, :4]].mean(axis=1)
+    u, v = -0.5 * xy[:, 1], 0.5 * xy[:, 0]
+    case.unkno[:, 3, 0] -= 0.5 * (u * u + v * v)
+    case.unkno[:, 1, 0] = 0.0
+    case.unkno[:, 2, 0] = 0.0
+    o = Oracle(case, strict=True)
+    o.rhs()
+    assert not np.isfinite(o.unold).all()
+    o = Oracle(case)
+    o.rhs()
+    assert np.isfinite(o.unold).all() and o.s.nweno_rest > 0
+
+
+def test_slip_wall_blast_matches_reference():
+    """Curved P2 blast against the bottom slip wall: troubled cells in the
+    wall row are boundary stencils, and the waves reflect off the wall; 10
+    steps bit-exact (the reference stops in its stencil builder later, when
+    the waves reach a side wall)."""
+    case, z = load("sedov_q2w_16x16")
+    assert list(z["steps"]) == [10]
+    o = Oracle(case)
+    o.step(10)
+    assert np.array_equal(o.unkno, z["s10_unkno"]) and np.array_equal(o.coord, z["s10_coord"])
