@@ -23,7 +23,7 @@ OBJS   := $(addprefix $(OUT)/obj/,$(addsuffix .o,$(UNITS)))
 
 all: $(OUT)/refobjs.a $(OUT)/lag2d_refdump
 
-drivers: $(OUT)/lag2d_drv_cpu $(OUT)/lag2d_drv_gpu
+drivers: $(OUT)/lag2d_drv_cpu $(OUT)/lag2d_drv_gpu $(OUT)/lag2d_drv_gpurhs
 
 $(OUT)/obj/dgflo2d.o: $(REFSRC)/dgflo2d.f90
 	@mkdir -p $(OUT)/obj $(OUT)/mod
@@ -64,3 +64,17 @@ $(OUT)/lag2d_drv_gpu: $(OUT)/obj/getsolvdg_lag_shim.o $(OUT)/obj/lag2d_c.o $(OBJ
 	    -L lag2d_amd -llag2d -Wl,-rpath,'$$ORIGIN/../../lag2d_amd'
 
 .PHONY: all drivers
+
+# The reference's program with its own getsolvdg_lag (the RK loop on the
+# host) and its getrhsdg_lag_mcnew demoted to a weak symbol, so that the
+# right-hand-side drop-in of lag2d_amd/fortran/getrhsdg_lag_mcnew_shim.f90
+# takes the per-stage call (Getrhslag.f90:1747-1750).
+REST_RHS := $(filter-out $(OUT)/obj/Getrhslagmc.o,$(OBJS))
+
+$(OUT)/obj/getrhsdg_lag_mcnew_shim.o: $(SHIM)/getrhsdg_lag_mcnew_shim.f90 $(OUT)/obj/lag2d_c.o
+	$(FC) $(FFLAGS) -c $< -o $@
+
+$(OUT)/lag2d_drv_gpurhs: $(OUT)/obj/getrhsdg_lag_mcnew_shim.o $(OUT)/obj/lag2d_c.o $(OBJS) lag2d_amd/liblag2d.so
+	objcopy --weaken-symbol=getrhsdg_lag_mcnew_ $(OUT)/obj/Getrhslagmc.o $(OUT)/obj/Getrhslagmc_weak.o
+	$(FC) -o $@ $(OUT)/obj/getrhsdg_lag_mcnew_shim.o $(OUT)/obj/lag2d_c.o $(OUT)/obj/Getrhslagmc_weak.o $(REST_RHS) \
+	    -L lag2d_amd -llag2d -Wl,-rpath,'$$ORIGIN/../../lag2d_amd'

@@ -138,3 +138,21 @@ def test_slip_wall_blast_matches_reference():
     o = Oracle(case)
     o.step(10)
     assert np.array_equal(o.unkno, z["s10_unkno"]) and np.array_equal(o.coord, z["s10_coord"])
+
+
+def test_initial_density_array_equal_to_the_problem_density_changes_nothing():
+    """The per-cell initial density (the oracle's rho0, the library's
+    lag2d_set_initial_density) replaces problem ncase's density in the
+    initial sub-cell masses and the WENO mass centres.  An array equal to
+    that density (1 everywhere for the Sedov problem, ncase 7) must give the
+    built-in path's results bit for bit; a different one must not."""
+    case, z = load("sedov_q2c_16x16")
+    a = Oracle(case)
+    b = Oracle(case, rho0=np.ones(case.ncell))
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.unkno, b.unkno) and np.array_equal(a.coord, b.coord)
+    assert np.array_equal(a.unkno, z["s10_unkno"])
+    c = Oracle(case, rho0=np.full(case.ncell, 1.0 + 1e-3))
+    c.step(10)
+    assert not np.array_equal(a.unkno, c.unkno)
