@@ -550,6 +550,16 @@ const char *const lag2d_kernel_names[KID_COUNT] = {
 
 static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
 
+// Name of kernel id k as timed on this context: the curved path's corner and
+// cell-RHS slots name the variant the autotuner kept.
+static const char *kernel_name(const lag2d_ctx *x, int k)
+{
+    if (k == KID_Q2_CORNER && x->corner_variant == 2) return "k_q2_corner_sc";
+    if (k == KID_CELLRHS_Q2 && x->rhs_variant == 2) return "k_cellrhs_q2_tpc";
+    if (k == KID_Q2_WENO && x->nd == 3) return "k_c1_weno";
+    return lag2d_kernel_names[k];
+}
+
 // The dtfix and epsaw that getsolvdg_lag sets at the start of step itime
 // (Getrhslag.f90:1663-1737) for the problems that fix them; returns whether
 // dt is fixed.  epsaw (the RZ basis-motion lead) is 0.1 otherwise.
@@ -1233,7 +1243,7 @@ extern "C" int lag2d_kernel_time(lag2d_ctx *x, int enable, double *avg_ms, const
     }
     if (best < 0) return fail(LAG2D_EINVAL, "no timed launches");
     if (avg_ms) *avg_ms = best_avg;
-    if (name) *name = lag2d_kernel_names[best];
+    if (name) *name = kernel_name(x, best);
     return 0;
 }
 
@@ -1242,7 +1252,7 @@ extern "C" int lag2d_kernel_stats(lag2d_ctx *x, int k, double *total_ms, int *ca
     if (!x || k < 0 || k >= KID_COUNT) return fail(LAG2D_EINVAL, "bad argument");
     if (total_ms) *total_ms = x->kernel_total_ms[k];
     if (calls) *calls = x->kernel_calls[k];
-    if (name) *name = lag2d_kernel_names[k];
+    if (name) *name = kernel_name(x, k);
     return 0;
 }
 
