@@ -71,8 +71,8 @@ struct Dev {
     double *frame;           // [4][nc]  principal-strain frame (row-major 2x2)
     double *nmm;             // [10][np] node max (0..4) and min (5..9)
     double *lim;             // [9][nc]  aflim(1..5), afvec (col-major 2x2)
-    double *dlt;             // [8][nc]  |du.n| of each corner face's impedance, recorded for the
-                             //          adjoint's frozen forward (null otherwise; straight P1 quads)
+    double *dlt;             // |du.n| of each corner face's impedance, recorded for the adjoint's
+                             // frozen forward: [8][nc] straight P1 quads, [32][nc] curved (null otherwise)
     double *rec;             // corner-face records
     double *red;             // reduction scratch
     double *scal;            // device scalars: 0 dtfix, 1 tacru
@@ -106,6 +106,14 @@ struct Dev {
     // cells, flags, real faces) and rho |J| w at the 25 Gauss points [25][nc]
     const int *wtop;
     const double *wrd;
+    // the curved path's frozen switches of the adjoint's recorded forward
+    // (null otherwise): the stage's WENO increment U_lim - U_in [4 nd][nc],
+    // and which pressure and sound-speed clamps max(eps, .) chose eps, per
+    // cell: corner kernel [18][nc] (centre pressure, sound speed, the 16
+    // sub-cell vertex pressures), domain integral [25][nc] (its points);
+    // the impedances' |du.n| live in dlt [32][nc]
+    double *wdu;
+    uint8_t *ccl, *dcl;
 };
 
 // Quotients a / b by one divisor b, bit-identical to the IEEE division.

@@ -22,126 +22,9 @@
 #include "lag2d_dev.h"
 #include "lag2d_host.h"
 #include "lag2d_kernels.h"
+#include "lag2d_q2geom.h"
 
 namespace lag2d {
-
-// sub-cell vertex -> local Q2 node (ipqsg)
-__constant__ static const int IPQSG[4][4] = {{0, 4, 8, 7}, {4, 1, 5, 8}, {8, 5, 2, 6}, {7, 8, 6, 3}};
-// signed face-normal slots (fnqsg): sign = orientation, |v|-1 = gesgq slot
-__constant__ static const int FNQSG[4][8] = {{8, 1, 13, -9, -9, 12, 12, 16},
-                                             {9, 13, 2, 3, 14, -10, -10, 9},
-                                             {-11, 10, 10, 14, 4, 5, 15, -11},
-                                             {16, -12, -12, 11, 11, 15, 6, 7}};
-__constant__ static const double XV9[9] = {-1.0, 1.0, 1.0, -1.0, 0.0, 1.0, 0.0, -1.0, 0.0};
-__constant__ static const double YV9[9] = {-1.0, -1.0, 1.0, 1.0, -1.0, 0.0, 1.0, 0.0, 0.0};
-// Riemann corner weights times 1/4 [isg][ivsg][f] (Getrhslaggauss.f90:10575-10599)
-__constant__ static double WFR[4][4][2];
-// face-integral weights: momentum/energy {0,1} and volume times 1/4 (:8573-8621)
-__constant__ static double WFM[4][4][2];
-__constant__ static double WFV[4][4][2];
-// 5x5 Gauss rule of ruqope (dgflo2d.f90:2734-2743 ordering)
-__constant__ static const double PG5[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
-                                           0.5384693101056830910363144, 0.9061798459386639927976269};
-__constant__ static const double WG5[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
-                                           0.5688888888888888888888889, 0.4786286704993664680412915,
-                                           0.2369268850561890875142640};
-// 5-point Gauss-Lobatto (ruqope_lobatto, dgflo2d.f90:2571-2583)
-__constant__ static const double PL5[5] = {-1.0, -0.6546536707079771437983, 0.0, 0.6546536707079771437983, 1.0};
-__constant__ static const double WL5[5] = {0.1, 0.544444444444444444444, 0.7111111111111111111111,
-                                           0.544444444444444444444, 0.1};
-
-// Shape-function table of the fixed quadrature points (k_q2_shape_tables):
-// derivatives at 25 Gauss points, 4 x 25 sub-cell points and 25 Lobatto
-// points, then the shape values at the 25 Gauss points.
-constexpr int SHP_POINTS = 150;
-constexpr int SHP_STRIDE = 18;   // dN/dr[9], dN/ds[9]
-
-__device__ __forceinline__ void shape9(double r, double s, double *sh, double *dr_, double *ds_)
-{
-    const double c10 = 1.0;
-    sh[0] = 0.25 * (c10 - r) * r * s * (c10 - s);
-    sh[1] = -0.25 * (c10 + r) * (c10 - s) * r * s;
-    sh[2] = 0.25 * (c10 + r) * (c10 + s) * r * s;
-    sh[3] = -0.25 * (c10 - r) * (c10 + s) * r * s;
-    sh[4] = -0.5 * (c10 - r * r) * (c10 - s) * s;
-    sh[5] = 0.5 * (c10 + r) * (c10 - s * s) * r;
-    sh[6] = 0.5 * (c10 - r * r) * (c10 + s) * s;
-    sh[7] = -0.5 * (c10 - r) * (c10 - s * s) * r;
-    sh[8] = (c10 - r * r) * (c10 - s * s);
-    dr_[0] = 0.25 * (2.0 * r - c10) * s * (s - 1.0);
-    dr_[1] = 0.25 * (2.0 * r + c10) * s * (s - 1.0);
-    dr_[2] = 0.25 * (2.0 * r + c10) * s * (s + 1.0);
-    dr_[3] = 0.25 * (2.0 * r - c10) * s * (s + 1.0);
-    dr_[4] = -r * s * (s - c10);
-    dr_[5] = 0.5 * (2.0 * r + c10) * (c10 - s * s);
-    dr_[6] = -r * s * (s + c10);
-    dr_[7] = 0.5 * (2.0 * r - c10) * (1.0 - s * s);
-    dr_[8] = -2.0 * r * (1.0 - s * s);
-    ds_[0] = 0.25 * (2.0 * s - c10) * r * (r - 1.0);
-    ds_[1] = 0.25 * (2.0 * s - c10) * r * (r + 1.0);
-    ds_[2] = 0.25 * (2.0 * s + c10) * r * (r + 1.0);
-    ds_[3] = 0.25 * (2.0 * s + c10) * r * (r - 1.0);
-    ds_[4] = 0.5 * (1.0 - r * r) * (2.0 * s - c10);
-    ds_[5] = -r * s * (r + c10);
-    ds_[6] = 0.5 * (1.0 - r * r) * (2.0 * s + c10);
-    ds_[7] = -r * s * (r - c10);
-    ds_[8] = -2.0 * s * (1.0 - r * r);
-}
-
-__device__ __forceinline__ void jac9(const double *xp, const double *yp, double r, double s, double J[4])
-{
-    double sh[9], dr_[9], ds_[9];
-    shape9(r, s, sh, dr_, ds_);
-    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
-    for (int k = 0; k < 9; ++k) {
-        dxdr = dxdr + dr_[k] * xp[k];
-        dxds = dxds + ds_[k] * xp[k];
-        dydr = dydr + dr_[k] * yp[k];
-        dyds = dyds + ds_[k] * yp[k];
-    }
-    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
-}
-
-__device__ __forceinline__ void map9(const double *xp, const double *yp, double r, double s, double *x, double *y)
-{
-    double sh[9], dr_[9], ds_[9];
-    shape9(r, s, sh, dr_, ds_);
-    double a = 0.0, b = 0.0;
-    for (int k = 0; k < 9; ++k) {
-        a = a + sh[k] * xp[k];
-        b = b + sh[k] * yp[k];
-    }
-    *x = a;
-    *y = b;
-}
-
-__device__ __forceinline__ void basis6(double r, double s, double rc, double sc, double g0, double g1, double g2,
-                                       double *b)
-{
-    b[0] = 1.0;
-    b[1] = (r - rc) / 1.0;
-    b[2] = (s - sc) / 1.0;
-    b[3] = 0.5 * b[1] * b[1] - g0;
-    b[4] = 0.5 * b[2] * b[2] - g1;
-    b[5] = b[1] * b[2] - g2;
-}
-
-__device__ __forceinline__ void load9(const Dev &d, const double *x, const double *y, int ie, double *xp, double *yp)
-{
-    for (int k = 0; k < 9; ++k) {
-        int p = d.ipq[k * d.nc + ie];
-        xp[k] = x[p];
-        yp[k] = y[p];
-    }
-}
-
-// Initial density (getrhoig_quadcurv2 / getrhog_initial) for the
-// piecewise-constant problems: Sod splits on the initial cell centre.
-__device__ __forceinline__ double rho0(int ncase, double xc, double yc)
-{
-    if (ncase == 6) return (sqrt(xc * xc + (yc - 0.0) * (yc - 0.0)) <= 0.5) ? 1.0 : 0.125;
-    return 1.0;
-}
 
 #define UD(buf, q, id, ie) (buf)[((q) * 6 + (id)) * nc + (ie)]
 // the same for kernels templated on the number of DOFs per variable ND
@@ -1298,7 +1181,24 @@ __global__ void __launch_bounds__(256) k_q2_shape_tables(double *tab)
 }
 
 
-template <int ND>
+// Frozen switches of the adjoint's recorded forward (MODE 1 records them,
+// MODE 2 replays them, MODE 0 is the plain step): a max(eps, v) clamp and
+// the impedance's |du.n|.
+template <int MODE>
+__device__ __forceinline__ double clamp_eps(double eps, double v, uint8_t *tape, size_t k)
+{
+    if constexpr (MODE == 0) {
+        return dmax(eps, v);
+    } else if constexpr (MODE == 1) {
+        const bool c = eps > v;
+        tape[k] = c ? 1 : 0;
+        return c ? eps : v;
+    } else {
+        return tape[k] ? eps : v;
+    }
+}
+
+template <int ND, int MODE = 0>
 __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
 {
     __shared__ double stab[125 * SHP_STRIDE];
@@ -1413,8 +1313,13 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
         for (int id = 0; id < ND; ++id) u[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
     const double rhomc = u[0][0], uctr = u[1][0], vctr = u[2][0], ectr = u[3][0];
     const double rhoct = 1.0 / rhomc;
-    const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
-    const double sdctr = sqrt(dmax(eps, d.gamlg * pctr / rhoct));
+    // the centre's clamps are the same on the 4 lanes of a cell: lane 0 records
+    uint8_t cdummy[2];
+    uint8_t *ctape = (MODE == 1 && isg != 0) ? cdummy : d.ccl;
+    const size_t co = (MODE == 1 && isg != 0) ? 0 : (size_t)ie;
+    const size_t cs = (MODE == 1 && isg != 0) ? 1 : (size_t)nc;
+    const double pctr = clamp_eps<MODE>(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)), ctape, co);
+    const double sdctr = sqrt(clamp_eps<MODE>(eps, d.gamlg * pctr / rhoct, ctape, co + cs));
     const double dcor = d.cdrho * (rsub - 1.0 / uq);
     for (int iv = 0; iv < 4; ++iv) {
         // ---- sub-cell corner (isg, iv)
@@ -1431,7 +1336,8 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
         const double rhovt = 1.0 / un[0];
         const double rhovsg = rhovt + dcor;
         const double uv = un[1], vv = un[2], ev = un[3];
-        const double pv = dmax(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)));
+        const double pv =
+            clamp_eps<MODE>(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)), d.ccl, (size_t)(2 + 4 * isg + iv) * nc + ie);
         const double dux = d.usx[p] - uv, duy = d.usy[p] - vv;
         const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
         for (int f = 0; f < 2; ++f) {
@@ -1440,7 +1346,14 @@ __global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
             const double sg = tt > 0 ? 1.0 : -1.0;
             const double nx = sg * sgq[cl][3 * sl + 0], ny = sg * sgq[cl][3 * sl + 1];
             const double area = sgq[cl][3 * sl + 2] * WFR[isg][iv][f];
-            const double deltu = fabs(dux * nx + duy * ny);
+            double deltu;
+            const size_t od = (size_t)(f + 2 * (iv + 4 * isg)) * nc + ie;
+            if constexpr (MODE == 2) {
+                deltu = d.dlt[od];
+            } else {
+                deltu = fabs(dux * nx + duy * ny);
+                if constexpr (MODE == 1) d.dlt[od] = deltu;
+            }
             const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
             double m11;
             if (a3 < 1.e12) {
@@ -1731,7 +1644,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
 // the shape-function derivatives are read from the table as wave-uniform
 // loads.  Every residual entry accumulates in the same order as in
 // k_cellrhs_q2: identical results.
-template <int ND>
+template <int ND, int MODE = 0>
 __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1810,7 +1723,8 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int 
                 for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
             const double rhoad = 1.0 / un[0];
             const double uu = un[1], vv = un[2];
-            const double pres = dmax(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)));
+            const double pres = clamp_eps<MODE>(eps, g1 * rhoad * (un[3] - 0.5 * (uu * uu + vv * vv)), d.dcl,
+                                                (size_t)k * nc + ie);
             for (int id = 0; id < ND; ++id) {
                 const double a = dyds * dbdr[id] + (-dydr) * dbds[id];
                 const double c = (-dxds) * dbdr[id] + dxdr * dbds[id];
@@ -2212,40 +2126,6 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
 #undef UD
 #undef UDN
 
-// ---- host-side table set-up ------------------------------------------
-static int q2_tables()
-{
-    static bool done = false;
-    if (done) return 0;
-    const double rt[4][4][2] = {
-        {{4.0 / 6.0, 4.0 / 6.0}, {4.0 / 3.0, 8.0 / 3.0}, {1.0, 1.0}, {8.0 / 3.0, 4.0 / 3.0}},
-        {{8.0 / 3.0, 4.0 / 3.0}, {4.0 / 6.0, 4.0 / 6.0}, {4.0 / 3.0, 8.0 / 3.0}, {1.0, 1.0}},
-        {{1.0, 1.0}, {8.0 / 3.0, 4.0 / 3.0}, {4.0 / 6.0, 4.0 / 6.0}, {4.0 / 3.0, 8.0 / 3.0}},
-        {{4.0 / 3.0, 8.0 / 3.0}, {1.0, 1.0}, {8.0 / 3.0, 4.0 / 3.0}, {4.0 / 6.0, 4.0 / 6.0}}};
-    const double mt[4][4][2] = {{{1, 1}, {1, 1}, {0, 0}, {1, 1}},
-                                {{1, 1}, {1, 1}, {1, 1}, {0, 0}},
-                                {{0, 0}, {1, 1}, {1, 1}, {1, 1}},
-                                {{1, 1}, {0, 0}, {1, 1}, {1, 1}}};
-    const double vt[4][4][2] = {
-        {{4.0 / 6.0, 4.0 / 6.0}, {4.0 / 3.0, 0.0}, {0.0, 0.0}, {0.0, 4.0 / 3.0}},
-        {{0.0 / 3.0, 4.0 / 3.0}, {4.0 / 6.0, 4.0 / 6.0}, {4.0 / 3.0, 0.0 / 3.0}, {0.0, 0.0}},
-        {{0.0, 0.0}, {0.0 / 3.0, 4.0 / 3.0}, {4.0 / 6.0, 4.0 / 6.0}, {4.0 / 3.0, 0.0 / 3.0}},
-        {{4.0 / 3.0, 0.0 / 3.0}, {0.0, 0.0}, {0.0 / 3.0, 4.0 / 3.0}, {4.0 / 6.0, 4.0 / 6.0}}};
-    double wfr[4][4][2], wfm[4][4][2], wfv[4][4][2];
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b)
-            for (int f = 0; f < 2; ++f) {
-                wfr[a][b][f] = 0.25 * rt[a][b][f];
-                wfm[a][b][f] = mt[a][b][f];
-                wfv[a][b][f] = 0.25 * vt[a][b][f];
-            }
-    if (hipMemcpyToSymbol(HIP_SYMBOL(WFR), wfr, sizeof(wfr)) != hipSuccess) return LAG2D_EDEVICE;
-    if (hipMemcpyToSymbol(HIP_SYMBOL(WFM), wfm, sizeof(wfm)) != hipSuccess) return LAG2D_EDEVICE;
-    if (hipMemcpyToSymbol(HIP_SYMBOL(WFV), wfv, sizeof(wfv)) != hipSuccess) return LAG2D_EDEVICE;
-    done = true;
-    return 0;
-}
-
 }  // namespace lag2d
 
 using namespace lag2d;
@@ -2334,8 +2214,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_weno_setup(Dev d, int *wtop,
 int upload_extra(lag2d_ctx &x)
 {
     if (x.cfg.ncurv != 1) return 0;
-    int rc = q2_tables();
-    if (rc) return rc;
+    int rc = 0;
     if (!x.d.shp) {
         double *t;
         if ((rc = x.alloc(&t, (size_t)SHP_POINTS * SHP_STRIDE + 25 * 9))) return rc;

@@ -1,0 +1,144 @@
+// Reference-element tables and geometry helpers of the curved (Q2) path,
+// shared by its forward kernels (lag2d_q2.hip) and their reverse
+// (lag2d_adjq2.hip).
+#ifndef LAG2D_Q2GEOM_H
+#define LAG2D_Q2GEOM_H
+#include "lag2d_dev.h"
+
+namespace lag2d {
+
+// sub-cell vertex -> local Q2 node (ipqsg)
+__constant__ static const int IPQSG[4][4] = {{0, 4, 8, 7}, {4, 1, 5, 8}, {8, 5, 2, 6}, {7, 8, 6, 3}};
+// signed face-normal slots (fnqsg): sign = orientation, |v|-1 = gesgq slot
+__constant__ static const int FNQSG[4][8] = {{8, 1, 13, -9, -9, 12, 12, 16},
+                                             {9, 13, 2, 3, 14, -10, -10, 9},
+                                             {-11, 10, 10, 14, 4, 5, 15, -11},
+                                             {16, -12, -12, 11, 11, 15, 6, 7}};
+__constant__ static const double XV9[9] = {-1.0, 1.0, 1.0, -1.0, 0.0, 1.0, 0.0, -1.0, 0.0};
+__constant__ static const double YV9[9] = {-1.0, -1.0, 1.0, 1.0, -1.0, 0.0, 1.0, 0.0, 0.0};
+// Riemann corner weights times 1/4 [isg][ivsg][f] (Getrhslaggauss.f90:10575-10599)
+__constant__ static const double WFR[4][4][2] = {
+    {{0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * 1.0, 0.25 * 1.0},
+     {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}},
+    {{0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)},
+     {0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * 1.0, 0.25 * 1.0}},
+    {{0.25 * 1.0, 0.25 * 1.0}, {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)},
+     {0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}},
+    {{0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * 1.0, 0.25 * 1.0}, {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)},
+     {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}}};
+// face-integral weights: momentum/energy {0,1} and volume times 1/4 (:8573-8621)
+__constant__ static const double WFM[4][4][2] = {{{1, 1}, {1, 1}, {0, 0}, {1, 1}},
+                                                 {{1, 1}, {1, 1}, {1, 1}, {0, 0}},
+                                                 {{0, 0}, {1, 1}, {1, 1}, {1, 1}},
+                                                 {{1, 1}, {0, 0}, {1, 1}, {1, 1}}};
+__constant__ static const double WFV[4][4][2] = {
+    {{0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}, {0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}},
+    {{0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}, {0.0, 0.0}},
+    {{0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}},
+    {{0.25 * (4.0 / 3.0), 0.0}, {0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}}};
+// 5x5 Gauss rule of ruqope (dgflo2d.f90:2734-2743 ordering)
+__constant__ static const double PG5[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
+                                           0.5384693101056830910363144, 0.9061798459386639927976269};
+__constant__ static const double WG5[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
+                                           0.5688888888888888888888889, 0.4786286704993664680412915,
+                                           0.2369268850561890875142640};
+// 5-point Gauss-Lobatto (ruqope_lobatto, dgflo2d.f90:2571-2583)
+__constant__ static const double PL5[5] = {-1.0, -0.6546536707079771437983, 0.0, 0.6546536707079771437983, 1.0};
+__constant__ static const double WL5[5] = {0.1, 0.544444444444444444444, 0.7111111111111111111111,
+                                           0.544444444444444444444, 0.1};
+
+// Shape-function table of the fixed quadrature points (k_q2_shape_tables):
+// derivatives at 25 Gauss points, 4 x 25 sub-cell points and 25 Lobatto
+// points, then the shape values at the 25 Gauss points.
+constexpr int SHP_POINTS = 150;
+constexpr int SHP_STRIDE = 18;   // dN/dr[9], dN/ds[9]
+
+__device__ __forceinline__ void shape9(double r, double s, double *sh, double *dr_, double *ds_)
+{
+    const double c10 = 1.0;
+    sh[0] = 0.25 * (c10 - r) * r * s * (c10 - s);
+    sh[1] = -0.25 * (c10 + r) * (c10 - s) * r * s;
+    sh[2] = 0.25 * (c10 + r) * (c10 + s) * r * s;
+    sh[3] = -0.25 * (c10 - r) * (c10 + s) * r * s;
+    sh[4] = -0.5 * (c10 - r * r) * (c10 - s) * s;
+    sh[5] = 0.5 * (c10 + r) * (c10 - s * s) * r;
+    sh[6] = 0.5 * (c10 - r * r) * (c10 + s) * s;
+    sh[7] = -0.5 * (c10 - r) * (c10 - s * s) * r;
+    sh[8] = (c10 - r * r) * (c10 - s * s);
+    dr_[0] = 0.25 * (2.0 * r - c10) * s * (s - 1.0);
+    dr_[1] = 0.25 * (2.0 * r + c10) * s * (s - 1.0);
+    dr_[2] = 0.25 * (2.0 * r + c10) * s * (s + 1.0);
+    dr_[3] = 0.25 * (2.0 * r - c10) * s * (s + 1.0);
+    dr_[4] = -r * s * (s - c10);
+    dr_[5] = 0.5 * (2.0 * r + c10) * (c10 - s * s);
+    dr_[6] = -r * s * (s + c10);
+    dr_[7] = 0.5 * (2.0 * r - c10) * (1.0 - s * s);
+    dr_[8] = -2.0 * r * (1.0 - s * s);
+    ds_[0] = 0.25 * (2.0 * s - c10) * r * (r - 1.0);
+    ds_[1] = 0.25 * (2.0 * s - c10) * r * (r + 1.0);
+    ds_[2] = 0.25 * (2.0 * s + c10) * r * (r + 1.0);
+    ds_[3] = 0.25 * (2.0 * s + c10) * r * (r - 1.0);
+    ds_[4] = 0.5 * (1.0 - r * r) * (2.0 * s - c10);
+    ds_[5] = -r * s * (r + c10);
+    ds_[6] = 0.5 * (1.0 - r * r) * (2.0 * s + c10);
+    ds_[7] = -r * s * (r - c10);
+    ds_[8] = -2.0 * s * (1.0 - r * r);
+}
+
+__device__ __forceinline__ void jac9(const double *xp, const double *yp, double r, double s, double J[4])
+{
+    double sh[9], dr_[9], ds_[9];
+    shape9(r, s, sh, dr_, ds_);
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        dxdr = dxdr + dr_[k] * xp[k];
+        dxds = dxds + ds_[k] * xp[k];
+        dydr = dydr + dr_[k] * yp[k];
+        dyds = dyds + ds_[k] * yp[k];
+    }
+    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
+}
+
+__device__ __forceinline__ void map9(const double *xp, const double *yp, double r, double s, double *x, double *y)
+{
+    double sh[9], dr_[9], ds_[9];
+    shape9(r, s, sh, dr_, ds_);
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        a = a + sh[k] * xp[k];
+        b = b + sh[k] * yp[k];
+    }
+    *x = a;
+    *y = b;
+}
+
+__device__ __forceinline__ void basis6(double r, double s, double rc, double sc, double g0, double g1, double g2,
+                                       double *b)
+{
+    b[0] = 1.0;
+    b[1] = (r - rc) / 1.0;
+    b[2] = (s - sc) / 1.0;
+    b[3] = 0.5 * b[1] * b[1] - g0;
+    b[4] = 0.5 * b[2] * b[2] - g1;
+    b[5] = b[1] * b[2] - g2;
+}
+
+__device__ __forceinline__ void load9(const Dev &d, const double *x, const double *y, int ie, double *xp, double *yp)
+{
+    for (int k = 0; k < 9; ++k) {
+        int p = d.ipq[k * d.nc + ie];
+        xp[k] = x[p];
+        yp[k] = y[p];
+    }
+}
+
+// Initial density (getrhoig_quadcurv2 / getrhog_initial) for the
+// piecewise-constant problems: Sod splits on the initial cell centre.
+__device__ __forceinline__ double rho0(int ncase, double xc, double yc)
+{
+    if (ncase == 6) return (sqrt(xc * xc + (yc - 0.0) * (yc - 0.0)) <= 0.5) ? 1.0 : 0.125;
+    return 1.0;
+}
+
+}  // namespace lag2d
+#endif
