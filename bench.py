@@ -11,8 +11,9 @@ A step is one full SSP-RK3 step (getsolvdg_lag) with the state resident in
 HBM; throughput = ncell * ndegr * 4 * steps / time.  The "batched" entry
 measures the ensemble mode: 256 seeded runs of uneven size advanced
 together, against the same runs advanced one after another.  The "adjoint"
-entry runs the discrete adjoint's configuration (Sod-type tube, P1 on
-200x200 quads, 400 SSP-RK2 steps, checkpoints every 20): the cost of the
+entry runs the discrete adjoint's configurations (Sedov blast, P2 on the
+200x200 curved mesh, 160 SSP-RK3 steps; and Sod-type tube, P1 on 200x200
+straight quads, 400 SSP-RK2 steps; checkpoints every 20): the cost of the
 gradient against the forward run, and its agreement with central
 differences of the frozen forward.  "roofline" is the dominant kernel's;
 "step_roofline" gives both fractions of the whole step (compulsory bytes
@@ -275,10 +276,15 @@ def main():
         from lag2d_amd import adjoint
         r = adjoint.check()
         adj = {k: v for k, v in r.items() if k != "fd"}
-        adj["workload"] = ("Sod-type tube on the unit square, P1 on 200x200 jittered straight quads, 400 SSP-RK2 "
-                           "steps, checkpoints every 20; J = |x_final - x_target|^2, target from a 3% Gaussian "
-                           "density bump; dJ/dtau0 by the discrete adjoint (limiter, impedance switch and dt "
-                           "frozen), checked by central differences along 16 seeded directions")
+        adj["workload"] = ("planar Sedov blast, P2 on the 200x200 curved Q2 mesh (curved sub-cell path), 160 "
+                           "SSP-RK3 steps, checkpoints every 20; J = |x_final - x_target|^2, target from a 3% "
+                           "Gaussian density bump; dJ/dtau0 by the discrete adjoint (WENO increment, |du.n|, "
+                           "clamp branches and dt frozen), checked by central differences (eps 1e-6 and their "
+                           "Richardson extrapolation with eps/2) along 16 seeded directions")
+        r = adjoint.check(nsteps=400, problem="sod", curved=False, nstag=2)
+        adj["straight"] = {k: v for k, v in r.items() if k != "fd"}
+        adj["straight"]["workload"] = ("Sod-type tube on the unit square, P1 on 200x200 jittered straight quads, "
+                                       "400 SSP-RK2 steps, checkpoints every 20, same objective and check")
 
     line = {
         "metric": "FP64 cell-DOF updates/s, Sedov P2 curved, 1 GPU",

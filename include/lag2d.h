@@ -236,32 +236,46 @@ int lag2d_division_check_noalloc(int n, const double *a, const double *b, double
  * counterpart in the reference. */
 int lag2d_use_graphs(lag2d_ctx *ctx, int enable);
 
-/* Discrete adjoint of the step (straight P1 planar path: ncurv=0, nrz=0,
- * quads, SSP-RK2).  No counterpart in the reference.  The limiter
- * coefficients, the impedance's |du.n| (the shock switch of the corner
- * Riemann coefficients, Limiterlag.f90:987-996) and the time step are
- * frozen at their forward values, so the gradient is that of the map the
- * recorded forward computes with them held fixed.
+/* Discrete adjoint of the step.  No counterpart in the reference.  Two
+ * paths: the straight P1 planar quads with SSP-RK2 (ncurv=0, nrz=0), whose
+ * limiter coefficients are frozen, and the curved sub-cell path (ncurv=1,
+ * nfint=4, P1 or P2, Q2 quads, SSP-RK2 or RK3), whose WENO increment
+ * U_lim - U_in per cell is frozen.  Both also freeze the corner impedances'
+ * |du.n| (the shock switch of the Riemann coefficients, Limiterlag.f90:987-996,
+ * Getrhslaggauss.f90:10810-10811), the branches of the max(eps, .) pressure and
+ * sound-speed clamps and the time step, so the gradient is that of the map
+ * the recorded forward computes with them held fixed.
  *
- * lag2d_adj_record: nsteps plain steps (itime0, itime0+1, ...) from the
- * resident state, as lag2d_step, recording every stage's limiter
- * coefficients and every step's dt, and checkpointing U, x and u* every
- * `every` steps.  Device memory: 34*ncell doubles per step plus a checkpoint
- * of 12*ncell + 4*npoin doubles per `every` steps plus the stage data of
- * `every` steps (104*ncell + 10*npoin doubles each).
+ * lag2d_adj_record: nsteps steps (itime0, itime0+1, ...) from the resident
+ * state, as lag2d_step, recording the frozen switches of every stage and
+ * every step's dt, and checkpointing U, x and u* every `every` steps.  The
+ * recorded forward is lag2d_step's, bit for bit.  Device memory, straight:
+ * 34*ncell doubles per step, a checkpoint of 12*ncell + 4*npoin doubles per
+ * `every` steps, the stage data of `every` steps (104*ncell + 10*npoin
+ * doubles each); curved (nd = ndegr, ns = nstag): ns*((8 nd + 32)*ncell
+ * doubles + 44*ncell bytes) per step, a checkpoint of 4 nd*ncell + 4*npoin
+ * doubles per `every` steps, the stage data of `every` steps
+ * (ns*((4 nd + 208)*ncell + 4*npoin) doubles each).
  *
  * lag2d_adj_replay: nsteps steps from the resident state with the recorded
- * limiter coefficients and dt (upload a perturbed initial state first): the
- * frozen forward map whose derivative the adjoint computes.
+ * switches and dt (upload a perturbed initial state first): the frozen
+ * forward map whose derivative the adjoint computes.  On the curved path a
+ * limited cell's DOFs are U_lim,rec + (U_in - U_in,rec), the recording's
+ * limited DOFs at the recorded input, so a replay from the recorded initial
+ * state reproduces the recording bit for bit.
  *
  * lag2d_adj_gradient: given lam_coord(2,npoin) = dJ/dx at the end of the
  * recorded steps, returns grad(ndegr,4,ncell) = dJ/dunkno at their start.
  * Each checkpointed segment is recomputed forward (stage data kept), then
  * reversed step by step.  Leaves the resident state unspecified: upload
- * before stepping again. */
+ * before stepping again.
+ *
+ * lag2d_adj_gradient_coord: after lag2d_adj_gradient, grad_coord(2,npoin) =
+ * dJ/dx at the start of the recorded steps (the initial node positions). */
 int lag2d_adj_record(lag2d_ctx *ctx, int itime0, int nsteps, int every);
 int lag2d_adj_replay(lag2d_ctx *ctx, int nsteps);
 int lag2d_adj_gradient(lag2d_ctx *ctx, const double *lam_coord, double *grad);
+int lag2d_adj_gradient_coord(lag2d_ctx *ctx, double *grad_coord);
 
 void lag2d_destroy(lag2d_ctx *ctx);
 const char *lag2d_last_error(void);

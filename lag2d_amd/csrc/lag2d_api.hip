@@ -557,6 +557,9 @@ static const char *kernel_name(const lag2d_ctx *x, int k)
     if (k == KID_Q2_CORNER && x->corner_variant == 2) return "k_q2_corner_sc";
     if (k == KID_CELLRHS_Q2 && x->rhs_variant == 2) return "k_cellrhs_q2_tpc";
     if (k == KID_Q2_WENO && x->nd == 3) return "k_c1_weno";
+    if (x->adj.curved && k == KID_ADJ_CELLRHS) return "k_adjc_cellrhs";
+    if (x->adj.curved && k == KID_ADJ_CORNER) return "k_adjc_corner";
+    if (x->adj.curved && k == KID_ADJ_NODES) return "k_adjc_move+node+gather";
     return lag2d_kernel_names[k];
 }
 
@@ -766,12 +769,223 @@ static bool adj_supported(const lag2d_ctx *x)
     return c.ncurv == 0 && c.npoly == 1 && c.nrz == 0 && c.nstag == 2 && x->ntria == 0 && x->nrun == 1;
 }
 
+// ---- discrete adjoint of the curved sub-cell path (P1 and P2 on Q2 quads,
+// SSP-RK2 and RK3).  The tapes of step n, stage s: WENO's limited and input
+// DOFs and the corner impedances' |du.n| [(8 nd + 32) nc doubles], the
+// clamp branches and the limited flags [44 nc bytes].  A segment slot keeps
+// per stage the limited DOFs, the stage coordinates, u*, the corner records
+// and the face slots.
+
+static bool adjc_supported(const lag2d_ctx *x)
+{
+    const lag2d_config &c = x->cfg;
+    return c.ncurv == 1 && c.nfint == 4 && (c.npoly == 1 || c.npoly == 2) && c.nrz == 0 &&
+           (c.nstag == 2 || c.nstag == 3) && x->ntria == 0 && x->nrun == 1;
+}
+
+static size_t adjc_tape_doubles(const lag2d_ctx *x) { return (size_t)(8 * x->nd + 32) * x->nc; }
+static size_t adjc_tape_bytes(const lag2d_ctx *x) { return (size_t)44 * x->nc; }
+static size_t adjc_stage_stride(const lag2d_ctx *x)
+{
+    return (size_t)(4 * x->nd + 160 + 48) * x->nc + (size_t)4 * x->np;
+}
+
+// One step of the curved path's recorded (RECORD), replayed (REPLAY) or
+// recomputed (SEGMENT, keeping the stage data in segment slot k) forward.
+static int adjc_fwd_step(lag2d_ctx *x, int itime, int n, int mode, int k)
+{
+    Dev &d = x->d;
+    lag2d_adj &A = x->adj;
+    const int nc = x->nc, np = x->np, ns = x->cfg.nstag;
+    const size_t nU = (size_t)4 * x->nd * nc, sst = adjc_stage_stride(x);
+    auto cp = [&](double *dst, const double *src, size_t m) {
+        return hipMemcpyAsync(dst, src, m * sizeof(double), hipMemcpyDeviceToDevice, x->stream);
+    };
+    for (int s = 0; s < ns; ++s) {
+        const double alfa = RKLAG[ns - 2][s];
+        Dev ds = d;
+        if (s == 0) {
+            ds.xs = d.xn;
+            ds.ys = d.yn;
+        }
+        const double *Uin = (s == 0) ? d.Un : d.Us;
+        const size_t ts = (size_t)n * ns + s;
+        ds.wdu = A.lim_tape + ts * adjc_tape_doubles(x);
+        ds.dlt = ds.wdu + 2 * nU;
+        ds.ccl = A.clamp_tape + ts * adjc_tape_bytes(x);
+        ds.dcl = ds.ccl + (size_t)18 * nc;
+        double *slot = A.seg + (size_t)k * A.seg_stride + (size_t)s * sst;
+        if (mode == ADJ_SEGMENT) {
+            HIPCHK(cp(slot + nU, ds.xs, np));
+            HIPCHK(cp(slot + nU + np, ds.ys, np));
+        }
+        int rc = launch_rhs_q2_adj(x, ds, Uin, alfa, s > 0, mode == ADJ_RECORD);
+        if (rc) return fail(rc, "curved adjoint forward");
+        if (mode == ADJ_SEGMENT) {
+            HIPCHK(cp(slot, d.Ul, nU));
+            HIPCHK(cp(slot + nU + 2 * (size_t)np, d.usx, np));
+            HIPCHK(cp(slot + nU + 3 * (size_t)np, d.usy, np));
+            HIPCHK(cp(slot + nU + 4 * (size_t)np, d.rec, (size_t)160 * nc));
+            HIPCHK(cp(slot + nU + 4 * (size_t)np + (size_t)160 * nc, d.gq, (size_t)48 * nc));
+        }
+        if (s == 0) {
+            if (mode == ADJ_RECORD) {
+                LAUNCH(KID_DT_PARTIAL, k_dt_partial, x->nblk_dt, ds, Uin, x->nblk_dt);
+                LAUNCH(KID_DT_FINAL, k_dt_final, 1, d, itime, x->nblk_dt);
+                HIPCHK(cp(A.dt_tape + n, d.scal, 1));
+            } else {
+                LAUNCH(KID_DT_FINAL, k_replay_dt, 1, d, A.dt_tape + n);
+            }
+            if (x->update_cells)
+                LAUNCH(KID_UPDATE, k_update_cells, nblocks(4 * nc), d, alfa);
+            else
+                LAUNCH(KID_UPDATE, k_update, nblocks(4 * x->nd * nc), d, alfa);
+        }
+        LAUNCH(KID_NODEMOVE, k_nodemove, nblocks(np), d, alfa, s == 0 ? d.xn : d.xs, s == 0 ? d.yn : d.ys);
+        launch_cellvol(x, d.xs, d.ys, 1, true);
+    }
+    HIPCHK(hipGetLastError());
+    std::swap(d.Un, d.Us);
+    std::swap(d.xn, d.xs);
+    std::swap(d.yn, d.ys);
+    x->parity ^= 1;
+    return 0;
+}
+
+static int adjc_record(lag2d_ctx *x, int itime0, int nsteps, int every)
+{
+    const int nc = x->nc, np = x->np, ns = x->cfg.nstag;
+    lag2d_adj &A = x->adj;
+    const int nckpt = (nsteps + every - 1) / every;
+    const size_t nU = (size_t)4 * x->nd * nc, ckw = nU + 4 * (size_t)np;
+    int rc;
+    if (!A.curved || A.cap_steps < (size_t)nsteps) {
+        if ((rc = x->alloc(&A.lim_tape, (size_t)nsteps * ns * adjc_tape_doubles(x))) ||
+            (rc = x->alloc(&A.clamp_tape, (size_t)nsteps * ns * adjc_tape_bytes(x))) ||
+            (rc = x->alloc(&A.dt_tape, (size_t)nsteps)))
+            return rc;
+        A.cap_steps = nsteps;
+    }
+    if (!A.curved || A.cap_ckpt < (size_t)nckpt) {
+        if ((rc = x->alloc(&A.ck, (size_t)nckpt * ckw))) return rc;
+        A.cap_ckpt = nckpt;
+    }
+    if (!A.curved || A.cap_seg < (size_t)every) {
+        A.seg_stride = (size_t)ns * adjc_stage_stride(x);
+        if ((rc = x->alloc(&A.seg, (size_t)every * A.seg_stride)) ||
+            (rc = x->alloc(&A.buf, 3 * nU + (size_t)11 * np + (size_t)(32 + 18 + 160 + 96) * nc)))
+            return rc;
+        A.cap_seg = every;
+    }
+    A.curved = true;
+    A.nsteps = nsteps;
+    A.every = every;
+    A.nckpt = nckpt;
+    A.itime0 = itime0;
+    Dev &d = x->d;
+    for (int n = 0; n < nsteps; ++n) {
+        if (n % every == 0) {
+            double *c = A.ck + (size_t)(n / every) * ckw;
+            HIPCHK(hipMemcpyAsync(c, d.Un, nU * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+            HIPCHK(hipMemcpyAsync(c + nU, d.xn, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+            HIPCHK(hipMemcpyAsync(c + nU + np, d.yn, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+            HIPCHK(hipMemcpyAsync(c + nU + 2 * (size_t)np, d.usx, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                  x->stream));
+            HIPCHK(hipMemcpyAsync(c + nU + 3 * (size_t)np, d.usy, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                  x->stream));
+        }
+        rc = adjc_fwd_step(x, itime0 + n, n, ADJ_RECORD, 0);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return 0;
+}
+
+static int adjc_gradient(lag2d_ctx *x, const double *lam_coord, double *grad)
+{
+    lag2d_adj &A = x->adj;
+    Dev &d = x->d;
+    const int nc = x->nc, np = x->np, ns = x->cfg.nstag, nd = x->nd;
+    const size_t nU = (size_t)4 * nd * nc, ckw = nU + 4 * (size_t)np, sst = adjc_stage_stride(x);
+    AdjStageC a{};
+    double *b = A.buf;
+    a.LU = b; b += nU;
+    a.LUN = b; b += nU;
+    a.LUI = b; b += nU;
+    a.LX = b; b += 2 * (size_t)np;
+    a.LXN = b; b += 2 * (size_t)np;
+    a.LXI = b; b += 2 * (size_t)np;
+    a.LUS = b; b += 2 * (size_t)np;
+    a.NG = b; b += 3 * (size_t)np;
+    a.CU = b; b += 32 * (size_t)nc;
+    a.CX = b; b += 18 * (size_t)nc;
+    a.LREC = b; b += 160 * (size_t)nc;
+    a.LUSE = b; b += 96 * (size_t)nc;
+    HIPCHK(hipMemsetAsync(A.buf, 0, (size_t)(b - A.buf) * sizeof(double), x->stream));
+    std::vector<double> h(2 * (size_t)np);
+    for (int p = 0; p < np; ++p) {
+        h[p] = lam_coord[2 * (size_t)p];
+        h[np + p] = lam_coord[2 * (size_t)p + 1];
+    }
+    HIPCHK(hipMemcpyAsync(a.LX, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    for (int sg = A.nckpt - 1; sg >= 0; --sg) {
+        const int n0 = sg * A.every, n1 = std::min(n0 + A.every, A.nsteps);
+        const double *c = A.ck + (size_t)sg * ckw;
+        HIPCHK(hipMemcpyAsync(d.Un, c, nU * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(d.xn, c + nU, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(d.yn, c + nU + np, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(d.usx, c + nU + 2 * (size_t)np, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        HIPCHK(hipMemcpyAsync(d.usy, c + nU + 3 * (size_t)np, np * sizeof(double), hipMemcpyDeviceToDevice, x->stream));
+        for (int n = n0; n < n1; ++n) {
+            int rc = adjc_fwd_step(x, A.itime0 + n, n, ADJ_SEGMENT, n - n0);
+            if (rc) return rc;
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            const double *slot0 = A.seg + (size_t)(n - n0) * A.seg_stride;
+            a.dt = A.dt_tape + n;
+            for (int s = ns - 1; s >= 0; --s) {
+                const double *slot = slot0 + (size_t)s * sst;
+                const size_t ts = (size_t)n * ns + s;
+                a.Ul = slot;
+                a.xin = slot + nU;
+                a.yin = a.xin + np;
+                a.usx = a.yin + np;
+                a.usy = a.usx + np;
+                a.rec = a.usy + np;
+                a.gq = a.rec + (size_t)160 * nc;
+                a.dlt = A.lim_tape + ts * adjc_tape_doubles(x) + 2 * nU;
+                a.ccl = A.clamp_tape + ts * adjc_tape_bytes(x);
+                a.dcl = a.ccl + (size_t)18 * nc;
+                int rc = launch_adjq2_stage(x, a, RKLAG[ns - 2][s]);
+                if (rc) return fail(rc, "curved adjoint stage");
+                if (s > 0) {
+                    std::swap(a.LU, a.LUI);
+                    std::swap(a.LX, a.LXI);
+                }
+            }
+            int rc = launch_adjq2_step_end(x, a);
+            if (rc) return fail(rc, "curved adjoint step end");
+        }
+    }
+    A.lam_x0 = a.LX;
+    std::vector<double> g(nU);
+    HIPCHK(hipMemcpyAsync(g.data(), a.LU, nU * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int ie = 0; ie < nc; ++ie)
+        for (int q = 0; q < 4; ++q)
+            for (int id = 0; id < nd; ++id)
+                grad[id + nd * (q + 4 * (size_t)ie)] = g[((size_t)q * nd + id) * nc + ie];
+    return 0;
+}
+
 extern "C" int lag2d_adj_record(lag2d_ctx *x, int itime0, int nsteps, int every)
 {
     if (!x) return fail(LAG2D_EINVAL, "null context");
-    if (!adj_supported(x))
-        return fail(LAG2D_EUNSUPPORTED, "the adjoint covers the straight P1 planar path on quads with SSP-RK2");
     if (nsteps < 1 || every < 1 || itime0 < 1) return fail(LAG2D_EINVAL, "nsteps, every and itime0 must be >= 1");
+    if (adjc_supported(x)) return adjc_record(x, itime0, nsteps, every);
+    if (!adj_supported(x))
+        return fail(LAG2D_EUNSUPPORTED, "the adjoint covers the straight P1 planar path on quads with SSP-RK2 "
+                                        "and the curved sub-cell path (P1, P2) on Q2 quads with SSP-RK2/3");
     const int nc = x->nc, np = x->np;
     lag2d_adj &A = x->adj;
     const int nckpt = (nsteps + every - 1) / every;
@@ -815,7 +1029,8 @@ extern "C" int lag2d_adj_replay(lag2d_ctx *x, int nsteps)
     if (x->adj.nsteps < 1) return fail(LAG2D_EINVAL, "nothing recorded (lag2d_adj_record)");
     if (nsteps < 0 || nsteps > x->adj.nsteps) return fail(LAG2D_EINVAL, "more steps than recorded");
     for (int n = 0; n < nsteps; ++n) {
-        int rc = adj_fwd_step(x, x->adj.itime0 + n, n, ADJ_REPLAY, 0);
+        int rc = x->adj.curved ? adjc_fwd_step(x, x->adj.itime0 + n, n, ADJ_REPLAY, 0)
+                               : adj_fwd_step(x, x->adj.itime0 + n, n, ADJ_REPLAY, 0);
         if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(x->stream));
@@ -861,6 +1076,7 @@ extern "C" int lag2d_adj_gradient(lag2d_ctx *x, const double *lam_coord, double 
     if (!x || !lam_coord || !grad) return fail(LAG2D_EINVAL, "null argument");
     lag2d_adj &A = x->adj;
     if (A.nsteps < 1) return fail(LAG2D_EINVAL, "nothing recorded (lag2d_adj_record)");
+    if (A.curved) return adjc_gradient(x, lam_coord, grad);
     Dev &d = x->d;
     const int nc = x->nc, np = x->np;
     const size_t nU = (size_t)12 * nc;
@@ -903,12 +1119,28 @@ extern "C" int lag2d_adj_gradient(lag2d_ctx *x, const double *lam_coord, double 
             if (rc) return rc;
         }
     }
+    A.lam_x0 = a.LX;
     std::vector<double> g(nU);
     HIPCHK(hipMemcpyAsync(g.data(), a.LU, nU * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     HIPCHK(hipStreamSynchronize(x->stream));
     for (int ie = 0; ie < nc; ++ie)
         for (int q = 0; q < 4; ++q)
             for (int id = 0; id < 3; ++id) grad[id + 3 * (q + 4 * (size_t)ie)] = g[((size_t)q * 3 + id) * nc + ie];
+    return 0;
+}
+
+extern "C" int lag2d_adj_gradient_coord(lag2d_ctx *x, double *grad_coord)
+{
+    if (!x || !grad_coord) return fail(LAG2D_EINVAL, "null argument");
+    if (!x->adj.lam_x0) return fail(LAG2D_EINVAL, "no gradient computed (lag2d_adj_gradient)");
+    const int np = x->np;
+    std::vector<double> h(2 * (size_t)np);
+    HIPCHK(hipMemcpyAsync(h.data(), x->adj.lam_x0, h.size() * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    HIPCHK(hipStreamSynchronize(x->stream));
+    for (int p = 0; p < np; ++p) {
+        grad_coord[2 * (size_t)p] = h[p];
+        grad_coord[2 * (size_t)p + 1] = h[np + p];
+    }
     return 0;
 }
 

@@ -107,11 +107,12 @@ struct Dev {
     const int *wtop;
     const double *wrd;
     // the curved path's frozen switches of the adjoint's recorded forward
-    // (null otherwise): the stage's WENO increment U_lim - U_in [4 nd][nc],
-    // and which pressure and sound-speed clamps max(eps, .) chose eps, per
-    // cell: corner kernel [18][nc] (centre pressure, sound speed, the 16
-    // sub-cell vertex pressures), domain integral [25][nc] (its points);
-    // the impedances' |du.n| live in dlt [32][nc]
+    // (null otherwise): WENO's limited and input DOFs of the cells it limited
+    // [2][4 nd][nc], and which pressure and sound-speed clamps max(eps, .)
+    // chose eps, per cell: corner kernel [18][nc] (centre pressure, sound
+    // speed, the 16 sub-cell vertex pressures), domain integral [25][nc]
+    // (its points), followed by the limited flags [nc]; the impedances'
+    // |du.n| live in dlt [32][nc]
     double *wdu;
     uint8_t *ccl, *dcl;
 };
@@ -182,6 +183,28 @@ struct AdjStage {
     double *CX, *CU;             // per-corner lam of x_in, u* [2][4][nc]
     double *LREC;                // lam of the corner records [40][nc]
     double *NG;                  // node coefficients of the reversed solve [3][np]
+};
+
+// One reversed stage of the curved (Q2 sub-cell) step: the stage's data
+// kept by the recomputed segment, its frozen switches, and the adjoint
+// buffers (lam = derivative of the objective with respect to that array).
+struct AdjStageC {
+    const double *Ul;            // limited stage DOFs [4 nd][nc]
+    const double *xin, *yin;     // stage input coordinates
+    const double *usx, *usy;     // this stage's nodal velocity
+    const double *rec;           // this stage's corner-face records (RQ2 layout)
+    const double *gq;            // this stage's face slots [48][nc]
+    const double *dlt;           // frozen |du.n| [32][nc]
+    const uint8_t *ccl, *dcl;    // frozen clamp branches [18][nc], [25][nc]
+    const double *dt;            // the step's dt (device scalar)
+    double *LU, *LUN, *LUI;      // lam of U_out, of U^n (accumulated), of U_in  [4 nd][nc]
+    double *LX, *LXN, *LXI;      // lam of x_out, of x^n (accumulated), of x_in  [2][np]
+    double *LUS;                 // lam of this stage's u* [2][np]
+    double *NG;                  // node coefficients of the reversed solve [3][np]
+    double *CU;                  // lam of u* per sub-cell corner [16][2][nc]
+    double *CX;                  // lam of the cell's 9 nodes [9][2][nc]
+    double *LREC;                // lam of the corner-face records [32][5][nc]
+    double *LUSE;                // lam of the face slots per corner face [32][3][nc]
 };
 
 __device__ __forceinline__ int cell_run(const Dev &d, int ie) { return d.nrun > 1 ? d.runc[ie] : 0; }

@@ -21,15 +21,22 @@ struct lag2d_timer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
-// Recording and buffers of the discrete adjoint (straight P1 planar quads).
+// Recording and buffers of the discrete adjoint (straight P1 planar quads,
+// and the curved sub-cell path).
 struct lag2d_adj {
     int nsteps = 0, every = 0, itime0 = 0, nckpt = 0;
-    double *lim_tape = nullptr;  // [nsteps][2][9*nc] limiter coefficients of every stage
+    bool curved = false;
+    size_t cap_steps = 0, cap_ckpt = 0, cap_seg = 0;  // allocated sizes
+    double *lim_tape = nullptr;  // straight: [nsteps][2][17*nc] limiter coefficients and |du.n| of every
+                                 // stage; curved: [nsteps][nstag][(8 nd + 32) nc] WENO's limited and
+                                 // input DOFs and |du.n|
+    uint8_t *clamp_tape = nullptr;  // curved: [nsteps][nstag][44 nc] clamp branches, limited flags
     double *dt_tape = nullptr;   // [nsteps]
     double *ck = nullptr;        // [nckpt][12*nc + 4*np] checkpoints: U, x, y, u*
     double *seg = nullptr;       // [every][seg_stride] stage data of a recomputed segment
     size_t seg_stride = 0;
     double *buf = nullptr;       // adjoint buffers
+    const double *lam_x0 = nullptr;  // dJ/dx at the recording's start, after lag2d_adj_gradient [2][np]
 };
 
 // A captured step: its launches depend on the parity of the state-buffer
@@ -84,3 +91,12 @@ int build_node_corners(lag2d_ctx &x, const std::vector<int> &ipq, const std::vec
 // time-invariant initial sub-cell masses of the curved path).
 int upload_extra(lag2d_ctx &x);
 int launch_rhs_q2(lag2d_ctx *x, lag2d::Dev &ds, const double *Uin, double alfa, int apply_update);
+// The curved path's stage RHS in the adjoint's recorded (record = 1) or
+// replayed (record = 0) forward: WENO increment, |du.n| and clamp branches
+// written to or read from the tapes ds.wdu, ds.dlt, ds.ccl, ds.dcl.
+int launch_rhs_q2_adj(lag2d_ctx *x, lag2d::Dev &ds, const double *Uin, double alfa, int apply_update, int record);
+// The reverse of one curved-path stage (lag2d_adjq2.hip): a.LU, a.LX hold
+// the adjoint of the stage's output; on return a.LUI, a.LXI hold that of its
+// input and a.LUN, a.LXN have gained the step input's direct part.
+int launch_adjq2_stage(lag2d_ctx *x, lag2d::AdjStageC &a, double alfa);
+int launch_adjq2_step_end(lag2d_ctx *x, lag2d::AdjStageC &a);

@@ -906,12 +906,6 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
 }
 
 // ------------------------------------------------------------- corners --
-// Corner-face records, cell-major: the 5 records of corner face cf of cell
-// ie at (32 ie + cf) 5 + r, so the two faces of a sub-cell corner are 10
-// consecutive doubles for the corner kernel that writes them, the node
-// gather and the cell RHS that read them.
-#define RQ2(ie, cf, r) (((size_t)(ie) * 32 + (cf)) * 5 + (r))
-
 // Face geometry (stored for the face integral), sub-cell density and mean
 // specific volume, vertex states and the 32 corner-face Riemann records.
 // Records: rec[RQ2(ie, cf, r)], cf = f + 2*(ivsg + 4*isg),
@@ -921,20 +915,6 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
 // (isg, ivsg) sums its sub-cell's terms in point order and writes the two
 // records of its sub-cell corner.
 constexpr int CORNER_CPB = 16;
-
-// Jacobian of a Q2 map from tabulated shape-function derivatives (the table
-// is built by k_q2_shape_tables with shape9, so the sums equal jac9's).
-__device__ __forceinline__ void jac9_tab(const double *xp, const double *yp, const double *t, double J[4])
-{
-    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
-    for (int k = 0; k < 9; ++k) {
-        dxdr = dxdr + t[k] * xp[k];
-        dxds = dxds + t[9 + k] * xp[k];
-        dydr = dydr + t[k] * yp[k];
-        dyds = dyds + t[9 + k] * yp[k];
-    }
-    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
-}
 
 __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
 {
@@ -2126,6 +2106,43 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
 #undef UD
 #undef UDN
 
+// The WENO reconstruction of the adjoint's recorded and replayed forward:
+// its increment U_lim - U_in per cell is a constant of the frozen map.
+// RECORD (after the plain WENO, which it leaves as it is, so the recorded
+// forward is the plain step): per cell whether WENO limited it and, where
+// it did, the limited and the input DOFs go to the tape.  REPLAY: U_lim =
+// U_lim,rec + (U_in - U_in,rec) on the limited cells (U_lim,rec itself at
+// the recorded input), U_in elsewhere, and every cell flagged so that the
+// sub-cell kernels read U_lim.  Tape: d.wdu = [U_lim,rec | U_in,rec]
+// [2][4 nd][nc], the limited flags after the clamp branches at
+// d.dcl + 25 nc.  Thread per cell.
+template <int ND, int RECORD>
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_weno_frozen(Dev d, const double *__restrict__ Uin)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc;
+    if (ie >= nc) return;
+    uint8_t *limited = d.dcl + (size_t)25 * nc;
+    const size_t nu = (size_t)4 * ND * nc;
+    if constexpr (RECORD) {
+        const bool lim = d.flag[ie] != 0;
+        limited[ie] = lim ? 1 : 0;
+        if (lim)
+            for (int k = 0; k < 4 * ND; ++k) {
+                const size_t o = (size_t)k * nc + ie;
+                d.wdu[o] = d.Ul[o];
+                d.wdu[nu + o] = Uin[o];
+            }
+    } else {
+        const bool lim = limited[ie] != 0;
+        for (int k = 0; k < 4 * ND; ++k) {
+            const size_t o = (size_t)k * nc + ie;
+            d.Ul[o] = lim ? d.wdu[o] + (Uin[o] - d.wdu[nu + o]) : Uin[o];
+        }
+        d.flag[ie] = 1;
+    }
+}
+
 }  // namespace lag2d
 
 using namespace lag2d;
@@ -2367,4 +2384,41 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
     launch_cellrhs(x, ds, alfa, apply_update, x->rhs_variant ? x->rhs_variant : 1);
     x->toc(KID_CELLRHS_Q2);
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
+}
+
+// One stage right-hand side of the adjoint's recorded (record = 1) or
+// replayed (record = 0) forward on the curved path: the plain WENO (record
+// only) and its frozen increment, then the corner (recording or reading the
+// |du.n| and clamp tapes in ds.dlt, ds.ccl, ds.dcl), the nodal solve and
+// the cell RHS with the fused update.  The one-lane-per-sub-cell corner and
+// thread-per-cell RHS variants run (their results equal the others').
+template <int ND>
+static int launch_rhs_q2_adj_nd(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int apply_update, int record)
+{
+    const int nc = x->nc, np = x->np;
+    ds.Ui = Uin;
+    if (record) {
+        if constexpr (ND == 3) {
+            hipLaunchKernelGGL(k_c1_weno, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
+        } else {
+            hipLaunchKernelGGL(k_q2_prep, dim3((nc + PREP_CELLS - 1) / PREP_CELLS), dim3(256), 0, x->stream, ds, Uin);
+            hipLaunchKernelGGL(k_q2_weno, dim3(nc < WENO_GRID ? nc : WENO_GRID), dim3(WENO_WG), 0, x->stream, ds, Uin);
+        }
+        hipLaunchKernelGGL((k_q2_weno_frozen<ND, 1>), dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
+        hipLaunchKernelGGL((k_q2_corner_sc<ND, 1>), dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
+        hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+        hipLaunchKernelGGL((k_cellrhs_q2_tpc<ND, 1>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
+    } else {
+        hipLaunchKernelGGL((k_q2_weno_frozen<ND, 0>), dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
+        hipLaunchKernelGGL((k_q2_corner_sc<ND, 2>), dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
+        hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+        hipLaunchKernelGGL((k_cellrhs_q2_tpc<ND, 2>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
+}
+
+int launch_rhs_q2_adj(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int apply_update, int record)
+{
+    return x->nd == 3 ? launch_rhs_q2_adj_nd<3>(x, ds, Uin, alfa, apply_update, record)
+                      : launch_rhs_q2_adj_nd<6>(x, ds, Uin, alfa, apply_update, record);
 }

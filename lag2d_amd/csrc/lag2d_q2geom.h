@@ -7,6 +7,12 @@
 
 namespace lag2d {
 
+// Corner-face records, cell-major: the 5 records of corner face cf of cell
+// ie at (32 ie + cf) 5 + r, so the two faces of a sub-cell corner are 10
+// consecutive doubles for the corner kernel that writes them, the node
+// gather and the cell RHS that read them.
+#define RQ2(ie, cf, r) (((size_t)(ie) * 32 + (cf)) * 5 + (r))
+
 // sub-cell vertex -> local Q2 node (ipqsg)
 __constant__ static const int IPQSG[4][4] = {{0, 4, 8, 7}, {4, 1, 5, 8}, {8, 5, 2, 6}, {7, 8, 6, 3}};
 // signed face-normal slots (fnqsg): sign = orientation, |v|-1 = gesgq slot
@@ -95,6 +101,20 @@ __device__ __forceinline__ void jac9(const double *xp, const double *yp, double 
         dxds = dxds + ds_[k] * xp[k];
         dydr = dydr + dr_[k] * yp[k];
         dyds = dyds + ds_[k] * yp[k];
+    }
+    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
+}
+
+// Jacobian of a Q2 map from tabulated shape-function derivatives (the table
+// is built by k_q2_shape_tables with shape9, so the sums equal jac9's).
+__device__ __forceinline__ void jac9_tab(const double *xp, const double *yp, const double *t, double J[4])
+{
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        dxdr = dxdr + t[k] * xp[k];
+        dxds = dxds + t[9 + k] * xp[k];
+        dydr = dydr + t[k] * yp[k];
+        dyds = dyds + t[9 + k] * yp[k];
     }
     J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
 }

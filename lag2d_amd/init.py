@@ -355,18 +355,49 @@ def sod_square_case(n=200, seed=2024, bump=None) -> LagCase:
     return initialise_p1(mesh, ctl, rho, st)
 
 
-def sedov_case(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None) -> LagCase:
+def sod_square_curved_case(n=200, seed=2024, bump=None, npoly=1, nstag=3, cfl=0.3) -> LagCase:
+    """The same Sod-type tube on an n x n curved Q2 mesh of the unit square
+    (vertices jittered by up to 0.1h, edge midpoints displaced by up to
+    0.05h), on the curved sub-cell path (ncurv=1, nfint=4) at P1 (npoly=1)
+    or P2, slip walls, cdrho 1; ``rho0`` holds the per-cell initial density
+    (with the bump, if any)."""
+    mesh = mg.quad_q2(n, n, 0.0, 1.0, 0.0, 1.0, seed=seed)
+    ctl = mg.Control(cfl=cfl, tend=10.0, ndegr=3 if npoly == 1 else 6, npoly=npoly, ncurv=1, nstag=nstag,
+                     ncase=6, nfint=4, cdrho=1.0)
+
+    def rho(x, y):
+        r = np.where(x < 0.5, 1.0, 0.125)
+        if bump is not None:
+            a, x0, y0, w = bump
+            r = r * (1.0 + a * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (w * w)))
+        return r
+
+    def st(x, y):
+        return 0.0, 0.0, np.where(x < 0.5, 1.0, 0.1)
+
+    return initialise(mesh, ctl, rho, st)
+
+
+def sedov_case(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None, bump=None) -> LagCase:
     """Planar Sedov blast on a jittered curved Q2 mesh, P2: rho = 1, u = 0,
     p = 1e-6, blast energy 0.244816 in the corner cell (or in each of the
-    four centre cells with ``centre``)."""
+    four centre cells with ``centre``).  ``bump`` = (amplitude, x0, y0,
+    width) multiplies the density by 1 + amplitude exp(-|x - (x0, y0)|^2 /
+    width^2)."""
     mesh, ctl = mg.sedov_problem(nx, seed=seed)
     if cdrho is not None:
         ctl.cdrho = cdrho
     if cimpd is not None:
         ctl.cimpd = cimpd
     cells = mg.centre_cells(nx, nx) if centre else (0,)
-    return initialise(mesh, ctl, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6),
-                      blast=(cells, mg.SEDOV_ENERGY))
+
+    def rho(x, y):
+        if bump is None:
+            return 1.0
+        a, x0, y0, w = bump
+        return 1.0 + a * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / (w * w))
+
+    return initialise(mesh, ctl, rho, lambda x, y: (0.0, 0.0, 1e-6), blast=(cells, mg.SEDOV_ENERGY))
 
 
 def sedov_state(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None):

@@ -20,7 +20,7 @@ EXPORTS = ("lag2d_create", "lag2d_create_hybrid", "lag2d_create_ensemble", "lag2
            "lag2d_get_time", "lag2d_check", "lag2d_sync", "lag2d_kernel_time", "lag2d_kernel_stats",
            "lag2d_division_check", "lag2d_get_mass_matrix", "lag2d_adj_record", "lag2d_adj_replay",
            "lag2d_adj_gradient", "lag2d_use_graphs", "lag2d_destroy", "lag2d_last_error",
-           "lag2d_division_check_noalloc")
+           "lag2d_division_check_noalloc", "lag2d_adj_gradient_coord")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -72,6 +72,7 @@ def load():
     lib.lag2d_adj_record.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     lib.lag2d_adj_replay.argtypes = [C.c_void_p, C.c_int]
     lib.lag2d_adj_gradient.argtypes = [C.c_void_p, _dp, _dp]
+    lib.lag2d_adj_gradient_coord.argtypes = [C.c_void_p, _dp]
     lib.lag2d_use_graphs.argtypes = [C.c_void_p, C.c_int]
     lib.lag2d_check.argtypes = [C.c_void_p]
     lib.lag2d_sync.argtypes = [C.c_void_p]
@@ -188,14 +189,15 @@ class Lag2D:
         _check(load().lag2d_use_graphs(self.h, 1 if enable else 0))
 
     def adj_record(self, nsteps, every):
-        """nsteps plain steps recording the limiter and dt, checkpoints every
+        """nsteps steps recording the frozen switches (limiter coefficients
+        or WENO increments, |du.n|, clamp branches) and dt, checkpoints every
         ``every`` steps (the discrete adjoint's forward pass)."""
         _check(load().lag2d_adj_record(self.h, self.itime + 1, nsteps, every))
         self.itime += nsteps
 
     def adj_replay(self, nsteps):
-        """The recorded steps again from the resident state with the limiter
-        and dt frozen at their recorded values."""
+        """The recorded steps again from the resident state with the frozen
+        switches and dt at their recorded values."""
         _check(load().lag2d_adj_replay(self.h, nsteps))
 
     def adj_gradient(self, lam_coord):
@@ -204,6 +206,13 @@ class Lag2D:
         lam = np.ascontiguousarray(lam_coord, dtype=np.float64)
         g = np.zeros((self.case.ncell, 4, self.case.ctl.ndegr))
         _check(load().lag2d_adj_gradient(self.h, _p(lam), _p(g)))
+        return g
+
+    def adj_gradient_coord(self):
+        """dJ/dcoord (npoin, 2) at the start of the recording, after
+        adj_gradient."""
+        g = np.zeros((self.case.coord.shape[0], 2))
+        _check(load().lag2d_adj_gradient_coord(self.h, _p(g)))
         return g
 
     def mass_matrix(self):

@@ -97,6 +97,12 @@ enum orc_phase {
 #ifndef ORC_FROZEN
 #define ORC_FROZEN(x) (x)
 #endif
+/* Tangent build: the curved path's WENO output takes the derivative of its
+ * input (the reconstruction's increment is held at its forward value).
+ * Nothing in the ordinary build. */
+#ifndef ORC_FROZEN_INCREMENT
+#define ORC_FROZEN_INCREMENT(out, in) ((void)0)
+#endif
 
 #ifdef __cplusplus
 }

@@ -1098,6 +1098,7 @@ static int weno(orc_case *c, const double *flag, const int *esqua)
             c->nweno_skipped++;
         }
     }
+    for (size_t i = 0; i < (size_t)nd * 4 * nc; ++i) ORC_FROZEN_INCREMENT(c->unold[i], unknr[i]);
     free(geoph);
     free(mapmt);
     free(unknr);
@@ -1252,7 +1253,7 @@ static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
                 double dux = X(c->ustar, 0, p) - uv, duy = X(c->ustar, 1, p) - vv;
                 double a3 = sqrt(dux * dux + duy * duy) / sdctr;
                 for (int f = 0; f < 2; ++f) {
-                    double deltu = fabs(dux * vn[iv][f][0] + duy * vn[iv][f][1]);
+                    double deltu = ORC_FROZEN(fabs(dux * vn[iv][f][0] + duy * vn[iv][f][1]));
                     double mur = rhoct * sdctr + c->cimpd * rhoct * c->slpdu * deltu;
                     double area = vn[iv][f][2];
                     double m11;

@@ -7,7 +7,8 @@
 // derivative of its step along the seeded direction.  Values are computed
 // with the plain double arithmetic, so they equal the ordinary build's.
 // ORC_FROZEN drops the derivative of the values the discrete adjoint holds
-// at their forward values.
+// at their forward values; ORC_FROZEN_INCREMENT gives the curved path's
+// WENO output the derivative of its input (its increment held fixed).
 #pragma once
 #include <math.h>
 #include <stdlib.h>
@@ -50,3 +51,4 @@ inline Dual fabs(Dual a) { return a.v < 0.0 ? Dual(-a.v, -a.d) : (a.v > 0.0 ? a 
 inline Dual log10(Dual a) { return Dual(::log10(a.v), a.d / (a.v * ::log(10.0))); }
 
 #define ORC_FROZEN(x) (Dual((x).v, 0.0))
+#define ORC_FROZEN_INCREMENT(out, in) ((out).d = (in).d)

@@ -23,7 +23,8 @@ namespace plain {
 
 // J = sum over nodes |x - xt|^2 after nsteps steps from the case's state,
 // and its derivative along the direction dunkno of the initial DOFs, with
-// the limiter coefficients, the corner impedance's |du.n| and dt frozen.
+// the limiter coefficients (straight path) or WENO increments (curved
+// path), the corner impedances' |du.n| and dt frozen.
 // The case's arrays are read, not modified.
 extern "C" int orc_tangent_J(const plain::orc_case *p, const real_t *dunkno, int nsteps, const real_t *xt,
                              real_t *J, real_t *dJ)
@@ -59,6 +60,8 @@ extern "C" int orc_tangent_J(const plain::orc_case *p, const real_t *dunkno, int
     c.aflim = dup(nullptr, 5 * (size_t)c.ncell);
     c.afvec = dup(nullptr, 4 * (size_t)c.ncell);
     c.troubled = nullptr;
+    Dual *rho0 = p->rho0 ? dup(p->rho0, (size_t)c.ncell) : nullptr;
+    c.rho0 = rho0;
     int rc = 0;
     for (int n = 0; n < nsteps && !rc; ++n) {
         c.itime += 1;
@@ -70,5 +73,6 @@ extern "C" int orc_tangent_J(const plain::orc_case *p, const real_t *dunkno, int
     *dJ = j.d;
     free(c.coord); free(c.cooro); free(c.coold); free(c.geoel); free(c.amatr);
     free(c.unkno); free(c.unold); free(c.ustar); free(c.rhsel); free(c.aflim); free(c.afvec);
+    free(rho0);
     return rc;
 }
