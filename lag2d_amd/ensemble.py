@@ -226,9 +226,10 @@ def _c4_run_args(a):
 
 def c4_runs(nrun=256, seed=2024, lo=1000, hi=40000, workers=1, kinds=(0,)):
     """The seeded runs (host set-up in ``workers`` processes); run r is of
-    kind kinds[r % len(kinds)].  The default is Sedov only: the Sod-type
-    cylindrical tube (kind 1) tangles the jittered curved mesh within a few
-    steps on this path, alone as in an ensemble."""
+    kind kinds[r % len(kinds)].  The default is Sedov only: most runs of
+    the Sod-type cylindrical tube (kind 1) go non-finite within 10-140
+    steps on the jittered curved mesh, alone as in an ensemble, and the
+    oracle's runs do so at the same step."""
     sizes = c4_sizes(nrun, seed, lo, hi)
     args = [(int(s), seed * 1000 + r, kinds[r % len(kinds)]) for r, s in enumerate(sizes)]
     if workers <= 1:
