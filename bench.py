@@ -91,9 +91,9 @@ def _reference_rate(nx, nsteps, dofs_per_step):
 def batched_gain(args):
     """Aggregate cell-DOF updates/s of the ensemble benchmark (seeded runs
     with log-uniform cell counts, curved P2) advanced as one ensemble -- one
-    launch per kernel and stage for all runs -- against the same runs
-    advanced one after another with the plain path, over the same number of
-    steps."""
+    launch per kernel and stage for all runs, steps replayed from captured
+    HIP graphs -- against the same runs advanced one after another with the
+    plain path, over the same number of steps."""
     import torch
     from lag2d_amd import ensemble as E
     from lag2d_amd.lib import Lag2D
@@ -103,6 +103,7 @@ def batched_gain(args):
     dofs = cells * 6 * 4
     k, w = args.batched_steps, 2
     ens = E.Ensemble(runs)
+    ens.use_graphs(True)  # steps replayed from captured HIP graphs (results unchanged)
     for _ in range(w):
         ens.step_async()
     ens.sync()
@@ -132,7 +133,7 @@ def batched_gain(args):
     return {"runs": len(runs), "cells": cells, "steps": k, "warmup": w,
             "value": dofs * k / t_ens, "sequential_value": dofs * k / t_seq,
             "gain": t_seq / t_ens, "unit": "cell-DOF updates/s", "ms_per_step": 1e3 * t_ens / k,
-            "state_valid": ok,
+            "state_valid": ok, "graphs": True,
             "workload": f"{len(runs)} seeded planar Sedov runs (centre blast), curved Q2 P2, cells log-uniform in "
                         f"[1000, 40000], 5% perturbations of rho and p from 8 Fourier modes; each run its own dt"}
 

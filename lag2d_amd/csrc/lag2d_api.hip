@@ -346,7 +346,7 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     x->ipq_host = std::move(ipq);
     if (const char *v = std::getenv("LAG2D_CORNER_VARIANT")) {
         const int k = std::atoi(v);
-        if (k == 1 || k == 2) x->corner_variant = k;
+        if (k >= 1 && k <= LAG2D_CORNER_VARIANTS) x->corner_variant = k;
     }
     // the update reads each cell's M^-1 and residual once where the state
     // outgrows the caches (LAG2D_UPDATE_VARIANT = 1 per DOF, 2 per cell)
@@ -357,7 +357,7 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     }
     if (const char *v = std::getenv("LAG2D_RHS_VARIANT")) {
         const int k = std::atoi(v);
-        if (k == 1 || k == 2) x->rhs_variant = k;
+        if (k >= 1 && k <= LAG2D_RHS_VARIANTS) x->rhs_variant = k;
     }
     if (q2 && c.npoly == 1) {  // curved P1 has one corner and one cell-RHS form
         x->corner_variant = 2;
@@ -554,8 +554,8 @@ static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.33333333333333
 // cell-RHS slots name the variant the autotuner kept.
 static const char *kernel_name(const lag2d_ctx *x, int k)
 {
-    if (k == KID_Q2_CORNER && x->corner_variant == 2) return "k_q2_corner_sc";
-    if (k == KID_CELLRHS_Q2 && x->rhs_variant == 2) return "k_cellrhs_q2_tpc";
+    if (k == KID_Q2_CORNER && x->corner_variant >= 2) return "k_q2_corner_sc";
+    if (k == KID_CELLRHS_Q2 && x->rhs_variant >= 2) return "k_cellrhs_q2_tpc";
     if (k == KID_Q2_WENO && x->nd == 3) return "k_c1_weno";
     if (x->adj.curved && k == KID_ADJ_CELLRHS) return "k_adjc_cellrhs";
     if (x->adj.curved && k == KID_ADJ_CORNER) return "k_adjc_corner";

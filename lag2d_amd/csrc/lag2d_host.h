@@ -8,6 +8,8 @@
 #include "../../include/lag2d.h"
 #include "lag2d_dev.h"
 
+constexpr int LAG2D_CORNER_VARIANTS = 6, LAG2D_RHS_VARIANTS = 4;
+
 enum {
     KID_CELLAVG_Q1 = 0, KID_NODEMINMAX, KID_LIMIT_Q1, KID_NODE_Q1, KID_CELLRHS_Q1, KID_DT_PARTIAL,
     KID_DT_FINAL, KID_UPDATE, KID_NODEMOVE, KID_CELLVOL,
@@ -54,10 +56,14 @@ struct lag2d_ctx {
     int nrec = 40;          // corner-face record doubles per cell
     int nblk_dt = 1;        // blocks per run of the dt reduction
     int nrun = 1;           // runs of an ensemble
-    int corner_variant = 0; // curved-path corner kernel: 0 = not tuned yet, 1 = lanes per cell, 2 = lane per sub-cell
-    float corner_ms[2] = {0.f, 0.f};  // tuning timings of the two variants
-    int rhs_variant = 0;    // curved-path cell RHS: 0 = not tuned yet, 1 = 8 cells per block, 2 = thread per cell
-    float rhs_ms[2] = {0.f, 0.f};
+    // curved-path kernel variants (launch_corner, launch_cellrhs in
+    // lag2d_q2.hip): 0 = not tuned yet; corner 1 = lanes per cell, 2-6 = a
+    // lane per sub-cell (cells per workgroup, LDS staging); cell RHS 1 = 8
+    // cells per workgroup, 2-4 = a thread per cell (workgroup size)
+    int corner_variant = 0;
+    float corner_ms[LAG2D_CORNER_VARIANTS] = {};  // tuning timings
+    int rhs_variant = 0;
+    float rhs_ms[LAG2D_RHS_VARIANTS] = {};
     bool update_cells = false;  // k_update_cells (large meshes) instead of k_update
     std::vector<double> tend_host;
     hipStream_t stream = nullptr;

@@ -1132,6 +1132,9 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
 // at the quadrature points are read from a table (built by
 // k_q2_shape_tables with the same arithmetic) staged in LDS.  Face slots
 // are shared through LDS.  Results are identical to k_q2_corner's.
+// Template shape: CPB cells per workgroup (4 CPB threads) and whether the
+// shape-function table is staged in LDS (STAGE) or read through the caches;
+// the autotuner picks among them.
 constexpr int CSC_CELLS = 64;
 
 __global__ void __launch_bounds__(256) k_q2_shape_tables(double *tab)
@@ -1178,14 +1181,16 @@ __device__ __forceinline__ double clamp_eps(double eps, double v, uint8_t *tape,
     }
 }
 
-template <int ND, int MODE = 0>
-__global__ void __launch_bounds__(256) k_q2_corner_sc(Dev d)
+template <int ND, int MODE = 0, int CPB = CSC_CELLS, int STAGE = 1>
+__global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
 {
-    __shared__ double stab[125 * SHP_STRIDE];
-    __shared__ double sgq[CSC_CELLS][16 * 3 + 1];
-    for (int k = threadIdx.x; k < 125 * SHP_STRIDE; k += 256) stab[k] = d.shp[k];
+    __shared__ double stab_lds[STAGE ? 125 * SHP_STRIDE : 1];
+    __shared__ double sgq[CPB][16 * 3 + 1];
+    if constexpr (STAGE)
+        for (int k = threadIdx.x; k < 125 * SHP_STRIDE; k += 4 * CPB) stab_lds[k] = d.shp[k];
+    const double *stab = STAGE ? stab_lds : d.shp;
     const int cl = threadIdx.x >> 2, isg = threadIdx.x & 3;
-    const int ie = blockIdx.x * CSC_CELLS + cl;
+    const int ie = blockIdx.x * CPB + cl;
     const int nc = d.nc;
     const bool active = ie < nc;
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
@@ -1624,8 +1629,8 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
 // the shape-function derivatives are read from the table as wave-uniform
 // loads.  Every residual entry accumulates in the same order as in
 // k_cellrhs_q2: identical results.
-template <int ND, int MODE = 0>
-__global__ void __launch_bounds__(256) k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
+template <int ND, int MODE = 0, int BS = 256>
+__global__ void __launch_bounds__(BS) k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc;
@@ -2253,91 +2258,97 @@ int upload_extra(lag2d_ctx &x)
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
 }
 
-// Corner kernel variants (identical results): 1 = 16 lanes per cell (short
-// latency, for small meshes), 2 = one lane per sub-cell with tabulated shape
-// derivatives (throughput, for large meshes).
+// Corner kernel variants (identical results), the autotuner's choices:
+//   1    16 lanes per cell, 16 cells per workgroup (short latency, small meshes)
+//   2-4  a lane per sub-cell, 64 / 32 / 16 cells per workgroup, the
+//        shape-function table staged in LDS (throughput, large meshes)
+//   5-6  a lane per sub-cell, 64 / 32 cells per workgroup, the table read
+//        through the caches (no LDS staging)
 static void launch_corner(lag2d_ctx *x, Dev &ds, int variant)
 {
     const int nc = x->nc;
-    if (variant == 2)
-        hipLaunchKernelGGL(k_q2_corner_sc<6>, dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
-    else
-        hipLaunchKernelGGL(k_q2_corner, dim3((nc + CORNER_CPB - 1) / CORNER_CPB), dim3(256), 0, x->stream, ds);
+    auto g = [nc](int cpb) { return dim3((nc + cpb - 1) / cpb); };
+    switch (variant) {
+    case 2: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 64, 1>), g(64), dim3(256), 0, x->stream, ds); break;
+    case 3: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 32, 1>), g(32), dim3(128), 0, x->stream, ds); break;
+    case 4: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 16, 1>), g(16), dim3(64), 0, x->stream, ds); break;
+    case 5: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 64, 0>), g(64), dim3(256), 0, x->stream, ds); break;
+    case 6: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 32, 0>), g(32), dim3(128), 0, x->stream, ds); break;
+    default: hipLaunchKernelGGL(k_q2_corner, g(CORNER_CPB), dim3(256), 0, x->stream, ds); break;
+    }
 }
 
-// Autotuning on the context's own mesh: time both variants four times in
-// turn (the first round warms them up; the fastest of the other three
-// counts) and keep the faster.  Both write the same
-// values, so running both costs only time.  LAG2D_CORNER_VARIANT = 1 or 2
-// in the environment at context creation fixes the choice instead.
-static int tune_corner(lag2d_ctx *x, Dev &ds)
-{
-    hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return LAG2D_EDEVICE;
-    float ms[3] = {0.f, 1e30f, 1e30f};
-    int rc = 0;
-    for (int rep = 0; rep < 4 && !rc; ++rep)
-        for (int v = 1; v <= 2 && !rc; ++v) {
-            if (hipEventRecord(a, x->stream) != hipSuccess) rc = LAG2D_EDEVICE;
-            launch_corner(x, ds, v);
-            if (!rc && hipEventRecord(b, x->stream) != hipSuccess) rc = LAG2D_EDEVICE;
-            if (!rc && hipEventSynchronize(b) != hipSuccess) rc = LAG2D_EDEVICE;
-            float t = 0.f;
-            if (!rc && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = LAG2D_EDEVICE;
-            if (!rc && rep > 0 && t < ms[v]) ms[v] = t;  // the first round warms both
-        }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    if (rc) return rc;
-    x->corner_variant = (ms[2] < ms[1]) ? 2 : 1;
-    x->corner_ms[0] = ms[1];
-    x->corner_ms[1] = ms[2];
-    return 0;
-}
-
-// Cell-RHS variants (identical results): 1 = 8 cells per block with staged
-// corner and point values (small meshes), 2 = one thread per cell (large
-// meshes).
+// Cell-RHS variants (identical results):
+//   1    8 cells per workgroup, corner and point values staged in LDS (small meshes)
+//   2-4  a thread per cell, workgroups of 256 / 128 / 64 threads (large meshes)
 static void launch_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update, int variant)
 {
     const int nc = x->nc;
-    if (variant == 2)
-        hipLaunchKernelGGL(k_cellrhs_q2_tpc<6>, dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
-    else
-        hipLaunchKernelGGL(k_cellrhs_q2, dim3((nc + RHS_CPB - 1) / RHS_CPB), dim3(256), 0, x->stream, ds, alfa,
-                           apply_update);
+    auto g = [nc](int bs) { return dim3((nc + bs - 1) / bs); };
+    switch (variant) {
+    case 2: hipLaunchKernelGGL((k_cellrhs_q2_tpc<6, 0, 256>), g(256), dim3(256), 0, x->stream, ds, alfa, apply_update); break;
+    case 3: hipLaunchKernelGGL((k_cellrhs_q2_tpc<6, 0, 128>), g(128), dim3(128), 0, x->stream, ds, alfa, apply_update); break;
+    case 4: hipLaunchKernelGGL((k_cellrhs_q2_tpc<6, 0, 64>), g(64), dim3(64), 0, x->stream, ds, alfa, apply_update); break;
+    default:
+        hipLaunchKernelGGL(k_cellrhs_q2, g(RHS_CPB), dim3(256), 0, x->stream, ds, alfa, apply_update);
+        break;
+    }
 }
 
-// As tune_corner.  The kernel reads the stage input (Ui, and Ul for troubled
-// cells), the corner records and u*, and writes R and the stage state Us.
-// Timing it several times is harmless only where it does not read what it
-// writes: without the fused RK update, and with an input other than Us.
-// Elsewhere (the stages after the first, where Ui == Us and the update is
-// applied) the default variant runs untimed and tuning waits for a call
-// that allows it.
-static int tune_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update)
+// Autotuning on the context's own mesh: every variant is timed four times,
+// in turn (the first round warms them up; the fastest of the other three
+// counts), and the fastest is kept.  All variants write the same values,
+// so running each costs only time.  Returns the variant, or 0 on a HIP
+// error.
+template <class Launch>
+static int tune_variants(lag2d_ctx *x, int nvar, float *ms, Launch launch)
 {
-    if (apply_update != 0 || ds.Ui == ds.Us) return 0;
     hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return LAG2D_EDEVICE;
-    float ms[3] = {0.f, 1e30f, 1e30f};
-    int rc = 0;
-    for (int rep = 0; rep < 4 && !rc; ++rep)
-        for (int v = 1; v <= 2 && !rc; ++v) {
-            if (hipEventRecord(a, x->stream) != hipSuccess) rc = LAG2D_EDEVICE;
-            launch_cellrhs(x, ds, alfa, apply_update, v);
-            if (!rc && hipEventRecord(b, x->stream) != hipSuccess) rc = LAG2D_EDEVICE;
-            if (!rc && hipEventSynchronize(b) != hipSuccess) rc = LAG2D_EDEVICE;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 0;
+    for (int v = 0; v < nvar; ++v) ms[v] = 1e30f;
+    bool ok = true;
+    for (int rep = 0; rep < 4 && ok; ++rep)
+        for (int v = 1; v <= nvar && ok; ++v) {
+            ok = hipEventRecord(a, x->stream) == hipSuccess;
+            launch(v);
+            ok = ok && hipEventRecord(b, x->stream) == hipSuccess && hipEventSynchronize(b) == hipSuccess;
             float t = 0.f;
-            if (!rc && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = LAG2D_EDEVICE;
-            if (!rc && rep > 0 && t < ms[v]) ms[v] = t;  // the first round warms both
+            ok = ok && hipEventElapsedTime(&t, a, b) == hipSuccess;
+            if (ok && rep > 0 && t < ms[v - 1]) ms[v - 1] = t;
         }
     hipEventDestroy(a);
     hipEventDestroy(b);
-    if (rc) return rc;
-    x->rhs_variant = (ms[2] < ms[1]) ? 2 : 1;
-    x->rhs_ms[0] = ms[1];
-    x->rhs_ms[1] = ms[2];
+    if (!ok) return 0;
+    int best = 1;
+    for (int v = 2; v <= nvar; ++v)
+        if (ms[v - 1] < ms[best - 1]) best = v;
+    return best;
+}
+
+// LAG2D_CORNER_VARIANT in the environment at context creation fixes the
+// choice instead.
+static int tune_corner(lag2d_ctx *x, Dev &ds)
+{
+    const int v = tune_variants(x, LAG2D_CORNER_VARIANTS, x->corner_ms, [&](int k) { launch_corner(x, ds, k); });
+    if (!v) return LAG2D_EDEVICE;
+    x->corner_variant = v;
+    return 0;
+}
+
+// The cell RHS reads the stage input (Ui, and Ul for troubled cells), the
+// corner records and u*, and writes R and the stage state Us.  Timing it
+// several times is harmless only where it does not read what it writes:
+// without the fused RK update, and with an input other than Us.  Elsewhere
+// (the stages after the first, where Ui == Us and the update is applied)
+// the default variant runs untimed and tuning waits for a call that allows
+// it.
+static int tune_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update)
+{
+    if (apply_update != 0 || ds.Ui == ds.Us) return 0;
+    const int v = tune_variants(x, LAG2D_RHS_VARIANTS, x->rhs_ms,
+                                [&](int k) { launch_cellrhs(x, ds, alfa, apply_update, k); });
+    if (!v) return LAG2D_EDEVICE;
+    x->rhs_variant = v;
     return 0;
 }
 

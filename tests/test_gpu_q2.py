@@ -1,7 +1,7 @@
 """HIP curved-quad P2 path (sub-cell Riemann solver, Persson indicator,
 characteristic WENO) against the reference's fixtures and the oracle.
-Bar: bit-identical states (same operation order, no contraction), for both
-variants of the corner and cell-RHS kernels."""
+Bar: bit-identical states (same operation order, no contraction), for every
+variant of the corner and cell-RHS kernels the autotuner chooses among."""
 import numpy as np
 import pytest
 
@@ -11,12 +11,19 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-@pytest.fixture(params=[1, 2], ids=["small-mesh-kernels", "large-mesh-kernels"])
+# (corner, cell RHS) variants: lanes per cell / 8 cells per workgroup; a lane
+# per sub-cell with 64, 32, 16 cells per workgroup and the shape table in LDS,
+# or 64, 32 cells reading it through the caches / a thread per cell in
+# workgroups of 256, 128, 64
+VARIANT_PAIRS = [(1, 1), (2, 2), (3, 3), (4, 4), (5, 2), (6, 3)]
+
+
+@pytest.fixture(params=VARIANT_PAIRS, ids=[f"corner{c}-rhs{r}" for c, r in VARIANT_PAIRS])
 def variant(request, monkeypatch):
     """Kernel variants (corner and cell RHS; the library picks them by timing)
     fixed for the contexts a test creates."""
-    monkeypatch.setenv("LAG2D_CORNER_VARIANT", str(request.param))
-    monkeypatch.setenv("LAG2D_RHS_VARIANT", str(request.param))
+    monkeypatch.setenv("LAG2D_CORNER_VARIANT", str(request.param[0]))
+    monkeypatch.setenv("LAG2D_RHS_VARIANT", str(request.param[1]))
     return request.param
 
 
