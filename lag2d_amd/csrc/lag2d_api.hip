@@ -251,7 +251,7 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         (rc = x->alloc(&d.usx, (size_t)npoin)) || (rc = x->alloc(&d.usy, (size_t)npoin)) ||
         (rc = x->alloc(&d.cavg, (size_t)5 * nc)) || (rc = x->alloc(&d.frame, (size_t)4 * nc)) ||
         (rc = x->alloc(&d.nmm, (size_t)10 * npoin)) || (rc = x->alloc(&d.lim, (size_t)9 * nc)) ||
-        (rc = x->alloc(&d.rec, (size_t)x->nrec * nc)) ||
+        (rc = x->alloc(&d.rec, (size_t)x->nrec * (q2 ? (nc + 63) / 64 * 64 : nc))) ||
         (rc = x->alloc(&d.red, (size_t)2 * nrun * x->nblk_dt)) || (rc = x->alloc(&d.scal, (size_t)2 * nrun + 2)))
         return bad(rc);
     d.nrun = nrun;
@@ -785,9 +785,11 @@ static bool adjc_supported(const lag2d_ctx *x)
 
 static size_t adjc_tape_doubles(const lag2d_ctx *x) { return (size_t)(8 * x->nd + 32) * x->nc; }
 static size_t adjc_tape_bytes(const lag2d_ctx *x) { return (size_t)44 * x->nc; }
+// records of the curved path: 160 per cell, in tiles of 64 cells
+static size_t adjc_rec_doubles(const lag2d_ctx *x) { return (size_t)160 * ((x->nc + 63) / 64 * 64); }
 static size_t adjc_stage_stride(const lag2d_ctx *x)
 {
-    return (size_t)(4 * x->nd + 160 + 48) * x->nc + (size_t)4 * x->np;
+    return (size_t)(4 * x->nd + 48) * x->nc + adjc_rec_doubles(x) + (size_t)4 * x->np;
 }
 
 // One step of the curved path's recorded (RECORD), replayed (REPLAY) or
@@ -825,8 +827,8 @@ static int adjc_fwd_step(lag2d_ctx *x, int itime, int n, int mode, int k)
             HIPCHK(cp(slot, d.Ul, nU));
             HIPCHK(cp(slot + nU + 2 * (size_t)np, d.usx, np));
             HIPCHK(cp(slot + nU + 3 * (size_t)np, d.usy, np));
-            HIPCHK(cp(slot + nU + 4 * (size_t)np, d.rec, (size_t)160 * nc));
-            HIPCHK(cp(slot + nU + 4 * (size_t)np + (size_t)160 * nc, d.gq, (size_t)48 * nc));
+            HIPCHK(cp(slot + nU + 4 * (size_t)np, d.rec, adjc_rec_doubles(x)));
+            HIPCHK(cp(slot + nU + 4 * (size_t)np + adjc_rec_doubles(x), d.gq, (size_t)48 * nc));
         }
         if (s == 0) {
             if (mode == ADJ_RECORD) {
@@ -952,7 +954,7 @@ static int adjc_gradient(lag2d_ctx *x, const double *lam_coord, double *grad)
                 a.usx = a.yin + np;
                 a.usy = a.usx + np;
                 a.rec = a.usy + np;
-                a.gq = a.rec + (size_t)160 * nc;
+                a.gq = a.rec + adjc_rec_doubles(x);
                 a.dlt = A.lim_tape + ts * adjc_tape_doubles(x) + 2 * nU;
                 a.ccl = A.clamp_tape + ts * adjc_tape_bytes(x);
                 a.dcl = a.ccl + (size_t)18 * nc;

@@ -7,11 +7,13 @@
 
 namespace lag2d {
 
-// Corner-face records, cell-major: the 5 records of corner face cf of cell
-// ie at (32 ie + cf) 5 + r, so the two faces of a sub-cell corner are 10
-// consecutive doubles for the corner kernel that writes them, the node
-// gather and the cell RHS that read them.
-#define RQ2(ie, cf, r) (((size_t)(ie) * 32 + (cf)) * 5 + (r))
+// Corner-face records in tiles of 64 cells: record r of corner face cf of
+// cell ie at ((ie / 64) 160 + 5 cf + r) 64 + ie % 64.  The 64 cells of a
+// wave read or write each record as 512 contiguous bytes (the thread-per-
+// cell kernels), and a node's corners in neighbouring cells share the
+// tile's lines (the node gather).  The array holds Q2_REC_CELLS(nc) cells.
+#define RQ2(ie, cf, r) (((((size_t)(ie)) >> 6) * 160 + (size_t)(cf) * 5 + (r)) * 64 + (((size_t)(ie)) & 63))
+#define Q2_REC_CELLS(nc) ((((size_t)(nc)) + 63) / 64 * 64)
 
 // sub-cell vertex -> local Q2 node (ipqsg)
 __constant__ static const int IPQSG[4][4] = {{0, 4, 8, 7}, {4, 1, 5, 8}, {8, 5, 2, 6}, {7, 8, 6, 3}};
