@@ -355,6 +355,14 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         const int k = std::atoi(v);
         if (k == 1 || k == 2) x->update_cells = k == 2;
     }
+    // cell volume and Persson indicator: a thread per cell where the mesh
+    // fills the chip with it (LAG2D_GEOM_VARIANT = 1 8 lanes per cell, 2 a
+    // thread per cell)
+    x->geom_cells = nc >= GEOM_CELLS_MIN;
+    if (const char *v = std::getenv("LAG2D_GEOM_VARIANT")) {
+        const int k = std::atoi(v);
+        if (k == 1 || k == 2) x->geom_cells = k == 2;
+    }
     if (const char *v = std::getenv("LAG2D_RHS_VARIANT")) {
         const int k = std::atoi(v);
         if (k >= 1 && k <= LAG2D_RHS_VARIANTS) x->rhs_variant = k;
@@ -598,10 +606,14 @@ static void launch_cellvol(lag2d_ctx *x, double *xs, double *ys, int set_centre,
 {
     const int grid = (x->nc + CV_CELLS - 1) / CV_CELLS;
     if (timed) x->tic(KID_CELLVOL);
-    if (x->cfg.nrz == 2)
-        hipLaunchKernelGGL(k_cellvol<1>, dim3(grid), dim3(LAG2D_BLOCK), 0, x->stream, x->d, xs, ys, set_centre);
-    else
-        hipLaunchKernelGGL(k_cellvol<0>, dim3(grid), dim3(LAG2D_BLOCK), 0, x->stream, x->d, xs, ys, set_centre);
+    const int rz = x->cfg.nrz == 2;
+    if (x->geom_cells) {
+        auto k = rz ? k_cellvol_tpc<1> : k_cellvol_tpc<0>;
+        hipLaunchKernelGGL(k, dim3(nblocks(x->nc)), dim3(LAG2D_BLOCK), 0, x->stream, x->d, xs, ys, set_centre);
+    } else {
+        auto k = rz ? k_cellvol<1> : k_cellvol<0>;
+        hipLaunchKernelGGL(k, dim3(grid), dim3(LAG2D_BLOCK), 0, x->stream, x->d, xs, ys, set_centre);
+    }
     if (timed) x->toc(KID_CELLVOL);
 }
 

@@ -279,11 +279,14 @@ __constant__ static const double PG6[6] = {0.2386191860831969086305017, -0.23861
 __constant__ static const double WG6[6] = {0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
                                            0.3607615730481386, 0.1713244923791704, 0.1713244923791704};
 
-// Cell volume and volume centre on the 6x6 Gauss rule, 8 lanes per cell:
-// the lanes evaluate the Jacobians, lane 0 sums them in point order.  With
-// set_centre (curved cells) the Q2 centre node is first reset to
-// -1/4 sum(corners) + 1/2 sum(mid-nodes) (Getrhslag.f90:1962-1971); it
-// belongs to this cell only, so the reset and the volume fuse.
+// Cell volume and volume centre on the 6x6 Gauss rule.  With set_centre
+// (curved cells) the Q2 centre node is first reset to -1/4 sum(corners) +
+// 1/2 sum(mid-nodes) (Getrhslag.f90:1962-1971); it belongs to this cell
+// only, so the reset and the volume fuse.  Two work splits with the same
+// arithmetic: k_cellvol, 8 lanes per cell (the lanes evaluate the
+// Jacobians, lane 0 sums them in point order), for meshes too small to fill
+// the chip with a thread per cell; k_cellvol_tpc, a thread per cell whose
+// unrolled point loop turns the shape functions into constants.
 // rutope 7-point area rule of the reference triangle (dgflo2d.f90:2234-2255)
 // The area-weighted rows of a cell from its radius-weighted volume, area
 // and area moments.
@@ -312,6 +315,162 @@ __constant__ static const double WT7_C[7] = {0.225000000000000, 0.13239415278850
 // + alfrz*y at each point) and the area-weighted rows follow
 // (Limiterlag.f90:2001-2035, :2140-2160): geoel(11) mean radius = volume /
 // area, geoel(12) area, and the area centre in geoel(5:6) and geoel(15:16).
+// The 9 Q2 nodes of quad ie: straight cells (ncurv=0) take the mid and
+// centre nodes from the corners; curved ones read them, or with set_centre
+// reset the centre (written back when write_centre).
+__device__ __forceinline__ void cellvol_nodes(const Dev &d, double *__restrict__ x, double *__restrict__ y, int ie,
+                                              int set_centre, bool write_centre, double *xp, double *yp)
+{
+    const int nc = d.nc;
+    if (d.ncurv == 0) {
+        for (int k = 0; k < 4; ++k) {
+            int p = d.ipq[k * nc + ie];
+            xp[k] = x[p];
+            yp[k] = y[p];
+        }
+        xp[4] = 0.5 * (xp[0] + xp[1]); yp[4] = 0.5 * (yp[0] + yp[1]);
+        xp[5] = 0.5 * (xp[1] + xp[2]); yp[5] = 0.5 * (yp[1] + yp[2]);
+        xp[6] = 0.5 * (xp[2] + xp[3]); yp[6] = 0.5 * (yp[2] + yp[3]);
+        xp[7] = 0.5 * (xp[3] + xp[0]); yp[7] = 0.5 * (yp[3] + yp[0]);
+        xp[8] = 0.5 * (xp[4] + xp[6]); yp[8] = 0.5 * (yp[4] + yp[6]);
+        return;
+    }
+    for (int k = 0; k < 8; ++k) {
+        int p = d.ipq[k * nc + ie];
+        xp[k] = x[p];
+        yp[k] = y[p];
+    }
+    const int pc = d.ipq[8 * nc + ie];
+    if (set_centre) {
+        const double cx = xp[0] + xp[1] + xp[2] + xp[3], mx = xp[4] + xp[5] + xp[6] + xp[7];
+        const double cy = yp[0] + yp[1] + yp[2] + yp[3], my = yp[4] + yp[5] + yp[6] + yp[7];
+        xp[8] = -0.25 * cx + 0.5 * mx;
+        yp[8] = -0.25 * cy + 0.5 * my;
+        if (write_centre) {
+            x[pc] = xp[8];
+            y[pc] = yp[8];
+        }
+    } else {
+        xp[8] = x[pc];
+        yp[8] = y[pc];
+    }
+}
+
+// Weighted Jacobian determinant of the Q2 map at 6x6 point g (and the
+// mapped y there, for RZ).
+__device__ __forceinline__ double cellvol_point(const double *xp, const double *yp, int g, double *yg)
+{
+    const int i = g / 6, j = g - 6 * i;
+    const double r = PG6[i], s = PG6[j], wi = WG6[i] * WG6[j];
+    double sh[9], dr_[9], ds_[9];
+    shape_q2_d(r, s, sh, dr_, ds_);
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        dxdr = dxdr + dr_[k] * xp[k];
+        dxds = dxds + ds_[k] * xp[k];
+        dydr = dydr + dr_[k] * yp[k];
+        dyds = dyds + ds_[k] * yp[k];
+    }
+    if (yg) {
+        double v = 0.0;
+        for (int k = 0; k < 9; ++k) v = v + sh[k] * yp[k];
+        *yg = v;
+    }
+    return wi * (dxdr * dyds - dydr * dxds);
+}
+
+// triangle: 7-point rule on the quadratic triangle map with chord
+// midpoints as mid-nodes (Limiterlag.f90:1898-2036)
+template <int RZ>
+__device__ __forceinline__ void cellvol_tria(const Dev &d, const double *__restrict__ x,
+                                             const double *__restrict__ y, int ie)
+{
+    const int nc = d.nc;
+    double xp[6], yp[6];
+    for (int k = 0; k < 3; ++k) {
+        const int p = d.ipq[k * nc + ie];
+        xp[k] = x[p];
+        yp[k] = y[p];
+    }
+    xp[3] = 0.5 * (xp[0] + xp[1]); yp[3] = 0.5 * (yp[0] + yp[1]);
+    xp[4] = 0.5 * (xp[1] + xp[2]); yp[4] = 0.5 * (yp[1] + yp[2]);
+    xp[5] = 0.5 * (xp[0] + xp[2]); yp[5] = 0.5 * (yp[0] + yp[2]);
+    const double c10 = 1.0;
+    double rcv = 0.0, scv = 0.0, vol = 0.0, area = 0.0, rca = 0.0, sca = 0.0;
+    for (int g = 0; g < 7; ++g) {
+        const double r = PT7R_C[g], s = PT7S_C[g], wi = WT7_C[g];
+        const double dr_[6] = {c10 - 4.0 * (c10 - r - s), -1.0 + 4.0 * r, 0.0, 4.0 * (1.0 - 2.0 * r - s),
+                               4.0 * s, -4.0 * s};
+        const double ds_[6] = {c10 - 4.0 * (c10 - r - s), 0.0, -1.0 + 4.0 * s, -4.0 * r, 4.0 * r,
+                               4.0 * (1.0 - r - 2.0 * s)};
+        double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+        for (int k = 0; k < 6; ++k) {
+            dxdr = dxdr + dr_[k] * xp[k];
+            dxds = dxds + ds_[k] * xp[k];
+            dydr = dydr + dr_[k] * yp[k];
+            dyds = dyds + ds_[k] * yp[k];
+        }
+        const double djaco = 0.5 * wi * (dxdr * dyds - dydr * dxds);
+        if (RZ) {
+            const double sh[6] = {-(c10 - r - s) * (c10 - 2.0 * (c10 - r - s)), -r * (c10 - 2.0 * r),
+                                  -s * (c10 - 2.0 * s), 4.0 * r * (c10 - r - s), 4.0 * r * s,
+                                  4.0 * s * (c10 - r - s)};
+            double yg = 0.0;
+            for (int k = 0; k < 6; ++k) yg = yg + sh[k] * yp[k];
+            const double rcoef = 1.0 - d.alfrz + d.alfrz * yg;
+            vol = vol + djaco * rcoef;
+            area = area + djaco;
+            rca = rca + r * djaco;
+            sca = sca + s * djaco;
+        } else {
+            vol = vol + djaco * 1.0;
+            rcv = rcv + r * djaco * 1.0;
+            scv = scv + s * djaco * 1.0;
+        }
+    }
+    d.G[2 * nc + ie] = vol;
+    if (RZ) {
+        rz_rows(d, ie, vol, area, rca, sca);
+    } else {
+        d.G[4 * nc + ie] = rcv / vol;
+        d.G[5 * nc + ie] = scv / vol;
+    }
+}
+
+// Sums over the 36 points in point order (planar: rcoef = 1 - alfrz +
+// alfrz*y = 1) and the rows they give.
+template <int RZ>
+struct CellvolSum {
+    double vol = 0.0, rcv = 0.0, scv = 0.0, area = 0.0, rca = 0.0, sca = 0.0;
+    __device__ __forceinline__ void add(const Dev &d, int g, double djaco, double yg)
+    {
+        const int i = g / 6, j = g - 6 * i;
+        const double r = PG6[i], s = PG6[j];
+        if (RZ) {
+            const double rcoef = 1.0 - d.alfrz + d.alfrz * yg;
+            vol = vol + djaco * rcoef;
+            area = area + djaco;
+            rca = rca + r * djaco;
+            sca = sca + s * djaco;
+        } else {
+            vol = vol + djaco * 1.0;
+            rcv = rcv + r * djaco * 1.0;
+            scv = scv + s * djaco * 1.0;
+        }
+    }
+    __device__ __forceinline__ void store(const Dev &d, int ie) const
+    {
+        const int nc = d.nc;
+        d.G[2 * nc + ie] = vol;
+        if (RZ) {
+            rz_rows(d, ie, vol, area, rca, sca);
+        } else {
+            d.G[4 * nc + ie] = rcv / vol;
+            d.G[5 * nc + ie] = scv / vol;
+        }
+    }
+};
+
 template <int RZ>
 __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y,
                                                  int set_centre)
@@ -323,146 +482,51 @@ __global__ void __launch_bounds__(256) k_cellvol(Dev d, double *__restrict__ x, 
     const int nc = d.nc;
     if (ie < nc && ie >= d.ntria) {
         double xp[9], yp[9];
-        if (d.ncurv == 0) {
-            for (int k = 0; k < 4; ++k) {
-                int p = d.ipq[k * nc + ie];
-                xp[k] = x[p];
-                yp[k] = y[p];
-            }
-            xp[4] = 0.5 * (xp[0] + xp[1]); yp[4] = 0.5 * (yp[0] + yp[1]);
-            xp[5] = 0.5 * (xp[1] + xp[2]); yp[5] = 0.5 * (yp[1] + yp[2]);
-            xp[6] = 0.5 * (xp[2] + xp[3]); yp[6] = 0.5 * (yp[2] + yp[3]);
-            xp[7] = 0.5 * (xp[3] + xp[0]); yp[7] = 0.5 * (yp[3] + yp[0]);
-            xp[8] = 0.5 * (xp[4] + xp[6]); yp[8] = 0.5 * (yp[4] + yp[6]);
-        } else {
-            for (int k = 0; k < 8; ++k) {
-                int p = d.ipq[k * nc + ie];
-                xp[k] = x[p];
-                yp[k] = y[p];
-            }
-            const int pc = d.ipq[8 * nc + ie];
-            if (set_centre) {
-                const double cx = xp[0] + xp[1] + xp[2] + xp[3], mx = xp[4] + xp[5] + xp[6] + xp[7];
-                const double cy = yp[0] + yp[1] + yp[2] + yp[3], my = yp[4] + yp[5] + yp[6] + yp[7];
-                xp[8] = -0.25 * cx + 0.5 * mx;
-                yp[8] = -0.25 * cy + 0.5 * my;
-                if (lane == 0) {
-                    x[pc] = xp[8];
-                    y[pc] = yp[8];
-                }
-            } else {
-                xp[8] = x[pc];
-                yp[8] = y[pc];
-            }
-        }
+        cellvol_nodes(d, x, y, ie, set_centre, lane == 0, xp, yp);
         for (int g = lane; g < 36; g += 8) {
-            const int i = g / 6, j = g - 6 * i;
-            const double r = PG6[i], s = PG6[j], wi = WG6[i] * WG6[j];
-            double sh[9], dr_[9], ds_[9];
-            shape_q2_d(r, s, sh, dr_, ds_);
-            double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
-            for (int k = 0; k < 9; ++k) {
-                dxdr = dxdr + dr_[k] * xp[k];
-                dxds = dxds + ds_[k] * xp[k];
-                dydr = dydr + dr_[k] * yp[k];
-                dyds = dyds + ds_[k] * yp[k];
-            }
-            sdj[cl][g] = wi * (dxdr * dyds - dydr * dxds);
-            if (RZ) {
-                double yg = 0.0;
-                for (int k = 0; k < 9; ++k) yg = yg + sh[k] * yp[k];
-                syg[RZ ? cl : 0][g] = yg;
-            }
+            double yg;
+            sdj[cl][g] = cellvol_point(xp, yp, g, RZ ? &yg : nullptr);
+            if (RZ) syg[RZ ? cl : 0][g] = yg;
         }
     }
     __syncthreads();
     if (ie >= nc || lane != 0) return;
     if (ie < d.ntria) {
-        // triangle: 7-point rule on the quadratic triangle map with chord
-        // midpoints as mid-nodes (Limiterlag.f90:1898-2036)
-        double xp[6], yp[6];
-        for (int k = 0; k < 3; ++k) {
-            const int p = d.ipq[k * nc + ie];
-            xp[k] = x[p];
-            yp[k] = y[p];
-        }
-        xp[3] = 0.5 * (xp[0] + xp[1]); yp[3] = 0.5 * (yp[0] + yp[1]);
-        xp[4] = 0.5 * (xp[1] + xp[2]); yp[4] = 0.5 * (yp[1] + yp[2]);
-        xp[5] = 0.5 * (xp[0] + xp[2]); yp[5] = 0.5 * (yp[0] + yp[2]);
-        const double c10 = 1.0;
-        double rcv = 0.0, scv = 0.0, vol = 0.0, area = 0.0, rca = 0.0, sca = 0.0;
-        for (int g = 0; g < 7; ++g) {
-            const double r = PT7R_C[g], s = PT7S_C[g], wi = WT7_C[g];
-            const double dr_[6] = {c10 - 4.0 * (c10 - r - s), -1.0 + 4.0 * r, 0.0, 4.0 * (1.0 - 2.0 * r - s),
-                                   4.0 * s, -4.0 * s};
-            const double ds_[6] = {c10 - 4.0 * (c10 - r - s), 0.0, -1.0 + 4.0 * s, -4.0 * r, 4.0 * r,
-                                   4.0 * (1.0 - r - 2.0 * s)};
-            double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
-            for (int k = 0; k < 6; ++k) {
-                dxdr = dxdr + dr_[k] * xp[k];
-                dxds = dxds + ds_[k] * xp[k];
-                dydr = dydr + dr_[k] * yp[k];
-                dyds = dyds + ds_[k] * yp[k];
-            }
-            const double djaco = 0.5 * wi * (dxdr * dyds - dydr * dxds);
-            if (RZ) {
-                const double sh[6] = {-(c10 - r - s) * (c10 - 2.0 * (c10 - r - s)), -r * (c10 - 2.0 * r),
-                                      -s * (c10 - 2.0 * s), 4.0 * r * (c10 - r - s), 4.0 * r * s,
-                                      4.0 * s * (c10 - r - s)};
-                double yg = 0.0;
-                for (int k = 0; k < 6; ++k) yg = yg + sh[k] * yp[k];
-                const double rcoef = 1.0 - d.alfrz + d.alfrz * yg;
-                vol = vol + djaco * rcoef;
-                area = area + djaco;
-                rca = rca + r * djaco;
-                sca = sca + s * djaco;
-            } else {
-                vol = vol + djaco * 1.0;
-                rcv = rcv + r * djaco * 1.0;
-                scv = scv + s * djaco * 1.0;
-            }
-        }
-        d.G[2 * nc + ie] = vol;
-        if (RZ) {
-            rz_rows(d, ie, vol, area, rca, sca);
-        } else {
-            d.G[4 * nc + ie] = rcv / vol;
-            d.G[5 * nc + ie] = scv / vol;
-        }
+        cellvol_tria<RZ>(d, x, y, ie);
         return;
     }
-    if (RZ) {
-        double vol = 0.0, area = 0.0, rca = 0.0, sca = 0.0;
-        for (int g = 0; g < 36; ++g) {
-            const int i = g / 6, j = g - 6 * i;
-            const double r = PG6[i], s = PG6[j], djaco = sdj[cl][g];
-            const double rcoef = 1.0 - d.alfrz + d.alfrz * syg[RZ ? cl : 0][g];
-            vol = vol + djaco * rcoef;
-            area = area + djaco;
-            rca = rca + r * djaco;
-            sca = sca + s * djaco;
-        }
-        d.G[2 * nc + ie] = vol;
-        rz_rows(d, ie, vol, area, rca, sca);
+    CellvolSum<RZ> acc;
+#pragma unroll
+    for (int g = 0; g < 36; ++g) acc.add(d, g, sdj[cl][g], RZ ? syg[RZ ? cl : 0][g] : 0.0);
+    acc.store(d, ie);
+}
+
+template <int RZ>
+__global__ void __launch_bounds__(256) k_cellvol_tpc(Dev d, double *__restrict__ x, double *__restrict__ y,
+                                                     int set_centre)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= d.nc) return;
+    if (ie < d.ntria) {
+        cellvol_tria<RZ>(d, x, y, ie);
         return;
     }
-    // planar: rcoef = 1 - alfrz + alfrz*y = 1
-    double rcv = 0.0, scv = 0.0, vol = 0.0;
+    double xp[9], yp[9];
+    cellvol_nodes(d, x, y, ie, set_centre, true, xp, yp);
+    CellvolSum<RZ> acc;
 #pragma unroll
     for (int g = 0; g < 36; ++g) {
-        const int i = g / 6, j = g - 6 * i;
-        const double r = PG6[i], s = PG6[j], djaco = sdj[cl][g];
-        vol = vol + djaco * 1.0;
-        rcv = rcv + r * djaco * 1.0;
-        scv = scv + s * djaco * 1.0;
+        double yg = 0.0;
+        const double djaco = cellvol_point(xp, yp, g, RZ ? &yg : nullptr);
+        acc.add(d, g, djaco, yg);
     }
-    d.G[2 * nc + ie] = vol;
-    d.G[4 * nc + ie] = rcv / vol;
-    d.G[5 * nc + ie] = scv / vol;
+    acc.store(d, ie);
 }
 
 template __global__ void k_cellvol<0>(Dev, double *__restrict__, double *__restrict__, int);
 template __global__ void k_cellvol<1>(Dev, double *__restrict__, double *__restrict__, int);
+template __global__ void k_cellvol_tpc<0>(Dev, double *__restrict__, double *__restrict__, int);
+template __global__ void k_cellvol_tpc<1>(Dev, double *__restrict__, double *__restrict__, int);
 
 // 5x5 Gauss rule of ruqope (dgflo2d.f90:2734-2743 ordering)
 __constant__ static const double PG5I[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,

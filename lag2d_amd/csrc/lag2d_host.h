@@ -65,6 +65,7 @@ struct lag2d_ctx {
     int rhs_variant = 0;
     float rhs_ms[LAG2D_RHS_VARIANTS] = {};
     bool update_cells = false;  // k_update_cells (large meshes) instead of k_update
+    bool geom_cells = false;    // k_cellvol_tpc, k_q2_prep_tpc (large meshes) instead of 8 lanes per cell
     std::vector<double> tend_host;
     hipStream_t stream = nullptr;
     lag2d::Dev d{};

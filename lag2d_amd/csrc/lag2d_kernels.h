@@ -22,6 +22,8 @@ __global__ void k_update_cells(Dev d, double alfa);
 __global__ void k_nodemove(Dev d, double alfa, const double *__restrict__ xsrc, const double *__restrict__ ysrc);
 template <int RZ>
 __global__ void k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y, int set_centre);
+template <int RZ>
+__global__ void k_cellvol_tpc(Dev d, double *__restrict__ x, double *__restrict__ y, int set_centre);
 __global__ void k_init_q2(Dev d, const double *__restrict__ in, double *A);
 
 // area-weighted RZ path (nrz=2), straight P1 cells
@@ -39,5 +41,6 @@ __global__ void k_adj_gather(Dev d, AdjStage a);
 __global__ void k_adj_step_end(Dev d, AdjStage a);
 __global__ void k_replay_dt(Dev d, const double *dt);
 constexpr int CV_CELLS = 32;  // cells per block of k_cellvol
+constexpr int GEOM_CELLS_MIN = 1 << 16;  // cells from which cell volume and prep run a thread per cell
 
 }  // namespace lag2d

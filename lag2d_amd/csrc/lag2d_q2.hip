@@ -41,9 +41,56 @@ __device__ __forceinline__ const double *cell_dofs(const Dev &d, int ie)
 // Evaluates the Persson indicator on tau, sets each cell's troubled flag
 // and appends troubled cells to the WENO list (the stage DOFs are not
 // copied: WENO writes only the troubled cells' limited DOFs to Ul, and
-// cell_dofs reads the other cells' from the stage input).  8 lanes per
-// cell evaluate the 25 Gauss points; lane 0 sums them in point order.
+// cell_dofs reads the other cells' from the stage input).  Two work splits
+// with the same arithmetic: k_q2_prep, 8 lanes per cell evaluating the 25
+// Gauss points and lane 0 summing them in point order, for meshes too small
+// to fill the chip with a thread per cell; k_q2_prep_tpc, a thread per cell
+// whose unrolled point loop turns the shape functions into constants.
 constexpr int PREP_CELLS = 32;
+
+// The cell's data the indicator reads.
+struct PrepCell {
+    double u0[6], xp[9], yp[9], rc, sc, g0, g1, g2;
+    __device__ __forceinline__ PrepCell(const Dev &d, const double *__restrict__ Uin, int ie)
+    {
+        const int nc = d.nc;
+        for (int id = 0; id < 6; ++id) u0[id] = UD(Uin, 0, id, ie);
+        load9(d, d.xs, d.ys, ie, xp, yp);
+        rc = d.G[0 * nc + ie];
+        sc = d.G[1 * nc + ie];
+        g0 = d.G[18 * nc + ie];
+        g1 = d.G[19 * nc + ie];
+        g2 = d.G[20 * nc + ie];
+    }
+    // the two integrands at 5x5 point g: |u|^2 and |u - u_P1|^2 times the
+    // weighted Jacobian
+    __device__ __forceinline__ void point(int g, double &a, double &b) const
+    {
+        const int i = g / 5, j = g - 5 * i;
+        const double r = PG5[i], s = PG5[j], wi = WG5[i] * WG5[j];
+        double J[4];
+        jac9(xp, yp, r, s, J);
+        const double djaco = wi * (J[0] * J[3] - J[2] * J[1]);
+        double bs[6];
+        basis6(r, s, rc, sc, g0, g1, g2, bs);
+        double un = 0.0, u1 = 0.0;
+        for (int id = 0; id < 6; ++id) un = un + u0[id] * bs[id];
+        for (int id = 0; id < 3; ++id) u1 = u1 + u0[id] * bs[id];
+        a = un * un * djaco;
+        b = (u1 - un) * (u1 - un) * djaco;
+    }
+};
+
+__device__ __forceinline__ void prep_flag(const Dev &d, int ie, double v1, double v2)
+{
+    const double thr = -3.0 * log10(4.0) - 4.0;
+    const bool troubled = !(log10(v2 / v1) <= thr) && run_active(d, cell_run(d, ie));
+    d.flag[ie] = troubled ? 1 : 0;
+    if (troubled) {
+        int pos = atomicAdd(d.tcount, 1);
+        d.tlist[pos] = ie;
+    }
+}
 
 __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict__ Uin)
 {
@@ -51,26 +98,8 @@ __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict
     const int nc = d.nc, ie0 = blockIdx.x * PREP_CELLS;
     const int cl = threadIdx.x >> 3, lane = threadIdx.x & 7, ie = ie0 + cl;
     if (ie < nc) {
-        double u0[6];
-        for (int id = 0; id < 6; ++id) u0[id] = UD(Uin, 0, id, ie);
-        double xp[9], yp[9];
-        load9(d, d.xs, d.ys, ie, xp, yp);
-        const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
-        const double g0 = d.G[18 * nc + ie], g1 = d.G[19 * nc + ie], g2 = d.G[20 * nc + ie];
-        for (int g = lane; g < 25; g += 8) {
-            const int i = g / 5, j = g - 5 * i;
-            const double r = PG5[i], s = PG5[j], wi = WG5[i] * WG5[j];
-            double J[4];
-            jac9(xp, yp, r, s, J);
-            const double djaco = wi * (J[0] * J[3] - J[2] * J[1]);
-            double b[6];
-            basis6(r, s, rc, sc, g0, g1, g2, b);
-            double un = 0.0, u1 = 0.0;
-            for (int id = 0; id < 6; ++id) un = un + u0[id] * b[id];
-            for (int id = 0; id < 3; ++id) u1 = u1 + u0[id] * b[id];
-            sv[cl][g] = un * un * djaco;
-            sv[cl][25 + g] = (u1 - un) * (u1 - un) * djaco;
-        }
+        const PrepCell c(d, Uin, ie);
+        for (int g = lane; g < 25; g += 8) c.point(g, sv[cl][g], sv[cl][25 + g]);
     }
     __syncthreads();
     if (ie >= nc || lane != 0) return;
@@ -80,13 +109,23 @@ __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict
         v2 = v2 + sv[cl][25 + g];
         v1 = v1 + sv[cl][g];
     }
-    const double thr = -3.0 * log10(4.0) - 4.0;
-    const bool troubled = !(log10(v2 / v1) <= thr) && run_active(d, cell_run(d, ie));
-    d.flag[ie] = troubled ? 1 : 0;
-    if (troubled) {
-        int pos = atomicAdd(d.tcount, 1);
-        d.tlist[pos] = ie;
+    prep_flag(d, ie, v1, v2);
+}
+
+__global__ void __launch_bounds__(256) k_q2_prep_tpc(Dev d, const double *__restrict__ Uin)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ie >= d.nc) return;
+    const PrepCell c(d, Uin, ie);
+    double v1 = 0.0, v2 = 0.0;
+#pragma unroll
+    for (int g = 0; g < 25; ++g) {
+        double a, b;
+        c.point(g, a, b);
+        v2 = v2 + b;
+        v1 = v1 + a;
     }
+    prep_flag(d, ie, v1, v2);
 }
 
 // ---------------------------------------------------------------- WENO --
@@ -2258,6 +2297,17 @@ int upload_extra(lag2d_ctx &x)
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
 }
 
+// Persson indicator work split: a thread per cell on meshes large enough to
+// fill the chip with it, else 8 lanes per cell.
+static void launch_prep(lag2d_ctx *x, Dev &ds, const double *Uin)
+{
+    const int nc = x->nc;
+    if (x->geom_cells)
+        hipLaunchKernelGGL(k_q2_prep_tpc, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
+    else
+        hipLaunchKernelGGL(k_q2_prep, dim3((nc + PREP_CELLS - 1) / PREP_CELLS), dim3(256), 0, x->stream, ds, Uin);
+}
+
 // Corner kernel variants (identical results), the autotuner's choices:
 //   1    16 lanes per cell, 16 cells per workgroup (short latency, small meshes)
 //   2-4  a lane per sub-cell, 64 / 32 / 16 cells per workgroup, the
@@ -2372,7 +2422,7 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
         return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
     }
     x->tic(KID_Q2_PREP);
-    hipLaunchKernelGGL(k_q2_prep, dim3((nc + PREP_CELLS - 1) / PREP_CELLS), dim3(256), 0, x->stream, ds, Uin);
+    launch_prep(x, ds, Uin);
     x->toc(KID_Q2_PREP);
     x->tic(KID_Q2_WENO);
     hipLaunchKernelGGL(k_q2_weno, dim3(nc < WENO_GRID ? nc : WENO_GRID), dim3(WENO_WG), 0, x->stream, ds, Uin);
@@ -2412,7 +2462,7 @@ static int launch_rhs_q2_adj_nd(lag2d_ctx *x, Dev &ds, const double *Uin, double
         if constexpr (ND == 3) {
             hipLaunchKernelGGL(k_c1_weno, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
         } else {
-            hipLaunchKernelGGL(k_q2_prep, dim3((nc + PREP_CELLS - 1) / PREP_CELLS), dim3(256), 0, x->stream, ds, Uin);
+            launch_prep(x, ds, Uin);
             hipLaunchKernelGGL(k_q2_weno, dim3(nc < WENO_GRID ? nc : WENO_GRID), dim3(WENO_WG), 0, x->stream, ds, Uin);
         }
         hipLaunchKernelGGL((k_q2_weno_frozen<ND, 1>), dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);

@@ -19,6 +19,15 @@ def _lag(case):
     return Lag2D(case)
 
 
+@pytest.fixture(params=[1, 2], ids=["geom-8lanes", "geom-tpc"])
+def geom(request, monkeypatch):
+    """The cell-volume work split (8 lanes per cell, or a thread per cell;
+    the library picks it by mesh size) fixed for the contexts a test
+    creates."""
+    monkeypatch.setenv("LAG2D_GEOM_VARIANT", str(request.param))
+    return request.param
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_rhs_matches_reference(name):
     case, z = load(name)
@@ -92,7 +101,7 @@ def test_hybrid_sod_total_energy_conserved():
     assert abs(e1 - e0) <= 1e-12 * abs(e0)
 
 
-def test_hybrid_matches_oracle_bitwise():
+def test_hybrid_matches_oracle_bitwise(geom):
     """Hybrid Sedov: 20 steps on the GPU and in the oracle (which matches the
     reference bit for bit) give the same bits."""
     from oracle import Oracle
@@ -106,7 +115,7 @@ def test_hybrid_matches_oracle_bitwise():
     assert np.array_equal(st["unkno"], o.unkno) and np.array_equal(st["coord"], o.coord)
 
 
-def test_rz_matches_oracle_bitwise():
+def test_rz_matches_oracle_bitwise(geom):
     """RZ hybrid Sedov: 25 steps on the GPU and in the oracle (bit-identical
     to the reference) give the same bits, geometry rows included."""
     from oracle import Oracle

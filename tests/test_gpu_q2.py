@@ -11,19 +11,21 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-# (corner, cell RHS) variants: lanes per cell / 8 cells per workgroup; a lane
-# per sub-cell with 64, 32, 16 cells per workgroup and the shape table in LDS,
-# or 64, 32 cells reading it through the caches / a thread per cell in
-# workgroups of 256, 128, 64
-VARIANT_PAIRS = [(1, 1), (2, 2), (3, 3), (4, 4), (5, 2), (6, 3)]
+# (corner, cell RHS, geometry) variants: lanes per cell / 8 cells per
+# workgroup; a lane per sub-cell with 64, 32, 16 cells per workgroup and the
+# shape table in LDS, or 64, 32 cells reading it through the caches / a
+# thread per cell in workgroups of 256, 128, 64; geometry (cell volume and
+# Persson indicator) 1 = 8 lanes per cell, 2 = a thread per cell
+VARIANT_PAIRS = [(1, 1, 1), (2, 2, 2), (3, 3, 1), (4, 4, 2), (5, 2, 1), (6, 3, 2)]
 
 
-@pytest.fixture(params=VARIANT_PAIRS, ids=[f"corner{c}-rhs{r}" for c, r in VARIANT_PAIRS])
+@pytest.fixture(params=VARIANT_PAIRS, ids=[f"corner{c}-rhs{r}-geom{g}" for c, r, g in VARIANT_PAIRS])
 def variant(request, monkeypatch):
-    """Kernel variants (corner and cell RHS; the library picks them by timing)
-    fixed for the contexts a test creates."""
+    """Kernel variants (corner and cell RHS, picked by timing; cell volume and
+    indicator, picked by mesh size) fixed for the contexts a test creates."""
     monkeypatch.setenv("LAG2D_CORNER_VARIANT", str(request.param[0]))
     monkeypatch.setenv("LAG2D_RHS_VARIANT", str(request.param[1]))
+    monkeypatch.setenv("LAG2D_GEOM_VARIANT", str(request.param[2]))
     return request.param
 
 
