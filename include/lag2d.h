@@ -255,8 +255,8 @@ int lag2d_use_graphs(lag2d_ctx *ctx, int enable);
  * doubles each); curved (nd = ndegr, ns = nstag): ns*((8 nd + 32)*ncell
  * doubles + 44*ncell bytes) per step, a checkpoint of 4 nd*ncell + 4*npoin
  * doubles per `every` steps, the stage data of `every` steps
- * (ns*((4 nd + 208)*ncell + 4*npoin) doubles each, ncell rounded up to a
- * multiple of 64 for the 160 record doubles).
+ * (ns*((4 nd + 128)*ncell + 4*npoin) doubles each, ncell rounded up to a
+ * multiple of 64 for the 80 corner-state doubles).
  *
  * lag2d_adj_replay: nsteps steps from the resident state with the recorded
  * switches and dt (upload a perturbed initial state first): the frozen

@@ -187,17 +187,15 @@ __global__ void __launch_bounds__(256) k_adjc_cellrhs(Dev d, AdjStageC a, double
             L[q] = s;
         }
         double nx[2], ny[2], ar[2], fx[2], fy[2], m11[2], sg[2], w[2], lfx[2], lfy[2];
+        const double uv = a.rec[RC2(ie, k, 2)], vv = a.rec[RC2(ie, k, 3)], pv = a.rec[RC2(ie, k, 4)];
         for (int f = 0; f < 2; ++f) {
             const int tt = FNQSG[isg][2 * iv + f];
-            const int sl = (tt > 0 ? tt : -tt) - 1;
             sg[f] = tt > 0 ? 1.0 : -1.0;
-            nx[f] = sg[f] * a.gq[(3 * sl + 0) * nc + ie];
-            ny[f] = sg[f] * a.gq[(3 * sl + 1) * nc + ie];
-            ar[f] = a.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
-            const int cf = f + 2 * k;
-            m11[f] = a.rec[RQ2(ie, cf, 0)];
-            const double m1 = a.rec[RQ2(ie, cf, 1)], m2 = a.rec[RQ2(ie, cf, 2)];
-            const double s1 = a.rec[RQ2(ie, cf, 3)], s2 = a.rec[RQ2(ie, cf, 4)];
+            double ga, m1, m2, s1, s2;
+            corner_face_geom(a.gq, nc, ie, isg, iv, f, nx[f], ny[f], ga);
+            ar[f] = ga * WFV[isg][iv][f];
+            m11[f] = a.rec[RC2(ie, k, f)];
+            corner_face_rec(m11[f], uv, vv, pv, nx[f], ny[f], ga * WFR[isg][iv][f], m1, m2, s1, s2);
             fx[f] = s1 + m11[f] * us1 + 0.0 * us2 - m1;
             fy[f] = s2 + 0.0 * us1 + m11[f] * us2 - m2;
             w[f] = WFM[isg][iv][f];
@@ -305,7 +303,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_adjc_node(Dev d, AdjStageC a)
         const int e = d.n2c[k], ie = e >> 4, slot = e & 15;
         lx += a.CU[(2 * slot + 0) * nc + ie];
         ly += a.CU[(2 * slot + 1) * nc + ie];
-        for (int f = 0; f < 2; ++f) m = m + a.rec[RQ2(ie, f + 2 * slot, 0)];
+        for (int f = 0; f < 2; ++f) m = m + a.rec[RC2(ie, slot, f)];
     }
     const uint8_t fl = d.nflag[p];
     if (fl & 2) lx = 0.0;

@@ -239,7 +239,7 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     uint8_t *t_fl, *t_col;
     const size_t nU = (size_t)4 * c.ndegr * nc;
     x->nblk_dt = nrun == 1 ? std::min(nblocks(nc), 1024) : 4;
-    if (q2) x->nrec = 160;
+    if (q2) x->nrec = Q2_REC;
     if ((rc = x->alloc(&t_ipq, ipq.size())) || (rc = x->alloc(&t_e1, e1.size())) ||
         (rc = x->alloc(&t_e2, e2.size())) || (rc = x->alloc(&t_n2c, n2c.size())) ||
         (rc = x->alloc(&t_n2co, n2c_off.size())) || (rc = x->alloc(&t_fl, (size_t)npoin)) ||
@@ -797,8 +797,8 @@ static bool adjc_supported(const lag2d_ctx *x)
 
 static size_t adjc_tape_doubles(const lag2d_ctx *x) { return (size_t)(8 * x->nd + 32) * x->nc; }
 static size_t adjc_tape_bytes(const lag2d_ctx *x) { return (size_t)44 * x->nc; }
-// records of the curved path: 160 per cell, in tiles of 64 cells
-static size_t adjc_rec_doubles(const lag2d_ctx *x) { return (size_t)160 * ((x->nc + 63) / 64 * 64); }
+// corner states of the curved path: Q2_REC per cell, in tiles of 64 cells
+static size_t adjc_rec_doubles(const lag2d_ctx *x) { return (size_t)Q2_REC * ((x->nc + 63) / 64 * 64); }
 static size_t adjc_stage_stride(const lag2d_ctx *x)
 {
     return (size_t)(4 * x->nd + 48) * x->nc + adjc_rec_doubles(x) + (size_t)4 * x->np;

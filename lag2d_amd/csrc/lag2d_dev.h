@@ -192,7 +192,7 @@ struct AdjStageC {
     const double *Ul;            // limited stage DOFs [4 nd][nc]
     const double *xin, *yin;     // stage input coordinates
     const double *usx, *usy;     // this stage's nodal velocity
-    const double *rec;           // this stage's corner-face records (RQ2 layout)
+    const double *rec;           // this stage's sub-cell corner states (RC2 layout)
     const double *gq;            // this stage's face slots [48][nc]
     const double *dlt;           // frozen |du.n| [32][nc]
     const uint8_t *ccl, *dcl;    // frozen clamp branches [18][nc], [25][nc]

@@ -945,14 +945,16 @@ __global__ void __launch_bounds__(WENO_WG) k_q2_weno(Dev d, const double *__rest
 }
 
 // ------------------------------------------------------------- corners --
-// Face geometry (stored for the face integral), sub-cell density and mean
-// specific volume, vertex states and the 32 corner-face Riemann records.
-// Records: rec[RQ2(ie, cf, r)], cf = f + 2*(ivsg + 4*isg),
-// r = 0: m = mu*A, 1-2: m u_c, 3-4: A sigma n.
+// Face geometry (stored for the face integral and the node solve), sub-cell
+// density and mean specific volume, and the 16 sub-cell corner states.
+// Corner states: rec[RC2(ie, ivsg + 4*isg, r)], r = 0-1: m = mu*A of the
+// corner's two faces, 2-3: u_c, 4: p_c; the node solve and the face
+// integral rebuild the corner-face records m u_c and A sigma n from them
+// and the face geometry (corner_face_rec).
 // 16 lanes per cell: first lane k evaluates face slot k and a quarter of
 // the 25 quadrature points of sub-cell k/4 (terms staged in LDS), then lane
-// (isg, ivsg) sums its sub-cell's terms in point order and writes the two
-// records of its sub-cell corner.
+// (isg, ivsg) sums its sub-cell's terms in point order and writes its
+// sub-cell corner's state.
 constexpr int CORNER_CPB = 16;
 
 __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
@@ -1140,6 +1142,9 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
     const double pv = dmax(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)));
     const double dux = d.usx[p] - uv, duy = d.usy[p] - vv;
     const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+    d.rec[RC2(ie, iv + 4 * isg, 2)] = uv;
+    d.rec[RC2(ie, iv + 4 * isg, 3)] = vv;
+    d.rec[RC2(ie, iv + 4 * isg, 4)] = pv;
     for (int f = 0; f < 2; ++f) {
         const int tt = FNQSG[isg][2 * iv + f];
         const int sl = (tt > 0 ? tt : -tt) - 1;
@@ -1155,12 +1160,7 @@ __global__ void __launch_bounds__(256) k_q2_corner(Dev d)
             const double nrm = sqrt(dux * dux + duy * duy);
             m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
         }
-        const int cf = f + 2 * (iv + 4 * isg);
-        d.rec[RQ2(ie, cf, 0)] = m11;
-        d.rec[RQ2(ie, cf, 1)] = m11 * uv + 0.0 * vv;
-        d.rec[RQ2(ie, cf, 2)] = 0.0 * uv + m11 * vv;
-        d.rec[RQ2(ie, cf, 3)] = -pv * area * nx + 0.0 * area * ny;
-        d.rec[RQ2(ie, cf, 4)] = 0.0 * area * nx + -pv * area * ny;
+        d.rec[RC2(ie, iv + 4 * isg, f)] = m11;
     }
 }
 
@@ -1364,6 +1364,9 @@ __global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
             clamp_eps<MODE>(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)), d.ccl, (size_t)(2 + 4 * isg + iv) * nc + ie);
         const double dux = d.usx[p] - uv, duy = d.usy[p] - vv;
         const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+        d.rec[RC2(ie, iv + 4 * isg, 2)] = uv;
+        d.rec[RC2(ie, iv + 4 * isg, 3)] = vv;
+        d.rec[RC2(ie, iv + 4 * isg, 4)] = pv;
         for (int f = 0; f < 2; ++f) {
             const int tt = FNQSG[isg][2 * iv + f];
             const int sl = (tt > 0 ? tt : -tt) - 1;
@@ -1386,12 +1389,7 @@ __global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
                 const double nrm = sqrt(dux * dux + duy * duy);
                 m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
             }
-            const int cf = f + 2 * (iv + 4 * isg);
-            d.rec[RQ2(ie, cf, 0)] = m11;
-            d.rec[RQ2(ie, cf, 1)] = m11 * uv + 0.0 * vv;
-            d.rec[RQ2(ie, cf, 2)] = 0.0 * uv + m11 * vv;
-            d.rec[RQ2(ie, cf, 3)] = -pv * area * nx + 0.0 * area * ny;
-            d.rec[RQ2(ie, cf, 4)] = 0.0 * area * nx + -pv * area * ny;
+            d.rec[RC2(ie, iv + 4 * isg, f)] = m11;
         }
     }
 }
@@ -1408,18 +1406,21 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
     double a11 = 0.0, a21 = 0.0, a12 = 0.0, a22 = 0.0, mu1 = 0.0, mu2 = 0.0, sn1 = 0.0, sn2 = 0.0;
     for (int k = d.n2c_off[p]; k < d.n2c_off[p + 1]; ++k) {
         const int e = d.n2c[k];
-        const int ie = e >> 4, slot = e & 15;
+        const int ie = e >> 4, slot = e & 15, isg = slot >> 2, iv = slot & 3;
+        const double uv = d.rec[RC2(ie, slot, 2)], vv = d.rec[RC2(ie, slot, 3)], pv = d.rec[RC2(ie, slot, 4)];
         for (int f = 0; f < 2; ++f) {
-            const int cf = f + 2 * slot;
-            double m = d.rec[RQ2(ie, cf, 0)];
+            double nx, ny, ga, m1, m2, s1, s2;
+            corner_face_geom(d.gq, nc, ie, isg, iv, f, nx, ny, ga);
+            const double m = d.rec[RC2(ie, slot, f)];
+            corner_face_rec(m, uv, vv, pv, nx, ny, ga * WFR[isg][iv][f], m1, m2, s1, s2);
             a11 = a11 + m;
             a21 = a21 + 0.0;
             a12 = a12 + 0.0;
             a22 = a22 + m;
-            mu1 = mu1 + d.rec[RQ2(ie, cf, 1)];
-            mu2 = mu2 + d.rec[RQ2(ie, cf, 2)];
-            sn1 = sn1 + d.rec[RQ2(ie, cf, 3)];
-            sn2 = sn2 + d.rec[RQ2(ie, cf, 4)];
+            mu1 = mu1 + m1;
+            mu2 = mu2 + m2;
+            sn1 = sn1 + s1;
+            sn2 = sn2 + s2;
         }
     }
     const uint8_t fl = d.nflag[p];
@@ -1502,17 +1503,14 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             const int p = d.ipq[ln * nc + ie];
             const double us1 = d.usx[p], us2 = d.usy[p];
             double nx[2], ny[2], ar[2], fx[2], fy[2];
+            const double uv = d.rec[RC2(ie, corner, 2)], vv = d.rec[RC2(ie, corner, 3)];
+            const double pv = d.rec[RC2(ie, corner, 4)];
             for (int f = 0; f < 2; ++f) {
-                const int tt = FNQSG[isg][2 * iv + f];
-                const int sl = (tt > 0 ? tt : -tt) - 1;
-                const double sg = tt > 0 ? 1.0 : -1.0;
-                nx[f] = sg * d.gq[(3 * sl + 0) * nc + ie];
-                ny[f] = sg * d.gq[(3 * sl + 1) * nc + ie];
-                ar[f] = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
-                const int cf = f + 2 * (iv + 4 * isg);
-                const double m11 = d.rec[RQ2(ie, cf, 0)];
-                const double m1 = d.rec[RQ2(ie, cf, 1)], m2 = d.rec[RQ2(ie, cf, 2)];
-                const double s1 = d.rec[RQ2(ie, cf, 3)], s2 = d.rec[RQ2(ie, cf, 4)];
+                double ga, m1, m2, s1, s2;
+                corner_face_geom(d.gq, nc, ie, isg, iv, f, nx[f], ny[f], ga);
+                ar[f] = ga * WFV[isg][iv][f];
+                const double m11 = d.rec[RC2(ie, corner, f)];
+                corner_face_rec(m11, uv, vv, pv, nx[f], ny[f], ga * WFR[isg][iv][f], m1, m2, s1, s2);
                 fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
                 fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
             }
@@ -1692,17 +1690,13 @@ __global__ void __launch_bounds__(BS) k_cellrhs_q2_tpc(Dev d, double alfa, int a
         const int p = d.ipq[ln * nc + ie];
         const double us1 = d.usx[p], us2 = d.usy[p];
         double nx[2], ny[2], ar[2], fx[2], fy[2];
+        const double uv = d.rec[RC2(ie, k, 2)], vv = d.rec[RC2(ie, k, 3)], pv = d.rec[RC2(ie, k, 4)];
         for (int f = 0; f < 2; ++f) {
-            const int tt = FNQSG[isg][2 * iv + f];
-            const int sl = (tt > 0 ? tt : -tt) - 1;
-            const double sg = tt > 0 ? 1.0 : -1.0;
-            nx[f] = sg * d.gq[(3 * sl + 0) * nc + ie];
-            ny[f] = sg * d.gq[(3 * sl + 1) * nc + ie];
-            ar[f] = d.gq[(3 * sl + 2) * nc + ie] * WFV[isg][iv][f];
-            const int cf = f + 2 * k;
-            const double m11 = d.rec[RQ2(ie, cf, 0)];
-            const double m1 = d.rec[RQ2(ie, cf, 1)], m2 = d.rec[RQ2(ie, cf, 2)];
-            const double s1 = d.rec[RQ2(ie, cf, 3)], s2 = d.rec[RQ2(ie, cf, 4)];
+            double ga, m1, m2, s1, s2;
+            corner_face_geom(d.gq, nc, ie, isg, iv, f, nx[f], ny[f], ga);
+            ar[f] = ga * WFV[isg][iv][f];
+            const double m11 = d.rec[RC2(ie, k, f)];
+            corner_face_rec(m11, uv, vv, pv, nx[f], ny[f], ga * WFR[isg][iv][f], m1, m2, s1, s2);
             fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
             fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
         }

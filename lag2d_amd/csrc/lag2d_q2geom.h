@@ -7,12 +7,15 @@
 
 namespace lag2d {
 
-// Corner-face records in tiles of 64 cells: record r of corner face cf of
-// cell ie at ((ie / 64) 160 + 5 cf + r) 64 + ie % 64.  The 64 cells of a
-// wave read or write each record as 512 contiguous bytes (the thread-per-
-// cell kernels), and a node's corners in neighbouring cells share the
-// tile's lines (the node gather).  The array holds Q2_REC_CELLS(nc) cells.
-#define RQ2(ie, cf, r) (((((size_t)(ie)) >> 6) * 160 + (size_t)(cf) * 5 + (r)) * 64 + (((size_t)(ie)) & 63))
+// Compact sub-cell corner states in tiles of 64 cells: entry r of corner
+// k = 4 isg + iv of cell ie at ((ie / 64) 80 + 5 k + r) 64 + ie % 64, with
+// r = 0, 1 the impedance times area m = mu A of the corner's faces f = 0, 1
+// and r = 2, 3, 4 the corner's velocity u_c, v_c and pressure p_c.  The
+// 64 cells of a wave read or write each entry as 512 contiguous bytes (the
+// thread-per-cell kernels), and a node's corners in neighbouring cells
+// share the tile's lines (the node gather).  The array holds
+// Q2_REC_CELLS(nc) cells, Q2_REC doubles each.
+#define RC2(ie, k, r) (((((size_t)(ie)) >> 6) * 80 + (size_t)(k) * 5 + (r)) * 64 + (((size_t)(ie)) & 63))
 #define Q2_REC_CELLS(nc) ((((size_t)(nc)) + 63) / 64 * 64)
 
 // sub-cell vertex -> local Q2 node (ipqsg)
@@ -143,6 +146,34 @@ __device__ __forceinline__ void basis6(double r, double s, double rc, double sc,
     b[3] = 0.5 * b[1] * b[1] - g0;
     b[4] = 0.5 * b[2] * b[2] - g1;
     b[5] = b[1] * b[2] - g2;
+}
+
+// Face f of sub-cell corner (isg, iv): its slot's outward unit normal
+// (signed by the slot orientation) and length from the face-geometry rows
+// gq[(3 slot + c) nc + ie] the corner solve stores.
+__device__ __forceinline__ void corner_face_geom(const double *__restrict__ gq, int nc, int ie, int isg, int iv,
+                                                 int f, double &nx, double &ny, double &ga)
+{
+    const int tt = FNQSG[isg][2 * iv + f];
+    const int sl = (tt > 0 ? tt : -tt) - 1;
+    const double sg = tt > 0 ? 1.0 : -1.0;
+    nx = sg * gq[(3 * sl + 0) * nc + ie];
+    ny = sg * gq[(3 * sl + 1) * nc + ie];
+    ga = gq[(3 * sl + 2) * nc + ie];
+}
+
+// The Riemann record of a corner face (getriem_quadsubg2,
+// Getrhslaggauss.f90:10639-10858): m u_c and A sigma n (A = the face length
+// times the corner weight), with the reference's expressions, from the
+// compact corner state; the node solve and the face integral rebuild them
+// exactly as the corner solve formed them.
+__device__ __forceinline__ void corner_face_rec(double m11, double uv, double vv, double pv, double nx, double ny,
+                                                double area, double &m1, double &m2, double &s1, double &s2)
+{
+    m1 = m11 * uv + 0.0 * vv;
+    m2 = 0.0 * uv + m11 * vv;
+    s1 = -pv * area * nx + 0.0 * area * ny;
+    s2 = 0.0 * area * nx + -pv * area * ny;
 }
 
 __device__ __forceinline__ void load9(const Dev &d, const double *x, const double *y, int ie, double *xp, double *yp)

@@ -43,9 +43,9 @@ _NODE_SHARE = 4.03  # nodes per cell of a Q2 quad mesh (58081 / 14400)
 BYTES_Q2 = {
     "k_q2_prep": 8 * (24 + 24 + 5) + 4 * 9 + 16 * _NODE_SHARE + 1,
     "k_q2_weno": 9 * (8 * (24 + 7) + 4 * 9 + 32 * 9) + 8 * 20,
-    "k_q2_corner": 8 * (24 + 5 + 4 + 48 + 160) + 4 * 9 + 32 * _NODE_SHARE,
-    "k_node_q2": 8 * 2 * 5 * 4 + 4 * 5 + 1 + 32,
-    "k_cellrhs_q2": 8 * (24 + 160 + 48 + 5 + 16 + 24 + 24) + 4 * 9 + 32 * _NODE_SHARE,
+    "k_q2_corner": 8 * (24 + 5 + 4 + 48 + 80) + 4 * 9 + 32 * _NODE_SHARE,
+    "k_node_q2": 8 * (4 * 5 + 48 / _NODE_SHARE) + 4 * 5 + 1 + 32,
+    "k_cellrhs_q2": 8 * (24 + 80 + 48 + 5 + 16 + 24 + 24) + 4 * 9 + 32 * _NODE_SHARE,
     "k_update": 8 * (16 + 24 * 4),
     "k_nodemove": 8 * 8,
     "k_cellvol": 8 * 3 + 4 * 9 + 16 * _NODE_SHARE + 2,
