@@ -368,7 +368,7 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         if (k >= 1 && k <= LAG2D_RHS_VARIANTS) x->rhs_variant = k;
     }
     if (q2 && c.npoly == 1) {  // curved P1 has one corner and one cell-RHS form
-        x->corner_variant = 2;
+        x->corner_variant = 8;
         x->rhs_variant = 2;
     }
     *out = x;
@@ -562,6 +562,7 @@ static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.33333333333333
 // cell-RHS slots name the variant the autotuner kept.
 static const char *kernel_name(const lag2d_ctx *x, int k)
 {
+    if (k == KID_Q2_CORNER && x->corner_variant >= 7) return "k_q2_corner_wu";
     if (k == KID_Q2_CORNER && x->corner_variant >= 2) return "k_q2_corner_sc";
     if (k == KID_CELLRHS_Q2 && x->rhs_variant >= 2) return "k_cellrhs_q2_tpc";
     if (k == KID_Q2_WENO && x->nd == 3) return "k_c1_weno";

@@ -1394,6 +1394,227 @@ __global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
     }
 }
 
+// The corner kernel with one wave per sub-cell: wave isg of a 256-thread
+// workgroup evaluates sub-cell isg of the workgroup's 64 cells, a lane per
+// cell.  The sub-cell index is uniform across the wave, so
+//   - the face slots 4 isg .. 4 isg + 3 and the sub-cell's nodes come from
+//     code specialised per sub-cell (corner_wu_faces<ISG>), where the shape
+//     functions at the fixed reference points are constants;
+//   - the quadrature loop reads the shape-function rows of its points as
+//     wave-uniform scalar loads (the table in the constant address space),
+//     which the FP64 instructions take as scalar operands;
+//   - every per-corner and per-face index (IPQSG, FNQSG, WFR, the corner
+//     state and tape offsets) is uniform, and each wave reads and writes
+//     its 64 cells' entries as contiguous runs.
+// Every sum runs in the order of k_q2_corner_sc: identical results.
+typedef __attribute__((address_space(4))) const double kdouble;
+
+__device__ __forceinline__ void jac9_k(const double *xp, const double *yp, const kdouble *t, double J[4])
+{
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 9; ++k) {
+        dxdr = dxdr + t[k] * xp[k];
+        dxds = dxds + t[9 + k] * xp[k];
+        dydr = dydr + t[k] * yp[k];
+        dyds = dyds + t[9 + k] * yp[k];
+    }
+    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
+}
+
+constexpr int CWU_CELLS = 64;
+
+// Face slots 4 ISG .. 4 ISG + 3 of the cell (8-11 interior half-edges, the
+// others boundary points) into sgq and gq, and the nodes of sub-cell ISG.
+template <int ISG>
+__device__ __forceinline__ void corner_wu_faces(const Dev &d, int ie, int cl, const double *xp, const double *yp,
+                                                double (*sgq)[CWU_CELLS], double *xs, double *ys)
+{
+    const int nc = d.nc;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int slot = 4 * ISG + u;
+        double gx, gy, ga;
+        if (slot >= 8 && slot < 12) {
+            const int k = slot - 8;
+            const double pq[4][2] = {{0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
+            double xv, yv;
+            map9(xp, yp, pq[k][0], pq[k][1], &xv, &yv);
+            const double fx[3] = {xp[4 + k], xp[8], xv};
+            const double fy[3] = {yp[4 + k], yp[8], yv};
+            const double r = -1.0;
+            const double dd[3] = {-0.5 + r, 0.5 + r, -2.0 * r};
+            double dxdr = 0.0, dydr = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                dxdr = dxdr + dd[i] * fx[i];
+                dydr = dydr + dd[i] * fy[i];
+            }
+            const double djaco = sqrt(dxdr * dxdr + dydr * dydr);
+            const double anx = -dydr, any = dxdr;
+            gx = anx / sqrt(anx * anx + any * any);
+            gy = any / sqrt(anx * anx + any * any);
+            ga = djaco * 2.0;
+        } else {
+            const int ig = slot < 8 ? slot : slot - 4;
+            const double pi[12][2] = {{-1, -1}, {1, -1}, {1, -1}, {1, 1}, {1, 1}, {-1, 1},
+                                      {-1, 1},  {-1, -1}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
+            const double vn[12][2] = {{0, -1}, {0, -1}, {1, 0}, {1, 0}, {0, 1}, {0, 1},
+                                      {-1, 0}, {-1, 0}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
+            double J[4];
+            jac9(xp, yp, pi[ig][0], pi[ig][1], J);
+            const double c11 = J[3], c12 = -J[2], c21 = -J[1], c22 = J[0];
+            const double anx = c11 * vn[ig][0] + c12 * vn[ig][1];
+            const double any = c21 * vn[ig][0] + c22 * vn[ig][1];
+            const double w1 = anx * 2.0, w2 = any * 2.0;
+            const double farea = sqrt(w1 * w1 + w2 * w2);
+            gx = anx * 2.0 / farea;
+            gy = any * 2.0 / farea;
+            ga = farea;
+        }
+        sgq[3 * slot + 0][cl] = gx;
+        sgq[3 * slot + 1][cl] = gy;
+        sgq[3 * slot + 2][cl] = ga;
+        d.gq[(3 * slot + 0) * nc + ie] = gx;
+        d.gq[(3 * slot + 1) * nc + ie] = gy;
+        d.gq[(3 * slot + 2) * nc + ie] = ga;
+    }
+    const double pc[12][2] = {{-0.5, -1.0}, {0.5, -1.0}, {1.0, -.5}, {1.0, .5}, {0.5, 1.0}, {-0.5, 1.0},
+                              {-1.0, .5},   {-1.0, -.5}, {0.0, -.5}, {0.5, 0.0}, {0.0, .5}, {-0.5, 0.0}};
+    const int ipsc[4][4] = {{0, 8, 11, 7}, {1, 2, 9, 8}, {9, 3, 4, 10}, {11, 10, 5, 6}};
+    const int ipqsg[4][4] = {{0, 4, 8, 7}, {4, 1, 5, 8}, {8, 5, 2, 6}, {7, 8, 6, 3}};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        xs[k] = xp[ipqsg[ISG][k]];
+        ys[k] = yp[ipqsg[ISG][k]];
+        const int c = ipsc[ISG][k];
+        map9(xp, yp, pc[c][0], pc[c][1], &xs[4 + k], &ys[4 + k]);
+    }
+    xs[8] = -0.25 * (xs[0] + xs[1] + xs[2] + xs[3]) + 0.5 * (xs[4] + xs[5] + xs[6] + xs[7]);
+    ys[8] = -0.25 * (ys[0] + ys[1] + ys[2] + ys[3]) + 0.5 * (ys[4] + ys[5] + ys[6] + ys[7]);
+}
+
+// WPE: the fewest waves per SIMD the register allocation must allow (1 no
+// limit, 204 VGPRs; 3 caps it at 168 VGPRs, without spills).
+template <int ND, int MODE = 0, int WPE = 1>
+__global__ void __launch_bounds__(4 * CWU_CELLS) __attribute__((amdgpu_waves_per_eu(WPE))) k_q2_corner_wu(Dev d)
+{
+    __shared__ double sgq[16 * 3][CWU_CELLS];
+    const int cl = threadIdx.x & (CWU_CELLS - 1);
+    const int isg = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ie = blockIdx.x * CWU_CELLS + cl;
+    const int nc = d.nc;
+    const bool active = ie < nc;
+    const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
+    double xp[9], yp[9], xs[9], ys[9];
+    double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
+    if (active) {
+        load9(d, d.xs, d.ys, ie, xp, yp);
+        rc = d.G[0 * nc + ie];
+        sc = d.G[1 * nc + ie];
+        gg0 = d.G[18 * nc + ie];
+        gg1 = d.G[19 * nc + ie];
+        gg2 = d.G[20 * nc + ie];
+        switch (isg) {
+        case 0: corner_wu_faces<0>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        case 1: corner_wu_faces<1>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        case 2: corner_wu_faces<2>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        default: corner_wu_faces<3>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        }
+    }
+    __syncthreads();  // face slots shared
+    if (!active) return;
+    // ---- sub-cell isg: area of its Q2 map and the mass-weighted mean
+    //      specific volume of the DG polynomial, summed in point order
+    double a = 0.0, masel = 0.0, volel = 0.0;
+    {
+        double u0[ND];
+        for (int id = 0; id < ND; ++id) u0[id] = UDN(cell_dofs(d, ie), 0, id, ie);
+        const double sr = (isg == 1 || isg == 2) ? 1.0 : -1.0, ss = isg >= 2 ? 1.0 : -1.0;
+        const double rhomc = u0[0];
+        const kdouble *stab = (const kdouble *)d.shp;
+#pragma unroll 1
+        for (int g = 0; g < 25; ++g) {
+            const int i = g / 5, j = g - 5 * i;
+            double J[4];
+            jac9_k(xs, ys, stab + g * SHP_STRIDE, J);
+            a = a + WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
+            const double r = (PG5[i] + sr) / 2.0;
+            const double s = (PG5[j] + ss) / 2.0;
+            double b[6];
+            basis6(r, s, rc, sc, gg0, gg1, gg2, b);
+            double t = 0.0;
+            for (int id = 0; id < ND; ++id) t = t + u0[id] * b[id];
+            t = rhomc + 1.0 * (t - rhomc);
+            jac9_k(xp, yp, stab + (25 + 25 * isg + g) * SHP_STRIDE, J);
+            const double djaco = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]) * 0.25;
+            masel = masel + 1.0 / t * djaco;
+            volel = volel + djaco;
+        }
+    }
+    const double rsub = 0.0 + (1.0 / a) * d.msub[isg * nc + ie];
+    const double uq = volel / masel;
+    double u[4][ND];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < ND; ++id) u[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
+    const double rhomc = u[0][0], uctr = u[1][0], vctr = u[2][0], ectr = u[3][0];
+    const double rhoct = 1.0 / rhomc;
+    // the centre's clamps are the same for the 4 sub-cells: wave 0 records
+    uint8_t cdummy[2];
+    uint8_t *ctape = (MODE == 1 && isg != 0) ? cdummy : d.ccl;
+    const size_t co = (MODE == 1 && isg != 0) ? 0 : (size_t)ie;
+    const size_t cs = (MODE == 1 && isg != 0) ? 1 : (size_t)nc;
+    const double pctr = clamp_eps<MODE>(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)), ctape, co);
+    const double sdctr = sqrt(clamp_eps<MODE>(eps, d.gamlg * pctr / rhoct, ctape, co + cs));
+    const double dcor = d.cdrho * (rsub - 1.0 / uq);
+#pragma unroll 1
+    for (int iv = 0; iv < 4; ++iv) {
+        // ---- sub-cell corner (isg, iv)
+        const int ln = IPQSG[isg][iv];
+        double un[4];
+        {
+            double b[6];
+            basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, b);
+            for (int q = 0; q < 4; ++q) un[q] = 0.0;
+            for (int id = 0; id < ND; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
+        }
+        const int p = d.ipq[ln * nc + ie];
+        const double rhovt = 1.0 / un[0];
+        const double rhovsg = rhovt + dcor;
+        const double uv = un[1], vv = un[2], ev = un[3];
+        const double pv =
+            clamp_eps<MODE>(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)), d.ccl, (size_t)(2 + 4 * isg + iv) * nc + ie);
+        const double dux = d.usx[p] - uv, duy = d.usy[p] - vv;
+        const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+        d.rec[RC2(ie, iv + 4 * isg, 2)] = uv;
+        d.rec[RC2(ie, iv + 4 * isg, 3)] = vv;
+        d.rec[RC2(ie, iv + 4 * isg, 4)] = pv;
+        for (int f = 0; f < 2; ++f) {
+            const int tt = FNQSG[isg][2 * iv + f];
+            const int sl = (tt > 0 ? tt : -tt) - 1;
+            const double sg = tt > 0 ? 1.0 : -1.0;
+            const double nx = sg * sgq[3 * sl + 0][cl], ny = sg * sgq[3 * sl + 1][cl];
+            const double area = sgq[3 * sl + 2][cl] * WFR[isg][iv][f];
+            double deltu;
+            const size_t od = (size_t)(f + 2 * (iv + 4 * isg)) * nc + ie;
+            if constexpr (MODE == 2) {
+                deltu = d.dlt[od];
+            } else {
+                deltu = fabs(dux * nx + duy * ny);
+                if constexpr (MODE == 1) d.dlt[od] = deltu;
+            }
+            const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
+            double m11;
+            if (a3 < 1.e12) {
+                m11 = mur * area * fabs(1.0);
+            } else {
+                const double nrm = sqrt(dux * dux + duy * duy);
+                m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
+            }
+            d.rec[RC2(ie, iv + 4 * isg, f)] = m11;
+        }
+    }
+}
+
 // Nodal solve over the sub-cell corners: contributions gathered in the
 // reference's accumulation order (cell, sub-cell, vertex, face); walls on
 // all three face nodes (Getrhslaggauss.f90:4494-4545).
@@ -2308,6 +2529,9 @@ static void launch_prep(lag2d_ctx *x, Dev &ds, const double *Uin)
 //        shape-function table staged in LDS (throughput, large meshes)
 //   5-6  a lane per sub-cell, 64 / 32 cells per workgroup, the table read
 //        through the caches (no LDS staging)
+//   7-8  a wave per sub-cell, 64 cells per workgroup (uniform sub-cell
+//        index: specialised face code, the table as scalar operands), with
+//        registers for 2 or 3 waves per SIMD
 static void launch_corner(lag2d_ctx *x, Dev &ds, int variant)
 {
     const int nc = x->nc;
@@ -2318,6 +2542,8 @@ static void launch_corner(lag2d_ctx *x, Dev &ds, int variant)
     case 4: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 16, 1>), g(16), dim3(64), 0, x->stream, ds); break;
     case 5: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 64, 0>), g(64), dim3(256), 0, x->stream, ds); break;
     case 6: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 32, 0>), g(32), dim3(128), 0, x->stream, ds); break;
+    case 7: hipLaunchKernelGGL((k_q2_corner_wu<6, 0, 1>), g(CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds); break;
+    case 8: hipLaunchKernelGGL((k_q2_corner_wu<6, 0, 3>), g(CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds); break;
     default: hipLaunchKernelGGL(k_q2_corner, g(CORNER_CPB), dim3(256), 0, x->stream, ds); break;
     }
 }
@@ -2405,7 +2631,7 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
         hipLaunchKernelGGL(k_c1_weno, dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
         x->toc(KID_Q2_WENO);
         x->tic(KID_Q2_CORNER);
-        hipLaunchKernelGGL(k_q2_corner_sc<3>, dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
+        hipLaunchKernelGGL((k_q2_corner_wu<3, 0, 3>), dim3((nc + CWU_CELLS - 1) / CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds);
         x->toc(KID_Q2_CORNER);
         x->tic(KID_NODE_Q2);
         hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
@@ -2445,7 +2671,7 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
 // replayed (record = 0) forward on the curved path: the plain WENO (record
 // only) and its frozen increment, then the corner (recording or reading the
 // |du.n| and clamp tapes in ds.dlt, ds.ccl, ds.dcl), the nodal solve and
-// the cell RHS with the fused update.  The one-lane-per-sub-cell corner and
+// the cell RHS with the fused update.  The wave-per-sub-cell corner and
 // thread-per-cell RHS variants run (their results equal the others').
 template <int ND>
 static int launch_rhs_q2_adj_nd(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int apply_update, int record)
@@ -2460,12 +2686,12 @@ static int launch_rhs_q2_adj_nd(lag2d_ctx *x, Dev &ds, const double *Uin, double
             hipLaunchKernelGGL(k_q2_weno, dim3(nc < WENO_GRID ? nc : WENO_GRID), dim3(WENO_WG), 0, x->stream, ds, Uin);
         }
         hipLaunchKernelGGL((k_q2_weno_frozen<ND, 1>), dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
-        hipLaunchKernelGGL((k_q2_corner_sc<ND, 1>), dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
+        hipLaunchKernelGGL((k_q2_corner_wu<ND, 1, 3>), dim3((nc + CWU_CELLS - 1) / CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds);
         hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
         hipLaunchKernelGGL((k_cellrhs_q2_tpc<ND, 1>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
     } else {
         hipLaunchKernelGGL((k_q2_weno_frozen<ND, 0>), dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
-        hipLaunchKernelGGL((k_q2_corner_sc<ND, 2>), dim3((nc + CSC_CELLS - 1) / CSC_CELLS), dim3(256), 0, x->stream, ds);
+        hipLaunchKernelGGL((k_q2_corner_wu<ND, 2, 3>), dim3((nc + CWU_CELLS - 1) / CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds);
         hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
         hipLaunchKernelGGL((k_cellrhs_q2_tpc<ND, 2>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
     }
