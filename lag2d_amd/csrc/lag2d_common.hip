@@ -519,6 +519,11 @@ __global__ void __launch_bounds__(256) k_cellvol_tpc(Dev d, double *__restrict__
         double yg = 0.0;
         const double djaco = cellvol_point(xp, yp, g, RZ ? &yg : nullptr);
         acc.add(d, g, djaco, yg);
+        // the nodes pass through an empty asm statement after each group of
+        // points, so the compiler keeps no products of them from one group to
+        // the next, which bounds the registers the unrolled loop holds
+        if (g % CV_GROUP == CV_GROUP - 1)
+            for (int k = 0; k < 9; ++k) asm volatile("" : "+v"(xp[k]), "+v"(yp[k]));
     }
     acc.store(d, ie);
 }

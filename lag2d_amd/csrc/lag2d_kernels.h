@@ -41,6 +41,7 @@ __global__ void k_adj_gather(Dev d, AdjStage a);
 __global__ void k_adj_step_end(Dev d, AdjStage a);
 __global__ void k_replay_dt(Dev d, const double *dt);
 constexpr int CV_CELLS = 32;  // cells per block of k_cellvol
+constexpr int CV_GROUP = 6;    // points scheduled together in k_cellvol_tpc
 constexpr int Q2_REC = 80;    // curved path: sub-cell corner-state doubles per cell (16 corners x 5)
 constexpr int GEOM_CELLS_MIN = 1 << 16;  // cells from which cell volume and prep run a thread per cell
 

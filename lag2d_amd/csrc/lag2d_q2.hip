@@ -47,6 +47,7 @@ __device__ __forceinline__ const double *cell_dofs(const Dev &d, int ie)
 // to fill the chip with a thread per cell; k_q2_prep_tpc, a thread per cell
 // whose unrolled point loop turns the shape functions into constants.
 constexpr int PREP_CELLS = 32;
+constexpr int PREP_GROUP = 5;  // points scheduled together in k_q2_prep_tpc
 
 // The cell's data the indicator reads.
 struct PrepCell {
@@ -61,6 +62,11 @@ struct PrepCell {
         g0 = d.G[18 * nc + ie];
         g1 = d.G[19 * nc + ie];
         g2 = d.G[20 * nc + ie];
+    }
+    __device__ __forceinline__ void opaque()
+    {
+        for (int k = 0; k < 9; ++k) asm volatile("" : "+v"(xp[k]), "+v"(yp[k]));
+        asm volatile("" : "+v"(rc), "+v"(sc));
     }
     // the two integrands at 5x5 point g: |u|^2 and |u - u_P1|^2 times the
     // weighted Jacobian
@@ -116,7 +122,7 @@ __global__ void __launch_bounds__(256) k_q2_prep_tpc(Dev d, const double *__rest
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     if (ie >= d.nc) return;
-    const PrepCell c(d, Uin, ie);
+    PrepCell c(d, Uin, ie);
     double v1 = 0.0, v2 = 0.0;
 #pragma unroll
     for (int g = 0; g < 25; ++g) {
@@ -124,6 +130,11 @@ __global__ void __launch_bounds__(256) k_q2_prep_tpc(Dev d, const double *__rest
         c.point(g, a, b);
         v2 = v2 + b;
         v1 = v1 + a;
+        // the node coordinates pass through an empty asm statement after each
+        // group of points, so the compiler keeps no products of them from
+        // one group to the next: left alone it shares them across all 25
+        // points and needs 428 registers
+        if (g % PREP_GROUP == PREP_GROUP - 1) c.opaque();
     }
     prep_flag(d, ie, v1, v2);
 }
@@ -1887,8 +1898,12 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
 // the shape-function derivatives are read from the table as wave-uniform
 // loads.  Every residual entry accumulates in the same order as in
 // k_cellrhs_q2: identical results.
+// Registers are capped for 2 waves per SIMD: uncapped, the P2 form takes
+// 266 (one wave per SIMD), and the 44 bytes it then spills cost far less
+// than the lost latency hiding.
 template <int ND, int MODE = 0, int BS = 256>
-__global__ void __launch_bounds__(BS) k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
+k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc;

@@ -48,14 +48,14 @@ __constant__ static const double WFV[4][4][2] = {
     {{0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}},
     {{0.25 * (4.0 / 3.0), 0.0}, {0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}}};
 // 5x5 Gauss rule of ruqope (dgflo2d.f90:2734-2743 ordering)
-__constant__ static const double PG5[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
+static constexpr double PG5[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
                                            0.5384693101056830910363144, 0.9061798459386639927976269};
-__constant__ static const double WG5[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
+static constexpr double WG5[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
                                            0.5688888888888888888888889, 0.4786286704993664680412915,
                                            0.2369268850561890875142640};
 // 5-point Gauss-Lobatto (ruqope_lobatto, dgflo2d.f90:2571-2583)
-__constant__ static const double PL5[5] = {-1.0, -0.6546536707079771437983, 0.0, 0.6546536707079771437983, 1.0};
-__constant__ static const double WL5[5] = {0.1, 0.544444444444444444444, 0.7111111111111111111111,
+static constexpr double PL5[5] = {-1.0, -0.6546536707079771437983, 0.0, 0.6546536707079771437983, 1.0};
+static constexpr double WL5[5] = {0.1, 0.544444444444444444444, 0.7111111111111111111111,
                                            0.544444444444444444444, 0.1};
 
 // Shape-function table of the fixed quadrature points (k_q2_shape_tables):
