@@ -29,9 +29,9 @@ namespace {
 __device__ __forceinline__ double xvq_(int iv) { return (iv == 1 || iv == 2) ? 1.0 : -1.0; }
 __device__ __forceinline__ double yvq_(int iv) { return (iv >= 2) ? 1.0 : -1.0; }
 
-__constant__ const double AG5P[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
+static constexpr double AG5P[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
                                      0.5384693101056830910363144, 0.9061798459386639927976269};
-__constant__ const double AG5W[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
+static constexpr double AG5W[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
                                      0.5688888888888888888888889, 0.4786286704993664680412915,
                                      0.2369268850561890875142640};
 

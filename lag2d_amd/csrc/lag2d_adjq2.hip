@@ -34,7 +34,7 @@ namespace {
 // FIRST[isg][iv]: local node IPQSG[isg][iv] appears in no sub-cell before
 // isg, so a node's (cell, sub-cell corner) list names each (cell, local
 // node) pair once by its first sub-cell.
-__constant__ const uint8_t FIRST[4][4] = {{1, 1, 1, 1}, {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 1}};
+static constexpr uint8_t FIRST[4][4] = {{1, 1, 1, 1}, {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 1}};
 
 // The corner faces that use face slot sl (the inverse of FNQSG: corner face
 // cf = 2 (ivsg + 4 isg) + f names slot |FNQSG[isg][2 ivsg + f]| - 1), in
@@ -64,7 +64,7 @@ constexpr bool slot_uses_consistent()
     return true;
 }
 static_assert(slot_uses_consistent(), "kSlotUses must invert FNQSG");
-__constant__ const int SLOT_USES[16][4] = {{1, -1, -1, -1},  {10, -1, -1, -1}, {11, -1, -1, -1}, {20, -1, -1, -1},
+static constexpr int SLOT_USES[16][4] = {{1, -1, -1, -1},  {10, -1, -1, -1}, {11, -1, -1, -1}, {20, -1, -1, -1},
                                            {21, -1, -1, -1}, {30, -1, -1, -1}, {31, -1, -1, -1}, {0, -1, -1, -1},
                                            {3, 4, 8, 15},    {13, 14, 17, 18}, {16, 23, 27, 28}, {5, 6, 25, 26},
                                            {2, 9, -1, -1},   {12, 19, -1, -1}, {22, 29, -1, -1}, {7, 24, -1, -1}};

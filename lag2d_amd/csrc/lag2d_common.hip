@@ -273,10 +273,10 @@ __device__ __forceinline__ void shape_q2_d(double r, double s, double *sh, doubl
 
 // 6x6 Gauss rule of ruqope (dgflo2d.f90:2752-2763): permuted points and
 // weights truncated to 16 digits, as the reference lists them.
-__constant__ static const double PG6[6] = {0.2386191860831969086305017, -0.2386191860831969086305017,
+static constexpr double PG6[6] = {0.2386191860831969086305017, -0.2386191860831969086305017,
                                            0.6612093864662645136613996, -0.6612093864662645136613996,
                                            -0.9324695142031520278123016, 0.9324695142031520278123016};
-__constant__ static const double WG6[6] = {0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
+static constexpr double WG6[6] = {0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
                                            0.3607615730481386, 0.1713244923791704, 0.1713244923791704};
 
 // Cell volume and volume centre on the 6x6 Gauss rule.  With set_centre
@@ -301,13 +301,13 @@ __device__ __forceinline__ void rz_rows(const Dev &d, int ie, double vol, double
     d.G[15 * nc + ie] = sca / area;
 }
 
-__constant__ static const double PT7R_C[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115,
+static constexpr double PT7R_C[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115,
                                               0.470142064105115, 0.797426985353087, 0.101286507323456,
                                               0.101286507323456};
-__constant__ static const double PT7S_C[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770,
+static constexpr double PT7S_C[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770,
                                               0.470142064105115, 0.101286507323456, 0.797426985353087,
                                               0.101286507323456};
-__constant__ static const double WT7_C[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506,
+static constexpr double WT7_C[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506,
                                              0.132394152788506, 0.125939180544827, 0.125939180544827,
                                              0.125939180544827};
 
@@ -534,9 +534,9 @@ template __global__ void k_cellvol_tpc<0>(Dev, double *__restrict__, double *__r
 template __global__ void k_cellvol_tpc<1>(Dev, double *__restrict__, double *__restrict__, int);
 
 // 5x5 Gauss rule of ruqope (dgflo2d.f90:2734-2743 ordering)
-__constant__ static const double PG5I[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
+static constexpr double PG5I[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
                                             0.5384693101056830910363144, 0.9061798459386639927976269};
-__constant__ static const double WG5I[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
+static constexpr double WG5I[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
                                             0.5688888888888888888888889, 0.4786286704993664680412915,
                                             0.2369268850561890875142640};
 

@@ -39,8 +39,8 @@ __device__ __forceinline__ double barthfct(double umax, double umin, double uctr
 }
 
 // Reference-quad vertex coordinates (corner order 1..4, CCW)
-__constant__ static const double XVQ[4] = {-1.0, 1.0, 1.0, -1.0};
-__constant__ static const double YVQ[4] = {-1.0, -1.0, 1.0, 1.0};
+static constexpr double XVQ[4] = {-1.0, 1.0, 1.0, -1.0};
+static constexpr double YVQ[4] = {-1.0, -1.0, 1.0, 1.0};
 
 struct Dev {
     int nc, np, nb, nd, nvq, nmatr;

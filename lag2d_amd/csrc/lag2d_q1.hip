@@ -37,16 +37,16 @@ __device__ __forceinline__ double xvt(int iv) { return iv == 1 ? 1.0 : 0.0; }
 __device__ __forceinline__ double yvt(int iv) { return iv == 2 ? 1.0 : 0.0; }
 // corner faces of vertex iv: (edge before, edge after), edge e joins nodes e
 // and e+1; and the far end of each of those edges
-__constant__ static const int TFE[3][2] = {{2, 0}, {0, 1}, {1, 2}};
-__constant__ static const int TNB[3][2] = {{2, 1}, {0, 2}, {1, 0}};
+static constexpr int TFE[3][2] = {{2, 0}, {0, 1}, {1, 2}};
+static constexpr int TNB[3][2] = {{2, 1}, {0, 2}, {1, 0}};
 // rutope 7-point area rule (dgflo2d.f90:2234-2255), its literal values
-__constant__ static const double PT7R[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115,
+static constexpr double PT7R[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115,
                                             0.470142064105115, 0.797426985353087, 0.101286507323456,
                                             0.101286507323456};
-__constant__ static const double PT7S[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770,
+static constexpr double PT7S[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770,
                                             0.470142064105115, 0.101286507323456, 0.797426985353087,
                                             0.101286507323456};
-__constant__ static const double WT7[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506,
+static constexpr double WT7[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506,
                                            0.132394152788506, 0.125939180544827, 0.125939180544827,
                                            0.125939180544827};
 

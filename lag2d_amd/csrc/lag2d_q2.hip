@@ -375,8 +375,8 @@ struct WenoShared {
     int face[4], fnei[4];
 };
 
-__constant__ static const int ST1[4][5] = {{1, 4, 5, 8, 7}, {2, 1, 6, 5, 8}, {3, 2, 7, 6, 5}, {4, 3, 8, 7, 6}};
-__constant__ static const int ST2[4][3] = {{1, 5, 8}, {2, 5, 6}, {3, 7, 6}, {4, 7, 8}};
+static constexpr int ST1[4][5] = {{1, 4, 5, 8, 7}, {2, 1, 6, 5, 8}, {3, 2, 7, 6, 5}, {4, 3, 8, 7, 6}};
+static constexpr int ST2[4][3] = {{1, 5, 8}, {2, 5, 6}, {3, 7, 6}, {4, 7, 8}};
 
 // Value of lane J of this lane's quad (lanes 4i..4i+3), by DPP quad_perm;
 // every lane of the quad must be active.
@@ -2114,7 +2114,7 @@ __device__ __forceinline__ void inv2(const double *a, double *b)
     b[3] = a[0] / det;
 }
 
-__constant__ static const int ST1P1[4][3] = {{1, 4, 5}, {2, 1, 6}, {3, 2, 7}, {4, 3, 8}};
+static constexpr int ST1P1[4][3] = {{1, 4, 5}, {2, 1, 6}, {3, 2, 7}, {4, 3, 8}};
 
 __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__restrict__ Uin)
 {

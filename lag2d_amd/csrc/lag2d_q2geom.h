@@ -19,16 +19,16 @@ namespace lag2d {
 #define Q2_REC_CELLS(nc) ((((size_t)(nc)) + 63) / 64 * 64)
 
 // sub-cell vertex -> local Q2 node (ipqsg)
-__constant__ static const int IPQSG[4][4] = {{0, 4, 8, 7}, {4, 1, 5, 8}, {8, 5, 2, 6}, {7, 8, 6, 3}};
+static constexpr int IPQSG[4][4] = {{0, 4, 8, 7}, {4, 1, 5, 8}, {8, 5, 2, 6}, {7, 8, 6, 3}};
 // signed face-normal slots (fnqsg): sign = orientation, |v|-1 = gesgq slot
-__constant__ static const int FNQSG[4][8] = {{8, 1, 13, -9, -9, 12, 12, 16},
+static constexpr int FNQSG[4][8] = {{8, 1, 13, -9, -9, 12, 12, 16},
                                              {9, 13, 2, 3, 14, -10, -10, 9},
                                              {-11, 10, 10, 14, 4, 5, 15, -11},
                                              {16, -12, -12, 11, 11, 15, 6, 7}};
-__constant__ static const double XV9[9] = {-1.0, 1.0, 1.0, -1.0, 0.0, 1.0, 0.0, -1.0, 0.0};
-__constant__ static const double YV9[9] = {-1.0, -1.0, 1.0, 1.0, -1.0, 0.0, 1.0, 0.0, 0.0};
+static constexpr double XV9[9] = {-1.0, 1.0, 1.0, -1.0, 0.0, 1.0, 0.0, -1.0, 0.0};
+static constexpr double YV9[9] = {-1.0, -1.0, 1.0, 1.0, -1.0, 0.0, 1.0, 0.0, 0.0};
 // Riemann corner weights times 1/4 [isg][ivsg][f] (Getrhslaggauss.f90:10575-10599)
-__constant__ static const double WFR[4][4][2] = {
+static constexpr double WFR[4][4][2] = {
     {{0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * 1.0, 0.25 * 1.0},
      {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}},
     {{0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)},
@@ -38,11 +38,11 @@ __constant__ static const double WFR[4][4][2] = {
     {{0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * 1.0, 0.25 * 1.0}, {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)},
      {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}}};
 // face-integral weights: momentum/energy {0,1} and volume times 1/4 (:8573-8621)
-__constant__ static const double WFM[4][4][2] = {{{1, 1}, {1, 1}, {0, 0}, {1, 1}},
+static constexpr double WFM[4][4][2] = {{{1, 1}, {1, 1}, {0, 0}, {1, 1}},
                                                  {{1, 1}, {1, 1}, {1, 1}, {0, 0}},
                                                  {{0, 0}, {1, 1}, {1, 1}, {1, 1}},
                                                  {{1, 1}, {0, 0}, {1, 1}, {1, 1}}};
-__constant__ static const double WFV[4][4][2] = {
+static constexpr double WFV[4][4][2] = {
     {{0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}, {0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}},
     {{0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}, {0.0, 0.0}},
     {{0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}},

@@ -28,22 +28,22 @@ namespace {
 #define UI(q, id) Uin[((q) * nd + (id)) * nc + ie]
 
 // rutope 7-point rule of the reference triangle (dgflo2d.f90:2234-2255)
-__constant__ const double T7R[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115, 0.470142064105115,
+static constexpr double T7R[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115, 0.470142064105115,
                                     0.797426985353087, 0.101286507323456, 0.101286507323456};
-__constant__ const double T7S[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770, 0.470142064105115,
+static constexpr double T7S[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770, 0.470142064105115,
                                     0.101286507323456, 0.797426985353087, 0.101286507323456};
-__constant__ const double T7W[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506, 0.132394152788506,
+static constexpr double T7W[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506, 0.132394152788506,
                                     0.125939180544827, 0.125939180544827, 0.125939180544827};
 // ruqope 5x5 and 6x6 Gauss points in the reference's order (dgflo2d.f90:2646-2783)
-__constant__ const double G5P[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
+static constexpr double G5P[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
                                     0.5384693101056830910363144, 0.9061798459386639927976269};
-__constant__ const double G5W[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
+static constexpr double G5W[5] = {0.2369268850561890875142640, 0.4786286704993664680412915,
                                     0.5688888888888888888888889, 0.4786286704993664680412915,
                                     0.2369268850561890875142640};
-__constant__ const double G6P[6] = {0.2386191860831969086305017, -0.2386191860831969086305017,
+static constexpr double G6P[6] = {0.2386191860831969086305017, -0.2386191860831969086305017,
                                     0.6612093864662645136613996, -0.6612093864662645136613996,
                                     -0.9324695142031520278123016, 0.9324695142031520278123016};
-__constant__ const double G6W[6] = {0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
+static constexpr double G6W[6] = {0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
                                     0.3607615730481386, 0.1713244923791704, 0.1713244923791704};
 
 // reference vertices: triangle (0,0), (1,0), (0,1); quad (-1,-1), (1,-1), (1,1), (-1,1)
