@@ -562,6 +562,7 @@ static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.33333333333333
 // cell-RHS slots name the variant the autotuner kept.
 static const char *kernel_name(const lag2d_ctx *x, int k)
 {
+    if (k == KID_Q2_CORNER && x->corner_variant == 9) return "k_q2_corner_wu2";
     if (k == KID_Q2_CORNER && x->corner_variant >= 7) return "k_q2_corner_wu";
     if (k == KID_Q2_CORNER && x->corner_variant >= 2) return "k_q2_corner_sc";
     if (k == KID_CELLRHS_Q2 && x->rhs_variant >= 2) return "k_cellrhs_q2_tpc";

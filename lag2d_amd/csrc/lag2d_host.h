@@ -8,7 +8,7 @@
 #include "../../include/lag2d.h"
 #include "lag2d_dev.h"
 
-constexpr int LAG2D_CORNER_VARIANTS = 8, LAG2D_RHS_VARIANTS = 4;
+constexpr int LAG2D_CORNER_VARIANTS = 9, LAG2D_RHS_VARIANTS = 4;
 
 enum {
     KID_CELLAVG_Q1 = 0, KID_NODEMINMAX, KID_LIMIT_Q1, KID_NODE_Q1, KID_CELLRHS_Q1, KID_DT_PARTIAL,
@@ -59,7 +59,8 @@ struct lag2d_ctx {
     // curved-path kernel variants (launch_corner, launch_cellrhs in
     // lag2d_q2.hip): 0 = not tuned yet; corner 1 = lanes per cell, 2-6 = a
     // lane per sub-cell (cells per workgroup, LDS staging), 7-8 = a wave per
-    // sub-cell (registers for 2 or 3 waves per SIMD); cell RHS 1 = 8
+    // sub-cell (registers for 2 or 3 waves per SIMD), 9 = two waves per
+    // sub-cell; cell RHS 1 = 8
     // cells per workgroup, 2-4 = a thread per cell (workgroup size)
     int corner_variant = 0;
     float corner_ms[LAG2D_CORNER_VARIANTS] = {};  // tuning timings

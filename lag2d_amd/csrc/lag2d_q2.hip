@@ -1436,13 +1436,14 @@ constexpr int CWU_CELLS = 64;
 
 // Face slots 4 ISG .. 4 ISG + 3 of the cell (8-11 interior half-edges, the
 // others boundary points) into sgq and gq, and the nodes of sub-cell ISG.
-template <int ISG>
+// U0 .. U1 - 1: the slots (of the four) this wave evaluates.
+template <int ISG, int U0 = 0, int U1 = 4>
 __device__ __forceinline__ void corner_wu_faces(const Dev &d, int ie, int cl, const double *xp, const double *yp,
                                                 double (*sgq)[CWU_CELLS], double *xs, double *ys)
 {
     const int nc = d.nc;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = U0; u < U1; ++u) {
         const int slot = 4 * ISG + u;
         double gx, gy, ga;
         if (slot >= 8 && slot < 12) {
@@ -1613,6 +1614,151 @@ __global__ void __launch_bounds__(4 * CWU_CELLS) __attribute__((amdgpu_waves_per
                 deltu = fabs(dux * nx + duy * ny);
                 if constexpr (MODE == 1) d.dlt[od] = deltu;
             }
+            const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
+            double m11;
+            if (a3 < 1.e12) {
+                m11 = mur * area * fabs(1.0);
+            } else {
+                const double nrm = sqrt(dux * dux + duy * duy);
+                m11 = mur * area * fabs(nx * (-dux / nrm) + ny * (-duy / nrm));
+            }
+            d.rec[RC2(ie, iv + 4 * isg, f)] = m11;
+        }
+    }
+}
+
+// The wave-per-sub-cell corner kernel with two waves per sub-cell, for
+// meshes too small to fill the chip, where a wave's instruction latency is
+// the kernel's duration.  Waves isg and isg + 4 share sub-cell isg of the
+// workgroup's 64 cells: each evaluates two of its four face slots and two of
+// its four corners; the first sums the quadrature terms of points
+// 0 .. CW2_SPLIT - 1 in order, the second evaluates the terms of the other
+// points into LDS, and the first then continues its sums with them, so
+// every sum runs in point order as in k_q2_corner_wu.
+constexpr int CW2_SPLIT = 13;
+
+template <int ND>
+__global__ void __launch_bounds__(8 * CWU_CELLS) k_q2_corner_wu2(Dev d)
+{
+    __shared__ double sgq[16 * 3][CWU_CELLS];
+    __shared__ double sterm[4][3][25 - CW2_SPLIT][CWU_CELLS];  // second wave's terms
+    __shared__ double ssum[4][3][CWU_CELLS];                   // a, masel, volel
+    const int cl = threadIdx.x & (CWU_CELLS - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int isg = w & 3, half = w >> 2;
+    const int ie = blockIdx.x * CWU_CELLS + cl;
+    const int nc = d.nc;
+    const bool active = ie < nc;
+    const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
+    double xp[9], yp[9], xs[9], ys[9];
+    double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
+    double a = 0.0, masel = 0.0, volel = 0.0;
+    if (active) {
+        load9(d, d.xs, d.ys, ie, xp, yp);
+        rc = d.G[0 * nc + ie];
+        sc = d.G[1 * nc + ie];
+        gg0 = d.G[18 * nc + ie];
+        gg1 = d.G[19 * nc + ie];
+        gg2 = d.G[20 * nc + ie];
+        switch (w) {
+        case 0: corner_wu_faces<0, 0, 2>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        case 1: corner_wu_faces<1, 0, 2>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        case 2: corner_wu_faces<2, 0, 2>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        case 3: corner_wu_faces<3, 0, 2>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        case 4: corner_wu_faces<0, 2, 4>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        case 5: corner_wu_faces<1, 2, 4>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        case 6: corner_wu_faces<2, 2, 4>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        default: corner_wu_faces<3, 2, 4>(d, ie, cl, xp, yp, sgq, xs, ys); break;
+        }
+        // ---- quadrature terms of sub-cell isg at this wave's points
+        double u0[ND];
+        for (int id = 0; id < ND; ++id) u0[id] = UDN(cell_dofs(d, ie), 0, id, ie);
+        const double sr = (isg == 1 || isg == 2) ? 1.0 : -1.0, ss = isg >= 2 ? 1.0 : -1.0;
+        const double rhomc = u0[0];
+        const kdouble *stab = (const kdouble *)d.shp;
+        const int gb = half ? CW2_SPLIT : 0, ge = half ? 25 : CW2_SPLIT;
+#pragma unroll 1
+        for (int g = gb; g < ge; ++g) {
+            const int i = g / 5, j = g - 5 * i;
+            double J[4];
+            jac9_k(xs, ys, stab + g * SHP_STRIDE, J);
+            const double ta = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]);
+            const double r = (PG5[i] + sr) / 2.0;
+            const double s = (PG5[j] + ss) / 2.0;
+            double b[6];
+            basis6(r, s, rc, sc, gg0, gg1, gg2, b);
+            double t = 0.0;
+            for (int id = 0; id < ND; ++id) t = t + u0[id] * b[id];
+            t = rhomc + 1.0 * (t - rhomc);
+            jac9_k(xp, yp, stab + (25 + 25 * isg + g) * SHP_STRIDE, J);
+            const double djaco = WG5[i] * WG5[j] * (J[0] * J[3] - J[2] * J[1]) * 0.25;
+            const double tm = 1.0 / t * djaco;
+            if (half == 0) {
+                a = a + ta;
+                masel = masel + tm;
+                volel = volel + djaco;
+            } else {
+                sterm[isg][0][g - CW2_SPLIT][cl] = ta;
+                sterm[isg][1][g - CW2_SPLIT][cl] = tm;
+                sterm[isg][2][g - CW2_SPLIT][cl] = djaco;
+            }
+        }
+    }
+    __syncthreads();  // face slots and the second wave's terms shared
+    if (half == 0 && active) {
+        for (int g = CW2_SPLIT; g < 25; ++g) {
+            a = a + sterm[isg][0][g - CW2_SPLIT][cl];
+            masel = masel + sterm[isg][1][g - CW2_SPLIT][cl];
+            volel = volel + sterm[isg][2][g - CW2_SPLIT][cl];
+        }
+        ssum[isg][0][cl] = a;
+        ssum[isg][1][cl] = masel;
+        ssum[isg][2][cl] = volel;
+    }
+    __syncthreads();  // the sub-cell sums shared
+    if (!active) return;
+    a = ssum[isg][0][cl];
+    masel = ssum[isg][1][cl];
+    volel = ssum[isg][2][cl];
+    const double rsub = 0.0 + (1.0 / a) * d.msub[isg * nc + ie];
+    const double uq = volel / masel;
+    double u[4][ND];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < ND; ++id) u[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
+    const double rhomc = u[0][0], uctr = u[1][0], vctr = u[2][0], ectr = u[3][0];
+    const double rhoct = 1.0 / rhomc;
+    const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+    const double sdctr = sqrt(dmax(eps, d.gamlg * pctr / rhoct));
+    const double dcor = d.cdrho * (rsub - 1.0 / uq);
+#pragma unroll 1
+    for (int iv = 2 * half; iv < 2 * half + 2; ++iv) {
+        // ---- sub-cell corner (isg, iv)
+        const int ln = IPQSG[isg][iv];
+        double un[4];
+        {
+            double b[6];
+            basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, b);
+            for (int q = 0; q < 4; ++q) un[q] = 0.0;
+            for (int id = 0; id < ND; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + u[q][id] * b[id];
+        }
+        const int p = d.ipq[ln * nc + ie];
+        const double rhovt = 1.0 / un[0];
+        const double rhovsg = rhovt + dcor;
+        const double uv = un[1], vv = un[2], ev = un[3];
+        const double pv = dmax(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)));
+        const double dux = d.usx[p] - uv, duy = d.usy[p] - vv;
+        const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+        d.rec[RC2(ie, iv + 4 * isg, 2)] = uv;
+        d.rec[RC2(ie, iv + 4 * isg, 3)] = vv;
+        d.rec[RC2(ie, iv + 4 * isg, 4)] = pv;
+        for (int f = 0; f < 2; ++f) {
+            const int tt = FNQSG[isg][2 * iv + f];
+            const int sl = (tt > 0 ? tt : -tt) - 1;
+            const double sg = tt > 0 ? 1.0 : -1.0;
+            const double nx = sg * sgq[3 * sl + 0][cl], ny = sg * sgq[3 * sl + 1][cl];
+            const double area = sgq[3 * sl + 2][cl] * WFR[isg][iv][f];
+            const double deltu = fabs(dux * nx + duy * ny);
             const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
             double m11;
             if (a3 < 1.e12) {
@@ -2547,6 +2693,8 @@ static void launch_prep(lag2d_ctx *x, Dev &ds, const double *Uin)
 //   7-8  a wave per sub-cell, 64 cells per workgroup (uniform sub-cell
 //        index: specialised face code, the table as scalar operands), with
 //        registers for 2 or 3 waves per SIMD
+//   9    two waves per sub-cell, 64 cells per workgroup (half the
+//        instructions per wave: short latency, small meshes)
 static void launch_corner(lag2d_ctx *x, Dev &ds, int variant)
 {
     const int nc = x->nc;
@@ -2559,6 +2707,7 @@ static void launch_corner(lag2d_ctx *x, Dev &ds, int variant)
     case 6: hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 32, 0>), g(32), dim3(128), 0, x->stream, ds); break;
     case 7: hipLaunchKernelGGL((k_q2_corner_wu<6, 0, 1>), g(CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds); break;
     case 8: hipLaunchKernelGGL((k_q2_corner_wu<6, 0, 3>), g(CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds); break;
+    case 9: hipLaunchKernelGGL(k_q2_corner_wu2<6>, g(CWU_CELLS), dim3(8 * CWU_CELLS), 0, x->stream, ds); break;
     default: hipLaunchKernelGGL(k_q2_corner, g(CORNER_CPB), dim3(256), 0, x->stream, ds); break;
     }
 }

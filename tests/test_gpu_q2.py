@@ -14,10 +14,11 @@ TOL = 1e-12
 # (corner, cell RHS, geometry) variants: lanes per cell / 8 cells per
 # workgroup; a lane per sub-cell with 64, 32, 16 cells per workgroup and the
 # shape table in LDS, or 64, 32 cells reading it through the caches; a wave
-# per sub-cell (2 or 3 waves per SIMD) / a thread per cell in workgroups of 256, 128, 64; geometry
+# per sub-cell (2 or 3 waves per SIMD), two waves per sub-cell / a thread
+# per cell in workgroups of 256, 128, 64; geometry
 # (cell volume and Persson indicator) 1 = 8 lanes per cell, 2 = a thread per
 # cell
-VARIANT_PAIRS = [(1, 1, 1), (2, 2, 2), (3, 3, 1), (4, 4, 2), (5, 2, 1), (6, 3, 2), (7, 4, 2), (8, 2, 1)]
+VARIANT_PAIRS = [(1, 1, 1), (2, 2, 2), (3, 3, 1), (4, 4, 2), (5, 2, 1), (6, 3, 2), (7, 4, 2), (8, 2, 1), (9, 1, 2)]
 
 
 @pytest.fixture(params=VARIANT_PAIRS, ids=[f"corner{c}-rhs{r}-geom{g}" for c, r, g in VARIANT_PAIRS])
