@@ -2095,11 +2095,13 @@ k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
                        us2 * fy[1] * b * w2;
         }
     }
-    // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form)
+    // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form);
+    //      the limited DOFs, read once, also feed the update below
+    double ul[4][ND];
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < ND; ++id) ul[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
     {
-        double u[4][ND];
-        for (int q = 0; q < 4; ++q)
-            for (int id = 0; id < ND; ++id) u[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
+        const double(&u)[4][ND] = ul;
         double xp[9], yp[9];
         load9(d, d.xs, d.ys, ie, xp, yp);
 #pragma unroll 1
@@ -2137,9 +2139,6 @@ k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
         }
     }
     // ---- derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708)
-    double ul[4][ND];
-    for (int q = 0; q < 4; ++q)
-        for (int id = 0; id < ND; ++id) ul[q][id] = UDN(cell_dofs(d, ie), q, id, ie);
     for (int id = 1; id < 3; ++id) {
         const double u1 = ul[1][id], u2 = ul[2][id];
         ul[0][id] = ul[0][id] * 1.0;
