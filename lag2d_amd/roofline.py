@@ -52,7 +52,8 @@ BYTES_Q2 = {
 }
 NODE_KERNELS = {"k_node_q2", "k_nodemove"}
 # the large-mesh variants do the same work (identical results)
-VARIANTS = {"k_q2_corner_sc": ("k_q2_corner", 2), "k_cellrhs_q2_tpc": ("k_cellrhs_q2", 2)}
+VARIANTS = {"k_q2_corner_sc": ("k_q2_corner", 2), "k_q2_corner_wu": ("k_q2_corner", 7),
+            "k_q2_corner_wu2": ("k_q2_corner", 9), "k_cellrhs_q2_tpc": ("k_cellrhs_q2", 2)}
 for _v, (_k, _) in VARIANTS.items():
     FLOPS_Q2[_v] = FLOPS_Q2[_k]
     BYTES_Q2[_v] = BYTES_Q2[_k]
@@ -64,15 +65,17 @@ for _v, (_k, _) in VARIANTS.items():
 # WRITE_SIZE all of them (factor 1.0).  Source: profiles/counters_sedov_p2_120x120.txt
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 MEASURED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) KiB per dispatch, 120x120 benchmark only
-    "k_q2_prep": (1773.9, 14.4),
-    "k_q2_weno": (623.1, 19.6),
-    "k_q2_corner": (4142.7, 23400.0),      # lanes-per-cell variant
-    "k_q2_corner_sc": (4003.8, 25528.0),   # lane-per-sub-cell variant
-    "k_node_q2": (15197.1, 907.6),
-    "k_cellrhs_q2": (15993.6, 4052.8),     # 8-cells-per-block variant
-    "k_cellrhs_q2_tpc": (14909.7, 4050.1),  # thread-per-cell variant
+    "k_q2_prep": (1774.4, 14.4),
+    "k_q2_weno": (622.5, 19.6),
+    "k_q2_corner": (4123.1, 14400.0),       # lanes-per-cell variant
+    "k_q2_corner_sc": (3963.4, 14400.0),    # lane-per-sub-cell variant (64 cells, table in LDS)
+    "k_q2_corner_wu": (3995.5, 14400.0),    # wave-per-sub-cell variant
+    "k_q2_corner_wu2": (4047.5, 14400.0),   # two waves per sub-cell
+    "k_node_q2": (9915.3, 907.6),
+    "k_cellrhs_q2": (11345.7, 4320.5),      # 8-cells-per-block variant
+    "k_cellrhs_q2_tpc": (10580.5, 5850.1),  # thread-per-cell variant (256-thread workgroups)
     "k_update": (10316.5, 2700.0),
-    "k_nodemove": (1227.3, 907.6),
+    "k_nodemove": (1226.8, 907.6),
     "k_cellvol": (1161.4, 768.5),
 }
 
@@ -81,8 +84,9 @@ MEASURED_KIB = {  # kernel: (FETCH_SIZE, WRITE_SIZE) KiB per dispatch, 120x120 b
 # with rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 on the 120x120
 # benchmark, by kernel variant.  Source: profiles/fp64_mix_sedov_p2_120x120.txt
 MEASURED_F64 = {
-    "k_q2_corner": {1: (3740400, 4834800, 532800), 2: (2630700, 3012300, 416700)},
-    "k_cellrhs_q2": {1: (1378800, 2021400, 43200), 2: (913050, 1247850, 33750)},
+    "k_q2_corner": {1: (3711600, 4770000, 532800), 2: (2601900, 2947500, 416700), 7: (2366550, 2398950, 348300),
+                    9: (2466000, 2455650, 381600)},
+    "k_cellrhs_q2": {1: (1407600, 1920600, 43200), 2: (963450, 1319850, 33750)},
 }
 
 
