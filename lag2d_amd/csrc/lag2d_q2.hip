@@ -2261,10 +2261,34 @@ __device__ __forceinline__ void inv2(const double *a, double *b)
 
 static constexpr int ST1P1[4][3] = {{1, 4, 5}, {2, 1, 6}, {3, 2, 7}, {4, 3, 8}};
 
+// One of four values by a face index 0..3 (selects, not an indexed array,
+// so nothing goes to scratch memory).
+__device__ __forceinline__ double sel4(int f, double a0, double a1, double a2, double a3)
+{
+    return f == 0 ? a0 : (f == 1 ? a1 : (f == 2 ? a2 : a3));
+}
+
+// Characteristic DOF (id, k) of candidate c: sum over the variables in
+// order, from 0.
+__device__ __forceinline__ double c1_cha(const double (&qi)[4][4], const double (&unksl)[5][4][3], int c, int k,
+                                         int id)
+{
+    double v = 0.0;
+    for (int q = 0; q < 4; ++q) v = v + qi[k][q] * unksl[c][q][id];
+    return v;
+}
+
+// The kernel keeps only what the reconstruction reads: the troubled cell's
+// frame geometry, projections and slopes; of the eight neighbours, their
+// cell averages and Lagrange-stencil moments (their own slope projections,
+// which the reference also forms, feed nothing for npoly = 1).  The loop
+// over the neighbours is unrolled, so every array index is static and the
+// state stays in registers.
 __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__restrict__ Uin)
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc;
+    const kdouble *shv = (const kdouble *)d.shv;  // wave-uniform rows: scalar loads
     bool limited = false;
     if (ie < nc) {
         const int flags = d.wtop[9 * nc + ie];
@@ -2290,16 +2314,23 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
             limited = true;
             int ielaj[9];
             for (int k = 0; k < 9; ++k) ielaj[k] = d.wtop[k * nc + ie];
-            double unkrl[9][4][3], unkpl[9][4][3];
-            for (int s = 0; s < 9; ++s)
-                for (int id = 0; id < 3; ++id) {
-                    for (int q = 0; q < 4; ++q) unkrl[s][q][id] = UD3(Uin, q, id, ielaj[s]);
-                    rotf(m, unkrl[s][1][id], unkrl[s][2][id]);
-                }
-            double ct1 = 0.0, ct2 = 0.0, ct3 = 0.0, xmcl = 0.0, ymcl = 0.0, dxcl = 1.0;
-            double mtbbq[4], mtbbi[4], ml[9][2];
+            // the stencil's cell averages in the frame (unkrl(1, :, s))
+            double avg[9][4];
+#pragma unroll
             for (int s = 0; s < 9; ++s) {
-                const int je = ielaj[s];
+                for (int q = 0; q < 4; ++q) avg[s][q] = UD3(Uin, q, 0, ielaj[s]);
+                rotf(m, avg[s][1], avg[s][2]);
+            }
+            double ct1, ct2, ct3, xmcl, ymcl, dxcl;
+            double mtbbq[4], mtbbi[4], unkp0[4][3];
+            {
+                // ---- the troubled cell (stencil cell 0)
+                const int je = ielaj[0];
+                double u0r[4][3];
+                for (int id = 0; id < 3; ++id) {
+                    for (int q = 0; q < 4; ++q) u0r[q][id] = UD3(Uin, q, id, je);
+                    rotf(m, u0r[1][id], u0r[2][id]);
+                }
                 double xn[9], yn[9], xr[9], yr[9];
                 for (int k = 0; k < 9; ++k) {
                     const int p = d.ipq[k * nc + je];
@@ -2315,7 +2346,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
                 // initial map (WENO.f90:5989-6061)
                 double xmc = 0.0, ymc = 0.0;
                 for (int g = 0; g < 25; ++g) {
-                    const double *sh = d.shv + 9 * g;
+                    const kdouble *sh = shv + 9 * g;
                     double xg = 0.0, yg = 0.0;
                     for (int k = 0; k < 9; ++k) {
                         xg = xg + sh[k] * xn[k];
@@ -2328,16 +2359,14 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
                 double xcl = xmc / masj, ycl = ymc / masj;
                 rotf(m, xcl, ycl);
                 const double dxcj = sqrt(volj);
-                if (s == 0) {
-                    xmcl = xcl;
-                    ymcl = ycl;
-                    dxcl = dxcj;
-                }
-                double v1 = 0.0, v2 = 0.0, v3 = 0.0, m0 = 0.0, m1 = 0.0;
+                xmcl = xcl;
+                ymcl = ycl;
+                dxcl = dxcj;
+                double v1 = 0.0, v2 = 0.0, v3 = 0.0;
                 double bb[4] = {0.0, 0.0, 0.0, 0.0}, bbq[4] = {0.0, 0.0, 0.0, 0.0};
                 for (int g = 0; g < 25; ++g) {
                     const int i = g / 5, j = g - 5 * i;
-                    const double *sh = d.shv + 9 * g;
+                    const kdouble *sh = shv + 9 * g;
                     double xl = 0.0, yl = 0.0;
                     for (int k = 0; k < 9; ++k) {
                         xl = xl + sh[k] * xr[k];
@@ -2345,11 +2374,9 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
                     }
                     const double rd = d.wrd[g * nc + je];
                     const double b2 = (xl - xcl) / dxcj, b3 = (yl - ycl) / dxcj;
-                    if (s == 0) {
-                        v1 = v1 + rd * b2 * b2;
-                        v2 = v2 + rd * b3 * b3;
-                        v3 = v3 + rd * b2 * b3;
-                    }
+                    v1 = v1 + rd * b2 * b2;
+                    v2 = v2 + rd * b3 * b3;
+                    v3 = v3 + rd * b2 * b3;
                     const double bq[3] = {1.0, (PG5[i] - rcj) / 1.0, (PG5[j] - scj) / 1.0};
                     const double bl[3] = {1.0, b2, b3};
                     for (int a = 1; a < 3; ++a)
@@ -2357,47 +2384,73 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
                             bb[(a - 1) + 2 * (c - 1)] = bb[(a - 1) + 2 * (c - 1)] + rd * bq[a] * bl[c];
                             bbq[(a - 1) + 2 * (c - 1)] = bbq[(a - 1) + 2 * (c - 1)] + rd * bl[a] * bl[c];
                         }
-                    if (s > 0) {  // Lagrange-stencil moments in the troubled cell's basis
-                        m0 = m0 + rd * ((xl - xmcl) / dxcl) / masj;
-                        m1 = m1 + rd * ((yl - ymcl) / dxcl) / masj;
-                    }
                 }
-                ml[s][0] = m0;
-                ml[s][1] = m1;
-                if (s == 0) {
-                    ct1 = 0.5 * (v1 / masj);
-                    ct2 = 0.5 * (v2 / masj);
-                    ct3 = v3 / masj;
-                }
+                ct1 = 0.5 * (v1 / masj);
+                ct2 = 0.5 * (v2 / masj);
+                ct3 = v3 / masj;
                 double rhs[4][3];
                 for (int q = 0; q < 4; ++q) rhs[q][0] = rhs[q][1] = rhs[q][2] = 0.0;
                 for (int a = 1; a < 3; ++a)
                     for (int c = 1; c < 3; ++c)
-                        for (int q = 0; q < 4; ++q) rhs[q][a] = rhs[q][a] + unkrl[s][q][c] * bb[(c - 1) + 2 * (a - 1)];
+                        for (int q = 0; q < 4; ++q) rhs[q][a] = rhs[q][a] + u0r[q][c] * bb[(c - 1) + 2 * (a - 1)];
                 double bbqi[4];
                 inv2(bbq, bbqi);
-                if (s == 0) {
-                    for (int k = 0; k < 4; ++k) mtbbq[k] = bbq[k];
-                    inv2(bb, mtbbi);
-                }
-                for (int q = 0; q < 4; ++q) unkpl[s][q][1] = unkpl[s][q][2] = 0.0;
+                for (int k = 0; k < 4; ++k) mtbbq[k] = bbq[k];
+                inv2(bb, mtbbi);
+                for (int q = 0; q < 4; ++q) unkp0[q][1] = unkp0[q][2] = 0.0;
                 for (int a = 1; a < 3; ++a)
                     for (int c = 1; c < 3; ++c)
                         for (int q = 0; q < 4; ++q)
-                            unkpl[s][q][a] = unkpl[s][q][a] + rhs[q][c] * bbqi[(c - 1) + 2 * (a - 1)];
-                for (int q = 0; q < 4; ++q) unkpl[s][q][0] = unkrl[s][q][0];
+                            unkp0[q][a] = unkp0[q][a] + rhs[q][c] * bbqi[(c - 1) + 2 * (a - 1)];
+                for (int q = 0; q < 4; ++q) unkp0[q][0] = u0r[q][0];
+            }
+            // ---- Lagrange-stencil moments of the neighbours in the troubled
+            //      cell's basis
+            double ml[9][2];
+            ml[0][0] = ml[0][1] = 0.0;
+#pragma unroll
+            for (int s = 1; s < 9; ++s) {
+                const int je = ielaj[s];
+                double xr[9], yr[9];
+                for (int k = 0; k < 9; ++k) {
+                    const int p = d.ipq[k * nc + je];
+                    xr[k] = d.xs[p];
+                    yr[k] = d.ys[p];
+                    rotf(m, xr[k], yr[k]);
+                }
+                const double masj = d.G[3 * nc + je];
+                double m0 = 0.0, m1 = 0.0;
+#pragma unroll 1
+                for (int g = 0; g < 25; ++g) {
+                    const kdouble *sh = shv + 9 * g;
+                    double xl = 0.0, yl = 0.0;
+                    for (int k = 0; k < 9; ++k) {
+                        xl = xl + sh[k] * xr[k];
+                        yl = yl + sh[k] * yr[k];
+                    }
+                    const double rd = d.wrd[g * nc + je];
+                    m0 = m0 + rd * ((xl - xmcl) / dxcl) / masj;
+                    m1 = m1 + rd * ((yl - ymcl) / dxcl) / masj;
+                }
+                ml[s][0] = m0;
+                ml[s][1] = m1;
+                // one neighbour at a time: its loads are not hoisted above the
+                // previous neighbour's loop
+                asm volatile("" ::: "memory");
+                __builtin_amdgcn_sched_barrier(0);
             }
             // candidates: the centre polynomial and the 4 Lagrange stencils
             double unksl[5][4][3];
             for (int c = 0; c < 5; ++c)
                 for (int q = 0; q < 4; ++q) {
-                    unksl[c][q][0] = unkpl[0][q][0];
+                    unksl[c][q][0] = unkp0[q][0];
                     unksl[c][q][1] = unksl[c][q][2] = 0.0;
                 }
             for (int q = 0; q < 4; ++q) {
-                unksl[0][q][1] = unkpl[0][q][1];
-                unksl[0][q][2] = unkpl[0][q][2];
+                unksl[0][q][1] = unkp0[q][1];
+                unksl[0][q][2] = unkp0[q][2];
             }
+#pragma unroll
             for (int is = 0; is < 4; ++is) {
                 const int s0 = ST1P1[is][0], s1 = ST1P1[is][1], s2 = ST1P1[is][2];
                 // matlg(3, 2) read as a column-major 2 x 2 from its first four elements
@@ -2406,8 +2459,8 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
                 inv2(a, mi);
                 double rhsl[4][2];
                 for (int q = 0; q < 4; ++q) {
-                    rhsl[q][0] = unkpl[s0][q][0] - unkpl[0][q][0];
-                    rhsl[q][1] = unkpl[s1][q][0] - unkpl[0][q][0];
+                    rhsl[q][0] = avg[s0][q] - unkp0[q][0];
+                    rhsl[q][1] = avg[s1][q] - unkp0[q][0];
                 }
                 for (int i = 1; i < 3; ++i)
                     for (int e = 0; e < 2; ++e)
@@ -2427,40 +2480,43 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
             const double dx2 = dxc * dxc, dy2 = dyc * dyc, dx4 = dx2 * dxc * dxc, dy4 = dy2 * dyc * dyc;
             const double dxy = dyc * dxc, dxy2 = dxy * dxy;
             const int nf = d.wtop[10 * nc + ie];
+            // the faces' contributions are averaged in face order as they come
+            const double wf = 1.0 / (double)nf;
             double uph[4][3];
             for (int q = 0; q < 4; ++q) uph[q][0] = uph[q][1] = uph[q][2] = 0.0;
-            double ufp[4][4][2];
+#pragma unroll 1
             for (int a = 0; a < nf; ++a) {
                 const int f = d.wtop[(11 + a) * nc + ie], f1 = (f + 1) & 3;
-                const double dtx = xp[f1] - xp[f], dty = yp[f1] - yp[f];
+                const double dtx = sel4(f1, xp[0], xp[1], xp[2], xp[3]) - sel4(f, xp[0], xp[1], xp[2], xp[3]);
+                const double dty = sel4(f1, yp[0], yp[1], yp[2], yp[3]) - sel4(f, yp[0], yp[1], yp[2], yp[3]);
                 const double dlgt = sqrt(dtx * dtx + dty * dty);
                 double dnx = dty / dlgt, dny = -dtx / dlgt;
                 rotf(m, dnx, dny);
                 double ul[4], ur[4];
                 for (int q = 0; q < 4; ++q) {
-                    ul[q] = unkrl[0][q][0];
-                    ur[q] = unkrl[1 + f][q][0];
+                    ul[q] = avg[0][q];
+                    ur[q] = sel4(f, avg[1][q], avg[2][q], avg[3][q], avg[4][q]);
                 }
                 double P[9];
                 char_face(d.gamlg, dnx, dny, ul, ur, P);
                 double qi[4][4];
                 for (int k = 0; k < 4; ++k)
                     for (int q = 0; q < 4; ++q) qi[k][q] = char_entry(d.gamlg, P, 4 * k + q);
-                // unksf_cha(3, 4, 10), column-major, zeroed
-                double cha[120];
-                for (int k = 0; k < 120; ++k) cha[k] = 0.0;
-                for (int c = 0; c < 5; ++c)
-                    for (int q = 0; q < 4; ++q)
-                        for (int id = 0; id < 3; ++id)
-                            for (int k = 0; k < 4; ++k)
-                                cha[id + 3 * k + 12 * c] = cha[id + 3 * k + 12 * c] + qi[k][q] * unksl[c][q][id];
+                // the candidates' characteristic DOFs (unksf_cha(3, 4, 10),
+                // column-major, zero past the 5 candidates): the smoothness
+                // indicator of candidate c reads its 12 entries and the
+                // next candidate's first 3
                 double wh[5], wt = 0.0;
+#pragma unroll
                 for (int c = 0; c < 5; ++c) {
+                    double ch[15];
+                    for (int k = 0; k < 4; ++k)
+                        for (int id = 0; id < 3; ++id) ch[id + 3 * k] = c1_cha(qi, unksl, c, k, id);
+                    for (int id = 0; id < 3; ++id) ch[12 + id] = (c < 4) ? c1_cha(qi, unksl, c + 1, 0, id) : 0.0;
                     double o[4];
                     for (int q = 0; q < 4; ++q) {
-                        const double u2 = cha[1 + 3 * q + 12 * c], u3 = cha[2 + 3 * q + 12 * c];
-                        const double u4 = cha[3 + 3 * q + 12 * c], u5 = cha[4 + 3 * q + 12 * c],
-                                     u6 = cha[5 + 3 * q + 12 * c];
+                        const double u2 = ch[1 + 3 * q], u3 = ch[2 + 3 * q];
+                        const double u4 = ch[3 + 3 * q], u5 = ch[4 + 3 * q], u6 = ch[5 + 3 * q];
                         const double sm1 =
                             1.0 / dx2 * (u2 * u2 + 2.0 * ct1 * (u4 * u4) + 2.0 * ct2 * (u6 * u6) + 2.0 * ct3 * u4 * u6);
                         const double sm2 =
@@ -2476,22 +2532,25 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_c1_weno(Dev d, const double *__
                     wh[c] = wl / (t * t);
                 }
                 for (int c = 0; c < 5; ++c) wt = wt + wh[c];
+                // the weighted sum recomputes the characteristic DOFs: qi
+                // passes through an empty asm statement, so the compiler
+                // does not keep all 60 of the first pass live instead
+                for (int k = 0; k < 4; ++k)
+                    for (int q = 0; q < 4; ++q) asm volatile("" : "+v"(qi[k][q]));
                 double fch[4][3];
                 for (int q = 0; q < 4; ++q) fch[q][0] = fch[q][1] = fch[q][2] = 0.0;
+#pragma unroll
                 for (int c = 0; c < 5; ++c)
                     for (int q = 0; q < 4; ++q)
                         for (int id = 0; id < 3; ++id)
-                            fch[q][id] = fch[q][id] + (wh[c] / wt) * cha[id + 3 * q + 12 * c];
-                for (int k = 0; k < 4; ++k) ufp[a][k][0] = ufp[a][k][1] = 0.0;
+                            fch[q][id] = fch[q][id] + (wh[c] / wt) * c1_cha(qi, unksl, c, q, id);
                 for (int q = 0; q < 4; ++q)
-                    for (int id = 1; id < 3; ++id)
-                        for (int k = 0; k < 4; ++k)
-                            ufp[a][k][id - 1] = ufp[a][k][id - 1] + char_entry(d.gamlg, P, 16 + 4 * k + q) * fch[q][id];
+                    for (int id = 1; id < 3; ++id) {
+                        double ufp = 0.0;
+                        for (int r = 0; r < 4; ++r) ufp = ufp + char_entry(d.gamlg, P, 16 + 4 * q + r) * fch[r][id];
+                        uph[q][id] = uph[q][id] + wf * ufp;
+                    }
             }
-            const double wf = 1.0 / (double)nf;
-            for (int a = 0; a < nf; ++a)
-                for (int q = 0; q < 4; ++q)
-                    for (int id = 1; id < 3; ++id) uph[q][id] = uph[q][id] + wf * ufp[a][q][id - 1];
             // back to the cell's Taylor basis, then the global frame
             double rhsuk[4][2], out[4][3];
             for (int q = 0; q < 4; ++q) rhsuk[q][0] = rhsuk[q][1] = 0.0;
