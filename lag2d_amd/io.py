@@ -70,7 +70,7 @@ def read_ctrl(path) -> tuple[Control, dict]:
                 nlimi=int(allv["nlimi"]), ncurv=int(allv["ncurv"]), ndens=int(allv["ndens"]),
                 ndt=int(allv["ndt"]), nstag=int(allv["nstag"]), ncase=int(allv["ncase"]),
                 nriem=int(allv["nriem"]), nfint=int(allv["nfint"]), nrz=int(allv["nrz"]),
-                ndump=int(allv["ndump"]))
+                ndump=int(allv["ndump"]), ngausf=int(allv["ngausf"]))
     return c, allv
 
 

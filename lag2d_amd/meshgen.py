@@ -311,6 +311,10 @@ class Control:
     nriem: int = 1
     nfint: int = 1
     nrz: int = 0
+    # face quadrature points per face: the nfint=5 branch needs the 5-point
+    # Gauss-Lobatto face rule (its tables are written for ngausf = 5,
+    # Getrhslaggauss.f90:17253-17258); the other branches do not read it
+    ngausf: int = 2
     # output interval of the driver's animation files; 0 = none.  The
     # reference's main program evaluates mod(itime, ndump) even when ndump = 0
     # (dgflo2d.f90:430, no short-circuit), so a run of that program needs a
@@ -421,7 +425,7 @@ def write_ctrl(path, c: Control):
         "nmatel", "1",
         "gamlg cdrho tend cimpd", f"{c.gamlg!r} {c.cdrho!r} {c.tend!r} {c.cimpd!r}",
         "ntime ndegr mdegr npoly nreco", f"{c.ntime} {c.ndegr} {c.ndegr} {c.npoly} 0",
-        "nlimi ngaus ngausf ngausd nint", f"{c.nlimi} 2 2 7 {c.nint}",
+        "nlimi ngaus ngausf ngausd nint", f"{c.nlimi} 2 {c.ngausf} 7 {c.nint}",
         "ngausf_geo ngausd_geo", "2 7",
         "ncurv", f"{c.ncurv}",
         "nresi", "1000000",

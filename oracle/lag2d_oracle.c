@@ -1986,7 +1986,7 @@ static int rhs_rz(orc_case *c)
 int orc_rhs(orc_case *c)
 {
     if (c->ncurv == 0 && c->nfint == 1) return c->nrz == 2 ? rhs_rz(c) : rhs_q1(c);
-    if (c->ncurv == 1 && c->nfint == 4) return orc_rhs_q2(c);
+    if (c->ncurv == 1 && (c->nfint == 4 || c->nfint == 5)) return orc_rhs_q2(c);
     return 2;
 }
 

@@ -1432,11 +1432,547 @@ static void domn_q2(orc_case *c)
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* nfint = 5: the sub-cell scheme with vertex and non-vertex Gauss points on
+ * the cell faces (getrhsdg_lag_mcnew branch Getrhslagmc.f90:7712-7829).
+ *   getriem_quadsubg_gauss    Getrhslaggauss.f90:16657-17157
+ *   getndvelo_lagsubg_gaussh  Getrhslaggauss.f90:17161-17546
+ *   getvelo_quadsubg_gs3      Getrhslaggauss.f90:19312-19990
+ *   getvar_face_general       Getrhslaggauss.f90:18184-18368
+ *   getvectf                  Getrhslaggauss.f90:16547-16590
+ *   rhsifacedg_lagsubgq_gs    Getrhslaggauss.f90:17550-17939
+ * The face geometry routines of this branch (getfnds_lagsubgc,
+ * getfnds_lag_simpsonh3) compute the quad slots with the same expressions as
+ * the nfint=4 ones (fnds_q2).  Persson's flag is cleared before
+ * barthlimit_lag_curvquadcb_HO runs (:7763-7766), so that limiter skips
+ * every cell (Limiterlag.f90:10276) and aflim / afvec are reset to the
+ * identity (:7782-7785): the whole limiting block is omitted. */
+
+/* Lobatto point of the face rule, +-sqrt(3/7) as the reference writes it */
+#define G5A 0.6546536707079771437983
+/* cell-local face Gauss points 12..19 (posiq 13..20, :19443-19455): face lf
+ * holds points 12 + 2 lf + k, k = 0 at r = -G5A along corner lf -> lf+1 */
+static const double PQG5[8][2] = {{-G5A, -1.0}, {G5A, -1.0}, {1.0, -G5A}, {1.0, G5A},
+                                  {G5A, 1.0},  {-G5A, 1.0}, {-1.0, G5A}, {-1.0, -G5A}};
+/* sub-cell of each face Gauss point (sglgq, :18217-18220) */
+static const int SGG5[8] = {0, 1, 1, 2, 2, 3, 3, 0};
+
+/* corner weights of the nfint=5 Riemann solver (wfgsq, :16777-16796) */
+static const double WR5[4][4][2] = {{{1, 1}, {.5, 1}, {0, 0}, {1, .5}},
+                                    {{1, .5}, {1, 1}, {.5, 1}, {0, 0}},
+                                    {{0, 0}, {1, .5}, {1, 1}, {.5, 1}},
+                                    {{.5, 1}, {0, 0}, {1, .5}, {1, 1}}};
+
+/* face-integral weights of rhsifacedg_lagsubgq_gs (:17651-17700) from the
+ * 5-point Lobatto weights w1, w3: momentum/energy wm (w3/1.0) and volume wv
+ * (w3/2.0), both halved */
+static void wfg_face5(double wm[4][4][2], double wv[4][4][2])
+{
+    const double w1 = 0.1, w3 = 0.7111111111111111111111;
+    /* 0: w1, 1: w3, 2: w3 / k (k = 1 momentum, 2 volume), 3: zero */
+    static const int t[4][4][2] = {{{0, 0}, {2, 1}, {3, 3}, {1, 2}},
+                                   {{1, 2}, {0, 0}, {2, 1}, {3, 3}},
+                                   {{3, 3}, {1, 2}, {0, 0}, {2, 1}},
+                                   {{2, 1}, {3, 3}, {1, 2}, {0, 0}}};
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+            for (int f = 0; f < 2; ++f) {
+                const int k = t[a][b][f];
+                const double m = k == 0 ? w1 : k == 1 ? w3 : k == 2 ? w3 / 1.0 : 0.0;
+                const double v = k == 0 ? w1 : k == 1 ? w3 : k == 2 ? w3 / 2.0 : 0.0;
+                wm[a][b][f] = 0.5 * m;
+                wv[a][b][f] = 0.5 * v;
+            }
+}
+
+/* getvectf: unit normal (dy/dr, -dx/dr) and 2 |dx/dr| of the quadratic face
+ * (corner a, corner b, mid-node) at the face Lobatto point k */
+static void face_normal5(const double fx[3], const double fy[3], int k, double n[3])
+{
+    const double r = k == 0 ? -G5A : G5A;
+    const double d[3] = {-0.5 + r, 0.5 + r, -2.0 * r};
+    double dxdr = 0.0, dydr = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        dxdr = dxdr + d[i] * fx[i];
+        dydr = dydr + d[i] * fy[i];
+    }
+    const double djaco = sqrt(dxdr * dxdr + dydr * dydr);
+    const double anx = dydr, any = -dxdr;
+    n[0] = anx / sqrt(anx * anx + any * any);
+    n[1] = any / sqrt(anx * anx + any * any);
+    n[2] = djaco * 2.0;
+}
+
+/* getvar_face_general: the cell's state at its face Gauss point g (0..7):
+ * v[0..3] the DG values, v[4] pressure, v[5] rho c of the cell average,
+ * v[6] c of the cell average */
+static void face_state5(const orc_case *c, int ie, int g, const double *drho, double v[7])
+{
+    const double g1 = c->gamlg - 1.0, eps = 1.0e-6;
+    const double rhoct = 1.0 / U(c, c->unold, 0, 0, ie);
+    const double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie), ectr = U(c, c->unold, 0, 3, ie);
+    const double eintc = ectr - 0.5 * (uctr * uctr + vctr * vctr);
+    const double pctr = dmax(eps, g1 * rhoct * eintc);
+    const double sdctr = sqrt(c->gamlg * pctr / rhoct);
+    double b[6];
+    basis_p2(PQG5[g][0], PQG5[g][1], GE(c, 0, ie), GE(c, 1, ie), &GE(c, 18, ie), b);
+    for (int q = 0; q < 4; ++q) v[q] = 0.0;
+    for (int id = 0; id < c->ndegr; ++id)
+        for (int q = 0; q < 4; ++q) v[q] = v[q] + U(c, c->unold, id, q, ie) * b[id];
+    const double rhovt = 1.0 / v[0] + c->cdrho * drho[4 * ie + SGG5[g]];
+    const double eintv = v[3] - 0.5 * (v[1] * v[1] + v[2] * v[2]);
+    v[4] = dmax(eps, g1 * rhovt * eintv);
+    v[5] = rhoct * sdctr;
+    v[6] = sdctr;
+}
+
+/* one side of a face Gauss point's Riemann problem (getvelo_quadsubg_gs3
+ * :19668-19701 with getriecoef_matrixnew, Limiterlag.f90:1037-1076):
+ * out[0] = M (isotropic), out[1..2] = M u, out[3..4] = A sigma n */
+static void face_riem5(const orc_case *c, const double v[7], const double uf[2], const double n[3], double out[5])
+{
+    const double eps11 = (double)1.e-11f;
+    double a1 = uf[0] - v[1], a2 = uf[1] - v[2];
+    const double acnx = a1, acny = a2;
+    if (sqrt(acnx * acnx + acny * acny) < eps11) {
+        a1 = eps11;
+        a2 = eps11;
+    } else {
+        a1 = a1 / sqrt(acnx * acnx + acny * acny);
+        a2 = a2 / sqrt(acnx * acnx + acny * acny);
+    }
+    const double a3 = sqrt(acnx * acnx + acny * acny) / v[6];
+    const double delu = c->cimpd * fabs(a1 * n[0] + a2 * n[1]);
+    const double mur = v[5] + v[5] / v[6] * c->slpdu * delu;
+    const double area = n[2];
+    double m;
+    if (a3 < 1.0e12) m = mur * area * fabs(1.0);
+    else m = mur * area * fabs(n[0] * a1 + n[1] * a2);
+    const double pv = v[4];
+    out[0] = m;
+    out[1] = m * v[1] + 0.0 * v[2];
+    out[2] = 0.0 * v[1] + m * v[2];
+    out[3] = -pv * area * n[0] + 0.0 * area * n[1];
+    out[4] = 0.0 * area * n[0] + -pv * area * n[1];
+}
+
+/* boundary condition of each cell face (0 interior; 1 v = 0, 2 u = 0,
+ * 3 u = v = 0) and whether the boundary face's nodes run against the cell's
+ * counter-clockwise order (its normal then points inwards) */
+static void face_bc5(const orc_case *c, int *bc, int *rev)
+{
+    const int nc = c->ncell;
+    int *mid = (int *)malloc(sizeof(int) * c->npoin);
+    for (int p = 0; p < c->npoin; ++p) mid[p] = -1;
+    for (int ie = 0; ie < nc; ++ie)
+        for (int lf = 0; lf < 4; ++lf) {
+            mid[c->ipqua[9 * ie + 4 + lf]] = 4 * ie + lf;
+            bc[4 * ie + lf] = 0;
+            rev[4 * ie + lf] = 0;
+        }
+    for (int f = 0; f < c->nbfac; ++f) {
+        const int *b = c->bface + 5 * f;
+        const int k = mid[b[4]];
+        if (k < 0) continue;
+        int code = 0;
+        if (b[2] == 22) code = b[3] == 221 ? 1 : b[3] == 222 ? 2 : 0;
+        else if (b[2] == 25 || b[2] == 27) code = 3;
+        bc[k] = code;
+        rev[k] = b[0] != c->ipqua[9 * (k / 4) + (k % 4)];
+    }
+    free(mid);
+}
+
+/* nodal solve (getndvelo_lagsubg_gaussh): corners as getriem_quadsubg_gauss
+ * computes them, the 2x2 node solve and walls, then the face Gauss points.
+ * Outputs: fstar (vertex corner forces, as ndvelo_q2), and per cell and
+ * local Gauss point fg[(j + 6*g) + 48*ie]: j = 0,1 the force, 2,3 the
+ * correction force, 4,5 the interpolated nodal velocity ufgpq */
+static void ndvelo_q2g(orc_case *c, const double *gq, double *fstar, double *fg)
+{
+    const int np = c->npoin, nc = c->ncell, nd = c->ndegr;
+    const double g1 = c->gamlg - 1.0, eps = 1.0e-6;
+    double *mn = (double *)calloc((size_t)4 * np, sizeof(double));
+    double *mu = (double *)calloc((size_t)2 * np, sizeof(double));
+    double *sn = (double *)calloc((size_t)2 * np, sizeof(double));
+    double *mcl = (double *)malloc(sizeof(double) * 32 * nc);
+    double *mul = (double *)malloc(sizeof(double) * 64 * nc);
+    double *snl = (double *)malloc(sizeof(double) * 64 * nc);
+    double *drho = (double *)malloc(sizeof(double) * 4 * nc);
+    for (int ie = 0; ie < nc; ++ie) {
+        const int *ip = c->ipqua + 9 * ie;
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double un[9][4];
+        for (int iv = 0; iv < 9; ++iv) {
+            double b[6];
+            basis_p2(XV9[iv], YV9[iv], rc, sc, &GE(c, 18, ie), b);
+            for (int q = 0; q < 4; ++q) un[iv][q] = 0.0;
+            for (int id = 0; id < nd; ++id)
+                for (int q = 0; q < 4; ++q) un[iv][q] = un[iv][q] + U(c, c->unold, id, q, ie) * b[id];
+        }
+        double rhomc = U(c, c->unold, 0, 0, ie);
+        double uctr = U(c, c->unold, 0, 1, ie), vctr = U(c, c->unold, 0, 2, ie), ectr = U(c, c->unold, 0, 3, ie);
+        double rhoct = 1.0 / rhomc;
+        double eintc = ectr - 0.5 * (uctr * uctr + vctr * vctr);
+        /* GetEOS (Getsolid.f90:319-397, nmatel = 1): no clamp on c^2 */
+        double pctr = dmax(eps, g1 * rhoct * eintc);
+        double sdctr = sqrt(c->gamlg * pctr / rhoct);
+        double rsub[4], uq[4];
+        subcell_density(c, ie, rsub);
+        subcell_tau(c, ie, uq);
+        const double *g = gq + 48 * ie;
+        for (int isg = 0; isg < 4; ++isg) {
+            drho[4 * ie + isg] = rsub[isg] - 1.0 / uq[isg];
+            double vn[4][2][3];
+            for (int iv = 0; iv < 4; ++iv)
+                for (int f = 0; f < 2; ++f) {
+                    int t = FNQSG[isg][2 * iv + f];
+                    int sl = abs(t) - 1;
+                    double sg = t > 0 ? 1.0 : -1.0;
+                    vn[iv][f][0] = sg * g[3 * sl];
+                    vn[iv][f][1] = sg * g[3 * sl + 1];
+                    vn[iv][f][2] = g[3 * sl + 2] * WR5[isg][iv][f];
+                }
+            for (int iv = 0; iv < 4; ++iv) {
+                int ln = IPQSG[isg][iv];
+                int p = ip[ln];
+                double rhovt = 1.0 / un[ln][0];
+                double rhovsg = rhovt + c->cdrho * (rsub[isg] - 1.0 / uq[isg]);
+                double uv = un[ln][1], vv = un[ln][2], ev = un[ln][3];
+                double eintv = ev - 0.5 * (uv * uv + vv * vv);
+                double pv = dmax(eps, g1 * rhovsg * eintv);
+                double dux = X(c->ustar, 0, p) - uv, duy = X(c->ustar, 1, p) - vv;
+                double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+                for (int f = 0; f < 2; ++f) {
+                    double deltu = c->cimpd * fabs(dux * vn[iv][f][0] + duy * vn[iv][f][1]);
+                    double mur = rhoct * sdctr + rhoct * c->slpdu * deltu;
+                    double area = vn[iv][f][2];
+                    double m11;
+                    if (a3 < 1.e12) {
+                        m11 = mur * area * fabs(1.0);
+                    } else {
+                        double nrm = sqrt(dux * dux + duy * duy);
+                        m11 = mur * area * fabs(vn[iv][f][0] * (dux / nrm) + vn[iv][f][1] * (duy / nrm));
+                    }
+                    double mu1 = m11 * uv + 0.0 * vv, mu2 = 0.0 * uv + m11 * vv;
+                    double sn1 = -pv * area * vn[iv][f][0] + 0.0 * area * vn[iv][f][1];
+                    double sn2 = 0.0 * area * vn[iv][f][0] + -pv * area * vn[iv][f][1];
+                    mn[0 + 4 * p] += m11;
+                    mn[1 + 4 * p] += 0.0;
+                    mn[2 + 4 * p] += 0.0;
+                    mn[3 + 4 * p] += m11;
+                    mu[2 * p] += mu1;
+                    mu[2 * p + 1] += mu2;
+                    sn[2 * p] += sn1;
+                    sn[2 * p + 1] += sn2;
+                    int o = f + 2 * (iv + 4 * isg);
+                    mcl[o + 32 * ie] = m11;
+                    mul[2 * o + 64 * ie] = mu1;
+                    mul[2 * o + 1 + 64 * ie] = mu2;
+                    snl[2 * o + 64 * ie] = sn1;
+                    snl[2 * o + 1 + 64 * ie] = sn2;
+                }
+            }
+        }
+    }
+    int *indnd = (int *)calloc((size_t)np, sizeof(int));
+    for (int f = 0; f < c->nbfac; ++f) {
+        const int *b = c->bface + 5 * f;
+        if (b[2] == 25) indnd[b[0]] = indnd[b[1]] = indnd[b[4]] = 1;
+    }
+    for (int p = 0; p < np; ++p) {
+        if (indnd[p]) continue;
+        double a11 = mn[4 * p], a21 = mn[4 * p + 1], a12 = mn[4 * p + 2], a22 = mn[4 * p + 3];
+        double det = a11 * a22 - a21 * a12;
+        double i11 = a22 / det, i12 = -a12 / det, i21 = -a21 / det, i22 = a11 / det;
+        double r1 = mu[2 * p] - sn[2 * p], r2 = mu[2 * p + 1] - sn[2 * p + 1];
+        X(c->ustar, 0, p) = i11 * r1 + i12 * r2;
+        X(c->ustar, 1, p) = i21 * r1 + i22 * r2;
+    }
+    for (int f = 0; f < c->nbfac; ++f) {
+        const int *b = c->bface + 5 * f;
+        int nodes[3] = {b[0], b[1], b[4]};
+        for (int k = 0; k < 3; ++k) {
+            if (b[2] == 22) {
+                if (b[3] == 221) X(c->ustar, 1, nodes[k]) = 0.0;
+                else if (b[3] == 222) X(c->ustar, 0, nodes[k]) = 0.0;
+            } else if (b[2] == 25) {
+                X(c->ustar, 1, nodes[k]) = 0.0;
+                X(c->ustar, 0, nodes[k]) = 0.0;
+            }
+        }
+    }
+    /* ufgpq: u* interpolated along each face at its two Lobatto points
+     * (:17416-17437) */
+    for (int ie = 0; ie < nc; ++ie) {
+        const int *ip = c->ipqua + 9 * ie;
+        for (int lf = 0; lf < 4; ++lf) {
+            const int pa = ip[lf], pb = ip[(lf + 1) % 4], pm = ip[4 + lf];
+            for (int k = 0; k < 2; ++k) {
+                const double r = k == 0 ? -G5A : G5A;
+                const double s1 = -0.5 * (1.0 - r) * r, s2 = 0.5 * (1.0 + r) * r, s3 = (1.0 + r) * (1.0 - r);
+                double *o = fg + 6 * (2 * lf + k) + 48 * ie;
+                for (int d = 0; d < 2; ++d)
+                    o[4 + d] = s1 * X(c->ustar, d, pa) + s2 * X(c->ustar, d, pb) + s3 * X(c->ustar, d, pm);
+            }
+        }
+    }
+    /* the face Gauss points' Riemann problems, one face at a time
+     * (getvelo_quadsubg_gs3): the cell on each side, its state at the
+     * point, the 2x2 solve and the velocity correction dufgq; then the
+     * forces of each side (:17470-17527) */
+    int *bc = (int *)malloc(sizeof(int) * 4 * nc), *rev = (int *)malloc(sizeof(int) * 4 * nc);
+    face_bc5(c, bc, rev);
+    int *nbr = (int *)malloc(sizeof(int) * 4 * nc);
+    {
+        int *mid = (int *)malloc(sizeof(int) * np);
+        for (int p = 0; p < np; ++p) mid[p] = -1;
+        for (int ie = 0; ie < nc; ++ie)
+            for (int lf = 0; lf < 4; ++lf) {
+                const int m = c->ipqua[9 * ie + 4 + lf];
+                nbr[4 * ie + lf] = -1;
+                if (mid[m] >= 0) {
+                    nbr[4 * ie + lf] = mid[m];
+                    nbr[mid[m]] = 4 * ie + lf;
+                } else {
+                    mid[m] = 4 * ie + lf;
+                }
+            }
+        free(mid);
+    }
+    for (int ie = 0; ie < nc; ++ie) {
+        const int *ip = c->ipqua + 9 * ie;
+        for (int lf = 0; lf < 4; ++lf) {
+            const int other = nbr[4 * ie + lf];
+            if (other >= 0 && other < 4 * ie + lf) continue; /* the face was done from the other side */
+            double fx[3], fy[3];
+            const int fn[3] = {ip[lf], ip[(lf + 1) % 4], ip[4 + lf]};
+            for (int i = 0; i < 3; ++i) {
+                fx[i] = X(c->cooro, 0, fn[i]);
+                fy[i] = X(c->cooro, 1, fn[i]);
+            }
+            for (int k = 0; k < 2; ++k) {
+                double n[3];
+                face_normal5(fx, fy, k, n);
+                const int gl = 2 * lf + k;
+                const double *uf = fg + 6 * gl + 48 * ie + 4;
+                double vl[7], rl[5];
+                face_state5(c, ie, gl, drho, vl);
+                if (other < 0) {
+                    /* boundary face: this side only, its normal as the
+                     * boundary face's nodes orient it */
+                    double nb[3] = {rev[4 * ie + lf] ? -n[0] : n[0], rev[4 * ie + lf] ? -n[1] : n[1], n[2]};
+                    face_riem5(c, vl, uf, nb, rl);
+                    double a11 = 0.0 + rl[0], a22 = 0.0 + rl[0], a21 = 0.0 + 0.0, a12 = 0.0 + 0.0;
+                    double m1 = 0.0 + rl[1], m2 = 0.0 + rl[2], s1 = 0.0 + rl[3], s2 = 0.0 + rl[4];
+                    double det = a11 * a22 - a21 * a12;
+                    double i11 = a22 / det, i12 = -a12 / det, i21 = -a21 / det, i22 = a11 / det;
+                    double r1 = m1 - s1 - 0.0, r2 = m2 - s2 - 0.0;
+                    double du1 = -uf[0] + (i11 * r1 + i12 * r2);
+                    double du2 = -uf[1] + (i21 * r1 + i22 * r2);
+                    const int b = bc[4 * ie + lf];
+                    if (b == 1) du2 = 0.0;
+                    else if (b == 2) du1 = 0.0;
+                    else if (b == 3) du1 = du2 = 0.0;
+                    double *o = fg + 6 * gl + 48 * ie;
+                    o[0] = rl[3] + rl[0] * uf[0] + 0.0 * uf[1] - rl[1];
+                    o[1] = rl[4] + rl[0] * uf[1] + 0.0 * uf[0] - rl[2];
+                    o[2] = rl[0] * du1 + 0.0 * du2;
+                    o[3] = rl[0] * du2 + 0.0 * du1;
+                } else {
+                    const int je = other / 4, gr = 2 * (other % 4) + (1 - k);
+                    const double *ufr = fg + 6 * gr + 48 * je + 4;
+                    double vr[7], rr[5];
+                    face_state5(c, je, gr, drho, vr);
+                    const double nr[3] = {-n[0], -n[1], n[2]};
+                    face_riem5(c, vl, ufr, n, rl);
+                    face_riem5(c, vr, ufr, nr, rr);
+                    double a11 = 0.0 + rl[0] + rr[0], a22 = 0.0 + rl[0] + rr[0];
+                    double a21 = 0.0 + 0.0 + 0.0, a12 = 0.0 + 0.0 + 0.0;
+                    double m1 = 0.0 + rl[1] + rr[1], m2 = 0.0 + rl[2] + rr[2];
+                    double s1 = 0.0 + rl[3] + rr[3], s2 = 0.0 + rl[4] + rr[4];
+                    double det = a11 * a22 - a21 * a12;
+                    double i11 = a22 / det, i12 = -a12 / det, i21 = -a21 / det, i22 = a11 / det;
+                    double r1 = m1 - s1, r2 = m2 - s2;
+                    const double *side[2] = {rl, rr};
+                    const int cell[2] = {ie, je}, gg[2] = {gl, gr};
+                    for (int sd = 0; sd < 2; ++sd) {
+                        const double *ufs = fg + 6 * gg[sd] + 48 * cell[sd] + 4;
+                        double du1 = -ufs[0] + (i11 * r1 + i12 * r2);
+                        double du2 = -ufs[1] + (i21 * r1 + i22 * r2);
+                        const double *rs = side[sd];
+                        double *o = fg + 6 * gg[sd] + 48 * cell[sd];
+                        o[0] = rs[3] + rs[0] * ufs[0] + 0.0 * ufs[1] - rs[1];
+                        o[1] = rs[4] + rs[0] * ufs[1] + 0.0 * ufs[0] - rs[2];
+                        o[2] = rs[0] * du1 + 0.0 * du2;
+                        o[3] = rs[0] * du2 + 0.0 * du1;
+                    }
+                }
+            }
+        }
+    }
+    /* the centre nodes' u* is zeroed before the vertex forces (:17475) */
+    for (int ie = 0; ie < nc; ++ie) {
+        const int p = c->ipqua[9 * ie + 8];
+        X(c->ustar, 0, p) = 0.0;
+        X(c->ustar, 1, p) = 0.0;
+    }
+    for (int ie = 0; ie < nc; ++ie) {
+        const int *ip = c->ipqua + 9 * ie;
+        for (int isg = 0; isg < 4; ++isg)
+            for (int iv = 0; iv < 4; ++iv)
+                for (int f = 0; f < 2; ++f) {
+                    int o = f + 2 * (iv + 4 * isg);
+                    int p = ip[IPQSG[isg][iv]];
+                    double m11 = mcl[o + 32 * ie];
+                    double mu1 = mul[2 * o + 64 * ie], mu2 = mul[2 * o + 1 + 64 * ie];
+                    double sn1 = snl[2 * o + 64 * ie], sn2 = snl[2 * o + 1 + 64 * ie];
+                    double us1 = X(c->ustar, 0, p), us2 = X(c->ustar, 1, p);
+                    fstar[2 * o + 64 * ie] = sn1 + m11 * us1 + 0.0 * us2 - mu1;
+                    fstar[2 * o + 1 + 64 * ie] = sn2 + 0.0 * us1 + m11 * us2 - mu2;
+                }
+    }
+    free(bc); free(rev); free(nbr); free(indnd); free(drho);
+    free(mn); free(mu); free(sn); free(mcl); free(mul); free(snl);
+}
+
+/* rhsifacedg_lagsubgq_gs (Getrhslaggauss.f90:17550-17939): vertex terms
+ * with the Lobatto-weighted corner faces, then per face the two Gauss
+ * points' forces and, in a second pass, their correction forces;
+ * overwrites rhsel */
+static void iface_q2g(orc_case *c, const double *gq, const double *fstar, const double *fg)
+{
+    double wm[4][4][2], wv[4][4][2];
+    wfg_face5(wm, wv);
+    const double wi = 0.544444444444444444444 / 2.0;
+    const int nd = c->ndegr;
+    for (int ie = 0; ie < c->ncell; ++ie) {
+        const int *ip = c->ipqua + 9 * ie;
+        double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
+        double bq[9][6];
+        for (int iv = 0; iv < 9; ++iv) basis_p2(XV9[iv], YV9[iv], rc, sc, &GE(c, 18, ie), bq[iv]);
+        double ul[6] = {0}, p1[6] = {0}, p2[6] = {0}, el[6] = {0};
+        const double *g = gq + 48 * ie;
+        for (int isg = 0; isg < 4; ++isg) {
+            double vn[4][2][3];
+            for (int iv = 0; iv < 4; ++iv)
+                for (int f = 0; f < 2; ++f) {
+                    int t = FNQSG[isg][2 * iv + f];
+                    int sl = abs(t) - 1;
+                    double sg = t > 0 ? 1.0 : -1.0;
+                    vn[iv][f][0] = sg * g[3 * sl];
+                    vn[iv][f][1] = sg * g[3 * sl + 1];
+                    vn[iv][f][2] = g[3 * sl + 2] * wv[isg][iv][f];
+                }
+            for (int iv = 0; iv < 4; ++iv) {
+                int ln = IPQSG[isg][iv];
+                int p = ip[ln];
+                double us1 = X(c->ustar, 0, p), us2 = X(c->ustar, 1, p);
+                int o0 = 0 + 2 * (iv + 4 * isg), o1 = 1 + 2 * (iv + 4 * isg);
+                const double *f0 = fstar + 2 * o0 + 64 * ie, *f1 = fstar + 2 * o1 + 64 * ie;
+                double w1 = wm[isg][iv][0], w2 = wm[isg][iv][1];
+                for (int id = 0; id < nd; ++id) {
+                    double b = bq[ln][id];
+                    ul[id] = ul[id] + us1 * vn[iv][0][0] * vn[iv][0][2] * b + us2 * vn[iv][0][1] * vn[iv][0][2] * b +
+                             us1 * vn[iv][1][0] * vn[iv][1][2] * b + us2 * vn[iv][1][1] * vn[iv][1][2] * b;
+                    p1[id] = p1[id] + f0[0] * b * w1 + f1[0] * b * w2;
+                    p2[id] = p2[id] + f0[1] * b * w1 + f1[1] * b * w2;
+                    el[id] = el[id] + us1 * f0[0] * b * w1 + us2 * f0[1] * b * w1 + us1 * f1[0] * b * w2 +
+                             us2 * f1[1] * b * w2;
+                }
+            }
+        }
+        for (int lf = 0; lf < 4; ++lf) {
+            double fx[3], fy[3];
+            const int fn[3] = {ip[lf], ip[(lf + 1) % 4], ip[4 + lf]};
+            for (int i = 0; i < 3; ++i) {
+                fx[i] = X(c->cooro, 0, fn[i]);
+                fy[i] = X(c->cooro, 1, fn[i]);
+            }
+            double bg[2][6], n[2][3];
+            for (int k = 0; k < 2; ++k) {
+                const int gl = 2 * lf + k;
+                basis_p2(PQG5[gl][0], PQG5[gl][1], rc, sc, &GE(c, 18, ie), bg[k]);
+                face_normal5(fx, fy, k, n[k]);
+            }
+            for (int k = 0; k < 2; ++k) {
+                const double *o = fg + 6 * (2 * lf + k) + 48 * ie;
+                const double uf1 = o[4], uf2 = o[5];
+                for (int id = 0; id < nd; ++id) {
+                    const double b = bg[k][id];
+                    ul[id] = ul[id] + uf1 * n[k][0] * n[k][2] * b * wi + uf2 * n[k][1] * n[k][2] * b * wi;
+                    p1[id] = p1[id] + o[0] * b * wi;
+                    p2[id] = p2[id] + o[1] * b * wi;
+                    el[id] = el[id] + uf1 * o[0] * b * wi + uf2 * o[1] * b * wi;
+                }
+            }
+            for (int k = 0; k < 2; ++k) {
+                const double *o = fg + 6 * (2 * lf + k) + 48 * ie;
+                const double uf1 = o[4], uf2 = o[5];
+                for (int id = 0; id < nd; ++id) {
+                    const double b = bg[k][id];
+                    p1[id] = p1[id] + o[2] * b * wi;
+                    p2[id] = p2[id] + o[3] * b * wi;
+                    el[id] = el[id] + uf1 * o[2] * b * wi + uf2 * o[3] * b * wi;
+                }
+            }
+        }
+        for (int id = 0; id < nd; ++id) {
+            U(c, c->rhsel, id, 0, ie) = ul[id];
+            U(c, c->rhsel, id, 1, ie) = p1[id];
+            U(c, c->rhsel, id, 2, ie) = p2[id];
+            U(c, c->rhsel, id, 3, ie) = el[id];
+        }
+    }
+}
+
+/* the nfint=5 right-hand side: geometry, the Gauss-point nodal solver, the
+ * face integral, the Lobatto domain integral (unlimited: geoel(10) = 0) and
+ * the identity derivative update */
+static int rhs_q2g(orc_case *c)
+{
+    const int nc = c->ncell;
+    double *gq = (double *)malloc(sizeof(double) * 48 * nc);
+    double *fstar = (double *)malloc(sizeof(double) * 64 * nc);
+    double *fg = (double *)malloc(sizeof(double) * 48 * nc);
+    fnds_q2(c, c->cooro, gq);
+    for (int ie = 0; ie < nc; ++ie) {
+        GE(c, 9, ie) = 0.0;
+        if (c->troubled) c->troubled[ie] = 0.0;
+        for (int q = 0; q < 5; ++q) c->aflim[q + 5 * ie] = 1.0;
+        c->afvec[4 * ie + 0] = 1.0;
+        c->afvec[4 * ie + 1] = 0.0;
+        c->afvec[4 * ie + 2] = 0.0;
+        c->afvec[4 * ie + 3] = 1.0;
+    }
+    ndvelo_q2g(c, gq, fstar, fg);
+    iface_q2g(c, gq, fstar, fg);
+    domn_q2(c);
+    /* derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7809-7823) */
+    for (int ie = 0; ie < nc; ++ie) {
+        U(c, c->unold, 1, 0, ie) = U(c, c->unold, 1, 0, ie) * 1.0;
+        U(c, c->unold, 2, 0, ie) = U(c, c->unold, 2, 0, ie) * 1.0;
+        double dudr = 1.0 * U(c, c->unold, 1, 1, ie) + 0.0 * U(c, c->unold, 1, 2, ie);
+        double duds = 1.0 * U(c, c->unold, 2, 1, ie) + 0.0 * U(c, c->unold, 2, 2, ie);
+        double dvdr = 0.0 * U(c, c->unold, 1, 1, ie) + 1.0 * U(c, c->unold, 1, 2, ie);
+        double dvds = 0.0 * U(c, c->unold, 2, 1, ie) + 1.0 * U(c, c->unold, 2, 2, ie);
+        U(c, c->unold, 1, 1, ie) = dudr;
+        U(c, c->unold, 2, 1, ie) = duds;
+        U(c, c->unold, 1, 2, ie) = dvdr;
+        U(c, c->unold, 2, 2, ie) = dvds;
+        U(c, c->unold, 1, 3, ie) = U(c, c->unold, 1, 3, ie) * 1.0;
+        U(c, c->unold, 2, 3, ie) = U(c, c->unold, 2, 3, ie) * 1.0;
+    }
+    free(gq); free(fstar); free(fg);
+    return 0;
+}
+
 int orc_rhs_q2(orc_case *c)
 {
     if (!((c->npoly == 2 && c->ndegr == 6) || (c->npoly == 1 && c->ndegr == 3)) || c->nrz != 0 ||
         c->ndens != 1 || c->nlimi != 6 || !c->esqua)
         return 2;
+    if (c->nfint == 5) return c->npoly == 2 ? rhs_q2g(c) : 2;
     const int nc = c->ncell;
     double *gq = (double *)malloc(sizeof(double) * 48 * nc);
     double *flag = (double *)malloc(sizeof(double) * nc);

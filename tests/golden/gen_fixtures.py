@@ -72,6 +72,15 @@ def _sod_annulus_p1():
     return mg.sod_annulus_p1_problem(6, 24, seed=17), "rotation"
 
 
+def _gauss(builder):
+    """The same problem on the nfint=5 branch (vertex and non-vertex face
+    Gauss points, the 5-point Lobatto face rule)."""
+    def b():
+        (mesh, ctl), ic = builder()
+        return (mesh, dataclasses.replace(ctl, nfint=5, ngausf=5)), ic
+    return b
+
+
 SPECS = {
     # name: (builder, steps, dump-every)
     "sod_q1_100x4": (_sod, 400, 25),
@@ -84,6 +93,8 @@ SPECS = {
     "sod_rz_40x8": (_sod_rz_hybrid, 120, 30),
     "sod_p1c_annulus": (_sod_annulus_p1, 20, 5),
     "sedov_q2w_16x16": (_sedov_q2_wall, 60, 10),
+    "sedov_q2g_16x16": (_gauss(_sedov_q2_centre), 30, 10),
+    "sedov_q2gw_16x16": (_gauss(_sedov_q2_wall), 8, 2),
 }
 
 
