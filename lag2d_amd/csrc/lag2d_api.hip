@@ -88,10 +88,23 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     const lag2d_config &c = *cfg;
     const bool q1 = c.ncurv == 0 && c.nfint == 1 && c.npoly == 1 && c.ndegr == 3;
     // the curved sub-cell branch (Getrhslagmc.f90:7556-7708), P2 or P1
-    const bool q2 = c.ncurv == 1 && c.nfint == 4 &&
-                    ((c.npoly == 2 && c.ndegr == 6) || (c.npoly == 1 && c.ndegr == 3));
+    const bool q2 = c.ncurv == 1 && ((c.nfint == 4 && ((c.npoly == 2 && c.ndegr == 6) || (c.npoly == 1 && c.ndegr == 3))) ||
+                                     (c.nfint == 5 && c.npoly == 2 && c.ndegr == 6));
     if (!(q1 || q2))
-        return fail(LAG2D_EUNSUPPORTED, "supported: ncurv=0/nfint=1/P1 or ncurv=1/nfint=4/P1 or P2");
+        return fail(LAG2D_EUNSUPPORTED, "supported: ncurv=0/nfint=1/P1, ncurv=1/nfint=4/P1 or P2, ncurv=1/nfint=5/P2");
+    if (c.nfint == 5) {
+        // the branch's face Gauss points need the 5-point Lobatto face rule
+        // (ngausf = 5, implied); ncase 2, 3, 4, 13 change its boundary
+        // treatment or its first cell (Getrhslagmc.f90:7717-7730,
+        // Getrhslaggauss.f90:17255-17261, GetBClag.f90:556-624), and only
+        // slip and fixed walls are implemented
+        if (c.ncase == 2 || c.ncase == 3 || c.ncase == 4 || c.ncase == 13 || c.ncase == -13)
+            return fail(LAG2D_EUNSUPPORTED, "nfint=5: ncase 2, 3, 4, 13, -13 are not implemented");
+        for (int f = 0; f < nbfac; ++f)
+            if (bface[5 * (size_t)f + 2] != 22 && bface[5 * (size_t)f + 2] != 25)
+                return fail(LAG2D_EUNSUPPORTED, "nfint=5: slip walls (code 22) and fixed walls (25) only");
+        if (rs.nrun > 1) return fail(LAG2D_EUNSUPPORTED, "nfint=5: ensembles run the nfint=4 branch");
+    }
     if ((c.nrz != 0 && c.nrz != 2) || c.ndens != 1 || c.nlimi != 6 || c.nriem != 1)
         return fail(LAG2D_EUNSUPPORTED, "supported: nrz=0 or 2, ndens=1, nlimi=6, nriem=1");
     if (c.nrz == 2) {
@@ -191,6 +204,8 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
             for (int k = 0; k < (nv == 9 ? 3 : 2); ++k) nflag[nodes[k]] |= 4;
         }
     }
+    if (c.nfint == 5)  // cell-centre nodes: u* = 0 after the solve (Getrhslaggauss.f90:17475)
+        for (int ie = 0; ie < nc; ++ie) nflag[ipq[(size_t)8 * nc + ie]] |= 16;
     if (c.ncase != 1) {  // Limiterlag.f90:8212-8238
         const int nf = nv == 9 ? 3 : 2;
         for (int f = 0; f < nbfac; ++f) {
@@ -314,8 +329,44 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         d.weno_strict = c.strict;
         if (hipMemcpy(t_esq, esq.data(), esq.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemset(d.tcount, 0, 2 * sizeof(int)) != hipSuccess ||
-            hipMemset(d.tsum, 0, sizeof(unsigned long long)) != hipSuccess)
+            hipMemset(d.tsum, 0, sizeof(unsigned long long)) != hipSuccess ||
+            hipMemset(d.flag, 0, (size_t)nc) != hipSuccess)
             return bad(fail(LAG2D_EDEVICE, "curved-path table upload"));
+        d.nfint = c.nfint;
+        if (c.nfint == 5) {
+            // the other side of every cell face: the neighbour sharing the
+            // face's mid-node, or the boundary face holding it
+            std::vector<int> fnb((size_t)4 * nc, -1), mid(npoin, -1);
+            for (int ie = 0; ie < nc; ++ie)
+                for (int lf = 0; lf < 4; ++lf) {
+                    const int m = ipq[(size_t)(4 + lf) * nc + ie], me = 4 * ie + lf;
+                    if (mid[m] < 0) {
+                        mid[m] = me;
+                        continue;
+                    }
+                    const int o = mid[m];
+                    if (fnb[(size_t)(o & 3) * nc + (o >> 2)] != -1)
+                        return bad(fail(LAG2D_EINVAL, "nfint=5: a mid-node shared by more than two cells"));
+                    fnb[(size_t)lf * nc + ie] = o;
+                    fnb[(size_t)(o & 3) * nc + (o >> 2)] = me;
+                }
+            for (int f = 0; f < nbfac; ++f) {
+                const int *b = bface + 5 * (size_t)f;
+                const int o = mid[b[4] - 1];
+                if (o < 0 || fnb[(size_t)(o & 3) * nc + (o >> 2)] >= 0)
+                    return bad(fail(LAG2D_EINVAL, "nfint=5: a boundary face's mid-node is not on a single cell face"));
+                const int bc = b[2] == 25 ? 3 : (b[3] == 221 ? 1 : (b[3] == 222 ? 2 : 0));
+                const int rev = (b[0] - 1) != ipq[(size_t)(o & 3) * nc + (o >> 2)];
+                fnb[(size_t)(o & 3) * nc + (o >> 2)] = -(1 + bc + 4 * rev);
+            }
+            int *t_fnb;
+            if ((rc = x->alloc(&d.drho, (size_t)4 * nc)) || (rc = x->alloc(&d.fg, (size_t)48 * nc)) ||
+                (rc = x->alloc(&t_fnb, fnb.size())))
+                return bad(rc);
+            d.fnb = t_fnb;
+            if (hipMemcpy(t_fnb, fnb.data(), fnb.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+                return bad(fail(LAG2D_EDEVICE, "face table upload"));
+        }
     }
     if (c.nrz == 2) {
         int *t_wall;
@@ -369,6 +420,10 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     }
     if (q2 && c.npoly == 1) {  // curved P1 has one corner and one cell-RHS form
         x->corner_variant = 8;
+        x->rhs_variant = 2;
+    }
+    if (q2 && c.nfint == 5) {  // the nfint=5 branch: a lane per sub-cell, a thread per cell
+        x->corner_variant = 2;
         x->rhs_variant = 2;
     }
     *out = x;
@@ -554,7 +609,7 @@ const char *const lag2d_kernel_names[KID_COUNT] = {
     "k_dt_final",   "k_update",     "k_nodemove",        "k_cellvol",
     "k_q2_prep",    "k_q2_weno",    "k_q2_corner",       "k_node_q2", "k_cellrhs_q2", "k_run_flags",
     "k_limit_corner_rz", "k_node_rz", "k_cellrhs_rz",    "k_schedule",
-    "k_adj_cellrhs", "k_adj_corner", "k_adj_node+gather+nodemove", "k_limit_corner_q1<frozen>"};
+    "k_adj_cellrhs", "k_adj_corner", "k_adj_node+gather+nodemove", "k_limit_corner_q1<frozen>", "k_q2g_face"};
 
 static const double RKLAG[2][3] = {{0.0, 0.5, 0.0}, {0.0, 0.75, 0.3333333333333333}};
 

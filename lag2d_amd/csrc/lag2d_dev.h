@@ -115,6 +115,16 @@ struct Dev {
     // |du.n| live in dlt [32][nc]
     double *wdu;
     uint8_t *ccl, *dcl;
+    // the nfint=5 branch (face Gauss points): nfint, the sub-cell density
+    // corrections of the corner solve [4][nc], the face kernel's outputs
+    // [48][nc] (FG5 in lag2d_q2geom.h), and each cell face's other side
+    // [4][nc]: neighbour * 4 + its local face, or -(1 + bc + 4 rev) on a
+    // boundary (bc 0 free, 1 v = 0, 2 u = 0, 3 u = v = 0; rev: the boundary
+    // face's nodes run clockwise round the cell)
+    int nfint;
+    double *drho;
+    double *fg;
+    const int *fnb;
 };
 
 // Quotients a / b by one divisor b, bit-identical to the IEEE division.

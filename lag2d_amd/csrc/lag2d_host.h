@@ -15,7 +15,7 @@ enum {
     KID_DT_FINAL, KID_UPDATE, KID_NODEMOVE, KID_CELLVOL,
     KID_Q2_PREP, KID_Q2_WENO, KID_Q2_CORNER, KID_NODE_Q2, KID_CELLRHS_Q2, KID_RUN_FLAGS,
     KID_LIMIT_RZ, KID_NODE_RZ, KID_CELLRHS_RZ, KID_SCHEDULE,
-    KID_ADJ_CELLRHS, KID_ADJ_CORNER, KID_ADJ_NODES, KID_LIMIT_FROZEN, KID_COUNT
+    KID_ADJ_CELLRHS, KID_ADJ_CORNER, KID_ADJ_NODES, KID_LIMIT_FROZEN, KID_Q2G_FACE, KID_COUNT
 };
 extern const char *const lag2d_kernel_names[KID_COUNT];
 

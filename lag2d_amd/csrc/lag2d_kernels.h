@@ -32,6 +32,9 @@ __global__ void k_node_rz(Dev d);
 __global__ void k_cellrhs_rz(Dev d, const double *__restrict__ Uin, double alfa, int apply_update, double epsaw);
 __global__ void k_schedule(Dev d, double dt);
 
+// curved P2 sub-cell path, nfint=5: the face Gauss points' Riemann problems
+__global__ void k_q2g_face(Dev d);
+
 // discrete adjoint of the straight P1 planar step (quads)
 __global__ void k_adj_nodemove(Dev d, AdjStage a, double alfa);
 __global__ void k_adj_cellrhs(Dev d, AdjStage a, double alfa);

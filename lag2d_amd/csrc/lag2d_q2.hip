@@ -1231,9 +1231,15 @@ __device__ __forceinline__ double clamp_eps(double eps, double v, uint8_t *tape,
     }
 }
 
-template <int ND, int MODE = 0, int CPB = CSC_CELLS, int STAGE = 1>
+// NF = 5: the nfint=5 branch's corners (getriem_quadsubg_gauss,
+// Getrhslaggauss.f90:16657-17157): its corner weights WR5, the sound speed
+// of GetEOS without a clamp on c^2 (Getsolid.f90:319-397), the impedance
+// rho c + rho slpdu (cimpd |du.n|), and the sub-cell density correction
+// drhosgq stored for the face Gauss points.
+template <int ND, int MODE = 0, int CPB = CSC_CELLS, int STAGE = 1, int NF = 4>
 __global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
 {
+    static_assert(NF == 4 || MODE == 0, "the adjoint's recorded forward runs the nfint=4 branch");
     __shared__ double stab_lds[STAGE ? 125 * SHP_STRIDE : 1];
     __shared__ double sgq[CPB][16 * 3 + 1];
     if constexpr (STAGE)
@@ -1354,7 +1360,13 @@ __global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
     const size_t co = (MODE == 1 && isg != 0) ? 0 : (size_t)ie;
     const size_t cs = (MODE == 1 && isg != 0) ? 1 : (size_t)nc;
     const double pctr = clamp_eps<MODE>(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)), ctape, co);
-    const double sdctr = sqrt(clamp_eps<MODE>(eps, d.gamlg * pctr / rhoct, ctape, co + cs));
+    double sdctr;
+    if constexpr (NF == 5) {
+        sdctr = sqrt(d.gamlg * pctr / rhoct);
+        d.drho[isg * nc + ie] = rsub - 1.0 / uq;
+    } else {
+        sdctr = sqrt(clamp_eps<MODE>(eps, d.gamlg * pctr / rhoct, ctape, co + cs));
+    }
     const double dcor = d.cdrho * (rsub - 1.0 / uq);
     for (int iv = 0; iv < 4; ++iv) {
         // ---- sub-cell corner (isg, iv)
@@ -1383,7 +1395,7 @@ __global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
             const int sl = (tt > 0 ? tt : -tt) - 1;
             const double sg = tt > 0 ? 1.0 : -1.0;
             const double nx = sg * sgq[cl][3 * sl + 0], ny = sg * sgq[cl][3 * sl + 1];
-            const double area = sgq[cl][3 * sl + 2] * WFR[isg][iv][f];
+            const double area = sgq[cl][3 * sl + 2] * wfr<NF>(isg, iv, f);
             double deltu;
             const size_t od = (size_t)(f + 2 * (iv + 4 * isg)) * nc + ie;
             if constexpr (MODE == 2) {
@@ -1392,7 +1404,11 @@ __global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
                 deltu = fabs(dux * nx + duy * ny);
                 if constexpr (MODE == 1) d.dlt[od] = deltu;
             }
-            const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
+            double mur;
+            if constexpr (NF == 5)
+                mur = rhoct * sdctr + rhoct * d.slpdu * (d.cimpd * deltu);
+            else
+                mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
             double m11;
             if (a3 < 1.e12) {
                 m11 = mur * area * fabs(1.0);
@@ -1775,6 +1791,11 @@ __global__ void __launch_bounds__(8 * CWU_CELLS) k_q2_corner_wu2(Dev d)
 // Nodal solve over the sub-cell corners: contributions gathered in the
 // reference's accumulation order (cell, sub-cell, vertex, face); walls on
 // all three face nodes (Getrhslaggauss.f90:4494-4545).
+// NF = 5 (getndvelo_lagsubg_gaussh, Getrhslaggauss.f90:17161-17546): the
+// corner weights WR5, and the cell-centre nodes' u* set to zero (:17475);
+// their solve divides by a zero impedance sum, as the centre's corner faces
+// carry zero weight.
+template <int NF = 4>
 __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1790,7 +1811,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
             double nx, ny, ga, m1, m2, s1, s2;
             corner_face_geom(d.gq, nc, ie, isg, iv, f, nx, ny, ga);
             const double m = d.rec[RC2(ie, slot, f)];
-            corner_face_rec(m, uv, vv, pv, nx, ny, ga * WFR[isg][iv][f], m1, m2, s1, s2);
+            corner_face_rec(m, uv, vv, pv, nx, ny, ga * wfr<NF>(isg, iv, f), m1, m2, s1, s2);
             a11 = a11 + m;
             a21 = a21 + 0.0;
             a12 = a12 + 0.0;
@@ -1812,6 +1833,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
     }
     if (fl & 1) uy = 0.0;
     if (fl & 2) ux = 0.0;
+    if (NF == 5 && (fl & 16)) ux = uy = 0.0;
     d.usx[p] = ux;
     d.usy[p] = uy;
 }
@@ -2047,7 +2069,11 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
 // Registers are capped for 2 waves per SIMD: uncapped, the P2 form takes
 // 266 (one wave per SIMD), and the 44 bytes it then spills cost far less
 // than the lost latency hiding.
-template <int ND, int MODE = 0, int BS = 256>
+// NF = 5 (rhsifacedg_lagsubgq_gs, Getrhslaggauss.f90:17550-17939): the
+// corner faces weighted by WR5 (records) and WM5 / WV5 (integral), then per
+// face its two Gauss points' forces and, in a second pass, their correction
+// forces, from the face kernel's outputs.
+template <int ND, int MODE = 0, int BS = 256, int NF = 4>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2)))
 k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
 {
@@ -2076,13 +2102,13 @@ k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
         for (int f = 0; f < 2; ++f) {
             double ga, m1, m2, s1, s2;
             corner_face_geom(d.gq, nc, ie, isg, iv, f, nx[f], ny[f], ga);
-            ar[f] = ga * WFV[isg][iv][f];
+            ar[f] = ga * wfv<NF>(isg, iv, f);
             const double m11 = d.rec[RC2(ie, k, f)];
-            corner_face_rec(m11, uv, vv, pv, nx[f], ny[f], ga * WFR[isg][iv][f], m1, m2, s1, s2);
+            corner_face_rec(m11, uv, vv, pv, nx[f], ny[f], ga * wfr<NF>(isg, iv, f), m1, m2, s1, s2);
             fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
             fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
         }
-        const double w1 = WFM[isg][iv][0], w2 = WFM[isg][iv][1];
+        const double w1 = wfm<NF>(isg, iv, 0), w2 = wfm<NF>(isg, iv, 1);
         double bq[6];
         basis6(XV9[ln], YV9[ln], rc, sc, gg0, gg1, gg2, bq);
         for (int id = 0; id < ND; ++id) {
@@ -2093,6 +2119,46 @@ k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
             r[2][id] = r[2][id] + fy[0] * b * w1 + fy[1] * b * w2;
             r[3][id] = r[3][id] + us1 * fx[0] * b * w1 + us2 * fy[0] * b * w1 + us1 * fx[1] * b * w2 +
                        us2 * fy[1] * b * w2;
+        }
+    }
+    if constexpr (NF == 5) {
+        // ---- face Gauss points: per face the forces of both points, then
+        //      their correction forces, each weighted by w2/2
+        const double wi = LW2 / 2.0;
+#pragma unroll 1
+        for (int lf = 0; lf < 4; ++lf) {
+            const int pn[3] = {d.ipq[lf * nc + ie], d.ipq[((lf + 1) & 3) * nc + ie], d.ipq[(4 + lf) * nc + ie]};
+            const double fxn[3] = {d.xs[pn[0]], d.xs[pn[1]], d.xs[pn[2]]};
+            const double fyn[3] = {d.ys[pn[0]], d.ys[pn[1]], d.ys[pn[2]]};
+            double bg[2][6], gnx[2], gny[2], gsa[2];
+            for (int k = 0; k < 2; ++k) {
+                const int g = 2 * lf + k;
+                basis6(PQG5[g][0], PQG5[g][1], rc, sc, gg0, gg1, gg2, bg[k]);
+                face_normal5(fxn, fyn, k, gnx[k], gny[k], gsa[k]);
+            }
+            for (int k = 0; k < 2; ++k) {
+                const int g = 2 * lf + k;
+                const double f1 = d.fg[FG5(g, 0) * nc + ie], f2 = d.fg[FG5(g, 1) * nc + ie];
+                const double uf1 = d.fg[FG5(g, 4) * nc + ie], uf2 = d.fg[FG5(g, 5) * nc + ie];
+                for (int id = 0; id < ND; ++id) {
+                    const double b = bg[k][id];
+                    r[0][id] = r[0][id] + uf1 * gnx[k] * gsa[k] * b * wi + uf2 * gny[k] * gsa[k] * b * wi;
+                    r[1][id] = r[1][id] + f1 * b * wi;
+                    r[2][id] = r[2][id] + f2 * b * wi;
+                    r[3][id] = r[3][id] + uf1 * f1 * b * wi + uf2 * f2 * b * wi;
+                }
+            }
+            for (int k = 0; k < 2; ++k) {
+                const int g = 2 * lf + k;
+                const double c1 = d.fg[FG5(g, 2) * nc + ie], c2 = d.fg[FG5(g, 3) * nc + ie];
+                const double uf1 = d.fg[FG5(g, 4) * nc + ie], uf2 = d.fg[FG5(g, 5) * nc + ie];
+                for (int id = 0; id < ND; ++id) {
+                    const double b = bg[k][id];
+                    r[1][id] = r[1][id] + c1 * b * wi;
+                    r[2][id] = r[2][id] + c2 * b * wi;
+                    r[3][id] = r[3][id] + uf1 * c1 * b * wi + uf2 * c2 * b * wi;
+                }
+            }
         }
     }
     // ---- domain integral (5x5 Gauss-Lobatto, djaco = w, cofactor form);
@@ -2856,10 +2922,30 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
         hipLaunchKernelGGL((k_q2_corner_wu<3, 0, 3>), dim3((nc + CWU_CELLS - 1) / CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds);
         x->toc(KID_Q2_CORNER);
         x->tic(KID_NODE_Q2);
-        hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+        hipLaunchKernelGGL(k_node_q2<4>, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
         x->toc(KID_NODE_Q2);
         x->tic(KID_CELLRHS_Q2);
         hipLaunchKernelGGL(k_cellrhs_q2_tpc<3>, dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
+        x->toc(KID_CELLRHS_Q2);
+        return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
+    }
+    if (x->cfg.nfint == 5) {
+        // no limiting on this branch (the Persson flag is cleared before
+        // the limiter reads it, Getrhslagmc.f90:7763-7766); Gauss-point
+        // corners, the node solve, the face points, the face and domain
+        // integrals
+        x->tic(KID_Q2_CORNER);
+        hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 64, 1, 5>), dim3((nc + 63) / 64), dim3(256), 0, x->stream, ds);
+        x->toc(KID_Q2_CORNER);
+        x->tic(KID_NODE_Q2);
+        hipLaunchKernelGGL(k_node_q2<5>, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+        x->toc(KID_NODE_Q2);
+        x->tic(KID_Q2G_FACE);
+        hipLaunchKernelGGL(k_q2g_face, dim3(nblk(nc), 8), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+        x->toc(KID_Q2G_FACE);
+        x->tic(KID_CELLRHS_Q2);
+        hipLaunchKernelGGL((k_cellrhs_q2_tpc<6, 0, 256, 5>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa,
+                           apply_update);
         x->toc(KID_CELLRHS_Q2);
         return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
     }
@@ -2877,7 +2963,7 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
     launch_corner(x, ds, x->corner_variant);
     x->toc(KID_Q2_CORNER);
     x->tic(KID_NODE_Q2);
-    hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+    hipLaunchKernelGGL(k_node_q2<4>, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
     x->toc(KID_NODE_Q2);
     if (x->rhs_variant == 0) {
         int rc = tune_cellrhs(x, ds, alfa, apply_update);
@@ -2909,12 +2995,12 @@ static int launch_rhs_q2_adj_nd(lag2d_ctx *x, Dev &ds, const double *Uin, double
         }
         hipLaunchKernelGGL((k_q2_weno_frozen<ND, 1>), dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
         hipLaunchKernelGGL((k_q2_corner_wu<ND, 1, 3>), dim3((nc + CWU_CELLS - 1) / CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds);
-        hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+        hipLaunchKernelGGL(k_node_q2<4>, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
         hipLaunchKernelGGL((k_cellrhs_q2_tpc<ND, 1>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
     } else {
         hipLaunchKernelGGL((k_q2_weno_frozen<ND, 0>), dim3(nblk(nc)), dim3(LAG2D_BLOCK), 0, x->stream, ds, Uin);
         hipLaunchKernelGGL((k_q2_corner_wu<ND, 2, 3>), dim3((nc + CWU_CELLS - 1) / CWU_CELLS), dim3(4 * CWU_CELLS), 0, x->stream, ds);
-        hipLaunchKernelGGL(k_node_q2, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
+        hipLaunchKernelGGL(k_node_q2<4>, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
         hipLaunchKernelGGL((k_cellrhs_q2_tpc<ND, 2>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa, apply_update);
     }
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;

@@ -47,6 +47,65 @@ static constexpr double WFV[4][4][2] = {
     {{0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}, {0.0, 0.0}},
     {{0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.0}},
     {{0.25 * (4.0 / 3.0), 0.0}, {0.0, 0.0}, {0.0, 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}}};
+// The nfint=5 branch (vertex and non-vertex face Gauss points):
+// Riemann corner weights (getriem_quadsubg_gauss, Getrhslaggauss.f90:16777-16796)
+static constexpr double WR5[4][4][2] = {{{1, 1}, {.5, 1}, {0, 0}, {1, .5}},
+                                        {{1, .5}, {1, 1}, {.5, 1}, {0, 0}},
+                                        {{0, 0}, {1, .5}, {1, 1}, {.5, 1}},
+                                        {{.5, 1}, {0, 0}, {1, .5}, {1, 1}}};
+// face-integral weights of rhsifacedg_lagsubgq_gs (:17651-17700): halved
+// 5-point Lobatto weights, momentum/energy with w3/1 and volume with w3/2
+// at the faces' ends on the cell edges
+constexpr double LW1 = 0.1, LW2 = 0.544444444444444444444, LW3 = 0.7111111111111111111111;
+static constexpr double WM5[4][4][2] = {
+    {{0.5 * LW1, 0.5 * LW1}, {0.5 * (LW3 / 1.0), 0.5 * LW3}, {0.0, 0.0}, {0.5 * LW3, 0.5 * (LW3 / 1.0)}},
+    {{0.5 * LW3, 0.5 * (LW3 / 1.0)}, {0.5 * LW1, 0.5 * LW1}, {0.5 * (LW3 / 1.0), 0.5 * LW3}, {0.0, 0.0}},
+    {{0.0, 0.0}, {0.5 * LW3, 0.5 * (LW3 / 1.0)}, {0.5 * LW1, 0.5 * LW1}, {0.5 * (LW3 / 1.0), 0.5 * LW3}},
+    {{0.5 * (LW3 / 1.0), 0.5 * LW3}, {0.0, 0.0}, {0.5 * LW3, 0.5 * (LW3 / 1.0)}, {0.5 * LW1, 0.5 * LW1}}};
+static constexpr double WV5[4][4][2] = {
+    {{0.5 * LW1, 0.5 * LW1}, {0.5 * (LW3 / 2.0), 0.5 * LW3}, {0.0, 0.0}, {0.5 * LW3, 0.5 * (LW3 / 2.0)}},
+    {{0.5 * LW3, 0.5 * (LW3 / 2.0)}, {0.5 * LW1, 0.5 * LW1}, {0.5 * (LW3 / 2.0), 0.5 * LW3}, {0.0, 0.0}},
+    {{0.0, 0.0}, {0.5 * LW3, 0.5 * (LW3 / 2.0)}, {0.5 * LW1, 0.5 * LW1}, {0.5 * (LW3 / 2.0), 0.5 * LW3}},
+    {{0.5 * (LW3 / 2.0), 0.5 * LW3}, {0.0, 0.0}, {0.5 * LW3, 0.5 * (LW3 / 2.0)}, {0.5 * LW1, 0.5 * LW1}}};
+// the weight tables of branch NF (4: nfint=4, 5: nfint=5)
+template <int NF>
+__device__ __forceinline__ double wfr(int isg, int iv, int f) { return NF == 5 ? WR5[isg][iv][f] : WFR[isg][iv][f]; }
+template <int NF>
+__device__ __forceinline__ double wfm(int isg, int iv, int f) { return NF == 5 ? WM5[isg][iv][f] : WFM[isg][iv][f]; }
+template <int NF>
+__device__ __forceinline__ double wfv(int isg, int iv, int f) { return NF == 5 ? WV5[isg][iv][f] : WFV[isg][iv][f]; }
+// face Lobatto point +-sqrt(3/7) as the reference writes it, and the
+// cell-local face Gauss points g = 2 lf + k of face lf (corner lf -> lf+1),
+// k = 0 at r = -G5A (posiq 13..20, :19443-19455), with their sub-cells
+// (sglgq, :18217-18220)
+constexpr double G5A = 0.6546536707079771437983;
+static constexpr double PQG5[8][2] = {{-G5A, -1.0}, {G5A, -1.0}, {1.0, -G5A}, {1.0, G5A},
+                                      {G5A, 1.0},  {-G5A, 1.0}, {-1.0, G5A}, {-1.0, -G5A}};
+static constexpr int SGG5[8] = {0, 1, 1, 2, 2, 3, 3, 0};
+// per cell and face Gauss point the face kernel's outputs, fg[(6 g + j) nc + ie]:
+// j = 0, 1 the force, 2, 3 the correction force, 4, 5 the face velocity
+#define FG5(g, j) ((6 * (g) + (j)))
+
+// getvectf (Getrhslaggauss.f90:16547-16590): unit normal (dy/dr, -dx/dr)
+// and 2 |dx/dr| of the quadratic face (corner a, corner b, mid-node) at its
+// Lobatto point k
+__device__ __forceinline__ void face_normal5(const double fx[3], const double fy[3], int k, double &nx, double &ny,
+                                             double &ga)
+{
+    const double r = k == 0 ? -G5A : G5A;
+    const double dd[3] = {-0.5 + r, 0.5 + r, -2.0 * r};
+    double dxdr = 0.0, dydr = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        dxdr = dxdr + dd[i] * fx[i];
+        dydr = dydr + dd[i] * fy[i];
+    }
+    const double djaco = sqrt(dxdr * dxdr + dydr * dydr);
+    const double anx = dydr, any = -dxdr;
+    nx = anx / sqrt(anx * anx + any * any);
+    ny = any / sqrt(anx * anx + any * any);
+    ga = djaco * 2.0;
+}
+
 // 5x5 Gauss rule of ruqope (dgflo2d.f90:2734-2743 ordering)
 static constexpr double PG5[5] = {-0.9061798459386639927976269, -0.5384693101056830910363144, 0.0,
                                            0.5384693101056830910363144, 0.9061798459386639927976269};

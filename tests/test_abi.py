@@ -106,6 +106,35 @@ def test_bad_csr_is_rejected_without_device(e2, needle):
     assert needle in L.lag2d_last_error()
 
 
+@pytest.mark.parametrize("change,needle", [
+    ({"npoly": 1, "ndegr": 3}, b"ncurv=1/nfint=5/P2"),
+    ({"ncase": 13}, b"nfint=5: ncase"),
+    ({"code": 21}, b"nfint=5: slip walls"),
+])
+def test_unsupported_nfint5_config_is_rejected_without_device(change, needle):
+    """The nfint=5 branch (face Gauss points) runs P2 with slip and fixed
+    walls; its other switches are refused before the device is touched."""
+    import ctypes
+    import numpy as np
+    from lag2d_amd import lib
+    L = lib.load()
+    kw = dict(ndegr=6, npoly=2, ncurv=1, nstag=3, ndens=1, nlimi=6, ncase=7, nrz=0, nfint=5,
+              nriem=1, ndt=1, gamlg=1.4, cfl=0.25, cimpd=1.0, cdrho=1.0, tend=1.0)
+    code = change.pop("code", 22)
+    kw.update(change)
+    cfg = lib.Config(**kw)
+    ipq = np.arange(1, 10, dtype=np.int32)
+    bf = np.array([1, 2, code, 221, 5], dtype=np.int32)
+    e1 = np.ones(9, dtype=np.int32)
+    e2 = np.arange(0, 10, dtype=np.int32)
+    h = ctypes.c_void_p()
+    P = ctypes.POINTER(ctypes.c_int)
+    rc = L.lag2d_create(ctypes.byref(cfg), 1, 9, 1, ipq.ctypes.data_as(P), bf.ctypes.data_as(P),
+                        e1.ctypes.data_as(P), e2.ctypes.data_as(P), ctypes.byref(h))
+    assert rc == 2
+    assert needle in L.lag2d_last_error()
+
+
 def test_ncase8_is_rejected_without_device():
     """ncase 8 lowers the CFL number mid-run (Getrhslag.f90:1785), which the
     library does not reproduce: it refuses the problem rather than drift."""
