@@ -70,6 +70,9 @@ if (.not. c_associated(ctx)) then
   cfg%nfint = nfint; cfg%nriem = nriem; cfg%ndt = ndt
   cfg%gamlg = gamlg; cfg%cfl = cfl; cfg%cimpd = cimpd; cfg%cdrho = cdrho; cfg%tend = tend
   cfg%dtfix = dtfix; cfg%tacru = tacru; cfg%strict = 1
+  ! the library's nfint=5 branch implements the 5-point Lobatto face rule,
+  ! the one that branch's tables are written for
+  if (nfint == 5 .and. ngausf /= 5) error stop 'lag2d: nfint=5 needs ngausf=5'
   if (ntria > 0) then
     call lag2d_require(lag2d_create_hybrid(cfg, ntria, nquad, npoin, nbfac, iptri, ipqua, bface, esuv1, esuv2, &
                                            ctx), 'create')

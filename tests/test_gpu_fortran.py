@@ -69,17 +69,37 @@ def test_reference_driver_with_gpu_step_curved_p2(tmp_path):
     assert got == ref
 
 
+def test_reference_driver_with_gpu_step_nfint5(tmp_path):
+    """The nfint=5 branch (vertex and non-vertex face Gauss points) through
+    the Fortran boundary: curved P2 Sedov on the jittered 60 x 60 Q2 mesh
+    with the reference's centre-cell blast, 12 SSP-RK3 steps; the restart
+    files must agree character for character."""
+    mesh, ctl = mg.sedov_problem(60, seed=7)
+    ctl = dataclasses.replace(ctl, ntime=12, ndump=1000000, nfint=5, ngausf=5)
+    for sub in ("cpu", "gpu"):
+        mg.write_case(str(tmp_path / sub), mesh, ctl)
+    ref = _run(DRV_CPU, str(tmp_path / "cpu"))
+    got = _run(DRV_GPU, str(tmp_path / "gpu"))
+    a = np.array(ref.split(), dtype=float)
+    assert a.size == 3600 * 4 * 6 and np.isfinite(a).all()
+    e = a.reshape(3600, 4, 6)[:, 3, 0]
+    assert np.count_nonzero(np.abs(e - e[0]) > 1e-3 * abs(e[0])) > 4
+    assert got == ref
+
+
 @pytest.mark.skipif(not os.path.exists(DRV_GPURHS), reason="reference drivers not built (make -f oracle/ref.mk drivers)")
-@pytest.mark.parametrize("kind", ["quads", "hybrid", "curved"])
+@pytest.mark.parametrize("kind", ["quads", "hybrid", "curved", "curved_nfint5"])
 def test_reference_driver_with_gpu_rhs(tmp_path, kind):
     """The right-hand-side boundary: the reference's own getsolvdg_lag keeps
     the RK loop on the host and calls the drop-in getrhsdg_lag_mcnew
     (lag2d_amd/fortran/getrhsdg_lag_mcnew_shim.f90) once per stage, which
     uploads the stage state and computes the right-hand side on the GPU.
     The restart file equals the CPU reference's, character for character."""
-    if kind == "curved":
+    if kind.startswith("curved"):
         mesh, ctl = mg.sedov_problem(60, seed=7)
         ctl = dataclasses.replace(ctl, ntime=12, ndump=1000000)
+        if kind == "curved_nfint5":
+            ctl = dataclasses.replace(ctl, nfint=5, ngausf=5)
     else:
         mesh, ctl = {"quads": lambda: mg.sod_problem(100, 4), "hybrid": lambda: mg.sod_hybrid_problem(40, 8)}[kind]()
         ctl = dataclasses.replace(ctl, ntime=40, ndump=1000000)
