@@ -42,7 +42,8 @@ typedef struct lag2d_config {
     int nlimi;    /* 6 */
     int ncase;    /* test-problem id (only boundary-colouring rules read it) */
     int nrz;      /* 0 planar, 2 area-weighted RZ (r = y; straight P1 only) */
-    int nfint;    /* 1 (ncurv=0) or 4 (ncurv=1 sub-cell scheme) */
+    int nfint;    /* 1 (ncurv=0); 4 (ncurv=1 sub-cell scheme) or 5 (ncurv=1 sub-cell scheme
+                     with face Gauss points, P2, the 5-point Lobatto face rule) */
     int nriem;    /* 1 */
     int ndt;      /* 1: CFL-controlled dt, 0: fixed dtfix */
     double gamlg, cfl, cimpd, cdrho, tend;
