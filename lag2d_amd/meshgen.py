@@ -119,6 +119,32 @@ def split_quads(mesh: Mesh, fraction=0.3, seed=0) -> Mesh:
                 iptri=np.asarray(tris, dtype=np.int32).reshape(-1, 3))
 
 
+def split_q2_quads(mesh: Mesh, fraction=0.3, seed=0, keep=None) -> Mesh:
+    """Split a seeded fraction of a Q2 mesh's quads into two six-node (P2)
+    triangles along a seeded diagonal.  The quad's centre node becomes the
+    diagonal's mid-node.  Triangle nodes are 3 corners counter-clockwise,
+    then the mid-nodes of edges 1-2, 2-3, 3-1 (GetFE.f90, Getrhslaggauss.f90
+    :12262-12265).  ``keep(e)`` True leaves quad e whole.  Triangles are
+    numbered before the quads, as the reference numbers them."""
+    if mesh.ncurv != 1:
+        raise ValueError("Q2 quads only")
+    rng = np.random.default_rng(seed)
+    nq = mesh.nquad
+    split = rng.random(nq) < fraction
+    diag = rng.integers(0, 2, nq)
+    tris, quads = [], []
+    for e in range(nq):
+        c1, c2, c3, c4, m12, m23, m34, m41, mc = (int(v) for v in mesh.ipqua[e])
+        if not split[e] or (keep is not None and keep(e)):
+            quads.append(tuple(int(v) for v in mesh.ipqua[e]))
+        elif diag[e] == 0:
+            tris += [(c1, c2, c3, m12, m23, mc), (c1, c3, c4, mc, m34, m41)]
+        else:
+            tris += [(c1, c2, c4, m12, mc, m41), (c2, c3, c4, m23, m34, mc)]
+    return Mesh(mesh.coord, np.asarray(quads, dtype=np.int32).reshape(-1, 9), mesh.bface, 1, mesh.nx, mesh.ny,
+                iptri=np.asarray(tris, dtype=np.int32).reshape(-1, 6))
+
+
 def quad_q2(nx, ny, x0, x1, y0, y1, seed=None, jitter=0.1, bulge=0.05) -> Mesh:
     """Q2 quads (ncurv=1) on a (2nx+1) x (2ny+1) node lattice.
 

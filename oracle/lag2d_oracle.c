@@ -68,17 +68,21 @@ void orc_cellvol(orc_case *c, const double *x)
 {
     const double c10 = 1.0;
     for (int it = 0; it < c->ntria; ++it) {
-        const int *ip = c->iptri + 3 * it;
+        /* straight triangles: chord-midpoint mid-nodes; curved (ncurv=1):
+         * the six nodes of the mesh (:1904-1914) */
+        const int nvt = c->ncurv == 1 ? 6 : 3;
+        const int *ip = c->iptri + nvt * it;
         double xp[2][6];
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < nvt; ++k) {
             xp[0][k] = X(x, 0, ip[k]);
             xp[1][k] = X(x, 1, ip[k]);
         }
-        for (int d = 0; d < 2; ++d) {
-            xp[d][3] = 0.5 * (xp[d][0] + xp[d][1]);
-            xp[d][4] = 0.5 * (xp[d][1] + xp[d][2]);
-            xp[d][5] = 0.5 * (xp[d][0] + xp[d][2]);
-        }
+        if (nvt == 3)
+            for (int d = 0; d < 2; ++d) {
+                xp[d][3] = 0.5 * (xp[d][0] + xp[d][1]);
+                xp[d][4] = 0.5 * (xp[d][1] + xp[d][2]);
+                xp[d][5] = 0.5 * (xp[d][0] + xp[d][2]);
+            }
         double rcv = 0.0, scv = 0.0, volel = 0.0, areel = 0.0, rcvrz = 0.0, scvrz = 0.0;
         for (int g = 0; g < 7; ++g) {
             const double r = PT7[g][0], s = PT7[g][1], wi = WT7[g];

@@ -12,6 +12,12 @@
  *   getrhosubg_averg / _daverg   Getrhslaggauss.f90:7067-7392, :10949-11109
  *   rhsifacedg_lagsubgq2         Getrhslaggauss.f90:8501-8745
  *   rhsdomndg_lagmc_curvquad     Getrhslag.f90:4716-5013
+ * Curved six-node triangles of hybrid meshes (cells 0 .. ntria-1):
+ *   getfnds_lagsmsif_hybrid, getfnds_lagsms_simpsonhybrid (triangle parts)
+ *   getriem_triasubg3             Getrhslaggauss.f90:12201-12590
+ *   getrhosubgt_average2          Getrhslaggauss.f90:12595-12940
+ *   rhsifacedg_lagsubgt3          Getrhslaggauss.f90:8297-8496
+ *   rhsdomndg_lagmc_curvtria      Getrhslag.f90:4449-4711
  * barthlimit_lagcurv is called by the reference on this branch but its
  * result (node max/min) is not read for P2; it is omitted.  For P1 the
  * reference also runs barthlimit_lagcurv and barthlimit_lag_curvquadcb
@@ -79,9 +85,12 @@ static void wfg_face(double wm[4][4][2], double wv[4][4][2])
             }
 }
 
+/* the nine nodes of quad cell ie (cells ntria .. ncell-1 are the quads) */
+#define QIP(c, ie) ((c)->ipqua + 9 * ((ie) - (c)->ntria))
+
 static void q2_nodes(const orc_case *c, const double *x, int ie, double xp[2][9])
 {
-    const int *ip = c->ipqua + 9 * ie;
+    const int *ip = QIP(c, ie);
     for (int k = 0; k < 9; ++k) {
         xp[0][k] = X(x, 0, ip[k]);
         xp[1][k] = X(x, 1, ip[k]);
@@ -143,7 +152,7 @@ static void fnds_q2(const orc_case *c, const double *x, double *gq)
     static const double vn0[12][2] = {{0, -1}, {0, -1}, {1, 0}, {1, 0}, {0, 1}, {0, 1},
                                       {-1, 0}, {-1, 0}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
     static const int slot[12] = {0, 1, 2, 3, 4, 5, 6, 7, 12, 13, 14, 15};
-    for (int ie = 0; ie < c->nquad; ++ie) {
+    for (int ie = c->ntria; ie < c->ncell; ++ie) {
         double xp[2][9];
         q2_nodes(c, x, ie, xp);
         double *g = gq + 48 * ie;
@@ -200,7 +209,8 @@ static void persson(const orc_case *c, const double *x, double *flag)
     double pos[50], w[25];
     ruqope2(5, pos, w);
     const double thr = -3.0 * log10(4.0) - 4.0;
-    for (int ie = 0; ie < c->nquad; ++ie) {
+    for (int ie = 0; ie < c->ntria; ++ie) flag[ie] = 0.0;
+    for (int ie = c->ntria; ie < c->ncell; ++ie) {
         double xp[2][9];
         q2_nodes(c, x, ie, xp);
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
@@ -444,7 +454,7 @@ static int stencil_cells(const orc_case *c, int ie, const int *esqua, int ielaj[
     ielaj[0] = ie;
     for (int f = 0; f < 4; ++f) ielaj[1 + f] = esqua[f + 4 * ie];
     for (int k = 5; k < 9; ++k) ielaj[k] = -2; /* unset */
-    const int *ip = c->ipqua + 9 * ie;
+    const int *ip = QIP(c, ie);
     int mismatch = 0;
     for (int iv = 0; iv < 4; ++iv) {
         int jel[3] = {-1, -1, -1}, nv = 0;
@@ -483,7 +493,7 @@ static int weno_cell(const weno_env *E, int ie, const double m[4], const int *es
     ielaj[0] = ie;
     for (int f = 0; f < 4; ++f) ielaj[1 + f] = esqua[f + 4 * ie];
     for (int k = 5; k < 9; ++k) ielaj[k] = -2; /* unset */
-    const int *ip = c->ipqua + 9 * ie;
+    const int *ip = QIP(c, ie);
     int mismatch = 0;
     for (int iv = 0; iv < 4; ++iv) {
         int jel[3] = {-1, -1, -1}, nv = 0;
@@ -1032,7 +1042,7 @@ static int weno(orc_case *c, const double *flag, const int *esqua)
     double *geoph = (double *)malloc(sizeof(double) * 2 * nc);
     double *mapmt = (double *)malloc(sizeof(double) * 4 * nc);
     ORC_PHASE(ORC_PH_WENO_MC);
-    for (int ie = 0; ie < nc; ++ie) {
+    for (int ie = c->ntria; ie < nc; ++ie) {
         double xp[2][9], xi[2][9];
         q2_nodes(c, c->cooro, ie, xp);
         q2_nodes(c, c->coold, ie, xi);
@@ -1071,8 +1081,17 @@ static int weno(orc_case *c, const double *flag, const int *esqua)
     int bad = 0;
     ORC_PHASE(ORC_PH_WENO);
     E.esqua = esqua;
-    for (int ie = 0; ie < nc; ++ie) {
+    for (int ie = c->ntria; ie < nc; ++ie) {
         if (flag[ie] < 10.0) continue;
+        /* On a hybrid mesh the reference reads the face-neighbour table by
+         * the cell's global number, not its quad number (WENO.f90:7659), so
+         * a troubled cell's stencil is undefined there: such cells are
+         * reported (strict) or keep their DOFs. */
+        if (c->ntria > 0) {
+            ++bad;
+            c->nweno_skipped++;
+            continue;
+        }
         /* A cell at rest has the zero frame (WENO.f90:6066-6081): every
          * stencil point maps to the origin, the normal equations vanish and
          * Matinv225 returns 0/0.  Strict mode keeps that arithmetic (the
@@ -1197,9 +1216,422 @@ static void subcell_tau(const orc_case *c, int ie, double uq[4])
     }
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* Curved six-node triangles (hybrid meshes): cells 0 .. ntria-1, nodes
+ * iptri[k + 6 it], corners 0-2 counter-clockwise then the mid-nodes of
+ * edges 1-2, 2-3, 3-1. */
+#define TIP(c, it) ((c)->iptri + 6 * (it))
+
+/* sub-cell vertex -> local node (iptsg, Getrhslaggauss.f90:12262-12265) */
+static const int IPTSG[4][3] = {{0, 3, 5}, {3, 4, 5}, {3, 1, 4}, {5, 4, 2}};
+/* signed face-normal slots (fntsg, :12267-12285): [isg][2 ivsg + f] */
+static const int FNTSG[4][6] = {{6, 1, 10, -7, -13, 12}, {7, 14, 8, 15, 9, 13},
+                                {-14, 10, 2, 3, 11, -8}, {12, -9, -15, 11, 4, 5}};
+/* corner weights times 1/4 (wfgst, :12287-12306; the volume weights of the
+ * face integral, wfgstm, Getrhslaggauss.f90:8349-8367, are the same):
+ * [isg][ivsg][f] */
+static const double WT2[4][3][2] = {
+    {{0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}},
+    {{0.25 * (8.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * (8.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * (8.0 / 3.0), 0.25 * (8.0 / 3.0)}},
+    {{0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}},
+    {{0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}}};
+/* reference coordinates of the six nodes */
+static const double XV6[6] = {0.0, 1.0, 0.0, 0.5, 0.5, 0.0};
+static const double YV6[6] = {0.0, 0.0, 1.0, 0.0, 0.5, 0.5};
+
+/* rutope's 7-point area rule (dgflo2d.f90:2234-2255), as the triangle
+ * routines read it with ngausd = 7 */
+static const double PT7Q[7][2] = {
+    {0.333333333333333, 0.333333333333333}, {0.059715871789770, 0.470142064105115},
+    {0.470142064105115, 0.059715871789770}, {0.470142064105115, 0.470142064105115},
+    {0.797426985353087, 0.101286507323456}, {0.101286507323456, 0.797426985353087},
+    {0.101286507323456, 0.101286507323456}};
+static const double WT7Q[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506, 0.132394152788506,
+                               0.125939180544827, 0.125939180544827, 0.125939180544827};
+
+/* quadratic triangle shape functions and reference derivatives
+ * (Getrhslaggauss.f90:9501-9523) */
+static void shape_t2(double r, double s, double sh[6], double dr_[6], double ds_[6])
+{
+    const double c10 = 1.0;
+    sh[0] = -(c10 - r - s) * (c10 - 2.0 * (c10 - r - s));
+    sh[1] = -r * (c10 - 2.0 * r);
+    sh[2] = -s * (c10 - 2.0 * s);
+    sh[3] = 4.0 * r * (c10 - r - s);
+    sh[4] = 4.0 * r * s;
+    sh[5] = 4.0 * s * (c10 - r - s);
+    dr_[0] = c10 - 4.0 * (c10 - r - s);
+    dr_[1] = -1.0 + 4.0 * r;
+    dr_[2] = 0.0;
+    dr_[3] = 4.0 * (1.0 - 2.0 * r - s);
+    dr_[4] = 4.0 * s;
+    dr_[5] = -4.0 * s;
+    ds_[0] = c10 - 4.0 * (c10 - r - s);
+    ds_[1] = 0.0;
+    ds_[2] = -1.0 + 4.0 * s;
+    ds_[3] = -4.0 * r;
+    ds_[4] = 4.0 * r;
+    ds_[5] = 4.0 * (1.0 - r - 2.0 * s);
+}
+
+static void t2_nodes(const orc_case *c, const double *x, int it, double xp[2][6])
+{
+    const int *ip = TIP(c, it);
+    for (int k = 0; k < 6; ++k) {
+        xp[0][k] = X(x, 0, ip[k]);
+        xp[1][k] = X(x, 1, ip[k]);
+    }
+}
+
+static void t2_map(const double xp[2][6], double r, double s, double *x, double *y)
+{
+    double sh[6], dr_[6], ds_[6];
+    shape_t2(r, s, sh, dr_, ds_);
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        a = a + sh[k] * xp[0][k];
+        b = b + sh[k] * xp[1][k];
+    }
+    *x = a;
+    *y = b;
+}
+
+static void t2_jac(const double xp[2][6], double r, double s, double J[4])
+{
+    double sh[6], dr_[6], ds_[6];
+    shape_t2(r, s, sh, dr_, ds_);
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    for (int k = 0; k < 6; ++k) {
+        dxdr = dxdr + dr_[k] * xp[0][k];
+        dxds = dxds + ds_[k] * xp[0][k];
+        dydr = dydr + dr_[k] * xp[1][k];
+        dyds = dyds + ds_[k] * xp[1][k];
+    }
+    J[0] = dxdr; J[1] = dxds; J[2] = dydr; J[3] = dyds;
+}
+
+/* P2 Taylor basis of a triangle about its mass centre (dr = ds = 0.5) */
+static void basis_t2(double r, double s, double rc, double sc, const double *g19, double b[6])
+{
+    b[0] = 1.0;
+    b[1] = (r - rc) / 0.5;
+    b[2] = (s - sc) / 0.5;
+    b[3] = 0.5 * b[1] * b[1] - g19[0];
+    b[4] = 0.5 * b[2] * b[2] - g19[1];
+    b[5] = b[1] * b[2] - g19[2];
+}
+
+/* Triangle face geometry gt[(k + 3 slot) + 45 it], 15 slots: the interior
+ * sub-faces between mid-nodes at both ends (getfnds_lagsmsif_hybrid,
+ * Getrhslaggauss.f90:3419-3496, slots 6-8 and 12-14) and the boundary-edge
+ * points by the cofactor of the map's Jacobian (getfnds_lagsms_simpsonhybrid,
+ * :9726-9822, slots 0-5 and 9-11). */
+static void fnds_t2(const orc_case *c, const double *x, double *gt)
+{
+    static const double postc[3][2] = {{0.25, 0.25}, {0.5, 0.25}, {0.25, 0.5}};
+    static const int ipift[3][3] = {{3, 5, 0}, {4, 3, 1}, {5, 4, 2}};
+    static const double posit[9][2] = {{0, 0}, {1, 0}, {1, 0}, {0, 1}, {0, 1}, {0, 0}, {.5, 0}, {.5, .5}, {0, .5}};
+    static const int slot[9] = {0, 1, 2, 3, 4, 5, 9, 10, 11};
+    for (int it = 0; it < c->ntria; ++it) {
+        double xp[2][6];
+        t2_nodes(c, x, it, xp);
+        double *g = gt + 45 * it;
+        double xv[3], yv[3];
+        for (int k = 0; k < 3; ++k) t2_map(xp, postc[k][0], postc[k][1], &xv[k], &yv[k]);
+        for (int f = 0; f < 3; ++f) {
+            double fx[3] = {xp[0][ipift[f][0]], xp[0][ipift[f][1]], xv[ipift[f][2]]};
+            double fy[3] = {xp[1][ipift[f][0]], xp[1][ipift[f][1]], yv[ipift[f][2]]};
+            for (int e = 0; e < 2; ++e) {
+                const double r = e == 0 ? -1.0 : 1.0;
+                const double d[3] = {-0.5 + r, 0.5 + r, -2.0 * r};
+                double dxdr = 0.0, dydr = 0.0;
+                for (int i = 0; i < 3; ++i) {
+                    dxdr = dxdr + d[i] * fx[i];
+                    dydr = dydr + d[i] * fy[i];
+                }
+                const double djaco = sqrt(dxdr * dxdr + dydr * dydr);
+                const double anx = -dydr, any = dxdr;
+                const int sl = (e == 0 ? 6 : 12) + f;
+                g[3 * sl + 0] = anx / sqrt(anx * anx + any * any);
+                g[3 * sl + 1] = any / sqrt(anx * anx + any * any);
+                g[3 * sl + 2] = djaco * 2.0;
+            }
+        }
+        for (int ig = 0; ig < 9; ++ig) {
+            double J[4];
+            t2_jac(xp, posit[ig][0], posit[ig][1], J);
+            const double c11 = J[3], c12 = -J[2], c21 = -J[1], c22 = J[0];
+            double v1, v2, larea;
+            if (ig == 0 || ig == 1 || ig == 6) {
+                v1 = 0.0; v2 = -1.0; larea = 1.0;
+            } else if (ig == 2 || ig == 3 || ig == 7) {
+                v1 = sqrt(2.0) * 0.5; v2 = sqrt(2.0) * 0.5; larea = sqrt(2.0);
+            } else {
+                v1 = -1.0; v2 = 0.0; larea = 1.0;
+            }
+            const double anx = c11 * v1 + c12 * v2;
+            const double any = c21 * v1 + c22 * v2;
+            const double w1 = anx * larea, w2 = any * larea;
+            const double farea = sqrt(w1 * w1 + w2 * w2);
+            g[3 * slot[ig] + 0] = anx * larea / farea;
+            g[3 * slot[ig] + 1] = any * larea / farea;
+            g[3 * slot[ig] + 2] = farea;
+        }
+    }
+}
+
+/* Initial density at a point of a curved triangle (getrhoig_triacurv2,
+ * Getrhslag.f90:17622-17757): the Sedov problem's uniform density, or the
+ * per-cell initial density when given. */
+static double rho_initial_t2(const orc_case *c, int it)
+{
+    return c->rho0 ? c->rho0[it] : 1.0;
+}
+
+/* getrhosubgt_average2 (Getrhslaggauss.f90:12595-12940): per sub-triangle
+ * the density initial mass / current area (dens) and the mass-weighted mean
+ * of tau as a density (tau) */
+static void subcell_t2(const orc_case *c, int it, double dens[4], double tau[4])
+{
+    static const double postc[9][2] = {{.25, 0.}, {.75, 0.}, {.75, .25}, {.25, .75}, {0., .75},
+                                       {0., .25}, {.25, .25}, {.5, .25}, {.25, .5}};
+    static const int iptsc[4][3] = {{0, 6, 5}, {7, 8, 6}, {1, 2, 7}, {8, 3, 4}};
+    double xp[2][6], xi[2][6];
+    t2_nodes(c, c->cooro, it, xp);
+    t2_nodes(c, c->coold, it, xi);
+    double xc[2][9], xic[2][9];
+    for (int k = 0; k < 9; ++k) {
+        t2_map(xp, postc[k][0], postc[k][1], &xc[0][k], &xc[1][k]);
+        t2_map(xi, postc[k][0], postc[k][1], &xic[0][k], &xic[1][k]);
+    }
+    const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+    const double rhomc = U(c, c->unold, 0, 0, it);
+    double f0[4];
+    for (int isg = 0; isg < 4; ++isg) {
+        double xs[2][6];
+        for (int d = 0; d < 2; ++d) {
+            for (int k = 0; k < 3; ++k) xs[d][k] = xp[d][IPTSG[isg][k]];
+            for (int k = 0; k < 3; ++k) xs[d][3 + k] = xc[d][iptsc[isg][k]];
+        }
+        double a = 0.0, masel = 0.0, volel = 0.0;
+        for (int g = 0; g < 7; ++g) {
+            const double r = PT7Q[g][0], s = PT7Q[g][1], wi = WT7Q[g];
+            const double shs[3] = {1.0 - r - s, r, s};
+            double rg = 0.0, sg = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                rg = rg + shs[k] * XV6[IPTSG[isg][k]];
+                sg = sg + shs[k] * YV6[IPTSG[isg][k]];
+            }
+            double b[6];
+            basis_t2(rg, sg, rc, sc, &GE(c, 18, it), b);
+            double t = 0.0;
+            for (int id = 0; id < c->ndegr; ++id) t = t + U(c, c->unold, id, 0, it) * b[id];
+            t = rhomc + 1.0 * (t - rhomc); /* nlimi=6 with aflim(1) = 1 */
+            double J[4];
+            t2_jac(xs, r, s, J);
+            const double djaco = 0.5 * wi * (J[0] * J[3] - J[2] * J[1]);
+            a = a + djaco;
+            masel = masel + 1.0 / t * djaco;
+            volel = volel + djaco;
+        }
+        f0[isg] = 1.0 / a;
+        tau[isg] = volel / masel;
+    }
+    for (int isg = 0; isg < 4; ++isg) {
+        double xs[2][6];
+        for (int d = 0; d < 2; ++d) {
+            for (int k = 0; k < 3; ++k) xs[d][k] = xi[d][IPTSG[isg][k]];
+            for (int k = 0; k < 3; ++k) xs[d][3 + k] = xic[d][iptsc[isg][k]];
+        }
+        double ms = 0.0;
+        for (int g = 0; g < 7; ++g) {
+            double J[4];
+            t2_jac(xs, PT7Q[g][0], PT7Q[g][1], J);
+            const double djaco = 0.5 * WT7Q[g] * (J[0] * J[3] - J[2] * J[1]);
+            ms = ms + rho_initial_t2(c, it) * 1.0 * djaco;
+        }
+        dens[isg] = 0.0 + f0[isg] * ms;
+    }
+}
+
+/* getriem_triasubg3 (Getrhslaggauss.f90:12201-12590): the triangle's
+ * sub-cell corners, scattered into the node sums and recorded per corner
+ * face (mcl / mul / snl at o = f + 2 (ivsg + 3 isg), 24 per triangle) */
+static void corners_t2(orc_case *c, const double *gt, double *mn, double *mu, double *sn, double *mcl, double *mul,
+                       double *snl)
+{
+    const double g1 = c->gamlg - 1.0, eps = 1.0e-15;
+    const int nd = c->ndegr;
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = TIP(c, it);
+        const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+        double un[6][4];
+        for (int iv = 0; iv < 6; ++iv) {
+            double b[6];
+            basis_t2(XV6[iv], YV6[iv], rc, sc, &GE(c, 18, it), b);
+            for (int q = 0; q < 4; ++q) un[iv][q] = 0.0;
+            for (int id = 0; id < nd; ++id)
+                for (int q = 0; q < 4; ++q) un[iv][q] = un[iv][q] + U(c, c->unold, id, q, it) * b[id];
+        }
+        const double rhomc = U(c, c->unold, 0, 0, it);
+        const double uctr = U(c, c->unold, 0, 1, it), vctr = U(c, c->unold, 0, 2, it), ectr = U(c, c->unold, 0, 3, it);
+        const double rhoct = 1.0 / rhomc;
+        const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
+        double dens[4], tau[4];
+        subcell_t2(c, it, dens, tau);
+        const double sdctr = sqrt(dmax(eps, c->gamlg * pctr / rhoct));
+        const double *g = gt + 45 * it;
+        for (int isg = 0; isg < 4; ++isg) {
+            double vn[3][2][3];
+            for (int iv = 0; iv < 3; ++iv)
+                for (int f = 0; f < 2; ++f) {
+                    const int t = FNTSG[isg][2 * iv + f];
+                    const int sl = abs(t) - 1;
+                    const double sg = t > 0 ? 1.0 : -1.0;
+                    vn[iv][f][0] = sg * g[3 * sl];
+                    vn[iv][f][1] = sg * g[3 * sl + 1];
+                    vn[iv][f][2] = g[3 * sl + 2] * WT2[isg][iv][f];
+                }
+            for (int iv = 0; iv < 3; ++iv) {
+                const int ln = IPTSG[isg][iv];
+                const int p = ip[ln];
+                const double rhovt = 1.0 / un[ln][0];
+                const double rhovsg = rhovt + 1.0 * (dens[isg] - 1.0 / tau[isg]);
+                const double uv = un[ln][1], vv = un[ln][2], ev = un[ln][3];
+                const double pv = dmax(eps, g1 * rhovsg * (ev - 0.5 * (uv * uv + vv * vv)));
+                const double dux = X(c->ustar, 0, p) - uv, duy = X(c->ustar, 1, p) - vv;
+                const double a3 = sqrt(dux * dux + duy * duy) / sdctr;
+                for (int f = 0; f < 2; ++f) {
+                    const double deltu = 0.0 * fabs(dux * vn[iv][f][0] + duy * vn[iv][f][1]);
+                    const double mur = rhoct * sdctr + rhoct * c->slpdu * deltu;
+                    const double area = vn[iv][f][2];
+                    double m11;
+                    if (a3 < 1.e12) {
+                        m11 = mur * area * fabs(1.0);
+                    } else {
+                        const double nrm = sqrt(dux * dux + duy * duy);
+                        m11 = mur * area * fabs(vn[iv][f][0] * (dux / nrm) + vn[iv][f][1] * (duy / nrm));
+                    }
+                    const double mu1 = m11 * uv + 0.0 * vv, mu2 = 0.0 * uv + m11 * vv;
+                    const double sn1 = -pv * area * vn[iv][f][0] + 0.0 * area * vn[iv][f][1];
+                    const double sn2 = 0.0 * area * vn[iv][f][0] + -pv * area * vn[iv][f][1];
+                    mn[0 + 4 * p] += m11;
+                    mn[1 + 4 * p] += 0.0;
+                    mn[2 + 4 * p] += 0.0;
+                    mn[3 + 4 * p] += m11;
+                    mu[2 * p] += mu1;
+                    mu[2 * p + 1] += mu2;
+                    sn[2 * p] += sn1;
+                    sn[2 * p + 1] += sn2;
+                    const int o = f + 2 * (iv + 3 * isg);
+                    mcl[o + 24 * it] = m11;
+                    mul[2 * o + 48 * it] = mu1;
+                    mul[2 * o + 1 + 48 * it] = mu2;
+                    snl[2 * o + 48 * it] = sn1;
+                    snl[2 * o + 1 + 48 * it] = sn2;
+                }
+            }
+        }
+    }
+}
+
+/* rhsifacedg_lagsubgt3 (Getrhslaggauss.f90:8297-8496): momentum and energy
+ * weights 1, volume weights wfgstm; overwrites rhsel */
+static void iface_t2(orc_case *c, const double *gt, const double *fstart)
+{
+    const int nd = c->ndegr;
+    for (int it = 0; it < c->ntria; ++it) {
+        const int *ip = TIP(c, it);
+        const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+        double bt[6][6];
+        for (int iv = 0; iv < 6; ++iv) basis_t2(XV6[iv], YV6[iv], rc, sc, &GE(c, 18, it), bt[iv]);
+        double ul[6] = {0}, p1[6] = {0}, p2[6] = {0}, el[6] = {0};
+        const double *g = gt + 45 * it;
+        for (int isg = 0; isg < 4; ++isg) {
+            double vn[3][2][3];
+            for (int iv = 0; iv < 3; ++iv)
+                for (int f = 0; f < 2; ++f) {
+                    const int t = FNTSG[isg][2 * iv + f];
+                    const int sl = abs(t) - 1;
+                    const double sg = t > 0 ? 1.0 : -1.0;
+                    vn[iv][f][0] = sg * g[3 * sl];
+                    vn[iv][f][1] = sg * g[3 * sl + 1];
+                    vn[iv][f][2] = g[3 * sl + 2] * WT2[isg][iv][f];
+                }
+            for (int iv = 0; iv < 3; ++iv) {
+                const int ln = IPTSG[isg][iv];
+                const int p = ip[ln];
+                const double us1 = X(c->ustar, 0, p), us2 = X(c->ustar, 1, p);
+                const int o0 = 0 + 2 * (iv + 3 * isg), o1 = 1 + 2 * (iv + 3 * isg);
+                const double *f0 = fstart + 2 * o0 + 48 * it, *f1 = fstart + 2 * o1 + 48 * it;
+                for (int id = 0; id < nd; ++id) {
+                    const double b = bt[ln][id];
+                    ul[id] = ul[id] + us1 * vn[iv][0][0] * vn[iv][0][2] * b + us2 * vn[iv][0][1] * vn[iv][0][2] * b +
+                             us1 * vn[iv][1][0] * vn[iv][1][2] * b + us2 * vn[iv][1][1] * vn[iv][1][2] * b;
+                    p1[id] = p1[id] + f0[0] * b * 1.0 + f1[0] * b * 1.0;
+                    p2[id] = p2[id] + f0[1] * b * 1.0 + f1[1] * b * 1.0;
+                    el[id] = el[id] + us1 * f0[0] * b * 1.0 + us2 * f0[1] * b * 1.0 + us1 * f1[0] * b * 1.0 +
+                             us2 * f1[1] * b * 1.0;
+                }
+            }
+        }
+        for (int id = 0; id < nd; ++id) {
+            U(c, c->rhsel, id, 0, it) = ul[id];
+            U(c, c->rhsel, id, 1, it) = p1[id];
+            U(c, c->rhsel, id, 2, it) = p2[id];
+            U(c, c->rhsel, id, 3, it) = el[id];
+        }
+    }
+}
+
+/* rhsdomndg_lagmc_curvtria (Getrhslag.f90:4449-4711): 7-point rule,
+ * djaco = w/2 with the cofactor form of the basis gradients; the pressure
+ * is not clamped here */
+static void domn_t2(orc_case *c)
+{
+    const double g1 = c->gamlg - 1.0;
+    const int nd = c->ndegr;
+    for (int it = 0; it < c->ntria; ++it) {
+        double xp[2][6];
+        t2_nodes(c, c->cooro, it, xp);
+        const double rc = GE(c, 0, it), sc = GE(c, 1, it);
+        for (int g = 0; g < 7; ++g) {
+            const double r = PT7Q[g][0], s = PT7Q[g][1], wi = WT7Q[g];
+            double J[4];
+            t2_jac(xp, r, s, J);
+            const double dxdr = J[0], dxds = J[1], dydr = J[2], dyds = J[3];
+            const double djaco = 0.5 * wi;
+            const double dr = 0.5, ds = 0.5;
+            const double dbdr[6] = {0.0, 1.0 / dr, 0.0, (r - rc) / (dr * dr), 0.0, (s - sc) / dr / ds};
+            const double dbds[6] = {0.0, 0.0, 1.0 / ds, 0.0, (s - sc) / (ds * ds), (r - rc) / dr / ds};
+            double b[6];
+            basis_t2(r, s, rc, sc, &GE(c, 18, it), b);
+            double un[4] = {0, 0, 0, 0};
+            for (int id = 0; id < nd; ++id)
+                for (int q = 0; q < 4; ++q) un[q] = un[q] + U(c, c->unold, id, q, it) * b[id];
+            const double rhoad = 1.0 / un[0];
+            const double u = un[1], v = un[2];
+            const double pres = g1 * rhoad * (un[3] - 0.5 * (u * u + v * v));
+            for (int id = 0; id < nd; ++id) {
+                const double a = dyds * dbdr[id] + (-dydr) * dbds[id];
+                const double cc = (-dxds) * dbdr[id] + dxdr * dbds[id];
+                const double f0 = a * u + cc * v;
+                const double f1 = a * (-pres);
+                const double f2 = cc * (-pres);
+                const double f3 = (a * u + cc * v) * (-pres);
+                U(c, c->rhsel, id, 0, it) = U(c, c->rhsel, id, 0, it) - f0 * djaco;
+                U(c, c->rhsel, id, 1, it) = U(c, c->rhsel, id, 1, it) - f1 * djaco;
+                U(c, c->rhsel, id, 2, it) = U(c, c->rhsel, id, 2, it) - f2 * djaco;
+                U(c, c->rhsel, id, 3, it) = U(c, c->rhsel, id, 3, it) - f3 * djaco;
+            }
+        }
+    }
+}
+
 /* sub-cell nodal solver (getndvelo_lagsubg + getriem_quadsubg2) and the
  * corner forces fstar[(d + 2*(f + 2*(ivsg + 4*isg))) + 32*ie] */
-static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
+static void ndvelo_q2(orc_case *c, const double *gq, double *fstar, const double *gt, double *fstart)
 {
     const int np = c->npoin, nc = c->ncell, nd = c->ndegr;
     const double g1 = c->gamlg - 1.0, eps = 1.0e-6;
@@ -1212,8 +1644,14 @@ static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
     double *mcl = (double *)malloc(sizeof(double) * 32 * nc);  /* m = mu*A per corner face */
     double *mul = (double *)malloc(sizeof(double) * 64 * nc);  /* m u_c */
     double *snl = (double *)malloc(sizeof(double) * 64 * nc);  /* A sigma n */
-    for (int ie = 0; ie < nc; ++ie) {
-        const int *ip = c->ipqua + 9 * ie;
+    /* triangles first, as the reference accumulates (Getrhslaggauss.f90:4476-4481) */
+    const int nt = c->ntria;
+    double *mclt = (double *)malloc(sizeof(double) * (24 * (size_t)nt + 1));
+    double *mult = (double *)malloc(sizeof(double) * (48 * (size_t)nt + 1));
+    double *snlt = (double *)malloc(sizeof(double) * (48 * (size_t)nt + 1));
+    corners_t2(c, gt, mn, mu, sn, mclt, mult, snlt);
+    for (int ie = c->ntria; ie < nc; ++ie) {
+        const int *ip = QIP(c, ie);
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double un[9][4];
         for (int iv = 0; iv < 9; ++iv) {
@@ -1316,8 +1754,24 @@ static void ndvelo_q2(orc_case *c, const double *gq, double *fstar)
         }
     }
     ORC_PHASE(ORC_PH_CELLRHS);
-    for (int ie = 0; ie < nc; ++ie) {
-        const int *ip = c->ipqua + 9 * ie;
+    for (int it = 0; it < nt; ++it) {
+        const int *ip = TIP(c, it);
+        for (int isg = 0; isg < 4; ++isg)
+            for (int iv = 0; iv < 3; ++iv)
+                for (int f = 0; f < 2; ++f) {
+                    const int o = f + 2 * (iv + 3 * isg);
+                    const int p = ip[IPTSG[isg][iv]];
+                    const double m11 = mclt[o + 24 * it];
+                    const double mu1 = mult[2 * o + 48 * it], mu2 = mult[2 * o + 1 + 48 * it];
+                    const double sn1 = snlt[2 * o + 48 * it], sn2 = snlt[2 * o + 1 + 48 * it];
+                    const double us1 = X(c->ustar, 0, p), us2 = X(c->ustar, 1, p);
+                    fstart[2 * o + 48 * it] = sn1 + m11 * us1 + 0.0 * us2 - mu1;
+                    fstart[2 * o + 1 + 48 * it] = sn2 + 0.0 * us1 + m11 * us2 - mu2;
+                }
+    }
+    free(mclt); free(mult); free(snlt);
+    for (int ie = c->ntria; ie < nc; ++ie) {
+        const int *ip = QIP(c, ie);
         for (int isg = 0; isg < 4; ++isg)
             for (int iv = 0; iv < 4; ++iv)
                 for (int f = 0; f < 2; ++f) {
@@ -1341,8 +1795,8 @@ static void iface_q2(orc_case *c, const double *gq, const double *fstar)
     double wm[4][4][2], wv[4][4][2];
     wfg_face(wm, wv);
     const int nd = c->ndegr;
-    for (int ie = 0; ie < c->ncell; ++ie) {
-        const int *ip = c->ipqua + 9 * ie;
+    for (int ie = c->ntria; ie < c->ncell; ++ie) {
+        const int *ip = QIP(c, ie);
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double bq[9][6];
         for (int iv = 0; iv < 9; ++iv) basis_p2(XV9[iv], YV9[iv], rc, sc, &GE(c, 18, ie), bq[iv]);
@@ -1393,7 +1847,7 @@ static void domn_q2(orc_case *c)
     const double wl[5] = {0.1, 0.544444444444444444444, 0.7111111111111111111111, 0.544444444444444444444, 0.1};
     const double g1 = c->gamlg - 1.0, eps = 1.0e-6;
     const int nd = c->ndegr;
-    for (int ie = 0; ie < c->ncell; ++ie) {
+    for (int ie = c->ntria; ie < c->ncell; ++ie) {
         double xp[2][9];
         q2_nodes(c, c->cooro, ie, xp);
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
@@ -1566,7 +2020,7 @@ static void face_bc5(const orc_case *c, int *bc, int *rev)
     for (int p = 0; p < c->npoin; ++p) mid[p] = -1;
     for (int ie = 0; ie < nc; ++ie)
         for (int lf = 0; lf < 4; ++lf) {
-            mid[c->ipqua[9 * ie + 4 + lf]] = 4 * ie + lf;
+            mid[QIP(c, ie)[4 + lf]] = 4 * ie + lf;
             bc[4 * ie + lf] = 0;
             rev[4 * ie + lf] = 0;
         }
@@ -1578,7 +2032,7 @@ static void face_bc5(const orc_case *c, int *bc, int *rev)
         if (b[2] == 22) code = b[3] == 221 ? 1 : b[3] == 222 ? 2 : 0;
         else if (b[2] == 25 || b[2] == 27) code = 3;
         bc[k] = code;
-        rev[k] = b[0] != c->ipqua[9 * (k / 4) + (k % 4)];
+        rev[k] = b[0] != QIP(c, k / 4)[k % 4];
     }
     free(mid);
 }
@@ -1600,7 +2054,7 @@ static void ndvelo_q2g(orc_case *c, const double *gq, double *fstar, double *fg)
     double *snl = (double *)malloc(sizeof(double) * 64 * nc);
     double *drho = (double *)malloc(sizeof(double) * 4 * nc);
     for (int ie = 0; ie < nc; ++ie) {
-        const int *ip = c->ipqua + 9 * ie;
+        const int *ip = QIP(c, ie);
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double un[9][4];
         for (int iv = 0; iv < 9; ++iv) {
@@ -1705,7 +2159,7 @@ static void ndvelo_q2g(orc_case *c, const double *gq, double *fstar, double *fg)
     /* ufgpq: u* interpolated along each face at its two Lobatto points
      * (:17416-17437) */
     for (int ie = 0; ie < nc; ++ie) {
-        const int *ip = c->ipqua + 9 * ie;
+        const int *ip = QIP(c, ie);
         for (int lf = 0; lf < 4; ++lf) {
             const int pa = ip[lf], pb = ip[(lf + 1) % 4], pm = ip[4 + lf];
             for (int k = 0; k < 2; ++k) {
@@ -1729,7 +2183,7 @@ static void ndvelo_q2g(orc_case *c, const double *gq, double *fstar, double *fg)
         for (int p = 0; p < np; ++p) mid[p] = -1;
         for (int ie = 0; ie < nc; ++ie)
             for (int lf = 0; lf < 4; ++lf) {
-                const int m = c->ipqua[9 * ie + 4 + lf];
+                const int m = QIP(c, ie)[4 + lf];
                 nbr[4 * ie + lf] = -1;
                 if (mid[m] >= 0) {
                     nbr[4 * ie + lf] = mid[m];
@@ -1741,7 +2195,7 @@ static void ndvelo_q2g(orc_case *c, const double *gq, double *fstar, double *fg)
         free(mid);
     }
     for (int ie = 0; ie < nc; ++ie) {
-        const int *ip = c->ipqua + 9 * ie;
+        const int *ip = QIP(c, ie);
         for (int lf = 0; lf < 4; ++lf) {
             const int other = nbr[4 * ie + lf];
             if (other >= 0 && other < 4 * ie + lf) continue; /* the face was done from the other side */
@@ -1813,12 +2267,12 @@ static void ndvelo_q2g(orc_case *c, const double *gq, double *fstar, double *fg)
     }
     /* the centre nodes' u* is zeroed before the vertex forces (:17475) */
     for (int ie = 0; ie < nc; ++ie) {
-        const int p = c->ipqua[9 * ie + 8];
+        const int p = QIP(c, ie)[8];
         X(c->ustar, 0, p) = 0.0;
         X(c->ustar, 1, p) = 0.0;
     }
     for (int ie = 0; ie < nc; ++ie) {
-        const int *ip = c->ipqua + 9 * ie;
+        const int *ip = QIP(c, ie);
         for (int isg = 0; isg < 4; ++isg)
             for (int iv = 0; iv < 4; ++iv)
                 for (int f = 0; f < 2; ++f) {
@@ -1847,7 +2301,7 @@ static void iface_q2g(orc_case *c, const double *gq, const double *fstar, const 
     const double wi = 0.544444444444444444444 / 2.0;
     const int nd = c->ndegr;
     for (int ie = 0; ie < c->ncell; ++ie) {
-        const int *ip = c->ipqua + 9 * ie;
+        const int *ip = QIP(c, ie);
         double rc = GE(c, 0, ie), sc = GE(c, 1, ie);
         double bq[9][6];
         for (int iv = 0; iv < 9; ++iv) basis_p2(XV9[iv], YV9[iv], rc, sc, &GE(c, 18, ie), bq[iv]);
@@ -1972,13 +2426,17 @@ int orc_rhs_q2(orc_case *c)
     if (!((c->npoly == 2 && c->ndegr == 6) || (c->npoly == 1 && c->ndegr == 3)) || c->nrz != 0 ||
         c->ndens != 1 || c->nlimi != 6 || !c->esqua)
         return 2;
-    if (c->nfint == 5) return c->npoly == 2 ? rhs_q2g(c) : 2;
+    if (c->nfint == 5) return (c->npoly == 2 && c->ntria == 0) ? rhs_q2g(c) : 2;
+    if (c->ntria > 0 && (c->npoly != 2 || c->ncase != 7)) return 2;
     const int nc = c->ncell;
     double *gq = (double *)malloc(sizeof(double) * 48 * nc);
     double *flag = (double *)malloc(sizeof(double) * nc);
     double *fstar = (double *)malloc(sizeof(double) * 64 * nc);
+    double *gt = (double *)malloc(sizeof(double) * (45 * (size_t)c->ntria + 1));
+    double *fstart = (double *)malloc(sizeof(double) * (48 * (size_t)c->ntria + 1));
     ORC_PHASE(ORC_PH_CORNER);
     fnds_q2(c, c->cooro, gq);
+    fnds_t2(c, c->cooro, gt);
     ORC_PHASE(ORC_PH_PREP);
     if (c->npoly == 2) {
         persson(c, c->cooro, flag);
@@ -1991,7 +2449,7 @@ int orc_rhs_q2(orc_case *c)
     }
     int rc = weno(c, flag, c->esqua);
     if (rc) {
-        free(gq); free(flag); free(fstar);
+        free(gq); free(flag); free(fstar); free(gt); free(fstart);
         return rc;
     }
     ORC_PHASE(ORC_PH_OTHER);
@@ -2004,9 +2462,11 @@ int orc_rhs_q2(orc_case *c)
         c->afvec[4 * ie + 2] = 0.0;
         c->afvec[4 * ie + 3] = 1.0;
     }
-    ndvelo_q2(c, gq, fstar);
+    ndvelo_q2(c, gq, fstar, gt, fstart);
     ORC_PHASE(ORC_PH_CELLRHS);
+    iface_t2(c, gt, fstart);
     iface_q2(c, gq, fstar);
+    domn_t2(c);
     domn_q2(c);
     /* derivative update with aflim = 1, afvec = I (Getrhslagmc.f90:7695-7708) */
     for (int ie = 0; ie < nc; ++ie) {
@@ -2023,6 +2483,6 @@ int orc_rhs_q2(orc_case *c)
         U(c, c->unold, 1, 3, ie) = U(c, c->unold, 1, 3, ie) * 1.0;
         U(c, c->unold, 2, 3, ie) = U(c, c->unold, 2, 3, ie) * 1.0;
     }
-    free(gq); free(flag); free(fstar);
+    free(gq); free(flag); free(fstar); free(gt); free(fstart);
     return 0;
 }
