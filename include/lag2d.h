@@ -74,6 +74,10 @@ int lag2d_create(const lag2d_config *cfg, int nquad, int npoin, int nbfac,
  * 7-point domain integral of Getrhslagmctria.f90 / Getrhslagmc.f90:14006-14160).
  * The problems ncase = 1, 2, 3, -7, whose triangle limiter overrides
  * boundary nodes (Getrhslagmctria.f90:539-582), return LAG2D_EUNSUPPORTED.
+ * On the curved P2 path (ncurv=1, nfint=4, npoly=2, ncase 7) the triangles
+ * are six-node curved triangles, iptri(6, ntria): corners, then the
+ * mid-nodes of edges 1-2, 2-3, 3-1 (Getrhslaggauss.f90:12201-12590,
+ * :8297-8496, Getrhslag.f90:4449-4711); esuv2(npoin+1) = 6 ntria + 9 nquad.
  * With cfg->nrz = 2 the cells take the area-weighted RZ routines instead
  * (Getrhslagrz.f90, Getrhslagmc.f90:6876-6971, :7299-7392); lag2d_create
  * accepts nrz = 2 on quad meshes as well.  RZ boundaries are slip walls

@@ -124,7 +124,13 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     // (Getrhslag.f90:1785); that switch is not implemented
     if (c.ncase == 8) return fail(LAG2D_EUNSUPPORTED, "ncase 8: the mid-run CFL switch (Getrhslag.f90:1785) "
                                                       "is not implemented");
-    if (ntria > 0 && !q1) return fail(LAG2D_EUNSUPPORTED, "triangles: straight P1 path only (ncurv=0, nfint=1)");
+    // curved six-node triangles ride on the curved P2 branch (nfint=4) with
+    // the uniform initial density of the Sedov problem (ncase 7) or a
+    // per-cell one (getrhoig_triacurv2, Getrhslag.f90:17622-17757)
+    const bool q2h = q2 && c.nfint == 4 && c.npoly == 2 && c.ncase == 7 && rs.nrun == 1;
+    if (ntria > 0 && !q1 && !q2h)
+        return fail(LAG2D_EUNSUPPORTED, "triangles: straight P1 path (ncurv=0, nfint=1), or curved P2 "
+                                        "(ncurv=1, nfint=4, npoly=2) with ncase 7 and one run");
     if (ntria > 0 && (c.ncase == 1 || c.ncase == 2 || c.ncase == 3 || c.ncase == -7))
         return fail(LAG2D_EUNSUPPORTED, "triangles: the boundary-node limiter overrides of ncase 1, 2, 3, -7 "
                                         "(Getrhslagmctria.f90:539-582) are not implemented");
@@ -158,10 +164,13 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
 
     // ---- host-side mesh preprocessing (0-based, SoA; triangles first, their
     // unused fourth node slot repeats the third node)
+    // triangle nodes per cell: 3 straight, 6 curved (corners, then the
+    // mid-nodes of edges 1-2, 2-3, 3-1)
+    const int nvt = c.ncurv == 1 ? 6 : 3;
     std::vector<int> ipq((size_t)nv * nc);
     for (int it = 0; it < ntria; ++it)
         for (int k = 0; k < nv; ++k) {
-            int p = iptri[(size_t)(k < 3 ? k : 2) + (size_t)3 * it] - 1;
+            int p = iptri[(size_t)(k < nvt ? k : 2) + (size_t)nvt * it] - 1;
             if (p < 0 || p >= npoin) return bad(fail(LAG2D_EINVAL, "iptri node out of range"));
             ipq[(size_t)k * nc + it] = p;
         }
@@ -172,7 +181,7 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
             ipq[(size_t)k * nc + ntria + iq] = p;
         }
     const int ne = esuv2[npoin];
-    if (ne != 3 * ntria + nv * nquad) return bad(fail(LAG2D_EINVAL, "esuv2(npoin+1) != 3*ntria + nvqua*nquad"));
+    if (ne != nvt * ntria + nv * nquad) return bad(fail(LAG2D_EINVAL, "esuv2(npoin+1) != nvtri*ntria + nvqua*nquad"));
     std::vector<int> e1(ne), e2(esuv2, esuv2 + npoin + 1);
     for (int k = 0; k < ne; ++k) {
         e1[k] = esuv1[k] - 1;
@@ -422,7 +431,9 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
         x->corner_variant = 8;
         x->rhs_variant = 2;
     }
-    if (q2 && c.nfint == 5) {  // the nfint=5 branch: a lane per sub-cell, a thread per cell
+    if (q2 && (c.nfint == 5 || ntria > 0)) {
+        // the nfint=5 branch and hybrid meshes: a lane per sub-cell corner,
+        // a thread per cell RHS (the forms that take their options)
         x->corner_variant = 2;
         x->rhs_variant = 2;
     }
@@ -521,8 +532,9 @@ extern "C" int lag2d_init_state(lag2d_ctx *x, const double *coord, const double 
     if (!x || !coord || !rho || !u || !v || !p || (nblast > 0 && (!blast || !energy)) || nblast < 0)
         return fail(LAG2D_EINVAL, "null or negative argument");
     const lag2d_config &c = x->cfg;
-    if (c.ncurv != 1 || c.npoly != 2 || c.nrz != 0)
-        return fail(LAG2D_EUNSUPPORTED, "device initialisation covers the planar curved P2 path (ncurv=1, npoly=2)");
+    if (c.ncurv != 1 || c.npoly != 2 || c.nrz != 0 || x->ntria > 0)
+        return fail(LAG2D_EUNSUPPORTED, "device initialisation covers the planar curved P2 path (ncurv=1, npoly=2) "
+                                        "on quads");
     const int nc = x->nc, np = x->np;
     for (int k = 0; k < nblast; ++k)
         if (blast[k] < 1 || blast[k] > nc) return fail(LAG2D_EINVAL, "blast cell out of range (1-based)");

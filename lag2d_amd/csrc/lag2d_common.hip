@@ -387,14 +387,19 @@ __device__ __forceinline__ void cellvol_tria(const Dev &d, const double *__restr
 {
     const int nc = d.nc;
     double xp[6], yp[6];
-    for (int k = 0; k < 3; ++k) {
+    // straight triangles: chord-midpoint mid-nodes; curved (ncurv=1): the
+    // mesh's six nodes (Limiterlag.f90:1904-1914)
+    const int nvt = d.ncurv == 1 ? 6 : 3;
+    for (int k = 0; k < nvt; ++k) {
         const int p = d.ipq[k * nc + ie];
         xp[k] = x[p];
         yp[k] = y[p];
     }
-    xp[3] = 0.5 * (xp[0] + xp[1]); yp[3] = 0.5 * (yp[0] + yp[1]);
-    xp[4] = 0.5 * (xp[1] + xp[2]); yp[4] = 0.5 * (yp[1] + yp[2]);
-    xp[5] = 0.5 * (xp[0] + xp[2]); yp[5] = 0.5 * (yp[0] + yp[2]);
+    if (nvt == 3) {
+        xp[3] = 0.5 * (xp[0] + xp[1]); yp[3] = 0.5 * (yp[0] + yp[1]);
+        xp[4] = 0.5 * (xp[1] + xp[2]); yp[4] = 0.5 * (yp[1] + yp[2]);
+        xp[5] = 0.5 * (xp[0] + xp[2]); yp[5] = 0.5 * (yp[0] + yp[2]);
+    }
     const double c10 = 1.0;
     double rcv = 0.0, scv = 0.0, vol = 0.0, area = 0.0, rca = 0.0, sca = 0.0;
     for (int g = 0; g < 7; ++g) {

@@ -34,6 +34,10 @@ __global__ void k_schedule(Dev d, double dt);
 
 // curved P2 sub-cell path, nfint=5: the face Gauss points' Riemann problems
 __global__ void k_q2g_face(Dev d);
+// curved P2 sub-cell path, six-node triangles of hybrid meshes
+__global__ void k_t2_submass(Dev d);
+__global__ void k_t2_corner(Dev d);
+__global__ void k_t2_cellrhs(Dev d, double alfa, int apply_update);
 
 // discrete adjoint of the straight P1 planar step (quads)
 __global__ void k_adj_nodemove(Dev d, AdjStage a, double alfa);

@@ -23,12 +23,20 @@ int build_node_corners(lag2d_ctx &x, const std::vector<int> &ipq, const std::vec
             if (ie < prev) return LAG2D_EINVAL;  // element lists must be ascending
             prev = ie;
             int iv = -1;
-            const int nvc = ie < x.ntria ? 3 : (nv == 4 ? 4 : 9);
+            const int nvc = ie < x.ntria ? (nv == 4 ? 3 : 6) : (nv == 4 ? 4 : 9);
             for (int j = 0; j < nvc; ++j)
                 if (ipq[(size_t)j * nc + ie] == p) iv = j;
             if (iv < 0) return LAG2D_EINVAL;
             if (nv == 4) {
                 n2c.push_back(ie * 4 + iv);
+            } else if (ie < x.ntria) {
+                // curved triangle: its sub-triangle corners at node p, in
+                // the order getriem_triasubg3 scatters them
+                // (Getrhslaggauss.f90:12540-12585), before every quad
+                static const int iptsg[4][3] = {{0, 3, 5}, {3, 4, 5}, {3, 1, 4}, {5, 4, 2}};
+                for (int isg = 0; isg < 4; ++isg)
+                    for (int ivsg = 0; ivsg < 3; ++ivsg)
+                        if (iptsg[isg][ivsg] == iv) n2c.push_back(ie * 16 + isg * 4 + ivsg);
             } else {
                 // sub-cell corners of this cell at node p, in the order the
                 // sub-cell solver scatters them (Getrhslaggauss.f90:10639-10858)

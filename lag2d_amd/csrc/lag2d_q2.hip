@@ -103,12 +103,13 @@ __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict
     __shared__ double sv[PREP_CELLS][2 * 25 + 1];
     const int nc = d.nc, ie0 = blockIdx.x * PREP_CELLS;
     const int cl = threadIdx.x >> 3, lane = threadIdx.x & 7, ie = ie0 + cl;
-    if (ie < nc) {
+    const bool quad = ie < nc && ie >= d.ntria;  // the indicator runs on quads (Limiterlag.f90:12104)
+    if (quad) {
         const PrepCell c(d, Uin, ie);
         for (int g = lane; g < 25; g += 8) c.point(g, sv[cl][g], sv[cl][25 + g]);
     }
     __syncthreads();
-    if (ie >= nc || lane != 0) return;
+    if (!quad || lane != 0) return;
     double v1 = 0.0, v2 = 0.0;
 #pragma unroll
     for (int g = 0; g < 25; ++g) {
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(256) k_q2_prep(Dev d, const double *__restrict
 __global__ void __launch_bounds__(256) k_q2_prep_tpc(Dev d, const double *__restrict__ Uin)
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ie >= d.nc) return;
+    if (ie >= d.nc || ie < d.ntria) return;
     PrepCell c(d, Uin, ie);
     double v1 = 0.0, v2 = 0.0;
 #pragma unroll
@@ -1248,7 +1249,7 @@ __global__ void __launch_bounds__(4 * CPB) k_q2_corner_sc(Dev d)
     const int cl = threadIdx.x >> 2, isg = threadIdx.x & 3;
     const int ie = blockIdx.x * CPB + cl;
     const int nc = d.nc;
-    const bool active = ie < nc;
+    const bool active = ie < nc && ie >= d.ntria;  // triangles: k_t2_corner
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
     double xp[9], yp[9];
     double rc = 0.0, sc = 0.0, gg0 = 0.0, gg1 = 0.0, gg2 = 0.0;
@@ -1808,10 +1809,16 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_node_q2(Dev d)
         const int ie = e >> 4, slot = e & 15, isg = slot >> 2, iv = slot & 3;
         const double uv = d.rec[RC2(ie, slot, 2)], vv = d.rec[RC2(ie, slot, 3)], pv = d.rec[RC2(ie, slot, 4)];
         for (int f = 0; f < 2; ++f) {
-            double nx, ny, ga, m1, m2, s1, s2;
-            corner_face_geom(d.gq, nc, ie, isg, iv, f, nx, ny, ga);
+            double nx, ny, ga, m1, m2, s1, s2, w;
+            if (ie < d.ntria) {  // curved triangle: its slots and weights
+                tri_face_geom(d.gq, nc, ie, isg, iv, f, nx, ny, ga);
+                w = WT2[isg][iv][f];
+            } else {
+                corner_face_geom(d.gq, nc, ie, isg, iv, f, nx, ny, ga);
+                w = wfr<NF>(isg, iv, f);
+            }
             const double m = d.rec[RC2(ie, slot, f)];
-            corner_face_rec(m, uv, vv, pv, nx, ny, ga * wfr<NF>(isg, iv, f), m1, m2, s1, s2);
+            corner_face_rec(m, uv, vv, pv, nx, ny, ga * w, m1, m2, s1, s2);
             a11 = a11 + m;
             a21 = a21 + 0.0;
             a12 = a12 + 0.0;
@@ -2083,7 +2090,7 @@ k_cellrhs_q2_tpc(Dev d, double alfa, int apply_update)
         *d.tsum += (unsigned long long)*d.tcount;
         *d.tcount = 0;
     }
-    if (ie >= nc) return;
+    if (ie >= nc || ie < d.ntria) return;  // triangles: k_t2_cellrhs
     const double g1 = d.gamlg - 1.0, eps = 1.0e-6;
     const double rc = d.G[0 * nc + ie], sc = d.G[1 * nc + ie];
     const double gg0 = d.G[18 * nc + ie], gg1 = d.G[19 * nc + ie], gg2 = d.G[20 * nc + ie];
@@ -2267,7 +2274,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_submass(Dev d)
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc;
-    if (ie >= nc) return;
+    if (ie >= nc || ie < d.ntria) return;
     double xi[9], yi[9];
     load9(d, d.x0, d.y0, ie, xi, yi);
     const double pc[12][2] = {{-0.5, -1.0}, {0.5, -1.0}, {1.0, -.5}, {1.0, .5}, {0.5, 1.0}, {-0.5, 1.0},
@@ -2709,7 +2716,7 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_weno_setup(Dev d, int *wtop,
 {
     const int ie = blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = d.nc;
-    if (ie >= nc) return;
+    if (ie >= nc || ie < d.ntria) return;
     int ielaj[9];
     ielaj[0] = ie;
     for (int f = 0; f < 4; ++f) ielaj[1 + f] = d.esqua[f * nc + ie];
@@ -2745,6 +2752,10 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_q2_weno_setup(Dev d, int *wtop,
     int skip = mismatch ? 1 : 0;
     for (int k = 0; k < 9; ++k)
         if (ielaj[k] < 0) skip = 1;
+    // on a hybrid mesh the reference reads its face-neighbour table by the
+    // cell's global number (WENO.f90:7659), so no troubled cell has a
+    // defined stencil: every one is skipped, and reported in strict mode
+    if (d.ntria > 0) mismatch = true, skip = 1;
     int nf = 0, face[4] = {0, 0, 0, 0};
     for (int f = 0; f < 4; ++f)
         if (ielaj[1 + f] < nc) face[nf++] = f;
@@ -2784,6 +2795,8 @@ int upload_extra(lag2d_ctx &x)
         hipLaunchKernelGGL(k_q2_shape_tables, dim3(1), dim3(256), 0, x.stream, t);
     }
     hipLaunchKernelGGL(k_q2_submass, dim3(nblk(x.nc)), dim3(LAG2D_BLOCK), 0, x.stream, x.d);
+    if (x.ntria > 0)
+        hipLaunchKernelGGL(k_t2_submass, dim3(nblk(x.ntria)), dim3(LAG2D_BLOCK), 0, x.stream, x.d);
     if (!x.d.wtop) {
         int *t;
         double *r;
@@ -2961,6 +2974,8 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
     }
     x->tic(KID_Q2_CORNER);
     launch_corner(x, ds, x->corner_variant);
+    if (x->ntria > 0)
+        hipLaunchKernelGGL(k_t2_corner, dim3(nblk(x->ntria)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
     x->toc(KID_Q2_CORNER);
     x->tic(KID_NODE_Q2);
     hipLaunchKernelGGL(k_node_q2<4>, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
@@ -2971,6 +2986,9 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
     }
     x->tic(KID_CELLRHS_Q2);
     launch_cellrhs(x, ds, alfa, apply_update, x->rhs_variant ? x->rhs_variant : 1);
+    if (x->ntria > 0)
+        hipLaunchKernelGGL(k_t2_cellrhs, dim3(nblk(x->ntria)), dim3(LAG2D_BLOCK), 0, x->stream, ds, alfa,
+                           apply_update);
     x->toc(KID_CELLRHS_Q2);
     return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
 }

@@ -221,6 +221,47 @@ __device__ __forceinline__ void corner_face_geom(const double *__restrict__ gq, 
     ga = gq[(3 * sl + 2) * nc + ie];
 }
 
+// Curved six-node triangles (cells 0 .. ntria-1 of a hybrid mesh): nodes
+// 0-2 the corners counter-clockwise, 3-5 the mid-nodes of edges 1-2, 2-3,
+// 3-1; four sub-triangles (iptsg, Getrhslaggauss.f90:12262-12265), their
+// signed face slots among the triangle's 15 (fntsg, :12267-12285) and the
+// corner weights times 1/4 (wfgst, :12287-12306, also the volume weights of
+// the face integral, :8349-8367)
+static constexpr int IPTSG[4][3] = {{0, 3, 5}, {3, 4, 5}, {3, 1, 4}, {5, 4, 2}};
+static constexpr int FNTSG[4][6] = {{6, 1, 10, -7, -13, 12}, {7, 14, 8, 15, 9, 13},
+                                    {-14, 10, 2, 3, 11, -8}, {12, -9, -15, 11, 4, 5}};
+static constexpr double WT2[4][3][2] = {
+    {{0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}, {0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)},
+     {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}},
+    {{0.25 * (8.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * (8.0 / 3.0), 0.25 * (8.0 / 3.0)},
+     {0.25 * (8.0 / 3.0), 0.25 * (8.0 / 3.0)}},
+    {{0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)}, {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)},
+     {0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}},
+    {{0.25 * (4.0 / 3.0), 0.25 * (8.0 / 3.0)}, {0.25 * (8.0 / 3.0), 0.25 * (4.0 / 3.0)},
+     {0.25 * (4.0 / 6.0), 0.25 * (4.0 / 6.0)}}};
+static constexpr double XV6[6] = {0.0, 1.0, 0.0, 0.5, 0.5, 0.0};
+static constexpr double YV6[6] = {0.0, 0.0, 1.0, 0.0, 0.5, 0.5};
+
+// rutope's 7-point area rule (dgflo2d.f90:2234-2255), literal values, as
+// the curved-triangle routines read it with ngausd = 7
+static constexpr double PT7T_R[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115, 0.470142064105115,
+                                     0.797426985353087, 0.101286507323456, 0.101286507323456};
+static constexpr double PT7T_S[7] = {0.333333333333333, 0.470142064105115, 0.059715871789770, 0.470142064105115,
+                                     0.101286507323456, 0.797426985353087, 0.101286507323456};
+static constexpr double WT7T[7] = {0.225000000000000, 0.132394152788506, 0.132394152788506, 0.132394152788506,
+                                   0.125939180544827, 0.125939180544827, 0.125939180544827};
+
+__device__ __forceinline__ void tri_face_geom(const double *__restrict__ gq, int nc, int ie, int isg, int iv, int f,
+                                              double &nx, double &ny, double &ga)
+{
+    const int tt = FNTSG[isg][2 * iv + f];
+    const int sl = (tt > 0 ? tt : -tt) - 1;
+    const double sg = tt > 0 ? 1.0 : -1.0;
+    nx = sg * gq[(3 * sl + 0) * nc + ie];
+    ny = sg * gq[(3 * sl + 1) * nc + ie];
+    ga = gq[(3 * sl + 2) * nc + ie];
+}
+
 // The Riemann record of a corner face (getriem_quadsubg2,
 // Getrhslaggauss.f90:10639-10858): m u_c and A sigma n (A = the face length
 // times the corner weight), with the reference's expressions, from the

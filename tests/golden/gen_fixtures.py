@@ -84,6 +84,12 @@ def _hybrid_q2():
     return (mg.split_q2_quads(mesh, 0.5, seed=3, keep=keep), ctl), "rotation_hot"
 
 
+def _hybrid_q2_32():
+    """A 32 x 32 curved hybrid mesh: 40% of the quads split (seeded)."""
+    mesh, ctl = mg.sedov_problem(32, seed=9)
+    return (mg.split_q2_quads(mesh, 0.4, seed=4), ctl), "rotation_hot"
+
+
 def _hybrid_q2_rest():
     (mesh, ctl), _ = _hybrid_q2()
     return (mesh, ctl), "rest_hot"
@@ -114,6 +120,7 @@ SPECS = {
     "sedov_q2gw_16x16": (_gauss(_sedov_q2_wall), 8, 2),
     "rot_q2h_16x16": (_hybrid_q2, 1, 1),
     "rest_q2h_16x16": (_hybrid_q2_rest, 5, 1),
+    "rot_q2h_32x32": (_hybrid_q2_32, 1, 1),
 }
 
 # The curved hybrid mesh's states.  The reference's WENO cannot run on

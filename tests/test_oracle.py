@@ -8,7 +8,8 @@ from oracle import Oracle
 
 CASES = ["sod_q1_100x4", "sedov_q1_16x16", "sedov_q2_12x12", "sedov_q2c_16x16", "sod_hyb_40x8", "sedov_hyb_16x16",
          "sedov_rz_16x16", "sod_rz_40x8", "sod_p1c_annulus", "sedov_q2w_16x16", "sedov_q2g_16x16",
-         "sedov_q2gw_16x16", "rot_q2h_16x16", "rest_q2h_16x16"]
+         "sedov_q2gw_16x16", "rot_q2h_16x16", "rest_q2h_16x16",
+         "rot_q2h_32x32"]
 
 
 @pytest.mark.parametrize("name", CASES)
