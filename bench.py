@@ -19,7 +19,8 @@ differences of the frozen forward.  "roofline" is the dominant kernel's;
 "step_roofline" gives both fractions of the whole step (compulsory bytes
 and algorithmic flops per step), and "scaled" the same problem at
 1200x1200, initialised on the device, where the state no longer fits the
-Infinity Cache.
+Infinity Cache.  "nfint5" times the same problem on the branch with face
+Gauss points.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 120]
 """
@@ -171,6 +172,43 @@ def scaled_step(nx, steps=10, warmup=3):
     return out
 
 
+def nfint5_step(nx, steps=10, warmup=3):
+    """The same Sedov P2 problem on the nfint=5 branch (vertex and non-vertex
+    face Gauss points, no limiting): its step time and per-kernel times.
+    The unlimited branch collapses dt after some twenty steps at the blast,
+    so the timed steps are the first ones after a short warmup."""
+    import dataclasses
+    import torch
+    from lag2d_amd import init
+    from lag2d_amd.lib import Lag2D
+    case = init.sedov_case(nx, seed=2024, centre=True, cdrho=1.0)
+    case = dataclasses.replace(case, ctl=dataclasses.replace(case.ctl, nfint=5, ngausf=5))
+    g = Lag2D(case)
+    for _ in range(warmup):
+        g.step_async()
+    g.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step_async()
+    g.sync()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ok = g.check() == 0
+    g.close()
+    g = Lag2D(case)
+    g.timing(True)
+    for _ in range(warmup + steps):
+        g.step_async()
+    g.sync()
+    _, _, stats = g.kernel_report()
+    g.close()
+    return {"workload": f"sedov_p2_curved_nfint5_{nx}x{nx}", "cells": case.ncell, "steps": steps, "warmup": warmup,
+            "ms_per_step": 1e3 * t / steps, "value": case.ncell * 6 * 4 * steps / t, "unit": "cell-DOF updates/s",
+            "state_valid": ok,
+            "kernels_us_per_launch": {k: 1e3 * v[0] / max(v[1], 1) for k, v in stats.items()}}
+
+
 def cpu_baseline(args, dofs_per_step):
     """The reference on the host's cores when its build is present, else the C
     restatement of the step (oracle/), both single-threaded."""
@@ -210,6 +248,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-branches", action="store_true", help="skip the nfint=5 branch's timing")
     ap.add_argument("--scaled-nx", type=int, default=1200,
                     help="mesh of the scaled step-roofline line (0: skip); device-initialised")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -272,6 +311,7 @@ def main():
     g.close()
     batched = None if args.no_batched else batched_gain(args)
     scaled = scaled_step(args.scaled_nx) if args.scaled_nx else None
+    nf5 = None if args.no_branches else nfint5_step(args.nx)
     adj = None
     if not args.no_adjoint:
         from lag2d_amd import adjoint
@@ -306,6 +346,7 @@ def main():
         "roofline": rf,
         "step_roofline": step_rf,
         "scaled": scaled,
+        "nfint5": nf5,
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adj,
