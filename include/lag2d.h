@@ -114,19 +114,24 @@ int lag2d_upload(lag2d_ctx *ctx, const double *coord, const double *coold,
                  const double *ustar);
 
 /* Driver-side initialisation on the device, in place of lag2d_upload, for
- * the planar curved P2 path (ncurv=1, npoly=2, nrz=0).  Replaces the start-up
- * sequence of dgflo2d.f90:170-282 for piecewise constant initial states:
- * getgeoel_initial_lag and getgeoelho_initial_lag (GetInitial.f90:1278-2096:
- * mass, volume, mass and volume centres, second moments on the 6x6 rule) and
- * getunkno_initial_lag (:151-693: the degree >= 1 mass matrix on the 5x5 rule
- * and its inverse by getinvmat's LU, dgflo2d.f90:2792-2908), with the
- * reference's loops and summation orders.  In: coord(2,npoin) (also taken as
- * coold; ustar = 0), rho, u, v, p per cell (ncell each), and nblast cells
- * (1-based) whose total energy is energy[k] (E = energy / mass, Sedov).
- * Sets geoel, amatr and unkno (tau = volume / mass on the 5x5 rule, u, v,
- * E = p / ((gamma-1) rho) + |u|^2/2, higher DOFs 0) and the initial density
- * of the sub-cell masses; download them with lag2d_download and
- * lag2d_get_mass_matrix.  Synchronises. */
+ * the planar curved P2 path on quads (ncurv=1, npoly=2, nrz=0) and the
+ * straight P1 path (ncurv=0, npoly=1; planar or nrz=2, triangles first).
+ * Replaces the start-up sequence of dgflo2d.f90:170-282 for piecewise
+ * constant initial states, with the reference's loops and summation orders:
+ * getgeoel_initial_lag (GetInitial.f90:1278-1697: mass, volume, mass and
+ * volume centres on the 6x6 quad and 7-point triangle rules; RZ the area
+ * mass, area centre, mean radius and area), for P2 getgeoelho_initial_lag
+ * (:1917-2096: second moments) and getunkno_initial_lag (:151-693: the
+ * degree >= 1 mass matrix on the 5x5 and 7-point rules and its inverse, by
+ * getinvmat's LU, dgflo2d.f90:2792-2908, for P2 and by the 2x2 cofactors
+ * for P1, RZ divided by the mean radius).  In: coord(2,npoin) (also taken
+ * as coold; ustar = 0), rho, u, v, p per cell (ncell each), and nblast
+ * cells (1-based) whose total energy is energy[k] (E = energy / mass,
+ * Sedov).  Sets geoel, amatr and unkno (tau = volume / mass on the mass
+ * matrix's rule, u, v, E = p / ((gamma-1) rho) + |u|^2/2 for P2,
+ * p tau / (gamma-1) + |u|^2/2 for P1, higher DOFs 0) and, for P2, the
+ * initial density of the sub-cell masses; download them with
+ * lag2d_download and lag2d_get_mass_matrix.  Synchronises. */
 int lag2d_init_state(lag2d_ctx *ctx, const double *coord, const double *rho, const double *u,
                      const double *v, const double *p, int nblast, const int *blast,
                      const double *energy);

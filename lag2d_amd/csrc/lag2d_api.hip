@@ -521,10 +521,11 @@ extern "C" int lag2d_upload(lag2d_ctx *x, const double *coord, const double *coo
     return 0;
 }
 
-// Driver-side initialisation on the device (curved P2 path): the arrays
-// the reference's start-up computes before the time loop -- geometry,
-// inverse mass matrices, DOFs -- from the coordinates and a piecewise
-// constant state per cell, in place of an upload of host-computed arrays.
+// Driver-side initialisation on the device (curved P2 path on quads,
+// straight P1 path planar or RZ): the arrays the reference's start-up
+// computes before the time loop -- geometry, inverse mass matrices, DOFs --
+// from the coordinates and a piecewise constant state per cell, in place of
+// an upload of host-computed arrays.
 extern "C" int lag2d_init_state(lag2d_ctx *x, const double *coord, const double *rho, const double *u,
                                 const double *v, const double *p, int nblast, const int *blast,
                                 const double *energy)
@@ -532,9 +533,10 @@ extern "C" int lag2d_init_state(lag2d_ctx *x, const double *coord, const double 
     if (!x || !coord || !rho || !u || !v || !p || (nblast > 0 && (!blast || !energy)) || nblast < 0)
         return fail(LAG2D_EINVAL, "null or negative argument");
     const lag2d_config &c = x->cfg;
-    if (c.ncurv != 1 || c.npoly != 2 || c.nrz != 0 || x->ntria > 0)
+    const bool p1 = c.ncurv == 0 && c.npoly == 1 && x->nd == 3;
+    if (!p1 && (c.ncurv != 1 || c.npoly != 2 || c.nrz != 0 || x->ntria > 0))
         return fail(LAG2D_EUNSUPPORTED, "device initialisation covers the planar curved P2 path (ncurv=1, npoly=2) "
-                                        "on quads");
+                                        "on quads and the straight P1 path (ncurv=0, npoly=1)");
     const int nc = x->nc, np = x->np;
     for (int k = 0; k < nblast; ++k)
         if (blast[k] < 1 || blast[k] > nc) return fail(LAG2D_EINVAL, "blast cell out of range (1-based)");
@@ -567,16 +569,27 @@ extern "C" int lag2d_init_state(lag2d_ctx *x, const double *coord, const double 
         h[5 * (size_t)nc + blast[k] - 1] = 1.0;
     }
     HIPCHK(up(d.R, 6 * (size_t)nc));
-    hipLaunchKernelGGL(k_init_q2, dim3(nblocks(nc)), dim3(LAG2D_BLOCK), 0, x->stream, d, (const double *)d.R, x->A);
+    if (p1) {
+        if (c.nrz == 2)
+            hipLaunchKernelGGL(k_init_p1<1>, dim3(nblocks(nc)), dim3(LAG2D_BLOCK), 0, x->stream, d,
+                               (const double *)d.R, x->A);
+        else
+            hipLaunchKernelGGL(k_init_p1<0>, dim3(nblocks(nc)), dim3(LAG2D_BLOCK), 0, x->stream, d,
+                               (const double *)d.R, x->A);
+    } else {
+        hipLaunchKernelGGL(k_init_q2, dim3(nblocks(nc)), dim3(LAG2D_BLOCK), 0, x->stream, d, (const double *)d.R,
+                           x->A);
+    }
     HIPCHK(hipGetLastError());
-    // the initial density of the sub-cell masses
-    if (!d.rhoi) {
+    // the initial density of the curved path's sub-cell masses
+    if (!p1 && !d.rhoi) {
         double *t;
         int rc = x->alloc(&t, (size_t)nc);
         if (rc) return rc;
         d.rhoi = t;
     }
-    HIPCHK(hipMemcpyAsync((void *)d.rhoi, rho, (size_t)nc * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    if (!p1)
+        HIPCHK(hipMemcpyAsync((void *)d.rhoi, rho, (size_t)nc * sizeof(double), hipMemcpyHostToDevice, x->stream));
     x->geo_host.resize((size_t)24 * nc);
     std::vector<double> g((size_t)24 * nc);
     HIPCHK(hipMemcpyAsync(g.data(), d.G, g.size() * sizeof(double), hipMemcpyDeviceToHost, x->stream));

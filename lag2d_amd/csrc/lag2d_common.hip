@@ -287,7 +287,6 @@ static constexpr double WG6[6] = {0.4679139345726910, 0.4679139345726910, 0.3607
 // Jacobians, lane 0 sums them in point order), for meshes too small to fill
 // the chip with a thread per cell; k_cellvol_tpc, a thread per cell whose
 // unrolled point loop turns the shape functions into constants.
-// rutope 7-point area rule of the reference triangle (dgflo2d.f90:2234-2255)
 // The area-weighted rows of a cell from its radius-weighted volume, area
 // and area moments.
 __device__ __forceinline__ void rz_rows(const Dev &d, int ie, double vol, double area, double rca, double sca)
@@ -301,6 +300,7 @@ __device__ __forceinline__ void rz_rows(const Dev &d, int ie, double vol, double
     d.G[15 * nc + ie] = sca / area;
 }
 
+// rutope 7-point area rule of the reference triangle (dgflo2d.f90:2234-2255)
 static constexpr double PT7R_C[7] = {0.333333333333333, 0.059715871789770, 0.470142064105115,
                                               0.470142064105115, 0.797426985353087, 0.101286507323456,
                                               0.101286507323456};
@@ -695,5 +695,185 @@ __global__ void __launch_bounds__(LAG2D_BLOCK) k_init_q2(Dev d, const double *__
     d.Un[(2 * 6) * nc + ie] = v0;
     d.Un[(3 * 6) * nc + ie] = e0;
 }
+
+// Point g of a straight P1 cell's rule, on the six-node triangle map (edge
+// midpoints as the extra nodes, the 7-point rule, djaco = wi/2 |J|) or the
+// nine-node quad map (the 6x6 rule for GEO, else 5x5; djaco = wi |J|):
+// reference coordinates, weighted Jacobian and the point's y.
+template <bool GEO>
+__device__ __forceinline__ void p1_point(bool tri, const double *xp, const double *yp, int g, double &r, double &s,
+                                         double &djaco, double &yg)
+{
+    double dxdr = 0.0, dxds = 0.0, dydr = 0.0, dyds = 0.0;
+    yg = 0.0;
+    if (tri) {
+        const double c10 = 1.0;
+        r = PT7R_C[g];
+        s = PT7S_C[g];
+        const double t = c10 - r - s;
+        const double sh[6] = {-t * (c10 - 2.0 * t), -r * (c10 - 2.0 * r), -s * (c10 - 2.0 * s),
+                              4.0 * r * t,          4.0 * r * s,          4.0 * s * t};
+        const double dr_[6] = {c10 - 4.0 * t, -1.0 + 4.0 * r, 0.0, 4.0 * (1.0 - 2.0 * r - s), 4.0 * s, -4.0 * s};
+        const double ds_[6] = {c10 - 4.0 * t, 0.0, -1.0 + 4.0 * s, -4.0 * r, 4.0 * r, 4.0 * (1.0 - r - 2.0 * s)};
+        for (int k = 0; k < 6; ++k) {
+            dxdr = dxdr + dr_[k] * xp[k];
+            dxds = dxds + ds_[k] * xp[k];
+            dydr = dydr + dr_[k] * yp[k];
+            dyds = dyds + ds_[k] * yp[k];
+        }
+        djaco = 0.5 * WT7_C[g] * (dxdr * dyds - dydr * dxds);
+        for (int k = 0; k < 6; ++k) yg = yg + sh[k] * yp[k];
+        return;
+    }
+    double wi;
+    if (GEO) {
+        const int i = g / 6, j = g - 6 * i;
+        r = PG6[i];
+        s = PG6[j];
+        wi = WG6[i] * WG6[j];
+    } else {
+        const int i = g / 5, j = g - 5 * i;
+        r = PG5I[i];
+        s = PG5I[j];
+        wi = WG5I[i] * WG5I[j];
+    }
+    double sh[9], dr_[9], ds_[9];
+    shape_q2_d(r, s, sh, dr_, ds_);
+    for (int k = 0; k < 9; ++k) {
+        dxdr = dxdr + dr_[k] * xp[k];
+        dxds = dxds + ds_[k] * xp[k];
+        dydr = dydr + dr_[k] * yp[k];
+        dyds = dyds + ds_[k] * yp[k];
+    }
+    djaco = wi * (dxdr * dyds - dydr * dxds);
+    for (int k = 0; k < 9; ++k) yg = yg + sh[k] * yp[k];
+}
+
+// Driver-side initialisation of the straight P1 path (triangles first,
+// then quads; planar, or area-weighted RZ with RZ = 1), one thread per cell,
+// with the reference's loops, products and summation orders:
+//   getgeoel_initial_lag (GetInitial.f90:1278-1697): mass, volume, mass and
+//     volume centres (rows 1-4, 7-8) and, RZ, the area mass, area centre,
+//     mean radius and area (rows 1-2, 4, 7-8, 11-12), on the 7-point
+//     triangle and 6x6 quad rules (its row 5, the corners' y extent, is
+//     not kept: the step keeps the volume centre there);
+//   getunkno_initial_lag (:151-693; it overwrites the amatr that
+//     getamatr_initial_lag, :697-1274, computes first with another product
+//     order): the slope mass matrix on the 7-point and 5x5 rules (dr = ds =
+//     1/2 on triangles, 1 on quads), amatr(1:3) its inverse by the 2x2
+//     cofactors, amatr(4) = 1 / f0, RZ all four divided by the mean radius.
+// The radius coefficient 1 - alfrz + alfrz y is evaluated as written (it is
+// exactly 1 when planar).  The state is piecewise constant per cell: in =
+// [rho; u; v; p; blast energy; blast flag] x nc.  The DOFs are tau =
+// (5x5 or 7-point volume) / f0, u, v, E = p tau / (gamma - 1) + |u|^2 / 2
+// (E = energy / mass in a blast cell), slopes 0.
+template <int RZ>
+__global__ void __launch_bounds__(LAG2D_BLOCK) k_init_p1(Dev d, const double *__restrict__ in, double *A)
+{
+    const int ie = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = d.nc;
+    if (ie >= nc) return;
+    const bool tri = ie < d.ntria;
+    const double rho = in[ie];
+    double xp[9], yp[9];
+    if (tri) {
+        for (int k = 0; k < 3; ++k) {
+            const int p = d.ipq[k * nc + ie];
+            xp[k] = d.xn[p];
+            yp[k] = d.yn[p];
+        }
+        xp[3] = 0.5 * (xp[0] + xp[1]); yp[3] = 0.5 * (yp[0] + yp[1]);
+        xp[4] = 0.5 * (xp[1] + xp[2]); yp[4] = 0.5 * (yp[1] + yp[2]);
+        xp[5] = 0.5 * (xp[0] + xp[2]); yp[5] = 0.5 * (yp[0] + yp[2]);
+    } else {
+        cellvol_nodes(d, d.xn, d.yn, ie, 0, false, xp, yp);
+    }
+    // ---- getgeoel_initial_lag
+    double masel = 0.0, volel = 0.0, xcent = 0.0, ycent = 0.0, xcel = 0.0, ycel = 0.0;
+    double maselrz = 0.0, xcerz = 0.0, ycerz = 0.0, xceli = 0.0, yceli = 0.0, areel = 0.0;
+    const int ng = tri ? 7 : 36;
+    for (int g = 0; g < ng; ++g) {
+        double r, s, djaco, yg;
+        p1_point<true>(tri, xp, yp, g, r, s, djaco, yg);
+        const double rcoef = 1.0 - d.alfrz + d.alfrz * yg;
+        masel = masel + rho * djaco * rcoef;
+        volel = volel + djaco * rcoef;
+        xcent = xcent + rho * r * djaco * rcoef;
+        ycent = ycent + rho * s * djaco * rcoef;
+        xcel = xcel + djaco * r * rcoef;
+        ycel = ycel + djaco * s * rcoef;
+        if (RZ) {
+            maselrz = maselrz + rho * djaco;
+            xcerz = xcerz + rho * r * djaco;
+            ycerz = ycerz + rho * s * djaco;
+            xceli = xceli + djaco * r;
+            yceli = yceli + djaco * s;
+            areel = areel + djaco;
+        }
+    }
+    for (int row = 0; row < 24; ++row) d.G[row * nc + ie] = 0.0;
+    double rc, sc, mass, radius = 1.0;
+    if (RZ) {
+        rc = xcerz / maselrz;
+        sc = ycerz / maselrz;
+        mass = maselrz;
+        radius = volel / areel;
+        d.G[6 * nc + ie] = xceli / areel;
+        d.G[7 * nc + ie] = yceli / areel;
+        d.G[10 * nc + ie] = radius;
+        d.G[11 * nc + ie] = areel;
+    } else {
+        rc = xcent / masel;
+        sc = ycent / masel;
+        mass = masel;
+        d.G[6 * nc + ie] = xcel / volel;
+        d.G[7 * nc + ie] = ycel / volel;
+    }
+    d.G[0 * nc + ie] = rc;
+    d.G[1 * nc + ie] = sc;
+    d.G[2 * nc + ie] = volel;
+    d.G[3 * nc + ie] = mass;
+    // ---- getunkno_initial_lag: the slope mass matrix and 1 / f0
+    const double dr = tri ? 0.5 : 1.0, ds = dr;
+    double f0 = 0.0, m11 = 0.0, m12 = 0.0, m22 = 0.0, volm = 0.0;
+    const int nm = tri ? 7 : 25;
+    for (int g = 0; g < nm; ++g) {
+        double r, s, djaco, yg;
+        p1_point<false>(tri, xp, yp, g, r, s, djaco, yg);
+        const double b2 = (r - rc) / dr, b3 = (s - sc) / ds;
+        if (RZ) {
+            f0 = f0 + rho * djaco;
+            m11 = m11 + rho * b2 * b2 * djaco;
+            m12 = m12 + rho * b2 * b3 * djaco;
+            m22 = m22 + rho * b3 * b3 * djaco;
+            volm = volm + djaco;
+        } else {
+            const double rcoef = 1.0 - d.alfrz + d.alfrz * yg;
+            f0 = f0 + rho * djaco * rcoef;
+            m11 = m11 + rho * b2 * b2 * djaco * rcoef;
+            m12 = m12 + rho * b2 * b3 * djaco * rcoef;
+            m22 = m22 + rho * b3 * b3 * djaco * rcoef;
+            volm = volm + djaco * rcoef;
+        }
+    }
+    const double det = m11 * m22 - m12 * m12;
+    double a[4] = {m22 / det, -m12 / det, m11 / det, 1.0 / f0};
+    for (int k = 0; k < 4; ++k) A[k * nc + ie] = RZ ? a[k] / radius : a[k];
+    // ---- DOFs
+    const double tau = volm / f0;
+    const double u0 = in[nc + ie], v0 = in[2 * nc + ie], p0 = in[3 * nc + ie];
+    const double g1 = d.gamlg - 1.0;
+    double e0 = p0 * tau / g1 + 0.5 * (u0 * u0 + v0 * v0);
+    if (in[5 * nc + ie] != 0.0) e0 = in[4 * nc + ie] / mass;
+    for (int q = 0; q < 4; ++q)
+        for (int id = 0; id < 3; ++id) d.Un[(q * 3 + id) * nc + ie] = 0.0;
+    d.Un[(0 * 3) * nc + ie] = tau;
+    d.Un[(1 * 3) * nc + ie] = u0;
+    d.Un[(2 * 3) * nc + ie] = v0;
+    d.Un[(3 * 3) * nc + ie] = e0;
+}
+
+template __global__ void k_init_p1<0>(Dev, const double *__restrict__, double *);
+template __global__ void k_init_p1<1>(Dev, const double *__restrict__, double *);
 
 }  // namespace lag2d

@@ -25,6 +25,8 @@ __global__ void k_cellvol(Dev d, double *__restrict__ x, double *__restrict__ y,
 template <int RZ>
 __global__ void k_cellvol_tpc(Dev d, double *__restrict__ x, double *__restrict__ y, int set_centre);
 __global__ void k_init_q2(Dev d, const double *__restrict__ in, double *A);
+template <int RZ>
+__global__ void k_init_p1(Dev d, const double *__restrict__ in, double *A);
 
 // area-weighted RZ path (nrz=2), straight P1 cells
 __global__ void k_limit_corner_rz(Dev d, const double *__restrict__ Uin, int rev);

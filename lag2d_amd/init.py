@@ -231,6 +231,39 @@ def _cell_integrals(xp, pts, w, shape, half, density, alfrz):
     return dj, rcoef, rho
 
 
+def _p1_mesh(mesh: mg.Mesh):
+    """0-based triangle and quad nodes, boundary faces, the hybrid
+    elements-surrounding-points CSR and the coordinates."""
+    ipt = (mesh.iptri - 1).astype(np.int32) if mesh.ntria else np.zeros((0, 3), dtype=np.int32)
+    ipq, bf, _, _ = mesh_arrays(mesh)
+    e1, e2 = build_esuv_hybrid(ipt, ipq, mesh.npoin)
+    return ipt, ipq, bf, e1, e2, mesh.coord.astype(np.float64)
+
+
+def initial_state_p1(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None):
+    """Inputs of the device initialisation of the straight P1 path (planar
+    or RZ, triangles first): the case with its mesh and zero-filled arrays,
+    and ``(rho, u, v, p, blast_cells, blast_energies)`` per cell, with
+    ``density`` and ``state`` evaluated at each cell's vertex mean (the
+    piecewise-constant problems).  ``blast`` = (cells, energy) puts total
+    energy ``energy`` into each listed cell (E = energy / mass on the
+    device, as for the curved path)."""
+    if ctl.ncurv != 0 or ctl.npoly != 1:
+        raise ValueError("straight P1 meshes only")
+    ipt, ipq, bf, e1, e2, coord = _p1_mesh(mesh)
+    nt, nq = ipt.shape[0], ipq.shape[0]
+    nc = nt + nq
+    xcen = np.concatenate([coord[ipt].mean(axis=1), coord[ipq].mean(axis=1)])
+    rho = np.asarray(density(xcen[:, 0], xcen[:, 1]), dtype=np.float64) * np.ones(nc)
+    u0, v0, p0 = (np.asarray(a, dtype=np.float64) * np.ones(nc) for a in state(xcen[:, 0], xcen[:, 1]))
+    case = LagCase(ctl=ctl, ipqua=ipq, bface=bf, esuv1=e1, esuv2=e2, coord=coord.copy(), coold=coord.copy(),
+                   geoel=np.zeros((nc, 24)), amatr=np.zeros((nc, 4)), unkno=np.zeros((nc, 4, ctl.ndegr)),
+                   ustar=np.zeros_like(coord), iptri=ipt if nt else None, dtfix=ctl.dtfix)
+    cells = np.asarray(blast[0] if blast is not None else [], dtype=np.int64)
+    energy = np.full(len(cells), float(blast[1]) if blast is not None else 0.0)
+    return case, (rho, u0, v0, p0, cells, energy)
+
+
 def initialise_p1(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) -> LagCase:
     """Initial arrays of the straight P1 path on a hybrid mesh (triangles
     first), planar (nrz=0) or area-weighted RZ (nrz=2): getgeoel_initial_lag
@@ -245,10 +278,7 @@ def initialise_p1(mesh: mg.Mesh, ctl: mg.Control, density, state, blast=None) ->
         raise ValueError("straight P1 meshes only")
     rz = ctl.nrz == 2
     alfrz = 1.0 if ctl.nrz in (1, 2) else 0.0
-    ipt = (mesh.iptri - 1).astype(np.int32) if mesh.ntria else np.zeros((0, 3), dtype=np.int32)
-    ipq, bf, _, _ = mesh_arrays(mesh)
-    e1, e2 = build_esuv_hybrid(ipt, ipq, mesh.npoin)
-    coord = mesh.coord.astype(np.float64)
+    ipt, ipq, bf, e1, e2, coord = _p1_mesh(mesh)
     nt, nq = ipt.shape[0], ipq.shape[0]
     nc = nt + nq
     geoel = np.zeros((nc, 24))
@@ -413,15 +443,19 @@ def sedov_state(nx=120, seed=2024, centre=False, cdrho=None, cimpd=None):
                          blast=(cells, mg.SEDOV_ENERGY))
 
 
+def sod_density(x, y):
+    """Sod's initial density by cell centre: 1 within radius 0.5, else 0.125."""
+    return np.where(np.sqrt(x * x + y * y) <= 0.5, 1.0, 0.125)
+
+
+def sod_velocity_pressure(x, y):
+    """Sod's initial (u, v, p) by cell centre: at rest, p = 1 within radius
+    0.5, else 0.1."""
+    inside = np.sqrt(x * x + y * y) <= 0.5
+    return 0.0, 0.0, np.where(inside, 1.0, 0.1)
+
+
 def sod_case(nx=100, ny=4) -> LagCase:
     """Planar Sod tube on straight quads, P1."""
     mesh, ctl = mg.sod_problem(nx, ny)
-
-    def rho(x, y):
-        return np.where(np.sqrt(x * x + y * y) <= 0.5, 1.0, 0.125)
-
-    def st(x, y):
-        inside = np.sqrt(x * x + y * y) <= 0.5
-        return 0.0, 0.0, np.where(inside, 1.0, 0.1)
-
-    return initialise(mesh, ctl, rho, st)
+    return initialise(mesh, ctl, sod_density, sod_velocity_pressure)

@@ -138,9 +138,10 @@ class Lag2D:
 
     @classmethod
     def from_state(cls, case: LagCase, states, strict: bool = False):
-        """A context initialised on the device (curved P2 path) from the
-        mesh of ``case`` and ``states`` = (rho, u, v, p, blast_cells,
-        blast_energies) per cell, as ``init.initial_state`` returns them."""
+        """A context initialised on the device (curved P2 path on quads,
+        straight P1 path) from the mesh of ``case`` and ``states`` = (rho,
+        u, v, p, blast_cells, blast_energies) per cell, as
+        ``init.initial_state`` and ``init.initial_state_p1`` return them."""
         g = cls(case, strict=strict, upload=False)
         g.init_state(states)
         return g

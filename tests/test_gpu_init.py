@@ -1,6 +1,7 @@
-"""Driver-side initialisation on the device (lag2d_init_state, curved P2
-path) against the reference's own initial arrays in the golden fixtures,
-and against the host initialisation (lag2d_amd.init) at full size.  The
+"""Driver-side initialisation on the device (lag2d_init_state: the curved
+P2 path, and the straight P1 path planar and RZ) against the reference's
+own initial arrays in the golden fixtures, and against the host
+initialisation (lag2d_amd.init) at full size.  The
 device follows the reference's loops, summation orders and LU inverse:
 its geometry rows and inverse mass matrices equal the reference's bit for
 bit.  The DOFs follow the host initialisation's definition (the reference
@@ -13,6 +14,7 @@ import pytest
 
 from fixtures import load, rel_l1, state_errors
 from lag2d_amd import init
+from lag2d_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
 
@@ -94,3 +96,75 @@ def test_ensemble_device_init_equals_runs_alone():
         g.step(3)
         st = g.download()
         assert np.array_equal(got[r]["unkno"], st["unkno"]) and np.array_equal(got[r]["coord"], st["coord"]), r
+
+
+# The straight P1 path (triangles first), planar and area-weighted RZ: the
+# reference's own start-up arrays in the fixtures, with the density and
+# state of the problem by cell centre.
+def _sedov_p1(problem):
+    return lambda: (problem, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6))
+
+
+P1_CASES = [("sod_q1_100x4", lambda: (mg.sod_problem(100, 4), init.sod_density, init.sod_velocity_pressure)),
+            ("sod_hyb_40x8", lambda: (mg.sod_hybrid_problem(40, 8), init.sod_density, init.sod_velocity_pressure)),
+            ("sedov_hyb_16x16", _sedov_p1(mg.sedov_hybrid_problem(16))),
+            ("sedov_rz_16x16", _sedov_p1(mg.sedov_rz_hybrid_problem(16)))]
+
+
+@pytest.mark.parametrize("name,make", P1_CASES, ids=[c[0] for c in P1_CASES])
+def test_device_init_p1_matches_reference(name, make):
+    """Mass centres, masses and volume centres (RZ: area mass, area centre)
+    and the inverse slope mass matrices bit for bit against the reference's
+    initialisation; the volume rows the step recomputes, and RZ's mean
+    radius and area, to round-off; the DOFs to 1e-12 (their definition is
+    the host initialisation's); the first steps from them to 1e-10."""
+    from lag2d_amd.lib import Lag2D
+    ref, z = load(name)
+    (mesh, ctl), density, state = make()
+    case, states = init.initial_state_p1(mesh, ctl, density, state)
+    # the Sedov set-ups' blast cell and its total energy, as the reference
+    # deposits it
+    e = ref.unkno[:, 3, 0]
+    hot = np.flatnonzero(e > 100 * np.median(np.abs(e)))
+    states = states[:4] + (hot, e[hot] * ref.geoel[hot, 3])
+    assert np.array_equal(case.coord, ref.coord) and np.array_equal(case.ipqua, ref.ipqua)
+    if ref.iptri is not None:
+        assert np.array_equal(case.iptri, ref.iptri)
+    g = Lag2D.from_state(case, states)
+    geoel, amatr, unkno = _download(g)
+    exact = [0, 1, 3, 6, 7]
+    same = {k: int(np.sum(geoel[:, k] == ref.geoel[:, k])) for k in exact}
+    print(name, "bit-equal geoel entries per row", same, "of", ref.ncell,
+          "; amatr", int(np.sum(amatr == ref.amatr)), "of", amatr.size, flush=True)
+    for k in exact:
+        assert np.array_equal(geoel[:, k], ref.geoel[:, k]), k
+    assert np.array_equal(amatr, ref.amatr)
+    for k in [2] + ([10, 11] if ctl.nrz == 2 else []):
+        assert np.abs(geoel[:, k] - ref.geoel[:, k]).max() <= 1e-13 * np.abs(ref.geoel[:, k]).max(), k
+    for q in range(4):
+        assert rel_l1(unkno[:, q], ref.unkno[:, q]) <= 1e-12, q
+    # the reference's first steps from its own arrays, against ours from the
+    # device initialisation
+    s = int(z["steps"][0])
+    g.step(s)
+    st = g.download()
+    err = state_errors(st["unkno"], st["coord"], z[f"s{s}_unkno"], z[f"s{s}_coord"])
+    print(name, "step", s, err, flush=True)
+    assert max(err.values()) <= 1e-10, err
+
+
+def test_device_init_p1_blast_energy():
+    """A blast cell's total energy: E = energy / mass with the device's mass
+    (the RZ area mass), the other cells' DOFs as without the blast."""
+    from lag2d_amd.lib import Lag2D
+    mesh, ctl = mg.sedov_rz_hybrid_problem(16)
+    cells = [0, 5, 200]
+    case, states = init.initial_state_p1(mesh, ctl, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6),
+                                         blast=(cells, 0.25))
+    g = Lag2D.from_state(case, states)
+    geoel, _, unkno = _download(g)
+    case0, states0 = init.initial_state_p1(mesh, ctl, lambda x, y: 1.0, lambda x, y: (0.0, 0.0, 1e-6))
+    _, _, unkno0 = _download(Lag2D.from_state(case0, states0))
+    assert np.array_equal(unkno[cells, 3, 0], 0.25 / geoel[cells, 3])
+    rest = np.setdiff1d(np.arange(case.geoel.shape[0]), cells)
+    assert np.array_equal(unkno[rest], unkno0[rest])
