@@ -433,9 +433,11 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     }
     if (q2 && (c.nfint == 5 || ntria > 0)) {
         // the nfint=5 branch and hybrid meshes: a lane per sub-cell corner,
-        // a thread per cell RHS (the forms that take their options)
+        // a thread per cell RHS (the forms that take their options); the
+        // nfint=5 branch's cell RHS 8 cells per workgroup where the mesh
+        // does not fill the chip with a thread per cell
         x->corner_variant = 2;
-        x->rhs_variant = 2;
+        x->rhs_variant = (c.nfint == 5 && ntria == 0 && !x->geom_cells) ? 1 : 2;
     }
     *out = x;
     return 0;

@@ -1869,6 +1869,46 @@ constexpr int RHS_BN = 16 * 6 + 4;   // basis at the corner's node
 constexpr int RHS_PS = 17;           // point values: a[6], c[6], u, v, p, w (+1 pad)
 constexpr int RHS_PT = 25 * RHS_PS + 1;
 
+constexpr int RHS_FS = 16;  // face-point values: b[6], uf1 nx A, uf2 ny A, f1, f2, uf1 f1, uf2 f2, c1, c2, uf1 c1, uf2 c2
+
+// The face Gauss points' terms of variable Q, DOF id, added to r in the
+// order of k_cellrhs_q2_tpc: per face the forces of its two points, then
+// their correction forces, each weighted by w2 / 2.  e: a cell's staged
+// face-point values (RHS_FS per point).
+template <int Q>
+__device__ __forceinline__ double face_points_q2(double r, const double *e, int id)
+{
+    const double wi = LW2 / 2.0;
+#pragma unroll
+    for (int lf = 0; lf < 4; ++lf) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const double *p = e + RHS_FS * (2 * lf + k);
+            const double b = p[id];
+            if (Q == 0) r = r + p[6] * b * wi + p[7] * b * wi;
+            if (Q == 1) r = r + p[8] * b * wi;
+            if (Q == 2) r = r + p[9] * b * wi;
+            if (Q == 3) r = r + p[10] * b * wi + p[11] * b * wi;
+        }
+        if (Q == 0) continue;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const double *p = e + RHS_FS * (2 * lf + k);
+            const double b = p[id];
+            if (Q == 1) r = r + p[12] * b * wi;
+            if (Q == 2) r = r + p[13] * b * wi;
+            if (Q == 3) r = r + p[14] * b * wi + p[15] * b * wi;
+        }
+    }
+    return r;
+}
+
+// NF = 5: the corner faces weighted by WR5 / WV5 / WM5, and the 8 face
+// Gauss points of each cell staged by threads 72-135 (basis at the point and
+// the point's force, correction and flux factors from the face kernel's
+// outputs), summed after the corners in the order of k_cellrhs_q2_tpc.
+
+template <int NF>
 __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int apply_update)
 {
     __shared__ double scn[RHS_CPB][RHS_CN];
@@ -1876,6 +1916,7 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
     __shared__ double spt[RHS_CPB][RHS_PT];
     __shared__ double sres[RHS_CPB][4][6];
     __shared__ double scl[RHS_CPB][RHS_CS];
+    __shared__ double sfp[NF == 5 ? RHS_CPB : 1][NF == 5 ? 8 * RHS_FS : 1];
     const int tid = threadIdx.x;
     const int nc = d.nc;
     const int ie0 = xcd_block(blockIdx.x, gridDim.x) * RHS_CPB;
@@ -1915,9 +1956,9 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             for (int f = 0; f < 2; ++f) {
                 double ga, m1, m2, s1, s2;
                 corner_face_geom(d.gq, nc, ie, isg, iv, f, nx[f], ny[f], ga);
-                ar[f] = ga * WFV[isg][iv][f];
+                ar[f] = ga * wfv<NF>(isg, iv, f);
                 const double m11 = d.rec[RC2(ie, corner, f)];
-                corner_face_rec(m11, uv, vv, pv, nx[f], ny[f], ga * WFR[isg][iv][f], m1, m2, s1, s2);
+                corner_face_rec(m11, uv, vv, pv, nx[f], ny[f], ga * wfr<NF>(isg, iv, f), m1, m2, s1, s2);
                 fx[f] = s1 + m11 * us1 + 0.0 * us2 - m1;
                 fy[f] = s2 + 0.0 * us1 + m11 * us2 - m2;
             }
@@ -1938,6 +1979,33 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
             const double *sc_ = scl[cl];
             basis6(XV9[ln], YV9[ln], sc_[42], sc_[43], sc_[44], sc_[45], sc_[46], bb);
             for (int id = 0; id < 6; ++id) sbn[cl][6 * corner + id] = bb[id];
+        }
+    }
+    if constexpr (NF == 5) {
+        const int u = tid - 72, cl = u >> 3, g = u & 7, ie = ie0 + cl;
+        if (u >= 0 && u < 8 * RHS_CPB && ie < nc) {
+            const int lf = g >> 1, k = g & 1;
+            const int pn[3] = {d.ipq[lf * nc + ie], d.ipq[((lf + 1) & 3) * nc + ie], d.ipq[(4 + lf) * nc + ie]};
+            const double fxn[3] = {d.xs[pn[0]], d.xs[pn[1]], d.xs[pn[2]]};
+            const double fyn[3] = {d.ys[pn[0]], d.ys[pn[1]], d.ys[pn[2]]};
+            double gnx, gny, gsa;
+            face_normal5(fxn, fyn, k, gnx, gny, gsa);
+            const double *sc_ = scl[cl];
+            double *e = &sfp[NF == 5 ? cl : 0][RHS_FS * g];
+            basis6(PQG5[g][0], PQG5[g][1], sc_[42], sc_[43], sc_[44], sc_[45], sc_[46], e);
+            const double f1 = d.fg[FG5(g, 0) * nc + ie], f2 = d.fg[FG5(g, 1) * nc + ie];
+            const double c1 = d.fg[FG5(g, 2) * nc + ie], c2 = d.fg[FG5(g, 3) * nc + ie];
+            const double uf1 = d.fg[FG5(g, 4) * nc + ie], uf2 = d.fg[FG5(g, 5) * nc + ie];
+            e[6] = uf1 * gnx * gsa;
+            e[7] = uf2 * gny * gsa;
+            e[8] = f1;
+            e[9] = f2;
+            e[10] = uf1 * f1;
+            e[11] = uf2 * f2;
+            e[12] = c1;
+            e[13] = c2;
+            e[14] = uf1 * c1;
+            e[15] = uf2 * c2;
         }
     }
     for (int u = (tid + 256 - RHS_CPB * 16) & 255; u < RHS_CPB * 25; u += 256) {
@@ -2006,6 +2074,8 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
                 const double b = bn[6 * k];
                 r = r + cn[12 * k + 0] * b + cn[12 * k + 1] * b + cn[12 * k + 2] * b + cn[12 * k + 3] * b;
             }
+            if constexpr (NF == 5)
+                r = face_points_q2<0>(r, sfp[NF == 5 ? cl : 0], id);
 #pragma unroll
             for (int k = 0; k < 25; ++k) {
                 const double *s = &pt[RHS_PS * k];
@@ -2017,8 +2087,14 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const double b = bn[6 * k];
-                const double w1 = WFM[k >> 2][k & 3][0], w2 = WFM[k >> 2][k & 3][1];
+                const double w1 = wfm<NF>(k >> 2, k & 3, 0), w2 = wfm<NF>(k >> 2, k & 3, 1);
                 r = r + cn[12 * k + o] * b * w1 + cn[12 * k + o + 1] * b * w2;
+            }
+            if constexpr (NF == 5) {
+                if (q == 1)
+                    r = face_points_q2<1>(r, sfp[NF == 5 ? cl : 0], id);
+                else
+                    r = face_points_q2<2>(r, sfp[NF == 5 ? cl : 0], id);
             }
             const int ac = (q == 1) ? 0 : 6;
 #pragma unroll
@@ -2030,10 +2106,12 @@ __global__ void __launch_bounds__(256) k_cellrhs_q2(Dev d, double alfa, int appl
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
                 const double b = bn[6 * k];
-                const double w1 = WFM[k >> 2][k & 3][0], w2 = WFM[k >> 2][k & 3][1];
+                const double w1 = wfm<NF>(k >> 2, k & 3, 0), w2 = wfm<NF>(k >> 2, k & 3, 1);
                 r = r + cn[12 * k + 8] * b * w1 + cn[12 * k + 9] * b * w1 + cn[12 * k + 10] * b * w2 +
                     cn[12 * k + 11] * b * w2;
             }
+            if constexpr (NF == 5)
+                r = face_points_q2<3>(r, sfp[NF == 5 ? cl : 0], id);
 #pragma unroll
             for (int k = 0; k < 25; ++k) {
                 const double *s = &pt[RHS_PS * k];
@@ -2861,7 +2939,7 @@ static void launch_cellrhs(lag2d_ctx *x, Dev &ds, double alfa, int apply_update,
     case 3: hipLaunchKernelGGL((k_cellrhs_q2_tpc<6, 0, 128>), g(128), dim3(128), 0, x->stream, ds, alfa, apply_update); break;
     case 4: hipLaunchKernelGGL((k_cellrhs_q2_tpc<6, 0, 64>), g(64), dim3(64), 0, x->stream, ds, alfa, apply_update); break;
     default:
-        hipLaunchKernelGGL(k_cellrhs_q2, g(RHS_CPB), dim3(256), 0, x->stream, ds, alfa, apply_update);
+        hipLaunchKernelGGL(k_cellrhs_q2<4>, g(RHS_CPB), dim3(256), 0, x->stream, ds, alfa, apply_update);
         break;
     }
 }
@@ -2957,8 +3035,12 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
         hipLaunchKernelGGL(k_q2g_face, dim3(nblk(nc), 8), dim3(LAG2D_BLOCK), 0, x->stream, ds);
         x->toc(KID_Q2G_FACE);
         x->tic(KID_CELLRHS_Q2);
-        hipLaunchKernelGGL((k_cellrhs_q2_tpc<6, 0, 256, 5>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa,
-                           apply_update);
+        if (x->rhs_variant == 1)  // 8 cells per workgroup (small meshes)
+            hipLaunchKernelGGL(k_cellrhs_q2<5>, dim3((nc + RHS_CPB - 1) / RHS_CPB), dim3(256), 0, x->stream, ds,
+                               alfa, apply_update);
+        else
+            hipLaunchKernelGGL((k_cellrhs_q2_tpc<6, 0, 256, 5>), dim3(nblk(nc)), dim3(256), 0, x->stream, ds, alfa,
+                               apply_update);
         x->toc(KID_CELLRHS_Q2);
         return hipGetLastError() == hipSuccess ? 0 : LAG2D_EDEVICE;
     }

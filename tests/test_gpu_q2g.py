@@ -15,8 +15,16 @@ def _lag(case):
     return Lag2D(case)
 
 
+@pytest.fixture(params=[1, 2], ids=["cellrhs-8-per-workgroup", "cellrhs-thread-per-cell"])
+def geom(request, monkeypatch):
+    """The cell-RHS form, which follows the mesh size: 8 cells per
+    workgroup (geometry variant 1) or a thread per cell (2)."""
+    monkeypatch.setenv("LAG2D_GEOM_VARIANT", str(request.param))
+    return request.param
+
+
 @pytest.mark.parametrize("name", ["sedov_q2g_16x16", "sedov_q2gw_16x16"])
-def test_rhs_matches_reference(name):
+def test_rhs_matches_reference(name, geom):
     case, z = load(name)
     g = _lag(case)
     r = g.rhs()
@@ -28,7 +36,7 @@ def test_rhs_matches_reference(name):
 
 
 @pytest.mark.parametrize("name", ["sedov_q2g_16x16", "sedov_q2gw_16x16"])
-def test_steps_match_reference(name):
+def test_steps_match_reference(name, geom):
     """Centre blast (30 steps, through the collapse of dt the unlimited
     branch shows after about 20 steps) and blast on the bottom slip wall
     (8 steps: the reference, and the oracle, go non-finite at step 10)."""
