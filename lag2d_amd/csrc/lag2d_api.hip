@@ -433,11 +433,13 @@ static int create_impl(const lag2d_config *cfg, int ntria, int nquad, int npoin,
     }
     if (q2 && (c.nfint == 5 || ntria > 0)) {
         // the nfint=5 branch and hybrid meshes: a lane per sub-cell corner,
-        // a thread per cell RHS (the forms that take their options); the
-        // nfint=5 branch's cell RHS 8 cells per workgroup where the mesh
-        // does not fill the chip with a thread per cell
-        x->corner_variant = 2;
-        x->rhs_variant = (c.nfint == 5 && ntria == 0 && !x->geom_cells) ? 1 : 2;
+        // a thread per cell RHS (the forms that take their options); on the
+        // nfint=5 branch two waves per sub-cell corner and 8 cells per
+        // workgroup in the cell RHS where the mesh does not fill the chip
+        // with a thread per cell
+        const bool small5 = c.nfint == 5 && ntria == 0 && !x->geom_cells;
+        x->corner_variant = small5 ? 9 : 2;
+        x->rhs_variant = small5 ? 1 : 2;
     }
     *out = x;
     return 0;

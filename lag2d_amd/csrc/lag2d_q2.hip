@@ -1654,7 +1654,7 @@ __global__ void __launch_bounds__(4 * CWU_CELLS) __attribute__((amdgpu_waves_per
 // every sum runs in point order as in k_q2_corner_wu.
 constexpr int CW2_SPLIT = 13;
 
-template <int ND>
+template <int ND, int NF = 4>
 __global__ void __launch_bounds__(8 * CWU_CELLS) k_q2_corner_wu2(Dev d)
 {
     __shared__ double sgq[16 * 3][CWU_CELLS];
@@ -1745,7 +1745,13 @@ __global__ void __launch_bounds__(8 * CWU_CELLS) k_q2_corner_wu2(Dev d)
     const double rhomc = u[0][0], uctr = u[1][0], vctr = u[2][0], ectr = u[3][0];
     const double rhoct = 1.0 / rhomc;
     const double pctr = dmax(eps, g1 * rhoct * (ectr - 0.5 * (uctr * uctr + vctr * vctr)));
-    const double sdctr = sqrt(dmax(eps, d.gamlg * pctr / rhoct));
+    double sdctr;
+    if constexpr (NF == 5) {
+        sdctr = sqrt(d.gamlg * pctr / rhoct);
+        if (half == 0) d.drho[isg * nc + ie] = rsub - 1.0 / uq;
+    } else {
+        sdctr = sqrt(dmax(eps, d.gamlg * pctr / rhoct));
+    }
     const double dcor = d.cdrho * (rsub - 1.0 / uq);
 #pragma unroll 1
     for (int iv = 2 * half; iv < 2 * half + 2; ++iv) {
@@ -1774,9 +1780,13 @@ __global__ void __launch_bounds__(8 * CWU_CELLS) k_q2_corner_wu2(Dev d)
             const int sl = (tt > 0 ? tt : -tt) - 1;
             const double sg = tt > 0 ? 1.0 : -1.0;
             const double nx = sg * sgq[3 * sl + 0][cl], ny = sg * sgq[3 * sl + 1][cl];
-            const double area = sgq[3 * sl + 2][cl] * WFR[isg][iv][f];
+            const double area = sgq[3 * sl + 2][cl] * wfr<NF>(isg, iv, f);
             const double deltu = fabs(dux * nx + duy * ny);
-            const double mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
+            double mur;
+            if constexpr (NF == 5)
+                mur = rhoct * sdctr + rhoct * d.slpdu * (d.cimpd * deltu);
+            else
+                mur = rhoct * sdctr + d.cimpd * rhoct * d.slpdu * deltu;
             double m11;
             if (a3 < 1.e12) {
                 m11 = mur * area * fabs(1.0);
@@ -3026,7 +3036,11 @@ int launch_rhs_q2(lag2d_ctx *x, Dev &ds, const double *Uin, double alfa, int app
         // corners, the node solve, the face points, the face and domain
         // integrals
         x->tic(KID_Q2_CORNER);
-        hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 64, 1, 5>), dim3((nc + 63) / 64), dim3(256), 0, x->stream, ds);
+        if (x->corner_variant == 9)  // two waves per sub-cell (small meshes)
+            hipLaunchKernelGGL((k_q2_corner_wu2<6, 5>), dim3((nc + CWU_CELLS - 1) / CWU_CELLS), dim3(8 * CWU_CELLS),
+                               0, x->stream, ds);
+        else
+            hipLaunchKernelGGL((k_q2_corner_sc<6, 0, 64, 1, 5>), dim3((nc + 63) / 64), dim3(256), 0, x->stream, ds);
         x->toc(KID_Q2_CORNER);
         x->tic(KID_NODE_Q2);
         hipLaunchKernelGGL(k_node_q2<5>, dim3(nblk(np)), dim3(LAG2D_BLOCK), 0, x->stream, ds);
